@@ -1,0 +1,303 @@
+"""ctypes front end of the test oracle.
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product package
+(fgt2d_amd) never imports this module.
+
+Two libraries are wrapped:
+
+* ``liborc.so`` -- the C restatement in ``fgt2d_oracle.c`` (built by
+  ``make -C oracle``);
+* ``_ref/libref2d.so`` -- the reference Fortran itself, compiled from
+  ``2d/planewave/src`` by ``make -C oracle ref`` where the reference sources
+  exist.  On hosts without it, ``have_ref()`` is False and the callers skip.
+
+Arrays follow the reference layout: points are (2, n) column-major, i.e. a
+C-contiguous numpy array of shape (n, 2); densities are (nd, n) column-major,
+i.e. C-contiguous (n, nd).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ORC = None
+_REF = None
+
+
+def _orc():
+    global _ORC
+    if _ORC is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liborc.so is not built (run make -C oracle)")
+        lib = C.CDLL(path)
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        lib.orc_fgt2d.restype = C.c_int
+        lib.orc_fgt2d.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, dp, C.c_int, dp, C.c_int,
+                                  C.c_int, dp, dp, dp, C.c_int, dp, C.c_int, dp, dp, dp]
+        lib.orc_direct2d.restype = None
+        lib.orc_direct2d.argtypes = [C.c_int, C.c_double, C.c_double, dp, C.c_int, dp, dp, C.c_int,
+                                     C.c_int, dp, dp, dp]
+        lib.orc_hlterms.restype = C.c_int
+        lib.orc_hlterms.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        lib.orc_pwterms.restype = None
+        lib.orc_pwterms.argtypes = [C.c_double, C.c_double, C.c_double, dp, ip]
+        lib.orc_pwnodes.restype = None
+        lib.orc_pwnodes.argtypes = [C.c_double, C.c_int, dp, dp]
+        lib.orc_boxsize0.restype = None
+        lib.orc_boxsize0.argtypes = [C.c_double, C.c_double, dp, C.c_int, dp, C.c_int, ip, dp, dp]
+        lib.orc_tree2d_build.restype = C.c_void_p
+        lib.orc_tree2d_build.argtypes = [C.c_double, C.c_double, C.c_int, dp, C.c_int, dp, ip]
+        lib.orc_tree2d_get.restype = None
+        lib.orc_tree2d_get.argtypes = [C.c_void_p] + [ip] * 7 + [dp, dp] + [ip] * 4 + [dp]
+        lib.orc_tree2d_free.restype = None
+        lib.orc_tree2d_free.argtypes = [C.c_void_p]
+        lib.orc_last_stats.restype = None
+        lib.orc_last_stats.argtypes = [ip]
+        lib.orc_last_time.restype = C.c_double
+        lib.orc_derr.restype = C.c_double
+        lib.orc_derr.argtypes = [dp, dp, C.c_int]
+        _ORC = lib
+    return _ORC
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _pts(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    assert a.ndim == 2 and a.shape[1] == 2
+    return a
+
+
+def _outs(nd, n, ifp):
+    return (np.zeros((n, nd)), np.zeros((n, 2, nd)), np.zeros((n, 3, nd))) if ifp > 0 else \
+           (np.zeros(nd), np.zeros(2 * nd), np.zeros(3 * nd))
+
+
+def oracle_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
+    """The C restatement of fgt2d (fgt2d.f:19) for charges.  Returns
+    (pot, grad, hess, pottarg, gradtarg, hesstarg) shaped (n, nd), (n, 2, nd),
+    (n, 3, nd) when requested."""
+    lib = _orc()
+    src = _pts(sources)
+    ns = src.shape[0]
+    chg = np.ascontiguousarray(charge, dtype=np.float64).reshape(ns, -1)
+    nd = chg.shape[1]
+    tg = _pts(targ) if targ is not None else np.zeros((0, 2))
+    nt = tg.shape[0]
+    pot, grad, hess = _outs(nd, ns, ifpgh)
+    pt, gt, ht = _outs(nd, max(nt, 1), ifpghtarg)
+    rc = lib.orc_fgt2d(nd, delta, eps, ns, _d(src), 1, _d(chg), 0, ifpgh, _d(pot), _d(grad), _d(hess),
+                       nt, _d(tg), ifpghtarg, _d(pt), _d(gt), _d(ht))
+    if rc != 0:
+        raise RuntimeError("orc_fgt2d returned %d" % rc)
+    return pot, grad, hess, pt[:nt], gt[:nt], ht[:nt]
+
+
+def oracle_stats():
+    lib = _orc()
+    out = np.zeros(80, dtype=np.int32)
+    lib.orc_last_stats(_i(out))
+    nlev = int(out[2])
+    return dict(npwlevel=int(out[0]), nlmax=int(out[1]), nlevels=nlev, nboxes=int(out[3]),
+                npw=int(out[4]), ntermmax=int(out[5]), nlocal=[int(v) for v in out[6:6 + nlev + 1]],
+                time=float(lib.orc_last_time()))
+
+
+def direct2d(delta, eps, sources, charge, targ, ifpgh=1):
+    """FP64 direct sum with the reference's cutoff r^2 < ln(1/eps) delta
+    (g2ddirect.f:32-76); eps=1e-100 is the reference test's no-cutoff oracle."""
+    lib = _orc()
+    src = _pts(sources)
+    chg = np.ascontiguousarray(charge, dtype=np.float64).reshape(src.shape[0], -1)
+    nd = chg.shape[1]
+    tg = _pts(targ)
+    nt = tg.shape[0]
+    pot, grad, hess = np.zeros((nt, nd)), np.zeros((nt, 2, nd)), np.zeros((nt, 3, nd))
+    lib.orc_direct2d(nd, delta, eps, _d(src), src.shape[0], _d(chg), _d(tg), nt, ifpgh,
+                     _d(pot), _d(grad), _d(hess))
+    return pot, grad, hess
+
+
+def derr(vref, vcomp):
+    """Relative l2 error of the reference test (test_fgt2d.f:204-225)."""
+    a = np.ascontiguousarray(vref, dtype=np.float64).ravel()
+    b = np.ascontiguousarray(vcomp, dtype=np.float64).ravel()
+    return float(_orc().orc_derr(_d(a), _d(b), a.size))
+
+
+def hlterms(ndim, bsize, delta, eps):
+    return int(_orc().orc_hlterms(ndim, bsize, delta, eps))
+
+
+def pwterms(bsize, delta, eps):
+    pm, n = C.c_double(), C.c_int()
+    _orc().orc_pwterms(bsize, delta, eps, C.byref(pm), C.byref(n))
+    return pm.value, n.value
+
+
+def pwnodes(pmax, npw):
+    ws, ts = np.zeros(npw), np.zeros(npw)
+    _orc().orc_pwnodes(pmax, npw, _d(ws), _d(ts))
+    return ws, ts
+
+
+def boxsize0(delta, eps, sources, targ):
+    src, tg = _pts(sources), _pts(targ)
+    lc, bs0, cen = C.c_int(), C.c_double(), np.zeros(2)
+    _orc().orc_boxsize0(delta, eps, _d(src), src.shape[0], _d(tg), tg.shape[0], C.byref(lc),
+                        C.byref(bs0), _d(cen))
+    return lc.value, bs0.value, cen
+
+
+def oracle_tree2d(delta, eps, sources, targ):
+    """The tree fgt2d builds (fgt2d.f:124-221), 0-based."""
+    lib = _orc()
+    src, tg = _pts(sources), _pts(targ)
+    info = np.zeros(4, dtype=np.int32)
+    h = lib.orc_tree2d_build(delta, eps, src.shape[0], _d(src), tg.shape[0], _d(tg), _i(info))
+    npwlevel, nlmax, nlevels, nboxes = (int(v) for v in info)
+    out = dict(npwlevel=npwlevel, nlmax=nlmax, nlevels=nlevels, nboxes=nboxes,
+               laddr=np.zeros((nlevels + 1, 2), np.int32), lev=np.zeros(nboxes, np.int32),
+               par=np.zeros(nboxes, np.int32), nch=np.zeros(nboxes, np.int32),
+               ch=np.zeros((nboxes, 4), np.int32), ncol=np.zeros(nboxes, np.int32),
+               col=np.zeros((nboxes, 9), np.int32), cen=np.zeros((nboxes, 2)),
+               bsz=np.zeros(nlevels + 1), isrc=np.zeros(src.shape[0], np.int32),
+               isrcse=np.zeros((nboxes, 2), np.int32), itarg=np.zeros(tg.shape[0], np.int32),
+               itargse=np.zeros((nboxes, 2), np.int32), bs0cen0=np.zeros(3))
+    o = out
+    lib.orc_tree2d_get(h, _i(o["laddr"]), _i(o["lev"]), _i(o["par"]), _i(o["nch"]), _i(o["ch"]),
+                       _i(o["ncol"]), _i(o["col"]), _d(o["cen"]), _d(o["bsz"]), _i(o["isrc"]),
+                       _i(o["isrcse"]), _i(o["itarg"]), _i(o["itargse"]), _d(o["bs0cen0"]))
+    lib.orc_tree2d_free(h)
+    return out
+
+
+# --------------------------------------------------------------------------
+# the reference itself (oracle/_ref/libref2d.so)
+# --------------------------------------------------------------------------
+
+def ref_path():
+    return os.path.join(_HERE, "_ref", "libref2d.so")
+
+
+def have_ref():
+    return os.path.exists(ref_path())
+
+
+def _ref():
+    global _REF
+    if _REF is None:
+        if not have_ref():
+            raise RuntimeError("oracle/_ref/libref2d.so is not built")
+        _REF = C.CDLL(ref_path())
+    return _REF
+
+
+def _ri(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _rd(v):
+    return C.byref(C.c_double(float(v)))
+
+
+class RefUndefined(RuntimeError):
+    """The reference's behaviour is undefined for this input."""
+
+
+def ref_defined(delta, eps, sources, targ):
+    """False when the reference reads uninitialised memory: with
+    npwlevel == nlevels g2dmpalloc returns before setting lmptot
+    (fgt2d.f:1567) and fgt2d allocates and zeroes rmlexp(lmptot) with that
+    garbage length (fgt2d.f:397-400)."""
+    t = oracle_tree2d(delta, eps, sources, targ if targ is not None else np.zeros((0, 2)))
+    return t["npwlevel"] != t["nlevels"]
+
+
+def ref_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
+    """Calls the reference's own fgt2d_ (fgt2d.f:19) through its Fortran ABI.
+    Raises RefUndefined where the reference reads uninitialised memory."""
+    lib = _ref()
+    if not ref_defined(delta, eps, sources, targ):
+        raise RefUndefined("npwlevel == nlevels: the reference's lmptot is uninitialised")
+    src = _pts(sources)
+    ns = src.shape[0]
+    chg = np.ascontiguousarray(charge, dtype=np.float64).reshape(ns, -1)
+    nd = chg.shape[1]
+    tg = _pts(targ) if targ is not None else np.zeros((1, 2))
+    nt = tg.shape[0] if targ is not None else 0
+    pot, grad, hess = _outs(nd, ns, ifpgh)
+    pt, gt, ht = _outs(nd, max(nt, 1), ifpghtarg)
+    dummy = np.zeros(2 * max(ns, nt, 1))
+    lib.fgt2d_(_ri(nd), _rd(delta), _rd(eps), _ri(ns), _d(src), _ri(1), _d(chg), _ri(0), _d(dummy),
+               _d(dummy), _ri(0), _ri(ifpgh), _d(pot), _d(grad), _d(hess), _ri(nt), _d(tg),
+               _ri(ifpghtarg), _d(pt), _d(gt), _d(ht))
+    return pot, grad, hess, pt[:nt], gt[:nt], ht[:nt]
+
+
+def ref_tree2d(delta, eps, sources, targ):
+    """The reference's own tree routines, called as fgt2d calls them
+    (fgt2d.f:124-221): pts_tree_boxsize0, nlmax loop, pts_tree_mem,
+    pts_tree_build, pts_tree_sort.  Returned 0-based."""
+    lib = _ref()
+    src, tg = _pts(sources), _pts(targ)
+    ns, nt = src.shape[0], tg.shape[0]
+    lc, bs0, cen0 = C.c_int(), C.c_double(), np.zeros(2)
+    lib.pts_tree_boxsize0_(_rd(delta), _rd(eps), _d(src), _ri(ns), _d(tg), _ri(nt), C.byref(lc),
+                           C.byref(bs0), _d(cen0))
+    npwlevel = lc.value
+    nlevstart = max(npwlevel, 0)
+    nlmax = npwlevel + 4
+    bsize = bs0.value / (2 ** nlevstart)
+    for i in range(nlevstart, 101):
+        nl = C.c_int()
+        lib.g2dhlterms_(_ri(2), _rd(bsize), _rd(delta), _rd(eps), C.byref(nl))
+        if nl.value <= 18:
+            nlmax = i
+            break
+        bsize = bsize / 2
+    nlevels, nboxes, ltree = C.c_int(), C.c_int(), C.c_int()
+    lib.pts_tree_mem_(_d(src), _ri(ns), _d(tg), _ri(nt), _ri(0), _ri(80), _ri(0), _ri(nlmax), _ri(0),
+                      _ri(10), _ri(npwlevel), C.byref(bs0), _d(cen0), C.byref(nlevels), C.byref(nboxes),
+                      C.byref(ltree))
+    nl_, nb_, lt_ = nlevels.value, nboxes.value, ltree.value
+    itree = np.zeros(lt_, np.int32)
+    iptr = np.zeros(8, np.int32)
+    cen = np.zeros((nb_, 2))
+    bsz = np.zeros(nl_ + 1)
+    lib.pts_tree_build_(_d(src), _ri(ns), _d(tg), _ri(nt), _ri(0), _ri(80), _ri(0), _ri(0), C.byref(nlevels),
+                        C.byref(nboxes), _ri(10), _ri(npwlevel), C.byref(bs0), _d(cen0), C.byref(ltree),
+                        _i(itree), _i(iptr), _d(cen), _d(bsz))
+    isrc, isrcse = np.zeros(ns, np.int32), np.zeros((nb_, 2), np.int32)
+    itarg, itargse = np.zeros(nt, np.int32), np.zeros((nb_, 2), np.int32)
+    lib.pts_tree_sort_(_ri(ns), _d(src), _i(itree), C.byref(ltree), C.byref(nboxes), C.byref(nlevels),
+                       _i(iptr), _d(cen), _i(isrc), _i(isrcse))
+    lib.pts_tree_sort_(_ri(nt), _d(tg), _i(itree), C.byref(ltree), C.byref(nboxes), C.byref(nlevels),
+                       _i(iptr), _d(cen), _i(itarg), _i(itargse))
+    p = [int(v) - 1 for v in iptr]
+
+    def z(v):  # 1-based box numbers to 0-based, "none" (<= 0) to -1
+        v = np.asarray(v, dtype=np.int32)
+        return np.where(v > 0, v - 1, -1).astype(np.int32)
+
+    nb = nb_
+    laddr = itree[p[0]:p[0] + 2 * (nl_ + 1)].reshape(nl_ + 1, 2) - 1
+    se_s = isrcse.copy()
+    se_s[:, 0] -= 1
+    se_t = itargse.copy()
+    se_t[:, 0] -= 1
+    return dict(npwlevel=npwlevel, nlmax=nlmax, nlevels=nl_, nboxes=nb, laddr=laddr,
+                lev=itree[p[1]:p[1] + nb].copy(), par=z(itree[p[2]:p[2] + nb]),
+                nch=itree[p[3]:p[3] + nb].copy(), ch=z(itree[p[4]:p[4] + 4 * nb]).reshape(nb, 4),
+                ncol=itree[p[5]:p[5] + nb].copy(), col=z(itree[p[6]:p[6] + 9 * nb]).reshape(nb, 9),
+                cen=cen, bsz=bsz, isrc=isrc - 1, isrcse=se_s, itarg=itarg - 1, itargse=se_t,
+                bs0cen0=np.array([bs0.value, cen0[0], cen0[1]]))
