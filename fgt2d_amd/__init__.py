@@ -1,0 +1,228 @@
+"""fgt2d_amd -- plane-wave fast Gauss transform on AMD MI355X (gfx950).
+
+Python side of the drop-in for the reference's planewave driver
+(2d/planewave/src/fgt2d.f:19-22).  The transform runs in the HIP library
+``libfgtpw.so`` (C-ABI in include/fgtpw.h); this module only marshals arrays.
+There is no CPU fallback: when the library is missing the calls raise.
+
+Arrays use the reference's layouts: ``sources`` (ns, 2) C-contiguous float64
+(= Fortran (2, ns)), ``charge`` (ns, nd) or (ns,), outputs (n, nd),
+(n, 2, nd), (n, 3, nd).  numpy arrays go through the host entry point;
+torch tensors on a GPU go through the device entry point on torch's current
+stream, with no host copies.
+"""
+import ctypes as _C
+import os as _os
+
+import numpy as _np
+
+__all__ = ["fgt2d", "fgt2dpart_direct", "FGTError", "last_stats", "last_tree2d",
+           "tree2d_from_leaves", "set_timing", "stage_times", "library_path", "load"]
+
+_HERE = _os.path.dirname(_os.path.abspath(__file__))
+_LIB = None
+
+FGTPW_OK, FGTPW_EINVAL, FGTPW_ENOMEM, FGTPW_EUNSUPPORTED, FGTPW_EDEVICE = 0, -1, -2, -3, -4
+
+
+class FGTError(RuntimeError):
+    """A non-zero status of the C-ABI; ``code`` is the FGTPW_* value."""
+
+    def __init__(self, code, msg):
+        super().__init__("fgtpw error %d: %s" % (code, msg))
+        self.code = code
+
+
+def library_path():
+    return _os.path.join(_HERE, "libfgtpw.so")
+
+
+def load():
+    """Loads libfgtpw.so (built by ``python -m fgt2d_amd.build``)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not _os.path.exists(path):
+        raise ImportError("fgt2d_amd: %s is not built; run python -m fgt2d_amd.build" % path)
+    lib = _C.CDLL(path)
+    dp, ip, vp, i, d = _C.c_void_p, _C.POINTER(_C.c_int), _C.c_void_p, _C.c_int, _C.c_double
+    lib.fgtpw_last_error.restype = _C.c_char_p
+    lib.fgtpw_init.argtypes = [i]
+    lib.fgtpw_fgt2d.argtypes = [i, d, d, i, dp, i, dp, i, dp, dp, i, i, dp, dp, dp, i, dp, i, dp, dp, dp]
+    lib.fgtpw_fgt2d_device.argtypes = lib.fgtpw_fgt2d.argtypes + [vp]
+    lib.fgtpw_fgt2dpart_direct.argtypes = [i, d, d, i, i, i, i, dp, i, dp, i, dp, dp, dp, i, dp, dp, dp]
+    lib.fgtpw_last_stats.argtypes = [ip, i]
+    lib.fgtpw_set_timing.argtypes = [i]
+    lib.fgtpw_stage_times.argtypes = [_C.POINTER(_C.c_double), i]
+    lib.fgtpw_stage_name.restype = _C.c_char_p
+    lib.fgtpw_stage_name.argtypes = [i]
+    lib.fgtpw_last_tree2d.argtypes = [ip, i] + [dp] * 12
+    lib.fgtpw_tree2d_from_leaves.argtypes = [i, i, d, dp, i, dp, dp, ip, i] + [dp] * 8
+    for f in ("fgtpw_init", "fgtpw_fgt2d", "fgtpw_fgt2d_device", "fgtpw_fgt2dpart_direct",
+              "fgtpw_last_stats", "fgtpw_set_timing", "fgtpw_stage_times", "fgtpw_last_tree2d",
+              "fgtpw_tree2d_from_leaves"):
+        getattr(lib, f).restype = _C.c_int
+    _LIB = lib
+    return lib
+
+
+def _check(rc):
+    if rc != FGTPW_OK:
+        raise FGTError(rc, load().fgtpw_last_error().decode())
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return _C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(_C.c_void_p)
+
+
+def _is_torch_gpu(a):
+    return hasattr(a, "is_cuda") and a.is_cuda
+
+
+def _outputs(mod, n, nd, ifp, like):
+    shapes = [(n, nd), (n, 2, nd), (n, 3, nd)]
+    if mod is None:
+        return [(_np.zeros(s) if k < ifp else None) for k, s in enumerate(shapes)]
+    return [(mod.zeros(s, dtype=mod.float64, device=like.device) if k < ifp else None)
+            for k, s in enumerate(shapes)]
+
+
+def fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, ifcharge=1,
+          ifdipole=0, rnormal=None, dipstr=None, iper=0):
+    """The transform of fgt2d (fgt2d.f:19): potentials (and gradients,
+    Hessians for ifpgh/ifpghtarg = 2, 3) of sum_j q_j exp(-|x - y_j|^2/delta)
+    at the sources and/or targets, to precision eps.
+
+    Returns (pot, grad, hess, pottarg, gradtarg, hesstarg), None for what was
+    not requested.  numpy in -> numpy out; torch GPU tensors in -> torch out.
+    """
+    lib = load()
+    gpu = _is_torch_gpu(sources)
+    if gpu:
+        import torch as mod
+        src = sources.contiguous().to(mod.float64)
+        ns = src.shape[0]
+        chg = charge.contiguous().to(mod.float64).reshape(ns, -1)
+        tg = targ.contiguous().to(mod.float64) if targ is not None else None
+    else:
+        mod = None
+        src = _np.ascontiguousarray(sources, dtype=_np.float64)
+        ns = src.shape[0]
+        chg = _np.ascontiguousarray(charge, dtype=_np.float64).reshape(ns, -1)
+        tg = _np.ascontiguousarray(targ, dtype=_np.float64) if targ is not None else None
+    if src.ndim != 2 or src.shape[1] != 2:
+        raise ValueError("sources must have shape (ns, 2)")
+    nd = chg.shape[1]
+    nt = 0 if tg is None else tg.shape[0]
+    if tg is not None and (tg.ndim != 2 or tg.shape[1] != 2):
+        raise ValueError("targ must have shape (nt, 2)")
+    outs = _outputs(mod, ns, nd, ifpgh, src) + _outputs(mod, max(nt, 1), nd, ifpghtarg, src)
+    dummy = _np.zeros(8) if mod is None else mod.zeros(8, dtype=mod.float64, device=src.device)
+    P = [_ptr(o if o is not None else dummy) for o in outs]
+    args = [nd, float(delta), float(eps), ns, _ptr(src), ifcharge, _ptr(chg), ifdipole,
+            _ptr(rnormal if rnormal is not None else dummy), _ptr(dipstr if dipstr is not None else dummy),
+            iper, ifpgh, P[0], P[1], P[2], nt, _ptr(tg if tg is not None else dummy), ifpghtarg,
+            P[3], P[4], P[5]]
+    if gpu:
+        stream = mod.cuda.current_stream(src.device).cuda_stream
+        _check(lib.fgtpw_fgt2d_device(*args, _C.c_void_p(stream)))
+    else:
+        _check(lib.fgtpw_fgt2d(*args))
+    if nt == 0:
+        outs[3:] = [None, None, None]
+    return tuple(outs)
+
+
+def fgt2dpart_direct(delta, eps, sources, charge, targ, ifpgh=1, pot=None, grad=None, hess=None):
+    """fgt2dpart_direct_vec (fgt2d.f:1354): adds the direct sum with cutoff
+    r^2 < ln(1/eps) delta over all sources to pot (grad, hess) at all
+    targets, on the GPU; returns the arrays."""
+    lib = load()
+    src = _np.ascontiguousarray(sources, dtype=_np.float64)
+    chg = _np.ascontiguousarray(charge, dtype=_np.float64).reshape(src.shape[0], -1)
+    tg = _np.ascontiguousarray(targ, dtype=_np.float64)
+    nd, ns, nt = chg.shape[1], src.shape[0], tg.shape[0]
+    pot = _np.zeros((nt, nd)) if pot is None else pot
+    grad = _np.zeros((nt, 2, nd)) if grad is None else grad
+    hess = _np.zeros((nt, 3, nd)) if hess is None else hess
+    dummy = _np.zeros(8)
+    _check(lib.fgtpw_fgt2dpart_direct(nd, float(delta), float(eps), 1, ns, 1, nt, _ptr(src), 1,
+                                      _ptr(chg), 0, _ptr(dummy), _ptr(dummy), _ptr(tg), ifpgh,
+                                      _ptr(pot), _ptr(grad), _ptr(hess)))
+    return pot, grad, hess
+
+
+def last_stats():
+    """npwlevel, nlmax, nlevels, nboxes, npw, leaves and nlocal per level of
+    the last transform (the integers fgt2d.f prints)."""
+    out = _np.zeros(80, dtype=_np.int32)
+    k = load().fgtpw_last_stats(out.ctypes.data_as(_C.POINTER(_C.c_int)), 80)
+    o = [int(v) for v in out[:k]]
+    keys = ["npwlevel", "nlmax", "nlevels", "nboxes", "npw", "nleaves"]
+    r = dict(zip(keys, o[:6]))
+    r["nlocal"] = o[6:]
+    return r
+
+
+def set_timing(on=True):
+    load().fgtpw_set_timing(1 if on else 0)
+
+
+def stage_times():
+    """Milliseconds per stage of the last transform (set_timing(True) first)."""
+    lib = load()
+    buf = (_C.c_double * 32)()
+    k = lib.fgtpw_stage_times(buf, 32)
+    return {lib.fgtpw_stage_name(i).decode(): buf[i] for i in range(k)}
+
+
+def _tree_arrays(nb, nlevels):
+    return dict(laddr=_np.zeros((nlevels + 1, 2), _np.int32), lev=_np.zeros(nb, _np.int32),
+                par=_np.zeros(nb, _np.int32), nch=_np.zeros(nb, _np.int32),
+                ch=_np.zeros((nb, 4), _np.int32), ncol=_np.zeros(nb, _np.int32),
+                col=_np.zeros((nb, 9), _np.int32), cen=_np.zeros((nb, 2)))
+
+
+def last_tree2d(ns, nt):
+    """The quadtree of the last transform (0-based, itree layout)."""
+    lib = load()
+    info = _np.zeros(4, _np.int32)
+    ip = info.ctypes.data_as(_C.POINTER(_C.c_int))
+    _check(lib.fgtpw_last_tree2d(ip, 0, *([None] * 12)))
+    nlevels, nb, nleaves, nlmax = (int(v) for v in info)
+    t = _tree_arrays(nb, nlevels)
+    t.update(isrc=_np.zeros(ns, _np.int32), isrcse=_np.zeros((nb, 2), _np.int32),
+             itarg=_np.zeros(max(nt, 1), _np.int32), itargse=_np.zeros((nb, 2), _np.int32))
+    _check(lib.fgtpw_last_tree2d(ip, nb, _ptr(t["laddr"]), _ptr(t["lev"]), _ptr(t["par"]),
+                                 _ptr(t["nch"]), _ptr(t["ch"]), _ptr(t["ncol"]), _ptr(t["col"]),
+                                 _ptr(t["cen"]), _ptr(t["isrc"]), _ptr(t["isrcse"]),
+                                 _ptr(t["itarg"]), _ptr(t["itargse"])))
+    t["itarg"] = t["itarg"][:nt]
+    t.update(nlevels=nlevels, nboxes=nb, nleaves=nleaves, nlmax=nlmax)
+    return t
+
+
+def tree2d_from_leaves(nlmax, npwlevel, bs0, cen0, keys, levels):
+    """Host-only tree construction from the non-empty leaves of the adaptive
+    tree (Morton prefixes and levels, Morton order)."""
+    lib = load()
+    keys = _np.ascontiguousarray(keys, dtype=_np.uint64)
+    levels = _np.ascontiguousarray(levels, dtype=_np.int32)
+    c0 = _np.ascontiguousarray(cen0, dtype=_np.float64)
+    info = _np.zeros(4, _np.int32)
+    ip = info.ctypes.data_as(_C.POINTER(_C.c_int))
+    _check(lib.fgtpw_tree2d_from_leaves(nlmax, npwlevel, bs0, _ptr(c0), len(keys), _ptr(keys),
+                                        _ptr(levels), ip, 0, *([None] * 8)))
+    nlevels, nb = int(info[0]), int(info[1])
+    t = _tree_arrays(nb, nlevels)
+    _check(lib.fgtpw_tree2d_from_leaves(nlmax, npwlevel, bs0, _ptr(c0), len(keys), _ptr(keys),
+                                        _ptr(levels), ip, nb, _ptr(t["laddr"]), _ptr(t["lev"]),
+                                        _ptr(t["par"]), _ptr(t["nch"]), _ptr(t["ch"]),
+                                        _ptr(t["ncol"]), _ptr(t["col"]), _ptr(t["cen"])))
+    t.update(nlevels=nlevels, nboxes=nb, nleaves=int(info[2]), nlmax=int(info[3]))
+    return t

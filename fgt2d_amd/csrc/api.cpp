@@ -1,0 +1,1029 @@
+// C-ABI and host runtime of the 2D plane-wave FGT.
+//
+// One call = one pass of fgt2d (2d/planewave/src/fgt2d.f:19-462) on the GPU:
+//   bbox  -> term counts on the host (boxsize0, nlmax: fgt2d.f:124-186)
+//   keys  -> radix sort -> leaf levels -> list of leaves  (device)
+//   tree  -> level restriction, lists, tables, work lists  (host, O(boxes))
+//   assign points to leaves, sort, gather                  (device)
+//   form -> merge -> m2l -> split -> eval -> near field -> scatter (device)
+// The host reads back three small things per call (bbox, leaf count, leaf
+// list); everything proportional to the number of points stays in HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fgtpw.h"
+#include "fgtpw_impl.hpp"
+
+namespace fgtpw {
+
+namespace {
+thread_local std::string t_err;
+thread_local std::vector<int> t_stats;
+thread_local std::vector<double> t_stage_ms;
+bool g_timing = false;
+std::mutex g_mutex;  // one transform at a time per process (shared workspace)
+int g_device = -1;
+
+const char* const kStages[] = {"bbox", "keys_sort", "leafdepth", "host_tree", "assign", "form",
+                               "merge", "m2l", "split", "eval", "direct", "scatter"};
+enum Stage { S_BBOX, S_KEYS, S_LEAF, S_HOST, S_ASSIGN, S_FORM, S_MERGE, S_M2L, S_SPLIT, S_EVAL,
+             S_DIRECT, S_SCATTER, S_N };
+}  // namespace
+
+void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void check_hip(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory) fail(FGTPW_ENOMEM, std::string(what) + ": out of device memory");
+    fail(FGTPW_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// workspace: named device buffers that only grow, plus pinned staging
+// ---------------------------------------------------------------------------
+struct Workspace {
+    struct Buf {
+        void* p = nullptr;
+        size_t cap = 0;
+    };
+    std::map<std::string, Buf> dev;
+    void* pin = nullptr;
+    size_t pin_cap = 0;
+    hipStream_t own = nullptr;
+    hipEvent_t ev[2 * S_N] = {};
+    // the tree of the last transform, kept for inspection
+    Tree2D last;
+    int last_ns = 0, last_nt = 0;
+    bool last_valid = false;
+
+    template <class T>
+    T* get(const std::string& name, size_t count)
+    {
+        size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+        Buf& b = dev[name];
+        if (b.cap < bytes) {
+            if (b.p) HIPCHECK(hipFree(b.p));
+            b.p = nullptr;
+            size_t want = bytes + bytes / 8;
+            HIPCHECK(hipMalloc(&b.p, want));
+            b.cap = want;
+        }
+        return (T*)b.p;
+    }
+    void* pinned(size_t bytes)
+    {
+        if (pin_cap < bytes) {
+            if (pin) HIPCHECK(hipHostFree(pin));
+            pin = nullptr;
+            size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 16);
+            HIPCHECK(hipHostMalloc(&pin, want, hipHostMallocDefault));
+            pin_cap = want;
+        }
+        return pin;
+    }
+};
+
+Workspace& workspace()
+{
+    static std::map<int, Workspace*> all;
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    Workspace*& w = all[dev];
+    if (!w) {
+        w = new Workspace();
+        HIPCHECK(hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking));
+        for (auto& e : w->ev) HIPCHECK(hipEventCreate(&e));
+    }
+    return *w;
+}
+
+struct Timer {
+    Workspace& W;
+    hipStream_t s;
+    bool on;
+    double host_ms = 0;
+    bool used[S_N] = {};
+    Timer(Workspace& w, hipStream_t st) : W(w), s(st), on(g_timing) {}
+    void begin(int k)
+    {
+        if (on) { HIPCHECK(hipEventRecord(W.ev[2 * k], s)); used[k] = true; }
+    }
+    void end(int k)
+    {
+        if (on) HIPCHECK(hipEventRecord(W.ev[2 * k + 1], s));
+    }
+    void finish()
+    {
+        t_stage_ms.assign(S_N, 0.0);
+        if (!on) return;
+        HIPCHECK(hipStreamSynchronize(s));
+        for (int k = 0; k < S_N; ++k) {
+            if (!used[k]) continue;
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, W.ev[2 * k], W.ev[2 * k + 1]));
+            t_stage_ms[k] = ms;
+        }
+        t_stage_ms[S_HOST] = host_ms;
+    }
+};
+
+struct Call {
+    int nd;
+    double delta, eps;
+    int ns;
+    const double* src;
+    const double* chg;
+    int ifpgh;
+    double *pot, *grad, *hess;
+    int nt;
+    const double* targ;
+    int ifpghtarg;
+    double *pottarg, *gradtarg, *hesstarg;
+};
+
+void copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// packed host->device upload of the per-call plan
+// ---------------------------------------------------------------------------
+struct Pack {
+    std::vector<char> bytes;
+    template <class T>
+    size_t add(const T* p, size_t n)
+    {
+        size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + std::max<size_t>(n * sizeof(T), 1));
+        if (n) std::memcpy(bytes.data() + off, p, n * sizeof(T));
+        return off;
+    }
+    template <class T>
+    size_t add(const std::vector<T>& v)
+    {
+        return add(v.data(), v.size());
+    }
+};
+
+// single Taylor expansion about the box centre for tiny boxes,
+// fgt2d_large_delta (fgt2d.f:1616-1733)
+void run_large_delta(Workspace& W, const Call& q, hipStream_t s, int npwlevel, double bs0,
+                     const double* cen0, Timer& tm);
+
+void run2d(Workspace& W, const Call& q, hipStream_t s)
+{
+    Timer tm(W, s);
+    const int ns = q.ns, nt = q.nt, nd = q.nd;
+    // ---------------- bounding box (pts_tree_boxsize0) ----------------
+    tm.begin(S_BBOX);
+    int nblk = (int)std::min<long>(1024, std::max<long>(1, ((long)ns + nt + 255) / 256));
+    double* part = W.get<double>("bbox_part", 5 * (size_t)nblk);
+    double* bbd = W.get<double>("bbox", 8);
+    launch_bbox(q.src, ns, q.targ, nt, part, nblk, bbd, s);
+    tm.end(S_BBOX);
+    double* hb = (double*)W.pinned(64);
+    copy_to_host(hb, bbd, 5 * sizeof(double), s);
+    if (hb[4] != 0) fail(FGTPW_EINVAL, "non-finite source or target coordinates");
+    double xmin = hb[0], xmax = hb[1], ymin = hb[2], ymax = hb[3];
+    double bs0 = xmax - xmin, sizey = ymax - ymin;
+    double cen0[2] = {(xmin + xmax) / 2, (ymin + ymax) / 2};
+    if (sizey > bs0) bs0 = sizey;
+    double dc = std::sqrt(q.delta * std::log(1.0 / q.eps));
+    int npwlevel;
+    if (bs0 > 0) {
+        double dl = std::log(bs0 / dc) / std::log(2.0);
+        if (dl > 29) fail(FGTPW_EUNSUPPORTED, "box too large relative to sqrt(delta): tree deeper than 30 levels");
+        npwlevel = (int)dl;
+        if (bs0 >= dc) {
+            bs0 = (double)(1 << (npwlevel + 1)) * dc;
+            npwlevel += 1;
+        }
+    } else {
+        npwlevel = -1000;  // all points coincide: one expansion covers them
+    }
+    if (npwlevel < -2) {
+        run_large_delta(W, q, s, npwlevel, bs0, cen0, tm);
+        return;
+    }
+    // deepest level: first whose Hermite length is <= 18 (fgt2d.f:175-186)
+    const int nlevstart = std::max(npwlevel, 0);
+    int nlmax = npwlevel + 4;
+    {
+        double bsize = bs0 / (double)(1 << nlevstart);
+        for (int i = nlevstart; i <= 100; ++i) {
+            if (hermite_terms(2, bsize, q.delta, q.eps) <= 18) {
+                nlmax = i;
+                break;
+            }
+            bsize = bsize / 2;
+        }
+    }
+    if (nlmax > 31) fail(FGTPW_EUNSUPPORTED, "tree deeper than 31 levels");
+
+    // ---------------- keys, sort, leaves of the adaptive tree ----------------
+    tm.begin(S_KEYS);
+    const int nmx = std::max(ns, nt);
+    using u64 = unsigned long long;
+    u64* ks = W.get<u64>("keys_s", ns);
+    u64* ks2 = W.get<u64>("keys_s2", ns);
+    unsigned* ix = W.get<unsigned>("idx_s", nmx);
+    unsigned* ix2 = W.get<unsigned>("idx_s2", ns);
+    size_t tmpb = std::max(sort_pairs_u64_temp(nmx), sort_pairs_u32_temp(nmx));
+    tmpb = std::max(tmpb, select_flagged_temp(ns));
+    void* tmp = W.get<char>("sort_tmp", tmpb);
+    launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, ks, ix, s);
+    sort_pairs_u64(tmp, tmpb, ks, ks2, ix, ix2, ns, std::max(2, 2 * nlmax), s);
+    tm.end(S_KEYS);
+    tm.begin(S_LEAF);
+    int* lvl = W.get<int>("leaf_level", ns);
+    unsigned char* flg = W.get<unsigned char>("leaf_flag", ns);
+    int* starts = W.get<int>("leaf_starts", ns);
+    int* nsel = W.get<int>("nsel", 4);
+    launch_leafdepth(ks2, ns, nlmax, npwlevel, 10, 80, lvl, flg, s);
+    select_flagged(tmp, tmpb, flg, ns, starts, nsel, s);
+    int* hn = (int*)W.pinned(64);
+    copy_to_host(hn, nsel, sizeof(int), s);
+    const int nleaf0 = hn[0];
+    LeafRun* runs = W.get<LeafRun>("leaf_runs", nleaf0);
+    launch_emit_leaves(starts, nsel, ns, ks2, lvl, nlmax, runs, s);
+    tm.end(S_LEAF);
+    std::vector<LeafRun> leaves(nleaf0);
+    {
+        void* hp = W.pinned(sizeof(LeafRun) * (size_t)nleaf0);
+        copy_to_host(hp, runs, sizeof(LeafRun) * (size_t)nleaf0, s);
+        std::memcpy(leaves.data(), hp, sizeof(LeafRun) * (size_t)nleaf0);
+    }
+
+    // ---------------- host: tree, lists, tables ----------------
+    auto th0 = std::chrono::steady_clock::now();
+    W.last_valid = false;
+    Tree2D& T = W.last;
+    build_tree2d(T, leaves, nlmax, npwlevel, bs0, cen0);
+    W.last_ns = ns;
+    W.last_nt = nt;
+    const int nb = T.nboxes(), nlevels = T.nlevels, nleaves = (int)T.leaf_of_rank.size();
+    const bool use_src = q.ifpgh > 0, use_trg = q.ifpghtarg > 0 && nt > 0;
+    const int npg = std::max(q.ifpgh, q.ifpghtarg);
+    const int nadd = npg;  // ifdipole = 0
+    std::vector<int> nlocal(nlevels + 1, 0);
+    int ntermmax = 0;
+    for (int i = nlevstart; i <= nlevels; ++i) {
+        nlocal[i] = hermite_terms(2, T.bsz[i], q.delta, q.eps);
+        if (q.ifpgh >= 2 || q.ifpghtarg >= 2) nlocal[i] += nadd;
+        ntermmax = std::max(ntermmax, nlocal[i]);
+    }
+    double pmax = 0;
+    int npw = 0;
+    {
+        double bsize = 2 * bs0 / std::pow(2.0, nlevstart);
+        double pweps = nadd > 2 ? q.eps / 100 : q.eps;
+        planewave_terms(bsize, q.delta, pweps, &pmax, &npw);
+    }
+    const bool expansions = npwlevel < nlevels;
+    const int half = npw / 2, nexp = half * npw;
+    const double dsq = 1.0 / std::sqrt(q.delta);
+
+    // box arrays
+    std::vector<int> child(4 * (size_t)nb), slot(nb, -1);
+    std::vector<double> cen(T.cen);
+    int nslot = 0;
+    for (int b = 0; b < nb; ++b) {
+        for (int j = 0; j < 4; ++j) child[4 * b + j] = T.ch[4 * b + j];
+        if (expansions && T.lev[b] >= nlevstart) slot[b] = nslot++;
+    }
+    Pack pk;
+    size_t o_child = pk.add(child), o_par = pk.add(T.par), o_cen = pk.add(cen);
+    size_t o_ncol = pk.add(T.ncol), o_col = pk.add(T.col), o_rank = pk.add(T.rank_of_box);
+    size_t o_rlo = pk.add(T.rank_lo), o_rhi = pk.add(T.rank_hi), o_slot = pk.add(slot);
+
+    // work lists of the expansion steps
+    std::vector<std::vector<int>> form_l(nlevels + 1), merge_l(nlevels + 1), split_l(nlevels + 1),
+        eval_l(nlevels + 1);
+    std::vector<int> m2l_l;
+    std::vector<size_t> o_form(nlevels + 1), o_merge(nlevels + 1), o_split(nlevels + 1),
+        o_eval(nlevels + 1);
+    size_t o_m2l = 0, o_h2x = 0, o_x2l = 0, o_rsq = 0, o_ts = 0, o_wms = 0, o_wm2l = 0;
+    if (expansions) {
+        std::vector<char> iflocal(nb, 0);
+        if (npwlevel >= 0 && npwlevel <= nlevels)
+            for (int b = T.laddr[2 * npwlevel]; b <= T.laddr[2 * npwlevel + 1]; ++b) {
+                if (T.nch[b] != 0) continue;
+                for (int i = 0; i < T.ncol[b]; ++i)
+                    if (T.nch[T.col[9 * b + i]] > 0) { iflocal[b] = 1; break; }
+            }
+        for (int l = 0; l <= nlevels; ++l)
+            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
+                if (T.nch[b] == 0 && l >= std::max(npwlevel + 1, 0)) form_l[l].push_back(b);
+                if (T.nch[b] > 0 && l >= nlevstart && l <= nlevels - 1) {
+                    merge_l[l].push_back(b);
+                    for (int j = 0; j < 4; ++j) split_l[l].push_back(T.ch[4 * b + j]);
+                }
+                if (l == nlevstart) m2l_l.push_back(b);
+                if (T.nch[b] == 0 && l >= nlevstart && !(l == npwlevel && !iflocal[b]))
+                    eval_l[l].push_back(b);
+            }
+        for (int l = 0; l <= nlevels; ++l) {
+            o_form[l] = pk.add(form_l[l]);
+            o_merge[l] = pk.add(merge_l[l]);
+            o_split[l] = pk.add(split_l[l]);
+            o_eval[l] = pk.add(eval_l[l]);
+        }
+        o_m2l = pk.add(m2l_l);
+        // tables: nodes, Hermite->PW, PW->Taylor, shifts
+        std::vector<double> ws(npw), ts(npw);
+        planewave_nodes(pmax, npw, ws.data(), ts.data());
+        const int mt = ntermmax + 1;
+        std::vector<double> h2x(2 * (size_t)mt * npw), x2l((size_t)npw * mt), rsq(mt + 1);
+        rsq[0] = 1;
+        for (int k = 1; k <= mt; ++k) rsq[k] = 1.0 / std::sqrt((double)k);
+        double fh = 1.0, fl = 1.0;
+        for (int j = 0; j < mt; ++j) {
+            if (j > 0) { fh = fh / std::sqrt((double)j); fl = fl * j; }
+            for (int a = 0; a < npw; ++a) {
+                double r = ws[a] * std::pow(-ts[a], j) * fh;  // ws (-ts)^j / sqrt(j!)
+                double re = 0, im = 0;                        // times i^j
+                switch (j & 3) {
+                case 0: re = r; break;
+                case 1: im = r; break;
+                case 2: re = -r; break;
+                default: im = -r; break;
+                }
+                h2x[2 * ((size_t)j * npw + a)] = re;
+                h2x[2 * ((size_t)j * npw + a) + 1] = im;
+                x2l[a + (size_t)npw * j] = std::pow(ts[a], j) / fl;
+            }
+        }
+        // merge/split tables: e^{i ts xmin2 2^(k-1)} by repeated squaring
+        // (merge_split_pw_matrices, g2drouts.f:2294-2350)
+        const int nmax = nlevels - nlevstart;
+        std::vector<double> wms(2 * (size_t)nexp * 4 * std::max(nmax, 1));
+        double xmin2 = T.bsz[nlevels] / std::sqrt(q.delta) / 2;
+        std::vector<double> wwr((size_t)npw * (nmax + 1)), wwi((size_t)npw * (nmax + 1));
+        for (int j = 0; j < npw; ++j) {
+            double zr = std::cos(ts[j] * xmin2), zi = std::sin(ts[j] * xmin2);
+            for (int k = 1; k <= nmax; ++k) {
+                wwr[j + (size_t)npw * k] = zr;
+                wwi[j + (size_t)npw * k] = zi;
+                double t2 = zr * zr - zi * zi;
+                zi = zr * zi + zi * zr;
+                zr = t2;
+            }
+        }
+        for (int k = 1; k <= nmax; ++k)
+            for (int qd = 0; qd < 4; ++qd)
+                for (int b = 0; b < npw; ++b)
+                    for (int a = 0; a < half; ++a) {
+                        double xr = wwr[a + (size_t)npw * k], xi = wwi[a + (size_t)npw * k];
+                        double yr = wwr[b + (size_t)npw * k], yi = wwi[b + (size_t)npw * k];
+                        if (qd >= 2) xi = -xi;  // x shift negative
+                        if (qd & 1) yi = -yi;   // y shift negative
+                        size_t o = 2 * ((size_t)(4 * (k - 1) + qd) * nexp + a + (size_t)half * b);
+                        wms[o] = xr * yr - xi * yi;
+                        wms[o + 1] = xr * yi + xi * yr;
+                    }
+        // m2l tables for shifts of -1, 0, 1 boxes (pw_translation_matrices,
+        // g2drouts.f:2235-2289)
+        std::vector<double> wm2l(2 * (size_t)nexp * 9);
+        double xm = (npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0]) / std::sqrt(q.delta);
+        for (int ky = -1; ky <= 1; ++ky)
+            for (int kx = -1; kx <= 1; ++kx)
+                for (int b = 0; b < npw; ++b)
+                    for (int a = 0; a < half; ++a) {
+                        double xr = 1, xi = 0, yr = 1, yi = 0;
+                        if (kx) { xr = std::cos(ts[a] * xm); xi = kx * std::sin(ts[a] * xm); }
+                        if (ky) { yr = std::cos(ts[b] * xm); yi = ky * std::sin(ts[b] * xm); }
+                        size_t o = 2 * ((size_t)((kx + 1) + 3 * (ky + 1)) * nexp + a + (size_t)half * b);
+                        wm2l[o] = xr * yr - xi * yi;
+                        wm2l[o + 1] = xr * yi + xi * yr;
+                    }
+        o_ts = pk.add(ts);
+        o_h2x = pk.add(h2x);
+        o_x2l = pk.add(x2l);
+        o_rsq = pk.add(rsq);
+        o_wms = pk.add(wms);
+        o_wm2l = pk.add(wm2l);
+    }
+
+    // near field: modified list1 (compute_modified_list1, tree_routs4.f:715-879)
+    // inverted to target units; a unit is a leaf at or above the cutoff level
+    // or a box of the cutoff level, and owns every target below it.
+    const int nlevend = std::min(npwlevel, nlevels);
+    std::vector<int> unit_ids, unit_ptr{0}, unit_list;
+    if (nlevend >= 0) {
+        std::vector<int> nl1(nb, 0), l1(13 * (size_t)nb, -1);
+        if (T.nch[0] == 0) { nl1[0] = 1; l1[0] = 0; }
+        for (int l = 1; l <= nlevels; ++l)
+            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
+                if (T.nch[b] != 0) continue;
+                for (int i = 0; i < T.ncol[b]; ++i) {
+                    int jb = T.col[9 * b + i];
+                    if (l >= npwlevel || T.nch[jb] == 0) {
+                        l1[13 * (size_t)b + nl1[b]++] = jb;
+                    } else {
+                        double tol = 1.05 * (T.bsz[l] + T.bsz[l + 1]) / 2.0;
+                        for (int j = 0; j < T.nch[jb]; ++j) {
+                            int kb = T.ch[4 * jb + j];
+                            if (std::fabs(T.cen[2 * kb] - T.cen[2 * b]) < tol &&
+                                std::fabs(T.cen[2 * kb + 1] - T.cen[2 * b + 1]) < tol) {
+                                l1[13 * (size_t)b + nl1[b]++] = kb;
+                                l1[13 * (size_t)kb + nl1[kb]++] = b;
+                            }
+                        }
+                    }
+                }
+            }
+        // sources of every target box, in the reference's order of source boxes
+        std::vector<std::vector<int>> srcl(nb);
+        for (int l = 0; l <= nlevend; ++l)
+            for (int sb = T.laddr[2 * l]; sb <= T.laddr[2 * l + 1]; ++sb) {
+                if (T.nch[sb] != 0) continue;
+                for (int i = 0; i < nl1[sb]; ++i) srcl[l1[13 * (size_t)sb + i]].push_back(sb);
+            }
+        for (int b = 0; b < nb; ++b) {
+            bool unit = (T.nch[b] == 0 && T.lev[b] <= npwlevel) || T.lev[b] == npwlevel;
+            if (!unit) continue;
+            size_t before = unit_list.size();
+            for (int a = b; a >= 0; a = T.par[a])
+                unit_list.insert(unit_list.end(), srcl[a].begin(), srcl[a].end());
+            if (unit_list.size() == before) continue;
+            unit_ids.push_back(b);
+            unit_ptr.push_back((int)unit_list.size());
+        }
+    }
+    size_t o_units = pk.add(unit_ids), o_uptr = pk.add(unit_ptr), o_ulist = pk.add(unit_list);
+    auto th1 = std::chrono::steady_clock::now();
+    tm.host_ms = std::chrono::duration<double, std::milli>(th1 - th0).count();
+
+    // upload the plan in one copy
+    char* dplan = W.get<char>("plan", pk.bytes.size());
+    {
+        void* hp = W.pinned(pk.bytes.size());
+        std::memcpy(hp, pk.bytes.data(), pk.bytes.size());
+        HIPCHECK(hipMemcpyAsync(dplan, hp, pk.bytes.size(), hipMemcpyHostToDevice, s));
+    }
+    auto D = [&](size_t off) { return (void*)(dplan + off); };
+    const int4* d_child = (const int4*)D(o_child);
+    const int* d_par = (const int*)D(o_par);
+    const double* d_cen = (const double*)D(o_cen);
+    const int* d_ncol = (const int*)D(o_ncol);
+    const int* d_col = (const int*)D(o_col);
+    const int* d_rank = (const int*)D(o_rank);
+    const int* d_rlo = (const int*)D(o_rlo);
+    const int* d_rhi = (const int*)D(o_rhi);
+    const int* d_slot = (const int*)D(o_slot);
+
+    // ---------------- points to leaves, tree order ----------------
+    tm.begin(S_ASSIGN);
+    unsigned* rk = W.get<unsigned>("rank", nmx);
+    unsigned* rk2 = W.get<unsigned>("rank2", nmx);
+    unsigned* isrc = W.get<unsigned>("isrc", ns);
+    int* ls = W.get<int>("leafstart", nleaves + 1);
+    int2* src_se = W.get<int2>("src_se", nb);
+    int2* trg_se = W.get<int2>("trg_se", nb);
+    const int rbits = std::max(1, (int)std::ceil(std::log2((double)nleaves + 1)));
+    double* xs = W.get<double>("xs", 2 * (size_t)ns);
+    double* qs = W.get<double>("qs", (size_t)nd * ns);
+    launch_descend_rank(ks2, ix2, ns, nlmax, d_child, d_rank, rk, s);
+    launch_iota(ix, ns, s);
+    sort_pairs_u32(tmp, tmpb, rk, rk2, ix, isrc, ns, rbits, s);
+    launch_leafstart(rk2, ns, nleaves, ls, s);
+    launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, s);
+    launch_gather(q.src, q.chg, nd, isrc, ns, xs, qs, s);
+    unsigned* itarg = nullptr;
+    double* xt = nullptr;
+    if (nt > 0) {
+        itarg = W.get<unsigned>("itarg", nt);
+        xt = W.get<double>("xt", 2 * (size_t)nt);
+        u64* kt = W.get<u64>("keys_t", nt);
+        launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, kt, nullptr, s);
+        launch_descend_rank(kt, nullptr, nt, nlmax, d_child, d_rank, rk, s);
+        launch_iota(ix, nt, s);
+        sort_pairs_u32(tmp, tmpb, rk, rk2, ix, itarg, nt, rbits, s);
+        launch_leafstart(rk2, nt, nleaves, ls, s);
+        launch_box_ranges(d_rlo, d_rhi, nb, ls, trg_se, s);
+        launch_gather(q.targ, nullptr, 0, itarg, nt, xt, nullptr, s);
+    } else {
+        HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));
+    }
+    tm.end(S_ASSIGN);
+
+    // sorted outputs
+    const int ws_ = use_src ? ns : 0, wt_ = use_trg ? nt : 0;
+    double *ps = nullptr, *gs = nullptr, *hs = nullptr, *pt = nullptr, *gt = nullptr, *ht = nullptr;
+    if (use_src) {
+        ps = W.get<double>("pot_s", (size_t)nd * ws_);
+        launch_zero(ps, (size_t)nd * ws_, s);
+        if (q.ifpgh >= 2) { gs = W.get<double>("grad_s", 2 * (size_t)nd * ws_); launch_zero(gs, 2 * (size_t)nd * ws_, s); }
+        if (q.ifpgh >= 3) { hs = W.get<double>("hess_s", 3 * (size_t)nd * ws_); launch_zero(hs, 3 * (size_t)nd * ws_, s); }
+    }
+    if (use_trg) {
+        pt = W.get<double>("pot_t", (size_t)nd * wt_);
+        launch_zero(pt, (size_t)nd * wt_, s);
+        if (q.ifpghtarg >= 2) { gt = W.get<double>("grad_t", 2 * (size_t)nd * wt_); launch_zero(gt, 2 * (size_t)nd * wt_, s); }
+        if (q.ifpghtarg >= 3) { ht = W.get<double>("hess_t", 3 * (size_t)nd * wt_); launch_zero(ht, 3 * (size_t)nd * wt_, s); }
+    }
+
+    // ---------------- expansions: steps 1-5 ----------------
+    if (expansions) {
+        size_t per = 2 * (size_t)nexp * nd;
+        double* mp = W.get<double>("mp", per * nslot);
+        double* lc = W.get<double>("loc", per * nslot);
+        launch_zero(mp, per * nslot, s);
+        launch_zero(lc, per * nslot, s);
+        const double* d_h2x = (const double*)D(o_h2x);
+        const double* d_x2l = (const double*)D(o_x2l);
+        const double* d_rsq = (const double*)D(o_rsq);
+        const double* d_ts = (const double*)D(o_ts);
+        const double* d_wms = (const double*)D(o_wms);
+        const double* d_wm2l = (const double*)D(o_wm2l);
+        size_t maxeval = 0;
+        for (int l = 0; l <= nlevels; ++l)
+            if (nlocal[l] <= 50)
+                maxeval = std::max(maxeval, eval_l[l].size() * (size_t)(nlocal[l] + 1) * (nlocal[l] + 1));
+        double* taylor = W.get<double>("taylor", maxeval);
+        for (int ind = 0; ind < nd; ++ind) {
+            tm.begin(S_FORM);
+            for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l) {
+                FormArgs fa{(const int*)D(o_form[l]), (int)form_l[l].size(), d_cen, src_se, d_slot,
+                            xs, qs, nd, ind, dsq, nlocal[l], d_h2x, d_rsq, npw, mp};
+                launch_form(fa, s);
+            }
+            tm.end(S_FORM);
+        }
+        tm.begin(S_MERGE);
+        for (int l = nlevels - 1; l >= nlevstart; --l)
+            launch_merge((const int*)D(o_merge[l]), (int)merge_l[l].size(), d_child, d_cen, src_se,
+                         d_slot, d_wms, 4 * (nlevels - l - 1), nd, nexp, mp, s);
+        tm.end(S_MERGE);
+        tm.begin(S_M2L);
+        launch_m2l((const int*)D(o_m2l), (int)m2l_l.size(), d_cen, d_ncol, d_col, src_se, trg_se,
+                   use_src, use_trg, d_slot, d_wm2l, npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0],
+                   nlevstart == npwlevel, nd, nexp, mp, lc, s);
+        tm.end(S_M2L);
+        tm.begin(S_SPLIT);
+        for (int l = nlevstart; l <= nlevels - 1; ++l)
+            launch_split((const int*)D(o_split[l]), (int)split_l[l].size(), d_par, d_cen, d_slot,
+                         d_wms, 4 * (nlevels - l - 1), nd, nexp, lc, s);
+        tm.end(S_SPLIT);
+        tm.begin(S_EVAL);
+        for (int ind = 0; ind < nd; ++ind)
+            for (int l = nlevstart; l <= nlevels; ++l) {
+                if (eval_l[l].empty()) continue;
+                EvalArgs ea{};
+                ea.boxes = (const int*)D(o_eval[l]);
+                ea.nbox = (int)eval_l[l].size();
+                ea.cen = d_cen;
+                ea.slot = d_slot;
+                ea.nd = nd;
+                ea.ind = ind;
+                ea.dsq = dsq;
+                ea.nl = nlocal[l];
+                ea.x2l = d_x2l;
+                ea.ts = d_ts;
+                ea.npw = npw;
+                ea.loc = lc;
+                ea.taylor = taylor;
+                bool pwroute = nlocal[l] > 50;  // nlocalswitch, fgt2d.f:855
+                if (!pwroute) launch_pw2local(ea, s);
+                for (int side = 0; side < 2; ++side) {
+                    if (side == 0 && !use_trg) continue;
+                    if (side == 1 && !use_src) continue;
+                    EvalArgs eb = ea;
+                    eb.se = side == 0 ? trg_se : src_se;
+                    eb.pts = side == 0 ? xt : xs;
+                    eb.ifpgh = side == 0 ? q.ifpghtarg : q.ifpgh;
+                    eb.pot = side == 0 ? pt : ps;
+                    eb.grad = side == 0 ? gt : gs;
+                    eb.hess = side == 0 ? ht : hs;
+                    if (pwroute)
+                        launch_pw_eval(eb, s);
+                    else
+                        launch_taylor_eval(eb, s);
+                }
+            }
+        tm.end(S_EVAL);
+    }
+
+    // ---------------- step 6: near field, cutoff eps/90 (fgt2d.f:1316) ----------------
+    tm.begin(S_DIRECT);
+    if (!unit_ids.empty()) {
+        DirectArgs da{};
+        da.units = (const int*)D(o_units);
+        da.nunits = (int)unit_ids.size();
+        da.list_ptr = (const int*)D(o_uptr);
+        da.list = (const int*)D(o_ulist);
+        da.src_se = src_se;
+        da.xs = xs;
+        da.qs = qs;
+        da.nd = nd;
+        da.delta = q.delta;
+        double e90 = q.eps / 90;
+        da.dmax = std::log(1.0 / e90) * q.delta;
+        if (use_trg) {
+            DirectArgs d2 = da;
+            d2.tgt_se = trg_se;
+            d2.xt = xt;
+            d2.ifpgh = q.ifpghtarg;
+            d2.pot = pt; d2.grad = gt; d2.hess = ht;
+            launch_direct(d2, s);
+        }
+        if (use_src) {
+            DirectArgs d2 = da;
+            d2.tgt_se = src_se;
+            d2.xt = xs;
+            d2.ifpgh = q.ifpgh;
+            d2.pot = ps; d2.grad = gs; d2.hess = hs;
+            launch_direct(d2, s);
+        }
+    }
+    tm.end(S_DIRECT);
+
+    // ---------------- back to input order ----------------
+    tm.begin(S_SCATTER);
+    if (use_src) {
+        launch_scatter(ps, isrc, ns, nd, q.pot, s);
+        if (q.ifpgh >= 2) launch_scatter(gs, isrc, ns, 2 * nd, q.grad, s);
+        if (q.ifpgh >= 3) launch_scatter(hs, isrc, ns, 3 * nd, q.hess, s);
+    }
+    if (use_trg) {
+        launch_scatter(pt, itarg, nt, nd, q.pottarg, s);
+        if (q.ifpghtarg >= 2) launch_scatter(gt, itarg, nt, 2 * nd, q.gradtarg, s);
+        if (q.ifpghtarg >= 3) launch_scatter(ht, itarg, nt, 3 * nd, q.hesstarg, s);
+    }
+    tm.end(S_SCATTER);
+    HIPCHECK(hipGetLastError());
+    tm.finish();
+    W.last_valid = true;
+
+    t_stats.assign(6 + nlevels + 1, 0);
+    t_stats[0] = npwlevel;
+    t_stats[1] = nlmax;
+    t_stats[2] = nlevels;
+    t_stats[3] = nb;
+    t_stats[4] = npw;
+    t_stats[5] = nleaves;
+    for (int l = 0; l <= nlevels; ++l) t_stats[6 + l] = nlocal[l];
+}
+
+}  // namespace
+
+namespace {
+
+// fgt2d_large_delta (fgt2d.f:1616-1733): the whole point set fits in a box
+// much smaller than sqrt(delta); one Taylor expansion about the bounding-box
+// centre (g2dformlc_vec, g2drouts.f:621) is evaluated at every point.  The
+// outputs are overwritten here; the reference adds into the caller's arrays
+// in this branch only.
+void run_large_delta(Workspace& W, const Call& q, hipStream_t s, int npwlevel, double bs0,
+                     const double* cen0, Timer& tm)
+{
+    const int nd = q.nd;
+    int nl = hermite_terms(2, bs0, q.delta, q.eps) + std::max(q.ifpgh, q.ifpghtarg) - 1;
+    if (nl < 0) fail(FGTPW_EUNSUPPORTED, "no Taylor order reaches eps for this box");
+    const int m = nl + 1;
+    const double dsq = 1.0 / std::sqrt(q.delta);
+    int nparts = std::max(1, std::min(512, (q.ns + 1023) / 1024));
+    double* part = W.get<double>("ld_part", (size_t)nparts * m * m);
+    double* L = W.get<double>("ld_local", (size_t)m * m * nd);
+    // one box (id 0) holding every point, evaluated in input order:
+    // ints {box, pad, 0, ns, 0, nt} give the box list and two int2 ranges
+    int hostarr[6] = {0, 0, 0, q.ns, 0, q.nt};
+    int* dint = W.get<int>("ld_int", 8);
+    double* dcen = W.get<double>("ld_cen", 2);
+    char* hp = (char*)W.pinned(128);
+    std::memcpy(hp, hostarr, sizeof(hostarr));
+    std::memcpy(hp + 64, cen0, 2 * sizeof(double));
+    HIPCHECK(hipMemcpyAsync(dint, hp, sizeof(hostarr), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(dcen, hp + 64, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+    tm.begin(S_FORM);
+    for (int ind = 0; ind < nd; ++ind)
+        launch_formlc(q.src, q.chg, nd, ind, q.ns, cen0[0], cen0[1], dsq, nl, part, nparts,
+                      L + (size_t)ind * m * m, s);
+    tm.end(S_FORM);
+    tm.begin(S_EVAL);
+    if (q.ifpgh > 0) {
+        launch_zero(q.pot, (size_t)nd * q.ns, s);
+        if (q.ifpgh >= 2) launch_zero(q.grad, 2 * (size_t)nd * q.ns, s);
+        if (q.ifpgh >= 3) launch_zero(q.hess, 3 * (size_t)nd * q.ns, s);
+    }
+    if (q.ifpghtarg > 0 && q.nt > 0) {
+        launch_zero(q.pottarg, (size_t)nd * q.nt, s);
+        if (q.ifpghtarg >= 2) launch_zero(q.gradtarg, 2 * (size_t)nd * q.nt, s);
+        if (q.ifpghtarg >= 3) launch_zero(q.hesstarg, 3 * (size_t)nd * q.nt, s);
+    }
+    for (int ind = 0; ind < nd; ++ind)
+        for (int side = 0; side < 2; ++side) {
+            EvalArgs e{};
+            e.boxes = dint;
+            e.nbox = 1;
+            e.cen = dcen;
+            e.nd = nd;
+            e.ind = ind;
+            e.dsq = dsq;
+            e.nl = nl;
+            e.taylor = L + (size_t)ind * m * m;
+            if (side == 0) {
+                if (q.ifpgh <= 0) continue;
+                e.se = (const int2*)(dint + 2);
+                e.pts = q.src;
+                e.ifpgh = q.ifpgh;
+                e.pot = q.pot; e.grad = q.grad; e.hess = q.hess;
+            } else {
+                if (q.ifpghtarg <= 0 || q.nt <= 0) continue;
+                e.se = (const int2*)(dint + 4);
+                e.pts = q.targ;
+                e.ifpgh = q.ifpghtarg;
+                e.pot = q.pottarg; e.grad = q.gradtarg; e.hess = q.hesstarg;
+            }
+            launch_taylor_eval(e, s);
+        }
+    tm.end(S_EVAL);
+    HIPCHECK(hipGetLastError());
+    tm.finish();
+    t_stats.assign(7, 0);
+    t_stats[0] = npwlevel;
+    t_stats[6] = nl;
+}
+
+int run_guarded(const std::function<void()>& f)
+{
+    t_err.clear();
+    try {
+        f();
+        return FGTPW_OK;
+    } catch (const Error& e) {
+        t_err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        t_err = "host allocation failed";
+        return FGTPW_ENOMEM;
+    } catch (const std::exception& e) {
+        t_err = e.what();
+        return FGTPW_EDEVICE;
+    }
+}
+
+void validate(int nd, double delta, double eps, int ns, int ifcharge, int ifdipole, int ifpgh,
+              int nt, int ifpghtarg)
+{
+    if (nd < 1) fail(FGTPW_EINVAL, "nd must be >= 1");
+    if (ns < 1) fail(FGTPW_EINVAL, "ns must be >= 1");
+    if (nt < 0) fail(FGTPW_EINVAL, "nt must be >= 0");
+    if (!(delta > 0) || !std::isfinite(delta)) fail(FGTPW_EINVAL, "delta must be positive");
+    if (!(eps > 0) || !(eps < 1)) fail(FGTPW_EINVAL, "eps must be in (0,1)");
+    if (ifpgh < 0 || ifpgh > 3 || ifpghtarg < 0 || ifpghtarg > 3)
+        fail(FGTPW_EINVAL, "ifpgh and ifpghtarg must be in 0..3");
+    if (ifdipole == 1) fail(FGTPW_EUNSUPPORTED, "dipole sources are not supported in this build");
+    if (ifcharge != 1) fail(FGTPW_EINVAL, "ifcharge must be 1");
+}
+
+void select_device()
+{
+    if (g_device >= 0) HIPCHECK(hipSetDevice(g_device));
+}
+}  // namespace
+}  // namespace fgtpw
+
+using namespace fgtpw;
+
+extern "C" {
+
+const char* fgtpw_last_error(void) { return t_err.c_str(); }
+
+int fgtpw_init(int device)
+{
+    return run_guarded([&] {
+        int n = 0;
+        HIPCHECK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) fail(FGTPW_EINVAL, "no such HIP device");
+        g_device = device;
+        HIPCHECK(hipSetDevice(device));
+    });
+}
+
+int fgtpw_fgt2d_device(int nd, double delta, double eps, int ns, const double* sources,
+                       int ifcharge, const double* charge, int ifdipole, const double* rnormal,
+                       const double* dipstr, int iper, int ifpgh, double* pot, double* grad,
+                       double* hess, int nt, const double* targ, int ifpghtarg, double* pottarg,
+                       double* gradtarg, double* hesstarg, void* stream)
+{
+    (void)rnormal; (void)dipstr; (void)iper;
+    return run_guarded([&] {
+        validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
+        std::lock_guard<std::mutex> lk(g_mutex);
+        select_device();
+        Workspace& W = workspace();
+        Call q{nd, delta, eps, ns, sources, charge, ifpgh, pot, grad, hess,
+               nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg};
+        hipStream_t s = (hipStream_t)stream;
+        run2d(W, q, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    });
+}
+
+int fgtpw_fgt2d(int nd, double delta, double eps, int ns, const double* sources, int ifcharge,
+                const double* charge, int ifdipole, const double* rnormal, const double* dipstr,
+                int iper, int ifpgh, double* pot, double* grad, double* hess, int nt,
+                const double* targ, int ifpghtarg, double* pottarg, double* gradtarg,
+                double* hesstarg)
+{
+    (void)rnormal; (void)dipstr; (void)iper;
+    return run_guarded([&] {
+        validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
+        std::lock_guard<std::mutex> lk(g_mutex);
+        select_device();
+        Workspace& W = workspace();
+        hipStream_t s = W.own;
+        const size_t n2s = 2 * (size_t)ns, n2t = 2 * (size_t)nt;
+        double* d_src = W.get<double>("in_src", n2s);
+        double* d_chg = W.get<double>("in_chg", (size_t)nd * ns);
+        double* d_trg = W.get<double>("in_trg", std::max<size_t>(n2t, 2));
+        HIPCHECK(hipMemcpyAsync(d_src, sources, n2s * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d_chg, charge, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+        if (nt > 0) HIPCHECK(hipMemcpyAsync(d_trg, targ, n2t * 8, hipMemcpyHostToDevice, s));
+        const size_t wps = ifpgh > 0 ? (size_t)ns : 0, wpt = ifpghtarg > 0 ? (size_t)nt : 0;
+        double* o_p = W.get<double>("out_pot", std::max<size_t>(nd * wps, 1));
+        double* o_g = W.get<double>("out_grad", std::max<size_t>(2 * nd * wps, 1));
+        double* o_h = W.get<double>("out_hess", std::max<size_t>(3 * nd * wps, 1));
+        double* o_pt = W.get<double>("out_pott", std::max<size_t>(nd * wpt, 1));
+        double* o_gt = W.get<double>("out_gradt", std::max<size_t>(2 * nd * wpt, 1));
+        double* o_ht = W.get<double>("out_hesst", std::max<size_t>(3 * nd * wpt, 1));
+        Call q{nd, delta, eps, ns, d_src, d_chg, ifpgh, o_p, o_g, o_h,
+               nt, d_trg, ifpghtarg, o_pt, o_gt, o_ht};
+        run2d(W, q, s);
+        if (ifpgh >= 1) HIPCHECK(hipMemcpyAsync(pot, o_p, nd * wps * 8, hipMemcpyDeviceToHost, s));
+        if (ifpgh >= 2) HIPCHECK(hipMemcpyAsync(grad, o_g, 2 * nd * wps * 8, hipMemcpyDeviceToHost, s));
+        if (ifpgh >= 3) HIPCHECK(hipMemcpyAsync(hess, o_h, 3 * nd * wps * 8, hipMemcpyDeviceToHost, s));
+        if (ifpghtarg >= 1 && nt > 0)
+            HIPCHECK(hipMemcpyAsync(pottarg, o_pt, nd * wpt * 8, hipMemcpyDeviceToHost, s));
+        if (ifpghtarg >= 2 && nt > 0)
+            HIPCHECK(hipMemcpyAsync(gradtarg, o_gt, 2 * nd * wpt * 8, hipMemcpyDeviceToHost, s));
+        if (ifpghtarg >= 3 && nt > 0)
+            HIPCHECK(hipMemcpyAsync(hesstarg, o_ht, 3 * nd * wpt * 8, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    });
+}
+
+int fgtpw_fgt2dpart_direct(int nd, double delta, double eps, int istart, int iend, int jstart,
+                           int jend, const double* source, int ifcharge, const double* charge,
+                           int ifdipole, const double* rnormal, const double* dipstr,
+                           const double* targ, int ifpgh, double* pot, double* grad, double* hess)
+{
+    (void)rnormal; (void)dipstr;
+    return run_guarded([&] {
+        if (nd < 1) fail(FGTPW_EINVAL, "nd must be >= 1");
+        if (ifdipole == 1) fail(FGTPW_EUNSUPPORTED, "dipole sources are not supported in this build");
+        if (ifcharge != 1) fail(FGTPW_EINVAL, "ifcharge must be 1");
+        if (ifpgh < 1 || ifpgh > 3) return;  // the reference does nothing for other values
+        if (!(delta > 0) || !(eps > 0)) fail(FGTPW_EINVAL, "delta and eps must be positive");
+        const int ns = iend - istart + 1, nt = jend - jstart + 1;
+        if (ns <= 0 || nt <= 0) return;
+        if (istart < 1 || jstart < 1) fail(FGTPW_EINVAL, "ranges are 1-based");
+        std::lock_guard<std::mutex> lk(g_mutex);
+        select_device();
+        Workspace& W = workspace();
+        hipStream_t s = W.own;
+        const double* src = source + 2 * (size_t)(istart - 1);
+        const double* chg = charge + (size_t)nd * (istart - 1);
+        const double* tg = targ + 2 * (size_t)(jstart - 1);
+        const size_t w = (size_t)nd * (ifpgh == 1 ? 1 : ifpgh == 2 ? 3 : 6);
+        double* d_src = W.get<double>("pd_src", 2 * (size_t)ns);
+        double* d_chg = W.get<double>("pd_chg", (size_t)nd * ns);
+        double* d_tg = W.get<double>("pd_tg", 2 * (size_t)nt);
+        double* d_p = W.get<double>("pd_pot", (size_t)nd * nt);
+        double* d_g = W.get<double>("pd_grad", 2 * (size_t)nd * nt);
+        double* d_h = W.get<double>("pd_hess", 3 * (size_t)nd * nt);
+        (void)w;
+        HIPCHECK(hipMemcpyAsync(d_src, src, 2 * (size_t)ns * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d_chg, chg, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d_tg, tg, 2 * (size_t)nt * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d_p, pot + (size_t)nd * (jstart - 1), (size_t)nd * nt * 8,
+                                hipMemcpyHostToDevice, s));
+        if (ifpgh >= 2)
+            HIPCHECK(hipMemcpyAsync(d_g, grad + 2 * (size_t)nd * (jstart - 1), 2 * (size_t)nd * nt * 8,
+                                    hipMemcpyHostToDevice, s));
+        if (ifpgh >= 3)
+            HIPCHECK(hipMemcpyAsync(d_h, hess + 3 * (size_t)nd * (jstart - 1), 3 * (size_t)nd * nt * 8,
+                                    hipMemcpyHostToDevice, s));
+        double dmax = std::log(1.0 / eps) * delta;
+        launch_direct_all(d_src, d_chg, nd, ns, d_tg, nt, delta, dmax, ifpgh, d_p, d_g, d_h, s);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(pot + (size_t)nd * (jstart - 1), d_p, (size_t)nd * nt * 8,
+                                hipMemcpyDeviceToHost, s));
+        if (ifpgh >= 2)
+            HIPCHECK(hipMemcpyAsync(grad + 2 * (size_t)nd * (jstart - 1), d_g, 2 * (size_t)nd * nt * 8,
+                                    hipMemcpyDeviceToHost, s));
+        if (ifpgh >= 3)
+            HIPCHECK(hipMemcpyAsync(hess + 3 * (size_t)nd * (jstart - 1), d_h, 3 * (size_t)nd * nt * 8,
+                                    hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    });
+}
+
+int fgtpw_last_tree2d(int* info, int cap, int* laddr, int* lev, int* par, int* nch, int* ch,
+                      int* ncol, int* col, double* cen, int* isrc, int* isrcse, int* itarg,
+                      int* itargse)
+{
+    return run_guarded([&] {
+        std::lock_guard<std::mutex> lk(g_mutex);
+        select_device();
+        Workspace& W = workspace();
+        if (!W.last_valid) fail(FGTPW_EINVAL, "no tree: the last call built none");
+        const Tree2D& T = W.last;
+        const int nb = T.nboxes();
+        info[0] = T.nlevels;
+        info[1] = nb;
+        info[2] = (int)T.leaf_of_rank.size();
+        info[3] = T.nlmax;
+        if (!laddr || cap < nb) return;
+        std::memcpy(laddr, T.laddr.data(), sizeof(int) * T.laddr.size());
+        std::memcpy(lev, T.lev.data(), sizeof(int) * nb);
+        std::memcpy(par, T.par.data(), sizeof(int) * nb);
+        std::memcpy(nch, T.nch.data(), sizeof(int) * nb);
+        std::memcpy(ch, T.ch.data(), sizeof(int) * 4 * nb);
+        std::memcpy(ncol, T.ncol.data(), sizeof(int) * nb);
+        std::memcpy(col, T.col.data(), sizeof(int) * 9 * nb);
+        std::memcpy(cen, T.cen.data(), sizeof(double) * 2 * nb);
+        hipStream_t s = W.own;
+        HIPCHECK(hipMemcpyAsync(isrc, W.get<unsigned>("isrc", W.last_ns), sizeof(int) * W.last_ns,
+                                hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(isrcse, W.get<int2>("src_se", nb), sizeof(int2) * nb,
+                                hipMemcpyDeviceToHost, s));
+        if (W.last_nt > 0) {
+            HIPCHECK(hipMemcpyAsync(itarg, W.get<unsigned>("itarg", W.last_nt),
+                                    sizeof(int) * W.last_nt, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipMemcpyAsync(itargse, W.get<int2>("trg_se", nb), sizeof(int2) * nb,
+                                    hipMemcpyDeviceToHost, s));
+        }
+        HIPCHECK(hipStreamSynchronize(s));
+    });
+}
+
+int fgtpw_tree2d_from_leaves(int nlmax, int npwlevel, double bs0, const double* cen0, int nleaves,
+                             const unsigned long long* keys, const int* levels, int* info,
+                             int cap, int* laddr, int* lev, int* par, int* nch, int* ch,
+                             int* ncol, int* col, double* cen)
+{
+    return run_guarded([&] {
+        std::vector<LeafRun> leaves(nleaves);
+        for (int i = 0; i < nleaves; ++i) leaves[i] = LeafRun{keys[i], levels[i], 0, 0};
+        Tree2D T;
+        build_tree2d(T, leaves, nlmax, npwlevel, bs0, cen0);
+        const int nb = T.nboxes();
+        info[0] = T.nlevels;
+        info[1] = nb;
+        info[2] = (int)T.leaf_of_rank.size();
+        info[3] = T.nlmax;
+        if (!laddr || cap < nb) return;
+        std::memcpy(laddr, T.laddr.data(), sizeof(int) * T.laddr.size());
+        std::memcpy(lev, T.lev.data(), sizeof(int) * nb);
+        std::memcpy(par, T.par.data(), sizeof(int) * nb);
+        std::memcpy(nch, T.nch.data(), sizeof(int) * nb);
+        std::memcpy(ch, T.ch.data(), sizeof(int) * 4 * nb);
+        std::memcpy(ncol, T.ncol.data(), sizeof(int) * nb);
+        std::memcpy(col, T.col.data(), sizeof(int) * 9 * nb);
+        std::memcpy(cen, T.cen.data(), sizeof(double) * 2 * nb);
+    });
+}
+
+int fgtpw_last_stats(int* out, int n)
+{
+    int k = std::min(n, (int)t_stats.size());
+    for (int i = 0; i < k; ++i) out[i] = t_stats[i];
+    return k;
+}
+
+int fgtpw_set_timing(int on)
+{
+    g_timing = on != 0;
+    return FGTPW_OK;
+}
+
+int fgtpw_stage_times(double* ms, int n)
+{
+    int k = std::min(n, (int)t_stage_ms.size());
+    for (int i = 0; i < k; ++i) ms[i] = t_stage_ms[i];
+    return k;
+}
+
+const char* fgtpw_stage_name(int i)
+{
+    if (i < 0 || i >= (int)(sizeof(kStages) / sizeof(kStages[0]))) return "";
+    return kStages[i];
+}
+
+}  // extern "C"

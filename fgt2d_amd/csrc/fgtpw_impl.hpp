@@ -1,0 +1,171 @@
+// Internal declarations shared by the host runtime (api.cpp, tree2d.cpp) and
+// the HIP kernels (kernels2d.hip).  Not part of the public C-ABI, which is
+// include/fgtpw.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fgtpw {
+
+// ---------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------
+struct Error {
+    int code;
+    std::string msg;
+};
+[[noreturn]] void fail(int code, const std::string& msg);
+void check_hip(hipError_t e, const char* what);
+#define HIPCHECK(x) ::fgtpw::check_hip((x), #x)
+
+// ---------------------------------------------------------------------------
+// term counts (g2dterms.f) and plane-wave nodes (g2drouts.f:2205)
+// ---------------------------------------------------------------------------
+int hermite_terms(int ndim, double bsize, double delta, double eps);
+void planewave_terms(double bsize, double delta, double eps, double* pmax, int* npw);
+void planewave_nodes(double pmax, int npw, double* ws, double* ts);
+
+// ---------------------------------------------------------------------------
+// host box tree (built from the GPU's list of pre-restriction leaves)
+// ---------------------------------------------------------------------------
+struct LeafRun {            // one non-empty leaf of the adaptive tree before
+    uint64_t key;           // level restriction: its Morton prefix (level
+    int level;              // digits, most significant first), its level,
+    int start;              // and its first position in the key-sorted
+    int count;              // source list, and its number of sources
+};
+
+struct Tree2D {
+    int nlmax = 0, npwlevel = 0, nlevels = 0;
+    double bs0 = 0, cen0[2] = {0, 0};
+    std::vector<double> bsz;   // box size per level
+    std::vector<int> laddr;    // first, last box of each level
+    std::vector<int> lev, par, nch, ch, ncol, col, flag;
+    std::vector<double> cen;
+    std::vector<uint64_t> pre;  // Morton prefix of every box
+    int nboxes() const { return (int)lev.size(); }
+    // leaves in Morton order and, per box, its interval of leaf ranks
+    std::vector<int> leaf_of_rank, rank_of_box, rank_lo, rank_hi;
+};
+
+void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int npwlevel,
+                  double bs0, const double* cen0);
+
+// ---------------------------------------------------------------------------
+// device kernels (kernels2d.hip)
+// ---------------------------------------------------------------------------
+struct DevArena;  // fwd
+
+void launch_bbox(const double* a, int na, const double* b, int nb, double* part, int nblk,
+                 double* out, hipStream_t s);
+void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, int nlmax,
+                   unsigned long long* key, unsigned* idx, hipStream_t s);
+void launch_leafdepth(const unsigned long long* key, int n, int nlmax, int levcut, int ndiv0,
+                      int ndiv, int* level, unsigned char* start_flag, hipStream_t s);
+size_t sort_pairs_u64_temp(int n);
+void sort_pairs_u64(void* tmp, size_t tmpb, const unsigned long long* kin, unsigned long long* kout,
+                    const unsigned* vin, unsigned* vout, int n, int endbit, hipStream_t s);
+size_t sort_pairs_u32_temp(int n);
+void sort_pairs_u32(void* tmp, size_t tmpb, const unsigned* kin, unsigned* kout, const unsigned* vin,
+                    unsigned* vout, int n, int endbit, hipStream_t s);
+size_t select_flagged_temp(int n);
+void select_flagged(void* tmp, size_t tmpb, const unsigned char* flags, int n, int* out, int* nsel,
+                    hipStream_t s);
+void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigned long long* key,
+                        const int* level, int nlmax, LeafRun* out, hipStream_t s);
+void launch_descend_rank(const unsigned long long* key, const unsigned* idx, int n, int nlmax,
+                         const int4* child, const int* rank_of_box, unsigned* rank_out,
+                         hipStream_t s);
+void launch_iota(unsigned* v, int n, hipStream_t s);
+void launch_leafstart(const unsigned* rank_sorted, int n, int nleaves, int* leafstart,
+                      hipStream_t s);
+void launch_box_ranges(const int* rank_lo, const int* rank_hi, int nb, const int* ls, int2* se,
+                       hipStream_t s);
+void launch_gather(const double* xy, const double* q, int nd, const unsigned* perm, int n,
+                   double* xy_s, double* q_s, hipStream_t s);
+void launch_zero(double* p, size_t n, hipStream_t s);
+void launch_scatter(const double* v_s, const unsigned* perm, int n, int w, double* v,
+                    hipStream_t s);
+
+struct FormArgs {
+    const int* boxes;
+    int nbox;
+    const double* cen;      // double2 per box
+    const int2* src_se;
+    const int* slot;
+    const double* xs;       // sorted source xy
+    const double* qs;       // sorted charges (n, nd)
+    int nd, ind;
+    double dsq;
+    int p;                  // Hermite order
+    const double* h2x;      // complex (p+1, npw): ws (-i ts)^j / sqrt(j!)
+    const double* rsq;      // 1/sqrt(k)
+    int npw;
+    double* mp;             // complex per slot (nd, nexp)
+};
+void launch_form(const FormArgs& a, hipStream_t s);
+
+void launch_merge(const int* parents, int np, const int4* child, const double* cen,
+                  const int2* src_se, const int* slot, const double* wms, int kidx_base,
+                  int nd, int nexp, double* mp, hipStream_t s);
+void launch_m2l(const int* boxes, int nb, const double* cen, const int* ncol, const int* col,
+                const int2* src_se, const int2* trg_se, int use_src, int use_trg,
+                const int* slot, const double* wm2l, double xmin, int do_list, int nd, int nexp,
+                const double* mp, double* loc, hipStream_t s);
+void launch_split(const int* children, int nc, const int* par, const double* cen,
+                  const int* slot, const double* wms, int kidx_base, int nd, int nexp,
+                  double* loc, hipStream_t s);
+
+struct EvalArgs {
+    const int* boxes;
+    int nbox;
+    const double* cen;
+    const int2* se;         // point ranges of the boxes
+    const int* slot;
+    const double* pts;      // sorted points
+    int nd, ind;
+    double dsq;
+    int ifpgh;
+    int nl;                 // Taylor order (Taylor route)
+    const double* x2l;      // real (npw, nl+1): ts^j / j!
+    const double* ts;       // nodes (PW route)
+    int npw;
+    const double* loc;      // PW local expansions per slot
+    double* taylor;         // scratch: (nbox, (nl+1)^2)
+    double *pot, *grad, *hess;  // sorted outputs
+};
+void launch_pw2local(const EvalArgs& a, hipStream_t s);
+void launch_taylor_eval(const EvalArgs& a, hipStream_t s);
+void launch_pw_eval(const EvalArgs& a, hipStream_t s);
+
+struct DirectArgs {
+    const int* units;       // target units (boxes)
+    int nunits;
+    const int* list_ptr;    // CSR over units: source boxes
+    const int* list;
+    const int2* src_se;     // source ranges of boxes (sources sorted)
+    const int2* tgt_se;     // target ranges of boxes
+    const double* xs;
+    const double* qs;
+    const double* xt;
+    int nd;
+    double delta, dmax;
+    int ifpgh;
+    double *pot, *grad, *hess;
+};
+void launch_direct(const DirectArgs& a, hipStream_t s);
+
+// Taylor expansion of all sources about one centre (large-delta branch)
+void launch_formlc(const double* xy, const double* q, int nd, int ind, int n, double cx,
+                   double cy, double dsq, int nl, double* part, int nparts, double* out,
+                   hipStream_t s);
+// all-pairs direct sum with a cutoff (fgt2dpart_direct_vec), accumulating
+void launch_direct_all(const double* src, const double* q, int nd, int ns, const double* tg,
+                       int nt, double delta, double dmax, int ifpgh, double* pot, double* grad,
+                       double* hess, hipStream_t s);
+
+}  // namespace fgtpw

@@ -1,0 +1,1116 @@
+// HIP kernels of the 2D plane-wave fast Gauss transform for gfx950.
+//
+// Layouts (all in HBM, tree order unless said otherwise):
+//   points      double[2*n]        x,y interleaved (the reference's (2,n))
+//   densities   double[nd*n]       (nd,n)
+//   expansions  double[2*nexp*nd]  per slot, complex interleaved, index
+//                                  a + (npw/2)*b (a: positive x nodes,
+//                                  b: all y nodes), as pwexp(npw/2,npw,nd)
+//   box arrays  indexed by box id of the host tree
+//
+// 64-wide wavefronts; 256-thread blocks (4 waves) throughout.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include "fgtpw_impl.hpp"
+
+namespace fgtpw {
+
+using u64 = unsigned long long;
+
+static inline unsigned grid_for(long n, int bs)
+{
+    long g = (n + bs - 1) / bs;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ---------------------------------------------------------------------------
+// bounding box of sources and targets (pts_tree_boxsize0, pts_tree2d4.f:1187)
+// min/max are exact, so the reduction order does not matter.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bbox(const double* __restrict__ a, int na,
+                                              const double* __restrict__ b, int nb,
+                                              double* __restrict__ part)
+{
+    __shared__ double s[5][256];
+    double xmn = HUGE_VAL, xmx = -HUGE_VAL, ymn = HUGE_VAL, ymx = -HUGE_VAL, bad = 0;
+    long n = (long)na + nb;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double* p = i < na ? a + 2 * i : b + 2 * (i - na);
+        double x = p[0], y = p[1];
+        if (!isfinite(x) || !isfinite(y)) bad = 1;
+        xmn = fmin(xmn, x);
+        xmx = fmax(xmx, x);
+        ymn = fmin(ymn, y);
+        ymx = fmax(ymx, y);
+    }
+    int t = threadIdx.x;
+    s[0][t] = xmn; s[1][t] = xmx; s[2][t] = ymn; s[3][t] = ymx; s[4][t] = bad;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s[0][t] = fmin(s[0][t], s[0][t + w]);
+            s[1][t] = fmax(s[1][t], s[1][t + w]);
+            s[2][t] = fmin(s[2][t], s[2][t + w]);
+            s[3][t] = fmax(s[3][t], s[3][t + w]);
+            s[4][t] = fmax(s[4][t], s[4][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t < 5) part[5 * blockIdx.x + t] = s[t][0];
+}
+
+__global__ __launch_bounds__(256) void k_bbox_final(const double* __restrict__ part, int nblk,
+                                                    double* __restrict__ out)
+{
+    __shared__ double s[5][256];
+    int t = threadIdx.x;
+    double v[5] = {HUGE_VAL, -HUGE_VAL, HUGE_VAL, -HUGE_VAL, 0};
+    for (int i = t; i < nblk; i += 256) {
+        v[0] = fmin(v[0], part[5 * i]);
+        v[1] = fmax(v[1], part[5 * i + 1]);
+        v[2] = fmin(v[2], part[5 * i + 2]);
+        v[3] = fmax(v[3], part[5 * i + 3]);
+        v[4] = fmax(v[4], part[5 * i + 4]);
+    }
+    for (int k = 0; k < 5; ++k) s[k][t] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s[0][t] = fmin(s[0][t], s[0][t + w]);
+            s[1][t] = fmax(s[1][t], s[1][t + w]);
+            s[2][t] = fmin(s[2][t], s[2][t + w]);
+            s[3][t] = fmax(s[3][t], s[3][t + w]);
+            s[4][t] = fmax(s[4][t], s[4][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t < 5) out[t] = s[t][0];
+}
+
+void launch_bbox(const double* a, int na, const double* b, int nb, double* part, int nblk,
+                 double* out, hipStream_t s)
+{
+    k_bbox<<<nblk, 256, 0, s>>>(a, na, b, nb, part);
+    k_bbox_final<<<1, 256, 0, s>>>(part, nblk, out);
+}
+
+// ---------------------------------------------------------------------------
+// full-depth Morton keys.  The centre of a child is parent +- bs/2 in FP64
+// and a point goes to the lower child when p - c < 0 (sort_pts_to_children,
+// pts_tree2d4.f:594-689; tree_refine_boxes, tree_routs4.f:34-39): every
+// comparison the reference makes along the point's path is made here with
+// bit-identical centres, so the key prefix of any level names the point's
+// box at that level.  Digit = xbit + 2*ybit, most significant = level 1.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_keys2d(const double* __restrict__ xy, int n, double cx0,
+                                                double cy0, double bs0, int nlmax,
+                                                u64* __restrict__ key, unsigned* __restrict__ idx)
+{
+#pragma clang fp contract(off)
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double x = xy[2 * (long)i], y = xy[2 * (long)i + 1];
+    double cx = cx0, cy = cy0, bs = bs0;
+    u64 k = 0;
+    for (int l = 0; l < nlmax; ++l) {
+        bs = bs / 2;
+        double h = bs / 2;
+        int xb = !(x - cx < 0.0);
+        int yb = !(y - cy < 0.0);
+        k = (k << 2) | (u64)(xb | (yb << 1));
+        cx = cx + (xb ? h : -h);
+        cy = cy + (yb ? h : -h);
+    }
+    key[i] = k;
+    if (idx) idx[i] = (unsigned)i;
+}
+
+void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, int nlmax, u64* key,
+                   unsigned* idx, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_keys2d<<<grid_for(n, 256), 256, 0, s>>>(xy, n, cx, cy, bs0, nlmax, key, idx);
+}
+
+// ---------------------------------------------------------------------------
+// Leaf level of every source in the adaptive (pre level-restriction) tree.
+// A box of level l is refined iff it holds more than ndiv(l) sources
+// (ndiv0 below the cutoff level, ndiv at or beyond it) and l < nlmax
+// (pts_tree_build, pts_tree2d4.f:507-523).  Counts shrink with depth and
+// ndiv(l) grows with l, so along a point's path the refined levels are
+// 0..L-1; L is found by widening the key-sorted run around the point from
+// the finest level upwards until it holds more than ndiv(l) points, which
+// never needs more than ndiv+1 neighbours on either side.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_leafdepth(const u64* __restrict__ key, int n, int nlmax,
+                                                   int levcut, int ndiv0, int ndiv,
+                                                   int* __restrict__ level,
+                                                   unsigned char* __restrict__ start)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    u64 ki = key[i];
+    int lo = i, hi = i + 1, L = 0;
+    for (int l = nlmax - 1; l >= 0; --l) {
+        int sh = 2 * (nlmax - l);
+        u64 p = ki >> sh;
+        int lim = l < levcut ? ndiv0 : ndiv;
+        while (lo > 0 && hi - lo <= lim && (key[lo - 1] >> sh) == p) --lo;
+        while (hi < n && hi - lo <= lim && (key[hi] >> sh) == p) ++hi;
+        if (hi - lo > lim) {
+            L = l + 1;
+            break;
+        }
+    }
+    level[i] = L;
+    int sh = 2 * (nlmax - L);
+    unsigned char st = 1;
+    if (i > 0 && sh < 64) st = (key[i - 1] >> sh) != (ki >> sh);
+    if (i > 0 && sh >= 64) st = 0;
+    start[i] = st;
+}
+
+void launch_leafdepth(const u64* key, int n, int nlmax, int levcut, int ndiv0, int ndiv,
+                      int* level, unsigned char* start_flag, hipStream_t s)
+{
+    k_leafdepth<<<grid_for(n, 256), 256, 0, s>>>(key, n, nlmax, levcut, ndiv0, ndiv, level,
+                                                 start_flag);
+}
+
+__global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ starts,
+                                                     const int* __restrict__ nsel, int n,
+                                                     const u64* __restrict__ key,
+                                                     const int* __restrict__ level, int nlmax,
+                                                     LeafRun* __restrict__ out)
+{
+    int r = blockIdx.x * 256 + threadIdx.x;
+    int m = *nsel;
+    if (r >= m) return;
+    int s = starts[r];
+    int e = r + 1 < m ? starts[r + 1] : n;
+    int L = level[s];
+    LeafRun o;
+    o.key = L == 0 ? 0ull : (key[s] >> (2 * (nlmax - L)));
+    o.level = L;
+    o.start = s;
+    o.count = e - s;
+    out[r] = o;
+}
+
+void launch_emit_leaves(const int* starts, const int* nsel, int n, const u64* key,
+                        const int* level, int nlmax, LeafRun* out, hipStream_t s)
+{
+    k_emit_leaves<<<grid_for(n, 256), 256, 0, s>>>(starts, nsel, n, key, level, nlmax, out);
+}
+
+// ---------------------------------------------------------------------------
+// sorting and selection (rocPRIM)
+// ---------------------------------------------------------------------------
+size_t sort_pairs_u64_temp(int n)
+{
+    size_t b = 0;
+    HIPCHECK(rocprim::radix_sort_pairs(nullptr, b, (const u64*)nullptr, (u64*)nullptr,
+                                       (const unsigned*)nullptr, (unsigned*)nullptr, n, 0, 64));
+    return b;
+}
+
+void sort_pairs_u64(void* tmp, size_t tmpb, const u64* kin, u64* kout, const unsigned* vin,
+                    unsigned* vout, int n, int endbit, hipStream_t s)
+{
+    HIPCHECK(rocprim::radix_sort_pairs(tmp, tmpb, kin, kout, vin, vout, n, 0, endbit, s));
+}
+
+size_t sort_pairs_u32_temp(int n)
+{
+    size_t b = 0;
+    HIPCHECK(rocprim::radix_sort_pairs(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                       (const unsigned*)nullptr, (unsigned*)nullptr, n, 0, 32));
+    return b;
+}
+
+void sort_pairs_u32(void* tmp, size_t tmpb, const unsigned* kin, unsigned* kout,
+                    const unsigned* vin, unsigned* vout, int n, int endbit, hipStream_t s)
+{
+    HIPCHECK(rocprim::radix_sort_pairs(tmp, tmpb, kin, kout, vin, vout, n, 0, endbit, s));
+}
+
+size_t select_flagged_temp(int n)
+{
+    size_t b = 0;
+    HIPCHECK(rocprim::select(nullptr, b, rocprim::counting_iterator<int>(0),
+                             (const unsigned char*)nullptr, (int*)nullptr, (int*)nullptr,
+                             (size_t)n));
+    return b;
+}
+
+void select_flagged(void* tmp, size_t tmpb, const unsigned char* flags, int n, int* out,
+                    int* nsel, hipStream_t s)
+{
+    HIPCHECK(rocprim::select(tmp, tmpb, rocprim::counting_iterator<int>(0), flags, out, nsel,
+                             (size_t)n, s));
+}
+
+// ---------------------------------------------------------------------------
+// point -> leaf of the final tree, by walking the key digits down the boxes
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_descend_rank(const u64* __restrict__ key,
+                                                      const unsigned* __restrict__ idx, int n,
+                                                      int nlmax, const int4* __restrict__ child,
+                                                      const int* __restrict__ rank_of_box,
+                                                      unsigned* __restrict__ rank_out)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    u64 k = key[i];
+    int b = 0;
+    for (int l = 0; l < nlmax; ++l) {
+        int4 c = child[b];
+        if (c.x < 0) break;
+        int d = (int)((k >> (2 * (nlmax - 1 - l))) & 3);
+        b = d == 0 ? c.x : d == 1 ? c.y : d == 2 ? c.z : c.w;
+    }
+    unsigned r = (unsigned)rank_of_box[b];
+    if (idx)
+        rank_out[idx[i]] = r;
+    else
+        rank_out[i] = r;
+}
+
+void launch_descend_rank(const u64* key, const unsigned* idx, int n, int nlmax, const int4* child,
+                         const int* rank_of_box, unsigned* rank_out, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_descend_rank<<<grid_for(n, 256), 256, 0, s>>>(key, idx, n, nlmax, child, rank_of_box,
+                                                    rank_out);
+}
+
+__global__ __launch_bounds__(256) void k_iota(unsigned* __restrict__ v, int n)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) v[i] = (unsigned)i;
+}
+
+void launch_iota(unsigned* v, int n, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_iota<<<grid_for(n, 256), 256, 0, s>>>(v, n);
+}
+
+// leafstart[r] = first sorted position whose leaf rank is >= r, r in [0, nleaves]
+__global__ __launch_bounds__(256) void k_leafstart(const unsigned* __restrict__ rk, int n,
+                                                   int nleaves, int* __restrict__ ls)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i > n) return;
+    int prev = i == 0 ? -1 : (int)rk[i - 1];
+    int cur = i == n ? nleaves : (int)rk[i];
+    for (int r = prev + 1; r <= cur; ++r) ls[r] = i;
+}
+
+void launch_leafstart(const unsigned* rank_sorted, int n, int nleaves, int* leafstart,
+                      hipStream_t s)
+{
+    k_leafstart<<<grid_for(n + 1, 256), 256, 0, s>>>(rank_sorted, n, nleaves, leafstart);
+}
+
+__global__ __launch_bounds__(256) void k_box_ranges(const int* __restrict__ lo,
+                                                    const int* __restrict__ hi, int nb,
+                                                    const int* __restrict__ ls,
+                                                    int2* __restrict__ se)
+{
+    int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nb) return;
+    se[b] = make_int2(ls[lo[b]], ls[hi[b]]);
+}
+
+void launch_box_ranges(const int* rank_lo, const int* rank_hi, int nb, const int* ls, int2* se,
+                       hipStream_t s)
+{
+    k_box_ranges<<<grid_for(nb, 256), 256, 0, s>>>(rank_lo, rank_hi, nb, ls, se);
+}
+
+__global__ __launch_bounds__(256) void k_gather(const double* __restrict__ xy,
+                                                const double* __restrict__ q, int nd,
+                                                const unsigned* __restrict__ perm, int n,
+                                                double* __restrict__ xys, double* __restrict__ qs)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    long j = perm[i];
+    xys[2 * (long)i] = xy[2 * j];
+    xys[2 * (long)i + 1] = xy[2 * j + 1];
+    if (q)
+        for (int d = 0; d < nd; ++d) qs[(long)nd * i + d] = q[(long)nd * j + d];
+}
+
+void launch_gather(const double* xy, const double* q, int nd, const unsigned* perm, int n,
+                   double* xy_s, double* q_s, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_gather<<<grid_for(n, 256), 256, 0, s>>>(xy, q, nd, perm, n, xy_s, q_s);
+}
+
+__global__ __launch_bounds__(256) void k_zero(double* __restrict__ p, size_t n)
+{
+    for (size_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0;
+}
+
+void launch_zero(double* p, size_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    size_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    k_zero<<<(unsigned)g, 256, 0, s>>>(p, n);
+}
+
+// v[perm[i]*w + c] = v_s[i*w + c]  (dreorderi / dreorderiv, fmmcommon2d.f:29-90)
+__global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ vs,
+                                                 const unsigned* __restrict__ perm, int n, int w,
+                                                 double* __restrict__ v)
+{
+    long t = blockIdx.x * 256L + threadIdx.x;
+    if (t >= (long)n * w) return;
+    long i = t / w;
+    int c = (int)(t - i * w);
+    v[(long)perm[i] * w + c] = vs[t];
+}
+
+void launch_scatter(const double* v_s, const unsigned* perm, int n, int w, double* v,
+                    hipStream_t s)
+{
+    if (n <= 0) return;
+    k_scatter<<<grid_for((long)n * w, 256), 256, 0, s>>>(v_s, perm, n, w, v);
+}
+
+// ---------------------------------------------------------------------------
+// Step 1, leaf formation (g2dformhc_vec g2drouts.f:100-168 followed by
+// g2dh2pw_real_vec :1242-1329).  One block per leaf.
+//   H(j,k)  = sum_i (q_i yh_i^k/sqrt(k!)) xh_i^j/sqrt(j!)      [LDS tiles]
+//   T(a,k)  = sum_j h2x(j,a) H(j,k)          a over the positive x nodes
+//   PW(a,b) = sum_k h2x(k,b) T(a,k)          b over all y nodes
+// with h2x(j,a) = ws_a (-i ts_a)^j / sqrt(j!).
+// ---------------------------------------------------------------------------
+template <int MAXE>
+__global__ __launch_bounds__(256) void k_form(FormArgs a, int chunk)
+{
+    extern __shared__ double sm[];
+    int bi = blockIdx.x;
+    int box = a.boxes[bi];
+    int2 se = a.src_se[box];
+    int n = se.y - se.x;
+    if (n <= 0) return;
+    const int m = a.p + 1, mm = m * m, t = threadIdx.x;
+    const double cx = a.cen[2 * box], cy = a.cen[2 * box + 1];
+    double* X = sm;              // chunk x m : xh^j / sqrt(j!)
+    double* Y = sm + chunk * m;  // chunk x m : q yh^k / sqrt(k!)
+    double acc[MAXE];
+#pragma unroll
+    for (int r = 0; r < MAXE; ++r) acc[r] = 0;
+    int jj[MAXE], kk[MAXE];
+#pragma unroll
+    for (int r = 0; r < MAXE; ++r) {
+        int e = t + 256 * r;
+        jj[r] = e % m;
+        kk[r] = e / m;
+    }
+    for (int s0 = 0; s0 < n; s0 += chunk) {
+        int cn = min(chunk, n - s0);
+        for (int u = t; u < cn; u += 256) {
+            long i = se.x + s0 + u;
+            double x = (a.xs[2 * i] - cx) * a.dsq;
+            double y = (a.xs[2 * i + 1] - cy) * a.dsq;
+            double px = 1.0, py = a.qs[(long)a.nd * i + a.ind];
+            X[u * m] = px;
+            Y[u * m] = py;
+            for (int j = 1; j < m; ++j) {
+                px = px * x * a.rsq[j];
+                py = py * y * a.rsq[j];
+                X[u * m + j] = px;
+                Y[u * m + j] = py;
+            }
+        }
+        __syncthreads();
+        for (int s = 0; s < cn; ++s) {
+#pragma unroll
+            for (int r = 0; r < MAXE; ++r)
+                if (t + 256 * r < mm) acc[r] += Y[s * m + kk[r]] * X[s * m + jj[r]];
+        }
+        __syncthreads();
+    }
+    double* H = sm;                      // m x m, j fastest
+    const int npw = a.npw, half = npw / 2;
+    double* T = sm + mm;                 // complex (half, m): a fastest
+#pragma unroll
+    for (int r = 0; r < MAXE; ++r)
+        if (t + 256 * r < mm) H[t + 256 * r] = acc[r];
+    __syncthreads();
+    for (int o = t; o < half * m; o += 256) {
+        int aa = o % half, k = o / half;
+        double re = 0, im = 0;
+        for (int j = 0; j < m; ++j) {
+            double h = H[j + m * k];
+            re += a.h2x[2 * (j * npw + aa)] * h;
+            im += a.h2x[2 * (j * npw + aa) + 1] * h;
+        }
+        T[2 * o] = re;
+        T[2 * o + 1] = im;
+    }
+    __syncthreads();
+    const int nexp = half * npw;
+    double* out = a.mp + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+    for (int o = t; o < nexp; o += 256) {
+        int aa = o % half, b = o / half;
+        double re = 0, im = 0;
+        for (int k = 0; k < m; ++k) {
+            double wr = a.h2x[2 * (k * npw + b)], wi = a.h2x[2 * (k * npw + b) + 1];
+            double tr = T[2 * (aa + half * k)], ti = T[2 * (aa + half * k) + 1];
+            re += wr * tr - wi * ti;
+            im += wr * ti + wi * tr;
+        }
+        out[2 * o] = re;
+        out[2 * o + 1] = im;
+    }
+}
+
+void launch_form(const FormArgs& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return;
+    int m = a.p + 1;
+    int chunk = 4096 / m;
+    if (chunk > 256) chunk = 256;
+    if (chunk < 16) chunk = 16;
+    size_t lds = (size_t)2 * chunk * m * sizeof(double);
+    size_t lds2 = ((size_t)m * m + (size_t)a.npw * m) * sizeof(double);
+    if (lds2 > lds) lds = lds2;
+    int ne = (m * m + 255) / 256;
+    if (ne <= 1)
+        k_form<1><<<a.nbox, 256, lds, s>>>(a, chunk);
+    else if (ne <= 2)
+        k_form<2><<<a.nbox, 256, lds, s>>>(a, chunk);
+    else if (ne <= 4)
+        k_form<4><<<a.nbox, 256, lds, s>>>(a, chunk);
+    else if (ne <= 8)
+        k_form<8><<<a.nbox, 256, lds, s>>>(a, chunk);
+    else if (ne <= 16)
+        k_form<16><<<a.nbox, 256, lds, s>>>(a, chunk);
+    else
+        fail(-3, "Hermite order too large for the formation kernel");
+}
+
+// ---------------------------------------------------------------------------
+// Steps 2-4: diagonal plane-wave translations (g2dshiftpw_vec,
+// g2drouts.f:2356-2402).  One thread per (box, coefficient); HBM-bound.
+// Quadrant of a shift from its sign pattern as fgt2d.f:988-998:
+// 0 = (+,+), 1 = (+,-), 2 = (-,+), 3 = (-,-).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int quadrant(double dx, double dy)
+{
+    return dx > 0 ? (dy > 0 ? 0 : 1) : (dy > 0 ? 2 : 3);
+}
+
+__global__ __launch_bounds__(256) void k_merge(const int* __restrict__ parents, int np,
+                                               const int4* __restrict__ child,
+                                               const double* __restrict__ cen,
+                                               const int2* __restrict__ src_se,
+                                               const int* __restrict__ slot,
+                                               const double* __restrict__ wms, int kbase, int nd,
+                                               int nexp, double* __restrict__ mp)
+{
+    long t = blockIdx.x * 256L + threadIdx.x;
+    int pi = (int)(t / nexp), e = (int)(t - (long)pi * nexp);
+    if (pi >= np) return;
+    int p = parents[pi];
+    int4 c4 = child[p];
+    int cs[4] = {c4.x, c4.y, c4.z, c4.w};
+    for (int ind = 0; ind < nd; ++ind) {
+        double* dst = mp + 2 * (((long)slot[p] * nd + ind) * nexp + e);
+        double re = dst[0], im = dst[1];
+        for (int j = 0; j < 4; ++j) {
+            int c = cs[j];
+            if (c < 0) continue;
+            int2 r = src_se[c];
+            if (r.y - r.x <= 0) continue;
+            int k = quadrant(cen[2 * p] - cen[2 * c], cen[2 * p + 1] - cen[2 * c + 1]);
+            const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
+            const double* v = mp + 2 * (((long)slot[c] * nd + ind) * nexp + e);
+            re += v[0] * w[0] - v[1] * w[1];
+            im += v[0] * w[1] + v[1] * w[0];
+        }
+        dst[0] = re;
+        dst[1] = im;
+    }
+}
+
+void launch_merge(const int* parents, int np, const int4* child, const double* cen,
+                  const int2* src_se, const int* slot, const double* wms, int kbase, int nd,
+                  int nexp, double* mp, hipStream_t s)
+{
+    if (np <= 0) return;
+    k_merge<<<grid_for((long)np * nexp, 256), 256, 0, s>>>(parents, np, child, cen, src_se, slot,
+                                                           wms, kbase, nd, nexp, mp);
+}
+
+// Step 3 (fgt2d.f:1026-1064): loc = sum over same-level neighbours j != self
+// of mp_j shifted by (jx, jy) boxes, then + mp_self.
+__global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ boxes, int nb,
+                                             const double* __restrict__ cen,
+                                             const int* __restrict__ ncol,
+                                             const int* __restrict__ col,
+                                             const int2* __restrict__ src_se,
+                                             const int2* __restrict__ trg_se, int use_src,
+                                             int use_trg, const int* __restrict__ slot,
+                                             const double* __restrict__ wm2l, double xmin,
+                                             int do_list, int nd, int nexp,
+                                             const double* __restrict__ mp,
+                                             double* __restrict__ loc)
+{
+    long t = blockIdx.x * 256L + threadIdx.x;
+    int bi = (int)(t / nexp), e = (int)(t - (long)bi * nexp);
+    if (bi >= nb) return;
+    int b = boxes[bi];
+    int np = 0;
+    if (use_trg) np += trg_se[b].y - trg_se[b].x;
+    if (use_src) np += src_se[b].y - src_se[b].x;
+    if (np <= 0) return;
+    for (int ind = 0; ind < nd; ++ind) {
+        double re = 0, im = 0;
+        if (do_list)
+            for (int i = 0; i < ncol[b]; ++i) {
+                int j = col[9 * b + i];
+                if (j == b) continue;
+                int jx = (int)lround((cen[2 * b] - cen[2 * j]) / xmin);
+                int jy = (int)lround((cen[2 * b + 1] - cen[2 * j + 1]) / xmin);
+                jx = jx < -1 ? -1 : (jx > 1 ? 1 : jx);  // colleagues are +-1 box away
+                jy = jy < -1 ? -1 : (jy > 1 ? 1 : jy);
+                const double* w = wm2l + 2 * ((long)((jx + 1) + 3 * (jy + 1)) * nexp + e);
+                const double* v = mp + 2 * (((long)slot[j] * nd + ind) * nexp + e);
+                re += v[0] * w[0] - v[1] * w[1];
+                im += v[0] * w[1] + v[1] * w[0];
+            }
+        const double* v = mp + 2 * (((long)slot[b] * nd + ind) * nexp + e);
+        re += v[0];
+        im += v[1];
+        double* d = loc + 2 * (((long)slot[b] * nd + ind) * nexp + e);
+        d[0] = re;
+        d[1] = im;
+    }
+}
+
+void launch_m2l(const int* boxes, int nb, const double* cen, const int* ncol, const int* col,
+                const int2* src_se, const int2* trg_se, int use_src, int use_trg,
+                const int* slot, const double* wm2l, double xmin, int do_list, int nd, int nexp,
+                const double* mp, double* loc, hipStream_t s)
+{
+    if (nb <= 0) return;
+    k_m2l<<<grid_for((long)nb * nexp, 256), 256, 0, s>>>(boxes, nb, cen, ncol, col, src_se,
+                                                         trg_se, use_src, use_trg, slot, wm2l,
+                                                         xmin, do_list, nd, nexp, mp, loc);
+}
+
+// Step 4 (fgt2d.f:1082-1125): loc(child) = loc(parent) shifted to the child.
+__global__ __launch_bounds__(256) void k_split(const int* __restrict__ children, int nc,
+                                               const int* __restrict__ par,
+                                               const double* __restrict__ cen,
+                                               const int* __restrict__ slot,
+                                               const double* __restrict__ wms, int kbase, int nd,
+                                               int nexp, double* __restrict__ loc)
+{
+    long t = blockIdx.x * 256L + threadIdx.x;
+    int ci = (int)(t / nexp), e = (int)(t - (long)ci * nexp);
+    if (ci >= nc) return;
+    int c = children[ci], p = par[c];
+    int k = quadrant(cen[2 * c] - cen[2 * p], cen[2 * c + 1] - cen[2 * p + 1]);
+    const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
+    for (int ind = 0; ind < nd; ++ind) {
+        const double* v = loc + 2 * (((long)slot[p] * nd + ind) * nexp + e);
+        double* d = loc + 2 * (((long)slot[c] * nd + ind) * nexp + e);
+        d[0] += v[0] * w[0] - v[1] * w[1];
+        d[1] += v[0] * w[1] + v[1] * w[0];
+    }
+}
+
+void launch_split(const int* children, int nc, const int* par, const double* cen,
+                  const int* slot, const double* wms, int kbase, int nd, int nexp, double* loc,
+                  hipStream_t s)
+{
+    if (nc <= 0) return;
+    k_split<<<grid_for((long)nc * nexp, 256), 256, 0, s>>>(children, nc, par, cen, slot, wms,
+                                                           kbase, nd, nexp, loc);
+}
+
+// ---------------------------------------------------------------------------
+// Step 5a: plane wave -> Taylor (g2dpw2local_real_vec, g2drouts.f:1417-1540)
+//   lx(k,b)  = i^k sum_a x2l(a,k) PW(a,b)          a positive x nodes
+//   L(k,l)   = 2 Re[ i^l sum_{b<half} x2l(b,l) (lx(k,b) + (-1)^l lx(k,b+half)) ]
+// with x2l(a,k) = ts_a^k / k!.  One block per leaf.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pw2local(EvalArgs a)
+{
+    extern __shared__ double sm[];
+    int bi = blockIdx.x;
+    int b = a.boxes[bi];
+    const int npw = a.npw, half = npw / 2, nexp = half * npw, m = a.nl + 1, t = threadIdx.x;
+    const double* pw = a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
+    double* P = sm;              // complex nexp
+    double* LX = sm + 2 * nexp;  // complex (m, npw): k fastest
+    for (int o = t; o < 2 * nexp; o += 256) P[o] = pw[o];
+    __syncthreads();
+    for (int o = t; o < m * npw; o += 256) {
+        int k = o % m, bb = o / m;
+        double cr = 0, ci = 0;
+        for (int aa = 0; aa < half; ++aa) {
+            double x = a.x2l[aa + npw * k];
+            cr += x * P[2 * (aa + half * bb)];
+            ci += x * P[2 * (aa + half * bb) + 1];
+        }
+        double r, im;
+        switch (k & 3) {
+        case 0: r = cr; im = ci; break;
+        case 1: r = -ci; im = cr; break;
+        case 2: r = -cr; im = -ci; break;
+        default: r = ci; im = -cr; break;
+        }
+        LX[2 * o] = r;
+        LX[2 * o + 1] = im;
+    }
+    __syncthreads();
+    double* L = a.taylor + (long)bi * m * m;
+    for (int o = t; o < m * m; o += 256) {
+        int k = o % m, l = o / m;
+        double sg = (l & 1) ? -1.0 : 1.0;
+        double cr = 0, ci = 0;
+        for (int bb = 0; bb < half; ++bb) {
+            double x = a.x2l[bb + npw * l];
+            cr += x * (LX[2 * (k + m * bb)] + sg * LX[2 * (k + m * (bb + half))]);
+            ci += x * (LX[2 * (k + m * bb) + 1] + sg * LX[2 * (k + m * (bb + half)) + 1]);
+        }
+        double v;
+        switch (l & 3) {
+        case 0: v = 2 * cr; break;
+        case 1: v = -2 * ci; break;
+        case 2: v = -2 * cr; break;
+        default: v = 2 * ci; break;
+        }
+        L[o] = v;
+    }
+}
+
+void launch_pw2local(const EvalArgs& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return;
+    int half = a.npw / 2, m = a.nl + 1;
+    size_t lds = (size_t)(2 * half * a.npw + 2 * m * a.npw) * sizeof(double);
+    k_pw2local<<<a.nbox, 256, lds, s>>>(a);
+}
+
+// Step 5b: Taylor evaluation (g2dlevalp/g/h_vec, g2drouts.f:847-1080) by
+// nested Horner schemes; derivatives carried along.  One block per leaf.
+template <int PGH>
+__global__ __launch_bounds__(256) void k_taylor_eval(EvalArgs a)
+{
+    extern __shared__ double sm[];
+    int bi = blockIdx.x;
+    int b = a.boxes[bi];
+    int2 se = a.se[b];
+    if (se.y <= se.x) return;
+    const int nl = a.nl, m = nl + 1, t = threadIdx.x;
+    const double* Lg = a.taylor + (long)bi * m * m;
+    for (int o = t; o < m * m; o += 256) sm[o] = Lg[o];
+    __syncthreads();
+    const double cx = a.cen[2 * b], cy = a.cen[2 * b + 1], dsq = a.dsq;
+    for (int i = se.x + t; i < se.y; i += 256) {
+        double x = (a.pts[2 * (long)i] - cx) * dsq, y = (a.pts[2 * (long)i + 1] - cy) * dsq;
+        // V = sum_k y^k P_k(x), U = sum_k y^k P_k'(x), Z = sum_k y^k P_k''(x)/2
+        double V = 0, Vy = 0, Vyy = 0, U = 0, Uy = 0, Z = 0;
+        for (int k = nl; k >= 0; --k) {
+            const double* c = sm + m * k;
+            double q = c[nl], q1 = 0, q2 = 0;
+            for (int j = nl - 1; j >= 0; --j) {
+                if (PGH >= 3) q2 = q2 * x + q1;
+                if (PGH >= 2) q1 = q1 * x + q;
+                q = q * x + c[j];
+            }
+            if (PGH >= 3) {
+                Vyy = Vyy * y + Vy;
+                Uy = Uy * y + U;
+                Z = Z * y + q2;
+            }
+            if (PGH >= 2) {
+                Vy = Vy * y + V;
+                U = U * y + q1;
+            }
+            V = V * y + q;
+        }
+        long o = (long)i * a.nd + a.ind;
+        a.pot[o] += V;
+        if (PGH >= 2) {
+            long g = ((long)i * 2) * a.nd + a.ind;
+            a.grad[g] += U * dsq;
+            a.grad[g + a.nd] += Vy * dsq;
+        }
+        if (PGH >= 3) {
+            long h = ((long)i * 3) * a.nd + a.ind;
+            a.hess[h] += 2 * Z * dsq * dsq;
+            a.hess[h + a.nd] += Uy * dsq * dsq;
+            a.hess[h + 2 * a.nd] += 2 * Vyy * dsq * dsq;
+        }
+    }
+}
+
+void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return;
+    int m = a.nl + 1;
+    size_t lds = (size_t)m * m * sizeof(double);
+    if (a.ifpgh >= 3)
+        k_taylor_eval<3><<<a.nbox, 256, lds, s>>>(a);
+    else if (a.ifpgh == 2)
+        k_taylor_eval<2><<<a.nbox, 256, lds, s>>>(a);
+    else
+        k_taylor_eval<1><<<a.nbox, 256, lds, s>>>(a);
+}
+
+// Step 5, plane-wave route (g2dpwevalp/g/h_vec, g2drouts.f:1887-2200):
+// pot = 2 Re sum_b w_y(b) sum_{a<half} PW(a,b) w_x(a), w(a) = e^{i ts_a u}
+// from one sincos per axis and its odd powers.  One block per leaf.
+constexpr int PW_MAXH = 32;
+
+template <int PGH>
+__global__ __launch_bounds__(256) void k_pw_eval(EvalArgs a)
+{
+    extern __shared__ double sm[];
+    int bi = blockIdx.x;
+    int b = a.boxes[bi];
+    int2 se = a.se[b];
+    if (se.y <= se.x) return;
+    const int npw = a.npw, half = npw / 2, nexp = half * npw, t = threadIdx.x;
+    const double* pw = a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
+    for (int o = t; o < 2 * nexp; o += 256) sm[o] = pw[o];
+    __syncthreads();
+    const double cx = a.cen[2 * b], cy = a.cen[2 * b + 1], dsq = a.dsq;
+    const double* ts = a.ts;
+    for (int i = se.x + t; i < se.y; i += 256) {
+        double x = (a.pts[2 * (long)i] - cx) * dsq, y = (a.pts[2 * (long)i + 1] - cy) * dsq;
+        double wr[PW_MAXH], wi[PW_MAXH];
+        {
+            double s, c;
+            sincos(ts[0] * x, &s, &c);
+            double sr = c * c - s * s, si = 2 * c * s, qr = c, qi = s;
+#pragma unroll
+            for (int aa = 0; aa < PW_MAXH; ++aa) {
+                wr[aa] = qr;
+                wi[aa] = qi;
+                double tmp = qr * sr - qi * si;
+                qi = qr * si + qi * sr;
+                qr = tmp;
+            }
+        }
+        double s, c;
+        sincos(ts[0] * y, &s, &c);
+        double sr = c * c - s * s, si = 2 * c * s, qr = c, qi = s;
+        double zr = 0, gx = 0, gy = 0, hxx = 0, hxy = 0, hyy = 0;
+        for (int bb = 0; bb < half; ++bb) {
+            // c_b and c_{b+half}: sums over the x nodes
+            double c0r = 0, c0i = 0, c1r = 0, c1i = 0;
+            double d0r = 0, d0i = 0, d1r = 0, d1i = 0;   // with factor ts_a
+            double e0r = 0, e0i = 0, e1r = 0, e1i = 0;   // with factor ts_a^2
+#pragma unroll
+            for (int aa = 0; aa < PW_MAXH; ++aa) {
+                if (aa < half) {
+                    const double* p0 = sm + 2 * (aa + half * bb);
+                    const double* p1 = sm + 2 * (aa + half * (bb + half));
+                    double v0r = p0[0] * wr[aa] - p0[1] * wi[aa], v0i = p0[0] * wi[aa] + p0[1] * wr[aa];
+                    double v1r = p1[0] * wr[aa] - p1[1] * wi[aa], v1i = p1[0] * wi[aa] + p1[1] * wr[aa];
+                    c0r += v0r; c0i += v0i; c1r += v1r; c1i += v1i;
+                    if (PGH >= 2) {
+                        double f = ts[aa];
+                        d0r += f * v0r; d0i += f * v0i; d1r += f * v1r; d1i += f * v1i;
+                        if (PGH >= 3) {
+                            double f2 = f * f;
+                            e0r += f2 * v0r; e0i += f2 * v0i; e1r += f2 * v1r; e1i += f2 * v1i;
+                        }
+                    }
+                }
+            }
+            // y phases: w(bb) = q, w(bb+half) = conj(q); node ts_bb and -ts_bb
+            double fy = ts[bb];
+            // Re[c0 q] + Re[c1 conj(q)]
+            zr += (c0r * qr - c0i * qi) + (c1r * qr + c1i * qi);
+            if (PGH >= 2) {
+                // d/dx: i ts_a -> Re[i d q] = -Im[d q]
+                gx += -(d0r * qi + d0i * qr) - (-d1r * qi + d1i * qr);
+                // d/dy: i ts_b: Re[i fy c0 q] + Re[-i fy c1 conj(q)]
+                gy += -fy * (c0r * qi + c0i * qr) + fy * (-c1r * qi + c1i * qr);
+                if (PGH >= 3) {
+                    hxx += -((e0r * qr - e0i * qi) + (e1r * qr + e1i * qi));
+                    // (i ts_a)(i ts_b) = -ts_a ts_b
+                    hxy += -fy * (d0r * qr - d0i * qi) + fy * (d1r * qr + d1i * qi);
+                    hyy += -fy * fy * ((c0r * qr - c0i * qi) + (c1r * qr + c1i * qi));
+                }
+            }
+            double tmp = qr * sr - qi * si;
+            qi = qr * si + qi * sr;
+            qr = tmp;
+        }
+        long o = (long)i * a.nd + a.ind;
+        a.pot[o] += 2 * zr;
+        if (PGH >= 2) {
+            long g = ((long)i * 2) * a.nd + a.ind;
+            a.grad[g] += 2 * gx * dsq;
+            a.grad[g + a.nd] += 2 * gy * dsq;
+        }
+        if (PGH >= 3) {
+            long h = ((long)i * 3) * a.nd + a.ind;
+            a.hess[h] += 2 * hxx * dsq * dsq;
+            a.hess[h + a.nd] += 2 * hxy * dsq * dsq;
+            a.hess[h + 2 * a.nd] += 2 * hyy * dsq * dsq;
+        }
+    }
+}
+
+void launch_pw_eval(const EvalArgs& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return;
+    if (a.npw / 2 > PW_MAXH) fail(-3, "plane-wave length too large for the evaluation kernel");
+    size_t lds = (size_t)a.npw * a.npw * sizeof(double);
+    if (a.ifpgh >= 3)
+        k_pw_eval<3><<<a.nbox, 256, lds, s>>>(a);
+    else if (a.ifpgh == 2)
+        k_pw_eval<2><<<a.nbox, 256, lds, s>>>(a);
+    else
+        k_pw_eval<1><<<a.nbox, 256, lds, s>>>(a);
+}
+
+// ---------------------------------------------------------------------------
+// Step 6, near field (fgt2dpart_direct_vec fgt2d.f:1354 -> g2d_directc*_vec,
+// g2ddirect.f:32-210), target-owned: one block per target unit, looping over
+// its source boxes in LDS tiles; pairs with r^2 >= dmax are skipped.
+// ---------------------------------------------------------------------------
+template <int PGH>
+__global__ __launch_bounds__(256) void k_direct(DirectArgs a)
+{
+    __shared__ double sx[256], sy[256], sq[256];
+    const int u = a.units[blockIdx.x];
+    const int2 tr = a.tgt_se[u];
+    const int l0 = a.list_ptr[blockIdx.x], l1 = a.list_ptr[blockIdx.x + 1];
+    const double inv = 1.0 / a.delta, rfac = -2.0 / a.delta;
+    const int t = threadIdx.x;
+    for (int ind = 0; ind < a.nd; ++ind)
+        for (int t0 = tr.x; t0 < tr.y; t0 += 256) {
+            int i = t0 + t;
+            bool act = i < tr.y;
+            double x = act ? a.xt[2 * (long)i] : 0, y = act ? a.xt[2 * (long)i + 1] : 0;
+            double p = 0, gx = 0, gy = 0, hxx = 0, hxy = 0, hyy = 0;
+            for (int li = l0; li < l1; ++li) {
+                int2 sr = a.src_se[a.list[li]];
+                for (int s0 = sr.x; s0 < sr.y; s0 += 256) {
+                    int cn = min(256, sr.y - s0);
+                    __syncthreads();
+                    if (t < cn) {
+                        long j = s0 + t;
+                        sx[t] = a.xs[2 * j];
+                        sy[t] = a.xs[2 * j + 1];
+                        sq[t] = a.qs[j * a.nd + ind];
+                    }
+                    __syncthreads();
+                    for (int s = 0; s < cn; ++s) {
+                        double dx = x - sx[s], dy = y - sy[s];
+                        double rr = dx * dx + dy * dy;
+                        if (rr < a.dmax) {
+                            double e = exp(-rr * inv) * sq[s];
+                            p += e;
+                            if (PGH >= 2) {
+                                double ux = rfac * dx, uy = rfac * dy;
+                                gx += ux * e;
+                                gy += uy * e;
+                                if (PGH >= 3) {
+                                    hxx += (rfac + ux * ux) * e;
+                                    hxy += ux * uy * e;
+                                    hyy += (rfac + uy * uy) * e;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            if (act) {
+                a.pot[(long)i * a.nd + ind] += p;
+                if (PGH >= 2) {
+                    long g = (long)i * 2 * a.nd + ind;
+                    a.grad[g] += gx;
+                    a.grad[g + a.nd] += gy;
+                }
+                if (PGH >= 3) {
+                    long h = (long)i * 3 * a.nd + ind;
+                    a.hess[h] += hxx;
+                    a.hess[h + a.nd] += hxy;
+                    a.hess[h + 2 * a.nd] += hyy;
+                }
+            }
+        }
+}
+
+void launch_direct(const DirectArgs& a, hipStream_t s)
+{
+    if (a.nunits <= 0) return;
+    if (a.ifpgh >= 3)
+        k_direct<3><<<a.nunits, 256, 0, s>>>(a);
+    else if (a.ifpgh == 2)
+        k_direct<2><<<a.nunits, 256, 0, s>>>(a);
+    else
+        k_direct<1><<<a.nunits, 256, 0, s>>>(a);
+}
+
+// ---------------------------------------------------------------------------
+// Large-delta branch: Taylor coefficients of sum_i q_i exp(-|t - s_i|^2/delta)
+// about one centre (g2dformlc_vec, g2drouts.f:621-680),
+//   L(j,k) = sum_i q_i h_j(xh_i) h_k(yh_i),
+//   h_0 = e^{-u^2}, h_1 = 2u e^{-u^2}, h_{j+1} = 2 (u h_j - h_{j-1})/(j+1).
+// Each block sums a slice of the sources; a second kernel adds the slices.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_formlc_part(const double* __restrict__ xy,
+                                                     const double* __restrict__ q, int nd,
+                                                     int ind, int n, double cx, double cy,
+                                                     double dsq, int nl, int nparts,
+                                                     double* __restrict__ part)
+{
+    extern __shared__ double sm[];
+    const int m = nl + 1, mm = m * m, t = threadIdx.x;
+    const int chunk = 64;
+    double* X = sm;
+    double* Y = sm + chunk * m;
+    long per = (n + nparts - 1) / nparts;
+    long i0 = blockIdx.x * per, i1 = min((long)n, i0 + per);
+    double* out = part + (long)blockIdx.x * mm;
+    for (int e = t; e < mm; e += 256) out[e] = 0;
+    for (long s0 = i0; s0 < i1; s0 += chunk) {
+        int cn = (int)min((long)chunk, i1 - s0);
+        __syncthreads();
+        if (t < cn) {
+            long i = s0 + t;
+            double x = (xy[2 * i] - cx) * dsq, y = (xy[2 * i + 1] - cy) * dsq;
+            double fx = exp(-x * x), fy = exp(-y * y), qq = q[(long)nd * i + ind];
+            double a0 = fx, a1 = 2.0 * x * fx, b0 = fy, b1 = 2.0 * y * fy;
+            X[t * m] = a0;
+            Y[t * m] = qq * b0;
+            if (m > 1) { X[t * m + 1] = a1; Y[t * m + 1] = qq * b1; }
+            for (int j = 1; j + 1 < m; ++j) {
+                double a2 = 2.0 * (x * a1 - a0) / (j + 1), b2 = 2.0 * (y * b1 - b0) / (j + 1);
+                X[t * m + j + 1] = a2;
+                Y[t * m + j + 1] = qq * b2;
+                a0 = a1; a1 = a2; b0 = b1; b1 = b2;
+            }
+        }
+        __syncthreads();
+        for (int e = t; e < mm; e += 256) {
+            int j = e % m, k = e / m;
+            double acc = 0;
+            for (int s = 0; s < cn; ++s) acc += Y[s * m + k] * X[s * m + j];
+            out[e] += acc;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, int nparts,
+                                                   int len, double* __restrict__ out)
+{
+    int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= len) return;
+    double acc = 0;
+    for (int p = 0; p < nparts; ++p) acc += part[(long)p * len + e];
+    out[e] = acc;
+}
+
+void launch_formlc(const double* xy, const double* q, int nd, int ind, int n, double cx,
+                   double cy, double dsq, int nl, double* part, int nparts, double* out,
+                   hipStream_t s)
+{
+    int m = nl + 1;
+    size_t lds = (size_t)2 * 64 * m * sizeof(double);
+    k_formlc_part<<<nparts, 256, lds, s>>>(xy, q, nd, ind, n, cx, cy, dsq, nl, nparts, part);
+    k_sum_parts<<<grid_for(m * m, 256), 256, 0, s>>>(part, nparts, m * m, out);
+}
+
+// fgt2dpart_direct_vec (fgt2d.f:1354): every target against every source
+// with the r^2 < dmax cutoff, accumulating.  Source tiles in LDS.
+template <int PGH>
+__global__ __launch_bounds__(256) void k_direct_all(const double* __restrict__ src,
+                                                    const double* __restrict__ q, int nd, int ns,
+                                                    const double* __restrict__ tg, int nt,
+                                                    double delta, double dmax,
+                                                    double* __restrict__ pot,
+                                                    double* __restrict__ grad,
+                                                    double* __restrict__ hess)
+{
+    __shared__ double sx[256], sy[256], sq[256];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * 256 + t;
+    const bool act = i < nt;
+    const double inv = 1.0 / delta, rfac = -2.0 / delta;
+    double x = act ? tg[2 * (long)i] : 0, y = act ? tg[2 * (long)i + 1] : 0;
+    for (int ind = 0; ind < nd; ++ind) {
+        double p = 0, gx = 0, gy = 0, hxx = 0, hxy = 0, hyy = 0;
+        for (int s0 = 0; s0 < ns; s0 += 256) {
+            int cn = min(256, ns - s0);
+            __syncthreads();
+            if (t < cn) {
+                long j = s0 + t;
+                sx[t] = src[2 * j];
+                sy[t] = src[2 * j + 1];
+                sq[t] = q[j * nd + ind];
+            }
+            __syncthreads();
+            for (int s = 0; s < cn; ++s) {
+                double dx = x - sx[s], dy = y - sy[s];
+                double rr = dx * dx + dy * dy;
+                if (rr < dmax) {
+                    double e = exp(-rr * inv) * sq[s];
+                    p += e;
+                    if (PGH >= 2) {
+                        double ux = rfac * dx, uy = rfac * dy;
+                        gx += ux * e;
+                        gy += uy * e;
+                        if (PGH >= 3) {
+                            hxx += (rfac + ux * ux) * e;
+                            hxy += ux * uy * e;
+                            hyy += (rfac + uy * uy) * e;
+                        }
+                    }
+                }
+            }
+        }
+        if (act) {
+            pot[(long)i * nd + ind] += p;
+            if (PGH >= 2) {
+                grad[(long)i * 2 * nd + ind] += gx;
+                grad[(long)i * 2 * nd + nd + ind] += gy;
+            }
+            if (PGH >= 3) {
+                hess[(long)i * 3 * nd + ind] += hxx;
+                hess[(long)i * 3 * nd + nd + ind] += hxy;
+                hess[(long)i * 3 * nd + 2 * nd + ind] += hyy;
+            }
+        }
+    }
+}
+
+void launch_direct_all(const double* src, const double* q, int nd, int ns, const double* tg,
+                       int nt, double delta, double dmax, int ifpgh, double* pot, double* grad,
+                       double* hess, hipStream_t s)
+{
+    if (nt <= 0 || ns <= 0) return;
+    unsigned g = grid_for(nt, 256);
+    if (ifpgh >= 3)
+        k_direct_all<3><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
+    else if (ifpgh == 2)
+        k_direct_all<2><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
+    else
+        k_direct_all<1><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
+}
+
+}  // namespace fgtpw

@@ -1,0 +1,121 @@
+/*
+ * fgtpw.h -- C-ABI of the MI355X plane-wave fast Gauss transform.
+ *
+ * Every entry point takes plain pointers and sizes and returns a status
+ * (FGTPW_OK or a negative code); fgtpw_last_error() gives the message of the
+ * last failure on the calling thread.  Array layouts are the reference's
+ * Fortran layouts: points (2,n) column-major (x,y interleaved), densities
+ * (nd,n), gradients (nd,2,n), Hessians (nd,3,n) ordered xx,xy,yy.
+ *
+ * Outputs are overwritten, not accumulated, for the quantities requested by
+ * ifpgh / ifpghtarg (1 pot, 2 pot+grad, 3 pot+grad+hess); arrays of
+ * quantities not requested are not touched and may be dummies.
+ *
+ * Supported in this build: charges (ifcharge = 1, ifdipole = 0), any nd >= 1.
+ * Dipoles return FGTPW_EUNSUPPORTED.
+ */
+#ifndef FGTPW_H
+#define FGTPW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGTPW_OK 0
+#define FGTPW_EINVAL (-1)      /* invalid argument (sizes, flags, non-finite points) */
+#define FGTPW_ENOMEM (-2)      /* device or host allocation failed */
+#define FGTPW_EUNSUPPORTED (-3) /* valid for the reference, not built here */
+#define FGTPW_EDEVICE (-4)     /* HIP runtime error */
+
+/* Text of the last error on this thread ("" if none). */
+const char *fgtpw_last_error(void);
+
+/* Optional: bind the library to a HIP device (default: the current device
+ * at first use).  Returns FGTPW_OK. */
+int fgtpw_init(int device);
+
+/*
+ * Replaces fgt2d (2d/planewave/src/fgt2d.f:19-22), same arguments and
+ * meaning, host arrays.  iper is ignored, as in the reference (fgt2d.f:40).
+ */
+int fgtpw_fgt2d(int nd, double delta, double eps, int ns, const double *sources, int ifcharge,
+                const double *charge, int ifdipole, const double *rnormal, const double *dipstr,
+                int iper, int ifpgh, double *pot, double *grad, double *hess, int nt,
+                const double *targ, int ifpghtarg, double *pottarg, double *gradtarg,
+                double *hesstarg);
+
+/*
+ * Same transform with every array already in device memory (HBM); runs on
+ * `stream` (a hipStream_t, NULL for the default stream) and returns when the
+ * outputs are complete.  No reference counterpart: it is the same fgt2d
+ * contract for callers whose data live on the GPU.
+ */
+int fgtpw_fgt2d_device(int nd, double delta, double eps, int ns, const double *sources,
+                       int ifcharge, const double *charge, int ifdipole, const double *rnormal,
+                       const double *dipstr, int iper, int ifpgh, double *pot, double *grad,
+                       double *hess, int nt, const double *targ, int ifpghtarg, double *pottarg,
+                       double *gradtarg, double *hesstarg, void *stream);
+
+/*
+ * Replaces fgt2dpart_direct_vec (2d/planewave/src/fgt2d.f:1354-1512): adds to
+ * pot/grad/hess at targets jstart..jend (1-based, inclusive) the direct
+ * Gaussian sum over sources istart..iend, pairs with r^2 >= ln(1/eps) delta
+ * skipped (g2ddirect.f:59-62).  Accumulates, as the reference does.  Host
+ * arrays.
+ */
+int fgtpw_fgt2dpart_direct(int nd, double delta, double eps, int istart, int iend, int jstart,
+                           int jend, const double *source, int ifcharge, const double *charge,
+                           int ifdipole, const double *rnormal, const double *dipstr,
+                           const double *targ, int ifpgh, double *pot, double *grad,
+                           double *hess);
+
+/*
+ * Diagnostics of the last transform on this thread: the integers the
+ * reference prints (fgt2d.f:127, :188, :196-197, :358-361) -- out[0]
+ * npwlevel, out[1] nlmax, out[2] nlevels, out[3] nboxes, out[4] npw,
+ * out[5] number of leaves, out[6..6+nlevels] nlocal per level.  n is the
+ * length of out.  Returns the number of entries written.
+ */
+int fgtpw_last_stats(int *out, int n);
+
+/*
+ * Stage timing of the next transforms: when on (1), each call records HIP
+ * events around its stages on its stream and fgtpw_stage_times() returns the
+ * milliseconds of the last call per stage, in the order of
+ * fgtpw_stage_name(i).  Returns the number of stages written.
+ */
+int fgtpw_set_timing(int on);
+int fgtpw_stage_times(double *ms, int n);
+const char *fgtpw_stage_name(int i);
+
+/*
+ * Inspection of the quadtree of the last successful fgtpw_fgt2d* call on this
+ * process (the tree fgt2d builds before its main routine, fgt2d.f:191-221),
+ * 0-based: info[0] nlevels, info[1] nboxes, info[2] leaves, info[3] nlmax.
+ * When laddr is non-NULL and cap >= nboxes the arrays are filled: laddr
+ * (2,nlevels+1), lev, par, nch (nboxes), ch (4,nboxes), ncol, col
+ * (9,nboxes), cen (2,nboxes), isrc (ns) and itarg (nt) permutations, and
+ * isrcse/itargse (2,nboxes) as first index and one past the last.  The
+ * layout is that of itree (pts_tree2d4.f:19-27) with 0-based box numbers.
+ * No reference counterpart: the reference keeps its tree internal.
+ */
+int fgtpw_last_tree2d(int *info, int cap, int *laddr, int *lev, int *par, int *nch, int *ch,
+                      int *ncol, int *col, double *cen, int *isrc, int *isrcse, int *itarg,
+                      int *itargse);
+
+/*
+ * Host-only: the box tree the runtime derives from a list of non-empty
+ * leaves of the adaptive tree (Morton prefix keys[i] of level levels[i], in
+ * Morton order) -- refinement, level restriction (pts_tree_fix_lr,
+ * pts_tree2d4.f:693) and renumbering -- with the outputs of
+ * fgtpw_last_tree2d.  No GPU needed; for tests of the host logic.
+ */
+int fgtpw_tree2d_from_leaves(int nlmax, int npwlevel, double bs0, const double *cen0,
+                             int nleaves, const unsigned long long *keys, const int *levels,
+                             int *info, int cap, int *laddr, int *lev, int *par, int *nch,
+                             int *ch, int *ncol, int *col, double *cen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,58 @@
+"""Test helpers: seeded inputs of the benchmark configurations and a numpy
+statement of the GPU's key / leaf-list stage (for host-only tree tests)."""
+import numpy as np
+
+
+def uniform(rng, n, d=2):
+    return rng.random((n, d))
+
+
+def mixture(rng, n, ncent=10, sigma=0.01, lo=0.1, hi=0.9):
+    """Gaussian mixture clipped to the unit square (the 2M/2M configuration)."""
+    mu = rng.uniform(lo, hi, (ncent, 2))
+    k = rng.integers(0, ncent, n)
+    return np.clip(mu[k] + rng.normal(0.0, sigma, (n, 2)), 0.0, 1.0)
+
+
+def keys2d(xy, cen0, bs0, nlmax):
+    """Full-depth Morton keys from the reference's comparisons p - c < 0 with
+    centres c +- bs/2 built level by level (pts_tree2d4.f:594-689)."""
+    x, y = xy[:, 0], xy[:, 1]
+    cx = np.full(len(x), float(cen0[0]))
+    cy = np.full(len(x), float(cen0[1]))
+    bs = float(bs0)
+    k = np.zeros(len(x), dtype=np.uint64)
+    for _ in range(nlmax):
+        bs = bs / 2
+        h = bs / 2
+        xb = ~(x - cx < 0)
+        yb = ~(y - cy < 0)
+        k = (k << np.uint64(2)) | (xb.astype(np.uint64) + np.uint64(2) * yb.astype(np.uint64))
+        cx = cx + np.where(xb, h, -h)
+        cy = cy + np.where(yb, h, -h)
+    return k
+
+
+def leaf_runs(keys_sorted, nlmax, levcut, ndiv0=10, ndiv=80):
+    """Non-empty leaves of the adaptive tree (before level restriction):
+    a level-l box splits when it holds more than ndiv0 (l < levcut) or ndiv
+    sources (pts_tree2d4.f:507-523), up to nlmax."""
+    n = len(keys_sorted)
+    L = np.full(n, -1)
+    for l in range(nlmax + 1):
+        pre = keys_sorted >> np.uint64(2 * (nlmax - l))
+        _, start, counts = np.unique(pre, return_index=True, return_counts=True)
+        cnt = np.repeat(counts, counts)
+        lim = ndiv0 if l < levcut else ndiv
+        here = (L < 0) & ((cnt <= lim) | (l == nlmax))
+        L[here] = l
+    keys, levels = [], []
+    prev = None
+    for i in range(n):
+        pre = int(keys_sorted[i]) >> (2 * (nlmax - L[i]))
+        tag = (L[i], pre)
+        if tag != prev:
+            keys.append(pre)
+            levels.append(L[i])
+            prev = tag
+    return np.array(keys, dtype=np.uint64), np.array(levels, dtype=np.int32)
