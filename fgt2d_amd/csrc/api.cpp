@@ -53,6 +53,14 @@ namespace {
 // ---------------------------------------------------------------------------
 // workspace: named device buffers that only grow, plus pinned staging
 // ---------------------------------------------------------------------------
+struct Tables2D {
+    bool valid = false;
+    int npw = 0, ntermmax = 0, nmax = 0;
+    double pmax = 0, xmin2 = 0, xm = 0;
+    const double *ts = nullptr, *h2x = nullptr, *h2xr = nullptr, *x2l = nullptr, *rsq = nullptr,
+                 *wms = nullptr, *wm2l = nullptr;
+};
+
 struct Workspace {
     struct Buf {
         void* p = nullptr;
@@ -63,6 +71,7 @@ struct Workspace {
     size_t pin_cap = 0;
     hipStream_t own = nullptr;
     hipEvent_t ev[2 * S_N] = {};
+    Tables2D tab;
     // the tree of the last transform, kept for inspection
     Tree2D last;
     int last_ns = 0, last_nt = 0;
@@ -178,6 +187,107 @@ struct Pack {
         return add(v.data(), v.size());
     }
 };
+
+// ---------------------------------------------------------------------------
+// plane-wave tables: nodes, Hermite->PW and PW->Taylor 1-D matrices, and the
+// diagonal translation factors.  They depend only on (npw, pmax, orders,
+// shift lengths), so they stay resident and are rebuilt only when those
+// change.
+// ---------------------------------------------------------------------------
+const Tables2D& tables2d(Workspace& W, int npw, double pmax, int ntermmax, int nmax, double xmin2,
+                         double xm, hipStream_t s)
+{
+    Tables2D& t = W.tab;
+    if (t.valid && t.npw == npw && t.pmax == pmax && t.ntermmax == ntermmax && t.nmax == nmax &&
+        t.xmin2 == xmin2 && t.xm == xm)
+        return t;
+    const int half = npw / 2, nexp = half * npw, mt = ntermmax + 1;
+    std::vector<double> ws(npw), ts(npw);
+    planewave_nodes(pmax, npw, ws.data(), ts.data());
+    // h2x(j,a) = ws_a (-i ts_a)^j / sqrt(j!) (g2dh2xmat, g2drouts.f:1601-1640);
+    // x2l(a,j) = ts_a^j / j! (g2drx2lmat, :1731-1769)
+    std::vector<double> h2x(2 * (size_t)mt * npw), h2xr(2 * (size_t)mt * npw), x2l((size_t)npw * mt),
+        rsq(mt + 1);
+    rsq[0] = 1;
+    for (int k = 1; k <= mt; ++k) rsq[k] = 1.0 / std::sqrt((double)k);
+    double fh = 1.0, fl = 1.0;
+    for (int j = 0; j < mt; ++j) {
+        if (j > 0) { fh = fh / std::sqrt((double)j); fl = fl * j; }
+        for (int a = 0; a < npw; ++a) {
+            double r = ws[a] * std::pow(-ts[a], j) * fh;
+            double re = 0, im = 0;  // times i^j
+            switch (j & 3) {
+            case 0: re = r; break;
+            case 1: im = r; break;
+            case 2: re = -r; break;
+            default: im = -r; break;
+            }
+            h2x[2 * ((size_t)j * npw + a)] = re;
+            h2x[2 * ((size_t)j * npw + a) + 1] = im;
+            // the same with 1/j! in place of 1/sqrt(j!): for raw moments
+            h2xr[2 * ((size_t)j * npw + a)] = re * fh;
+            h2xr[2 * ((size_t)j * npw + a) + 1] = im * fh;
+            x2l[a + (size_t)npw * j] = std::pow(ts[a], j) / fl;
+        }
+    }
+    // merge/split factors e^{i ts xmin2 2^(k-1)} by repeated squaring
+    // (merge_split_pw_matrices, g2drouts.f:2294-2350); 4 sign patterns
+    std::vector<double> wms(2 * (size_t)nexp * 4 * std::max(nmax, 1));
+    std::vector<double> wwr((size_t)npw * (nmax + 1)), wwi((size_t)npw * (nmax + 1));
+    for (int j = 0; j < npw; ++j) {
+        double zr = std::cos(ts[j] * xmin2), zi = std::sin(ts[j] * xmin2);
+        for (int k = 1; k <= nmax; ++k) {
+            wwr[j + (size_t)npw * k] = zr;
+            wwi[j + (size_t)npw * k] = zi;
+            double t2 = zr * zr - zi * zi;
+            zi = zr * zi + zi * zr;
+            zr = t2;
+        }
+    }
+    for (int k = 1; k <= nmax; ++k)
+        for (int qd = 0; qd < 4; ++qd)
+            for (int b = 0; b < npw; ++b)
+                for (int a = 0; a < half; ++a) {
+                    double xr = wwr[a + (size_t)npw * k], xi = wwi[a + (size_t)npw * k];
+                    double yr = wwr[b + (size_t)npw * k], yi = wwi[b + (size_t)npw * k];
+                    if (qd >= 2) xi = -xi;  // x shift negative
+                    if (qd & 1) yi = -yi;   // y shift negative
+                    size_t o = 2 * ((size_t)(4 * (k - 1) + qd) * nexp + a + (size_t)half * b);
+                    wms[o] = xr * yr - xi * yi;
+                    wms[o + 1] = xr * yi + xi * yr;
+                }
+    // m2l factors for shifts of -1, 0, 1 boxes (pw_translation_matrices,
+    // g2drouts.f:2235-2289)
+    std::vector<double> wm2l(2 * (size_t)nexp * 9);
+    for (int ky = -1; ky <= 1; ++ky)
+        for (int kx = -1; kx <= 1; ++kx)
+            for (int b = 0; b < npw; ++b)
+                for (int a = 0; a < half; ++a) {
+                    double xr = 1, xi = 0, yr = 1, yi = 0;
+                    if (kx) { xr = std::cos(ts[a] * xm); xi = kx * std::sin(ts[a] * xm); }
+                    if (ky) { yr = std::cos(ts[b] * xm); yi = ky * std::sin(ts[b] * xm); }
+                    size_t o = 2 * ((size_t)((kx + 1) + 3 * (ky + 1)) * nexp + a + (size_t)half * b);
+                    wm2l[o] = xr * yr - xi * yi;
+                    wm2l[o + 1] = xr * yi + xi * yr;
+                }
+    Pack pk;
+    size_t o_ts = pk.add(ts), o_h2x = pk.add(h2x), o_h2xr = pk.add(h2xr), o_x2l = pk.add(x2l),
+           o_rsq = pk.add(rsq);
+    size_t o_wms = pk.add(wms), o_wm2l = pk.add(wm2l);
+    char* d = W.get<char>("tables", pk.bytes.size());
+    HIPCHECK(hipMemcpyAsync(d, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));  // pk is freed on return
+    t.ts = (const double*)(d + o_ts);
+    t.h2x = (const double*)(d + o_h2x);
+    t.h2xr = (const double*)(d + o_h2xr);
+    t.x2l = (const double*)(d + o_x2l);
+    t.rsq = (const double*)(d + o_rsq);
+    t.wms = (const double*)(d + o_wms);
+    t.wm2l = (const double*)(d + o_wm2l);
+    t.npw = npw; t.pmax = pmax; t.ntermmax = ntermmax; t.nmax = nmax; t.xmin2 = xmin2; t.xm = xm;
+    t.valid = true;
+    return t;
+}
 
 // single Taylor expansion about the box centre for tiny boxes,
 // fgt2d_large_delta (fgt2d.f:1616-1733)
@@ -316,7 +426,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     std::vector<int> m2l_l;
     std::vector<size_t> o_form(nlevels + 1), o_merge(nlevels + 1), o_split(nlevels + 1),
         o_eval(nlevels + 1);
-    size_t o_m2l = 0, o_h2x = 0, o_x2l = 0, o_rsq = 0, o_ts = 0, o_wms = 0, o_wm2l = 0;
+    size_t o_m2l = 0;
     if (expansions) {
         std::vector<char> iflocal(nb, 0);
         if (npwlevel >= 0 && npwlevel <= nlevels)
@@ -343,80 +453,13 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             o_eval[l] = pk.add(eval_l[l]);
         }
         o_m2l = pk.add(m2l_l);
-        // tables: nodes, Hermite->PW, PW->Taylor, shifts
-        std::vector<double> ws(npw), ts(npw);
-        planewave_nodes(pmax, npw, ws.data(), ts.data());
-        const int mt = ntermmax + 1;
-        std::vector<double> h2x(2 * (size_t)mt * npw), x2l((size_t)npw * mt), rsq(mt + 1);
-        rsq[0] = 1;
-        for (int k = 1; k <= mt; ++k) rsq[k] = 1.0 / std::sqrt((double)k);
-        double fh = 1.0, fl = 1.0;
-        for (int j = 0; j < mt; ++j) {
-            if (j > 0) { fh = fh / std::sqrt((double)j); fl = fl * j; }
-            for (int a = 0; a < npw; ++a) {
-                double r = ws[a] * std::pow(-ts[a], j) * fh;  // ws (-ts)^j / sqrt(j!)
-                double re = 0, im = 0;                        // times i^j
-                switch (j & 3) {
-                case 0: re = r; break;
-                case 1: im = r; break;
-                case 2: re = -r; break;
-                default: im = -r; break;
-                }
-                h2x[2 * ((size_t)j * npw + a)] = re;
-                h2x[2 * ((size_t)j * npw + a) + 1] = im;
-                x2l[a + (size_t)npw * j] = std::pow(ts[a], j) / fl;
-            }
-        }
-        // merge/split tables: e^{i ts xmin2 2^(k-1)} by repeated squaring
-        // (merge_split_pw_matrices, g2drouts.f:2294-2350)
-        const int nmax = nlevels - nlevstart;
-        std::vector<double> wms(2 * (size_t)nexp * 4 * std::max(nmax, 1));
-        double xmin2 = T.bsz[nlevels] / std::sqrt(q.delta) / 2;
-        std::vector<double> wwr((size_t)npw * (nmax + 1)), wwi((size_t)npw * (nmax + 1));
-        for (int j = 0; j < npw; ++j) {
-            double zr = std::cos(ts[j] * xmin2), zi = std::sin(ts[j] * xmin2);
-            for (int k = 1; k <= nmax; ++k) {
-                wwr[j + (size_t)npw * k] = zr;
-                wwi[j + (size_t)npw * k] = zi;
-                double t2 = zr * zr - zi * zi;
-                zi = zr * zi + zi * zr;
-                zr = t2;
-            }
-        }
-        for (int k = 1; k <= nmax; ++k)
-            for (int qd = 0; qd < 4; ++qd)
-                for (int b = 0; b < npw; ++b)
-                    for (int a = 0; a < half; ++a) {
-                        double xr = wwr[a + (size_t)npw * k], xi = wwi[a + (size_t)npw * k];
-                        double yr = wwr[b + (size_t)npw * k], yi = wwi[b + (size_t)npw * k];
-                        if (qd >= 2) xi = -xi;  // x shift negative
-                        if (qd & 1) yi = -yi;   // y shift negative
-                        size_t o = 2 * ((size_t)(4 * (k - 1) + qd) * nexp + a + (size_t)half * b);
-                        wms[o] = xr * yr - xi * yi;
-                        wms[o + 1] = xr * yi + xi * yr;
-                    }
-        // m2l tables for shifts of -1, 0, 1 boxes (pw_translation_matrices,
-        // g2drouts.f:2235-2289)
-        std::vector<double> wm2l(2 * (size_t)nexp * 9);
-        double xm = (npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0]) / std::sqrt(q.delta);
-        for (int ky = -1; ky <= 1; ++ky)
-            for (int kx = -1; kx <= 1; ++kx)
-                for (int b = 0; b < npw; ++b)
-                    for (int a = 0; a < half; ++a) {
-                        double xr = 1, xi = 0, yr = 1, yi = 0;
-                        if (kx) { xr = std::cos(ts[a] * xm); xi = kx * std::sin(ts[a] * xm); }
-                        if (ky) { yr = std::cos(ts[b] * xm); yi = ky * std::sin(ts[b] * xm); }
-                        size_t o = 2 * ((size_t)((kx + 1) + 3 * (ky + 1)) * nexp + a + (size_t)half * b);
-                        wm2l[o] = xr * yr - xi * yi;
-                        wm2l[o + 1] = xr * yi + xi * yr;
-                    }
-        o_ts = pk.add(ts);
-        o_h2x = pk.add(h2x);
-        o_x2l = pk.add(x2l);
-        o_rsq = pk.add(rsq);
-        o_wms = pk.add(wms);
-        o_wm2l = pk.add(wm2l);
     }
+    const Tables2D& tab = expansions ? tables2d(W, npw, pmax, ntermmax, nlevels - nlevstart,
+                                                T.bsz[nlevels] / std::sqrt(q.delta) / 2,
+                                                (npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0]) /
+                                                    std::sqrt(q.delta),
+                                                s)
+                                     : W.tab;
 
     // near field: modified list1 (compute_modified_list1, tree_routs4.f:715-879)
     // inverted to target units; a unit is a leaf at or above the cutoff level
@@ -426,7 +469,8 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     if (nlevend >= 0) {
         std::vector<int> nl1(nb, 0), l1(13 * (size_t)nb, -1);
         if (T.nch[0] == 0) { nl1[0] = 1; l1[0] = 0; }
-        for (int l = 1; l <= nlevels; ++l)
+        // only lists of boxes at or above the cutoff level are ever read
+        for (int l = 1; l <= nlevend; ++l)
             for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
                 if (T.nch[b] != 0) continue;
                 for (int i = 0; i < T.ncol[b]; ++i) {
@@ -544,22 +588,64 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         double* lc = W.get<double>("loc", per * nslot);
         launch_zero(mp, per * nslot, s);
         launch_zero(lc, per * nslot, s);
-        const double* d_h2x = (const double*)D(o_h2x);
-        const double* d_x2l = (const double*)D(o_x2l);
-        const double* d_rsq = (const double*)D(o_rsq);
-        const double* d_ts = (const double*)D(o_ts);
-        const double* d_wms = (const double*)D(o_wms);
-        const double* d_wm2l = (const double*)D(o_wm2l);
+        const double* d_h2x = tab.h2x;
+        const double* d_h2xr = tab.h2xr;
+        const double* d_x2l = tab.x2l;
+        const double* d_rsq = tab.rsq;
+        const double* d_ts = tab.ts;
+        const double* d_wms = tab.wms;
+        const double* d_wm2l = tab.wm2l;
         size_t maxeval = 0;
         for (int l = 0; l <= nlevels; ++l)
             if (nlocal[l] <= 50)
                 maxeval = std::max(maxeval, eval_l[l].size() * (size_t)(nlocal[l] + 1) * (nlocal[l] + 1));
         double* taylor = W.get<double>("taylor", maxeval);
+        // segment numbering of every formation and evaluation list (one
+        // launch): formation over sources in segments of SEGF, evaluation
+        // over targets / sources in segments of SEGE
+        auto segf = [&](int l) { int tm_ = (nlocal[l] + 16) / 16; return tm_ <= 2 ? 128 : 64; };
+        const int SEGE = 256, GRIDMAX = 4096;
+        std::vector<int*> ts_form(nlevels + 1, nullptr), ts_trg(nlevels + 1, nullptr),
+            ts_src(nlevels + 1, nullptr);
+        {
+            size_t ntot = 0;
+            for (int l = 0; l <= nlevels; ++l)
+                ntot += form_l[l].size() + 2 * eval_l[l].size() + 3;
+            int* pool = W.get<int>("tstart", ntot);
+            SegLists L{};
+            auto add = [&](const std::vector<int>& v, size_t off, const int2* se, int seg) {
+                int* t = pool;
+                pool += v.size() + 1;
+                if (L.count == 32) {
+                    launch_seg_scan(L, s);
+                    L.count = 0;
+                }
+                L.l[L.count++] = SegList{(const int*)D(off), (int)v.size(), se, seg, t};
+                return t;
+            };
+            for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l)
+                if (!form_l[l].empty()) ts_form[l] = add(form_l[l], o_form[l], src_se, segf(l));
+            for (int l = nlevstart; l <= nlevels; ++l) {
+                if (eval_l[l].empty()) continue;
+                if (use_trg) ts_trg[l] = add(eval_l[l], o_eval[l], trg_se, SEGE);
+                if (use_src) ts_src[l] = add(eval_l[l], o_eval[l], src_se, SEGE);
+            }
+            launch_seg_scan(L, s);
+        }
+        size_t maxpart = 0;
+        for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l) {
+            size_t m = nlocal[l] + 1;
+            maxpart = std::max(maxpart, (form_l[l].size() + ns / segf(l) + 1) * m * m);
+        }
+        double* partial = W.get<double>("form_partial", maxpart);
         for (int ind = 0; ind < nd; ++ind) {
             tm.begin(S_FORM);
             for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l) {
-                FormArgs fa{(const int*)D(o_form[l]), (int)form_l[l].size(), d_cen, src_se, d_slot,
-                            xs, qs, nd, ind, dsq, nlocal[l], d_h2x, d_rsq, npw, mp};
+                if (form_l[l].empty()) continue;
+                const int nb_l = (int)form_l[l].size(), m = nlocal[l] + 1;
+                FormArgs fa{(const int*)D(o_form[l]), nb_l, d_cen, src_se, d_slot, xs, qs, nd, ind,
+                            dsq, nlocal[l], d_h2x, d_h2xr, d_rsq, npw, mp, ts_form[l], segf(l),
+                            (int)std::min<long>(GRIDMAX, nb_l + ns / segf(l) + 1), partial, m * m};
                 launch_form(fa, s);
             }
             tm.end(S_FORM);
@@ -603,6 +689,9 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                     if (side == 0 && !use_trg) continue;
                     if (side == 1 && !use_src) continue;
                     EvalArgs eb = ea;
+                    eb.tstart = side == 0 ? ts_trg[l] : ts_src[l];
+                    eb.seg = SEGE;
+                    eb.grid = (int)std::min<long>(GRIDMAX, ea.nbox + (side == 0 ? nt : ns) / SEGE + 1);
                     eb.se = side == 0 ? trg_se : src_se;
                     eb.pts = side == 0 ? xt : xs;
                     eb.ifpgh = side == 0 ? q.ifpghtarg : q.ifpgh;
@@ -943,6 +1032,7 @@ int fgtpw_last_tree2d(int* info, int cap, int* laddr, int* lev, int* par, int* n
         select_device();
         Workspace& W = workspace();
         if (!W.last_valid) fail(FGTPW_EINVAL, "no tree: the last call built none");
+        complete_colleagues(W.last);
         const Tree2D& T = W.last;
         const int nb = T.nboxes();
         info[0] = T.nlevels;
@@ -983,6 +1073,7 @@ int fgtpw_tree2d_from_leaves(int nlmax, int npwlevel, double bs0, const double* 
         for (int i = 0; i < nleaves; ++i) leaves[i] = LeafRun{keys[i], levels[i], 0, 0};
         Tree2D T;
         build_tree2d(T, leaves, nlmax, npwlevel, bs0, cen0);
+        complete_colleagues(T);
         const int nb = T.nboxes();
         info[0] = T.nlevels;
         info[1] = nb;

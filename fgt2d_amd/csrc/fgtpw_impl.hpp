@@ -47,6 +47,8 @@ struct Tree2D {
     std::vector<int> lev, par, nch, ch, ncol, col, flag;
     std::vector<double> cen;
     std::vector<uint64_t> pre;  // Morton prefix of every box
+    std::vector<int> ix, iy;    // integer box coordinates within the level
+    int coll_levels = -1;       // colleague lists are valid for levels <= this
     int nboxes() const { return (int)lev.size(); }
     // leaves in Morton order and, per box, its interval of leaf ranks
     std::vector<int> leaf_of_rank, rank_of_box, rank_lo, rank_hi;
@@ -54,6 +56,7 @@ struct Tree2D {
 
 void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int npwlevel,
                   double bs0, const double* cen0);
+void complete_colleagues(Tree2D& T);
 
 // ---------------------------------------------------------------------------
 // device kernels (kernels2d.hip)
@@ -103,11 +106,34 @@ struct FormArgs {
     double dsq;
     int p;                  // Hermite order
     const double* h2x;      // complex (p+1, npw): ws (-i ts)^j / sqrt(j!)
+    const double* h2xr;     // complex (p+1, npw): ws (-i ts)^j / j!
     const double* rsq;      // 1/sqrt(k)
     int npw;
     double* mp;             // complex per slot (nd, nexp)
+    // source segments: tstart[i] = first segment of boxes[i] (exclusive scan
+    // of ceil(count/seg)), partial moments of each segment (stride pstride)
+    const int* tstart;
+    int seg, grid;
+    double* partial;
+    int pstride;
 };
 void launch_form(const FormArgs& a, hipStream_t s);
+
+// Exclusive scans of per-box segment counts, one list per block (up to 32
+// lists per launch): tstart[i] = sum_{j<i} ceil(len(se[boxes[j]]) / seg),
+// tstart[n] = total.
+struct SegList {
+    const int* boxes;
+    int n;
+    const int2* se;
+    int seg;
+    int* tstart;
+};
+struct SegLists {
+    SegList l[32];
+    int count;
+};
+void launch_seg_scan(const SegLists& L, hipStream_t s);
 
 void launch_merge(const int* parents, int np, const int4* child, const double* cen,
                   const int2* src_se, const int* slot, const double* wms, int kidx_base,
@@ -137,6 +163,9 @@ struct EvalArgs {
     const double* loc;      // PW local expansions per slot
     double* taylor;         // scratch: (nbox, (nl+1)^2)
     double *pot, *grad, *hess;  // sorted outputs
+    // point segments (as FormArgs); tstart == nullptr: one block per box
+    const int* tstart;
+    int seg, grid;
 };
 void launch_pw2local(const EvalArgs& a, hipStream_t s);
 void launch_taylor_eval(const EvalArgs& a, hipStream_t s);

@@ -11,6 +11,7 @@
 // 64-wide wavefronts; 256-thread blocks (4 waves) throughout.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_select.hpp>
@@ -143,9 +144,11 @@ void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, in
 // (ndiv0 below the cutoff level, ndiv at or beyond it) and l < nlmax
 // (pts_tree_build, pts_tree2d4.f:507-523).  Counts shrink with depth and
 // ndiv(l) grows with l, so along a point's path the refined levels are
-// 0..L-1; L is found by widening the key-sorted run around the point from
-// the finest level upwards until it holds more than ndiv(l) points, which
-// never needs more than ndiv+1 neighbours on either side.
+// 0..L-1.  L is found from the finest level upwards: the point's run of equal
+// level-l prefixes in the sorted keys is located by two binary searches in
+// the window [i - lim, i + lim] (prefixes are sorted, so "same prefix" is a
+// contiguous interval); a run that fills the window on either side holds more
+// than lim points.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_leafdepth(const u64* __restrict__ key, int n, int nlmax,
                                                    int levcut, int ndiv0, int ndiv,
@@ -154,24 +157,35 @@ __global__ __launch_bounds__(256) void k_leafdepth(const u64* __restrict__ key, 
 {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    u64 ki = key[i];
-    int lo = i, hi = i + 1, L = 0;
+    const u64 ki = key[i];
+    int L = 0;
     for (int l = nlmax - 1; l >= 0; --l) {
-        int sh = 2 * (nlmax - l);
-        u64 p = ki >> sh;
-        int lim = l < levcut ? ndiv0 : ndiv;
-        while (lo > 0 && hi - lo <= lim && (key[lo - 1] >> sh) == p) --lo;
-        while (hi < n && hi - lo <= lim && (key[hi] >> sh) == p) ++hi;
-        if (hi - lo > lim) {
+        const int sh = 2 * (nlmax - l);
+        const u64 p = ki >> sh;
+        const int lim = l < levcut ? ndiv0 : ndiv;
+        // first index in [max(0, i - lim), i] with prefix p
+        int a = max(0, i - lim), b = i;
+        while (a < b) {
+            int m = (a + b) >> 1;
+            if ((key[m] >> sh) == p) b = m; else a = m + 1;
+        }
+        const int lo = a;
+        // last index in [i, min(n - 1, i + lim)] with prefix p
+        a = i;
+        b = min(n - 1, i + lim);
+        while (a < b) {
+            int m = (a + b + 1) >> 1;
+            if ((key[m] >> sh) == p) a = m; else b = m - 1;
+        }
+        if (a - lo + 1 > lim) {
             L = l + 1;
             break;
         }
     }
     level[i] = L;
-    int sh = 2 * (nlmax - L);
+    const int sh = 2 * (nlmax - L);
     unsigned char st = 1;
-    if (i > 0 && sh < 64) st = (key[i - 1] >> sh) != (ki >> sh);
-    if (i > 0 && sh >= 64) st = 0;
+    if (i > 0) st = sh >= 64 ? 0 : ((key[i - 1] >> sh) != (ki >> sh));
     start[i] = st;
 }
 
@@ -477,10 +491,266 @@ __global__ __launch_bounds__(256) void k_form(FormArgs a, int chunk)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Work decomposition.  Leaves hold from one to thousands of points, so the
+// heavy steps run over fixed-size segments of each box's point range rather
+// than over boxes: tstart (an exclusive scan of the per-box segment counts)
+// numbers the segments, a persistent grid strides over them, and a block
+// finds its box by binary search in tstart.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_seg_scan(SegLists L)
+{
+    __shared__ int buf[1024];
+    __shared__ int carry;
+    const SegList d = L.l[blockIdx.x];
+    const int t = threadIdx.x;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base <= d.n; base += 1024) {
+        const int i = base + t;
+        int c = 0;
+        if (i < d.n) {
+            int2 r = d.se[d.boxes[i]];
+            int len = r.y - r.x;
+            c = len > 0 ? (len + d.seg - 1) / d.seg : 0;
+        }
+        buf[t] = c;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int v = t >= off ? buf[t - off] : 0;
+            __syncthreads();
+            buf[t] += v;
+            __syncthreads();
+        }
+        const int incl = buf[t];
+        if (i <= d.n) d.tstart[i] = carry + incl - c;
+        __syncthreads();
+        if (t == 1023) carry += incl;
+        __syncthreads();
+    }
+}
+
+void launch_seg_scan(const SegLists& L, hipStream_t s)
+{
+    if (L.count <= 0) return;
+    k_seg_scan<<<L.count, 1024, 0, s>>>(L);
+}
+
+struct SegRange {
+    int li, p0, p1;
+};
+
+// segment g of a list: its box index li and point range [p0, p1)
+__device__ __forceinline__ SegRange seg_at(const int* tstart, int nbox, const int* boxes,
+                                           const int2* se, int seg, int g)
+{
+    SegRange r;
+    if (!tstart) {
+        int2 x = se[boxes[g]];
+        r.li = g;
+        r.p0 = x.x;
+        r.p1 = x.y;
+        return r;
+    }
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (tstart[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    int2 x = se[boxes[lo]];
+    r.li = lo;
+    r.p0 = x.x + (g - tstart[lo]) * seg;
+    r.p1 = min(x.y, r.p0 + seg);
+    return r;
+}
+
+__device__ __forceinline__ int seg_count(const int* tstart, int nbox)
+{
+    return tstart ? tstart[nbox] : nbox;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 matrix cores.  v_mfma_f64_16x16x4f64: D(16x16) += A(16x4) B(4x16) per
+// wavefront; lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
+// D[l/16 + 4r][l%16] in element r of the result (lane layout measured on
+// gfx950).
+// ---------------------------------------------------------------------------
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+// Kernels that take more than 64 KB of dynamic LDS (gfx950 has 160 KB per
+// workgroup) declare it once per instantiation.
+template <class F>
+static void allow_lds(F* kernel, size_t bytes)
+{
+    if (bytes > 65536)
+        HIPCHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)bytes));
+}
+
+__device__ __forceinline__ d4_t mfma16x16x4(double a, double b, d4_t c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// LDS writes of one wavefront made visible to its other lanes
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Step 1 on matrix cores, in two kernels.
+// (a) per segment of a leaf's sources, the raw moments M = X diag(q) Y^T with
+//     X(j,s) = xh_s^j and Y(k,s) = yh_s^k staged in LDS; each wave multiplies
+//     4 sources per MFMA into TM x TM tiles of 16x16; the four waves' tiles are
+//     summed in a fixed order and written as the segment's partial moments.
+// (b) per leaf, the partial moments of its segments are summed in segment
+//     order and converted, PW = h2xr^T M h2xr with h2xr(j,a) =
+//     ws_a (-i ts_a)^j / j! (the 1/sqrt(j!) Hermite normalisation of
+//     g2dformhc_vec twice), as g2dh2pw_real_vec.
+template <int TM>
+__global__ __launch_bounds__(256) void k_form_moments(FormArgs a)
+{
+    extern __shared__ double sm[];
+    constexpr int MP = 16 * TM, RS = MP + 1;  // odd row stride: no LDS bank conflicts
+    const int m = a.p + 1, t = threadIdx.x, lane = t & 63, w = t >> 6;
+    double* X = sm;               // seg x RS, zero padded
+    double* Y = sm + a.seg * RS;  // seg x RS, q folded in
+    const int total = seg_count(a.tstart, a.nbox);
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const SegRange r = seg_at(a.tstart, a.nbox, a.boxes, a.src_se, a.seg, g);
+        const int box = a.boxes[r.li], cn = r.p1 - r.p0, cpad = (cn + 15) & ~15;
+        const double cx = a.cen[2 * box], cy = a.cen[2 * box + 1];
+        __syncthreads();
+        for (int u = t; u < cpad; u += 256) {
+            double* xr = X + u * RS;
+            double* yr = Y + u * RS;
+            int j0 = 0;
+            if (u < cn) {
+                long i = r.p0 + u;
+                double x = (a.xs[2 * i] - cx) * a.dsq, y = (a.xs[2 * i + 1] - cy) * a.dsq;
+                double px = 1.0, py = a.qs[(long)a.nd * i + a.ind];
+                xr[0] = px;
+                yr[0] = py;
+                for (int j = 1; j < m; ++j) {
+                    px *= x;
+                    py *= y;
+                    xr[j] = px;
+                    yr[j] = py;
+                }
+                j0 = m;
+            }
+            for (int j = j0; j < MP; ++j) {
+                xr[j] = 0;
+                yr[j] = 0;
+            }
+        }
+        __syncthreads();
+        d4_t acc[TM][TM];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TM; ++j) acc[i][j] = (d4_t){0, 0, 0, 0};
+        const int s_l = lane >> 4, i_l = lane & 15;
+        for (int gg = w; gg < cpad / 4; gg += 4) {
+            const int src = 4 * gg + s_l;
+            double av[TM], bv[TM];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+                av[i] = X[src * RS + i_l + 16 * i];
+                bv[i] = Y[src * RS + i_l + 16 * i];
+            }
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TM; ++j) acc[i][j] = mfma16x16x4(av[i], bv[j], acc[i][j]);
+        }
+        __syncthreads();
+        double* Hf = sm;  // MP x MP, row = x order, column = y order (reuses X)
+        for (int ww = 0; ww < 4; ++ww) {
+            if (w == ww) {
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TM; ++j)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            int row = (lane >> 4) + 4 * q + 16 * i, col = (lane & 15) + 16 * j;
+                            double v = acc[i][j][q];
+                            Hf[row * MP + col] = ww == 0 ? v : Hf[row * MP + col] + v;
+                        }
+            }
+            __syncthreads();
+        }
+        double* out = a.partial + (long)g * a.pstride;  // m x m, x order fastest
+        for (int o = t; o < m * m; o += 256) out[o] = Hf[(o % m) * MP + o / m];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_form_h2pw(FormArgs a)
+{
+    extern __shared__ double sm[];
+    const int li = blockIdx.x;
+    const int box = a.boxes[li];
+    const int g0 = a.tstart[li], g1 = a.tstart[li + 1];
+    if (g1 <= g0) return;  // no sources: the expansion stays zero
+    const int m = a.p + 1, t = threadIdx.x, npw = a.npw, half = npw / 2, nexp = half * npw;
+    double* H = sm;                // m x m, x order fastest
+    double* hx = H + m * m;        // complex (m, npw)
+    double* T = hx + 2 * m * npw;  // complex (half, m)
+    for (int o = t; o < m * m; o += 256) {
+        double v = 0;
+        for (int g = g0; g < g1; ++g) v += a.partial[(long)g * a.pstride + o];
+        H[o] = v;
+    }
+    for (int o = t; o < 2 * m * npw; o += 256) hx[o] = a.h2xr[o];
+    __syncthreads();
+    for (int o = t; o < half * m; o += 256) {
+        int aa = o % half, k = o / half;
+        double re = 0, im = 0;
+        for (int j = 0; j < m; ++j) {
+            double h = H[j + m * k];
+            re += hx[2 * (j * npw + aa)] * h;
+            im += hx[2 * (j * npw + aa) + 1] * h;
+        }
+        T[2 * o] = re;
+        T[2 * o + 1] = im;
+    }
+    __syncthreads();
+    double* out = a.mp + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+    for (int o = t; o < nexp; o += 256) {
+        int aa = o % half, b = o / half;
+        double re = 0, im = 0;
+        for (int k = 0; k < m; ++k) {
+            double wr = hx[2 * (k * npw + b)], wi = hx[2 * (k * npw + b) + 1];
+            double tr = T[2 * (aa + half * k)], ti = T[2 * (aa + half * k) + 1];
+            re += wr * tr - wi * ti;
+            im += wr * ti + wi * tr;
+        }
+        out[2 * o] = re;
+        out[2 * o + 1] = im;
+    }
+}
+
 void launch_form(const FormArgs& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
     int m = a.p + 1;
+    if (m <= 64 && a.tstart) {
+        const int TM = (m + 15) / 16, MP = 16 * TM;
+        size_t lds = std::max((size_t)2 * a.seg * (MP + 1), (size_t)MP * MP) * sizeof(double);
+        switch (TM) {
+        case 1: allow_lds(k_form_moments<1>, lds); k_form_moments<1><<<a.grid, 256, lds, s>>>(a); break;
+        case 2: allow_lds(k_form_moments<2>, lds); k_form_moments<2><<<a.grid, 256, lds, s>>>(a); break;
+        case 3: allow_lds(k_form_moments<3>, lds); k_form_moments<3><<<a.grid, 256, lds, s>>>(a); break;
+        default: allow_lds(k_form_moments<4>, lds); k_form_moments<4><<<a.grid, 256, lds, s>>>(a); break;
+        }
+        size_t lds2 = ((size_t)m * m + 2 * (size_t)m * a.npw + (size_t)a.npw * m) * sizeof(double);
+        allow_lds(k_form_h2pw, lds2);
+        k_form_h2pw<<<a.nbox, 256, lds2, s>>>(a);
+        return;
+    }
     int chunk = 4096 / m;
     if (chunk > 256) chunk = 256;
     if (chunk < 16) chunk = 16;
@@ -714,16 +984,17 @@ template <int PGH>
 __global__ __launch_bounds__(256) void k_taylor_eval(EvalArgs a)
 {
     extern __shared__ double sm[];
-    int bi = blockIdx.x;
-    int b = a.boxes[bi];
-    int2 se = a.se[b];
-    if (se.y <= se.x) return;
     const int nl = a.nl, m = nl + 1, t = threadIdx.x;
-    const double* Lg = a.taylor + (long)bi * m * m;
+    const int total = seg_count(a.tstart, a.nbox);
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g);
+    const int b = a.boxes[sr.li];
+    const double* Lg = a.taylor + (long)sr.li * m * m;
+    __syncthreads();
     for (int o = t; o < m * m; o += 256) sm[o] = Lg[o];
     __syncthreads();
     const double cx = a.cen[2 * b], cy = a.cen[2 * b + 1], dsq = a.dsq;
-    for (int i = se.x + t; i < se.y; i += 256) {
+    for (int i = sr.p0 + t; i < sr.p1; i += 256) {
         double x = (a.pts[2 * (long)i] - cx) * dsq, y = (a.pts[2 * (long)i + 1] - cy) * dsq;
         // V = sum_k y^k P_k(x), U = sum_k y^k P_k'(x), Z = sum_k y^k P_k''(x)/2
         double V = 0, Vy = 0, Vyy = 0, U = 0, Uy = 0, Z = 0;
@@ -760,19 +1031,182 @@ __global__ __launch_bounds__(256) void k_taylor_eval(EvalArgs a)
             a.hess[h + 2 * a.nd] += 2 * Vyy * dsq * dsq;
         }
     }
+    }
+}
+
+// Step 5b on matrix cores.  For a tile of 16 points of a leaf the MFMA
+// computes D(k,t) = sum_j L(j,k) xh_t^j with A = L^T (LDS, x-derivative
+// variants Lx, Lxx for gradients and Hessians) and B(j,t) = xh_t^j (running
+// products in registers).  Lane l then holds y orders k = l/16 + 4r (+16 per
+// column tile) of point t = l%16, so sum_k D(k,t) yh_t^k is a running
+// product in registers plus a reduction over the 4 lanes of the point.
+// Outputs are stored, not accumulated: each point is evaluated by exactly
+// one leaf and the buffers start at zero.
+template <int TM, int PGH>
+__global__ __launch_bounds__(256) void k_taylor_mfma(EvalArgs a)
+{
+    extern __shared__ double sm[];
+    constexpr int MP = 16 * TM, NV = PGH >= 3 ? 3 : (PGH >= 2 ? 2 : 1);
+    const int m = a.nl + 1, t = threadIdx.x, lane = t & 63, w = t >> 6;
+    double* Lb = sm;  // NV x MP x MP, row j, column k
+    const int total = seg_count(a.tstart, a.nbox);
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g);
+    const int b = a.boxes[sr.li];
+    const double* Lg = a.taylor + (long)sr.li * m * m;  // L(j,k) at j + m k
+    __syncthreads();
+    for (int o = t; o < MP * MP; o += 256) {
+        int j = o / MP, k = o % MP;
+        Lb[o] = (j < m && k < m) ? Lg[j + m * k] : 0.0;
+        if (NV >= 2) Lb[MP * MP + o] = (j + 1 < m && k < m) ? (j + 1) * Lg[(j + 1) + m * k] : 0.0;
+        if (NV >= 3)
+            Lb[2 * MP * MP + o] =
+                (j + 2 < m && k < m) ? (double)((j + 1) * (j + 2)) * Lg[(j + 2) + m * k] : 0.0;
+    }
+    __syncthreads();
+    const double cx = a.cen[2 * b], cy = a.cen[2 * b + 1], dsq = a.dsq;
+    const int col = lane & 15, kq = lane >> 4;
+    const int npts = sr.p1 - sr.p0;
+    for (int base = 16 * w; base < npts; base += 64) {
+        const int ip = base + col;
+        const bool act = ip < npts;
+        const long i = sr.p0 + (act ? ip : 0);
+        const double x = act ? (a.pts[2 * i] - cx) * dsq : 0.0;
+        const double y = act ? (a.pts[2 * i + 1] - cy) * dsq : 0.0;
+        double p = 1.0;
+        for (int r = 0; r < kq; ++r) p *= x;  // x^(l/16)
+        double x4 = x * x;
+        x4 *= x4;
+        d4_t acc[NV][TM];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int j = 0; j < TM; ++j) acc[v][j] = (d4_t){0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < MP / 4; ++ks) {
+            const int row = 4 * ks + kq;
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int j = 0; j < TM; ++j)
+                    acc[v][j] = mfma16x16x4(Lb[v * MP * MP + row * MP + col + 16 * j], p, acc[v][j]);
+            p *= x4;
+        }
+        // this lane: orders k = kq + 4r + 16j of point col
+        double yk = 1.0;  // y^kq
+        for (int r = 0; r < kq; ++r) yk *= y;
+        double y4 = y * y;
+        y4 *= y4;
+        double V = 0, Vy = 0, Vyy = 0, U = 0, Uy = 0, Z = 0;
+#pragma unroll
+        for (int j = 0; j < TM; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                V += acc[0][j][r] * yk;
+                if (PGH >= 2) U += acc[1][j][r] * yk;
+                if (PGH >= 3) Z += acc[2][j][r] * yk;
+                yk *= y4;
+            }
+        if (PGH >= 2) {
+            // y derivatives: k y^(k-1) and k (k-1) y^(k-2) along this lane's
+            // orders; e1, e2 start at the first order where they are nonzero
+            const int f1 = kq >= 1 ? 0 : 1, f2 = kq >= 2 ? 0 : 1;
+            const int p1 = kq >= 1 ? kq - 1 : 3, p2 = kq >= 2 ? kq - 2 : kq + 2;
+            double e1 = 1.0, e2 = 1.0;
+            for (int r = 0; r < p1; ++r) e1 *= y;
+            for (int r = 0; r < p2; ++r) e2 *= y;
+            int st = 0;
+#pragma unroll
+            for (int j = 0; j < TM; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r, ++st) {
+                    const double k = kq + 4 * r + 16 * j;
+                    if (st >= f1) {
+                        Vy += k * acc[0][j][r] * e1;
+                        if (PGH >= 3) Uy += k * acc[1][j][r] * e1;
+                        e1 *= y4;
+                    }
+                    if (PGH >= 3 && st >= f2) {
+                        Vyy += k * (k - 1) * acc[0][j][r] * e2;
+                        e2 *= y4;
+                    }
+                }
+        }
+        // sum over the four lanes that hold this point
+        V += __shfl_xor(V, 16);
+        V += __shfl_xor(V, 32);
+        if (PGH >= 2) {
+            U += __shfl_xor(U, 16);
+            U += __shfl_xor(U, 32);
+            Vy += __shfl_xor(Vy, 16);
+            Vy += __shfl_xor(Vy, 32);
+        }
+        if (PGH >= 3) {
+            Z += __shfl_xor(Z, 16);
+            Z += __shfl_xor(Z, 32);
+            Uy += __shfl_xor(Uy, 16);
+            Uy += __shfl_xor(Uy, 32);
+            Vyy += __shfl_xor(Vyy, 16);
+            Vyy += __shfl_xor(Vyy, 32);
+        }
+        if (kq == 0 && act) {
+            a.pot[i * a.nd + a.ind] = V;
+            if (PGH >= 2) {
+                const long g = i * 2 * a.nd + a.ind;
+                a.grad[g] = U * dsq;
+                a.grad[g + a.nd] = Vy * dsq;
+            }
+            if (PGH >= 3) {
+                const long h = i * 3 * a.nd + a.ind;
+                a.hess[h] = Z * dsq * dsq;
+                a.hess[h + a.nd] = Uy * dsq * dsq;
+                a.hess[h + 2 * a.nd] = Vyy * dsq * dsq;
+            }
+        }
+    }
+    }
+}
+
+template <int TM>
+static void taylor_mfma_launch(const EvalArgs& a, hipStream_t s)
+{
+    constexpr int MP = 16 * TM;
+    const int nv = a.ifpgh >= 3 ? 3 : (a.ifpgh == 2 ? 2 : 1);
+    size_t lds = (size_t)nv * MP * MP * sizeof(double);
+    const int grid = a.tstart ? a.grid : a.nbox;
+    if (a.ifpgh >= 3) {
+        allow_lds(k_taylor_mfma<TM, 3>, lds);
+        k_taylor_mfma<TM, 3><<<grid, 256, lds, s>>>(a);
+    } else if (a.ifpgh == 2) {
+        allow_lds(k_taylor_mfma<TM, 2>, lds);
+        k_taylor_mfma<TM, 2><<<grid, 256, lds, s>>>(a);
+    } else {
+        allow_lds(k_taylor_mfma<TM, 1>, lds);
+        k_taylor_mfma<TM, 1><<<grid, 256, lds, s>>>(a);
+    }
 }
 
 void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
     int m = a.nl + 1;
+    if (m <= 64) {
+        switch ((m + 15) / 16) {
+        case 1: taylor_mfma_launch<1>(a, s); break;
+        case 2: taylor_mfma_launch<2>(a, s); break;
+        case 3: taylor_mfma_launch<3>(a, s); break;
+        default: taylor_mfma_launch<4>(a, s); break;
+        }
+        return;
+    }
     size_t lds = (size_t)m * m * sizeof(double);
+    const int grid = a.tstart ? a.grid : a.nbox;
     if (a.ifpgh >= 3)
-        k_taylor_eval<3><<<a.nbox, 256, lds, s>>>(a);
+        k_taylor_eval<3><<<grid, 256, lds, s>>>(a);
     else if (a.ifpgh == 2)
-        k_taylor_eval<2><<<a.nbox, 256, lds, s>>>(a);
+        k_taylor_eval<2><<<grid, 256, lds, s>>>(a);
     else
-        k_taylor_eval<1><<<a.nbox, 256, lds, s>>>(a);
+        k_taylor_eval<1><<<grid, 256, lds, s>>>(a);
 }
 
 // Step 5, plane-wave route (g2dpwevalp/g/h_vec, g2drouts.f:1887-2200):
@@ -784,17 +1218,18 @@ template <int PGH>
 __global__ __launch_bounds__(256) void k_pw_eval(EvalArgs a)
 {
     extern __shared__ double sm[];
-    int bi = blockIdx.x;
-    int b = a.boxes[bi];
-    int2 se = a.se[b];
-    if (se.y <= se.x) return;
     const int npw = a.npw, half = npw / 2, nexp = half * npw, t = threadIdx.x;
+    const int total = seg_count(a.tstart, a.nbox);
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g);
+    const int b = a.boxes[sr.li];
     const double* pw = a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
+    __syncthreads();
     for (int o = t; o < 2 * nexp; o += 256) sm[o] = pw[o];
     __syncthreads();
     const double cx = a.cen[2 * b], cy = a.cen[2 * b + 1], dsq = a.dsq;
     const double* ts = a.ts;
-    for (int i = se.x + t; i < se.y; i += 256) {
+    for (int i = sr.p0 + t; i < sr.p1; i += 256) {
         double x = (a.pts[2 * (long)i] - cx) * dsq, y = (a.pts[2 * (long)i + 1] - cy) * dsq;
         double wr[PW_MAXH], wi[PW_MAXH];
         {
@@ -871,6 +1306,7 @@ __global__ __launch_bounds__(256) void k_pw_eval(EvalArgs a)
             a.hess[h + 2 * a.nd] += 2 * hyy * dsq * dsq;
         }
     }
+    }
 }
 
 void launch_pw_eval(const EvalArgs& a, hipStream_t s)
@@ -878,12 +1314,13 @@ void launch_pw_eval(const EvalArgs& a, hipStream_t s)
     if (a.nbox <= 0) return;
     if (a.npw / 2 > PW_MAXH) fail(-3, "plane-wave length too large for the evaluation kernel");
     size_t lds = (size_t)a.npw * a.npw * sizeof(double);
+    const int grid = a.tstart ? a.grid : a.nbox;
     if (a.ifpgh >= 3)
-        k_pw_eval<3><<<a.nbox, 256, lds, s>>>(a);
+        k_pw_eval<3><<<grid, 256, lds, s>>>(a);
     else if (a.ifpgh == 2)
-        k_pw_eval<2><<<a.nbox, 256, lds, s>>>(a);
+        k_pw_eval<2><<<grid, 256, lds, s>>>(a);
     else
-        k_pw_eval<1><<<a.nbox, 256, lds, s>>>(a);
+        k_pw_eval<1><<<grid, 256, lds, s>>>(a);
 }
 
 // ---------------------------------------------------------------------------

@@ -10,6 +10,15 @@
 // then applies the reference's level-restriction procedure
 // (pts_tree_fix_lr, pts_tree2d4.f:693-996) step for step, because the set of
 // boxes it produces is part of the result the transform must match.
+//
+// Adjacency.  The reference decides "touching" and "colleague" with FP64
+// centre distances against 1.05x the half-sum of the box sizes.  Centres of
+// one level differ by whole multiples of the box size (up to rounding
+// ~1e-16 relative), so those tests are exactly integer tests on the box
+// coordinates (ix, iy) of each level, which is what is used here:
+//   same level:          |dix| <= 1 and |diy| <= 1
+//   levels l and l+1:    the finer box's coordinate lies in
+//                        [2 ic - 1, 2 ic + 2] on both axes.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -85,9 +94,11 @@ struct Builder {
     Tree2D& T;
     explicit Builder(Tree2D& t) : T(t) {}
 
-    int add_box(int level, int parent, double cx, double cy, uint64_t pre)
+    int add_box(int level, int parent, double cx, double cy, uint64_t pre, int ix, int iy)
     {
         int id = T.nboxes();
+        T.ix.push_back(ix);
+        T.iy.push_back(iy);
         T.lev.push_back(level);
         T.par.push_back(parent);
         T.nch.push_back(0);
@@ -112,22 +123,30 @@ struct Builder {
         for (int j = 0; j < 4; ++j) {
             double hx = (j & 1) ? bs / 2 : -bs / 2;
             double hy = (j < 2) ? -bs / 2 : bs / 2;
-            int c = add_box(l + 1, b, T.cen[2 * b] + hx, T.cen[2 * b + 1] + hy, (T.pre[b] << 2) | j);
+            int c = add_box(l + 1, b, T.cen[2 * b] + hx, T.cen[2 * b + 1] + hy, (T.pre[b] << 2) | j,
+                            2 * T.ix[b] + (j & 1), 2 * T.iy[b] + (j >> 1));
             T.ch[4 * b + j] = c;
         }
         T.nch[b] = 4;
         return first;
     }
 
-    bool touch(int a, int b, double tol) const
+    // same-level boxes a, b: neighbours (or equal)
+    bool near_same(int a, int b) const
     {
-        return std::fabs(T.cen[2 * a] - T.cen[2 * b]) <= tol &&
-               std::fabs(T.cen[2 * a + 1] - T.cen[2 * b + 1]) <= tol;
+        int dx = T.ix[a] - T.ix[b], dy = T.iy[a] - T.iy[b];
+        return dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1;
+    }
+    // coarse box c (level l) and fine box f (level l+1) share a point
+    bool near_finer(int c, int f) const
+    {
+        int dx = T.ix[f] - 2 * T.ix[c], dy = T.iy[f] - 2 * T.iy[c];
+        return dx >= -1 && dx <= 2 && dy >= -1 && dy <= 2;
     }
 
     // Same-level neighbours (self included), searched among the children of
-    // the parent's neighbours, |dc| <= 1.05 bs per axis, with the 1.05 a
-    // single-precision constant as in computecoll (tree_routs4.f:100-211).
+    // the parent's neighbours in the reference's order (computecoll,
+    // tree_routs4.f:100-211).
     void colleagues(int b)
     {
         int l = T.lev[b];
@@ -138,19 +157,18 @@ struct Builder {
             return;
         }
         int dad = T.par[b];
-        double tol = (double)1.05f * T.bsz[l];
         for (int i = 0; i < T.ncol[dad]; ++i) {
             int jb = T.col[9 * dad + i];
             for (int j = 0; j < 4; ++j) {
                 int kb = T.ch[4 * jb + j];
-                if (kb >= 0 && touch(kb, b, tol)) T.col[9 * b + T.ncol[b]++] = kb;
+                if (kb >= 0 && near_same(kb, b)) T.col[9 * b + T.ncol[b]++] = kb;
             }
         }
     }
 
-    void all_colleagues()
+    void all_colleagues(int maxlev)
     {
-        for (int l = 0; l <= T.nlevels; ++l)
+        for (int l = 0; l <= std::min(T.nlevels, maxlev); ++l)
             for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) colleagues(b);
     }
 
@@ -169,43 +187,50 @@ struct Builder {
 
     // Renumber so that each level lists its original boxes, then the boxes
     // appended to it (pts_tree_reorg, pts_tree2d4.f:1001-1150).
+    std::vector<char> scratch;
+    template <class V>
+    void permute(V& v, int width, const std::vector<int>& map)
+    {
+        using E = typename V::value_type;
+        scratch.resize(v.size() * sizeof(E));
+        E* old = (E*)scratch.data();
+        std::memcpy(old, v.data(), v.size() * sizeof(E));
+        for (size_t b = 0; b < map.size(); ++b)
+            for (int k = 0; k < width; ++k) v[(size_t)width * map[b] + k] = old[(size_t)width * b + k];
+    }
     void reorg(const std::vector<int>& tail)
     {
-        int n = T.nboxes();
-        Tree2D old = T;
+        const int n = T.nboxes();
         std::vector<int> map(n);
         for (int l = 0; l <= 1 && l <= T.nlevels; ++l)
-            for (int b = old.laddr[2 * l]; b <= old.laddr[2 * l + 1]; ++b) map[b] = b;
-        int cur = old.laddr[4];
+            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) map[b] = b;
+        int cur = T.laddr[4];
         for (int l = 2; l <= T.nlevels; ++l) {
+            int f0 = T.laddr[2 * l], f1 = T.laddr[2 * l + 1];
             T.laddr[2 * l] = cur;
-            for (int pass = 0; pass < 2; ++pass) {
-                int a = pass ? tail[2 * l] : old.laddr[2 * l];
-                int e = pass ? tail[2 * l + 1] : old.laddr[2 * l + 1];
-                for (int b = a; b <= e; ++b) {
-                    T.lev[cur] = old.lev[b];
-                    T.nch[cur] = old.nch[b];
-                    T.cen[2 * cur] = old.cen[2 * b];
-                    T.cen[2 * cur + 1] = old.cen[2 * b + 1];
-                    T.flag[cur] = old.flag[b];
-                    T.pre[cur] = old.pre[b];
-                    map[b] = cur++;
-                }
-            }
+            for (int b = f0; b <= f1; ++b) map[b] = cur++;
+            for (int b = tail[2 * l]; b <= tail[2 * l + 1]; ++b) map[b] = cur++;
             T.laddr[2 * l + 1] = cur - 1;
         }
-        for (int b = 0; b < n; ++b) {
-            T.par[map[b]] = old.par[b] < 0 ? -1 : map[old.par[b]];
-            for (int j = 0; j < 4; ++j)
-                T.ch[4 * map[b] + j] = old.ch[4 * b + j] < 0 ? -1 : map[old.ch[4 * b + j]];
-        }
+        permute(T.lev, 1, map);
+        permute(T.nch, 1, map);
+        permute(T.cen, 2, map);
+        permute(T.flag, 1, map);
+        permute(T.pre, 1, map);
+        permute(T.ix, 1, map);
+        permute(T.iy, 1, map);
+        permute(T.par, 1, map);
+        permute(T.ch, 4, map);
+        for (int& p : T.par)
+            if (p >= 0) p = map[p];
+        for (int& c : T.ch)
+            if (c >= 0) c = map[c];
     }
 
     // flag++ boxes with a touching grandchild-level box that has children are
     // marked for refinement (updateflags, tree_routs4.f:217-329).
-    void update_flags(int curlev, int b0, int b1)
+    void update_flags(int b0, int b1)
     {
-        double tol = 1.05 * (T.bsz[curlev] + T.bsz[curlev + 1]) / 2.0;
         for (int b = b0; b <= b1; ++b) {
             if (T.flag[b] != 3) continue;
             T.flag[b] = 0;
@@ -213,7 +238,7 @@ struct Builder {
                 int jb = T.col[9 * b + i];
                 for (int j = 0; j < 4; ++j) {
                     int kb = T.ch[4 * jb + j];
-                    if (kb >= 0 && T.nch[kb] > 0 && touch(kb, b, tol)) {
+                    if (kb >= 0 && T.nch[kb] > 0 && near_finer(b, kb)) {
                         T.flag[b] = 1;
                         break;
                     }
@@ -222,34 +247,36 @@ struct Builder {
         }
     }
 
-    // Level restriction, pts_tree_fix_lr (pts_tree2d4.f:693-996).
-    void fix_level_restriction()
+    // Level restriction, pts_tree_fix_lr (pts_tree2d4.f:693-996).  Returns
+    // the deepest level whose colleague lists are still valid afterwards.
+    int fix_level_restriction()
     {
         const int L = T.nlevels;
         std::fill(T.flag.begin(), T.flag.end(), 0);
         // leaves touching a box two levels finer, seen from its parent
-        for (int l = L; l >= 2; --l) {
-            double tol = 1.05 * (T.bsz[l - 1] + T.bsz[l - 2]) / 2.0;
+        int nflag = 0;
+        for (int l = L; l >= 2; --l)
             for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
                 int dad = T.par[b], gd = T.par[dad];
                 for (int i = 0; i < T.ncol[gd]; ++i) {
                     int jb = T.col[9 * gd + i];
-                    if (T.nch[jb] == 0 && T.flag[jb] == 0 && touch(jb, dad, tol)) T.flag[jb] = 1;
+                    if (T.nch[jb] == 0 && T.flag[jb] == 0 && near_finer(jb, dad)) {
+                        T.flag[jb] = 1;
+                        ++nflag;
+                    }
                 }
             }
-        }
+        if (nflag == 0) return L - 1;  // already level restricted: nothing below changes
         // coarser leaves touching a flagged box, propagated upwards
-        for (int l = L; l >= 1; --l) {
-            double tol = 1.05 * (T.bsz[l] + T.bsz[l - 1]) / 2.0;
+        for (int l = L; l >= 1; --l)
             for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
                 if (T.flag[b] != 1 && T.flag[b] != 2) continue;
                 int dad = T.par[b];
                 for (int i = 0; i < T.ncol[dad]; ++i) {
                     int jb = T.col[9 * dad + i];
-                    if (T.nch[jb] == 0 && T.flag[jb] == 0 && touch(jb, b, tol)) T.flag[jb] = 2;
+                    if (T.nch[jb] == 0 && T.flag[jb] == 0 && near_finer(jb, b)) T.flag[jb] = 2;
                 }
             }
-        }
         std::vector<int> tail(2 * (L + 1));
         for (int l = 0; l <= L; ++l) { tail[2 * l] = 0; tail[2 * l + 1] = -1; }
         for (int l = 1; l <= L - 2; ++l) {
@@ -258,13 +285,17 @@ struct Builder {
             tail[2 * (l + 1) + 1] = T.nboxes() - 1;
         }
         reorg(tail);
-        all_colleagues();
-        for (int& f : T.flag)
+        all_colleagues(L - 1);
+        int n3 = 0;
+        for (int& f : T.flag) {
             if (f != 3) f = 0;
+            else ++n3;
+        }
+        if (n3 == 0) return L - 1;  // no flag++ box: the second pass adds nothing
         for (int l = 0; l <= L; ++l) { tail[2 * l] = 0; tail[2 * l + 1] = -1; }
         for (int l = 2; l <= L - 2; ++l) {
-            update_flags(l, T.laddr[2 * l], T.laddr[2 * l + 1]);
-            update_flags(l, tail[2 * l], tail[2 * l + 1]);
+            update_flags(T.laddr[2 * l], T.laddr[2 * l + 1]);
+            update_flags(tail[2 * l], tail[2 * l + 1]);
             tail[2 * (l + 1)] = T.nboxes();
             refine_flagged(T.laddr[2 * l], T.laddr[2 * l + 1]);
             refine_flagged(tail[2 * l], tail[2 * l + 1]);
@@ -272,11 +303,19 @@ struct Builder {
             for (int b = tail[2 * (l + 1)]; b <= tail[2 * (l + 1) + 1]; ++b) colleagues(b);
         }
         reorg(tail);
-        all_colleagues();
+        return -1;
     }
 };
 
 }  // namespace
+
+void complete_colleagues(Tree2D& T)
+{
+    Builder B(T);
+    for (int l = T.coll_levels + 1; l <= T.nlevels; ++l)
+        for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) B.colleagues(b);
+    T.coll_levels = T.nlevels;
+}
 
 void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int npwlevel,
                   double bs0, const double* cen0)
@@ -288,8 +327,17 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
     T.cen0[0] = cen0[0];
     T.cen0[1] = cen0[1];
     Builder B(T);
+    {
+        // the adaptive tree has at most 4 boxes per distinct proper prefix of
+        // the leaves; the restriction pass adds a fraction more
+        size_t cap = 4 * leaves.size() + 64;
+        cap += cap / 2;
+        T.lev.reserve(cap); T.par.reserve(cap); T.nch.reserve(cap); T.ch.reserve(4 * cap);
+        T.ncol.reserve(cap); T.col.reserve(9 * cap); T.flag.reserve(cap); T.cen.reserve(2 * cap);
+        T.pre.reserve(cap); T.ix.reserve(cap); T.iy.reserve(cap);
+    }
     T.bsz.push_back(bs0);
-    B.add_box(0, -1, cen0[0], cen0[1], 0);
+    B.add_box(0, -1, cen0[0], cen0[1], 0, 0, 0);
     T.laddr = {0, 0};
     // frontier: boxes of the current level with their range of leaf records
     std::vector<int> fbox{0}, fa{0}, fe{(int)leaves.size()};
@@ -328,22 +376,38 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
         ++l;
     }
     T.nlevels = l;
-    B.all_colleagues();
-    if (T.nlevels >= 2) B.fix_level_restriction();
+    // Colleague lists: the level restriction reads them down to level
+    // nlevels-1; the transform itself only at and above the cutoff level
+    // (m2l, iflocal, near-field lists).  Deeper lists are filled on demand
+    // by complete_colleagues().
+    B.all_colleagues(T.nlevels - 1);
+    int valid = T.nlevels - 1;
+    if (T.nlevels >= 2) valid = B.fix_level_restriction();
+    const int keep = std::min(std::max(npwlevel, 0), T.nlevels);
+    if (valid < keep) {
+        B.all_colleagues(keep);
+        valid = keep;
+    }
+    T.coll_levels = valid;
 
-    // leaves in Morton order: their prefixes aligned to full depth are distinct
+    // leaves in Morton order: depth-first, children in digit order
     int nb = T.nboxes();
-    std::vector<std::pair<uint64_t, int>> lk;
-    for (int b = 0; b < nb; ++b)
-        if (T.nch[b] == 0) lk.push_back({T.pre[b] << (2 * (nlmax - T.lev[b])), b});
-    std::sort(lk.begin(), lk.end());
-    T.leaf_of_rank.resize(lk.size());
+    T.leaf_of_rank.clear();
     T.rank_of_box.assign(nb, -1);
     T.rank_lo.assign(nb, 0);
     T.rank_hi.assign(nb, 0);
-    for (size_t r = 0; r < lk.size(); ++r) {
-        T.leaf_of_rank[r] = lk[r].second;
-        T.rank_of_box[lk[r].second] = (int)r;
+    {
+        std::vector<int> stack{0};
+        while (!stack.empty()) {
+            int b = stack.back();
+            stack.pop_back();
+            if (T.nch[b] == 0) {
+                T.rank_of_box[b] = (int)T.leaf_of_rank.size();
+                T.leaf_of_rank.push_back(b);
+            } else {
+                for (int j = 3; j >= 0; --j) stack.push_back(T.ch[4 * b + j]);
+            }
+        }
     }
     for (int lv = T.nlevels; lv >= 0; --lv)
         for (int b = T.laddr[2 * lv]; b <= T.laddr[2 * lv + 1]; ++b) {

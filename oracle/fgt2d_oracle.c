@@ -1293,3 +1293,22 @@ double orc_derr(const double *vref, const double *vcomp, int n)
     if (sqrt(ra) / n < 1.0e-10) return sqrt(err) / n;
     return sqrt(err / ra);
 }
+
+/* DLARAN (common/dlaran.f:40-89) behind hkrand (common/hkrand.f): the 48-bit
+ * multiplicative congruential generator seed <- seed * 33952834046453 mod 2^48
+ * (multiplier 494,322,2508,2549 in base 4096), value seed / 2^48 (exact in
+ * FP64).  hkrand's default seed is (0,0,0,1), i.e. 1.  Draws n values,
+ * continuing from *state (pass 1 for a fresh hkrand stream). */
+void orc_dlaran(uint64_t *state, int n, double *out)
+{
+    const uint64_t M = 33952834046453ull, MASK = (1ull << 48) - 1;
+    uint64_t s = *state;
+    for (int i = 0; i < n; i++) {
+        /* 48x48-bit product mod 2^48 without overflow: split the multiplier */
+        uint64_t lo = (s * (M & 0xFFFFFFull)) & MASK;
+        uint64_t hi = ((s * (M >> 24)) & ((1ull << 24) - 1)) << 24;
+        s = (lo + hi) & MASK;
+        out[i] = (double)s / 281474976710656.0;
+    }
+    *state = s;
+}

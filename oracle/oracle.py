@@ -59,6 +59,8 @@ def _orc():
         lib.orc_last_time.restype = C.c_double
         lib.orc_derr.restype = C.c_double
         lib.orc_derr.argtypes = [dp, dp, C.c_int]
+        lib.orc_dlaran.restype = None
+        lib.orc_dlaran.argtypes = [C.POINTER(C.c_uint64), C.c_int, dp]
         _ORC = lib
     return _ORC
 
@@ -132,6 +134,45 @@ def derr(vref, vcomp):
     a = np.ascontiguousarray(vref, dtype=np.float64).ravel()
     b = np.ascontiguousarray(vcomp, dtype=np.float64).ravel()
     return float(_orc().orc_derr(_d(a), _d(b), a.size))
+
+
+class Hkrand:
+    """The reference test's random stream (hkrand -> DLARAN, seed 1)."""
+
+    def __init__(self):
+        self.state = C.c_uint64(1)
+
+    def draw(self, n):
+        out = np.zeros(n)
+        _orc().orc_dlaran(C.byref(self.state), n, _d(out))
+        return out
+
+
+def reftest_inputs(nsrc, ntarg):
+    """Inputs of the reference's own test driver (2d/planewave/test/
+    test_fgt2d.f:81-120) for nsrc sources and ntarg targets: per source seven
+    draws (angle, x, y, two normal components, charge, dipole strength), the
+    corner points (0,0) and (1,1) with zero charge and a unit charge at
+    (0.05, 0.05), then two draws per target."""
+    r = Hkrand()
+    d = r.draw(7 * nsrc).reshape(nsrc, 7)
+    src = d[:, 1:3].copy()
+    chg = d[:, 5:6].copy()
+    chg[0:2] = 0.0
+    chg[2] = 1.0
+    src[0] = (0.0, 0.0)
+    src[1] = (1.0, 1.0)
+    src[2] = (0.05, 0.05)
+    tg = r.draw(2 * ntarg).reshape(ntarg, 2)
+    return src, chg, tg
+
+
+def ref_hkrand(n):
+    """n draws of the reference's own hkrand(0) from oracle/_ref (fresh
+    process state only: hkrand keeps its seed between calls)."""
+    lib = _ref()
+    lib.hkrand_.restype = C.c_double
+    return np.array([lib.hkrand_(_ri(0)) for _ in range(n)])
 
 
 def hlterms(ndim, bsize, delta, eps):
@@ -237,6 +278,34 @@ def ref_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
     nt = tg.shape[0] if targ is not None else 0
     pot, grad, hess = _outs(nd, ns, ifpgh)
     pt, gt, ht = _outs(nd, max(nt, 1), ifpghtarg)
+    lc, bs0, cen0 = C.c_int(), C.c_double(), np.zeros(2)
+    lib.pts_tree_boxsize0_(_rd(delta), _rd(eps), _d(src), _ri(ns), _d(tg), _ri(nt), C.byref(lc),
+                           C.byref(bs0), _d(cen0))
+    if lc.value < -2:
+        # fgt2d_large_delta (fgt2d.f:1616-1735) allocates local(0:nlocal,
+        # 0:nlocal,nd) and lets g2dformlc_vec add into it without zeroing
+        # it first (fgt2d.f:1678-1690, g2drouts.f:621-682), so its result
+        # depends on leftover heap contents.  Run its steps -- the same
+        # reference routines, in the same order -- with local zeroed.
+        nl = C.c_int()
+        lib.g2dhlterms_(_ri(2), _rd(bs0.value), _rd(delta), _rd(eps), C.byref(nl))
+        nloc = nl.value + max(ifpgh, ifpghtarg) - 1          # fgt2d.f:134-135
+        local = np.zeros(nd * (nloc + 1) ** 2)
+        chg_f = np.ascontiguousarray(chg)                    # (ns, nd) = charge(nd, ns)
+        lib.g2dformlc_vec_(_ri(nd), _rd(delta), _d(src), _ri(ns), _d(chg_f), _d(cen0), _ri(nloc),
+                           _d(local))
+        legs = [(ifpghtarg, tg, nt, pt, gt, ht), (ifpgh, src, ns, pot, grad, hess)]
+        for ifp, x, n, p, g, h in legs:                      # targets first, fgt2d.f:1697-1732
+            if ifp == 1:
+                lib.g2dlevalp_vec_(_ri(nd), _rd(delta), _d(cen0), _ri(nloc), _d(local), _d(x),
+                                   _ri(n), _d(p))
+            elif ifp == 2:
+                lib.g2dlevalg_vec_(_ri(nd), _rd(delta), _d(cen0), _ri(nloc), _d(local), _d(x),
+                                   _ri(n), _d(p), _d(g))
+            elif ifp == 3:
+                lib.g2dlevalh_vec_(_ri(nd), _rd(delta), _d(cen0), _ri(nloc), _d(local), _d(x),
+                                   _ri(n), _d(p), _d(g), _d(h))
+        return pot, grad, hess, pt[:nt], gt[:nt], ht[:nt]
     dummy = np.zeros(2 * max(ns, nt, 1))
     lib.fgt2d_(_ri(nd), _rd(delta), _rd(eps), _ri(ns), _d(src), _ri(1), _d(chg), _ri(0), _d(dummy),
                _d(dummy), _ri(0), _ri(ifpgh), _d(pot), _d(grad), _d(hess), _ri(nt), _d(tg),

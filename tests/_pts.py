@@ -56,3 +56,35 @@ def leaf_runs(keys_sorted, nlmax, levcut, ndiv0=10, ndiv=80):
             levels.append(L[i])
             prev = tag
     return np.array(keys, dtype=np.uint64), np.array(levels, dtype=np.int32)
+
+
+# ---------------------------------------------------------------------------
+# inputs regenerable bit-for-bit on any host: DLARAN uniforms (the reference
+# test's generator) combined with exact IEEE arithmetic only
+# ---------------------------------------------------------------------------
+def dlaran_points(kind, ns, nt, nd=1):
+    """Sources, charges (ns, nd) and targets drawn from one DLARAN stream.
+
+    uniform  : points uniform in [0,1)^2, charges 2u-1.
+    cluster  : 8 centres in [0.2,0.8]^2; offsets 0.01*(u1+u2+u3+u4-2) per axis
+               (a bell of width ~0.01); charges 2u-1.
+    reftest  : the reference test driver's own inputs (test_fgt2d.f:81-120).
+    """
+    from oracle import oracle as O
+    if kind == "reftest":
+        src, chg, tg = O.reftest_inputs(ns, nt)
+        return src, chg, tg
+    r = O.Hkrand()
+
+    def pts(n):
+        if kind == "uniform":
+            return r.draw(2 * n).reshape(n, 2)
+        cen = 0.2 + 0.6 * r.draw(16).reshape(8, 2)
+        k = np.minimum((r.draw(n) * 8).astype(np.int64), 7)
+        off = r.draw(8 * n).reshape(n, 2, 4).sum(axis=2) - 2.0
+        return cen[k] + 0.01 * off
+
+    src = pts(ns)
+    chg = 2.0 * r.draw(ns * nd).reshape(ns, nd) - 1.0
+    tg = pts(nt)
+    return src, chg, tg

@@ -1,0 +1,100 @@
+"""The oracle (C restatement) against the reference: golden fixtures written
+by the reference build (tests/golden/make_golden.py), the reference library
+itself where it is built (oracle/_ref), and the reference test's own
+direct-sum check (test_fgt2d.f:161-188)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import _pts
+
+GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+NAMES = ["pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"]
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p)[:-4] for p in GOLD])
+def test_oracle_matches_golden(path):
+    g = load(path)
+    ns, nt, nd = int(g["ns"]), int(g["nt"]), int(g["nd"])
+    src, chg, tg = _pts.dlaran_points(str(g["recipe"]), ns, nt, nd)
+    delta, eps = float(g["delta"]), float(g["eps"])
+    out = O.oracle_fgt2d(delta, eps, src, chg, tg, int(g["ifpgh"]), int(g["ifpghtarg"]))
+    for k, name in enumerate(NAMES):
+        if name in g:
+            sel = g["isample_src"] if k < 3 else g["isample_trg"]
+            # same algorithm, same operation order: agreement to the last bits
+            assert _rel(out[k][sel], g[name]) < 1e-13, name
+    st = O.oracle_stats()
+    assert st["npwlevel"] == g["ints"][0]
+    if st["npwlevel"] >= -2:
+        assert [st["nlmax"], st["nlevels"], st["nboxes"]] == list(g["ints"][1:])
+    # the reference test's accuracy check: direct sum at the first 20 sources
+    if "pot" in g:
+        ex = O.direct2d(delta, 1e-100, src, chg, src[:20])[0]
+        assert _rel(ex, g["pot_direct20"]) < 1e-13
+        assert O.derr(g["pot_direct20"], g["pot"][:20]) < 30 * eps
+
+
+def test_dlaran_closed_form():
+    """DLARAN is x <- 33952834046453 x mod 2^48 from x = 1; value x / 2^48."""
+    vals = O.Hkrand().draw(50)
+    x = 1
+    for v in vals:
+        x = (x * 33952834046453) % (1 << 48)
+        assert v == x / 2.0 ** 48
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not present")
+def test_dlaran_matches_reference_hkrand():
+    # hkrand keeps its seed in the process: compare against a fresh stream
+    # continued from however many draws this process already made
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r); from oracle import oracle as O; "
+            "import numpy as np; a = O.ref_hkrand(2000); b = O.Hkrand().draw(2000); "
+            "print(int(np.array_equal(a, b)))" % os.path.dirname(os.path.dirname(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=60)
+    assert r.stdout.strip().endswith("1"), r.stdout + r.stderr
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("recipe,ns,nt,nd,delta,eps,ifs,ift", [
+    ("cluster", 20000, 7000, 1, 1e-5, 1e-6, 1, 1),
+    ("uniform", 30000, 10000, 2, 1e-3, 1e-9, 3, 3),
+    ("cluster", 40000, 5000, 1, 3e-6, 1e-3, 2, 1),
+    ("uniform", 2000, 500, 1, 5.0, 1e-9, 1, 1),
+])
+def test_oracle_matches_reference_library(recipe, ns, nt, nd, delta, eps, ifs, ift):
+    src, chg, tg = _pts.dlaran_points(recipe, ns, nt, nd)
+    if not O.ref_defined(delta, eps, src, tg):
+        pytest.skip("reference undefined (npwlevel == nlevels)")
+    r = O.ref_fgt2d(delta, eps, src, chg, tg, ifs, ift)
+    o = O.oracle_fgt2d(delta, eps, src, chg, tg, ifs, ift)
+    for k in range(6):
+        if (k < 3 and k < ifs) or (k >= 3 and k - 3 < ift):
+            assert _rel(o[k], r[k]) < 1e-13, NAMES[k]
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("recipe,n,delta,eps", [("uniform", 5000, 1e-4, 1e-6), ("cluster", 40000, 1e-5, 1e-6),
+                                                 ("cluster", 20000, 1e-4, 1e-9)])
+def test_oracle_tree_matches_reference_tree(recipe, n, delta, eps):
+    src, _, tg = _pts.dlaran_points(recipe, n, n // 3)
+    a, b = O.oracle_tree2d(delta, eps, src, tg), O.ref_tree2d(delta, eps, src, tg)
+    for k in ("npwlevel", "nlmax", "nlevels", "nboxes"):
+        assert a[k] == b[k], k
+    for k in ("laddr", "lev", "par", "nch", "ch", "ncol", "isrc", "isrcse", "itarg", "itargse"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["cen"], b["cen"]) and np.array_equal(a["bsz"], b["bsz"])
