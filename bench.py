@@ -39,6 +39,95 @@ def workload(ns=NS, nt=NT, seed=SEED):
     return src, chg, tg
 
 
+# MI355X published peaks
+PEAK = {"mfma": (78.6, "TFLOP/s"), "hbm": (8000.0, "GB/s")}
+
+
+def per_unit(kernel, npw, nd):
+    """Algorithmic work of one unit of an instrumented kernel: (bound, amount,
+    what).  Formation and evaluation count the plane-wave contraction of a
+    point, 8*(npw/2)*npw flop to form (plus 8*npw to evaluate); translations
+    count 48 bytes per complex coefficient and box (16 B read, 32 B
+    read-modify-write); gathers and scatters their bytes per point."""
+    half = npw // 2
+    if kernel == "form_moments":
+        return "mfma", 8.0 * half * npw, "flop per source"
+    if kernel in ("taylor_eval", "pw_eval"):
+        return "mfma", 8.0 * half * npw + 8.0 * npw, "flop per evaluation point"
+    if kernel in ("merge", "m2l", "split"):
+        return "hbm", 48.0 * half * npw * nd, "bytes per box"
+    if kernel == "gather":
+        return "hbm", 2 * 8.0 * (2 + nd) + 4, "bytes per point"
+    if kernel == "scatter":
+        return "hbm", 2 * 8.0 * nd + 4, "bytes per point"
+    return None
+
+
+def rocprof_name(kernel, nl, ifpgh):
+    """Symbol of the HIP kernel an instrumented launch runs (as rocprofv3
+    lists it), for the Taylor order nl of its level."""
+    tm = min(4, (nl + 1 + 15) // 16)
+    return {"form_moments": "k_form_moments<%d>" % tm,
+            "taylor_eval": "k_taylor_mfma<%d, %d>" % (tm, ifpgh)}.get(kernel, "k_" + kernel)
+
+
+def read_traffic(name):
+    """HBM bytes per launch of kernel `name` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel factor),
+    or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_c2.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = dict(kv.split("=", 1) for kv in line.split() if "=" in kv)
+        if f.get("kernel") == name and "hbm_bytes" in f:
+            return float(f["hbm_bytes"])
+    return None
+
+
+def kernel_profile(step, torch, reps=5):
+    """Runs `reps` more transforms with timing on; per (kernel, level): total
+    ms, launches and units, plus the stage times of the last transform."""
+    import fgt2d_amd
+    fgt2d_amd.set_timing(True)
+    agg = {}
+    try:
+        for _ in range(reps):
+            step()
+            torch.cuda.synchronize()
+            for r in fgt2d_amd.kernel_launches():
+                a = agg.setdefault((r["kernel"], r["level"]), [0.0, 0, 0])
+                a[0] += r["ms"]
+                a[1] += 1
+                a[2] += r["units"]
+        st = fgt2d_amd.stage_times()
+    finally:
+        fgt2d_amd.set_timing(False)
+    return agg, st
+
+
+def roofline(agg, stats, nd):
+    npw, nlocal = stats["npw"], stats["nlocal"]
+    ranked = sorted(agg.items(), key=lambda kv: -kv[1][0])
+    for (kernel, level), (ms, n, units) in ranked:
+        pu = per_unit(kernel, npw, nd)
+        if pu is None:
+            continue
+        bound, amount, what = pu
+        launch_s = ms / n / 1e3
+        work = amount * units / n
+        peak, unit = PEAK[bound]
+        achieved = work / launch_s / (1e12 if bound == "mfma" else 1e9)
+        nl = nlocal[level] if 0 <= level < len(nlocal) else 0
+        name = rocprof_name(kernel, nl, 1)
+        return {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 4), "traffic": read_traffic(name),
+                "kernel": name, "level": level, "launch_us": round(launch_s * 1e6, 2),
+                "units_per_launch": units // n, "per_unit": "%g %s" % (amount, what),
+                "share_of_kernel_time": round(ms / sum(v[0] for v in agg.values()), 3)}
+    return None
+
+
 def cpu_baseline(src, chg, tg, budget_s=30.0):
     """The reference itself (oracle/_ref, single core: its OpenMP build does
     not compile) on the same workload, or the C restatement when the
@@ -46,23 +135,32 @@ def cpu_baseline(src, chg, tg, budget_s=30.0):
     from oracle import oracle as O
     kind = "reference" if O.have_ref() else "port"
     f = O.ref_fgt2d if kind == "reference" else O.oracle_fgt2d
-    # bounded sample: whole transforms of the first n points of the workload,
-    # n halved until one call takes under budget_s / 3
-    n = len(src)
-    reps, elapsed = 0, 0.0
-    while True:
-        t0 = time.perf_counter()
-        f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1)
-        dt = time.perf_counter() - t0
-        if dt < budget_s / 3 or n <= 100_000:
-            break
-        n //= 2
-    reps, elapsed = 1, dt
-    while elapsed < budget_s / 3 and reps < 3:
-        t0 = time.perf_counter()
-        f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1)
-        elapsed += time.perf_counter() - t0
-        reps += 1
+    # the reference prints its diagnostics on stdout: keep them off the
+    # result line
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        # bounded sample: whole transforms of the first n points of the
+        # workload, n halved until one call takes under budget_s / 3
+        n = len(src)
+        while True:
+            t0 = time.perf_counter()
+            f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1)
+            dt = time.perf_counter() - t0
+            if dt < budget_s / 3 or n <= 100_000:
+                break
+            n //= 2
+        reps, elapsed = 1, dt
+        while elapsed < budget_s / 3 and reps < 3:
+            t0 = time.perf_counter()
+            f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1)
+            elapsed += time.perf_counter() - t0
+            reps += 1
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
     rate = 2 * n * reps / elapsed / 1e6
     return {"value": round(rate, 4), "unit": "Mpts/s", "cores": 1, "kind": kind,
             "sample": "%d full transforms of the first %d sources and %d targets of the workload, "
@@ -76,6 +174,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--stages", action="store_true", help="print per-stage times to stderr")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="end with one 256 MiB device copy (the reference kernel for PMC byte counts)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus,
@@ -106,14 +206,14 @@ def main():
     t = (time.perf_counter() - t0) / args.steps
     stats = fgt2d_amd.last_stats()
 
-    # per-stage times of one more call, HIP events on the call's stream
-    fgt2d_amd.set_timing(True)
-    step()
-    torch.cuda.synchronize()
-    st = fgt2d_amd.stage_times()
-    fgt2d_amd.set_timing(False)
+    # per-kernel and per-stage times of 5 more calls, HIP events on the
+    # call's stream around each instrumented launch
+    agg, st = kernel_profile(step, torch)
     if args.stages:
-        print(json.dumps({"stages_ms": st, "stats": stats}), file=sys.stderr)
+        ks = {"%s@%d" % k: [round(v[0] / v[1] * 1e3, 2), v[1], v[2] // v[1]]
+              for k, v in sorted(agg.items(), key=lambda kv: -kv[1][0])}
+        print(json.dumps({"stages_ms": st, "stats": stats, "kernels_us_n_units": ks}),
+              file=sys.stderr)
 
     # accuracy of this very output against the FP64 direct sum without
     # cutoff on 10k targets (the GPU all-pairs kernel, itself parity-tested
@@ -146,7 +246,14 @@ def main():
         "rel_l2_err_vs_direct": err,
         "direct_oracle_check": direct_check,
         "stages_ms": {k: round(v, 4) for k, v in st.items()},
+        "roofline": roofline(agg, stats, 1),
     }
+    if args.copy_calib:
+        a = torch.empty(32 << 20, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(src, chg, tg)
     print(json.dumps(rec))

@@ -57,6 +57,10 @@ def load():
     lib.fgtpw_stage_times.argtypes = [_C.POINTER(_C.c_double), i]
     lib.fgtpw_stage_name.restype = _C.c_char_p
     lib.fgtpw_stage_name.argtypes = [i]
+    lib.fgtpw_kernel_launches.argtypes = [ip, ip, _C.POINTER(_C.c_double),
+                                          _C.POINTER(_C.c_longlong), i]
+    lib.fgtpw_kernel_name.restype = _C.c_char_p
+    lib.fgtpw_kernel_name.argtypes = [i]
     lib.fgtpw_last_tree2d.argtypes = [ip, i] + [dp] * 12
     lib.fgtpw_tree2d_from_leaves.argtypes = [i, i, d, dp, i, dp, dp, ip, i] + [dp] * 8
     for f in ("fgtpw_init", "fgtpw_fgt2d", "fgtpw_fgt2d_device", "fgtpw_fgt2dpart_direct",
@@ -179,6 +183,21 @@ def stage_times():
     buf = (_C.c_double * 32)()
     k = lib.fgtpw_stage_times(buf, 32)
     return {lib.fgtpw_stage_name(i).decode(): buf[i] for i in range(k)}
+
+
+def kernel_launches():
+    """Instrumented kernel launches of the last transform made with timing
+    on: list of dicts (kernel, level, ms, units), in launch order."""
+    lib = load()
+    n = lib.fgtpw_kernel_launches(None, None, None, None, 0)
+    ids, lev = _np.zeros(max(n, 1), _np.int32), _np.zeros(max(n, 1), _np.int32)
+    ms, un = _np.zeros(max(n, 1)), _np.zeros(max(n, 1), _np.int64)
+    lib.fgtpw_kernel_launches(ids.ctypes.data_as(_C.POINTER(_C.c_int)),
+                              lev.ctypes.data_as(_C.POINTER(_C.c_int)),
+                              ms.ctypes.data_as(_C.POINTER(_C.c_double)),
+                              un.ctypes.data_as(_C.POINTER(_C.c_longlong)), n)
+    return [dict(kernel=lib.fgtpw_kernel_name(int(ids[k])).decode(), level=int(lev[k]),
+                 ms=float(ms[k]), units=int(un[k])) for k in range(n)]
 
 
 def _tree_arrays(nb, nlevels):

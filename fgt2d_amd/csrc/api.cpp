@@ -33,11 +33,14 @@ bool g_timing = false;
 std::mutex g_mutex;  // one transform at a time per process (shared workspace)
 int g_device = -1;
 
-const char* const kStages[] = {"bbox", "keys_sort", "leafdepth", "host_tree", "assign", "form",
-                               "merge", "m2l", "split", "eval", "direct", "scatter"};
-enum Stage { S_BBOX, S_KEYS, S_LEAF, S_HOST, S_ASSIGN, S_FORM, S_MERGE, S_M2L, S_SPLIT, S_EVAL,
-             S_DIRECT, S_SCATTER, S_N };
+const char* const kStages[] = {"bbox",  "keys_sort", "leafdepth", "host_tree", "assign",
+                               "ranges", "form",     "merge",     "m2l",       "split",
+                               "eval",  "direct",    "scatter"};
+enum Stage { S_BBOX, S_KEYS, S_LEAF, S_HOST, S_ASSIGN, S_RANGES, S_FORM, S_MERGE, S_M2L, S_SPLIT,
+             S_EVAL, S_DIRECT, S_SCATTER, S_N };
 }  // namespace
+
+thread_local KernelClock* t_kclock = nullptr;
 
 void fail(int code, const std::string& msg) { throw Error{code, msg}; }
 
@@ -71,6 +74,7 @@ struct Workspace {
     size_t pin_cap = 0;
     hipStream_t own = nullptr;
     hipEvent_t ev[2 * S_N] = {};
+    std::vector<hipEvent_t> kev;  // per-kernel timing events, grown on demand
     Tables2D tab;
     // the tree of the last transform, kept for inspection
     Tree2D last;
@@ -118,13 +122,77 @@ Workspace& workspace()
     return *w;
 }
 
+const char* const kKernels[KID_N] = {"form_moments", "form_h2pw", "form_valu", "merge",
+                                     "m2l",          "split",     "pw2local",  "taylor_eval",
+                                     "pw_eval",      "direct",    "gather",    "scatter"};
+struct KLaunch {
+    int id, level;
+    double ms;
+    long long units;
+};
+thread_local std::vector<KLaunch> t_klaunch;
+
+// Events around each instrumented kernel launch of one call.  The unit count
+// of a launch (points, boxes) is resolved after the call from the point
+// ranges of its box list: side 0 sources, 1 targets, 2 the list length,
+// -1 a count given directly.
+struct LaunchClock : KernelClock {
+    struct Rec {
+        int id, level, side;
+        const std::vector<int>* boxes;
+        long long units;
+        hipEvent_t e0, e1;
+    };
+    std::vector<hipEvent_t>& pool;
+    std::vector<Rec> recs;
+    int level = -1, side = -1;
+    const std::vector<int>* boxes = nullptr;
+    long long units = 0;
+    explicit LaunchClock(std::vector<hipEvent_t>& p) : pool(p) {}
+    void ctx(int l, const std::vector<int>* b, int sd, long long u = 0)
+    {
+        level = l, boxes = b, side = sd, units = u;
+    }
+    hipEvent_t ev(size_t i)
+    {
+        while (pool.size() <= i) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[i];
+    }
+    void mark(int id, bool end, hipStream_t st) override
+    {
+        if (!end) {
+            Rec r{id, level, side, boxes, units, ev(2 * recs.size()), ev(2 * recs.size() + 1)};
+            HIPCHECK(hipEventRecord(r.e0, st));
+            recs.push_back(r);
+        } else {
+            HIPCHECK(hipEventRecord(recs.back().e1, st));
+        }
+    }
+};
+
 struct Timer {
     Workspace& W;
     hipStream_t s;
     bool on;
     double host_ms = 0;
     bool used[S_N] = {};
-    Timer(Workspace& w, hipStream_t st) : W(w), s(st), on(g_timing) {}
+    LaunchClock clock;
+    const int2 *src_se = nullptr, *trg_se = nullptr;
+    int nb = 0;
+    Timer(Workspace& w, hipStream_t st) : W(w), s(st), on(g_timing), clock(w.kev)
+    {
+        t_klaunch.clear();
+        if (on) t_kclock = &clock;
+    }
+    ~Timer() { t_kclock = nullptr; }
+    void ctx(int l, const std::vector<int>* b, int sd, long long u = 0)
+    {
+        if (on) clock.ctx(l, b, sd, u);
+    }
     void begin(int k)
     {
         if (on) { HIPCHECK(hipEventRecord(W.ev[2 * k], s)); used[k] = true; }
@@ -145,6 +213,25 @@ struct Timer {
             t_stage_ms[k] = ms;
         }
         t_stage_ms[S_HOST] = host_ms;
+        std::vector<int2> hs, ht;
+        if (nb > 0) {
+            hs.resize(nb);
+            ht.resize(nb);
+            HIPCHECK(hipMemcpy(hs.data(), src_se, sizeof(int2) * nb, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(ht.data(), trg_se, sizeof(int2) * nb, hipMemcpyDeviceToHost));
+        }
+        for (const auto& r : clock.recs) {
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, r.e0, r.e1));
+            long long u = r.units;
+            if (r.boxes && r.side == 2) u = (long long)r.boxes->size();
+            if (r.boxes && (r.side == 0 || r.side == 1) && nb > 0) {
+                const auto& se = r.side == 0 ? hs : ht;
+                u = 0;
+                for (int b : *r.boxes) u += se[b].y - se[b].x;
+            }
+            t_klaunch.push_back(KLaunch{r.id, r.level, ms, u});
+        }
     }
 };
 
@@ -348,15 +435,19 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     tm.begin(S_KEYS);
     const int nmx = std::max(ns, nt);
     using u64 = unsigned long long;
+    // Points are kept in full-depth Morton order: every leaf of the final
+    // tree is a Morton interval, so this order groups them by leaf (leaves in
+    // depth-first order).  Equal keys keep their input order (stable sort).
     u64* ks = W.get<u64>("keys_s", ns);
     u64* ks2 = W.get<u64>("keys_s2", ns);
     unsigned* ix = W.get<unsigned>("idx_s", nmx);
-    unsigned* ix2 = W.get<unsigned>("idx_s2", ns);
+    unsigned* isrc = W.get<unsigned>("isrc", ns);
     size_t tmpb = std::max(sort_pairs_u64_temp(nmx), sort_pairs_u32_temp(nmx));
     tmpb = std::max(tmpb, select_flagged_temp(ns));
     void* tmp = W.get<char>("sort_tmp", tmpb);
+    const int keybits = std::max(2, 2 * nlmax);
     launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, ks, ix, s);
-    sort_pairs_u64(tmp, tmpb, ks, ks2, ix, ix2, ns, std::max(2, 2 * nlmax), s);
+    sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
     tm.end(S_KEYS);
     tm.begin(S_LEAF);
     int* lvl = W.get<int>("leaf_level", ns);
@@ -377,6 +468,29 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         copy_to_host(hp, runs, sizeof(LeafRun) * (size_t)nleaf0, s);
         std::memcpy(leaves.data(), hp, sizeof(LeafRun) * (size_t)nleaf0);
     }
+
+    // Work that needs no tree, queued now so that the GPU runs it while the
+    // host builds the tree: targets to Morton order, points and charges
+    // gathered into that order.
+    tm.begin(S_ASSIGN);
+    double* xs = W.get<double>("xs", 2 * (size_t)ns);
+    double* qs = W.get<double>("qs", (size_t)nd * ns);
+    tm.ctx(-1, nullptr, -1, ns);
+    launch_gather(q.src, q.chg, nd, isrc, ns, xs, qs, s);
+    unsigned* itarg = nullptr;
+    u64* kt2 = nullptr;
+    double* xt = nullptr;
+    if (nt > 0) {
+        itarg = W.get<unsigned>("itarg", nt);
+        xt = W.get<double>("xt", 2 * (size_t)nt);
+        u64* kt = W.get<u64>("keys_t", nt);
+        kt2 = W.get<u64>("keys_t2", nt);
+        launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, kt, ix, s);
+        sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
+        tm.ctx(-1, nullptr, -1, nt);
+        launch_gather(q.targ, nullptr, 0, itarg, nt, xt, nullptr, s);
+    }
+    tm.end(S_ASSIGN);
 
     // ---------------- host: tree, lists, tables ----------------
     auto th0 = std::chrono::steady_clock::now();
@@ -530,40 +644,26 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     const int* d_rhi = (const int*)D(o_rhi);
     const int* d_slot = (const int*)D(o_slot);
 
-    // ---------------- points to leaves, tree order ----------------
-    tm.begin(S_ASSIGN);
+    // ---------------- point ranges of the boxes ----------------
+    // leaf rank of every point (non-decreasing in Morton order), first point
+    // of every leaf, then the range of every box
+    tm.begin(S_RANGES);
     unsigned* rk = W.get<unsigned>("rank", nmx);
-    unsigned* rk2 = W.get<unsigned>("rank2", nmx);
-    unsigned* isrc = W.get<unsigned>("isrc", ns);
     int* ls = W.get<int>("leafstart", nleaves + 1);
     int2* src_se = W.get<int2>("src_se", nb);
     int2* trg_se = W.get<int2>("trg_se", nb);
-    const int rbits = std::max(1, (int)std::ceil(std::log2((double)nleaves + 1)));
-    double* xs = W.get<double>("xs", 2 * (size_t)ns);
-    double* qs = W.get<double>("qs", (size_t)nd * ns);
-    launch_descend_rank(ks2, ix2, ns, nlmax, d_child, d_rank, rk, s);
-    launch_iota(ix, ns, s);
-    sort_pairs_u32(tmp, tmpb, rk, rk2, ix, isrc, ns, rbits, s);
-    launch_leafstart(rk2, ns, nleaves, ls, s);
+    launch_descend_rank(ks2, nullptr, ns, nlmax, d_child, d_rank, rk, s);
+    launch_leafstart(rk, ns, nleaves, ls, s);
     launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, s);
-    launch_gather(q.src, q.chg, nd, isrc, ns, xs, qs, s);
-    unsigned* itarg = nullptr;
-    double* xt = nullptr;
+    tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
     if (nt > 0) {
-        itarg = W.get<unsigned>("itarg", nt);
-        xt = W.get<double>("xt", 2 * (size_t)nt);
-        u64* kt = W.get<u64>("keys_t", nt);
-        launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, kt, nullptr, s);
-        launch_descend_rank(kt, nullptr, nt, nlmax, d_child, d_rank, rk, s);
-        launch_iota(ix, nt, s);
-        sort_pairs_u32(tmp, tmpb, rk, rk2, ix, itarg, nt, rbits, s);
-        launch_leafstart(rk2, nt, nleaves, ls, s);
+        launch_descend_rank(kt2, nullptr, nt, nlmax, d_child, d_rank, rk, s);
+        launch_leafstart(rk, nt, nleaves, ls, s);
         launch_box_ranges(d_rlo, d_rhi, nb, ls, trg_se, s);
-        launch_gather(q.targ, nullptr, 0, itarg, nt, xt, nullptr, s);
     } else {
         HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));
     }
-    tm.end(S_ASSIGN);
+    tm.end(S_RANGES);
 
     // sorted outputs
     const int ws_ = use_src ? ns : 0, wt_ = use_trg ? nt : 0;
@@ -646,23 +746,25 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                 FormArgs fa{(const int*)D(o_form[l]), nb_l, d_cen, src_se, d_slot, xs, qs, nd, ind,
                             dsq, nlocal[l], d_h2x, d_h2xr, d_rsq, npw, mp, ts_form[l], segf(l),
                             (int)std::min<long>(GRIDMAX, nb_l + ns / segf(l) + 1), partial, m * m};
+                tm.ctx(l, &form_l[l], 0);
                 launch_form(fa, s);
             }
             tm.end(S_FORM);
         }
         tm.begin(S_MERGE);
         for (int l = nlevels - 1; l >= nlevstart; --l)
-            launch_merge((const int*)D(o_merge[l]), (int)merge_l[l].size(), d_child, d_cen, src_se,
+            tm.ctx(l, &merge_l[l], 2), launch_merge((const int*)D(o_merge[l]), (int)merge_l[l].size(), d_child, d_cen, src_se,
                          d_slot, d_wms, 4 * (nlevels - l - 1), nd, nexp, mp, s);
         tm.end(S_MERGE);
         tm.begin(S_M2L);
+        tm.ctx(nlevstart, &m2l_l, 2);
         launch_m2l((const int*)D(o_m2l), (int)m2l_l.size(), d_cen, d_ncol, d_col, src_se, trg_se,
                    use_src, use_trg, d_slot, d_wm2l, npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0],
                    nlevstart == npwlevel, nd, nexp, mp, lc, s);
         tm.end(S_M2L);
         tm.begin(S_SPLIT);
         for (int l = nlevstart; l <= nlevels - 1; ++l)
-            launch_split((const int*)D(o_split[l]), (int)split_l[l].size(), d_par, d_cen, d_slot,
+            tm.ctx(l, &split_l[l], 2), launch_split((const int*)D(o_split[l]), (int)split_l[l].size(), d_par, d_cen, d_slot,
                          d_wms, 4 * (nlevels - l - 1), nd, nexp, lc, s);
         tm.end(S_SPLIT);
         tm.begin(S_EVAL);
@@ -684,6 +786,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                 ea.loc = lc;
                 ea.taylor = taylor;
                 bool pwroute = nlocal[l] > 50;  // nlocalswitch, fgt2d.f:855
+                tm.ctx(l, &eval_l[l], 2);
                 if (!pwroute) launch_pw2local(ea, s);
                 for (int side = 0; side < 2; ++side) {
                     if (side == 0 && !use_trg) continue;
@@ -698,6 +801,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                     eb.pot = side == 0 ? pt : ps;
                     eb.grad = side == 0 ? gt : gs;
                     eb.hess = side == 0 ? ht : hs;
+                    tm.ctx(l, &eval_l[l], side == 0 ? 1 : 0);
                     if (pwroute)
                         launch_pw_eval(eb, s);
                     else
@@ -728,6 +832,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             d2.xt = xt;
             d2.ifpgh = q.ifpghtarg;
             d2.pot = pt; d2.grad = gt; d2.hess = ht;
+            tm.ctx(npwlevel, nullptr, -1, nt);
             launch_direct(d2, s);
         }
         if (use_src) {
@@ -736,6 +841,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             d2.xt = xs;
             d2.ifpgh = q.ifpgh;
             d2.pot = ps; d2.grad = gs; d2.hess = hs;
+            tm.ctx(npwlevel, nullptr, -1, ns);
             launch_direct(d2, s);
         }
     }
@@ -744,11 +850,13 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     // ---------------- back to input order ----------------
     tm.begin(S_SCATTER);
     if (use_src) {
+        tm.ctx(-1, nullptr, -1, ns);
         launch_scatter(ps, isrc, ns, nd, q.pot, s);
         if (q.ifpgh >= 2) launch_scatter(gs, isrc, ns, 2 * nd, q.grad, s);
         if (q.ifpgh >= 3) launch_scatter(hs, isrc, ns, 3 * nd, q.hess, s);
     }
     if (use_trg) {
+        tm.ctx(-1, nullptr, -1, nt);
         launch_scatter(pt, itarg, nt, nd, q.pottarg, s);
         if (q.ifpghtarg >= 2) launch_scatter(gt, itarg, nt, 2 * nd, q.gradtarg, s);
         if (q.ifpghtarg >= 3) launch_scatter(ht, itarg, nt, 3 * nd, q.hesstarg, s);
@@ -1060,6 +1168,15 @@ int fgtpw_last_tree2d(int* info, int cap, int* laddr, int* lev, int* par, int* n
                                     hipMemcpyDeviceToHost, s));
         }
         HIPCHECK(hipStreamSynchronize(s));
+        // The transform keeps each leaf's points in Morton order; the
+        // reference's stable partitions (sort_pts_to_children,
+        // pts_tree2d4.f:594-689) keep them in input order.  Same sets per
+        // leaf, so sorting each leaf's indices gives the reference order.
+        for (int b = 0; b < nb; ++b) {
+            if (T.nch[b] != 0) continue;
+            std::sort(isrc + isrcse[2 * b], isrc + isrcse[2 * b + 1]);
+            if (W.last_nt > 0) std::sort(itarg + itargse[2 * b], itarg + itargse[2 * b + 1]);
+        }
     });
 }
 
@@ -1109,6 +1226,24 @@ int fgtpw_stage_times(double* ms, int n)
     int k = std::min(n, (int)t_stage_ms.size());
     for (int i = 0; i < k; ++i) ms[i] = t_stage_ms[i];
     return k;
+}
+
+int fgtpw_kernel_launches(int* id, int* level, double* ms, long long* units, int cap)
+{
+    int n = (int)t_klaunch.size();
+    for (int i = 0; i < std::min(n, cap); ++i) {
+        id[i] = t_klaunch[i].id;
+        level[i] = t_klaunch[i].level;
+        ms[i] = t_klaunch[i].ms;
+        units[i] = t_klaunch[i].units;
+    }
+    return n;
+}
+
+const char* fgtpw_kernel_name(int i)
+{
+    if (i < 0 || i >= KID_N) return "";
+    return kKernels[i];
 }
 
 const char* fgtpw_stage_name(int i)

@@ -23,6 +23,24 @@ void check_hip(hipError_t e, const char* what);
 #define HIPCHECK(x) ::fgtpw::check_hip((x), #x)
 
 // ---------------------------------------------------------------------------
+// per-kernel timing: with a clock installed on the calling thread, the
+// launchers bracket the kernels named here with HIP events on their stream
+// ---------------------------------------------------------------------------
+enum KernelId {
+    KID_FORM_MOMENTS, KID_FORM_H2PW, KID_FORM_VALU, KID_MERGE, KID_M2L, KID_SPLIT, KID_PW2LOCAL,
+    KID_TAYLOR_EVAL, KID_PW_EVAL, KID_DIRECT, KID_GATHER, KID_SCATTER, KID_N
+};
+struct KernelClock {
+    virtual void mark(int id, bool end, hipStream_t s) = 0;
+    virtual ~KernelClock() = default;
+};
+extern thread_local KernelClock* t_kclock;
+inline void kmark(int id, bool end, hipStream_t s)
+{
+    if (t_kclock) t_kclock->mark(id, end, s);
+}
+
+// ---------------------------------------------------------------------------
 // term counts (g2dterms.f) and plane-wave nodes (g2drouts.f:2205)
 // ---------------------------------------------------------------------------
 int hermite_terms(int ndim, double bsize, double delta, double eps);
@@ -52,6 +70,9 @@ struct Tree2D {
     int nboxes() const { return (int)lev.size(); }
     // leaves in Morton order and, per box, its interval of leaf ranks
     std::vector<int> leaf_of_rank, rank_of_box, rank_lo, rank_hi;
+    // builder scratch, kept to reuse its storage
+    std::vector<int> map;
+    std::vector<char> scratch;
 };
 
 void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int npwlevel,

@@ -366,7 +366,9 @@ void launch_gather(const double* xy, const double* q, int nd, const unsigned* pe
                    double* xy_s, double* q_s, hipStream_t s)
 {
     if (n <= 0) return;
+    kmark(KID_GATHER, false, s);
     k_gather<<<grid_for(n, 256), 256, 0, s>>>(xy, q, nd, perm, n, xy_s, q_s);
+    kmark(KID_GATHER, true, s);
 }
 
 __global__ __launch_bounds__(256) void k_zero(double* __restrict__ p, size_t n)
@@ -398,7 +400,9 @@ void launch_scatter(const double* v_s, const unsigned* perm, int n, int w, doubl
                     hipStream_t s)
 {
     if (n <= 0) return;
+    kmark(KID_SCATTER, false, s);
     k_scatter<<<grid_for((long)n * w, 256), 256, 0, s>>>(v_s, perm, n, w, v);
+    kmark(KID_SCATTER, true, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -740,15 +744,19 @@ void launch_form(const FormArgs& a, hipStream_t s)
     if (m <= 64 && a.tstart) {
         const int TM = (m + 15) / 16, MP = 16 * TM;
         size_t lds = std::max((size_t)2 * a.seg * (MP + 1), (size_t)MP * MP) * sizeof(double);
+        kmark(KID_FORM_MOMENTS, false, s);
         switch (TM) {
         case 1: allow_lds(k_form_moments<1>, lds); k_form_moments<1><<<a.grid, 256, lds, s>>>(a); break;
         case 2: allow_lds(k_form_moments<2>, lds); k_form_moments<2><<<a.grid, 256, lds, s>>>(a); break;
         case 3: allow_lds(k_form_moments<3>, lds); k_form_moments<3><<<a.grid, 256, lds, s>>>(a); break;
         default: allow_lds(k_form_moments<4>, lds); k_form_moments<4><<<a.grid, 256, lds, s>>>(a); break;
         }
+        kmark(KID_FORM_MOMENTS, true, s);
         size_t lds2 = ((size_t)m * m + 2 * (size_t)m * a.npw + (size_t)a.npw * m) * sizeof(double);
         allow_lds(k_form_h2pw, lds2);
-        k_form_h2pw<<<a.nbox, 256, lds2, s>>>(a);
+        kmark(KID_FORM_H2PW, false, s);
+    k_form_h2pw<<<a.nbox, 256, lds2, s>>>(a);
+    kmark(KID_FORM_H2PW, true, s);
         return;
     }
     int chunk = 4096 / m;
@@ -758,6 +766,7 @@ void launch_form(const FormArgs& a, hipStream_t s)
     size_t lds2 = ((size_t)m * m + (size_t)a.npw * m) * sizeof(double);
     if (lds2 > lds) lds = lds2;
     int ne = (m * m + 255) / 256;
+    kmark(KID_FORM_VALU, false, s);
     if (ne <= 1)
         k_form<1><<<a.nbox, 256, lds, s>>>(a, chunk);
     else if (ne <= 2)
@@ -770,6 +779,7 @@ void launch_form(const FormArgs& a, hipStream_t s)
         k_form<16><<<a.nbox, 256, lds, s>>>(a, chunk);
     else
         fail(-3, "Hermite order too large for the formation kernel");
+    kmark(KID_FORM_VALU, true, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -821,8 +831,10 @@ void launch_merge(const int* parents, int np, const int4* child, const double* c
                   int nexp, double* mp, hipStream_t s)
 {
     if (np <= 0) return;
+    kmark(KID_MERGE, false, s);
     k_merge<<<grid_for((long)np * nexp, 256), 256, 0, s>>>(parents, np, child, cen, src_se, slot,
                                                            wms, kbase, nd, nexp, mp);
+    kmark(KID_MERGE, true, s);
 }
 
 // Step 3 (fgt2d.f:1026-1064): loc = sum over same-level neighbours j != self
@@ -877,9 +889,11 @@ void launch_m2l(const int* boxes, int nb, const double* cen, const int* ncol, co
                 const double* mp, double* loc, hipStream_t s)
 {
     if (nb <= 0) return;
+    kmark(KID_M2L, false, s);
     k_m2l<<<grid_for((long)nb * nexp, 256), 256, 0, s>>>(boxes, nb, cen, ncol, col, src_se,
                                                          trg_se, use_src, use_trg, slot, wm2l,
                                                          xmin, do_list, nd, nexp, mp, loc);
+    kmark(KID_M2L, true, s);
 }
 
 // Step 4 (fgt2d.f:1082-1125): loc(child) = loc(parent) shifted to the child.
@@ -909,8 +923,10 @@ void launch_split(const int* children, int nc, const int* par, const double* cen
                   hipStream_t s)
 {
     if (nc <= 0) return;
+    kmark(KID_SPLIT, false, s);
     k_split<<<grid_for((long)nc * nexp, 256), 256, 0, s>>>(children, nc, par, cen, slot, wms,
                                                            kbase, nd, nexp, loc);
+    kmark(KID_SPLIT, true, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -975,7 +991,9 @@ void launch_pw2local(const EvalArgs& a, hipStream_t s)
     if (a.nbox <= 0) return;
     int half = a.npw / 2, m = a.nl + 1;
     size_t lds = (size_t)(2 * half * a.npw + 2 * m * a.npw) * sizeof(double);
+    kmark(KID_PW2LOCAL, false, s);
     k_pw2local<<<a.nbox, 256, lds, s>>>(a);
+    kmark(KID_PW2LOCAL, true, s);
 }
 
 // Step 5b: Taylor evaluation (g2dlevalp/g/h_vec, g2drouts.f:847-1080) by
@@ -1190,6 +1208,7 @@ void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
     int m = a.nl + 1;
+    kmark(KID_TAYLOR_EVAL, false, s);
     if (m <= 64) {
         switch ((m + 15) / 16) {
         case 1: taylor_mfma_launch<1>(a, s); break;
@@ -1197,6 +1216,7 @@ void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
         case 3: taylor_mfma_launch<3>(a, s); break;
         default: taylor_mfma_launch<4>(a, s); break;
         }
+        kmark(KID_TAYLOR_EVAL, true, s);
         return;
     }
     size_t lds = (size_t)m * m * sizeof(double);
@@ -1207,6 +1227,7 @@ void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
         k_taylor_eval<2><<<grid, 256, lds, s>>>(a);
     else
         k_taylor_eval<1><<<grid, 256, lds, s>>>(a);
+    kmark(KID_TAYLOR_EVAL, true, s);
 }
 
 // Step 5, plane-wave route (g2dpwevalp/g/h_vec, g2drouts.f:1887-2200):
@@ -1315,12 +1336,14 @@ void launch_pw_eval(const EvalArgs& a, hipStream_t s)
     if (a.npw / 2 > PW_MAXH) fail(-3, "plane-wave length too large for the evaluation kernel");
     size_t lds = (size_t)a.npw * a.npw * sizeof(double);
     const int grid = a.tstart ? a.grid : a.nbox;
+    kmark(KID_PW_EVAL, false, s);
     if (a.ifpgh >= 3)
         k_pw_eval<3><<<grid, 256, lds, s>>>(a);
     else if (a.ifpgh == 2)
         k_pw_eval<2><<<grid, 256, lds, s>>>(a);
     else
         k_pw_eval<1><<<grid, 256, lds, s>>>(a);
+    kmark(KID_PW_EVAL, true, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1395,12 +1418,14 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
 void launch_direct(const DirectArgs& a, hipStream_t s)
 {
     if (a.nunits <= 0) return;
+    kmark(KID_DIRECT, false, s);
     if (a.ifpgh >= 3)
         k_direct<3><<<a.nunits, 256, 0, s>>>(a);
     else if (a.ifpgh == 2)
         k_direct<2><<<a.nunits, 256, 0, s>>>(a);
     else
         k_direct<1><<<a.nunits, 256, 0, s>>>(a);
+    kmark(KID_DIRECT, true, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -187,12 +187,12 @@ struct Builder {
 
     // Renumber so that each level lists its original boxes, then the boxes
     // appended to it (pts_tree_reorg, pts_tree2d4.f:1001-1150).
-    std::vector<char> scratch;
     template <class V>
     void permute(V& v, int width, const std::vector<int>& map)
     {
         using E = typename V::value_type;
-        scratch.resize(v.size() * sizeof(E));
+        std::vector<char>& scratch = T.scratch;
+        if (scratch.size() < v.size() * sizeof(E)) scratch.resize(v.size() * sizeof(E));
         E* old = (E*)scratch.data();
         std::memcpy(old, v.data(), v.size() * sizeof(E));
         for (size_t b = 0; b < map.size(); ++b)
@@ -201,7 +201,8 @@ struct Builder {
     void reorg(const std::vector<int>& tail)
     {
         const int n = T.nboxes();
-        std::vector<int> map(n);
+        std::vector<int>& map = T.map;
+        map.resize(n);
         for (int l = 0; l <= 1 && l <= T.nlevels; ++l)
             for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) map[b] = b;
         int cur = T.laddr[4];
@@ -320,7 +321,14 @@ void complete_colleagues(Tree2D& T)
 void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int npwlevel,
                   double bs0, const double* cen0)
 {
-    T = Tree2D();
+    // storage is reused from call to call: clear, keep the capacity
+    for (auto* v : {&T.laddr, &T.lev, &T.par, &T.nch, &T.ch, &T.ncol, &T.col, &T.flag, &T.ix,
+                    &T.iy, &T.leaf_of_rank, &T.rank_of_box, &T.rank_lo, &T.rank_hi})
+        v->clear();
+    T.bsz.clear();
+    T.cen.clear();
+    T.pre.clear();
+    T.coll_levels = -1;
     T.nlmax = nlmax;
     T.npwlevel = npwlevel;
     T.bs0 = bs0;
@@ -340,7 +348,7 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
     B.add_box(0, -1, cen0[0], cen0[1], 0, 0, 0);
     T.laddr = {0, 0};
     // frontier: boxes of the current level with their range of leaf records
-    std::vector<int> fbox{0}, fa{0}, fe{(int)leaves.size()};
+    std::vector<int> fbox{0}, fa{0}, fe{(int)leaves.size()}, nb, na, ne;
     int l = 0;
     for (;;) {
         bool any = false;
@@ -349,7 +357,9 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
         if (!any) break;
         T.bsz.push_back(T.bsz[l] / 2);
         int first = T.nboxes();
-        std::vector<int> nb, na, ne;
+        nb.clear();
+        na.clear();
+        ne.clear();
         for (size_t i = 0; i < fbox.size(); ++i) {
             int a = fa[i], e = fe[i];
             if (!(a < e && leaves[a].level > l)) continue;
@@ -391,13 +401,13 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
     T.coll_levels = valid;
 
     // leaves in Morton order: depth-first, children in digit order
-    int nb = T.nboxes();
-    T.leaf_of_rank.clear();
-    T.rank_of_box.assign(nb, -1);
-    T.rank_lo.assign(nb, 0);
-    T.rank_hi.assign(nb, 0);
+    int nbox = T.nboxes();
+    T.rank_of_box.assign(nbox, -1);
+    T.rank_lo.assign(nbox, 0);
+    T.rank_hi.assign(nbox, 0);
     {
-        std::vector<int> stack{0};
+        std::vector<int>& stack = T.map;
+        stack.assign(1, 0);
         while (!stack.empty()) {
             int b = stack.back();
             stack.pop_back();

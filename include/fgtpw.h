@@ -89,6 +89,19 @@ int fgtpw_stage_times(double *ms, int n);
 const char *fgtpw_stage_name(int i);
 
 /*
+ * Per-kernel timing of the last call made with timing on: one entry per
+ * launch of an instrumented kernel, in launch order -- kernel id (name from
+ * fgtpw_kernel_name), tree level (-1 when not per level), milliseconds
+ * between HIP events recorded on the launch's stream right before and after
+ * it, and the units it processed (formation, evaluation: points of its
+ * boxes; translations: boxes; near field, gather, scatter: points).  Fills
+ * up to cap entries and returns the number of launches.  No reference
+ * counterpart (the reference prints step times, fgt2d.f:1338).
+ */
+int fgtpw_kernel_launches(int *id, int *level, double *ms, long long *units, int cap);
+const char *fgtpw_kernel_name(int id);
+
+/*
  * Inspection of the quadtree of the last successful fgtpw_fgt2d* call on this
  * process (the tree fgt2d builds before its main routine, fgt2d.f:191-221),
  * 0-based: info[0] nlevels, info[1] nboxes, info[2] leaves, info[3] nlmax.
