@@ -75,6 +75,9 @@ struct Workspace {
     hipStream_t own = nullptr;
     hipEvent_t ev[2 * S_N] = {};
     std::vector<hipEvent_t> kev;  // per-kernel timing events, grown on demand
+    // host scratch of the plan, reused from call to call
+    std::vector<int> h_units, h_uptr, h_ulist, h_nl1, h_l1, h_sp, h_sl, h_fill, h_slot;
+    std::vector<char> h_plan;
     Tables2D tab;
     // the tree of the last transform, kept for inspection
     Tree2D last;
@@ -259,7 +262,10 @@ void copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t s)
 // packed host->device upload of the per-call plan
 // ---------------------------------------------------------------------------
 struct Pack {
-    std::vector<char> bytes;
+    std::vector<char> own;
+    std::vector<char>& bytes;
+    Pack() : bytes(own) {}
+    explicit Pack(std::vector<char>& storage) : bytes(storage) { bytes.clear(); }
     template <class T>
     size_t add(const T* p, size_t n)
     {
@@ -522,15 +528,14 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     const double dsq = 1.0 / std::sqrt(q.delta);
 
     // box arrays
-    std::vector<int> child(4 * (size_t)nb), slot(nb, -1);
-    std::vector<double> cen(T.cen);
+    std::vector<int>& slot = W.h_slot;
+    slot.assign(nb, -1);
     int nslot = 0;
-    for (int b = 0; b < nb; ++b) {
-        for (int j = 0; j < 4; ++j) child[4 * b + j] = T.ch[4 * b + j];
-        if (expansions && T.lev[b] >= nlevstart) slot[b] = nslot++;
-    }
-    Pack pk;
-    size_t o_child = pk.add(child), o_par = pk.add(T.par), o_cen = pk.add(cen);
+    if (expansions)
+        for (int b = 0; b < nb; ++b)
+            if (T.lev[b] >= nlevstart) slot[b] = nslot++;
+    Pack pk(W.h_plan);
+    size_t o_child = pk.add(T.ch), o_par = pk.add(T.par), o_cen = pk.add(T.cen);
     size_t o_ncol = pk.add(T.ncol), o_col = pk.add(T.col), o_rank = pk.add(T.rank_of_box);
     size_t o_rlo = pk.add(T.rank_lo), o_rhi = pk.add(T.rank_hi), o_slot = pk.add(slot);
 
@@ -579,9 +584,17 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     // inverted to target units; a unit is a leaf at or above the cutoff level
     // or a box of the cutoff level, and owns every target below it.
     const int nlevend = std::min(npwlevel, nlevels);
-    std::vector<int> unit_ids, unit_ptr{0}, unit_list;
+    std::vector<int>& unit_ids = W.h_units;
+    std::vector<int>& unit_ptr = W.h_uptr;
+    std::vector<int>& unit_list = W.h_ulist;
+    unit_ids.clear();
+    unit_ptr.assign(1, 0);
+    unit_list.clear();
     if (nlevend >= 0) {
-        std::vector<int> nl1(nb, 0), l1(13 * (size_t)nb, -1);
+        std::vector<int>& nl1 = W.h_nl1;
+        std::vector<int>& l1 = W.h_l1;
+        nl1.assign(nb, 0);
+        if (l1.size() < 13 * (size_t)nb) l1.resize(13 * (size_t)nb);
         if (T.nch[0] == 0) { nl1[0] = 1; l1[0] = 0; }
         // only lists of boxes at or above the cutoff level are ever read
         for (int l = 1; l <= nlevend; ++l)
@@ -604,19 +617,31 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                     }
                 }
             }
-        // sources of every target box, in the reference's order of source boxes
-        std::vector<std::vector<int>> srcl(nb);
+        // sources of every target box, in the reference's order of source
+        // boxes (CSR: count, prefix sum, fill)
+        std::vector<int>& sp = W.h_sp;
+        std::vector<int>& sl = W.h_sl;
+        sp.assign(nb + 1, 0);
         for (int l = 0; l <= nlevend; ++l)
             for (int sb = T.laddr[2 * l]; sb <= T.laddr[2 * l + 1]; ++sb) {
                 if (T.nch[sb] != 0) continue;
-                for (int i = 0; i < nl1[sb]; ++i) srcl[l1[13 * (size_t)sb + i]].push_back(sb);
+                for (int i = 0; i < nl1[sb]; ++i) ++sp[l1[13 * (size_t)sb + i] + 1];
+            }
+        for (int b = 0; b < nb; ++b) sp[b + 1] += sp[b];
+        sl.resize(sp[nb]);
+        std::vector<int>& fill = W.h_fill;
+        fill.assign(sp.begin(), sp.end() - 1);
+        for (int l = 0; l <= nlevend; ++l)
+            for (int sb = T.laddr[2 * l]; sb <= T.laddr[2 * l + 1]; ++sb) {
+                if (T.nch[sb] != 0) continue;
+                for (int i = 0; i < nl1[sb]; ++i) sl[fill[l1[13 * (size_t)sb + i]]++] = sb;
             }
         for (int b = 0; b < nb; ++b) {
             bool unit = (T.nch[b] == 0 && T.lev[b] <= npwlevel) || T.lev[b] == npwlevel;
             if (!unit) continue;
             size_t before = unit_list.size();
             for (int a = b; a >= 0; a = T.par[a])
-                unit_list.insert(unit_list.end(), srcl[a].begin(), srcl[a].end());
+                unit_list.insert(unit_list.end(), sl.begin() + sp[a], sl.begin() + sp[a + 1]);
             if (unit_list.size() == before) continue;
             unit_ids.push_back(b);
             unit_ptr.push_back((int)unit_list.size());

@@ -71,7 +71,7 @@ struct Tree2D {
     // leaves in Morton order and, per box, its interval of leaf ranks
     std::vector<int> leaf_of_rank, rank_of_box, rank_lo, rank_hi;
     // builder scratch, kept to reuse its storage
-    std::vector<int> map;
+    std::vector<int> map, tail;
     std::vector<char> scratch;
 };
 

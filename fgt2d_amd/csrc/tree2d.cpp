@@ -92,55 +92,62 @@ namespace {
 
 struct Builder {
     Tree2D& T;
-    explicit Builder(Tree2D& t) : T(t) {}
+    // raw views of the per-box arrays, refreshed whenever they grow
+    int *lev, *par, *nch, *ch, *ncol, *col, *flag, *ix, *iy;
+    double* cen;
+    uint64_t* pre;
+    explicit Builder(Tree2D& t) : T(t) { view(); }
 
-    int add_box(int level, int parent, double cx, double cy, uint64_t pre, int ix, int iy)
+    void view()
     {
-        int id = T.nboxes();
-        T.ix.push_back(ix);
-        T.iy.push_back(iy);
-        T.lev.push_back(level);
-        T.par.push_back(parent);
-        T.nch.push_back(0);
-        for (int j = 0; j < 4; ++j) T.ch.push_back(-1);
-        T.ncol.push_back(0);
-        for (int j = 0; j < 9; ++j) T.col.push_back(-1);
-        T.flag.push_back(0);
-        T.cen.push_back(cx);
-        T.cen.push_back(cy);
-        T.pre.push_back(pre);
-        return id;
+        lev = T.lev.data(), par = T.par.data(), nch = T.nch.data(), ch = T.ch.data();
+        ncol = T.ncol.data(), col = T.col.data(), flag = T.flag.data(), ix = T.ix.data();
+        iy = T.iy.data(), cen = T.cen.data(), pre = T.pre.data();
+    }
+    // append k boxes with no children, no colleagues and flag 0
+    int grow(int k)
+    {
+        const size_t n = T.lev.size(), m = n + k;
+        T.lev.resize(m);
+        T.par.resize(m);
+        T.nch.resize(m, 0);
+        T.ch.resize(4 * m, -1);
+        T.ncol.resize(m, 0);
+        T.col.resize(9 * m, -1);
+        T.flag.resize(m, 0);
+        T.cen.resize(2 * m);
+        T.pre.resize(m);
+        T.ix.resize(m);
+        T.iy.resize(m);
+        view();
+        return (int)n;
     }
 
-    // Four children of box b, numbered consecutively from the current end;
-    // child j sits at c + (-1)^(j+1) bs/2 in x and at -bs/2 (j < 2) or +bs/2
-    // in y, bs the child size (tree_refine_boxes, tree_routs4.f:28-44).
-    int add_children(int b)
+    // Four children of box b at first..first+3; child j sits at
+    // c + (-1)^(j+1) bs/2 in x and at -bs/2 (j < 2) or +bs/2 in y, bs the
+    // child size (tree_refine_boxes, tree_routs4.f:28-44).
+    void set_children(int b, int first)
     {
-        int l = T.lev[b];
-        double bs = T.bsz[l + 1];
-        int first = T.nboxes();
+        const int l = lev[b];
+        const double bs = T.bsz[l + 1];
         for (int j = 0; j < 4; ++j) {
-            double hx = (j & 1) ? bs / 2 : -bs / 2;
-            double hy = (j < 2) ? -bs / 2 : bs / 2;
-            int c = add_box(l + 1, b, T.cen[2 * b] + hx, T.cen[2 * b + 1] + hy, (T.pre[b] << 2) | j,
-                            2 * T.ix[b] + (j & 1), 2 * T.iy[b] + (j >> 1));
-            T.ch[4 * b + j] = c;
+            const int c = first + j;
+            lev[c] = l + 1;
+            par[c] = b;
+            cen[2 * c] = cen[2 * b] + ((j & 1) ? bs / 2 : -bs / 2);
+            cen[2 * c + 1] = cen[2 * b + 1] + ((j < 2) ? -bs / 2 : bs / 2);
+            pre[c] = (pre[b] << 2) | (uint64_t)j;
+            ix[c] = 2 * ix[b] + (j & 1);
+            iy[c] = 2 * iy[b] + (j >> 1);
+            ch[4 * b + j] = c;
         }
-        T.nch[b] = 4;
-        return first;
+        nch[b] = 4;
     }
 
-    // same-level boxes a, b: neighbours (or equal)
-    bool near_same(int a, int b) const
-    {
-        int dx = T.ix[a] - T.ix[b], dy = T.iy[a] - T.iy[b];
-        return dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1;
-    }
     // coarse box c (level l) and fine box f (level l+1) share a point
     bool near_finer(int c, int f) const
     {
-        int dx = T.ix[f] - 2 * T.ix[c], dy = T.iy[f] - 2 * T.iy[c];
+        int dx = ix[f] - 2 * ix[c], dy = iy[f] - 2 * iy[c];
         return dx >= -1 && dx <= 2 && dy >= -1 && dy <= 2;
     }
 
@@ -149,27 +156,78 @@ struct Builder {
     // tree_routs4.f:100-211).
     void colleagues(int b)
     {
-        int l = T.lev[b];
-        T.ncol[b] = 0;
-        if (l == 0) {
-            T.ncol[b] = 1;
-            T.col[9 * b] = b;
+        if (lev[b] == 0) {
+            ncol[b] = 1;
+            col[9 * b] = b;
             return;
         }
-        int dad = T.par[b];
-        for (int i = 0; i < T.ncol[dad]; ++i) {
-            int jb = T.col[9 * dad + i];
-            for (int j = 0; j < 4; ++j) {
-                int kb = T.ch[4 * jb + j];
-                if (kb >= 0 && near_same(kb, b)) T.col[9 * b + T.ncol[b]++] = kb;
+        // A child of the parent's neighbour jb, at offset o = (jb - dad) in
+        // {-1,0,1} per axis, with bit c on that axis, is a neighbour of b
+        // (bit t) iff |2o + c - t| <= 1: any c for o = 0, c = 0 and t = 1
+        // for o = 1, c = 1 and t = 0 for o = -1.  ok[o+1][t] is the mask of
+        // the allowed c (bit 0: c = 0, bit 1: c = 1).
+        static const unsigned char ok[3][2] = {{2, 0}, {3, 3}, {0, 1}};
+        const int dad = par[b], tx = ix[b] & 1, ty = iy[b] & 1;
+        const int px = ix[dad], py = iy[dad];
+        int n = 0;
+        int* out = col + 9 * b;
+        for (int i = 0; i < ncol[dad]; ++i) {
+            const int jb = col[9 * dad + i];
+            if (nch[jb] == 0) continue;
+            const int mx = ok[ix[jb] - px + 1][tx], my = ok[iy[jb] - py + 1][ty];
+            const int* cj = ch + 4 * jb;
+            for (int j = 0; j < 4; ++j)
+                if (((mx >> (j & 1)) & 1) && ((my >> (j >> 1)) & 1)) out[n++] = cj[j];
+        }
+        for (int i = n; i < ncol[b]; ++i) out[i] = -1;
+        ncol[b] = n;
+    }
+
+    // colleagues() for the four children of dad at once: one pass over the
+    // parent's neighbours; fam[o][t] is the 4-bit set of children of a
+    // neighbour at offset o = 3 (oy+1) + (ox+1) that touch sibling t.
+    void family_colleagues(int dad)
+    {
+        static unsigned char fam[9][4];
+        static bool init = false;
+        if (!init) {
+            static const unsigned char ok[3][2] = {{2, 0}, {3, 3}, {0, 1}};
+            for (int o = 0; o < 9; ++o)
+                for (int t = 0; t < 4; ++t) {
+                    int mx = ok[o % 3][t & 1], my = ok[o / 3][t >> 1], m = 0;
+                    for (int j = 0; j < 4; ++j)
+                        if (((mx >> (j & 1)) & 1) && ((my >> (j >> 1)) & 1)) m |= 1 << j;
+                    fam[o][t] = (unsigned char)m;
+                }
+            init = true;
+        }
+        const int px = ix[dad], py = iy[dad];
+        const int* kids = ch + 4 * dad;
+        int n[4] = {0, 0, 0, 0};
+        for (int i = 0; i < ncol[dad]; ++i) {
+            const int jb = col[9 * dad + i];
+            if (nch[jb] == 0) continue;
+            const int o = 3 * (iy[jb] - py + 1) + (ix[jb] - px + 1);
+            const int* cj = ch + 4 * jb;
+            for (int t = 0; t < 4; ++t) {
+                int* out = col + 9 * kids[t];
+                for (unsigned m = fam[o][t]; m; m &= m - 1) out[n[t]++] = cj[__builtin_ctz(m)];
             }
+        }
+        for (int t = 0; t < 4; ++t) {
+            const int b = kids[t];
+            for (int i = n[t]; i < ncol[b]; ++i) col[9 * b + i] = -1;
+            ncol[b] = n[t];
         }
     }
 
     void all_colleagues(int maxlev)
     {
-        for (int l = 0; l <= std::min(T.nlevels, maxlev); ++l)
-            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) colleagues(b);
+        ncol[0] = 1;
+        col[0] = 0;
+        for (int l = 0; l < std::min(T.nlevels, maxlev); ++l)
+            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b)
+                if (nch[b] > 0) family_colleagues(b);
     }
 
     // Refine the boxes of [b0,b1] whose flag is positive; children of flag-1
@@ -177,11 +235,16 @@ struct Builder {
     // (tree_refine_boxes_flag, tree_routs4.f:334-388).
     void refine_flagged(int b0, int b1)
     {
+        int k = 0;
+        for (int b = b0; b <= b1; ++b) k += flag[b] > 0;
+        if (k == 0) return;
+        int next = grow(4 * k);
         for (int b = b0; b <= b1; ++b) {
-            if (T.flag[b] <= 0) continue;
-            int first = add_children(b);
-            int f = T.flag[b] == 1 ? 3 : 0;
-            for (int j = 0; j < 4; ++j) T.flag[first + j] = f;
+            if (flag[b] <= 0) continue;
+            set_children(b, next);
+            const int f = flag[b] == 1 ? 3 : 0;
+            for (int j = 0; j < 4; ++j) flag[next + j] = f;
+            next += 4;
         }
     }
 
@@ -195,8 +258,10 @@ struct Builder {
         if (scratch.size() < v.size() * sizeof(E)) scratch.resize(v.size() * sizeof(E));
         E* old = (E*)scratch.data();
         std::memcpy(old, v.data(), v.size() * sizeof(E));
+        E* dst = v.data();
+        const int* mp = map.data();
         for (size_t b = 0; b < map.size(); ++b)
-            for (int k = 0; k < width; ++k) v[(size_t)width * map[b] + k] = old[(size_t)width * b + k];
+            for (int k = 0; k < width; ++k) dst[(size_t)width * mp[b] + k] = old[(size_t)width * b + k];
     }
     void reorg(const std::vector<int>& tail)
     {
@@ -222,10 +287,12 @@ struct Builder {
         permute(T.iy, 1, map);
         permute(T.par, 1, map);
         permute(T.ch, 4, map);
-        for (int& p : T.par)
-            if (p >= 0) p = map[p];
-        for (int& c : T.ch)
-            if (c >= 0) c = map[c];
+        view();
+        const int* mp = map.data();
+        for (int b = 0; b < n; ++b)
+            if (par[b] >= 0) par[b] = mp[par[b]];
+        for (int i = 0; i < 4 * n; ++i)
+            if (ch[i] >= 0) ch[i] = mp[ch[i]];
     }
 
     // flag++ boxes with a touching grandchild-level box that has children are
@@ -233,14 +300,14 @@ struct Builder {
     void update_flags(int b0, int b1)
     {
         for (int b = b0; b <= b1; ++b) {
-            if (T.flag[b] != 3) continue;
-            T.flag[b] = 0;
-            for (int i = 0; i < T.ncol[b] && T.flag[b] == 0; ++i) {
-                int jb = T.col[9 * b + i];
+            if (flag[b] != 3) continue;
+            flag[b] = 0;
+            for (int i = 0; i < ncol[b] && flag[b] == 0; ++i) {
+                int jb = col[9 * b + i];
                 for (int j = 0; j < 4; ++j) {
-                    int kb = T.ch[4 * jb + j];
-                    if (kb >= 0 && T.nch[kb] > 0 && near_finer(b, kb)) {
-                        T.flag[b] = 1;
+                    int kb = ch[4 * jb + j];
+                    if (kb >= 0 && nch[kb] > 0 && near_finer(b, kb)) {
+                        flag[b] = 1;
                         break;
                     }
                 }
@@ -255,14 +322,17 @@ struct Builder {
         const int L = T.nlevels;
         std::fill(T.flag.begin(), T.flag.end(), 0);
         // leaves touching a box two levels finer, seen from its parent
+        // (the test depends on the parent only: once per refined box of
+        // levels 1..L-1)
         int nflag = 0;
-        for (int l = L; l >= 2; --l)
-            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
-                int dad = T.par[b], gd = T.par[dad];
-                for (int i = 0; i < T.ncol[gd]; ++i) {
-                    int jb = T.col[9 * gd + i];
-                    if (T.nch[jb] == 0 && T.flag[jb] == 0 && near_finer(jb, dad)) {
-                        T.flag[jb] = 1;
+        for (int l = L - 1; l >= 1; --l)
+            for (int dad = T.laddr[2 * l]; dad <= T.laddr[2 * l + 1]; ++dad) {
+                if (nch[dad] == 0) continue;
+                int gd = par[dad];
+                for (int i = 0; i < ncol[gd]; ++i) {
+                    int jb = col[9 * gd + i];
+                    if (nch[jb] == 0 && flag[jb] == 0 && near_finer(jb, dad)) {
+                        flag[jb] = 1;
                         ++nflag;
                     }
                 }
@@ -271,15 +341,16 @@ struct Builder {
         // coarser leaves touching a flagged box, propagated upwards
         for (int l = L; l >= 1; --l)
             for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
-                if (T.flag[b] != 1 && T.flag[b] != 2) continue;
-                int dad = T.par[b];
-                for (int i = 0; i < T.ncol[dad]; ++i) {
-                    int jb = T.col[9 * dad + i];
-                    if (T.nch[jb] == 0 && T.flag[jb] == 0 && near_finer(jb, b)) T.flag[jb] = 2;
+                if (flag[b] != 1 && flag[b] != 2) continue;
+                int dad = par[b];
+                for (int i = 0; i < ncol[dad]; ++i) {
+                    int jb = col[9 * dad + i];
+                    if (nch[jb] == 0 && flag[jb] == 0 && near_finer(jb, b)) flag[jb] = 2;
                 }
             }
-        std::vector<int> tail(2 * (L + 1));
-        for (int l = 0; l <= L; ++l) { tail[2 * l] = 0; tail[2 * l + 1] = -1; }
+        std::vector<int>& tail = T.tail;
+        tail.assign(2 * (L + 1), 0);
+        for (int l = 0; l <= L; ++l) tail[2 * l + 1] = -1;
         for (int l = 1; l <= L - 2; ++l) {
             tail[2 * (l + 1)] = T.nboxes();
             refine_flagged(T.laddr[2 * l], T.laddr[2 * l + 1]);
@@ -288,8 +359,8 @@ struct Builder {
         reorg(tail);
         all_colleagues(L - 1);
         int n3 = 0;
-        for (int& f : T.flag) {
-            if (f != 3) f = 0;
+        for (int b = 0, n = T.nboxes(); b < n; ++b) {
+            if (flag[b] != 3) flag[b] = 0;
             else ++n3;
         }
         if (n3 == 0) return L - 1;  // no flag++ box: the second pass adds nothing
@@ -313,8 +384,9 @@ struct Builder {
 void complete_colleagues(Tree2D& T)
 {
     Builder B(T);
-    for (int l = T.coll_levels + 1; l <= T.nlevels; ++l)
-        for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) B.colleagues(b);
+    for (int l = std::max(T.coll_levels, 0); l < T.nlevels; ++l)
+        for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b)
+            if (B.nch[b] > 0) B.family_colleagues(b);
     T.coll_levels = T.nlevels;
 }
 
@@ -334,7 +406,6 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
     T.bs0 = bs0;
     T.cen0[0] = cen0[0];
     T.cen0[1] = cen0[1];
-    Builder B(T);
     {
         // the adaptive tree has at most 4 boxes per distinct proper prefix of
         // the leaves; the restriction pass adds a fraction more
@@ -344,26 +415,34 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
         T.ncol.reserve(cap); T.col.reserve(9 * cap); T.flag.reserve(cap); T.cen.reserve(2 * cap);
         T.pre.reserve(cap); T.ix.reserve(cap); T.iy.reserve(cap);
     }
+    Builder B(T);
     T.bsz.push_back(bs0);
-    B.add_box(0, -1, cen0[0], cen0[1], 0, 0, 0);
+    B.grow(1);
+    B.lev[0] = 0;
+    B.par[0] = -1;
+    B.cen[0] = cen0[0];
+    B.cen[1] = cen0[1];
+    B.pre[0] = 0;
+    B.ix[0] = B.iy[0] = 0;
     T.laddr = {0, 0};
     // frontier: boxes of the current level with their range of leaf records
     std::vector<int> fbox{0}, fa{0}, fe{(int)leaves.size()}, nb, na, ne;
     int l = 0;
     for (;;) {
-        bool any = false;
+        int nref = 0;
         for (size_t i = 0; i < fbox.size(); ++i)
-            if (fa[i] < fe[i] && leaves[fa[i]].level > l) any = true;
-        if (!any) break;
+            if (fa[i] < fe[i] && leaves[fa[i]].level > l) ++nref;
+        if (nref == 0) break;
         T.bsz.push_back(T.bsz[l] / 2);
-        int first = T.nboxes();
+        const int first = B.grow(4 * nref);
+        int c0 = first;
         nb.clear();
         na.clear();
         ne.clear();
         for (size_t i = 0; i < fbox.size(); ++i) {
             int a = fa[i], e = fe[i];
             if (!(a < e && leaves[a].level > l)) continue;
-            int c0 = B.add_children(fbox[i]);
+            B.set_children(fbox[i], c0);
             int pos = a;
             for (int d = 0; d < 4; ++d) {
                 int s = pos;
@@ -377,6 +456,7 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
                 na.push_back(s);
                 ne.push_back(pos);
             }
+            c0 += 4;
         }
         T.laddr.push_back(first);
         T.laddr.push_back(T.nboxes() - 1);
