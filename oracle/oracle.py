@@ -370,3 +370,260 @@ def ref_tree2d(delta, eps, sources, targ):
                 ncol=itree[p[5]:p[5] + nb].copy(), col=z(itree[p[6]:p[6] + 9 * nb]).reshape(nb, 9),
                 cen=cen, bsz=bsz, isrc=isrc - 1, isrcse=se_s, itarg=itarg - 1, itargse=se_t,
                 bs0cen0=np.array([bs0.value, cen0[0], cen0[1]]))
+
+
+# --------------------------------------------------------------------------
+# 3D: the C restatement of fgt3d (fgt3d_oracle.c) and the reference's own
+# 3D library (oracle/_ref/libref3d.so)
+# --------------------------------------------------------------------------
+
+def reftest3_inputs(nsrc, ntarg):
+    """Inputs of the reference's own 3D test driver (3d/planewave/test/
+    test_fgt3d.f:48-98): per source ten draws (theta, phi, x, y, z, three
+    normal components, charge, dipole strength) with (x, y, z) kept, the
+    corners (0,0,0) and (1,1,1) with zero charge and a unit charge at
+    (0.05,0.05,0.05), then three draws per target."""
+    r = Hkrand()
+    d = r.draw(10 * nsrc).reshape(nsrc, 10)
+    src = d[:, 2:5].copy()
+    chg = d[:, 8:9].copy()
+    chg[0:2] = 0.0
+    chg[2] = 1.0
+    src[0] = (0.0, 0.0, 0.0)
+    src[1] = (1.0, 1.0, 1.0)
+    src[2] = (0.05, 0.05, 0.05)
+    tg = r.draw(3 * ntarg).reshape(ntarg, 3)
+    return src, chg, tg
+
+
+def _pts3(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    assert a.ndim == 2 and a.shape[1] == 3
+    return a
+
+
+def _outs3(nd, n, ifp):
+    return (np.zeros((n, nd)), np.zeros((n, 3, nd)), np.zeros((n, 6, nd))) if ifp > 0 else \
+           (np.zeros(nd), np.zeros(3 * nd), np.zeros(6 * nd))
+
+
+_ORC3 = False
+
+
+def _orc3():
+    global _ORC3
+    lib = _orc()
+    if not _ORC3:
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        lib.orc_fgt3d.restype = C.c_int
+        lib.orc_fgt3d.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, dp, C.c_int, dp, C.c_int,
+                                  C.c_int, dp, dp, dp, C.c_int, dp, C.c_int, dp, dp, dp]
+        lib.orc_direct3d.restype = None
+        lib.orc_direct3d.argtypes = [C.c_int, C.c_double, C.c_double, dp, C.c_int, dp, dp, C.c_int,
+                                     C.c_int, dp, dp, dp]
+        lib.orc3_boxsize0.restype = None
+        lib.orc3_boxsize0.argtypes = [C.c_double, C.c_double, dp, C.c_int, dp, C.c_int, ip, dp, dp]
+        lib.orc3_pwnodes.restype = None
+        lib.orc3_pwnodes.argtypes = [C.c_double, C.c_int, dp, dp]
+        lib.orc_tree3d_build.restype = C.c_void_p
+        lib.orc_tree3d_build.argtypes = [C.c_double, C.c_double, C.c_int, dp, C.c_int, dp, ip]
+        lib.orc_tree3d_get.restype = None
+        lib.orc_tree3d_get.argtypes = [C.c_void_p] + [ip] * 7 + [dp, dp] + [ip] * 4 + [dp]
+        lib.orc_tree3d_free.restype = None
+        lib.orc_tree3d_free.argtypes = [C.c_void_p]
+        lib.orc3_last_stats.restype = None
+        lib.orc3_last_stats.argtypes = [ip]
+        _ORC3 = True
+    return lib
+
+
+def oracle_fgt3d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
+    """The C restatement of fgt3d (fgt3d.f:19) for charges.  Returns
+    (pot, grad, hess, pottarg, gradtarg, hesstarg) shaped (n, nd), (n, 3, nd),
+    (n, 6, nd); Hessians ordered xx, yy, zz, xy, xz, yz."""
+    lib = _orc3()
+    src = _pts3(sources)
+    ns = src.shape[0]
+    chg = np.ascontiguousarray(charge, dtype=np.float64).reshape(ns, -1)
+    nd = chg.shape[1]
+    tg = _pts3(targ) if targ is not None else np.zeros((0, 3))
+    nt = tg.shape[0]
+    pot, grad, hess = _outs3(nd, ns, ifpgh)
+    pt, gt, ht = _outs3(nd, max(nt, 1), ifpghtarg)
+    rc = lib.orc_fgt3d(nd, delta, eps, ns, _d(src), 1, _d(chg), 0, ifpgh, _d(pot), _d(grad),
+                       _d(hess), nt, _d(tg), ifpghtarg, _d(pt), _d(gt), _d(ht))
+    if rc != 0:
+        raise RuntimeError("orc_fgt3d returned %d" % rc)
+    return pot, grad, hess, pt[:nt], gt[:nt], ht[:nt]
+
+
+def oracle_stats3d():
+    out = np.zeros(8, dtype=np.int32)
+    _orc3().orc3_last_stats(_i(out))
+    return dict(npwlevel=int(out[0]), nlmax=int(out[1]), nlevels=int(out[2]), nboxes=int(out[3]),
+                npw=int(out[4]), nleaves=int(out[5]))
+
+
+def direct3d(delta, dmax, sources, charge, targ, ifpgh=1):
+    """FP64 3D direct sum over pairs with r^2 < dmax (g3ddirect.f:3-206,
+    fgt3dpart_direct_vec takes dmax itself); dmax=1e30 is the reference
+    3D test's no-cutoff oracle."""
+    lib = _orc3()
+    src = _pts3(sources)
+    chg = np.ascontiguousarray(charge, dtype=np.float64).reshape(src.shape[0], -1)
+    nd = chg.shape[1]
+    tg = _pts3(targ)
+    nt = tg.shape[0]
+    pot, grad, hess = np.zeros((nt, nd)), np.zeros((nt, 3, nd)), np.zeros((nt, 6, nd))
+    lib.orc_direct3d(nd, delta, dmax, _d(src), src.shape[0], _d(chg), _d(tg), nt, ifpgh,
+                     _d(pot), _d(grad), _d(hess))
+    return pot, grad, hess
+
+
+def boxsize0_3d(delta, eps, sources, targ):
+    src, tg = _pts3(sources), _pts3(targ)
+    lc, bs0, cen = C.c_int(), C.c_double(), np.zeros(3)
+    _orc3().orc3_boxsize0(delta, eps, _d(src), src.shape[0], _d(tg), tg.shape[0], C.byref(lc),
+                          C.byref(bs0), _d(cen))
+    return lc.value, bs0.value, cen
+
+
+def pwnodes3d(pmax, npw):
+    ws, ts = np.zeros(npw), np.zeros(npw)
+    _orc3().orc3_pwnodes(pmax, npw, _d(ws), _d(ts))
+    return ws, ts
+
+
+def oracle_tree3d(delta, eps, sources, targ):
+    """The octree fgt3d builds (fgt3d.f:124-221), 0-based."""
+    lib = _orc3()
+    src, tg = _pts3(sources), _pts3(targ)
+    info = np.zeros(4, dtype=np.int32)
+    h = lib.orc_tree3d_build(delta, eps, src.shape[0], _d(src), tg.shape[0], _d(tg), _i(info))
+    npwlevel, nlmax, nlevels, nboxes = (int(v) for v in info)
+    o = dict(npwlevel=npwlevel, nlmax=nlmax, nlevels=nlevels, nboxes=nboxes,
+             laddr=np.zeros((nlevels + 1, 2), np.int32), lev=np.zeros(nboxes, np.int32),
+             par=np.zeros(nboxes, np.int32), nch=np.zeros(nboxes, np.int32),
+             ch=np.zeros((nboxes, 8), np.int32), ncol=np.zeros(nboxes, np.int32),
+             col=np.zeros((nboxes, 27), np.int32), cen=np.zeros((nboxes, 3)),
+             bsz=np.zeros(nlevels + 1), isrc=np.zeros(src.shape[0], np.int32),
+             isrcse=np.zeros((nboxes, 2), np.int32), itarg=np.zeros(tg.shape[0], np.int32),
+             itargse=np.zeros((nboxes, 2), np.int32), bs0cen0=np.zeros(4))
+    lib.orc_tree3d_get(h, _i(o["laddr"]), _i(o["lev"]), _i(o["par"]), _i(o["nch"]), _i(o["ch"]),
+                       _i(o["ncol"]), _i(o["col"]), _d(o["cen"]), _d(o["bsz"]), _i(o["isrc"]),
+                       _i(o["isrcse"]), _i(o["itarg"]), _i(o["itargse"]), _d(o["bs0cen0"]))
+    lib.orc_tree3d_free(h)
+    return o
+
+
+_REF3 = None
+
+
+def ref3_path():
+    return os.path.join(_HERE, "_ref", "libref3d.so")
+
+
+def have_ref3():
+    return os.path.exists(ref3_path())
+
+
+def _ref3():
+    global _REF3
+    if _REF3 is None:
+        if not have_ref3():
+            raise RuntimeError("oracle/_ref/libref3d.so is not built")
+        _REF3 = C.CDLL(ref3_path())
+    return _REF3
+
+
+def ref3_defined(delta, eps, sources, targ):
+    """False where the 3D reference reads uninitialised memory: npwlevel ==
+    nlevels (g3dmpalloc returns before setting lmptot, fgt3d.f:1752, and
+    fgt3d zeroes rmlexp(lmptot), :402-405), and the large-delta branch
+    (g3dformlc_vec leaves z undefined, g3drouts.f:803-804)."""
+    tg = targ if targ is not None else np.zeros((0, 3))
+    lc = boxsize0_3d(delta, eps, sources, tg)[0]
+    if lc < -2:
+        return False
+    t = oracle_tree3d(delta, eps, sources, tg)
+    return t["npwlevel"] != t["nlevels"]
+
+
+def ref_fgt3d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
+    """Calls the reference's own fgt3d_ (fgt3d.f:19) through its Fortran ABI."""
+    lib = _ref3()
+    if not ref3_defined(delta, eps, sources, targ):
+        raise RefUndefined("the 3D reference reads uninitialised memory for this input")
+    src = _pts3(sources)
+    ns = src.shape[0]
+    chg = np.ascontiguousarray(charge, dtype=np.float64).reshape(ns, -1)
+    nd = chg.shape[1]
+    tg = _pts3(targ) if targ is not None else np.zeros((1, 3))
+    nt = tg.shape[0] if targ is not None else 0
+    pot, grad, hess = _outs3(nd, ns, ifpgh)
+    pt, gt, ht = _outs3(nd, max(nt, 1), ifpghtarg)
+    dummy = np.zeros(3 * max(ns, nt, 1))
+    lib.fgt3d_(_ri(nd), _rd(delta), _rd(eps), _ri(ns), _d(src), _ri(1), _d(chg), _ri(0), _d(dummy),
+               _d(dummy), _ri(0), _ri(ifpgh), _d(pot), _d(grad), _d(hess), _ri(nt), _d(tg),
+               _ri(ifpghtarg), _d(pt), _d(gt), _d(ht))
+    return pot, grad, hess, pt[:nt], gt[:nt], ht[:nt]
+
+
+def ref_tree3d(delta, eps, sources, targ):
+    """The 3D reference's own tree routines, called as fgt3d calls them
+    (fgt3d.f:124-221; ltree and iptr are integer*8 there).  0-based."""
+    lib = _ref3()
+    src, tg = _pts3(sources), _pts3(targ)
+    ns, nt = src.shape[0], tg.shape[0]
+    lc, bs0, cen0 = C.c_int(), C.c_double(), np.zeros(3)
+    lib.pts_tree_boxsize0_(_rd(delta), _rd(eps), _d(src), _ri(ns), _d(tg), _ri(nt), C.byref(lc),
+                           C.byref(bs0), _d(cen0))
+    npwlevel = lc.value
+    nlevstart = max(npwlevel, 0)
+    nlmax = npwlevel + 4
+    bsize = bs0.value / (2 ** nlevstart)
+    for i in range(nlevstart, 101):
+        nl = C.c_int()
+        lib.g3dhlterms_(_ri(3), _rd(bsize), _rd(delta), _rd(eps), C.byref(nl))
+        if nl.value <= 15:
+            nlmax = i
+            break
+        bsize = bsize / 2
+    nlevels, nboxes, ltree = C.c_int(), C.c_int(), C.c_int64()
+    lib.pts_tree_mem_(_d(src), _ri(ns), _d(tg), _ri(nt), _ri(0), _ri(100), _ri(0), _ri(nlmax), _ri(0),
+                      _ri(0), _ri(40), _ri(npwlevel), C.byref(bs0), _d(cen0), C.byref(nlevels),
+                      C.byref(nboxes), C.byref(ltree))
+    nl_, nb_, lt_ = nlevels.value, nboxes.value, ltree.value
+    itree = np.zeros(lt_, np.int32)
+    iptr = np.zeros(8, np.int64)
+    cen = np.zeros((nb_, 3))
+    bsz = np.zeros(nl_ + 1)
+    lib.pts_tree_build_(_d(src), _ri(ns), _d(tg), _ri(nt), _ri(0), _ri(100), _ri(0), _ri(nlmax), _ri(0),
+                        _ri(0), C.byref(nlevels), C.byref(nboxes), _ri(40), _ri(npwlevel),
+                        C.byref(bs0), _d(cen0), C.byref(ltree), _i(itree),
+                        iptr.ctypes.data_as(C.POINTER(C.c_int64)), _d(cen), _d(bsz))
+    isrc, isrcse = np.zeros(ns, np.int32), np.zeros((nb_, 2), np.int32)
+    itarg, itargse = np.zeros(nt, np.int32), np.zeros((nb_, 2), np.int32)
+    ip8 = iptr.ctypes.data_as(C.POINTER(C.c_int64))
+    lib.pts_tree_sort_(_ri(ns), _d(src), _i(itree), C.byref(ltree), C.byref(nboxes), C.byref(nlevels),
+                       ip8, _d(cen), _i(isrc), _i(isrcse))
+    lib.pts_tree_sort_(_ri(nt), _d(tg), _i(itree), C.byref(ltree), C.byref(nboxes), C.byref(nlevels),
+                       ip8, _d(cen), _i(itarg), _i(itargse))
+    p = [int(v) - 1 for v in iptr]
+
+    def z(v):
+        v = np.asarray(v, dtype=np.int32)
+        return np.where(v > 0, v - 1, -1).astype(np.int32)
+
+    nb = nb_
+    laddr = itree[p[0]:p[0] + 2 * (nl_ + 1)].reshape(nl_ + 1, 2) - 1
+    se_s = isrcse.copy()
+    se_s[:, 0] -= 1
+    se_t = itargse.copy()
+    se_t[:, 0] -= 1
+    return dict(npwlevel=npwlevel, nlmax=nlmax, nlevels=nl_, nboxes=nb, laddr=laddr,
+                lev=itree[p[1]:p[1] + nb].copy(), par=z(itree[p[2]:p[2] + nb]),
+                nch=itree[p[3]:p[3] + nb].copy(), ch=z(itree[p[4]:p[4] + 8 * nb]).reshape(nb, 8),
+                ncol=itree[p[5]:p[5] + nb].copy(), col=z(itree[p[6]:p[6] + 27 * nb]).reshape(nb, 27),
+                cen=cen, bsz=bsz, isrc=isrc - 1, isrcse=se_s, itarg=itarg - 1, itargse=se_t,
+                bs0cen0=np.array([bs0.value, cen0[0], cen0[1], cen0[2]]))

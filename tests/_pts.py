@@ -88,3 +88,29 @@ def dlaran_points(kind, ns, nt, nd=1):
     chg = 2.0 * r.draw(ns * nd).reshape(ns, nd) - 1.0
     tg = pts(nt)
     return src, chg, tg
+
+
+def dlaran_points3d(kind, ns, nt, nd=1):
+    """3D sources, charges (ns, nd) and targets from one DLARAN stream.
+
+    uniform : points uniform in [0,1)^3, charges 2u-1.
+    shell   : sources on a shell of radius 0.4 about (0.5,0.5,0.5) with
+              radial offsets 0.002*(u1+u2+u3+u4-2) (directions from three
+              centred sums of four uniforms, normalised); targets uniform;
+              charges 2u-1.
+    reftest : the reference 3D test driver's own inputs (test_fgt3d.f:48-98).
+    """
+    from oracle import oracle as O
+    if kind == "reftest":
+        return O.reftest3_inputs(ns, nt)
+    r = O.Hkrand()
+    if kind == "uniform":
+        src = r.draw(3 * ns).reshape(ns, 3)
+    else:
+        v = r.draw(12 * ns).reshape(ns, 3, 4).sum(axis=2) - 2.0
+        v = v / np.sqrt((v * v).sum(axis=1))[:, None]
+        rad = 0.4 + 0.002 * (r.draw(4 * ns).reshape(ns, 4).sum(axis=1) - 2.0)
+        src = 0.5 + rad[:, None] * v
+    chg = 2.0 * r.draw(ns * nd).reshape(ns, nd) - 1.0
+    tg = r.draw(3 * nt).reshape(nt, 3)
+    return src, chg, tg

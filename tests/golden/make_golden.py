@@ -1,9 +1,11 @@
 """Writes the golden fixtures of tests/golden/ from the reference itself.
 
 Run in the container that holds the reference sources, after
-``make -C oracle ref`` built oracle/_ref/libref2d.so from them:
+``make -C oracle ref`` built oracle/_ref/libref2d.so and libref3d.so from
+them:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py        # 2D fixtures
+    python tests/golden/make_golden.py 3d     # 3D fixtures (dim = 3)
 
 Every fixture is data only: the name of its input recipe
 (tests/_pts.dlaran_points, DLARAN uniforms and exact arithmetic, so the
@@ -37,6 +39,15 @@ CASES = {
     "uniform_eps3": ("uniform", 50_000, 20_000, 1, 1e-3, 1e-3, 2, 3),
     "large_delta": ("uniform", 5_000, 2_000, 2, 5.0, 1e-6, 3, 3),
 }
+# 3D, from the 3D reference library (oracle/_ref/libref3d.so)
+CASES3D = {
+    # the reference 3D test driver's own configuration (test_fgt3d.f:30-120),
+    # with targets evaluated too
+    "reftest3d_300k": ("reftest", 300_000, 300_000, 1, 1.2e-3, 1e-6, 1, 1),
+    "uniform3d_gh_nd2": ("uniform", 40_000, 5_000, 2, 1e-2, 1e-3, 3, 2),
+    "uniform3d_eps9": ("uniform", 50_000, 3_000, 1, 3e-3, 1e-9, 2, 3),
+    "shell3d": ("shell", 60_000, 20_000, 1, 2e-3, 1e-6, 1, 1),
+}
 NSAMPLE = 1000
 
 
@@ -47,7 +58,35 @@ def sample(n, seed):
     return np.sort(np.concatenate([np.arange(20), 20 + rng.choice(n - 20, NSAMPLE - 20, replace=False)]))
 
 
+def main3d(only=None):
+    if not O.have_ref3():
+        sys.exit("oracle/_ref/libref3d.so missing: run make -C oracle ref first")
+    for name, (recipe, ns, nt, nd, delta, eps, ifs, ift) in CASES3D.items():
+        if only and name not in only:
+            continue
+        src, chg, tg = _pts.dlaran_points3d(recipe, ns, nt, nd)
+        if not O.ref3_defined(delta, eps, src, tg):
+            raise SystemExit("%s: the 3D reference is undefined for this input" % name)
+        out = O.ref_fgt3d(delta, eps, src, chg, tg, ifpgh=ifs, ifpghtarg=ift)
+        t = O.oracle_tree3d(delta, eps, src, tg)
+        isrc, itrg = sample(ns, 1), sample(nt, 2)
+        rec = dict(dim=3, recipe=recipe, ns=ns, nt=nt, nd=nd, delta=delta, eps=eps, ifpgh=ifs,
+                   ifpghtarg=ift, isample_src=isrc, isample_trg=itrg,
+                   ints=np.array([t["npwlevel"], t["nlmax"], t["nlevels"], t["nboxes"]]))
+        names = ["pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"]
+        for k, v in enumerate(out):
+            if (k < 3 and k < ifs) or (k >= 3 and k - 3 < ift):
+                rec[names[k]] = v[isrc if k < 3 else itrg]
+        # the 3D reference test's oracle: direct sum with dmax = 1e30 at the
+        # first 20 sources (test_fgt3d.f:142-152)
+        rec["pot_direct20"] = O.direct3d(delta, 1e30, src, chg, src[:20])[0]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+        print("wrote", name, rec["ints"])
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "3d":
+        return main3d(sys.argv[2:])
     if not O.have_ref():
         sys.exit("oracle/_ref/libref2d.so missing: run make -C oracle ref first")
     for name, (recipe, ns, nt, nd, delta, eps, ifs, ift) in CASES.items():

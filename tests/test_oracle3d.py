@@ -1,0 +1,124 @@
+"""The 3D oracle (oracle/fgt3d_oracle.c) against the 3D reference: golden
+fixtures written by the reference build (tests/golden/*3d*.npz), the
+reference library itself where it is built (oracle/_ref/libref3d.so),
+the FP64 direct sum, and closed forms."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import _pts
+
+GOLD3 = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
+               if "3d" in os.path.basename(p))
+NAMES = ["pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"]
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("path", GOLD3, ids=[os.path.basename(p)[:-4] for p in GOLD3])
+def test_oracle3d_matches_golden(path):
+    with np.load(path, allow_pickle=False) as z:
+        g = {k: z[k] for k in z.files}
+    ns, nt, nd = int(g["ns"]), int(g["nt"]), int(g["nd"])
+    if ns > 100_000:
+        pytest.skip("large fixture: checked on the GPU path")
+    src, chg, tg = _pts.dlaran_points3d(str(g["recipe"]), ns, nt, nd)
+    delta, eps = float(g["delta"]), float(g["eps"])
+    out = O.oracle_fgt3d(delta, eps, src, chg, tg, int(g["ifpgh"]), int(g["ifpghtarg"]))
+    for k, name in enumerate(NAMES):
+        if name in g:
+            sel = g["isample_src"] if k < 3 else g["isample_trg"]
+            assert _rel(out[k][sel], g[name]) < 1e-13, name
+    st = O.oracle_stats3d()
+    assert [st["npwlevel"], st["nlmax"], st["nlevels"], st["nboxes"]] == list(g["ints"])
+    if "pot" in g:
+        ex = O.direct3d(delta, 1e30, src, chg, src[:20])[0]
+        assert _rel(ex, g["pot_direct20"]) < 1e-13
+        assert O.derr(g["pot_direct20"], g["pot"][:20]) < 30 * eps
+
+
+@pytest.mark.skipif(not O.have_ref3(), reason="3D reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("recipe,ns,nt,nd,delta,eps,ifs,ift", [
+    ("uniform", 3000, 1000, 1, 1e-2, 1e-6, 1, 1),
+    ("uniform", 30000, 5000, 2, 1e-2, 1e-3, 3, 2),
+    ("shell", 20000, 4000, 1, 2e-3, 1e-6, 2, 1),
+    ("uniform", 20000, 2000, 1, 2e-2, 1e-3, 1, 3),
+])
+def test_oracle3d_matches_reference_library(recipe, ns, nt, nd, delta, eps, ifs, ift):
+    src, chg, tg = _pts.dlaran_points3d(recipe, ns, nt, nd)
+    if not O.ref3_defined(delta, eps, src, tg):
+        pytest.skip("reference undefined for this input")
+    r = O.ref_fgt3d(delta, eps, src, chg, tg, ifs, ift)
+    o = O.oracle_fgt3d(delta, eps, src, chg, tg, ifs, ift)
+    for k in range(6):
+        if (k < 3 and k < ifs) or (k >= 3 and k - 3 < ift):
+            assert _rel(o[k], r[k]) < 1e-13, NAMES[k]
+
+
+@pytest.mark.skipif(not O.have_ref3(), reason="3D reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("recipe,n,delta,eps", [("uniform", 20000, 1e-2, 1e-6),
+                                                ("shell", 30000, 2e-3, 1e-6),
+                                                ("uniform", 40000, 3e-3, 1e-9)])
+def test_oracle3d_tree_matches_reference_tree(recipe, n, delta, eps):
+    src, _, tg = _pts.dlaran_points3d(recipe, n, n // 4)
+    t = O.oracle_tree3d(delta, eps, src, tg)
+    r = O.ref_tree3d(delta, eps, src, tg)
+    assert t["nboxes"] == r["nboxes"] and t["nlevels"] == r["nlevels"]
+    for k in ("laddr", "lev", "par", "nch", "ch", "ncol", "isrc", "isrcse", "itarg", "itargse"):
+        assert np.array_equal(t[k], r[k]), k
+    assert np.array_equal(t["cen"], r["cen"])
+    for b in range(t["nboxes"]):
+        m = t["ncol"][b]
+        assert np.array_equal(t["col"][b, :m], r["col"][b, :m])
+
+
+@pytest.mark.parametrize("eps", [1e-3, 1e-6, 1e-9])
+def test_planewave_quadrature_3d_nodes(eps):
+    """The ascending 3D nodes (j + 1/2) h, j = -npw/2 .. npw/2-1, give the
+    same quadrature of e^{-u^2} as the 2D ones (g3drouts.f:2669-2690)."""
+    delta = 1e-3
+    dc = math.sqrt(delta * math.log(1 / eps))
+    pmax, npw = O.pwterms(2 * dc, delta, eps)
+    ws, ts = O.pwnodes3d(pmax, npw)
+    assert np.all(np.diff(ts) > 0) and np.allclose(ts, -ts[::-1], rtol=0, atol=1e-15)
+    rmax = 2 * dc / math.sqrt(delta)
+    u = np.linspace(-0.9 * rmax, 0.9 * rmax, 801)
+    approx = (ws[None, :] * np.exp(1j * ts[None, :] * u[:, None])).sum(axis=1)
+    assert np.abs(approx - np.exp(-u ** 2)).max() < 3 * eps
+
+
+def test_large_delta_3d_against_direct_sum():
+    """The large-delta branch (one Taylor expansion) within eps of the direct
+    sum.  The reference's own g3dformlc_vec loses z (g3drouts.f:803-804), so
+    this branch is pinned by the mathematics, not by the reference."""
+    src, chg, tg = _pts.dlaran_points3d("uniform", 500, 200, 1)
+    delta, eps = 30.0, 1e-9
+    assert O.boxsize0_3d(delta, eps, src, tg)[0] < -2
+    out = O.oracle_fgt3d(delta, eps, src, chg, tg, 3, 3)
+    ex = O.direct3d(delta, 1e30, src, chg, tg, 3)
+    for k in range(3):
+        assert _rel(out[3 + k], ex[k]) < 50 * eps, k
+    exs = O.direct3d(delta, 1e30, src, chg, src, 1)[0]
+    assert _rel(out[0], exs) < 50 * eps
+
+
+def test_direct3d_brute_force():
+    rng = np.random.default_rng(2)
+    src, tg = rng.random((60, 3)), rng.random((7, 3))
+    q = rng.normal(size=(60, 2))
+    delta, dmax = 0.05, 0.3
+    p, g, h = O.direct3d(delta, dmax, src, q, tg, 3)
+    d = tg[:, None, :] - src[None, :, :]
+    r2 = (d ** 2).sum(-1)
+    e = np.where(r2 < dmax, np.exp(-r2 / delta), 0.0)
+    assert np.allclose(p, e @ q, rtol=1e-13, atol=0)
+    gx = (-2 / delta) * d[..., 0] * e
+    assert np.allclose(g[:, 0, :], gx @ q, rtol=1e-12, atol=1e-300)
+    hxy = (4 / delta ** 2) * d[..., 0] * d[..., 1] * e
+    assert np.allclose(h[:, 3, :], hxy @ q, rtol=1e-12, atol=1e-300)
