@@ -62,10 +62,15 @@ def load():
     lib.fgtpw_kernel_name.restype = _C.c_char_p
     lib.fgtpw_kernel_name.argtypes = [i]
     lib.fgtpw_last_tree2d.argtypes = [ip, i] + [dp] * 12
+    lib.fgtpw_fgt3d.argtypes = lib.fgtpw_fgt2d.argtypes
+    lib.fgtpw_fgt3d_device.argtypes = lib.fgtpw_fgt2d_device.argtypes
+    lib.fgtpw_fgt3dpart_direct.argtypes = lib.fgtpw_fgt2dpart_direct.argtypes
+    lib.fgtpw_last_tree3d.argtypes = [ip, i] + [dp] * 12
     lib.fgtpw_tree2d_from_leaves.argtypes = [i, i, d, dp, i, dp, dp, ip, i] + [dp] * 8
     for f in ("fgtpw_init", "fgtpw_fgt2d", "fgtpw_fgt2d_device", "fgtpw_fgt2dpart_direct",
               "fgtpw_last_stats", "fgtpw_set_timing", "fgtpw_stage_times", "fgtpw_last_tree2d",
-              "fgtpw_tree2d_from_leaves"):
+              "fgtpw_tree2d_from_leaves", "fgtpw_fgt3d", "fgtpw_fgt3d_device",
+              "fgtpw_fgt3dpart_direct", "fgtpw_last_tree3d"):
         getattr(lib, f).restype = _C.c_int
     _LIB = lib
     return lib
@@ -88,8 +93,8 @@ def _is_torch_gpu(a):
     return hasattr(a, "is_cuda") and a.is_cuda
 
 
-def _outputs(mod, n, nd, ifp, like):
-    shapes = [(n, nd), (n, 2, nd), (n, 3, nd)]
+def _outputs(mod, n, nd, ifp, like, dim=2):
+    shapes = [(n, nd), (n, dim, nd), (n, 3 if dim == 2 else 6, nd)]
     if mod is None:
         return [(_np.zeros(s) if k < ifp else None) for k, s in enumerate(shapes)]
     return [(mod.zeros(s, dtype=mod.float64, device=like.device) if k < ifp else None)
@@ -105,6 +110,21 @@ def fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, ifcharge
     Returns (pot, grad, hess, pottarg, gradtarg, hesstarg), None for what was
     not requested.  numpy in -> numpy out; torch GPU tensors in -> torch out.
     """
+    return _transform(2, delta, eps, sources, charge, targ, ifpgh, ifpghtarg, ifcharge,
+                      ifdipole, rnormal, dipstr, iper)
+
+
+def fgt3d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, ifcharge=1,
+          ifdipole=0, rnormal=None, dipstr=None, iper=0):
+    """The transform of fgt3d (fgt3d.f:19) for points of shape (n, 3):
+    gradients (n, 3, nd), Hessians (n, 6, nd) ordered xx, yy, zz, xy, xz, yz.
+    Same conventions as fgt2d."""
+    return _transform(3, delta, eps, sources, charge, targ, ifpgh, ifpghtarg, ifcharge,
+                      ifdipole, rnormal, dipstr, iper)
+
+
+def _transform(dim, delta, eps, sources, charge, targ, ifpgh, ifpghtarg, ifcharge, ifdipole,
+               rnormal, dipstr, iper):
     lib = load()
     gpu = _is_torch_gpu(sources)
     if gpu:
@@ -119,13 +139,14 @@ def fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, ifcharge
         ns = src.shape[0]
         chg = _np.ascontiguousarray(charge, dtype=_np.float64).reshape(ns, -1)
         tg = _np.ascontiguousarray(targ, dtype=_np.float64) if targ is not None else None
-    if src.ndim != 2 or src.shape[1] != 2:
-        raise ValueError("sources must have shape (ns, 2)")
+    if src.ndim != 2 or src.shape[1] != dim:
+        raise ValueError("sources must have shape (ns, %d)" % dim)
     nd = chg.shape[1]
     nt = 0 if tg is None else tg.shape[0]
-    if tg is not None and (tg.ndim != 2 or tg.shape[1] != 2):
-        raise ValueError("targ must have shape (nt, 2)")
-    outs = _outputs(mod, ns, nd, ifpgh, src) + _outputs(mod, max(nt, 1), nd, ifpghtarg, src)
+    if tg is not None and (tg.ndim != 2 or tg.shape[1] != dim):
+        raise ValueError("targ must have shape (nt, %d)" % dim)
+    outs = (_outputs(mod, ns, nd, ifpgh, src, dim) +
+            _outputs(mod, max(nt, 1), nd, ifpghtarg, src, dim))
     dummy = _np.zeros(8) if mod is None else mod.zeros(8, dtype=mod.float64, device=src.device)
     P = [_ptr(o if o is not None else dummy) for o in outs]
     args = [nd, float(delta), float(eps), ns, _ptr(src), ifcharge, _ptr(chg), ifdipole,
@@ -134,12 +155,34 @@ def fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, ifcharge
             P[3], P[4], P[5]]
     if gpu:
         stream = mod.cuda.current_stream(src.device).cuda_stream
-        _check(lib.fgtpw_fgt2d_device(*args, _C.c_void_p(stream)))
+        f = lib.fgtpw_fgt2d_device if dim == 2 else lib.fgtpw_fgt3d_device
+        _check(f(*args, _C.c_void_p(stream)))
     else:
-        _check(lib.fgtpw_fgt2d(*args))
+        _check((lib.fgtpw_fgt2d if dim == 2 else lib.fgtpw_fgt3d)(*args))
     if nt == 0:
         outs[3:] = [None, None, None]
     return tuple(outs)
+
+
+def fgt3dpart_direct(delta, dmax, sources, charge, targ, ifpgh=1, pot=None, grad=None, hess=None):
+    """fgt3dpart_direct_vec (fgt3d.f:1533): adds the direct sum over pairs with
+    r^2 < dmax over all sources to pot (grad, hess) at all targets, on the
+    GPU; returns the arrays."""
+    lib = load()
+    src = _np.ascontiguousarray(sources, dtype=_np.float64)
+    ns = src.shape[0]
+    chg = _np.ascontiguousarray(charge, dtype=_np.float64).reshape(ns, -1)
+    nd = chg.shape[1]
+    tg = _np.ascontiguousarray(targ, dtype=_np.float64)
+    nt = tg.shape[0]
+    pot = _np.zeros((nt, nd)) if pot is None else pot
+    grad = _np.zeros((nt, 3, nd)) if grad is None else grad
+    hess = _np.zeros((nt, 6, nd)) if hess is None else hess
+    dummy = _np.zeros(8)
+    _check(lib.fgtpw_fgt3dpart_direct(nd, float(delta), float(dmax), 1, ns, 1, nt, _ptr(src), 1,
+                                      _ptr(chg), 0, _ptr(dummy), _ptr(dummy), _ptr(tg), ifpgh,
+                                      _ptr(pot), _ptr(grad), _ptr(hess)))
+    return pot, grad, hess
 
 
 def fgt2dpart_direct(delta, eps, sources, charge, targ, ifpgh=1, pot=None, grad=None, hess=None):
@@ -244,4 +287,26 @@ def tree2d_from_leaves(nlmax, npwlevel, bs0, cen0, keys, levels):
                                         _ptr(t["par"]), _ptr(t["nch"]), _ptr(t["ch"]),
                                         _ptr(t["ncol"]), _ptr(t["col"]), _ptr(t["cen"])))
     t.update(nlevels=nlevels, nboxes=nb, nleaves=int(info[2]), nlmax=int(info[3]))
+    return t
+
+
+def last_tree3d(ns, nt):
+    """The octree of the last 3D transform (0-based, itree layout)."""
+    lib = load()
+    info = _np.zeros(4, _np.int32)
+    ip = info.ctypes.data_as(_C.POINTER(_C.c_int))
+    _check(lib.fgtpw_last_tree3d(ip, 0, *([None] * 12)))
+    nlevels, nb, nleaves, nlmax = (int(v) for v in info)
+    t = dict(laddr=_np.zeros((nlevels + 1, 2), _np.int32), lev=_np.zeros(nb, _np.int32),
+             par=_np.zeros(nb, _np.int32), nch=_np.zeros(nb, _np.int32),
+             ch=_np.zeros((nb, 8), _np.int32), ncol=_np.zeros(nb, _np.int32),
+             col=_np.zeros((nb, 27), _np.int32), cen=_np.zeros((nb, 3)),
+             isrc=_np.zeros(ns, _np.int32), isrcse=_np.zeros((nb, 2), _np.int32),
+             itarg=_np.zeros(max(nt, 1), _np.int32), itargse=_np.zeros((nb, 2), _np.int32))
+    _check(lib.fgtpw_last_tree3d(ip, nb, _ptr(t["laddr"]), _ptr(t["lev"]), _ptr(t["par"]),
+                                 _ptr(t["nch"]), _ptr(t["ch"]), _ptr(t["ncol"]), _ptr(t["col"]),
+                                 _ptr(t["cen"]), _ptr(t["isrc"]), _ptr(t["isrcse"]),
+                                 _ptr(t["itarg"]), _ptr(t["itargse"])))
+    t["itarg"] = t["itarg"][:nt]
+    t.update(nlevels=nlevels, nboxes=nb, nleaves=nleaves, nlmax=nlmax)
     return t

@@ -30,7 +30,8 @@ DEVICE = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "
           "-packed-fp32-ops", "-munsafe-fp-atomics"]
 HOST = ["-ffp-contract=off"]
 
-SOURCES = [("kernels2d.hip", DEVICE), ("api.cpp", DEVICE), ("tree2d.cpp", HOST)]
+SOURCES = [("kernels2d.hip", DEVICE), ("kernels3d.hip", DEVICE), ("api.cpp", DEVICE),
+           ("tree.cpp", HOST)]
 
 
 def _run(cmd):

@@ -17,6 +17,7 @@
 #include <functional>
 #include <map>
 #include <mutex>
+#include <climits>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,14 @@ struct Tables2D {
                  *wms = nullptr, *wm2l = nullptr;
 };
 
+struct Tables3D {
+    bool valid = false;
+    int npw = 0, ntermmax = 0, nmax = 0;
+    double pmax = 0, xmin2 = 0, xm = 0;
+    const double *ws = nullptr, *ts = nullptr, *coef = nullptr, *dcoef = nullptr, *wms = nullptr,
+                 *wm2l = nullptr;
+};
+
 struct Workspace {
     struct Buf {
         void* p = nullptr;
@@ -79,8 +88,11 @@ struct Workspace {
     std::vector<int> h_units, h_uptr, h_ulist, h_nl1, h_l1, h_sp, h_sl, h_fill, h_slot;
     std::vector<char> h_plan;
     Tables2D tab;
+    Tables3D tab3;
     // the tree of the last transform, kept for inspection
     Tree2D last;
+    Tree3D last3;
+    int last_dim = 2;
     int last_ns = 0, last_nt = 0;
     bool last_valid = false;
 
@@ -382,6 +394,120 @@ const Tables2D& tables2d(Workspace& W, int npw, double pmax, int ntermmax, int n
     return t;
 }
 
+// 3D plane-wave tables (ascending nodes, g3drouts.f:2669-3146): C(j,a) =
+// ws_a (-i ts_a)^j / j! for raw moments (the reference's ws (-ts)^j/sqrt(j!)
+// with its i^j grouping and the Hermite 1/sqrt(j!)), D(j,a) = (i ts_a)^j / j!
+// (g3drx2lmat with the i^j grouping), merge/split factors for the 8 octants
+// per shift length (merge_split_pw_matrices, :2760-2842) and the 27 m2l
+// factors (pw_translation_matrices, :2697-2757).  Cached while unchanged.
+const Tables3D& tables3d(Workspace& W, int npw, double pmax, int ntermmax, int nmax, double xmin2,
+                         double xm, hipStream_t s)
+{
+    Tables3D& t = W.tab3;
+    if (t.valid && t.npw == npw && t.pmax == pmax && t.ntermmax == ntermmax && t.nmax == nmax &&
+        t.xmin2 == xmin2 && t.xm == xm)
+        return t;
+    const int half = npw / 2, mt = ntermmax + 1;
+    const size_t nexp = (size_t)npw * npw * half;
+    std::vector<double> ws(npw), ts(npw);
+    planewave_nodes3(pmax, npw, ws.data(), ts.data());
+    std::vector<double> coef(2 * (size_t)mt * npw), dcoef(2 * (size_t)mt * npw);
+    double fh = 1.0, fl = 1.0;
+    for (int j = 0; j < mt; ++j) {
+        if (j > 0) { fh = fh / std::sqrt((double)j); fl = fl * j; }
+        for (int a = 0; a < npw; ++a) {
+            const double r = ws[a] * std::pow(-ts[a], j) * fh * fh;
+            const double d = std::pow(ts[a], j) / fl;
+            double* c = &coef[2 * (a + (size_t)npw * j)];
+            double* e = &dcoef[2 * (a + (size_t)npw * j)];
+            switch (j & 3) {  // times i^j
+            case 0: c[0] = r; c[1] = 0; e[0] = d; e[1] = 0; break;
+            case 1: c[0] = 0; c[1] = r; e[0] = 0; e[1] = d; break;
+            case 2: c[0] = -r; c[1] = 0; e[0] = -d; e[1] = 0; break;
+            default: c[0] = 0; c[1] = -r; e[0] = 0; e[1] = -d; break;
+            }
+        }
+    }
+    const int nm = std::max(nmax, 1);
+    std::vector<double> wms(2 * nexp * 8 * nm), wm2l(2 * nexp * 27);
+    {
+        std::vector<double> wr((size_t)npw * (nm + 1)), wi((size_t)npw * (nm + 1));
+        for (int j = 0; j < npw; ++j) {
+            double zr = std::cos(ts[j] * xmin2), zi = std::sin(ts[j] * xmin2);
+            for (int k = 1; k <= nm; ++k) {
+                wr[j * (nm + 1) + k] = zr;
+                wi[j * (nm + 1) + k] = zi;
+                const double tr = zr * zr - zi * zi, ti = zr * zi + zi * zr;
+                zr = tr;
+                zi = ti;
+            }
+        }
+        for (int k1 = 1; k1 <= nm; ++k1)
+            for (int c = 0; c < half; ++c)
+                for (int b = 0; b < npw; ++b)
+                    for (int a = 0; a < npw; ++a) {
+                        const size_t e = a + (size_t)npw * (b + (size_t)npw * c);
+                        const double ar = wr[a * (nm + 1) + k1], ai0 = wi[a * (nm + 1) + k1];
+                        const double br = wr[b * (nm + 1) + k1], bi0 = wi[b * (nm + 1) + k1];
+                        const double cr = wr[c * (nm + 1) + k1], ci0 = wi[c * (nm + 1) + k1];
+                        for (int k = 0; k < 8; ++k) {
+                            // x conjugated for k in {2,3,6,7}, y for odd k, z for k >= 4
+                            const double ai = (k == 2 || k == 3 || k == 6 || k == 7) ? -ai0 : ai0;
+                            const double bi = (k & 1) ? -bi0 : bi0;
+                            const double ci = k >= 4 ? -ci0 : ci0;
+                            const double tr = ar * br - ai * bi, ti = ar * bi + ai * br;
+                            double* o = &wms[2 * ((size_t)(k + 8 * (k1 - 1)) * nexp + e)];
+                            o[0] = tr * cr - ti * ci;
+                            o[1] = tr * ci + ti * cr;
+                        }
+                    }
+    }
+    {
+        std::vector<double> pr(npw * 3), pi(npw * 3);
+        for (int j = 0; j < npw; ++j) {
+            const double zr = std::cos(ts[j] * xm), zi = std::sin(ts[j] * xm);
+            pr[3 * j + 0] = zr; pi[3 * j + 0] = -zi;   // offset -1
+            pr[3 * j + 1] = 1;  pi[3 * j + 1] = 0;     // offset 0
+            pr[3 * j + 2] = zr; pi[3 * j + 2] = zi;    // offset +1
+        }
+        for (int kz = 0; kz < 3; ++kz)
+            for (int ky = 0; ky < 3; ++ky)
+                for (int kx = 0; kx < 3; ++kx)
+                    for (int c = 0; c < half; ++c)
+                        for (int b = 0; b < npw; ++b)
+                            for (int a = 0; a < npw; ++a) {
+                                const size_t e = a + (size_t)npw * (b + (size_t)npw * c);
+                                const double ar = pr[3 * a + kx], ai = pi[3 * a + kx];
+                                const double br = pr[3 * b + ky], bi = pi[3 * b + ky];
+                                const double cr = pr[3 * c + kz], ci = pi[3 * c + kz];
+                                const double tr = ar * br - ai * bi, ti = ar * bi + ai * br;
+                                double* o = &wm2l[2 * ((size_t)(kx + 3 * ky + 9 * kz) * nexp + e)];
+                                o[0] = tr * cr - ti * ci;
+                                o[1] = tr * ci + ti * cr;
+                            }
+    }
+    Pack pk;
+    size_t o_ws = pk.add(ws), o_ts = pk.add(ts), o_c = pk.add(coef), o_d = pk.add(dcoef);
+    size_t o_ms = pk.add(wms), o_ml = pk.add(wm2l);
+    char* d = W.get<char>("tables3", pk.bytes.size());
+    HIPCHECK(hipMemcpyAsync(d, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));  // pk is freed on return
+    t.ws = (const double*)(d + o_ws);
+    t.ts = (const double*)(d + o_ts);
+    t.coef = (const double*)(d + o_c);
+    t.dcoef = (const double*)(d + o_d);
+    t.wms = (const double*)(d + o_ms);
+    t.wm2l = (const double*)(d + o_ml);
+    t.valid = true;
+    t.npw = npw;
+    t.pmax = pmax;
+    t.ntermmax = ntermmax;
+    t.nmax = nmax;
+    t.xmin2 = xmin2;
+    t.xm = xm;
+    return t;
+}
+
 // single Taylor expansion about the box centre for tiny boxes,
 // fgt2d_large_delta (fgt2d.f:1616-1733)
 void run_large_delta(Workspace& W, const Call& q, hipStream_t s, int npwlevel, double bs0,
@@ -505,6 +631,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     build_tree2d(T, leaves, nlmax, npwlevel, bs0, cen0);
     W.last_ns = ns;
     W.last_nt = nt;
+    W.last_dim = 2;
     const int nb = T.nboxes(), nlevels = T.nlevels, nleaves = (int)T.leaf_of_rank.size();
     const bool use_src = q.ifpgh > 0, use_trg = q.ifpghtarg > 0 && nt > 0;
     const int npg = std::max(q.ifpgh, q.ifpghtarg);
@@ -981,6 +1108,633 @@ void run_large_delta(Workspace& W, const Call& q, hipStream_t s, int npwlevel, d
     t_stats[6] = nl;
 }
 
+// fgt3d_large_delta (fgt3d.f:1783-1914): one Taylor expansion of all sources
+// about the bounding-box centre, evaluated at every point (outputs
+// overwritten).  The expansion uses the z offset for z; the reference's
+// g3dformlc_vec loses it (g3drouts.f:803-804).
+void run_large_delta3(Workspace& W, const Call& q, hipStream_t s, int npwlevel, double bs0,
+                      const double* cen0, Timer& tm)
+{
+    const int nd = q.nd;
+    int nl = hermite_terms(3, bs0, q.delta, q.eps) + std::max(q.ifpgh, q.ifpghtarg) - 1;
+    if (nl < 0) fail(FGTPW_EUNSUPPORTED, "no Taylor order reaches eps for this box");
+    const int m = nl + 1;
+    const size_t m3 = (size_t)m * m * m;
+    const double dsq = 1.0 / std::sqrt(q.delta);
+    int nparts = std::max(1, std::min(256, (q.ns + 4095) / 4096));
+    double* part = W.get<double>("ld3_part", (size_t)nparts * m3);
+    double* L = W.get<double>("ld3_local", m3 * nd);
+    // one box (id 0) holding every point in input order: ints {box, pad,
+    // 0, ns, 0, nt, 0, 1} give the box list, two int2 ranges and a scan
+    int hostarr[8] = {0, 0, 0, q.ns, 0, q.nt, 0, 0};
+    int* dint = W.get<int>("ld3_int", 16);
+    double* dcen = W.get<double>("ld3_cen", 4);
+    char* hp = (char*)W.pinned(256);
+    std::memcpy(hp, hostarr, sizeof(hostarr));
+    std::memcpy(hp + 64, cen0, 3 * sizeof(double));
+    HIPCHECK(hipMemcpyAsync(dint, hp, sizeof(hostarr), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(dcen, hp + 64, 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    tm.begin(S_FORM);
+    for (int ind = 0; ind < nd; ++ind)
+        launch_formlc3(q.src, q.chg, nd, ind, q.ns, cen0, dsq, nl, part, nparts, L + ind * m3, s);
+    tm.end(S_FORM);
+    tm.begin(S_EVAL);
+    if (q.ifpgh > 0) {
+        launch_zero(q.pot, (size_t)nd * q.ns, s);
+        if (q.ifpgh >= 2) launch_zero(q.grad, 3 * (size_t)nd * q.ns, s);
+        if (q.ifpgh >= 3) launch_zero(q.hess, 6 * (size_t)nd * q.ns, s);
+    }
+    if (q.ifpghtarg > 0 && q.nt > 0) {
+        launch_zero(q.pottarg, (size_t)nd * q.nt, s);
+        if (q.ifpghtarg >= 2) launch_zero(q.gradtarg, 3 * (size_t)nd * q.nt, s);
+        if (q.ifpghtarg >= 3) launch_zero(q.hesstarg, 6 * (size_t)nd * q.nt, s);
+    }
+    const int SEG = 256;
+    for (int side = 0; side < 2; ++side) {
+        const int n = side == 0 ? q.ns : q.nt;
+        const int ifp = side == 0 ? q.ifpgh : q.ifpghtarg;
+        if (ifp <= 0 || n <= 0) continue;
+        // segment scan of the one box: {0, ceil(n/SEG)}
+        int* tst = W.get<int>(side == 0 ? "ld3_ts0" : "ld3_ts1", 2);
+        int hs[2] = {0, (n + SEG - 1) / SEG};
+        int* hp2 = (int*)((char*)W.pinned(256) + 128 + 16 * side);
+        std::memcpy(hp2, hs, sizeof(hs));
+        HIPCHECK(hipMemcpyAsync(tst, hp2, sizeof(hs), hipMemcpyHostToDevice, s));
+        for (int ind = 0; ind < nd; ++ind) {
+            Eval3Args e{};
+            e.boxes = dint;
+            e.nbox = 1;
+            e.cen = dcen;
+            e.se = (const int2*)(dint + (side == 0 ? 2 : 4));
+            e.pts = side == 0 ? q.src : q.targ;
+            e.nd = nd;
+            e.ind = ind;
+            e.dsq = dsq;
+            e.ifpgh = ifp;
+            e.nl = nl;
+            e.taylor = L + ind * m3;
+            e.pot = side == 0 ? q.pot : q.pottarg;
+            e.grad = side == 0 ? q.grad : q.gradtarg;
+            e.hess = side == 0 ? q.hess : q.hesstarg;
+            e.tstart = tst;
+            e.seg = SEG;
+            e.grid = std::min(4096, hs[1]);
+            launch_taylor_eval3(e, s);
+        }
+    }
+    tm.end(S_EVAL);
+    HIPCHECK(hipStreamSynchronize(s));  // the pinned staging above is reused by the next call
+    HIPCHECK(hipGetLastError());
+    tm.finish();
+    t_stats.assign(7, 0);
+    t_stats[0] = npwlevel;
+    t_stats[6] = nl;
+}
+
+// The 3D transform (fgt3d, fgt3d.f:19-480, and fgt3dmain, :483-1530) on one
+// stream: octree from Morton keys, host level restriction and lists, then
+// the six steps.  Differences from run2d follow the reference's 3D code:
+// ndiv0 = 40, ndiv = 100, nlmax at nlocal <= 15, formation and evaluation
+// routes chosen by (p+1)/npw and the box's point count (fgt3d.f:971-983,
+// :1297-1310), listpw without leaf colleagues, near field with
+// dmax = ln(1/eps) delta and the 3D modified list1 (tree_routs3d.f:511-620).
+void run3d(Workspace& W, const Call& q, hipStream_t s)
+{
+    Timer tm(W, s);
+    const int ns = q.ns, nt = q.nt, nd = q.nd;
+    tm.begin(S_BBOX);
+    int nblk = (int)std::min<long>(1024, std::max<long>(1, ((long)ns + nt + 255) / 256));
+    double* part = W.get<double>("bbox_part", 7 * (size_t)nblk);
+    double* bbd = W.get<double>("bbox", 8);
+    launch_bbox3(q.src, ns, q.targ, nt, part, nblk, bbd, s);
+    tm.end(S_BBOX);
+    double* hb = (double*)W.pinned(64);
+    copy_to_host(hb, bbd, 7 * sizeof(double), s);
+    if (hb[6] != 0) fail(FGTPW_EINVAL, "non-finite source or target coordinates");
+    double bs0 = hb[1] - hb[0];
+    const double sy = hb[3] - hb[2], sz = hb[5] - hb[4];
+    if (sy > bs0) bs0 = sy;
+    if (sz > bs0) bs0 = sz;
+    const double cen0[3] = {(hb[0] + hb[1]) / 2, (hb[2] + hb[3]) / 2, (hb[4] + hb[5]) / 2};
+    const double dc = std::sqrt(q.delta * std::log(1.0 / q.eps));
+    int npwlevel;
+    if (bs0 > 0) {
+        double dl = std::log(bs0 / dc) / std::log(2.0);
+        if (dl > 20) fail(FGTPW_EUNSUPPORTED, "box too large relative to sqrt(delta): octree deeper than 21 levels");
+        npwlevel = (int)dl;
+        if (bs0 >= dc) {
+            bs0 = (double)(1 << (npwlevel + 1)) * dc;
+            npwlevel += 1;
+        }
+    } else {
+        npwlevel = -1000;
+    }
+    if (npwlevel < -2) {
+        W.last_valid = false;
+        run_large_delta3(W, q, s, npwlevel, bs0, cen0, tm);
+        return;
+    }
+    const int nlevstart = std::max(npwlevel, 0);
+    int nlmax = npwlevel + 4;
+    {
+        double bsize = bs0 / (double)(1 << nlevstart);
+        for (int i = nlevstart; i <= 100; ++i) {
+            if (hermite_terms(3, bsize, q.delta, q.eps) <= 15) {
+                nlmax = i;
+                break;
+            }
+            bsize = bsize / 2;
+        }
+    }
+    if (nlmax > 21) fail(FGTPW_EUNSUPPORTED, "octree deeper than 21 levels");
+
+    // ---------------- keys, sort, leaves of the adaptive tree ----------------
+    tm.begin(S_KEYS);
+    using u64 = unsigned long long;
+    const int nmx = std::max(ns, nt);
+    u64* ks = W.get<u64>("keys_s", ns);
+    u64* ks2 = W.get<u64>("keys_s2", ns);
+    unsigned* ix = W.get<unsigned>("idx_s", nmx);
+    unsigned* isrc = W.get<unsigned>("isrc", ns);
+    size_t tmpb = std::max(sort_pairs_u64_temp(nmx), select_flagged_temp(ns));
+    void* tmp = W.get<char>("sort_tmp", tmpb);
+    const int keybits = std::max(3, 3 * nlmax);
+    launch_keys3d(q.src, ns, cen0, bs0, nlmax, ks, ix, s);
+    sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
+    tm.end(S_KEYS);
+    tm.begin(S_LEAF);
+    int* lvl = W.get<int>("leaf_level", ns);
+    unsigned char* flg = W.get<unsigned char>("leaf_flag", ns);
+    int* starts = W.get<int>("leaf_starts", ns);
+    int* nsel = W.get<int>("nsel", 4);
+    launch_leafdepth(ks2, ns, nlmax, npwlevel, 40, 100, lvl, flg, s, 3);
+    select_flagged(tmp, tmpb, flg, ns, starts, nsel, s);
+    int* hn = (int*)W.pinned(64);
+    copy_to_host(hn, nsel, sizeof(int), s);
+    const int nleaf0 = hn[0];
+    LeafRun* runs = W.get<LeafRun>("leaf_runs", nleaf0);
+    launch_emit_leaves(starts, nsel, ns, ks2, lvl, nlmax, runs, s, 3);
+    tm.end(S_LEAF);
+    std::vector<LeafRun> leaves(nleaf0);
+    {
+        void* hp = W.pinned(sizeof(LeafRun) * (size_t)nleaf0);
+        copy_to_host(hp, runs, sizeof(LeafRun) * (size_t)nleaf0, s);
+        std::memcpy(leaves.data(), hp, sizeof(LeafRun) * (size_t)nleaf0);
+    }
+    tm.begin(S_ASSIGN);
+    double* xs = W.get<double>("xs", 3 * (size_t)ns);
+    double* qs = W.get<double>("qs", (size_t)nd * ns);
+    tm.ctx(-1, nullptr, -1, ns);
+    launch_gather3(q.src, q.chg, nd, isrc, ns, xs, qs, s);
+    unsigned* itarg = nullptr;
+    u64* kt2 = nullptr;
+    double* xt = nullptr;
+    if (nt > 0) {
+        itarg = W.get<unsigned>("itarg", nt);
+        xt = W.get<double>("xt", 3 * (size_t)nt);
+        u64* kt = W.get<u64>("keys_t", nt);
+        kt2 = W.get<u64>("keys_t2", nt);
+        launch_keys3d(q.targ, nt, cen0, bs0, nlmax, kt, ix, s);
+        sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
+        tm.ctx(-1, nullptr, -1, nt);
+        launch_gather3(q.targ, nullptr, 0, itarg, nt, xt, nullptr, s);
+    }
+    tm.end(S_ASSIGN);
+
+    // ---------------- host: tree, lists, tables ----------------
+    auto th0 = std::chrono::steady_clock::now();
+    W.last_valid = false;
+    Tree3D& T = W.last3;
+    build_tree<3>(T, leaves, nlmax, npwlevel, bs0, cen0);
+    W.last_ns = ns;
+    W.last_nt = nt;
+    W.last_dim = 3;
+    const int nb = T.nboxes(), nlevels = T.nlevels, nleaves = (int)T.leaf_of_rank.size();
+    const bool use_src = q.ifpgh > 0, use_trg = q.ifpghtarg > 0 && nt > 0;
+    const int nadd = std::max(q.ifpgh, q.ifpghtarg);
+    std::vector<int> nlocal(nlevels + 1, 0);
+    int ntermmax = 0;
+    for (int i = nlevstart; i <= nlevels; ++i) {
+        nlocal[i] = hermite_terms(3, T.bsz[i], q.delta, q.eps);
+        if (q.ifpgh >= 2 || q.ifpghtarg >= 2) nlocal[i] += nadd;
+        ntermmax = std::max(ntermmax, nlocal[i]);
+    }
+    double pmax = 0;
+    int npw = 0;
+    {
+        double bsize = 2 * bs0 / std::pow(2.0, nlevstart);
+        double pweps = nadd > 2 ? q.eps / 100 : q.eps;
+        planewave_terms(bsize, q.delta, pweps, &pmax, &npw);
+    }
+    const bool expansions = npwlevel < nlevels;
+    const int half = npw / 2;
+    const long nexp = (long)npw * npw * half;
+    const double dsq = 1.0 / std::sqrt(q.delta);
+    if (expansions && npw > 64) fail(FGTPW_EUNSUPPORTED, "3D plane-wave length above 64");
+    // Route per level (fgt3d.f:971-983, :1297-1310): with (p+1)^3 >= 3 npw^3
+    // the reference forms and evaluates the plane waves directly for boxes
+    // under 10^6 points, else through Hermite / Taylor expansions.  thresh:
+    // 0 = Hermite/Taylor for every box, 10^6 = the reference's switch,
+    // INT_MAX = plane waves for every box (orders above 31, where the
+    // Hermite/Taylor kernels stop; differs from the reference only for boxes
+    // of 10^6 points or more).
+    std::vector<int> form_thresh(nlevels + 1, 0), eval_thresh(nlevels + 1, 0);
+    if (expansions)
+        for (int l = nlevstart; l <= nlevels; ++l) {
+            const double r1 = (nlocal[l] + 1.0) / npw;
+            const bool pw = r1 * r1 * r1 >= 3;
+            const bool big = nlocal[l] + 1 > 32;
+            if (big && !pw)
+                fail(FGTPW_EUNSUPPORTED, "3D expansion order above 31 (eps too small for this box size)");
+            form_thresh[l] = eval_thresh[l] = pw ? (big ? INT_MAX : 1000000) : 0;
+        }
+
+    std::vector<int>& slot = W.h_slot;
+    slot.assign(nb, -1);
+    int nslot = 0;
+    if (expansions)
+        for (int b = 0; b < nb; ++b)
+            if (T.lev[b] >= nlevstart) slot[b] = nslot++;
+    Pack pk(W.h_plan);
+    size_t o_child = pk.add(T.ch), o_par = pk.add(T.par), o_cen = pk.add(T.cen);
+    size_t o_ncol = pk.add(T.ncol), o_col = pk.add(T.col), o_rank = pk.add(T.rank_of_box);
+    size_t o_rlo = pk.add(T.rank_lo), o_rhi = pk.add(T.rank_hi), o_slot = pk.add(slot);
+    size_t o_nch = pk.add(T.nch);
+
+    std::vector<std::vector<int>> form_l(nlevels + 1), merge_l(nlevels + 1), split_l(nlevels + 1),
+        eval_l(nlevels + 1);
+    std::vector<int> m2l_l;
+    std::vector<size_t> o_form(nlevels + 1), o_merge(nlevels + 1), o_split(nlevels + 1),
+        o_eval(nlevels + 1);
+    size_t o_m2l = 0;
+    if (expansions) {
+        std::vector<char> iflocal(nb, 0);
+        if (npwlevel >= 0 && npwlevel <= nlevels)
+            for (int b = T.laddr[2 * npwlevel]; b <= T.laddr[2 * npwlevel + 1]; ++b) {
+                if (T.nch[b] != 0) continue;
+                for (int i = 0; i < T.ncol[b]; ++i)
+                    if (T.nch[T.col[27 * b + i]] > 0) { iflocal[b] = 1; break; }
+            }
+        for (int l = 0; l <= nlevels; ++l)
+            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
+                if (T.nch[b] == 0 && l >= std::max(npwlevel + 1, 0)) form_l[l].push_back(b);
+                if (T.nch[b] > 0 && l >= nlevstart && l <= nlevels - 1) {
+                    merge_l[l].push_back(b);
+                    for (int j = 0; j < 8; ++j) split_l[l].push_back(T.ch[8 * b + j]);
+                }
+                if (l == nlevstart) m2l_l.push_back(b);
+                if (T.nch[b] == 0 && l >= nlevstart && !(l == npwlevel && !iflocal[b]))
+                    eval_l[l].push_back(b);
+            }
+        for (int l = 0; l <= nlevels; ++l) {
+            o_form[l] = pk.add(form_l[l]);
+            o_merge[l] = pk.add(merge_l[l]);
+            o_split[l] = pk.add(split_l[l]);
+            o_eval[l] = pk.add(eval_l[l]);
+        }
+        o_m2l = pk.add(m2l_l);
+    }
+    const Tables3D& tab = expansions
+                              ? tables3d(W, npw, pmax, ntermmax, nlevels - nlevstart,
+                                         T.bsz[nlevels] / std::sqrt(q.delta) / 2,
+                                         T.bsz[nlevstart] / std::sqrt(q.delta), s)
+                              : W.tab3;
+
+    // near field: modified list1 of the source leaves at or above the cutoff
+    // (compute_modified_list1, tree_routs3d.f:511-620, FP64 distance tests
+    // as there), inverted to target units as in 2D
+    const int nlevend = std::min(npwlevel, nlevels);
+    std::vector<int>& unit_ids = W.h_units;
+    std::vector<int>& unit_ptr = W.h_uptr;
+    std::vector<int>& unit_list = W.h_ulist;
+    unit_ids.clear();
+    unit_ptr.assign(1, 0);
+    unit_list.clear();
+    if (nlevend >= 0) {
+        const int MNL = 27 + 27 * 8 + 27;
+        std::vector<int>& nl1 = W.h_nl1;
+        std::vector<int>& l1 = W.h_l1;
+        nl1.assign(nb, 0);
+        if (l1.size() < (size_t)MNL * nb) l1.resize((size_t)MNL * nb);
+        if (T.nch[0] == 0) { nl1[0] = 1; l1[0] = 0; }
+        auto touch = [&](int a, int b, double tol) {
+            for (int d = 0; d < 3; ++d)
+                if (!(std::fabs(T.cen[3 * a + d] - T.cen[3 * b + d]) < tol)) return false;
+            return true;
+        };
+        for (int l = 1; l <= nlevend; ++l)
+            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
+                if (T.nch[b] != 0) continue;
+                int* out = &l1[(size_t)MNL * b];
+                for (int i = 0; i < T.ncol[b]; ++i) {
+                    int jb = T.col[27 * b + i];
+                    if (l >= npwlevel || T.nch[jb] == 0) {
+                        out[nl1[b]++] = jb;
+                    } else {
+                        const double tol = 1.05 * (T.bsz[l] + T.bsz[l + 1]) / 2.0;
+                        for (int j = 0; j < 8; ++j) {
+                            int kb = T.ch[8 * jb + j];
+                            if (kb >= 0 && touch(kb, b, tol)) out[nl1[b]++] = kb;
+                        }
+                    }
+                }
+                const int dad = T.par[b];
+                const double tol = 1.05 * (T.bsz[l] + T.bsz[l - 1]) / 2.0;
+                for (int i = 0; i < T.ncol[dad]; ++i) {
+                    int jb = T.col[27 * dad + i];
+                    if (T.nch[jb] == 0 && touch(jb, b, tol)) out[nl1[b]++] = jb;
+                }
+            }
+        std::vector<int>& sp = W.h_sp;
+        std::vector<int>& sl = W.h_sl;
+        sp.assign(nb + 1, 0);
+        for (int l = 0; l <= nlevend; ++l)
+            for (int sb = T.laddr[2 * l]; sb <= T.laddr[2 * l + 1]; ++sb) {
+                if (T.nch[sb] != 0) continue;
+                for (int i = 0; i < nl1[sb]; ++i) ++sp[l1[(size_t)MNL * sb + i] + 1];
+            }
+        for (int b = 0; b < nb; ++b) sp[b + 1] += sp[b];
+        sl.resize(sp[nb]);
+        std::vector<int>& fill = W.h_fill;
+        fill.assign(sp.begin(), sp.end() - 1);
+        for (int l = 0; l <= nlevend; ++l)
+            for (int sb = T.laddr[2 * l]; sb <= T.laddr[2 * l + 1]; ++sb) {
+                if (T.nch[sb] != 0) continue;
+                for (int i = 0; i < nl1[sb]; ++i) sl[fill[l1[(size_t)MNL * sb + i]]++] = sb;
+            }
+        for (int b = 0; b < nb; ++b) {
+            bool unit = (T.nch[b] == 0 && T.lev[b] <= npwlevel) || T.lev[b] == npwlevel;
+            if (!unit) continue;
+            size_t before = unit_list.size();
+            for (int a = b; a >= 0; a = T.par[a])
+                unit_list.insert(unit_list.end(), sl.begin() + sp[a], sl.begin() + sp[a + 1]);
+            if (unit_list.size() == before) continue;
+            unit_ids.push_back(b);
+            unit_ptr.push_back((int)unit_list.size());
+        }
+    }
+    size_t o_units = pk.add(unit_ids), o_uptr = pk.add(unit_ptr), o_ulist = pk.add(unit_list);
+    auto th1 = std::chrono::steady_clock::now();
+    tm.host_ms = std::chrono::duration<double, std::milli>(th1 - th0).count();
+
+    char* dplan = W.get<char>("plan", pk.bytes.size());
+    {
+        void* hp = W.pinned(pk.bytes.size());
+        std::memcpy(hp, pk.bytes.data(), pk.bytes.size());
+        HIPCHECK(hipMemcpyAsync(dplan, hp, pk.bytes.size(), hipMemcpyHostToDevice, s));
+    }
+    auto D = [&](size_t off) { return (void*)(dplan + off); };
+    const int* d_child = (const int*)D(o_child);
+    const int* d_par = (const int*)D(o_par);
+    const double* d_cen = (const double*)D(o_cen);
+    const int* d_ncol = (const int*)D(o_ncol);
+    const int* d_col = (const int*)D(o_col);
+    const int* d_rank = (const int*)D(o_rank);
+    const int* d_rlo = (const int*)D(o_rlo);
+    const int* d_rhi = (const int*)D(o_rhi);
+    const int* d_slot = (const int*)D(o_slot);
+    const int* d_nch = (const int*)D(o_nch);
+
+    // ---------------- point ranges of the boxes ----------------
+    tm.begin(S_RANGES);
+    unsigned* rk = W.get<unsigned>("rank", nmx);
+    int* ls = W.get<int>("leafstart", nleaves + 1);
+    int2* src_se = W.get<int2>("src_se", nb);
+    int2* trg_se = W.get<int2>("trg_se", nb);
+    launch_descend_rank3(ks2, ns, nlmax, d_child, d_rank, rk, s);
+    launch_leafstart(rk, ns, nleaves, ls, s);
+    launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, s);
+    tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
+    if (nt > 0) {
+        launch_descend_rank3(kt2, nt, nlmax, d_child, d_rank, rk, s);
+        launch_leafstart(rk, nt, nleaves, ls, s);
+        launch_box_ranges(d_rlo, d_rhi, nb, ls, trg_se, s);
+    } else {
+        HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));
+    }
+    tm.end(S_RANGES);
+
+    const int ws_ = use_src ? ns : 0, wt_ = use_trg ? nt : 0;
+    double *ps = nullptr, *gs = nullptr, *hs = nullptr, *pt = nullptr, *gt = nullptr, *ht = nullptr;
+    if (use_src) {
+        ps = W.get<double>("pot_s", (size_t)nd * ws_);
+        launch_zero(ps, (size_t)nd * ws_, s);
+        if (q.ifpgh >= 2) { gs = W.get<double>("grad_s", 3 * (size_t)nd * ws_); launch_zero(gs, 3 * (size_t)nd * ws_, s); }
+        if (q.ifpgh >= 3) { hs = W.get<double>("hess_s", 6 * (size_t)nd * ws_); launch_zero(hs, 6 * (size_t)nd * ws_, s); }
+    }
+    if (use_trg) {
+        pt = W.get<double>("pot_t", (size_t)nd * wt_);
+        launch_zero(pt, (size_t)nd * wt_, s);
+        if (q.ifpghtarg >= 2) { gt = W.get<double>("grad_t", 3 * (size_t)nd * wt_); launch_zero(gt, 3 * (size_t)nd * wt_, s); }
+        if (q.ifpghtarg >= 3) { ht = W.get<double>("hess_t", 6 * (size_t)nd * wt_); launch_zero(ht, 6 * (size_t)nd * wt_, s); }
+    }
+
+    // ---------------- expansions: steps 1-5 ----------------
+    if (expansions) {
+        const size_t per = 2 * (size_t)nexp * nd;
+        double* mp = W.get<double>("mp", per * nslot);
+        double* lc = W.get<double>("loc", per * nslot);
+        launch_zero(mp, per * nslot, s);
+        launch_zero(lc, per * nslot, s);
+        const int GRID3 = 1024, SEGE = 256, GRIDMAX = 4096;
+        // segment scans of the evaluation lists
+        std::vector<int*> ts_trg(nlevels + 1, nullptr), ts_src(nlevels + 1, nullptr);
+        {
+            size_t ntot = 0;
+            for (int l = 0; l <= nlevels; ++l) ntot += 2 * eval_l[l].size() + 2;
+            int* pool = W.get<int>("tstart", ntot);
+            SegLists L{};
+            auto add = [&](const std::vector<int>& v, size_t off, const int2* se) {
+                int* t = pool;
+                pool += v.size() + 1;
+                if (L.count == 32) {
+                    launch_seg_scan(L, s);
+                    L.count = 0;
+                }
+                L.l[L.count++] = SegList{(const int*)D(off), (int)v.size(), se, SEGE, t};
+                return t;
+            };
+            for (int l = nlevstart; l <= nlevels; ++l) {
+                if (eval_l[l].empty()) continue;
+                if (use_trg) ts_trg[l] = add(eval_l[l], o_eval[l], trg_se);
+                if (use_src) ts_src[l] = add(eval_l[l], o_eval[l], src_se);
+            }
+            if (L.count) launch_seg_scan(L, s);
+        }
+        // formation scratch: per block m^3 + 2 m^2 (npw/2)
+        long sstride = 0;
+        for (int l = std::max(npwlevel + 1, 0); l <= nlevels; ++l) {
+            if (form_thresh[l] == INT_MAX) continue;
+            long m = nlocal[l] + 1;
+            sstride = std::max(sstride, m * m * m + 2 * m * m * half);
+        }
+        double* fscr = W.get<double>("form3_scratch", (size_t)GRID3 * std::max(sstride, 1L));
+        size_t maxtay = 0;
+        for (int l = nlevstart; l <= nlevels; ++l) {
+            if (eval_thresh[l] == INT_MAX) continue;
+            size_t m = nlocal[l] + 1;
+            maxtay = std::max(maxtay, eval_l[l].size() * m * m * m);
+        }
+        double* taylor = W.get<double>("taylor3", std::max<size_t>(maxtay, 1));
+        for (int ind = 0; ind < nd; ++ind) {
+            tm.begin(S_FORM);
+            for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l) {
+                if (form_l[l].empty()) continue;
+                Form3Args fa{};
+                fa.boxes = (const int*)D(o_form[l]);
+                fa.nbox = (int)form_l[l].size();
+                fa.cen = d_cen;
+                fa.src_se = src_se;
+                fa.slot = d_slot;
+                fa.xs = xs;
+                fa.qs = qs;
+                fa.nd = nd;
+                fa.ind = ind;
+                fa.dsq = dsq;
+                fa.p = nlocal[l];
+                fa.npw = npw;
+                fa.coef = tab.coef;
+                fa.cstride = npw;
+                fa.ws = tab.ws;
+                fa.ts = tab.ts;
+                fa.mp = mp;
+                fa.scratch = fscr;
+                fa.sstride = sstride;
+                fa.grid = std::min(GRID3, fa.nbox);
+                fa.thresh = form_thresh[l];
+                tm.ctx(l, &form_l[l], 0);
+                if (fa.thresh != INT_MAX) launch_form3_hermite(fa, s);
+                if (fa.thresh > 0) launch_form3_pw(fa, s);
+            }
+            tm.end(S_FORM);
+        }
+        tm.begin(S_MERGE);
+        for (int l = nlevels - 1; l >= nlevstart; --l)
+            tm.ctx(l, &merge_l[l], 2), launch_merge3((const int*)D(o_merge[l]), (int)merge_l[l].size(),
+                                                     d_child, d_cen, src_se, d_slot, tab.wms,
+                                                     8 * (nlevels - l - 1), nd, nexp, mp, s);
+        tm.end(S_MERGE);
+        tm.begin(S_M2L);
+        tm.ctx(nlevstart, &m2l_l, 2);
+        launch_m2l3((const int*)D(o_m2l), (int)m2l_l.size(), d_cen, d_ncol, d_col, d_nch, src_se,
+                    trg_se, use_src, use_trg, d_slot, tab.wm2l, T.bsz[nlevstart],
+                    nlevstart == npwlevel, nd, nexp, mp, lc, s);
+        tm.end(S_M2L);
+        tm.begin(S_SPLIT);
+        for (int l = nlevstart; l <= nlevels - 1; ++l)
+            tm.ctx(l, &split_l[l], 2), launch_split3((const int*)D(o_split[l]), (int)split_l[l].size(),
+                                                     d_par, d_cen, d_slot, tab.wms,
+                                                     8 * (nlevels - l - 1), nd, nexp, lc, s);
+        tm.end(S_SPLIT);
+        tm.begin(S_EVAL);
+        for (int ind = 0; ind < nd; ++ind)
+            for (int l = nlevstart; l <= nlevels; ++l) {
+                if (eval_l[l].empty()) continue;
+                Eval3Args ea{};
+                ea.boxes = (const int*)D(o_eval[l]);
+                ea.nbox = (int)eval_l[l].size();
+                ea.cen = d_cen;
+                ea.slot = d_slot;
+                ea.nd = nd;
+                ea.ind = ind;
+                ea.dsq = dsq;
+                ea.nl = nlocal[l];
+                ea.dcoef = tab.dcoef;
+                ea.cstride = npw;
+                ea.ts = tab.ts;
+                ea.npw = npw;
+                ea.loc = lc;
+                ea.taylor = taylor;
+                ea.rs = src_se;
+                ea.rt = trg_se;
+                ea.thresh = eval_thresh[l];
+                ea.grid = std::min(2048, ea.nbox);
+                tm.ctx(l, &eval_l[l], 2);
+                if (ea.thresh != INT_MAX) launch_pw2local3(ea, s);
+                for (int side = 0; side < 2; ++side) {
+                    if (side == 0 && !use_trg) continue;
+                    if (side == 1 && !use_src) continue;
+                    Eval3Args eb = ea;
+                    eb.tstart = side == 0 ? ts_trg[l] : ts_src[l];
+                    eb.seg = SEGE;
+                    eb.grid = (int)std::min<long>(GRIDMAX, ea.nbox + (side == 0 ? nt : ns) / SEGE + 1);
+                    eb.se = side == 0 ? trg_se : src_se;
+                    eb.pts = side == 0 ? xt : xs;
+                    eb.ifpgh = side == 0 ? q.ifpghtarg : q.ifpgh;
+                    eb.pot = side == 0 ? pt : ps;
+                    eb.grad = side == 0 ? gt : gs;
+                    eb.hess = side == 0 ? ht : hs;
+                    tm.ctx(l, &eval_l[l], side == 0 ? 1 : 0);
+                    if (eb.thresh != INT_MAX) launch_taylor_eval3(eb, s);
+                    if (eb.thresh > 0) launch_pw_eval3(eb, s);
+                }
+            }
+        tm.end(S_EVAL);
+    }
+
+    // ---------------- step 6: near field, dmax = ln(1/eps) delta (fgt3d.f:1466) ----------------
+    tm.begin(S_DIRECT);
+    if (!unit_ids.empty()) {
+        DirectArgs da{};
+        da.units = (const int*)D(o_units);
+        da.nunits = (int)unit_ids.size();
+        da.list_ptr = (const int*)D(o_uptr);
+        da.list = (const int*)D(o_ulist);
+        da.src_se = src_se;
+        da.xs = xs;
+        da.qs = qs;
+        da.nd = nd;
+        da.delta = q.delta;
+        da.dmax = std::log(1.0 / q.eps) * q.delta;
+        if (use_trg) {
+            DirectArgs d2 = da;
+            d2.tgt_se = trg_se;
+            d2.xt = xt;
+            d2.ifpgh = q.ifpghtarg;
+            d2.pot = pt; d2.grad = gt; d2.hess = ht;
+            tm.ctx(npwlevel, nullptr, -1, nt);
+            launch_direct3(d2, s);
+        }
+        if (use_src) {
+            DirectArgs d2 = da;
+            d2.tgt_se = src_se;
+            d2.xt = xs;
+            d2.ifpgh = q.ifpgh;
+            d2.pot = ps; d2.grad = gs; d2.hess = hs;
+            tm.ctx(npwlevel, nullptr, -1, ns);
+            launch_direct3(d2, s);
+        }
+    }
+    tm.end(S_DIRECT);
+
+    tm.begin(S_SCATTER);
+    if (use_src) {
+        tm.ctx(-1, nullptr, -1, ns);
+        launch_scatter(ps, isrc, ns, nd, q.pot, s);
+        if (q.ifpgh >= 2) launch_scatter(gs, isrc, ns, 3 * nd, q.grad, s);
+        if (q.ifpgh >= 3) launch_scatter(hs, isrc, ns, 6 * nd, q.hess, s);
+    }
+    if (use_trg) {
+        tm.ctx(-1, nullptr, -1, nt);
+        launch_scatter(pt, itarg, nt, nd, q.pottarg, s);
+        if (q.ifpghtarg >= 2) launch_scatter(gt, itarg, nt, 3 * nd, q.gradtarg, s);
+        if (q.ifpghtarg >= 3) launch_scatter(ht, itarg, nt, 6 * nd, q.hesstarg, s);
+    }
+    tm.end(S_SCATTER);
+    HIPCHECK(hipGetLastError());
+    tm.finish();
+    W.last_valid = true;
+
+    t_stats.assign(6 + nlevels + 1, 0);
+    t_stats[0] = npwlevel;
+    t_stats[1] = nlmax;
+    t_stats[2] = nlevels;
+    t_stats[3] = nb;
+    t_stats[4] = npw;
+    t_stats[5] = nleaves;
+    for (int l = 0; l <= nlevels; ++l) t_stats[6 + l] = nlocal[l];
+}
+
 int run_guarded(const std::function<void()>& f)
 {
     t_err.clear();
@@ -1057,6 +1811,119 @@ int fgtpw_fgt2d_device(int nd, double delta, double eps, int ns, const double* s
     });
 }
 
+}  // extern "C"
+
+namespace fgtpw {
+namespace {
+// host-array entry (fgt2d / fgt3d): copies in, runs on the library's own
+// stream, copies the requested outputs back
+int run_host(int dim, int nd, double delta, double eps, int ns, const double* sources,
+             int ifcharge, const double* charge, int ifdipole, int ifpgh, double* pot,
+             double* grad, double* hess, int nt, const double* targ, int ifpghtarg,
+             double* pottarg, double* gradtarg, double* hesstarg)
+{
+    return run_guarded([&] {
+        validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
+        std::lock_guard<std::mutex> lk(g_mutex);
+        select_device();
+        Workspace& W = workspace();
+        hipStream_t s = W.own;
+        const size_t ng = dim, nh = dim == 2 ? 3 : 6;
+        const size_t nps = dim * (size_t)ns, npt = dim * (size_t)nt;
+        double* d_src = W.get<double>("in_src", nps);
+        double* d_chg = W.get<double>("in_chg", (size_t)nd * ns);
+        double* d_trg = W.get<double>("in_trg", std::max<size_t>(npt, 3));
+        HIPCHECK(hipMemcpyAsync(d_src, sources, nps * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d_chg, charge, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+        if (nt > 0) HIPCHECK(hipMemcpyAsync(d_trg, targ, npt * 8, hipMemcpyHostToDevice, s));
+        const size_t wps = ifpgh > 0 ? (size_t)ns : 0, wpt = ifpghtarg > 0 ? (size_t)nt : 0;
+        double* o_p = W.get<double>("out_pot", std::max<size_t>(nd * wps, 1));
+        double* o_g = W.get<double>("out_grad", std::max<size_t>(ng * nd * wps, 1));
+        double* o_h = W.get<double>("out_hess", std::max<size_t>(nh * nd * wps, 1));
+        double* o_pt = W.get<double>("out_pott", std::max<size_t>(nd * wpt, 1));
+        double* o_gt = W.get<double>("out_gradt", std::max<size_t>(ng * nd * wpt, 1));
+        double* o_ht = W.get<double>("out_hesst", std::max<size_t>(nh * nd * wpt, 1));
+        Call q{nd, delta, eps, ns, d_src, d_chg, ifpgh, o_p, o_g, o_h,
+               nt, d_trg, ifpghtarg, o_pt, o_gt, o_ht};
+        if (dim == 2)
+            run2d(W, q, s);
+        else
+            run3d(W, q, s);
+        if (ifpgh >= 1) HIPCHECK(hipMemcpyAsync(pot, o_p, nd * wps * 8, hipMemcpyDeviceToHost, s));
+        if (ifpgh >= 2) HIPCHECK(hipMemcpyAsync(grad, o_g, ng * nd * wps * 8, hipMemcpyDeviceToHost, s));
+        if (ifpgh >= 3) HIPCHECK(hipMemcpyAsync(hess, o_h, nh * nd * wps * 8, hipMemcpyDeviceToHost, s));
+        if (ifpghtarg >= 1 && nt > 0)
+            HIPCHECK(hipMemcpyAsync(pottarg, o_pt, nd * wpt * 8, hipMemcpyDeviceToHost, s));
+        if (ifpghtarg >= 2 && nt > 0)
+            HIPCHECK(hipMemcpyAsync(gradtarg, o_gt, ng * nd * wpt * 8, hipMemcpyDeviceToHost, s));
+        if (ifpghtarg >= 3 && nt > 0)
+            HIPCHECK(hipMemcpyAsync(hesstarg, o_ht, nh * nd * wpt * 8, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    });
+}
+
+// fgt2dpart_direct_vec / fgt3dpart_direct_vec: direct sums over ranges with
+// a cutoff r^2 < dmax, accumulated into host arrays
+int run_part_direct(int dim, int nd, double delta, double dmax, int istart, int iend, int jstart,
+                    int jend, const double* source, int ifcharge, const double* charge,
+                    int ifdipole, const double* targ, int ifpgh, double* pot, double* grad,
+                    double* hess)
+{
+    return run_guarded([&] {
+        if (nd < 1) fail(FGTPW_EINVAL, "nd must be >= 1");
+        if (ifdipole == 1) fail(FGTPW_EUNSUPPORTED, "dipole sources are not supported in this build");
+        if (ifcharge != 1) fail(FGTPW_EINVAL, "ifcharge must be 1");
+        if (ifpgh < 1 || ifpgh > 3) return;  // the reference does nothing for other values
+        if (!(delta > 0) || std::isnan(dmax)) fail(FGTPW_EINVAL, "delta must be positive");
+        const int ns = iend - istart + 1, nt = jend - jstart + 1;
+        if (ns <= 0 || nt <= 0) return;
+        if (istart < 1 || jstart < 1) fail(FGTPW_EINVAL, "ranges are 1-based");
+        std::lock_guard<std::mutex> lk(g_mutex);
+        select_device();
+        Workspace& W = workspace();
+        hipStream_t s = W.own;
+        const size_t ng = dim, nh = dim == 2 ? 3 : 6;
+        const double* src = source + dim * (size_t)(istart - 1);
+        const double* chg = charge + (size_t)nd * (istart - 1);
+        const double* tg = targ + dim * (size_t)(jstart - 1);
+        double* d_src = W.get<double>("pd_src", dim * (size_t)ns);
+        double* d_chg = W.get<double>("pd_chg", (size_t)nd * ns);
+        double* d_tg = W.get<double>("pd_tg", dim * (size_t)nt);
+        double* d_p = W.get<double>("pd_pot", (size_t)nd * nt);
+        double* d_g = W.get<double>("pd_grad", ng * nd * nt);
+        double* d_h = W.get<double>("pd_hess", nh * nd * nt);
+        HIPCHECK(hipMemcpyAsync(d_src, src, dim * (size_t)ns * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d_chg, chg, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d_tg, tg, dim * (size_t)nt * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d_p, pot + (size_t)nd * (jstart - 1), (size_t)nd * nt * 8,
+                                hipMemcpyHostToDevice, s));
+        if (ifpgh >= 2)
+            HIPCHECK(hipMemcpyAsync(d_g, grad + ng * nd * (jstart - 1), ng * nd * nt * 8,
+                                    hipMemcpyHostToDevice, s));
+        if (ifpgh >= 3)
+            HIPCHECK(hipMemcpyAsync(d_h, hess + nh * nd * (jstart - 1), nh * nd * nt * 8,
+                                    hipMemcpyHostToDevice, s));
+        if (dim == 2)
+            launch_direct_all(d_src, d_chg, nd, ns, d_tg, nt, delta, dmax, ifpgh, d_p, d_g, d_h, s);
+        else
+            launch_direct_all3(d_src, d_chg, nd, ns, d_tg, nt, delta, dmax, ifpgh, d_p, d_g, d_h, s);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(pot + (size_t)nd * (jstart - 1), d_p, (size_t)nd * nt * 8,
+                                hipMemcpyDeviceToHost, s));
+        if (ifpgh >= 2)
+            HIPCHECK(hipMemcpyAsync(grad + ng * nd * (jstart - 1), d_g, ng * nd * nt * 8,
+                                    hipMemcpyDeviceToHost, s));
+        if (ifpgh >= 3)
+            HIPCHECK(hipMemcpyAsync(hess + nh * nd * (jstart - 1), d_h, nh * nd * nt * 8,
+                                    hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    });
+}
+}  // namespace
+}  // namespace fgtpw
+
+extern "C" {
+
 int fgtpw_fgt2d(int nd, double delta, double eps, int ns, const double* sources, int ifcharge,
                 const double* charge, int ifdipole, const double* rnormal, const double* dipstr,
                 int iper, int ifpgh, double* pot, double* grad, double* hess, int nt,
@@ -1064,40 +1931,49 @@ int fgtpw_fgt2d(int nd, double delta, double eps, int ns, const double* sources,
                 double* hesstarg)
 {
     (void)rnormal; (void)dipstr; (void)iper;
+    return run_host(2, nd, delta, eps, ns, sources, ifcharge, charge, ifdipole, ifpgh, pot, grad,
+                    hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg);
+}
+
+int fgtpw_fgt3d(int nd, double delta, double eps, int ns, const double* sources, int ifcharge,
+                const double* charge, int ifdipole, const double* rnormal, const double* dipstr,
+                int iper, int ifpgh, double* pot, double* grad, double* hess, int nt,
+                const double* targ, int ifpghtarg, double* pottarg, double* gradtarg,
+                double* hesstarg)
+{
+    (void)rnormal; (void)dipstr; (void)iper;
+    return run_host(3, nd, delta, eps, ns, sources, ifcharge, charge, ifdipole, ifpgh, pot, grad,
+                    hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg);
+}
+
+int fgtpw_fgt3d_device(int nd, double delta, double eps, int ns, const double* sources,
+                       int ifcharge, const double* charge, int ifdipole, const double* rnormal,
+                       const double* dipstr, int iper, int ifpgh, double* pot, double* grad,
+                       double* hess, int nt, const double* targ, int ifpghtarg, double* pottarg,
+                       double* gradtarg, double* hesstarg, void* stream)
+{
+    (void)rnormal; (void)dipstr; (void)iper;
     return run_guarded([&] {
         validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
         std::lock_guard<std::mutex> lk(g_mutex);
         select_device();
         Workspace& W = workspace();
-        hipStream_t s = W.own;
-        const size_t n2s = 2 * (size_t)ns, n2t = 2 * (size_t)nt;
-        double* d_src = W.get<double>("in_src", n2s);
-        double* d_chg = W.get<double>("in_chg", (size_t)nd * ns);
-        double* d_trg = W.get<double>("in_trg", std::max<size_t>(n2t, 2));
-        HIPCHECK(hipMemcpyAsync(d_src, sources, n2s * 8, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d_chg, charge, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
-        if (nt > 0) HIPCHECK(hipMemcpyAsync(d_trg, targ, n2t * 8, hipMemcpyHostToDevice, s));
-        const size_t wps = ifpgh > 0 ? (size_t)ns : 0, wpt = ifpghtarg > 0 ? (size_t)nt : 0;
-        double* o_p = W.get<double>("out_pot", std::max<size_t>(nd * wps, 1));
-        double* o_g = W.get<double>("out_grad", std::max<size_t>(2 * nd * wps, 1));
-        double* o_h = W.get<double>("out_hess", std::max<size_t>(3 * nd * wps, 1));
-        double* o_pt = W.get<double>("out_pott", std::max<size_t>(nd * wpt, 1));
-        double* o_gt = W.get<double>("out_gradt", std::max<size_t>(2 * nd * wpt, 1));
-        double* o_ht = W.get<double>("out_hesst", std::max<size_t>(3 * nd * wpt, 1));
-        Call q{nd, delta, eps, ns, d_src, d_chg, ifpgh, o_p, o_g, o_h,
-               nt, d_trg, ifpghtarg, o_pt, o_gt, o_ht};
-        run2d(W, q, s);
-        if (ifpgh >= 1) HIPCHECK(hipMemcpyAsync(pot, o_p, nd * wps * 8, hipMemcpyDeviceToHost, s));
-        if (ifpgh >= 2) HIPCHECK(hipMemcpyAsync(grad, o_g, 2 * nd * wps * 8, hipMemcpyDeviceToHost, s));
-        if (ifpgh >= 3) HIPCHECK(hipMemcpyAsync(hess, o_h, 3 * nd * wps * 8, hipMemcpyDeviceToHost, s));
-        if (ifpghtarg >= 1 && nt > 0)
-            HIPCHECK(hipMemcpyAsync(pottarg, o_pt, nd * wpt * 8, hipMemcpyDeviceToHost, s));
-        if (ifpghtarg >= 2 && nt > 0)
-            HIPCHECK(hipMemcpyAsync(gradtarg, o_gt, 2 * nd * wpt * 8, hipMemcpyDeviceToHost, s));
-        if (ifpghtarg >= 3 && nt > 0)
-            HIPCHECK(hipMemcpyAsync(hesstarg, o_ht, 3 * nd * wpt * 8, hipMemcpyDeviceToHost, s));
+        Call q{nd, delta, eps, ns, sources, charge, ifpgh, pot, grad, hess,
+               nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg};
+        hipStream_t s = (hipStream_t)stream;
+        run3d(W, q, s);
         HIPCHECK(hipStreamSynchronize(s));
     });
+}
+
+int fgtpw_fgt3dpart_direct(int nd, double delta, double dmax, int istart, int iend, int jstart,
+                           int jend, const double* source, int ifcharge, const double* charge,
+                           int ifdipole, const double* rnormal, const double* dipstr,
+                           const double* targ, int ifpgh, double* pot, double* grad, double* hess)
+{
+    (void)rnormal; (void)dipstr;
+    return run_part_direct(3, nd, delta, dmax, istart, iend, jstart, jend, source, ifcharge,
+                           charge, ifdipole, targ, ifpgh, pot, grad, hess);
 }
 
 int fgtpw_fgt2dpart_direct(int nd, double delta, double eps, int istart, int iend, int jstart,
@@ -1106,54 +1982,12 @@ int fgtpw_fgt2dpart_direct(int nd, double delta, double eps, int istart, int ien
                            const double* targ, int ifpgh, double* pot, double* grad, double* hess)
 {
     (void)rnormal; (void)dipstr;
-    return run_guarded([&] {
-        if (nd < 1) fail(FGTPW_EINVAL, "nd must be >= 1");
-        if (ifdipole == 1) fail(FGTPW_EUNSUPPORTED, "dipole sources are not supported in this build");
-        if (ifcharge != 1) fail(FGTPW_EINVAL, "ifcharge must be 1");
-        if (ifpgh < 1 || ifpgh > 3) return;  // the reference does nothing for other values
-        if (!(delta > 0) || !(eps > 0)) fail(FGTPW_EINVAL, "delta and eps must be positive");
-        const int ns = iend - istart + 1, nt = jend - jstart + 1;
-        if (ns <= 0 || nt <= 0) return;
-        if (istart < 1 || jstart < 1) fail(FGTPW_EINVAL, "ranges are 1-based");
-        std::lock_guard<std::mutex> lk(g_mutex);
-        select_device();
-        Workspace& W = workspace();
-        hipStream_t s = W.own;
-        const double* src = source + 2 * (size_t)(istart - 1);
-        const double* chg = charge + (size_t)nd * (istart - 1);
-        const double* tg = targ + 2 * (size_t)(jstart - 1);
-        const size_t w = (size_t)nd * (ifpgh == 1 ? 1 : ifpgh == 2 ? 3 : 6);
-        double* d_src = W.get<double>("pd_src", 2 * (size_t)ns);
-        double* d_chg = W.get<double>("pd_chg", (size_t)nd * ns);
-        double* d_tg = W.get<double>("pd_tg", 2 * (size_t)nt);
-        double* d_p = W.get<double>("pd_pot", (size_t)nd * nt);
-        double* d_g = W.get<double>("pd_grad", 2 * (size_t)nd * nt);
-        double* d_h = W.get<double>("pd_hess", 3 * (size_t)nd * nt);
-        (void)w;
-        HIPCHECK(hipMemcpyAsync(d_src, src, 2 * (size_t)ns * 8, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d_chg, chg, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d_tg, tg, 2 * (size_t)nt * 8, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d_p, pot + (size_t)nd * (jstart - 1), (size_t)nd * nt * 8,
-                                hipMemcpyHostToDevice, s));
-        if (ifpgh >= 2)
-            HIPCHECK(hipMemcpyAsync(d_g, grad + 2 * (size_t)nd * (jstart - 1), 2 * (size_t)nd * nt * 8,
-                                    hipMemcpyHostToDevice, s));
-        if (ifpgh >= 3)
-            HIPCHECK(hipMemcpyAsync(d_h, hess + 3 * (size_t)nd * (jstart - 1), 3 * (size_t)nd * nt * 8,
-                                    hipMemcpyHostToDevice, s));
-        double dmax = std::log(1.0 / eps) * delta;
-        launch_direct_all(d_src, d_chg, nd, ns, d_tg, nt, delta, dmax, ifpgh, d_p, d_g, d_h, s);
-        HIPCHECK(hipGetLastError());
-        HIPCHECK(hipMemcpyAsync(pot + (size_t)nd * (jstart - 1), d_p, (size_t)nd * nt * 8,
-                                hipMemcpyDeviceToHost, s));
-        if (ifpgh >= 2)
-            HIPCHECK(hipMemcpyAsync(grad + 2 * (size_t)nd * (jstart - 1), d_g, 2 * (size_t)nd * nt * 8,
-                                    hipMemcpyDeviceToHost, s));
-        if (ifpgh >= 3)
-            HIPCHECK(hipMemcpyAsync(hess + 3 * (size_t)nd * (jstart - 1), d_h, 3 * (size_t)nd * nt * 8,
-                                    hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
-    });
+    if (!(eps > 0)) {
+        t_err = "eps must be positive";
+        return FGTPW_EINVAL;
+    }
+    return run_part_direct(2, nd, delta, std::log(1.0 / eps) * delta, istart, iend, jstart, jend,
+                           source, ifcharge, charge, ifdipole, targ, ifpgh, pot, grad, hess);
 }
 
 int fgtpw_last_tree2d(int* info, int cap, int* laddr, int* lev, int* par, int* nch, int* ch,
@@ -1165,6 +1999,7 @@ int fgtpw_last_tree2d(int* info, int cap, int* laddr, int* lev, int* par, int* n
         select_device();
         Workspace& W = workspace();
         if (!W.last_valid) fail(FGTPW_EINVAL, "no tree: the last call built none");
+        if (W.last_dim != 2) fail(FGTPW_EINVAL, "the last call was a 3D transform");
         complete_colleagues(W.last);
         const Tree2D& T = W.last;
         const int nb = T.nboxes();
@@ -1197,6 +2032,53 @@ int fgtpw_last_tree2d(int* info, int cap, int* laddr, int* lev, int* par, int* n
         // reference's stable partitions (sort_pts_to_children,
         // pts_tree2d4.f:594-689) keep them in input order.  Same sets per
         // leaf, so sorting each leaf's indices gives the reference order.
+        for (int b = 0; b < nb; ++b) {
+            if (T.nch[b] != 0) continue;
+            std::sort(isrc + isrcse[2 * b], isrc + isrcse[2 * b + 1]);
+            if (W.last_nt > 0) std::sort(itarg + itargse[2 * b], itarg + itargse[2 * b + 1]);
+        }
+    });
+}
+
+int fgtpw_last_tree3d(int* info, int cap, int* laddr, int* lev, int* par, int* nch, int* ch,
+                      int* ncol, int* col, double* cen, int* isrc, int* isrcse, int* itarg,
+                      int* itargse)
+{
+    return run_guarded([&] {
+        std::lock_guard<std::mutex> lk(g_mutex);
+        select_device();
+        Workspace& W = workspace();
+        if (!W.last_valid) fail(FGTPW_EINVAL, "no tree: the last call built none");
+        if (W.last_dim != 3) fail(FGTPW_EINVAL, "the last call was a 2D transform");
+        complete_colleagues(W.last3);
+        const Tree3D& T = W.last3;
+        const int nb = T.nboxes();
+        info[0] = T.nlevels;
+        info[1] = nb;
+        info[2] = (int)T.leaf_of_rank.size();
+        info[3] = T.nlmax;
+        if (!laddr || cap < nb) return;
+        std::memcpy(laddr, T.laddr.data(), sizeof(int) * T.laddr.size());
+        std::memcpy(lev, T.lev.data(), sizeof(int) * nb);
+        std::memcpy(par, T.par.data(), sizeof(int) * nb);
+        std::memcpy(nch, T.nch.data(), sizeof(int) * nb);
+        std::memcpy(ch, T.ch.data(), sizeof(int) * 8 * nb);
+        std::memcpy(ncol, T.ncol.data(), sizeof(int) * nb);
+        std::memcpy(col, T.col.data(), sizeof(int) * 27 * nb);
+        std::memcpy(cen, T.cen.data(), sizeof(double) * 3 * nb);
+        hipStream_t s = W.own;
+        HIPCHECK(hipMemcpyAsync(isrc, W.get<unsigned>("isrc", W.last_ns), sizeof(int) * W.last_ns,
+                                hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(isrcse, W.get<int2>("src_se", nb), sizeof(int2) * nb,
+                                hipMemcpyDeviceToHost, s));
+        if (W.last_nt > 0) {
+            HIPCHECK(hipMemcpyAsync(itarg, W.get<unsigned>("itarg", W.last_nt),
+                                    sizeof(int) * W.last_nt, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipMemcpyAsync(itargse, W.get<int2>("trg_se", nb), sizeof(int2) * nb,
+                                    hipMemcpyDeviceToHost, s));
+        }
+        HIPCHECK(hipStreamSynchronize(s));
+        // reference order inside each leaf (stable partitions keep input order)
         for (int b = 0; b < nb; ++b) {
             if (T.nch[b] != 0) continue;
             std::sort(isrc + isrcse[2 * b], isrc + isrcse[2 * b + 1]);

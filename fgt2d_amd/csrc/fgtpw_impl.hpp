@@ -57,15 +57,19 @@ struct LeafRun {            // one non-empty leaf of the adaptive tree before
     int count;              // source list, and its number of sources
 };
 
-struct Tree2D {
+// Box tree in the reference's itree layout (D = 2: quadtree, D = 3: octree)
+template <int D>
+struct TreeD {
+    static constexpr int CH = 1 << D;          // children per box
+    static constexpr int NC = D == 2 ? 9 : 27;  // colleague slots per box
     int nlmax = 0, npwlevel = 0, nlevels = 0;
-    double bs0 = 0, cen0[2] = {0, 0};
+    double bs0 = 0, cen0[D] = {};
     std::vector<double> bsz;   // box size per level
     std::vector<int> laddr;    // first, last box of each level
     std::vector<int> lev, par, nch, ch, ncol, col, flag;
-    std::vector<double> cen;
+    std::vector<double> cen;   // D per box
     std::vector<uint64_t> pre;  // Morton prefix of every box
-    std::vector<int> ix, iy;    // integer box coordinates within the level
+    std::vector<int> ic[D];     // integer box coordinates within the level, per axis
     int coll_levels = -1;       // colleague lists are valid for levels <= this
     int nboxes() const { return (int)lev.size(); }
     // leaves in Morton order and, per box, its interval of leaf ranks
@@ -74,10 +78,20 @@ struct Tree2D {
     std::vector<int> map, tail;
     std::vector<char> scratch;
 };
+using Tree2D = TreeD<2>;
+using Tree3D = TreeD<3>;
 
-void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int npwlevel,
-                  double bs0, const double* cen0);
-void complete_colleagues(Tree2D& T);
+template <int D>
+void build_tree(TreeD<D>& T, const std::vector<LeafRun>& leaves, int nlmax, int npwlevel,
+                double bs0, const double* cen0);
+template <int D>
+void complete_colleagues(TreeD<D>& T);
+inline void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int npwlevel,
+                         double bs0, const double* cen0)
+{
+    build_tree<2>(T, leaves, nlmax, npwlevel, bs0, cen0);
+}
+void planewave_nodes3(double pmax, int npw, double* ws, double* ts);
 
 // ---------------------------------------------------------------------------
 // device kernels (kernels2d.hip)
@@ -88,8 +102,10 @@ void launch_bbox(const double* a, int na, const double* b, int nb, double* part,
                  double* out, hipStream_t s);
 void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, int nlmax,
                    unsigned long long* key, unsigned* idx, hipStream_t s);
+// bits: key bits per level (2 in 2D, 3 in 3D)
 void launch_leafdepth(const unsigned long long* key, int n, int nlmax, int levcut, int ndiv0,
-                      int ndiv, int* level, unsigned char* start_flag, hipStream_t s);
+                      int ndiv, int* level, unsigned char* start_flag, hipStream_t s,
+                      int bits = 2);
 size_t sort_pairs_u64_temp(int n);
 void sort_pairs_u64(void* tmp, size_t tmpb, const unsigned long long* kin, unsigned long long* kout,
                     const unsigned* vin, unsigned* vout, int n, int endbit, hipStream_t s);
@@ -100,7 +116,7 @@ size_t select_flagged_temp(int n);
 void select_flagged(void* tmp, size_t tmpb, const unsigned char* flags, int n, int* out, int* nsel,
                     hipStream_t s);
 void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigned long long* key,
-                        const int* level, int nlmax, LeafRun* out, hipStream_t s);
+                        const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits = 2);
 void launch_descend_rank(const unsigned long long* key, const unsigned* idx, int n, int nlmax,
                          const int4* child, const int* rank_of_box, unsigned* rank_out,
                          hipStream_t s);
@@ -217,5 +233,89 @@ void launch_formlc(const double* xy, const double* q, int nd, int ind, int n, do
 void launch_direct_all(const double* src, const double* q, int nd, int ns, const double* tg,
                        int nt, double delta, double dmax, int ifpgh, double* pot, double* grad,
                        double* hess, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// 3D kernels (kernels3d.hip).  Points (3,n), expansions complex
+// (npw, npw, npw/2) per slot, ascending nodes.
+// ---------------------------------------------------------------------------
+void launch_bbox3(const double* a, int na, const double* b, int nb, double* part, int nblk,
+                  double* out, hipStream_t s);  // out: xmin,xmax,ymin,ymax,zmin,zmax,bad
+void launch_keys3d(const double* p, int n, const double* c0, double bs0, int nlmax,
+                   unsigned long long* key, unsigned* idx, hipStream_t s);
+void launch_descend_rank3(const unsigned long long* key, int n, int nlmax, const int* child8,
+                          const int* rank_of_box, unsigned* rank_out, hipStream_t s);
+void launch_gather3(const double* p, const double* q, int nd, const unsigned* perm, int n,
+                    double* ps, double* qs, hipStream_t s);
+
+struct Form3Args {
+    const int* boxes;       // leaves to form
+    int nbox;
+    const double* cen;      // 3 per box
+    const int2* src_se;
+    const int* slot;
+    const double* xs;       // sorted sources (3,n)
+    const double* qs;       // sorted charges (nd,n)
+    int nd, ind;
+    double dsq;
+    int p;                  // Hermite order
+    int npw;
+    const double* coef;     // complex C(j,a) = ws_a (-i ts_a)^j / j! at a + cstride*j
+    int cstride;
+    const double *ws, *ts;  // nodes (ascending)
+    double* mp;             // per slot (nd, nexp) complex
+    double* scratch;        // per block: m^3 + 2 m^2 (npw/2) doubles
+    long sstride;
+    int grid;
+    int thresh;             // Hermite route for boxes with more than thresh
+                            // sources, direct plane-wave route for the others
+};
+void launch_form3_hermite(const Form3Args& a, hipStream_t s);
+void launch_form3_pw(const Form3Args& a, hipStream_t s);
+
+void launch_merge3(const int* parents, int np, const int* child8, const double* cen,
+                   const int2* src_se, const int* slot, const double* wms, int kbase, int nd,
+                   long nexp, double* mp, hipStream_t s);
+void launch_m2l3(const int* boxes, int nb, const double* cen, const int* ncol, const int* col27,
+                 const int* nch, const int2* src_se, const int2* trg_se, int use_src, int use_trg,
+                 const int* slot, const double* wm2l, double xmin, int do_list, int nd, long nexp,
+                 const double* mp, double* loc, hipStream_t s);
+void launch_split3(const int* children, int nc, const int* par, const double* cen,
+                   const int* slot, const double* wms, int kbase, int nd, long nexp, double* loc,
+                   hipStream_t s);
+
+struct Eval3Args {
+    const int* boxes;
+    int nbox;
+    const double* cen;
+    const int2* se;         // point ranges of the boxes
+    const int* slot;
+    const double* pts;      // sorted points (3,n)
+    int nd, ind;
+    double dsq;
+    int ifpgh;
+    int nl;                 // Taylor order
+    const double* dcoef;    // complex D(j,a) = (i ts_a)^j / j! at a + cstride*j
+    int cstride;
+    const double* ts;       // nodes (ascending)
+    int npw;
+    const double* loc;      // PW local expansions per slot
+    double* taylor;         // (nbox, (nl+1)^3)
+    double *pot, *grad, *hess;
+    const int* tstart;      // segments of the point ranges (evaluation kernels)
+    int seg, grid;
+    // route: boxes whose source + target count is below thresh are evaluated
+    // from the plane waves directly, the others through Taylor
+    const int2 *rs, *rt;
+    int thresh;
+};
+void launch_pw2local3(const Eval3Args& a, hipStream_t s);
+void launch_taylor_eval3(const Eval3Args& a, hipStream_t s);
+void launch_pw_eval3(const Eval3Args& a, hipStream_t s);
+void launch_direct3(const DirectArgs& a, hipStream_t s);  // points (3,n)
+void launch_direct_all3(const double* src, const double* q, int nd, int ns, const double* tg,
+                        int nt, double delta, double dmax, int ifpgh, double* pot, double* grad,
+                        double* hess, hipStream_t s);
+void launch_formlc3(const double* p, const double* q, int nd, int ind, int n, const double* c,
+                    double dsq, int nl, double* part, int nparts, double* out, hipStream_t s);
 
 }  // namespace fgtpw

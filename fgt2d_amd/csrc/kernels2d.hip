@@ -151,7 +151,7 @@ void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, in
 // than lim points.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_leafdepth(const u64* __restrict__ key, int n, int nlmax,
-                                                   int levcut, int ndiv0, int ndiv,
+                                                   int levcut, int ndiv0, int ndiv, int bits,
                                                    int* __restrict__ level,
                                                    unsigned char* __restrict__ start)
 {
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_leafdepth(const u64* __restrict__ key, 
     const u64 ki = key[i];
     int L = 0;
     for (int l = nlmax - 1; l >= 0; --l) {
-        const int sh = 2 * (nlmax - l);
+        const int sh = bits * (nlmax - l);
         const u64 p = ki >> sh;
         const int lim = l < levcut ? ndiv0 : ndiv;
         // first index in [max(0, i - lim), i] with prefix p
@@ -183,16 +183,16 @@ __global__ __launch_bounds__(256) void k_leafdepth(const u64* __restrict__ key, 
         }
     }
     level[i] = L;
-    const int sh = 2 * (nlmax - L);
+    const int sh = bits * (nlmax - L);
     unsigned char st = 1;
     if (i > 0) st = sh >= 64 ? 0 : ((key[i - 1] >> sh) != (ki >> sh));
     start[i] = st;
 }
 
 void launch_leafdepth(const u64* key, int n, int nlmax, int levcut, int ndiv0, int ndiv,
-                      int* level, unsigned char* start_flag, hipStream_t s)
+                      int* level, unsigned char* start_flag, hipStream_t s, int bits)
 {
-    k_leafdepth<<<grid_for(n, 256), 256, 0, s>>>(key, n, nlmax, levcut, ndiv0, ndiv, level,
+    k_leafdepth<<<grid_for(n, 256), 256, 0, s>>>(key, n, nlmax, levcut, ndiv0, ndiv, bits, level,
                                                  start_flag);
 }
 
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ sta
                                                      const int* __restrict__ nsel, int n,
                                                      const u64* __restrict__ key,
                                                      const int* __restrict__ level, int nlmax,
-                                                     LeafRun* __restrict__ out)
+                                                     int bits, LeafRun* __restrict__ out)
 {
     int r = blockIdx.x * 256 + threadIdx.x;
     int m = *nsel;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ sta
     int e = r + 1 < m ? starts[r + 1] : n;
     int L = level[s];
     LeafRun o;
-    o.key = L == 0 ? 0ull : (key[s] >> (2 * (nlmax - L)));
+    o.key = L == 0 ? 0ull : (key[s] >> (bits * (nlmax - L)));
     o.level = L;
     o.start = s;
     o.count = e - s;
@@ -217,9 +217,9 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ sta
 }
 
 void launch_emit_leaves(const int* starts, const int* nsel, int n, const u64* key,
-                        const int* level, int nlmax, LeafRun* out, hipStream_t s)
+                        const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits)
 {
-    k_emit_leaves<<<grid_for(n, 256), 256, 0, s>>>(starts, nsel, n, key, level, nlmax, out);
+    k_emit_leaves<<<grid_for(n, 256), 256, 0, s>>>(starts, nsel, n, key, level, nlmax, bits, out);
 }
 
 // ---------------------------------------------------------------------------

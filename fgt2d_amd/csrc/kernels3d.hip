@@ -1,0 +1,1130 @@
+// HIP kernels of the 3D plane-wave fast Gauss transform for gfx950.
+//
+// Layouts (HBM, tree order):
+//   points      double[3*n]          x,y,z interleaved (the reference's (3,n))
+//   densities   double[nd*n]
+//   expansions  double[2*nexp*nd]    per slot, complex interleaved, index
+//                                    a + npw*(b + npw*c): a, b all nodes on
+//                                    x, y; c the first npw/2 nodes on z, as
+//                                    pwexp(npw,npw,npw/2,nd) (g3drouts.f)
+//   nodes       ascending, ts_a = (a - npw/2 + 1/2) h (g3drouts.f:2669-2690);
+//               the mirror of node a is npw-1-a
+//
+// 256-thread blocks (4 waves).  FP64 throughout.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "fgtpw_impl.hpp"
+
+namespace fgtpw {
+
+using u64 = unsigned long long;
+
+static inline unsigned gridn(long n, int bs)
+{
+    long g = (n + bs - 1) / bs;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ---------------------------------------------------------------------------
+// bounding box (pts_tree_boxsize0, pts_tree3d.f:1597-1680): mins and maxes
+// of x, y, z over sources and targets, plus a non-finite flag
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bbox3(const double* __restrict__ a, int na,
+                                               const double* __restrict__ b, int nb,
+                                               double* __restrict__ part)
+{
+    __shared__ double s[7][256];
+    double v[7] = {HUGE_VAL, -HUGE_VAL, HUGE_VAL, -HUGE_VAL, HUGE_VAL, -HUGE_VAL, 0};
+    long n = (long)na + nb;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double* p = i < na ? a + 3 * i : b + 3 * (i - na);
+        for (int d = 0; d < 3; ++d) {
+            double x = p[d];
+            if (!isfinite(x)) v[6] = 1;
+            v[2 * d] = fmin(v[2 * d], x);
+            v[2 * d + 1] = fmax(v[2 * d + 1], x);
+        }
+    }
+    int t = threadIdx.x;
+    for (int k = 0; k < 7; ++k) s[k][t] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 7; ++k)
+                s[k][t] = (k < 6 && !(k & 1)) ? fmin(s[k][t], s[k][t + w]) : fmax(s[k][t], s[k][t + w]);
+        __syncthreads();
+    }
+    if (t < 7) part[7 * blockIdx.x + t] = s[t][0];
+}
+
+__global__ __launch_bounds__(256) void k_bbox3_final(const double* __restrict__ part, int nblk,
+                                                     double* __restrict__ out)
+{
+    __shared__ double s[7][256];
+    int t = threadIdx.x;
+    double v[7] = {HUGE_VAL, -HUGE_VAL, HUGE_VAL, -HUGE_VAL, HUGE_VAL, -HUGE_VAL, 0};
+    for (int i = t; i < nblk; i += 256)
+        for (int k = 0; k < 7; ++k)
+            v[k] = (k < 6 && !(k & 1)) ? fmin(v[k], part[7 * i + k]) : fmax(v[k], part[7 * i + k]);
+    for (int k = 0; k < 7; ++k) s[k][t] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 7; ++k)
+                s[k][t] = (k < 6 && !(k & 1)) ? fmin(s[k][t], s[k][t + w]) : fmax(s[k][t], s[k][t + w]);
+        __syncthreads();
+    }
+    if (t < 7) out[t] = s[t][0];
+}
+
+void launch_bbox3(const double* a, int na, const double* b, int nb, double* part, int nblk,
+                  double* out, hipStream_t s)
+{
+    k_bbox3<<<nblk, 256, 0, s>>>(a, na, b, nb, part);
+    k_bbox3_final<<<1, 256, 0, s>>>(part, nblk, out);
+}
+
+// ---------------------------------------------------------------------------
+// full-depth 3-bit Morton keys with the reference's FP64 centre recursion and
+// "p - c < 0" tests (sort_pts_to_children, pts_tree3d.f:662-854;
+// tree_refine_boxes, tree_routs3d.f:28-46).  Digit = x + 2y + 4z.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_keys3d(const double* __restrict__ p, int n, double cx0,
+                                                double cy0, double cz0, double bs0, int nlmax,
+                                                u64* __restrict__ key, unsigned* __restrict__ idx)
+{
+#pragma clang fp contract(off)
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double x = p[3 * (long)i], y = p[3 * (long)i + 1], z = p[3 * (long)i + 2];
+    double cx = cx0, cy = cy0, cz = cz0, bs = bs0;
+    u64 k = 0;
+    for (int l = 0; l < nlmax; ++l) {
+        bs = bs / 2;
+        double h = bs / 2;
+        int xb = !(x - cx < 0.0), yb = !(y - cy < 0.0), zb = !(z - cz < 0.0);
+        k = (k << 3) | (u64)(xb | (yb << 1) | (zb << 2));
+        cx = cx + (xb ? h : -h);
+        cy = cy + (yb ? h : -h);
+        cz = cz + (zb ? h : -h);
+    }
+    key[i] = k;
+    if (idx) idx[i] = (unsigned)i;
+}
+
+void launch_keys3d(const double* p, int n, const double* c0, double bs0, int nlmax, u64* key,
+                   unsigned* idx, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_keys3d<<<gridn(n, 256), 256, 0, s>>>(p, n, c0[0], c0[1], c0[2], bs0, nlmax, key, idx);
+}
+
+// leaf rank of each point by walking its key digits down the final octree
+__global__ __launch_bounds__(256) void k_descend_rank3(const u64* __restrict__ key, int n,
+                                                       int nlmax, const int* __restrict__ child,
+                                                       const int* __restrict__ rank_of_box,
+                                                       unsigned* __restrict__ rank_out)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    u64 k = key[i];
+    int b = 0;
+    for (int l = 0; l < nlmax; ++l) {
+        int c0 = child[8 * b];
+        if (c0 < 0) break;
+        int d = (int)((k >> (3 * (nlmax - 1 - l))) & 7);
+        b = child[8 * b + d];
+    }
+    rank_out[i] = (unsigned)rank_of_box[b];
+}
+
+void launch_descend_rank3(const u64* key, int n, int nlmax, const int* child8,
+                          const int* rank_of_box, unsigned* rank_out, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_descend_rank3<<<gridn(n, 256), 256, 0, s>>>(key, n, nlmax, child8, rank_of_box, rank_out);
+}
+
+__global__ __launch_bounds__(256) void k_gather3(const double* __restrict__ p,
+                                                 const double* __restrict__ q, int nd,
+                                                 const unsigned* __restrict__ perm, int n,
+                                                 double* __restrict__ ps, double* __restrict__ qs)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    long j = perm[i];
+    ps[3 * (long)i] = p[3 * j];
+    ps[3 * (long)i + 1] = p[3 * j + 1];
+    ps[3 * (long)i + 2] = p[3 * j + 2];
+    if (q)
+        for (int d = 0; d < nd; ++d) qs[(long)nd * i + d] = q[(long)nd * j + d];
+}
+
+void launch_gather3(const double* p, const double* q, int nd, const unsigned* perm, int n,
+                    double* ps, double* qs, hipStream_t s)
+{
+    if (n <= 0) return;
+    kmark(KID_GATHER, false, s);
+    k_gather3<<<gridn(n, 256), 256, 0, s>>>(p, q, nd, perm, n, ps, qs);
+    kmark(KID_GATHER, true, s);
+}
+
+// ---------------------------------------------------------------------------
+// Step 1, Hermite route (g3dformhc_vec g3drouts.f:109-195 followed by
+// g3dh2pw_real_vec :1565-1690).  With raw moments M(j,k,l) = sum q x^j y^k
+// z^l (x = (s - c)/sqrt(delta)) and C(j,a) = ws_a (-i ts_a)^j / j! -- the
+// reference's ws (-ts)^j/sqrt(j!) times its i^j grouping, times the
+// 1/sqrt(j!) of its Hermite normalisation --
+//     PW(a,b,c) = sum_{j,k,l} M(j,k,l) C(j,a) C(k,b) C(l,c).
+// One block per leaf (grid-strided), four phases through a per-block slice of
+// global scratch (L2-resident): moments; N_c(j,k) = sum_l M C(l,c) for every
+// z node c of the first half; then per c, P(a,k) = sum_j C(j,a) N_c(j,k)
+// (LDS) and PW(a,b,c) = sum_k C(k,b) P(a,k).
+// ---------------------------------------------------------------------------
+constexpr int F3_CHUNK = 32;  // sources staged per pass
+constexpr int F3_MAXM = 32;   // Hermite orders up to 31
+
+__global__ __launch_bounds__(256) void k_form3_hermite(Form3Args a)
+{
+    extern __shared__ double sm[];
+    const int m = a.p + 1, npw = a.npw, half = npw / 2, t = threadIdx.x;
+    const long nexp = (long)npw * npw * half;
+    double* Mg = a.scratch + (long)blockIdx.x * a.sstride;    // m^3
+    double* Ng = Mg + (long)m * m * m;                        // complex (half, m, m)
+    for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
+        const int box = a.boxes[li];
+        const int2 r = a.src_se[box];
+        if (r.y - r.x <= a.thresh) continue;  // empty, or on the direct plane-wave route
+        const double cx = a.cen[3 * box], cy = a.cen[3 * box + 1], cz = a.cen[3 * box + 2];
+        // ---- moments: thread per (k,l), all j in registers ----
+        double* xp = sm;                          // [F3_CHUNK][m]
+        double* yp = sm + F3_CHUNK * m;
+        double* zq = sm + 2 * F3_CHUNK * m;       // q z^l
+        for (int p0 = 0; p0 < m * m; p0 += 256) {
+            const int pr = p0 + t, k = pr % m, l = pr / m;
+            const bool act = pr < m * m;
+            double acc[F3_MAXM];
+#pragma unroll
+            for (int j = 0; j < F3_MAXM; ++j) acc[j] = 0;
+            for (int s0 = r.x; s0 < r.y; s0 += F3_CHUNK) {
+                const int cn = min(F3_CHUNK, r.y - s0);
+                __syncthreads();
+                if (t < cn) {
+                    long i = s0 + t;
+                    double x = (a.xs[3 * i] - cx) * a.dsq, y = (a.xs[3 * i + 1] - cy) * a.dsq,
+                           z = (a.xs[3 * i + 2] - cz) * a.dsq;
+                    double px = 1, py = 1, pz = a.qs[(long)a.nd * i + a.ind];
+                    for (int j = 0; j < m; ++j) {
+                        xp[t * m + j] = px;
+                        yp[t * m + j] = py;
+                        zq[t * m + j] = pz;
+                        px *= x;
+                        py *= y;
+                        pz *= z;
+                    }
+                }
+                __syncthreads();
+                if (act)
+                    for (int s = 0; s < cn; ++s) {
+                        const double w = yp[s * m + k] * zq[s * m + l];
+                        const double* xr = xp + s * m;
+#pragma unroll
+                        for (int j = 0; j < F3_MAXM; ++j)
+                            if (j < m) acc[j] += w * xr[j];
+                    }
+            }
+            if (act)
+#pragma unroll
+                for (int j = 0; j < F3_MAXM; ++j)
+                    if (j < m) Mg[j + m * (k + m * l)] = acc[j];
+        }
+        __syncthreads();
+        // ---- C table in LDS: complex (npw, m), index a + npw*j ----
+        double* Cs = sm;
+        for (int o = t; o < npw * m; o += 256) {
+            Cs[2 * o] = a.coef[2 * (o % npw + a.cstride * (o / npw))];
+            Cs[2 * o + 1] = a.coef[2 * (o % npw + a.cstride * (o / npw)) + 1];
+        }
+        __syncthreads();
+        // ---- N_c(j,k) = sum_l M(j,k,l) C(l,c), c < half ----
+        for (int pr = t; pr < m * m; pr += 256) {
+            for (int c0 = 0; c0 < half; c0 += 8) {
+                double nr[8], ni[8];
+#pragma unroll
+                for (int cc = 0; cc < 8; ++cc) nr[cc] = ni[cc] = 0;
+                for (int l = 0; l < m; ++l) {
+                    const double mv = Mg[pr + (long)m * m * l];
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc) {
+                        if (c0 + cc < half) {
+                            nr[cc] += mv * Cs[2 * (c0 + cc + npw * l)];
+                            ni[cc] += mv * Cs[2 * (c0 + cc + npw * l) + 1];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int cc = 0; cc < 8; ++cc)
+                    if (c0 + cc < half) {
+                        Ng[2 * (pr + (long)m * m * (c0 + cc))] = nr[cc];
+                        Ng[2 * (pr + (long)m * m * (c0 + cc)) + 1] = ni[cc];
+                    }
+            }
+        }
+        __syncthreads();
+        double* Ps = Cs + 2 * npw * m;  // complex (npw, m): P(a,k)
+        double* out = a.mp + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+        for (int c = 0; c < half; ++c) {
+            const double* Nc = Ng + 2L * m * m * c;
+            for (int o = t; o < npw * m; o += 256) {
+                const int aa = o % npw, k = o / npw;
+                double pr_ = 0, pi_ = 0;
+                for (int j = 0; j < m; ++j) {
+                    const double cr = Cs[2 * (aa + npw * j)], ci = Cs[2 * (aa + npw * j) + 1];
+                    const double nr = Nc[2 * (j + m * k)], ni = Nc[2 * (j + m * k) + 1];
+                    pr_ += cr * nr - ci * ni;
+                    pi_ += cr * ni + ci * nr;
+                }
+                Ps[2 * o] = pr_;
+                Ps[2 * o + 1] = pi_;
+            }
+            __syncthreads();
+            for (int o = t; o < npw * npw; o += 256) {
+                const int aa = o % npw, b = o / npw;
+                double vr = 0, vi = 0;
+                for (int k = 0; k < m; ++k) {
+                    const double cr = Cs[2 * (b + npw * k)], ci = Cs[2 * (b + npw * k) + 1];
+                    const double pr_ = Ps[2 * (aa + npw * k)], pi_ = Ps[2 * (aa + npw * k) + 1];
+                    vr += cr * pr_ - ci * pi_;
+                    vi += cr * pi_ + ci * pr_;
+                }
+                double* d = out + 2 * (aa + (long)npw * (b + (long)npw * c));
+                d[0] = vr;
+                d[1] = vi;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+void launch_form3_hermite(const Form3Args& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return;
+    const int m = a.p + 1;
+    if (m > F3_MAXM) fail(-3, "3D Hermite order above 31 is not supported");
+    size_t lds = std::max((size_t)3 * F3_CHUNK * m, (size_t)4 * a.npw * m) * sizeof(double);
+    kmark(KID_FORM_MOMENTS, false, s);
+    k_form3_hermite<<<a.grid, 256, lds, s>>>(a);
+    kmark(KID_FORM_MOMENTS, true, s);
+}
+
+// Step 1, direct plane-wave route (g3dformpwc_vec, g3drouts.f:1950-2040):
+// PW(a,b,c) = sum_s q ws_a ws_b ws_c e^{-i(ts_a x + ts_b y + ts_c z)}.  One
+// block per leaf; thread per (a,b), z nodes of the first half in chunks of 8.
+constexpr int P3_CHUNK = 16;
+
+__global__ __launch_bounds__(256) void k_form3_pw(Form3Args a)
+{
+    extern __shared__ double sm[];
+    const int npw = a.npw, half = npw / 2, t = threadIdx.x;
+    const long nexp = (long)npw * npw * half;
+    double* ex = sm;                       // [chunk][npw] complex
+    double* ey = sm + 2 * P3_CHUNK * npw;
+    double* ez = sm + 4 * P3_CHUNK * npw;  // [chunk][half] complex, q folded in
+    for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
+        const int box = a.boxes[li];
+        const int2 r = a.src_se[box];
+        if (r.y <= r.x || r.y - r.x > a.thresh) continue;  // empty, or on the Hermite route
+        const double c3[3] = {a.cen[3 * box], a.cen[3 * box + 1], a.cen[3 * box + 2]};
+        double* out = a.mp + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+        for (int p0 = 0; p0 < npw * npw; p0 += 256) {
+            const int pr = p0 + t, aa = pr % npw, b = pr / npw;
+            const bool act = pr < npw * npw;
+            for (int c0 = 0; c0 < half; c0 += 8) {
+                double vr[8], vi[8];
+#pragma unroll
+                for (int cc = 0; cc < 8; ++cc) vr[cc] = vi[cc] = 0;
+                for (int s0 = r.x; s0 < r.y; s0 += P3_CHUNK) {
+                    const int cn = min(P3_CHUNK, r.y - s0);
+                    __syncthreads();
+                    for (int o = t; o < cn * npw; o += 256) {
+                        const int s = o / npw, node = o % npw;
+                        const long i = s0 + s;
+                        const double q = a.qs[(long)a.nd * i + a.ind];
+                        for (int d = 0; d < 3; ++d) {
+                            if (d == 2 && node >= half) continue;
+                            const double u = (a.xs[3 * i + d] - c3[d]) * a.dsq;
+                            double sn, cs;
+                            sincos(-a.ts[node] * u, &sn, &cs);
+                            const double w = a.ws[node] * (d == 2 ? q : 1.0);
+                            double* e = d == 0 ? ex + 2 * (s * npw + node)
+                                      : d == 1 ? ey + 2 * (s * npw + node) : ez + 2 * (s * half + node);
+                            e[0] = cs * w;
+                            e[1] = sn * w;
+                        }
+                    }
+                    __syncthreads();
+                    if (act)
+                        for (int s = 0; s < cn; ++s) {
+                            const double xr = ex[2 * (s * npw + aa)], xi = ex[2 * (s * npw + aa) + 1];
+                            const double yr = ey[2 * (s * npw + b)], yi = ey[2 * (s * npw + b) + 1];
+                            const double tr = xr * yr - xi * yi, ti = xr * yi + xi * yr;
+#pragma unroll
+                            for (int cc = 0; cc < 8; ++cc)
+                                if (c0 + cc < half) {
+                                    const double zr = ez[2 * (s * half + c0 + cc)],
+                                                 zi = ez[2 * (s * half + c0 + cc) + 1];
+                                    vr[cc] += tr * zr - ti * zi;
+                                    vi[cc] += tr * zi + ti * zr;
+                                }
+                        }
+                }
+                if (act)
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc)
+                        if (c0 + cc < half) {
+                            double* d = out + 2 * (aa + (long)npw * (b + (long)npw * (c0 + cc)));
+                            d[0] = vr[cc];
+                            d[1] = vi[cc];
+                        }
+            }
+        }
+    }
+}
+
+void launch_form3_pw(const Form3Args& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return;
+    size_t lds = (size_t)2 * P3_CHUNK * (2 * a.npw + a.npw / 2) * sizeof(double);
+    kmark(KID_FORM_VALU, false, s);
+    k_form3_pw<<<a.grid, 256, lds, s>>>(a);
+    kmark(KID_FORM_VALU, true, s);
+}
+
+// ---------------------------------------------------------------------------
+// Steps 2-4: diagonal plane-wave translations (g3dshiftpw_vec,
+// g3drouts.f:2844-2882).  Octant of a shift by its sign pattern, fgt3d.f:1120
+// -1136: 0 (+,+,+), 1 (+,-,+), 2 (-,+,+), 3 (-,-,+), 4..7 the same with z<0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int octant(double dx, double dy, double dz)
+{
+    int k = dx > 0 ? (dy > 0 ? 0 : 1) : (dy > 0 ? 2 : 3);
+    return dz > 0 ? k : k + 4;
+}
+
+__global__ __launch_bounds__(256) void k_merge3(const int* __restrict__ parents, int np,
+                                                const int* __restrict__ child,
+                                                const double* __restrict__ cen,
+                                                const int2* __restrict__ src_se,
+                                                const int* __restrict__ slot,
+                                                const double* __restrict__ wms, int kbase, int nd,
+                                                long nexp, double* __restrict__ mp)
+{
+    long t = blockIdx.x * 256L + threadIdx.x;
+    long pi = t / nexp, e = t - pi * nexp;
+    if (pi >= np) return;
+    int p = parents[pi];
+    for (int ind = 0; ind < nd; ++ind) {
+        double* dst = mp + 2 * (((long)slot[p] * nd + ind) * nexp + e);
+        double re = dst[0], im = dst[1];
+        for (int j = 0; j < 8; ++j) {
+            int c = child[8 * p + j];
+            if (c < 0) continue;
+            int2 r = src_se[c];
+            if (r.y - r.x <= 0) continue;
+            int k = octant(cen[3 * p] - cen[3 * c], cen[3 * p + 1] - cen[3 * c + 1],
+                           cen[3 * p + 2] - cen[3 * c + 2]);
+            const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
+            const double* v = mp + 2 * (((long)slot[c] * nd + ind) * nexp + e);
+            re += v[0] * w[0] - v[1] * w[1];
+            im += v[0] * w[1] + v[1] * w[0];
+        }
+        dst[0] = re;
+        dst[1] = im;
+    }
+}
+
+void launch_merge3(const int* parents, int np, const int* child8, const double* cen,
+                   const int2* src_se, const int* slot, const double* wms, int kbase, int nd,
+                   long nexp, double* mp, hipStream_t s)
+{
+    if (np <= 0) return;
+    kmark(KID_MERGE, false, s);
+    k_merge3<<<gridn((long)np * nexp, 256), 256, 0, s>>>(parents, np, child8, cen, src_se, slot,
+                                                         wms, kbase, nd, nexp, mp);
+    kmark(KID_MERGE, true, s);
+}
+
+// Step 3 (fgt3d.f:1162-1210): at the cutoff level, loc = sum over colleagues
+// j != self that have children (gt3d_computelistpw, tree_routs3d.f:695) of
+// mp_j shifted by (jx, jy, jz) boxes, then + mp_self; only boxes holding
+// evaluation points.
+__global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int nb,
+                                              const double* __restrict__ cen,
+                                              const int* __restrict__ ncol,
+                                              const int* __restrict__ col,
+                                              const int* __restrict__ nch,
+                                              const int2* __restrict__ src_se,
+                                              const int2* __restrict__ trg_se, int use_src,
+                                              int use_trg, const int* __restrict__ slot,
+                                              const double* __restrict__ wm2l, double xmin,
+                                              int do_list, int nd, long nexp,
+                                              const double* __restrict__ mp,
+                                              double* __restrict__ loc)
+{
+    long t = blockIdx.x * 256L + threadIdx.x;
+    long bi = t / nexp, e = t - bi * nexp;
+    if (bi >= nb) return;
+    int b = boxes[bi];
+    int npts = 0;
+    if (use_trg) npts += trg_se[b].y - trg_se[b].x;
+    if (use_src) npts += src_se[b].y - src_se[b].x;
+    if (npts <= 0) return;
+    for (int ind = 0; ind < nd; ++ind) {
+        double* d = loc + 2 * (((long)slot[b] * nd + ind) * nexp + e);
+        double re = d[0], im = d[1];
+        if (do_list)
+            for (int i = 0; i < ncol[b]; ++i) {
+                int j = col[27 * b + i];
+                if (j == b || nch[j] <= 0) continue;
+                int jx = (int)rint((cen[3 * b] - cen[3 * j]) / xmin);
+                int jy = (int)rint((cen[3 * b + 1] - cen[3 * j + 1]) / xmin);
+                int jz = (int)rint((cen[3 * b + 2] - cen[3 * j + 2]) / xmin);
+                const double* w = wm2l + 2 * ((long)((jx + 1) + 3 * (jy + 1) + 9 * (jz + 1)) * nexp + e);
+                const double* v = mp + 2 * (((long)slot[j] * nd + ind) * nexp + e);
+                re += v[0] * w[0] - v[1] * w[1];
+                im += v[0] * w[1] + v[1] * w[0];
+            }
+        const double* v = mp + 2 * (((long)slot[b] * nd + ind) * nexp + e);
+        d[0] = re + v[0];
+        d[1] = im + v[1];
+    }
+}
+
+void launch_m2l3(const int* boxes, int nb, const double* cen, const int* ncol, const int* col27,
+                 const int* nch, const int2* src_se, const int2* trg_se, int use_src, int use_trg,
+                 const int* slot, const double* wm2l, double xmin, int do_list, int nd, long nexp,
+                 const double* mp, double* loc, hipStream_t s)
+{
+    if (nb <= 0) return;
+    kmark(KID_M2L, false, s);
+    k_m2l3<<<gridn((long)nb * nexp, 256), 256, 0, s>>>(boxes, nb, cen, ncol, col27, nch, src_se,
+                                                       trg_se, use_src, use_trg, slot, wm2l, xmin,
+                                                       do_list, nd, nexp, mp, loc);
+    kmark(KID_M2L, true, s);
+}
+
+// Step 4 (fgt3d.f:1215-1276): loc(child) += loc(parent) shifted to the child.
+__global__ __launch_bounds__(256) void k_split3(const int* __restrict__ children, int nc,
+                                                const int* __restrict__ par,
+                                                const double* __restrict__ cen,
+                                                const int* __restrict__ slot,
+                                                const double* __restrict__ wms, int kbase, int nd,
+                                                long nexp, double* __restrict__ loc)
+{
+    long t = blockIdx.x * 256L + threadIdx.x;
+    long ci = t / nexp, e = t - ci * nexp;
+    if (ci >= nc) return;
+    int c = children[ci], p = par[c];
+    int k = octant(cen[3 * c] - cen[3 * p], cen[3 * c + 1] - cen[3 * p + 1],
+                   cen[3 * c + 2] - cen[3 * p + 2]);
+    const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
+    for (int ind = 0; ind < nd; ++ind) {
+        const double* v = loc + 2 * (((long)slot[p] * nd + ind) * nexp + e);
+        double* d = loc + 2 * (((long)slot[c] * nd + ind) * nexp + e);
+        d[0] += v[0] * w[0] - v[1] * w[1];
+        d[1] += v[0] * w[1] + v[1] * w[0];
+    }
+}
+
+void launch_split3(const int* children, int nc, const int* par, const double* cen,
+                   const int* slot, const double* wms, int kbase, int nd, long nexp, double* loc,
+                   hipStream_t s)
+{
+    if (nc <= 0) return;
+    kmark(KID_SPLIT, false, s);
+    k_split3<<<gridn((long)nc * nexp, 256), 256, 0, s>>>(children, nc, par, cen, slot, wms, kbase,
+                                                         nd, nexp, loc);
+    kmark(KID_SPLIT, true, s);
+}
+
+// ---------------------------------------------------------------------------
+// Step 5, Taylor route (g3dpw2local_real_vec g3drouts.f:1774-1945, then
+// g3dlevalp/g/h_vec :1045-1365).  With D(j,a) = (i ts_a)^j / j!,
+//   L(j3,j2,j1) = 2 Re sum_{c<half} D(j1,c) sum_b D(j2,b) sum_a D(j3,a) PW(a,b,c).
+// One block per leaf; per z node c: S1(j3,b) in LDS, then each thread owns
+// (j3,j2) pairs and keeps L(j3,j2,.) in registers across c.
+// ---------------------------------------------------------------------------
+constexpr int L3_MAXM = 32;
+
+// true when box b is evaluated from its plane waves directly (fewer than
+// thresh sources + targets, fgt3d.f:1321-1324)
+__device__ __forceinline__ bool pw_box(const Eval3Args& a, int b)
+{
+    if (a.thresh <= 0) return false;
+    const int n = (a.rs[b].y - a.rs[b].x) + (a.rt[b].y - a.rt[b].x);
+    return n < a.thresh;
+}
+
+__global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
+{
+    extern __shared__ double sm[];
+    const int m = a.nl + 1, npw = a.npw, half = npw / 2, t = threadIdx.x;
+    const long nexp = (long)npw * npw * half;
+    double* Ds = sm;                  // complex (npw, m): a + npw*j
+    double* S1 = sm + 2 * npw * m;    // complex (m, npw): j3 + m*b
+    for (int o = t; o < npw * m; o += 256) {
+        Ds[2 * o] = a.dcoef[2 * (o % npw + a.cstride * (o / npw))];
+        Ds[2 * o + 1] = a.dcoef[2 * (o % npw + a.cstride * (o / npw)) + 1];
+    }
+    for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
+        const int box = a.boxes[li];
+        if (pw_box(a, box)) continue;
+        const double* pw = a.loc + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+        double* L = a.taylor + (long)li * m * m * m;
+        for (int p0 = 0; p0 < m * m; p0 += 256) {
+            const int pr = p0 + t, j3 = pr % m, j2 = pr / m;
+            const bool act = pr < m * m;
+            double acc[L3_MAXM];
+#pragma unroll
+            for (int j = 0; j < L3_MAXM; ++j) acc[j] = 0;
+            for (int c = 0; c < half; ++c) {
+                __syncthreads();
+                for (int o = t; o < m * npw; o += 256) {
+                    const int jj = o % m, b = o / m;
+                    double sr = 0, si = 0;
+                    const double* row = pw + 2 * ((long)npw * (b + (long)npw * c));
+                    for (int aa = 0; aa < npw; ++aa) {
+                        const double dr = Ds[2 * (aa + npw * jj)], di = Ds[2 * (aa + npw * jj) + 1];
+                        const double pr_ = row[2 * aa], pi_ = row[2 * aa + 1];
+                        sr += dr * pr_ - di * pi_;
+                        si += dr * pi_ + di * pr_;
+                    }
+                    S1[2 * o] = sr;
+                    S1[2 * o + 1] = si;
+                }
+                __syncthreads();
+                if (act) {
+                    double s2r = 0, s2i = 0;
+                    for (int b = 0; b < npw; ++b) {
+                        const double dr = Ds[2 * (b + npw * j2)], di = Ds[2 * (b + npw * j2) + 1];
+                        const double xr = S1[2 * (j3 + m * b)], xi = S1[2 * (j3 + m * b) + 1];
+                        s2r += dr * xr - di * xi;
+                        s2i += dr * xi + di * xr;
+                    }
+                    // 2 Re(D(j1,c) S2)
+#pragma unroll
+                    for (int j1 = 0; j1 < L3_MAXM; ++j1)
+                        if (j1 < m) {
+                            const double dr = Ds[2 * (c + npw * j1)], di = Ds[2 * (c + npw * j1) + 1];
+                            acc[j1] += 2 * (dr * s2r - di * s2i);
+                        }
+                }
+            }
+            if (act)
+#pragma unroll
+                for (int j1 = 0; j1 < L3_MAXM; ++j1)
+                    if (j1 < m) L[j3 + m * (j2 + (long)m * j1)] = acc[j1];
+        }
+        __syncthreads();
+    }
+}
+
+void launch_pw2local3(const Eval3Args& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return;
+    const int m = a.nl + 1;
+    if (m > L3_MAXM) fail(-3, "3D Taylor order above 31 is not supported");
+    size_t lds = (size_t)4 * a.npw * m * sizeof(double);
+    kmark(KID_PW2LOCAL, false, s);
+    k_pw2local3<<<a.grid, 256, lds, s>>>(a);
+    kmark(KID_PW2LOCAL, true, s);
+}
+
+// Taylor evaluation over 256-point segments: sum L(j3,j2,j1) x^j3 y^j2 z^j1
+// and its derivatives (Horner in x, explicit powers in y and z); outputs
+// accumulated; Hessian order xx, yy, zz, xy, xz, yz.
+template <int PGH>
+__global__ __launch_bounds__(256) void k_taylor_eval3(Eval3Args a)
+{
+    const int nl = a.nl, m = nl + 1, t = threadIdx.x;
+    const int total = a.tstart[a.nbox];
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        int lo = 0, hi = a.nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (a.tstart[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        const int b = a.boxes[lo];
+        if (pw_box(a, b)) continue;
+        const int2 rr = a.se[b];
+        const int p0 = rr.x + (g - a.tstart[lo]) * a.seg, p1 = min(rr.y, p0 + a.seg);
+        const double* L = a.taylor + (long)lo * m * m * m;
+        const double cx = a.cen[3 * b], cy = a.cen[3 * b + 1], cz = a.cen[3 * b + 2], dsq = a.dsq;
+        for (int i = p0 + t; i < p1; i += 256) {
+            const double x = (a.pts[3 * (long)i] - cx) * dsq, y = (a.pts[3 * (long)i + 1] - cy) * dsq,
+                         z = (a.pts[3 * (long)i + 2] - cz) * dsq;
+            double pot = 0, gx = 0, gy = 0, gz = 0, hxx = 0, hyy = 0, hzz = 0, hxy = 0, hxz = 0,
+                   hyz = 0;
+            double zp = 1, zpm1 = 0, zpm2 = 0;  // z^j1, j1 z^(j1-1), j1(j1-1) z^(j1-2)
+            for (int j1 = 0; j1 < m; ++j1) {
+                double A = 0, Ay = 0, Ayy = 0, B = 0, By = 0, Cx = 0;
+                double yp = 1, ypm1 = 0, ypm2 = 0;
+                for (int j2 = 0; j2 < m; ++j2) {
+                    const double* c = L + (long)m * (j2 + (long)m * j1);
+                    double q = c[nl], q1 = 0, q2 = 0;
+                    for (int j3 = nl - 1; j3 >= 0; --j3) {
+                        if (PGH >= 3) q2 = q2 * x + q1;
+                        if (PGH >= 2) q1 = q1 * x + q;
+                        q = q * x + c[j3];
+                    }
+                    A += yp * q;
+                    if (PGH >= 2) {
+                        Ay += ypm1 * q;
+                        B += yp * q1;
+                    }
+                    if (PGH >= 3) {
+                        Ayy += ypm2 * q;
+                        By += ypm1 * q1;
+                        Cx += yp * 2 * q2;
+                    }
+                    ypm2 = (PGH >= 3) ? ypm2 * y + 2 * ypm1 : 0;
+                    ypm1 = ypm1 * y + yp;
+                    yp = yp * y;
+                }
+                pot += zp * A;
+                if (PGH >= 2) {
+                    gx += zp * B;
+                    gy += zp * Ay;
+                    gz += zpm1 * A;
+                }
+                if (PGH >= 3) {
+                    hxx += zp * Cx;
+                    hyy += zp * Ayy;
+                    hzz += zpm2 * A;
+                    hxy += zp * By;
+                    hxz += zpm1 * B;
+                    hyz += zpm1 * Ay;
+                }
+                zpm2 = (PGH >= 3) ? zpm2 * z + 2 * zpm1 : 0;
+                zpm1 = zpm1 * z + zp;
+                zp = zp * z;
+            }
+            const long o = (long)i * a.nd + a.ind;
+            a.pot[o] += pot;
+            if (PGH >= 2) {
+                const long gi = (long)i * 3 * a.nd + a.ind;
+                a.grad[gi] += gx * dsq;
+                a.grad[gi + a.nd] += gy * dsq;
+                a.grad[gi + 2 * a.nd] += gz * dsq;
+            }
+            if (PGH >= 3) {
+                const long hi2 = (long)i * 6 * a.nd + a.ind;
+                const double d2 = dsq * dsq;
+                a.hess[hi2] += hxx * d2;
+                a.hess[hi2 + a.nd] += hyy * d2;
+                a.hess[hi2 + 2 * a.nd] += hzz * d2;
+                a.hess[hi2 + 3 * a.nd] += hxy * d2;
+                a.hess[hi2 + 4 * a.nd] += hxz * d2;
+                a.hess[hi2 + 5 * a.nd] += hyz * d2;
+            }
+        }
+    }
+}
+
+void launch_taylor_eval3(const Eval3Args& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return;
+    kmark(KID_TAYLOR_EVAL, false, s);
+    if (a.ifpgh >= 3)
+        k_taylor_eval3<3><<<a.grid, 256, 0, s>>>(a);
+    else if (a.ifpgh == 2)
+        k_taylor_eval3<2><<<a.grid, 256, 0, s>>>(a);
+    else
+        k_taylor_eval3<1><<<a.grid, 256, 0, s>>>(a);
+    kmark(KID_TAYLOR_EVAL, true, s);
+}
+
+// Step 5, plane-wave route (g3dpwevalp/g/h_vec, g3drouts.f:2268-2668):
+// 2 Re sum_{a,b,c<half} PW(a,b,c) e^{i ts_a x} e^{i ts_b y} e^{i ts_c z} and
+// derivatives (factors i ts dsq).  Phases by recurrence from the first node:
+// e^{i ts_a u} = e^{i ts_0 u} (e^{i h u})^a.
+template <int PGH>
+__global__ __launch_bounds__(256) void k_pw_eval3(Eval3Args a)
+{
+    const int npw = a.npw, half = npw / 2, t = threadIdx.x;
+    const long nexp = (long)npw * npw * half;
+    const double h = a.ts[1] - a.ts[0];
+    const int total = a.tstart[a.nbox];
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        int lo = 0, hi = a.nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (a.tstart[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        const int b = a.boxes[lo];
+        if (!pw_box(a, b)) continue;
+        const int2 rr = a.se[b];
+        const int p0 = rr.x + (g - a.tstart[lo]) * a.seg, p1 = min(rr.y, p0 + a.seg);
+        const double* pw = a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
+        const double cx = a.cen[3 * b], cy = a.cen[3 * b + 1], cz = a.cen[3 * b + 2], dsq = a.dsq;
+        for (int i = p0 + t; i < p1; i += 256) {
+            const double u[3] = {(a.pts[3 * (long)i] - cx) * dsq, (a.pts[3 * (long)i + 1] - cy) * dsq,
+                                 (a.pts[3 * (long)i + 2] - cz) * dsq};
+            double w0r[3], w0i[3], str[3], sti[3];
+            for (int d = 0; d < 3; ++d) {
+                sincos(a.ts[0] * u[d], &w0i[d], &w0r[d]);
+                sincos(h * u[d], &sti[d], &str[d]);
+            }
+            // accumulators (complex): pot, gx, gy, gz, hxx, hyy, hzz, hxy, hxz, hyz
+            double R[10][2] = {};
+            double wzr = w0r[2], wzi = w0i[2];
+            for (int c = 0; c < half; ++c) {
+                double d1[6][2] = {};  // d1, d1x, d1y, d1xx, d1yy, d1xy
+                double wyr = w0r[1], wyi = w0i[1];
+                for (int bb = 0; bb < npw; ++bb) {
+                    double d2[3][2] = {};  // d2, d2x, d2xx
+                    double wxr = w0r[0], wxi = w0i[0];
+                    const double* row = pw + 2 * ((long)npw * (bb + (long)npw * c));
+                    for (int aa = 0; aa < npw; ++aa) {
+                        const double pr_ = row[2 * aa], pi_ = row[2 * aa + 1];
+                        const double vr = pr_ * wxr - pi_ * wxi, vi = pr_ * wxi + pi_ * wxr;
+                        d2[0][0] += vr;
+                        d2[0][1] += vi;
+                        if (PGH >= 2) {
+                            const double f = a.ts[aa] * dsq;  // (i f) v
+                            d2[1][0] += -f * vi;
+                            d2[1][1] += f * vr;
+                            if (PGH >= 3) {
+                                d2[2][0] += -f * f * vr;
+                                d2[2][1] += -f * f * vi;
+                            }
+                        }
+                        const double tr = wxr * str[0] - wxi * sti[0];
+                        wxi = wxr * sti[0] + wxi * str[0];
+                        wxr = tr;
+                    }
+                    const double fy = a.ts[bb] * dsq;
+                    auto madd = [](double* acc, const double* v, double wr, double wi) {
+                        acc[0] += v[0] * wr - v[1] * wi;
+                        acc[1] += v[0] * wi + v[1] * wr;
+                    };
+                    madd(d1[0], d2[0], wyr, wyi);
+                    if (PGH >= 2) {
+                        madd(d1[1], d2[1], wyr, wyi);
+                        madd(d1[2], d2[0], -fy * wyi, fy * wyr);
+                    }
+                    if (PGH >= 3) {
+                        madd(d1[3], d2[2], wyr, wyi);
+                        madd(d1[4], d2[0], -fy * fy * wyr, -fy * fy * wyi);
+                        madd(d1[5], d2[1], -fy * wyi, fy * wyr);
+                    }
+                    const double tr = wyr * str[1] - wyi * sti[1];
+                    wyi = wyr * sti[1] + wyi * str[1];
+                    wyr = tr;
+                }
+                const double fz = a.ts[c] * dsq;
+                auto madd = [](double* acc, const double* v, double wr, double wi) {
+                    acc[0] += v[0] * wr - v[1] * wi;
+                    acc[1] += v[0] * wi + v[1] * wr;
+                };
+                madd(R[0], d1[0], wzr, wzi);
+                if (PGH >= 2) {
+                    madd(R[1], d1[1], wzr, wzi);
+                    madd(R[2], d1[2], wzr, wzi);
+                    madd(R[3], d1[0], -fz * wzi, fz * wzr);
+                }
+                if (PGH >= 3) {
+                    madd(R[4], d1[3], wzr, wzi);
+                    madd(R[5], d1[4], wzr, wzi);
+                    madd(R[6], d1[0], -fz * fz * wzr, -fz * fz * wzi);
+                    madd(R[7], d1[5], wzr, wzi);
+                    madd(R[8], d1[1], -fz * wzi, fz * wzr);
+                    madd(R[9], d1[2], -fz * wzi, fz * wzr);
+                }
+                const double tr = wzr * str[2] - wzi * sti[2];
+                wzi = wzr * sti[2] + wzi * str[2];
+                wzr = tr;
+            }
+            a.pot[(long)i * a.nd + a.ind] += 2 * R[0][0];
+            if (PGH >= 2) {
+                const long gi = (long)i * 3 * a.nd + a.ind;
+                for (int k = 0; k < 3; ++k) a.grad[gi + k * a.nd] += 2 * R[1 + k][0];
+            }
+            if (PGH >= 3) {
+                const long hi2 = (long)i * 6 * a.nd + a.ind;
+                for (int k = 0; k < 6; ++k) a.hess[hi2 + k * a.nd] += 2 * R[4 + k][0];
+            }
+        }
+    }
+}
+
+void launch_pw_eval3(const Eval3Args& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return;
+    kmark(KID_PW_EVAL, false, s);
+    if (a.ifpgh >= 3)
+        k_pw_eval3<3><<<a.grid, 256, 0, s>>>(a);
+    else if (a.ifpgh == 2)
+        k_pw_eval3<2><<<a.grid, 256, 0, s>>>(a);
+    else
+        k_pw_eval3<1><<<a.grid, 256, 0, s>>>(a);
+    kmark(KID_PW_EVAL, true, s);
+}
+
+// ---------------------------------------------------------------------------
+// Step 6, near field (g3d_directcp/cg/ch_vec g3ddirect.f:3-206 called from
+// fgt3d.f:1460-1512): one block per target unit, its source boxes staged
+// in LDS; pairs with r^2 < dmax; accumulated.
+// ---------------------------------------------------------------------------
+template <int PGH>
+__global__ __launch_bounds__(256) void k_direct3(DirectArgs a)
+{
+    __shared__ double sx[256], sy[256], sz[256], sq[256];
+    const int u = a.units[blockIdx.x];
+    const int2 tr = a.tgt_se[u];
+    const int l0 = a.list_ptr[blockIdx.x], l1 = a.list_ptr[blockIdx.x + 1];
+    const double inv = 1.0 / a.delta, rfac = -2.0 / a.delta;
+    const int t = threadIdx.x;
+    for (int ind = 0; ind < a.nd; ++ind)
+        for (int t0 = tr.x; t0 < tr.y; t0 += 256) {
+            const int i = t0 + t;
+            const bool act = i < tr.y;
+            const double x = act ? a.xt[3 * (long)i] : 0, y = act ? a.xt[3 * (long)i + 1] : 0,
+                         z = act ? a.xt[3 * (long)i + 2] : 0;
+            double p = 0, gx = 0, gy = 0, gz = 0, h[6] = {0, 0, 0, 0, 0, 0};
+            for (int li = l0; li < l1; ++li) {
+                const int2 sr = a.src_se[a.list[li]];
+                for (int s0 = sr.x; s0 < sr.y; s0 += 256) {
+                    const int cn = min(256, sr.y - s0);
+                    __syncthreads();
+                    if (t < cn) {
+                        const long j = s0 + t;
+                        sx[t] = a.xs[3 * j];
+                        sy[t] = a.xs[3 * j + 1];
+                        sz[t] = a.xs[3 * j + 2];
+                        sq[t] = a.qs[j * a.nd + ind];
+                    }
+                    __syncthreads();
+                    for (int s = 0; s < cn; ++s) {
+                        const double dx = x - sx[s], dy = y - sy[s], dz = z - sz[s];
+                        const double rr = dx * dx + dy * dy + dz * dz;
+                        if (rr < a.dmax) {
+                            const double e = exp(-rr * inv) * sq[s];
+                            p += e;
+                            if (PGH >= 2) {
+                                const double ux = rfac * dx, uy = rfac * dy, uz = rfac * dz;
+                                gx += ux * e;
+                                gy += uy * e;
+                                gz += uz * e;
+                                if (PGH >= 3) {
+                                    h[0] += (rfac + ux * ux) * e;
+                                    h[1] += (rfac + uy * uy) * e;
+                                    h[2] += (rfac + uz * uz) * e;
+                                    h[3] += ux * uy * e;
+                                    h[4] += ux * uz * e;
+                                    h[5] += uy * uz * e;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            if (act) {
+                a.pot[(long)i * a.nd + ind] += p;
+                if (PGH >= 2) {
+                    const long g = (long)i * 3 * a.nd + ind;
+                    a.grad[g] += gx;
+                    a.grad[g + a.nd] += gy;
+                    a.grad[g + 2 * a.nd] += gz;
+                }
+                if (PGH >= 3) {
+                    const long hh = (long)i * 6 * a.nd + ind;
+                    for (int k = 0; k < 6; ++k) a.hess[hh + k * a.nd] += h[k];
+                }
+            }
+        }
+}
+
+void launch_direct3(const DirectArgs& a, hipStream_t s)
+{
+    if (a.nunits <= 0) return;
+    kmark(KID_DIRECT, false, s);
+    if (a.ifpgh >= 3)
+        k_direct3<3><<<a.nunits, 256, 0, s>>>(a);
+    else if (a.ifpgh == 2)
+        k_direct3<2><<<a.nunits, 256, 0, s>>>(a);
+    else
+        k_direct3<1><<<a.nunits, 256, 0, s>>>(a);
+    kmark(KID_DIRECT, true, s);
+}
+
+// all-pairs 3D direct sum with a cutoff (fgt3dpart_direct_vec,
+// fgt3d.f:1533-1684), accumulating; thread per target, sources through LDS
+template <int PGH>
+__global__ __launch_bounds__(256) void k_direct_all3(const double* __restrict__ src,
+                                                     const double* __restrict__ q, int nd, int ns,
+                                                     const double* __restrict__ tg, int nt,
+                                                     double delta, double dmax,
+                                                     double* __restrict__ pot,
+                                                     double* __restrict__ grad,
+                                                     double* __restrict__ hess)
+{
+    __shared__ double sx[256], sy[256], sz[256], sq[256];
+    const int t = threadIdx.x, i = blockIdx.x * 256 + t;
+    const bool act = i < nt;
+    const double x = act ? tg[3 * (long)i] : 0, y = act ? tg[3 * (long)i + 1] : 0,
+                 z = act ? tg[3 * (long)i + 2] : 0;
+    const double inv = 1.0 / delta, rfac = -2.0 / delta;
+    for (int ind = 0; ind < nd; ++ind) {
+        double p = 0, gx = 0, gy = 0, gz = 0, h[6] = {0, 0, 0, 0, 0, 0};
+        for (int s0 = 0; s0 < ns; s0 += 256) {
+            const int cn = min(256, ns - s0);
+            __syncthreads();
+            if (t < cn) {
+                const long j = s0 + t;
+                sx[t] = src[3 * j];
+                sy[t] = src[3 * j + 1];
+                sz[t] = src[3 * j + 2];
+                sq[t] = q[j * nd + ind];
+            }
+            __syncthreads();
+            for (int s = 0; s < cn; ++s) {
+                const double dx = x - sx[s], dy = y - sy[s], dz = z - sz[s];
+                const double rr = dx * dx + dy * dy + dz * dz;
+                if (rr < dmax) {
+                    const double e = exp(-rr * inv) * sq[s];
+                    p += e;
+                    if (PGH >= 2) {
+                        const double ux = rfac * dx, uy = rfac * dy, uz = rfac * dz;
+                        gx += ux * e;
+                        gy += uy * e;
+                        gz += uz * e;
+                        if (PGH >= 3) {
+                            h[0] += (rfac + ux * ux) * e;
+                            h[1] += (rfac + uy * uy) * e;
+                            h[2] += (rfac + uz * uz) * e;
+                            h[3] += ux * uy * e;
+                            h[4] += ux * uz * e;
+                            h[5] += uy * uz * e;
+                        }
+                    }
+                }
+            }
+        }
+        if (act) {
+            pot[(long)i * nd + ind] += p;
+            if (PGH >= 2) {
+                const long g = (long)i * 3 * nd + ind;
+                grad[g] += gx;
+                grad[g + nd] += gy;
+                grad[g + 2 * nd] += gz;
+            }
+            if (PGH >= 3) {
+                const long hh = (long)i * 6 * nd + ind;
+                for (int k = 0; k < 6; ++k) hess[hh + k * nd] += h[k];
+            }
+        }
+    }
+}
+
+void launch_direct_all3(const double* src, const double* q, int nd, int ns, const double* tg,
+                        int nt, double delta, double dmax, int ifpgh, double* pot, double* grad,
+                        double* hess, hipStream_t s)
+{
+    if (nt <= 0 || ns <= 0) return;
+    const unsigned g = gridn(nt, 256);
+    if (ifpgh >= 3)
+        k_direct_all3<3><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
+    else if (ifpgh == 2)
+        k_direct_all3<2><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
+    else
+        k_direct_all3<1><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
+}
+
+// ---------------------------------------------------------------------------
+// large delta (fgt3d_large_delta, fgt3d.f:1783-1914): one Taylor expansion
+// about the bounding-box centre, L(j3,j2,j1) = sum q h_j3(x) h_j2(y) h_j1(z)
+// with the Hermite-function recurrence h_0 = e^{-u^2}, h_1 = 2u h_0,
+// h_{j+1} = 2 (u h_j - h_{j-1})/(j+1) (g3dformlc_vec, g3drouts.f:784-856, with
+// the z offset used for z).  Partial sums per block in a fixed order, then a
+// fixed-order reduction: deterministic.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_formlc3_part(const double* __restrict__ p,
+                                                      const double* __restrict__ q, int nd,
+                                                      int ind, int n, double cx, double cy,
+                                                      double cz, double dsq, int nl, int nparts,
+                                                      double* __restrict__ part)
+{
+    extern __shared__ double sm[];
+    const int m = nl + 1, t = threadIdx.x;
+    const long per = (n + nparts - 1) / nparts;
+    const long s0 = blockIdx.x * per, s1 = min((long)n, s0 + per);
+    double* hx = sm;                 // [32][m]
+    double* hy = sm + 32 * m;
+    double* hz = sm + 64 * m;        // q folded in
+    double* out = part + (long)blockIdx.x * m * m * m;
+    for (int p0 = 0; p0 < m * m; p0 += 256) {
+        const int pr = p0 + t, j2 = pr % m, j1 = pr / m;
+        const bool act = pr < m * m;
+        double acc[L3_MAXM];
+#pragma unroll
+        for (int j = 0; j < L3_MAXM; ++j) acc[j] = 0;
+        for (long c0 = s0; c0 < s1; c0 += 32) {
+            const int cn = (int)min(32L, s1 - c0);
+            __syncthreads();
+            if (t < cn) {
+                const long i = c0 + t;
+                const double c3[3] = {cx, cy, cz};
+                double* hh[3] = {hx + t * m, hy + t * m, hz + t * m};
+                for (int d = 0; d < 3; ++d) {
+                    const double u = (p[3 * i + d] - c3[d]) * dsq;
+                    const double f = exp(-u * u);
+                    double* H = hh[d];
+                    H[0] = f;
+                    if (m > 1) H[1] = 2.0 * u * f;
+                    for (int j = 1; j < nl; ++j) H[j + 1] = 2.0 * (u * H[j] - H[j - 1]) / (j + 1);
+                }
+                const double qq = q[(long)nd * i + ind];
+                for (int j = 0; j < m; ++j) hz[t * m + j] *= qq;
+            }
+            __syncthreads();
+            if (act)
+                for (int s = 0; s < cn; ++s) {
+                    const double w = hz[s * m + j1] * hy[s * m + j2];
+#pragma unroll
+                    for (int j = 0; j < L3_MAXM; ++j)
+                        if (j < m) acc[j] += w * hx[s * m + j];
+                }
+        }
+        if (act)
+#pragma unroll
+            for (int j = 0; j < L3_MAXM; ++j)
+                if (j < m) out[j + m * (j2 + (long)m * j1)] = acc[j];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sum_parts3(const double* __restrict__ part, int nparts,
+                                                    long len, double* __restrict__ out)
+{
+    long o = blockIdx.x * 256L + threadIdx.x;
+    if (o >= len) return;
+    double v = 0;
+    for (int k = 0; k < nparts; ++k) v += part[(long)k * len + o];
+    out[o] = v;
+}
+
+void launch_formlc3(const double* p, const double* q, int nd, int ind, int n, const double* c,
+                    double dsq, int nl, double* part, int nparts, double* out, hipStream_t s)
+{
+    const int m = nl + 1;
+    if (m > L3_MAXM) fail(-3, "3D Taylor order above 31 is not supported");
+    size_t lds = (size_t)3 * 32 * m * sizeof(double);
+    k_formlc3_part<<<nparts, 256, lds, s>>>(p, q, nd, ind, n, c[0], c[1], c[2], dsq, nl, nparts,
+                                            part);
+    const long len = (long)m * m * m;
+    k_sum_parts3<<<gridn(len, 256), 256, 0, s>>>(part, nparts, len, out);
+}
+
+}  // namespace fgtpw

@@ -1,24 +1,28 @@
-// Host side of the quadtree: term counts, plane-wave nodes and the box-level
-// tree built from the GPU's list of non-empty leaves.
+// Host side of the box trees (quadtree, D = 2; octree, D = 3): term
+// counts, plane-wave nodes and the box-level tree built from the GPU's list
+// of non-empty leaves.
 //
 // The GPU sorts the sources by their full-depth Morton key and finds, per
 // point, the level at which the reference's refinement rule stops
-// (kernels2d.hip).  What comes back is one record per non-empty leaf of the
-// adaptive tree, in Morton order.  From those records this file rebuilds the
-// adaptive tree with the reference's box numbering (level by level, children
-// of refined boxes in parent order; pts_tree_build, pts_tree2d4.f:363-590),
-// then applies the reference's level-restriction procedure
-// (pts_tree_fix_lr, pts_tree2d4.f:693-996) step for step, because the set of
-// boxes it produces is part of the result the transform must match.
+// (kernels2d.hip, kernels3d.hip).  What comes back is one record per
+// non-empty leaf of the adaptive tree, in Morton order.  From those records
+// this file rebuilds the adaptive tree with the reference's box numbering
+// (level by level, children of refined boxes in parent order; pts_tree_build,
+// pts_tree2d4.f:363-590 and pts_tree3d.f:402-660), then applies the
+// reference's level-restriction procedure (pts_tree_fix_lr,
+// pts_tree2d4.f:693-996, pts_tree3d.f:855-1171) step for step, because the
+// set of boxes it produces is part of the result the transform must match.
 //
 // Adjacency.  The reference decides "touching" and "colleague" with FP64
 // centre distances against 1.05x the half-sum of the box sizes.  Centres of
 // one level differ by whole multiples of the box size (up to rounding
 // ~1e-16 relative), so those tests are exactly integer tests on the box
-// coordinates (ix, iy) of each level, which is what is used here:
-//   same level:          |dix| <= 1 and |diy| <= 1
-//   levels l and l+1:    the finer box's coordinate lies in
-//                        [2 ic - 1, 2 ic + 2] on both axes.
+// coordinates of each level, which is what is used here:
+//   same level:          |d| <= 1 on every axis
+//   levels l and l+1:    the finer box's coordinate lies in [2c - 1, 2c + 2]
+//                        on every axis.
+// Child j of a box has bit (j >> a) & 1 on axis a (x, y, z), as the
+// reference numbers children (tree_routs4.f:28-44, tree_routs3d.f:28-46).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -41,10 +45,11 @@ static double ipow(double a, int n)
     return inv ? 1 / r : r;
 }
 
-// Hermite/Taylor order for boxes of side bsize, g2dhlterms (g2dterms.f:19-66):
-// the smallest p >= int(e r^2)+1 with d*rk*p^(-1/4)*rp^p/(1-rp) < eps,
-// r = bsize/sqrt(2 delta), rp = r sqrt(e/p), rk = 1.09 (a single-precision
-// constant in the reference) * (2 pi)^(-1/4).
+// Hermite/Taylor order for boxes of side bsize, g2dhlterms (g2dterms.f:19-66;
+// g3dhlterms, g3dterms.f:13-60, is the same routine): the smallest
+// p >= int(e r^2)+1 with d*rk*p^(-1/4)*rp^p/(1-rp) < eps, r = bsize/sqrt(2
+// delta), rp = r sqrt(e/p), rk = 1.09 (a single-precision constant in the
+// reference) * (2 pi)^(-1/4).
 int hermite_terms(int ndim, double bsize, double delta, double eps)
 {
     const double e = std::exp(1.0);
@@ -59,9 +64,10 @@ int hermite_terms(int ndim, double bsize, double delta, double eps)
     return -1;
 }
 
-// Plane-wave length, g2dpwterms (g2dterms.f:71-133): Fourier truncation at
-// pmax = 2 sqrt(ln 1/eps) and step h = 2 pi/(rmax + sqrt(ln 1/eps)), so that
-// the aliased images of the Gaussian stay below eps; npw = 2 int(pmax/h).
+// Plane-wave length, g2dpwterms (g2dterms.f:71-133; g3dpwterms identical):
+// Fourier truncation at pmax = 2 sqrt(ln 1/eps) and step
+// h = 2 pi/(rmax + sqrt(ln 1/eps)), so that the aliased images of the
+// Gaussian stay below eps; npw = 2 int(pmax/h).
 void planewave_terms(double bsize, double delta, double eps, double* pmax, int* npw)
 {
     const double pi = 4.0 * std::atan(1.0);
@@ -88,21 +94,39 @@ void planewave_nodes(double pmax, int npw, double* ws, double* ts)
     }
 }
 
+// The 3D ordering of the same nodes (get_pwnodes, g3drouts.f:2669-2690):
+// ascending, ts_j = (j + 1/2) h for j = -npw/2 .. npw/2-1.
+void planewave_nodes3(double pmax, int npw, double* ws, double* ts)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    int half = npw / 2;
+    double h = pmax / half;
+    double w = h / (2.0 * std::sqrt(pi));
+    for (int j = -half; j < half; ++j) {
+        ts[j + half] = (j + 0.5) * h;
+        ws[j + half] = w * std::exp(-ts[j + half] * ts[j + half] / 4);
+    }
+}
+
 namespace {
 
+template <int D>
 struct Builder {
-    Tree2D& T;
+    static constexpr int CH = TreeD<D>::CH, NC = TreeD<D>::NC;
+    TreeD<D>& T;
     // raw views of the per-box arrays, refreshed whenever they grow
-    int *lev, *par, *nch, *ch, *ncol, *col, *flag, *ix, *iy;
+    int *lev, *par, *nch, *ch, *ncol, *col, *flag;
+    int* ic[D];
     double* cen;
     uint64_t* pre;
-    explicit Builder(Tree2D& t) : T(t) { view(); }
+    explicit Builder(TreeD<D>& t) : T(t) { view(); }
 
     void view()
     {
         lev = T.lev.data(), par = T.par.data(), nch = T.nch.data(), ch = T.ch.data();
-        ncol = T.ncol.data(), col = T.col.data(), flag = T.flag.data(), ix = T.ix.data();
-        iy = T.iy.data(), cen = T.cen.data(), pre = T.pre.data();
+        ncol = T.ncol.data(), col = T.col.data(), flag = T.flag.data();
+        cen = T.cen.data(), pre = T.pre.data();
+        for (int a = 0; a < D; ++a) ic[a] = T.ic[a].data();
     }
     // append k boxes with no children, no colleagues and flag 0
     int grow(int k)
@@ -111,114 +135,129 @@ struct Builder {
         T.lev.resize(m);
         T.par.resize(m);
         T.nch.resize(m, 0);
-        T.ch.resize(4 * m, -1);
+        T.ch.resize(CH * m, -1);
         T.ncol.resize(m, 0);
-        T.col.resize(9 * m, -1);
+        T.col.resize(NC * m, -1);
         T.flag.resize(m, 0);
-        T.cen.resize(2 * m);
+        T.cen.resize(D * m);
         T.pre.resize(m);
-        T.ix.resize(m);
-        T.iy.resize(m);
+        for (int a = 0; a < D; ++a) T.ic[a].resize(m);
         view();
         return (int)n;
     }
 
-    // Four children of box b at first..first+3; child j sits at
-    // c + (-1)^(j+1) bs/2 in x and at -bs/2 (j < 2) or +bs/2 in y, bs the
-    // child size (tree_refine_boxes, tree_routs4.f:28-44).
+    // The CH children of box b at first..first+CH-1: child j sits at
+    // c +- bs/2 on axis a by bit (j >> a) & 1, bs the child size
+    // (tree_refine_boxes, tree_routs4.f:28-44, tree_routs3d.f:28-46).
     void set_children(int b, int first)
     {
         const int l = lev[b];
         const double bs = T.bsz[l + 1];
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < CH; ++j) {
             const int c = first + j;
             lev[c] = l + 1;
             par[c] = b;
-            cen[2 * c] = cen[2 * b] + ((j & 1) ? bs / 2 : -bs / 2);
-            cen[2 * c + 1] = cen[2 * b + 1] + ((j < 2) ? -bs / 2 : bs / 2);
-            pre[c] = (pre[b] << 2) | (uint64_t)j;
-            ix[c] = 2 * ix[b] + (j & 1);
-            iy[c] = 2 * iy[b] + (j >> 1);
-            ch[4 * b + j] = c;
+            for (int a = 0; a < D; ++a) {
+                const int bit = (j >> a) & 1;
+                cen[D * c + a] = cen[D * b + a] + (bit ? bs / 2 : -bs / 2);
+                ic[a][c] = 2 * ic[a][b] + bit;
+            }
+            pre[c] = (pre[b] << D) | (uint64_t)j;
+            ch[CH * b + j] = c;
         }
-        nch[b] = 4;
+        nch[b] = CH;
     }
 
     // coarse box c (level l) and fine box f (level l+1) share a point
     bool near_finer(int c, int f) const
     {
-        int dx = ix[f] - 2 * ix[c], dy = iy[f] - 2 * iy[c];
-        return dx >= -1 && dx <= 2 && dy >= -1 && dy <= 2;
+        for (int a = 0; a < D; ++a) {
+            int d = ic[a][f] - 2 * ic[a][c];
+            if (d < -1 || d > 2) return false;
+        }
+        return true;
     }
 
-    // Same-level neighbours (self included), searched among the children of
+    // Same-level neighbours (self included), found among the children of
     // the parent's neighbours in the reference's order (computecoll,
-    // tree_routs4.f:100-211).
+    // tree_routs4.f:100-211, tree_routs3d.f:108-225).  A child of the
+    // parent's neighbour at offset o in {-1,0,1} on an axis, with bit c on
+    // that axis, touches a box with bit t there iff |2o + c - t| <= 1: any c
+    // for o = 0, c = 0 and t = 1 for o = 1, c = 1 and t = 0 for o = -1.
+    // fam[o][t] is the CH-bit set of such children for the offset vector
+    // o (base-3 digits, x first) and sibling t.
+    struct Fam {
+        unsigned short m[NC][CH];
+        Fam()
+        {
+            static const unsigned char ok[3][2] = {{2, 0}, {3, 3}, {0, 1}};
+            for (int o = 0; o < NC; ++o)
+                for (int t = 0; t < CH; ++t) {
+                    unsigned s = 0;
+                    for (int j = 0; j < CH; ++j) {
+                        bool in = true;
+                        for (int a = 0, oo = o; a < D; ++a, oo /= 3)
+                            in = in && ((ok[oo % 3][(t >> a) & 1] >> ((j >> a) & 1)) & 1);
+                        if (in) s |= 1u << j;
+                    }
+                    m[o][t] = (unsigned short)s;
+                }
+        }
+    };
+    static const Fam& fam()
+    {
+        static const Fam f;
+        return f;
+    }
+
+    // colleagues of the CH children of dad, in one pass over dad's list
+    void family_colleagues(int dad)
+    {
+        const Fam& F = fam();
+        const int* kids = ch + CH * dad;
+        int n[CH] = {};
+        for (int i = 0; i < ncol[dad]; ++i) {
+            const int jb = col[NC * dad + i];
+            if (nch[jb] == 0) continue;
+            int o = 0;
+            for (int a = D - 1; a >= 0; --a) o = 3 * o + (ic[a][jb] - ic[a][dad] + 1);
+            const int* cj = ch + CH * jb;
+            for (int t = 0; t < CH; ++t) {
+                int* out = col + NC * kids[t];
+                for (unsigned m = F.m[o][t]; m; m &= m - 1) out[n[t]++] = cj[__builtin_ctz(m)];
+            }
+        }
+        for (int t = 0; t < CH; ++t) {
+            const int b = kids[t];
+            for (int i = n[t]; i < ncol[b]; ++i) col[NC * b + i] = -1;
+            ncol[b] = n[t];
+        }
+    }
+
+    // colleagues of one box (its siblings may not all need them)
     void colleagues(int b)
     {
         if (lev[b] == 0) {
             ncol[b] = 1;
-            col[9 * b] = b;
+            col[NC * b] = b;
             return;
         }
-        // A child of the parent's neighbour jb, at offset o = (jb - dad) in
-        // {-1,0,1} per axis, with bit c on that axis, is a neighbour of b
-        // (bit t) iff |2o + c - t| <= 1: any c for o = 0, c = 0 and t = 1
-        // for o = 1, c = 1 and t = 0 for o = -1.  ok[o+1][t] is the mask of
-        // the allowed c (bit 0: c = 0, bit 1: c = 1).
-        static const unsigned char ok[3][2] = {{2, 0}, {3, 3}, {0, 1}};
-        const int dad = par[b], tx = ix[b] & 1, ty = iy[b] & 1;
-        const int px = ix[dad], py = iy[dad];
+        const Fam& F = fam();
+        const int dad = par[b];
+        int t = 0;
+        for (int a = 0; a < D; ++a) t |= (ic[a][b] & 1) << a;
         int n = 0;
-        int* out = col + 9 * b;
+        int* out = col + NC * b;
         for (int i = 0; i < ncol[dad]; ++i) {
-            const int jb = col[9 * dad + i];
+            const int jb = col[NC * dad + i];
             if (nch[jb] == 0) continue;
-            const int mx = ok[ix[jb] - px + 1][tx], my = ok[iy[jb] - py + 1][ty];
-            const int* cj = ch + 4 * jb;
-            for (int j = 0; j < 4; ++j)
-                if (((mx >> (j & 1)) & 1) && ((my >> (j >> 1)) & 1)) out[n++] = cj[j];
+            int o = 0;
+            for (int a = D - 1; a >= 0; --a) o = 3 * o + (ic[a][jb] - ic[a][dad] + 1);
+            const int* cj = ch + CH * jb;
+            for (unsigned m = F.m[o][t]; m; m &= m - 1) out[n++] = cj[__builtin_ctz(m)];
         }
         for (int i = n; i < ncol[b]; ++i) out[i] = -1;
         ncol[b] = n;
-    }
-
-    // colleagues() for the four children of dad at once: one pass over the
-    // parent's neighbours; fam[o][t] is the 4-bit set of children of a
-    // neighbour at offset o = 3 (oy+1) + (ox+1) that touch sibling t.
-    void family_colleagues(int dad)
-    {
-        static unsigned char fam[9][4];
-        static bool init = false;
-        if (!init) {
-            static const unsigned char ok[3][2] = {{2, 0}, {3, 3}, {0, 1}};
-            for (int o = 0; o < 9; ++o)
-                for (int t = 0; t < 4; ++t) {
-                    int mx = ok[o % 3][t & 1], my = ok[o / 3][t >> 1], m = 0;
-                    for (int j = 0; j < 4; ++j)
-                        if (((mx >> (j & 1)) & 1) && ((my >> (j >> 1)) & 1)) m |= 1 << j;
-                    fam[o][t] = (unsigned char)m;
-                }
-            init = true;
-        }
-        const int px = ix[dad], py = iy[dad];
-        const int* kids = ch + 4 * dad;
-        int n[4] = {0, 0, 0, 0};
-        for (int i = 0; i < ncol[dad]; ++i) {
-            const int jb = col[9 * dad + i];
-            if (nch[jb] == 0) continue;
-            const int o = 3 * (iy[jb] - py + 1) + (ix[jb] - px + 1);
-            const int* cj = ch + 4 * jb;
-            for (int t = 0; t < 4; ++t) {
-                int* out = col + 9 * kids[t];
-                for (unsigned m = fam[o][t]; m; m &= m - 1) out[n[t]++] = cj[__builtin_ctz(m)];
-            }
-        }
-        for (int t = 0; t < 4; ++t) {
-            const int b = kids[t];
-            for (int i = n[t]; i < ncol[b]; ++i) col[9 * b + i] = -1;
-            ncol[b] = n[t];
-        }
     }
 
     void all_colleagues(int maxlev)
@@ -232,24 +271,25 @@ struct Builder {
 
     // Refine the boxes of [b0,b1] whose flag is positive; children of flag-1
     // boxes are marked 3 (still to be checked), of flag-2 boxes 0
-    // (tree_refine_boxes_flag, tree_routs4.f:334-388).
+    // (tree_refine_boxes_flag, tree_routs4.f:334-388, tree_routs3d.f:345-410).
     void refine_flagged(int b0, int b1)
     {
         int k = 0;
         for (int b = b0; b <= b1; ++b) k += flag[b] > 0;
         if (k == 0) return;
-        int next = grow(4 * k);
+        int next = grow(CH * k);
         for (int b = b0; b <= b1; ++b) {
             if (flag[b] <= 0) continue;
             set_children(b, next);
             const int f = flag[b] == 1 ? 3 : 0;
-            for (int j = 0; j < 4; ++j) flag[next + j] = f;
-            next += 4;
+            for (int j = 0; j < CH; ++j) flag[next + j] = f;
+            next += CH;
         }
     }
 
     // Renumber so that each level lists its original boxes, then the boxes
-    // appended to it (pts_tree_reorg, pts_tree2d4.f:1001-1150).
+    // appended to it (pts_tree_reorg, pts_tree2d4.f:1001-1150,
+    // pts_tree3d.f:1172-1329).
     template <class V>
     void permute(V& v, int width, const std::vector<int>& map)
     {
@@ -280,32 +320,32 @@ struct Builder {
         }
         permute(T.lev, 1, map);
         permute(T.nch, 1, map);
-        permute(T.cen, 2, map);
+        permute(T.cen, D, map);
         permute(T.flag, 1, map);
         permute(T.pre, 1, map);
-        permute(T.ix, 1, map);
-        permute(T.iy, 1, map);
+        for (int a = 0; a < D; ++a) permute(T.ic[a], 1, map);
         permute(T.par, 1, map);
-        permute(T.ch, 4, map);
+        permute(T.ch, CH, map);
         view();
         const int* mp = map.data();
         for (int b = 0; b < n; ++b)
             if (par[b] >= 0) par[b] = mp[par[b]];
-        for (int i = 0; i < 4 * n; ++i)
+        for (int i = 0; i < CH * n; ++i)
             if (ch[i] >= 0) ch[i] = mp[ch[i]];
     }
 
     // flag++ boxes with a touching grandchild-level box that has children are
-    // marked for refinement (updateflags, tree_routs4.f:217-329).
+    // marked for refinement (updateflags, tree_routs4.f:217-329,
+    // tree_routs3d.f:226-344).
     void update_flags(int b0, int b1)
     {
         for (int b = b0; b <= b1; ++b) {
             if (flag[b] != 3) continue;
             flag[b] = 0;
             for (int i = 0; i < ncol[b] && flag[b] == 0; ++i) {
-                int jb = col[9 * b + i];
-                for (int j = 0; j < 4; ++j) {
-                    int kb = ch[4 * jb + j];
+                int jb = col[NC * b + i];
+                for (int j = 0; j < CH; ++j) {
+                    int kb = ch[CH * jb + j];
                     if (kb >= 0 && nch[kb] > 0 && near_finer(b, kb)) {
                         flag[b] = 1;
                         break;
@@ -315,8 +355,9 @@ struct Builder {
         }
     }
 
-    // Level restriction, pts_tree_fix_lr (pts_tree2d4.f:693-996).  Returns
-    // the deepest level whose colleague lists are still valid afterwards.
+    // Level restriction, pts_tree_fix_lr (pts_tree2d4.f:693-996,
+    // pts_tree3d.f:855-1171).  Returns the deepest level whose colleague
+    // lists are still valid afterwards.
     int fix_level_restriction()
     {
         const int L = T.nlevels;
@@ -330,7 +371,7 @@ struct Builder {
                 if (nch[dad] == 0) continue;
                 int gd = par[dad];
                 for (int i = 0; i < ncol[gd]; ++i) {
-                    int jb = col[9 * gd + i];
+                    int jb = col[NC * gd + i];
                     if (nch[jb] == 0 && flag[jb] == 0 && near_finer(jb, dad)) {
                         flag[jb] = 1;
                         ++nflag;
@@ -344,7 +385,7 @@ struct Builder {
                 if (flag[b] != 1 && flag[b] != 2) continue;
                 int dad = par[b];
                 for (int i = 0; i < ncol[dad]; ++i) {
-                    int jb = col[9 * dad + i];
+                    int jb = col[NC * dad + i];
                     if (nch[jb] == 0 && flag[jb] == 0 && near_finer(jb, b)) flag[jb] = 2;
                 }
             }
@@ -381,22 +422,26 @@ struct Builder {
 
 }  // namespace
 
-void complete_colleagues(Tree2D& T)
+template <int D>
+void complete_colleagues(TreeD<D>& T)
 {
-    Builder B(T);
+    Builder<D> B(T);
     for (int l = std::max(T.coll_levels, 0); l < T.nlevels; ++l)
         for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b)
             if (B.nch[b] > 0) B.family_colleagues(b);
     T.coll_levels = T.nlevels;
 }
 
-void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int npwlevel,
-                  double bs0, const double* cen0)
+template <int D>
+void build_tree(TreeD<D>& T, const std::vector<LeafRun>& leaves, int nlmax, int npwlevel,
+                double bs0, const double* cen0)
 {
+    constexpr int CH = TreeD<D>::CH, NC = TreeD<D>::NC;
     // storage is reused from call to call: clear, keep the capacity
-    for (auto* v : {&T.laddr, &T.lev, &T.par, &T.nch, &T.ch, &T.ncol, &T.col, &T.flag, &T.ix,
-                    &T.iy, &T.leaf_of_rank, &T.rank_of_box, &T.rank_lo, &T.rank_hi})
+    for (auto* v : {&T.laddr, &T.lev, &T.par, &T.nch, &T.ch, &T.ncol, &T.col, &T.flag,
+                    &T.leaf_of_rank, &T.rank_of_box, &T.rank_lo, &T.rank_hi})
         v->clear();
+    for (int a = 0; a < D; ++a) T.ic[a].clear();
     T.bsz.clear();
     T.cen.clear();
     T.pre.clear();
@@ -404,26 +449,27 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
     T.nlmax = nlmax;
     T.npwlevel = npwlevel;
     T.bs0 = bs0;
-    T.cen0[0] = cen0[0];
-    T.cen0[1] = cen0[1];
+    for (int a = 0; a < D; ++a) T.cen0[a] = cen0[a];
     {
-        // the adaptive tree has at most 4 boxes per distinct proper prefix of
-        // the leaves; the restriction pass adds a fraction more
-        size_t cap = 4 * leaves.size() + 64;
+        // the adaptive tree has at most CH boxes per distinct proper prefix
+        // of the leaves; the restriction pass adds a fraction more
+        size_t cap = CH * leaves.size() + 64;
         cap += cap / 2;
-        T.lev.reserve(cap); T.par.reserve(cap); T.nch.reserve(cap); T.ch.reserve(4 * cap);
-        T.ncol.reserve(cap); T.col.reserve(9 * cap); T.flag.reserve(cap); T.cen.reserve(2 * cap);
-        T.pre.reserve(cap); T.ix.reserve(cap); T.iy.reserve(cap);
+        T.lev.reserve(cap); T.par.reserve(cap); T.nch.reserve(cap); T.ch.reserve(CH * cap);
+        T.ncol.reserve(cap); T.col.reserve(NC * cap); T.flag.reserve(cap); T.cen.reserve(D * cap);
+        T.pre.reserve(cap);
+        for (int a = 0; a < D; ++a) T.ic[a].reserve(cap);
     }
-    Builder B(T);
+    Builder<D> B(T);
     T.bsz.push_back(bs0);
     B.grow(1);
     B.lev[0] = 0;
     B.par[0] = -1;
-    B.cen[0] = cen0[0];
-    B.cen[1] = cen0[1];
+    for (int a = 0; a < D; ++a) {
+        B.cen[a] = cen0[a];
+        B.ic[a][0] = 0;
+    }
     B.pre[0] = 0;
-    B.ix[0] = B.iy[0] = 0;
     T.laddr = {0, 0};
     // frontier: boxes of the current level with their range of leaf records
     std::vector<int> fbox{0}, fa{0}, fe{(int)leaves.size()}, nb, na, ne;
@@ -434,7 +480,7 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
             if (fa[i] < fe[i] && leaves[fa[i]].level > l) ++nref;
         if (nref == 0) break;
         T.bsz.push_back(T.bsz[l] / 2);
-        const int first = B.grow(4 * nref);
+        const int first = B.grow(CH * nref);
         int c0 = first;
         nb.clear();
         na.clear();
@@ -444,11 +490,11 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
             if (!(a < e && leaves[a].level > l)) continue;
             B.set_children(fbox[i], c0);
             int pos = a;
-            for (int d = 0; d < 4; ++d) {
+            for (int d = 0; d < CH; ++d) {
                 int s = pos;
                 while (pos < e) {
                     const LeafRun& r = leaves[pos];
-                    int digit = (int)((r.key >> (2 * (r.level - (l + 1)))) & 3);
+                    int digit = (int)((r.key >> (D * (r.level - (l + 1)))) & (CH - 1));
                     if (digit != d) break;
                     ++pos;
                 }
@@ -456,7 +502,7 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
                 na.push_back(s);
                 ne.push_back(pos);
             }
-            c0 += 4;
+            c0 += CH;
         }
         T.laddr.push_back(first);
         T.laddr.push_back(T.nboxes() - 1);
@@ -495,7 +541,7 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
                 T.rank_of_box[b] = (int)T.leaf_of_rank.size();
                 T.leaf_of_rank.push_back(b);
             } else {
-                for (int j = 3; j >= 0; --j) stack.push_back(T.ch[4 * b + j]);
+                for (int j = CH - 1; j >= 0; --j) stack.push_back(T.ch[CH * b + j]);
             }
         }
     }
@@ -506,8 +552,8 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
                 T.rank_hi[b] = T.rank_of_box[b] + 1;
             } else {
                 int lo = 1 << 30, hi = -1;
-                for (int j = 0; j < 4; ++j) {
-                    int c = T.ch[4 * b + j];
+                for (int j = 0; j < CH; ++j) {
+                    int c = T.ch[CH * b + j];
                     lo = std::min(lo, T.rank_lo[c]);
                     hi = std::max(hi, T.rank_hi[c]);
                 }
@@ -516,5 +562,12 @@ void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlmax, int 
             }
         }
 }
+
+template void build_tree<2>(TreeD<2>&, const std::vector<LeafRun>&, int, int, double,
+                            const double*);
+template void build_tree<3>(TreeD<3>&, const std::vector<LeafRun>&, int, int, double,
+                            const double*);
+template void complete_colleagues<2>(TreeD<2>&);
+template void complete_colleagues<3>(TreeD<3>&);
 
 }  // namespace fgtpw

@@ -5,6 +5,8 @@
 ! on the GPU:
 !   fgt2d                 2d/planewave/src/fgt2d.f:19-22
 !   fgt2dpart_direct_vec  2d/planewave/src/fgt2d.f:1354-1357
+!   fgt3d                 3d/planewave/src/fgt3d.f:19-22
+!   fgt3dpart_direct_vec  3d/planewave/src/fgt3d.f:1533-1536
 !   prini, prin2, prinf, prinf_long   common/prini_new.f (the reference test
 !                                      driver calls them for its printing)
 ! Each call goes through the C-ABI of include/fgtpw.h (iso_c_binding, all
@@ -34,6 +36,26 @@ module fgtpw_c
        real(c_double) :: source(*), charge(*), rnormal(*), dipstr(*), targ(*)
        real(c_double) :: pot(*), grad(*), hess(*)
      end function fgtpw_fgt2dpart_direct
+
+     integer(c_int) function fgtpw_fgt3d(nd, delta, eps, ns, sources, ifcharge, charge, &
+          ifdipole, rnormal, dipstr, iper, ifpgh, pot, grad, hess, nt, targ, ifpghtarg, &
+          pottarg, gradtarg, hesstarg) bind(C, name="fgtpw_fgt3d")
+       import :: c_int, c_double
+       integer(c_int), value :: nd, ns, ifcharge, ifdipole, iper, ifpgh, nt, ifpghtarg
+       real(c_double), value :: delta, eps
+       real(c_double) :: sources(*), charge(*), rnormal(*), dipstr(*)
+       real(c_double) :: pot(*), grad(*), hess(*), targ(*), pottarg(*), gradtarg(*), hesstarg(*)
+     end function fgtpw_fgt3d
+
+     integer(c_int) function fgtpw_fgt3dpart_direct(nd, delta, dmax, istart, iend, jstart, jend, &
+          source, ifcharge, charge, ifdipole, rnormal, dipstr, targ, ifpgh, pot, grad, hess) &
+          bind(C, name="fgtpw_fgt3dpart_direct")
+       import :: c_int, c_double
+       integer(c_int), value :: nd, istart, iend, jstart, jend, ifcharge, ifdipole, ifpgh
+       real(c_double), value :: delta, dmax
+       real(c_double) :: source(*), charge(*), rnormal(*), dipstr(*), targ(*)
+       real(c_double) :: pot(*), grad(*), hess(*)
+     end function fgtpw_fgt3dpart_direct
 
      type(c_ptr) function fgtpw_last_error() bind(C, name="fgtpw_last_error")
        import :: c_ptr
@@ -91,6 +113,37 @@ subroutine fgt2dpart_direct_vec(nd, delta, eps, istart, iend, jstart, jend, sour
        charge, ifdipole, rnormal, dipstr, targ, ifpgh, pot, grad, hess)
   call fgtpw_stop_on_error(ier, 'fgt2dpart_direct_vec')
 end subroutine fgt2dpart_direct_vec
+
+
+subroutine fgt3d(nd, delta, eps, ns, sources, ifcharge, charge, ifdipole, rnormal, dipstr, &
+     iper, ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg)
+  use fgtpw_c
+  implicit none
+  integer :: nd, ns, ifcharge, ifdipole, iper, ifpgh, nt, ifpghtarg
+  real(8) :: delta, eps
+  real(8) :: sources(3, *), charge(nd, *), rnormal(3, *), dipstr(nd, *)
+  real(8) :: pot(nd, *), grad(nd, 3, *), hess(nd, 6, *)
+  real(8) :: targ(3, *), pottarg(nd, *), gradtarg(nd, 3, *), hesstarg(nd, 6, *)
+  integer(c_int) :: ier
+  ier = fgtpw_fgt3d(nd, delta, eps, ns, sources, ifcharge, charge, ifdipole, rnormal, dipstr, &
+       iper, ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg)
+  call fgtpw_stop_on_error(ier, 'fgt3d')
+end subroutine fgt3d
+
+
+subroutine fgt3dpart_direct_vec(nd, delta, dmax, istart, iend, jstart, jend, source, ifcharge, &
+     charge, ifdipole, rnormal, dipstr, targ, ifpgh, pot, grad, hess)
+  use fgtpw_c
+  implicit none
+  integer :: nd, istart, iend, jstart, jend, ifcharge, ifdipole, ifpgh
+  real(8) :: delta, dmax
+  real(8) :: source(3, *), charge(nd, *), rnormal(3, *), dipstr(nd, *), targ(3, *)
+  real(8) :: pot(nd, *), grad(nd, 3, *), hess(nd, 6, *)
+  integer(c_int) :: ier
+  ier = fgtpw_fgt3dpart_direct(nd, delta, dmax, istart, iend, jstart, jend, source, ifcharge, &
+       charge, ifdipole, rnormal, dipstr, targ, ifpgh, pot, grad, hess)
+  call fgtpw_stop_on_error(ier, 'fgt3dpart_direct_vec')
+end subroutine fgt3dpart_direct_vec
 
 
 ! Printing helpers with the calling sequences of the reference's prini family

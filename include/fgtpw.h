@@ -11,8 +11,8 @@
  * ifpgh / ifpghtarg (1 pot, 2 pot+grad, 3 pot+grad+hess); arrays of
  * quantities not requested are not touched and may be dummies.
  *
- * Supported in this build: charges (ifcharge = 1, ifdipole = 0), any nd >= 1.
- * Dipoles return FGTPW_EUNSUPPORTED.
+ * Supported in this build: charges (ifcharge = 1, ifdipole = 0), any nd >= 1,
+ * in 2D (fgt2d) and 3D (fgt3d).  Dipoles return FGTPW_EUNSUPPORTED.
  */
 #ifndef FGTPW_H
 #define FGTPW_H
@@ -70,6 +70,38 @@ int fgtpw_fgt2dpart_direct(int nd, double delta, double eps, int istart, int ien
                            double *hess);
 
 /*
+ * Replaces fgt3d (3d/planewave/src/fgt3d.f:19-63), same arguments and
+ * meaning, host arrays: points (3,n), gradients (nd,3,n), Hessians (nd,6,n)
+ * ordered xx, yy, zz, xy, xz, yz.  iper is ignored, as in the reference.
+ */
+int fgtpw_fgt3d(int nd, double delta, double eps, int ns, const double *sources, int ifcharge,
+                const double *charge, int ifdipole, const double *rnormal, const double *dipstr,
+                int iper, int ifpgh, double *pot, double *grad, double *hess, int nt,
+                const double *targ, int ifpghtarg, double *pottarg, double *gradtarg,
+                double *hesstarg);
+
+/* fgtpw_fgt3d with every array in device memory, on `stream` (a hipStream_t,
+ * NULL for the default stream); returns when the outputs are complete.  No
+ * reference counterpart. */
+int fgtpw_fgt3d_device(int nd, double delta, double eps, int ns, const double *sources,
+                       int ifcharge, const double *charge, int ifdipole, const double *rnormal,
+                       const double *dipstr, int iper, int ifpgh, double *pot, double *grad,
+                       double *hess, int nt, const double *targ, int ifpghtarg, double *pottarg,
+                       double *gradtarg, double *hesstarg, void *stream);
+
+/*
+ * Replaces fgt3dpart_direct_vec (3d/planewave/src/fgt3d.f:1533-1684): adds
+ * to pot/grad/hess at targets jstart..jend (1-based) the direct Gaussian sum
+ * over sources istart..iend, pairs with r^2 >= dmax skipped (the 3D routine
+ * takes dmax itself, g3ddirect.f:3-206).  Accumulates.  Host arrays.
+ */
+int fgtpw_fgt3dpart_direct(int nd, double delta, double dmax, int istart, int iend, int jstart,
+                           int jend, const double *source, int ifcharge, const double *charge,
+                           int ifdipole, const double *rnormal, const double *dipstr,
+                           const double *targ, int ifpgh, double *pot, double *grad,
+                           double *hess);
+
+/*
  * Diagnostics of the last transform on this thread: the integers the
  * reference prints (fgt2d.f:127, :188, :196-197, :358-361) -- out[0]
  * npwlevel, out[1] nlmax, out[2] nlevels, out[3] nboxes, out[4] npw,
@@ -113,6 +145,16 @@ const char *fgtpw_kernel_name(int id);
  * No reference counterpart: the reference keeps its tree internal.
  */
 int fgtpw_last_tree2d(int *info, int cap, int *laddr, int *lev, int *par, int *nch, int *ch,
+                      int *ncol, int *col, double *cen, int *isrc, int *isrcse, int *itarg,
+                      int *itargse);
+
+/*
+ * The octree of the last successful fgtpw_fgt3d* call (fgt3d.f:191-221),
+ * 0-based, arrays as fgtpw_last_tree2d with ch (8,nboxes), col (27,nboxes)
+ * and cen (3,nboxes) (pts_tree3d.f:34-400 itree layout).  No reference
+ * counterpart.
+ */
+int fgtpw_last_tree3d(int *info, int cap, int *laddr, int *lev, int *par, int *nch, int *ch,
                       int *ncol, int *col, double *cen, int *isrc, int *isrcse, int *itarg,
                       int *itargse);
 
