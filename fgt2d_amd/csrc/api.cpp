@@ -1335,17 +1335,18 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     // the reference forms and evaluates the plane waves directly for boxes
     // under 10^6 points, else through Hermite / Taylor expansions.  thresh:
     // 0 = Hermite/Taylor for every box, 10^6 = the reference's switch,
-    // INT_MAX = plane waves for every box (orders above 31, where the
+    // INT_MAX = plane waves for every box (orders above 63, where the
     // Hermite/Taylor kernels stop; differs from the reference only for boxes
-    // of 10^6 points or more).
+    // of 10^6 points or more).  The orders above 31 run on 64-wide register
+    // tiles.
     std::vector<int> form_thresh(nlevels + 1, 0), eval_thresh(nlevels + 1, 0);
     if (expansions)
         for (int l = nlevstart; l <= nlevels; ++l) {
             const double r1 = (nlocal[l] + 1.0) / npw;
             const bool pw = r1 * r1 * r1 >= 3;
-            const bool big = nlocal[l] + 1 > 32;
+            const bool big = nlocal[l] + 1 > 64;
             if (big && !pw)
-                fail(FGTPW_EUNSUPPORTED, "3D expansion order above 31 (eps too small for this box size)");
+                fail(FGTPW_EUNSUPPORTED, "3D expansion order above 63 (eps too small for this box size)");
             form_thresh[l] = eval_thresh[l] = pw ? (big ? INT_MAX : 1000000) : 0;
         }
 

@@ -184,8 +184,18 @@ void launch_gather3(const double* p, const double* q, int nd, const unsigned* pe
 // (LDS) and PW(a,b,c) = sum_k C(k,b) P(a,k).
 // ---------------------------------------------------------------------------
 constexpr int F3_CHUNK = 32;  // sources staged per pass
-constexpr int F3_MAXM = 32;   // Hermite orders up to 31
 
+template <class F>
+static void allow_lds3(F* kernel, size_t bytes)
+{
+    if (bytes > 65536)
+        HIPCHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)bytes));
+}
+
+// F3_MAXM: register tile of the moment accumulators (Hermite orders up to
+// F3_MAXM - 1); instantiated for 32 and 64
+template <int F3_MAXM>
 __global__ __launch_bounds__(256) void k_form3_hermite(Form3Args a)
 {
     extern __shared__ double sm[];
@@ -312,10 +322,16 @@ void launch_form3_hermite(const Form3Args& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
     const int m = a.p + 1;
-    if (m > F3_MAXM) fail(-3, "3D Hermite order above 31 is not supported");
+    if (m > 64) fail(-3, "3D Hermite order above 63 is not supported");
     size_t lds = std::max((size_t)3 * F3_CHUNK * m, (size_t)4 * a.npw * m) * sizeof(double);
     kmark(KID_FORM_MOMENTS, false, s);
-    k_form3_hermite<<<a.grid, 256, lds, s>>>(a);
+    if (m <= 32) {
+        allow_lds3(k_form3_hermite<32>, lds);
+        k_form3_hermite<32><<<a.grid, 256, lds, s>>>(a);
+    } else {
+        allow_lds3(k_form3_hermite<64>, lds);
+        k_form3_hermite<64><<<a.grid, 256, lds, s>>>(a);
+    }
     kmark(KID_FORM_MOMENTS, true, s);
 }
 
@@ -556,7 +572,7 @@ void launch_split3(const int* children, int nc, const int* par, const double* ce
 // One block per leaf; per z node c: S1(j3,b) in LDS, then each thread owns
 // (j3,j2) pairs and keeps L(j3,j2,.) in registers across c.
 // ---------------------------------------------------------------------------
-constexpr int L3_MAXM = 32;
+constexpr int L3_MAXM = 64;  // largest Taylor order + 1 of the 3D kernels
 
 // true when box b is evaluated from its plane waves directly (fewer than
 // thresh sources + targets, fgt3d.f:1321-1324)
@@ -567,6 +583,7 @@ __device__ __forceinline__ bool pw_box(const Eval3Args& a, int b)
     return n < a.thresh;
 }
 
+template <int MM>
 __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
 {
     extern __shared__ double sm[];
@@ -586,9 +603,9 @@ __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
         for (int p0 = 0; p0 < m * m; p0 += 256) {
             const int pr = p0 + t, j3 = pr % m, j2 = pr / m;
             const bool act = pr < m * m;
-            double acc[L3_MAXM];
+            double acc[MM];
 #pragma unroll
-            for (int j = 0; j < L3_MAXM; ++j) acc[j] = 0;
+            for (int j = 0; j < MM; ++j) acc[j] = 0;
             for (int c = 0; c < half; ++c) {
                 __syncthreads();
                 for (int o = t; o < m * npw; o += 256) {
@@ -615,7 +632,7 @@ __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
                     }
                     // 2 Re(D(j1,c) S2)
 #pragma unroll
-                    for (int j1 = 0; j1 < L3_MAXM; ++j1)
+                    for (int j1 = 0; j1 < MM; ++j1)
                         if (j1 < m) {
                             const double dr = Ds[2 * (c + npw * j1)], di = Ds[2 * (c + npw * j1) + 1];
                             acc[j1] += 2 * (dr * s2r - di * s2i);
@@ -624,7 +641,7 @@ __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
             }
             if (act)
 #pragma unroll
-                for (int j1 = 0; j1 < L3_MAXM; ++j1)
+                for (int j1 = 0; j1 < MM; ++j1)
                     if (j1 < m) L[j3 + m * (j2 + (long)m * j1)] = acc[j1];
         }
         __syncthreads();
@@ -635,10 +652,16 @@ void launch_pw2local3(const Eval3Args& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
     const int m = a.nl + 1;
-    if (m > L3_MAXM) fail(-3, "3D Taylor order above 31 is not supported");
+    if (m > L3_MAXM) fail(-3, "3D Taylor order above 63 is not supported");
     size_t lds = (size_t)4 * a.npw * m * sizeof(double);
     kmark(KID_PW2LOCAL, false, s);
-    k_pw2local3<<<a.grid, 256, lds, s>>>(a);
+    if (m <= 32) {
+        allow_lds3(k_pw2local3<32>, lds);
+        k_pw2local3<32><<<a.grid, 256, lds, s>>>(a);
+    } else {
+        allow_lds3(k_pw2local3<64>, lds);
+        k_pw2local3<64><<<a.grid, 256, lds, s>>>(a);
+    }
     kmark(KID_PW2LOCAL, true, s);
 }
 
@@ -1051,6 +1074,7 @@ void launch_direct_all3(const double* src, const double* q, int nd, int ns, cons
 // the z offset used for z).  Partial sums per block in a fixed order, then a
 // fixed-order reduction: deterministic.
 // ---------------------------------------------------------------------------
+template <int MM>
 __global__ __launch_bounds__(256) void k_formlc3_part(const double* __restrict__ p,
                                                       const double* __restrict__ q, int nd,
                                                       int ind, int n, double cx, double cy,
@@ -1068,9 +1092,9 @@ __global__ __launch_bounds__(256) void k_formlc3_part(const double* __restrict__
     for (int p0 = 0; p0 < m * m; p0 += 256) {
         const int pr = p0 + t, j2 = pr % m, j1 = pr / m;
         const bool act = pr < m * m;
-        double acc[L3_MAXM];
+        double acc[MM];
 #pragma unroll
-        for (int j = 0; j < L3_MAXM; ++j) acc[j] = 0;
+        for (int j = 0; j < MM; ++j) acc[j] = 0;
         for (long c0 = s0; c0 < s1; c0 += 32) {
             const int cn = (int)min(32L, s1 - c0);
             __syncthreads();
@@ -1094,13 +1118,13 @@ __global__ __launch_bounds__(256) void k_formlc3_part(const double* __restrict__
                 for (int s = 0; s < cn; ++s) {
                     const double w = hz[s * m + j1] * hy[s * m + j2];
 #pragma unroll
-                    for (int j = 0; j < L3_MAXM; ++j)
+                    for (int j = 0; j < MM; ++j)
                         if (j < m) acc[j] += w * hx[s * m + j];
                 }
         }
         if (act)
 #pragma unroll
-            for (int j = 0; j < L3_MAXM; ++j)
+            for (int j = 0; j < MM; ++j)
                 if (j < m) out[j + m * (j2 + (long)m * j1)] = acc[j];
     }
 }
@@ -1119,10 +1143,14 @@ void launch_formlc3(const double* p, const double* q, int nd, int ind, int n, co
                     double dsq, int nl, double* part, int nparts, double* out, hipStream_t s)
 {
     const int m = nl + 1;
-    if (m > L3_MAXM) fail(-3, "3D Taylor order above 31 is not supported");
+    if (m > L3_MAXM) fail(-3, "3D Taylor order above 63 is not supported");
     size_t lds = (size_t)3 * 32 * m * sizeof(double);
-    k_formlc3_part<<<nparts, 256, lds, s>>>(p, q, nd, ind, n, c[0], c[1], c[2], dsq, nl, nparts,
-                                            part);
+    if (m <= 32)
+        k_formlc3_part<32><<<nparts, 256, lds, s>>>(p, q, nd, ind, n, c[0], c[1], c[2], dsq, nl,
+                                                    nparts, part);
+    else
+        k_formlc3_part<64><<<nparts, 256, lds, s>>>(p, q, nd, ind, n, c[0], c[1], c[2], dsq, nl,
+                                                    nparts, part);
     const long len = (long)m * m * m;
     k_sum_parts3<<<gridn(len, 256), 256, 0, s>>>(part, nparts, len, out);
 }
