@@ -106,9 +106,13 @@ def kernel_profile(step, torch, reps=5):
     return agg, st
 
 
-def roofline(agg, stats, nd):
+def rooflines(agg, stats, nd, count=4):
+    """Roofline records of the instrumented kernels with an algorithmic
+    per-unit figure, largest total time first."""
     npw, nlocal = stats["npw"], stats["nlocal"]
     ranked = sorted(agg.items(), key=lambda kv: -kv[1][0])
+    total = sum(v[0] for v in agg.values())
+    out = []
     for (kernel, level), (ms, n, units) in ranked:
         pu = per_unit(kernel, npw, nd)
         if pu is None:
@@ -120,47 +124,56 @@ def roofline(agg, stats, nd):
         achieved = work / launch_s / (1e12 if bound == "mfma" else 1e9)
         nl = nlocal[level] if 0 <= level < len(nlocal) else 0
         name = rocprof_name(kernel, nl, 1)
-        return {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 4), "traffic": read_traffic(name),
-                "kernel": name, "level": level, "launch_us": round(launch_s * 1e6, 2),
-                "units_per_launch": units // n, "per_unit": "%g %s" % (amount, what),
-                "share_of_kernel_time": round(ms / sum(v[0] for v in agg.values()), 3)}
-    return None
+        out.append({"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                    "frac": round(achieved / peak, 4), "traffic": read_traffic(name),
+                    "kernel": name, "level": level, "launch_us": round(launch_s * 1e6, 2),
+                    "units_per_launch": units // n, "per_unit": "%g %s" % (amount, what),
+                    "share_of_kernel_time": round(ms / total, 3)})
+        if len(out) == count:
+            break
+    return out
 
 
-def cpu_baseline(src, chg, tg, budget_s=30.0):
+def cpu_baseline():
+    """Runs the CPU leg in a child process (the reference prints through the
+    Fortran runtime, which flushes at process exit) and returns its record."""
+    import subprocess
+    import tempfile
+    fd, path = tempfile.mkstemp(suffix=".json")
+    os.close(fd)
+    try:
+        subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", path],
+                       stdout=sys.stderr, stderr=sys.stderr, check=True, timeout=600)
+        with open(path) as f:
+            return json.load(f)
+    finally:
+        os.unlink(path)
+
+
+def cpu_leg(src, chg, tg, budget_s=30.0):
     """The reference itself (oracle/_ref, single core: its OpenMP build does
     not compile) on the same workload, or the C restatement when the
     reference build is absent."""
     from oracle import oracle as O
     kind = "reference" if O.have_ref() else "port"
     f = O.ref_fgt2d if kind == "reference" else O.oracle_fgt2d
-    # the reference prints its diagnostics on stdout: keep them off the
-    # result line
-    sys.stdout.flush()
-    saved = os.dup(1)
-    os.dup2(2, 1)
-    try:
-        # bounded sample: whole transforms of the first n points of the
-        # workload, n halved until one call takes under budget_s / 3
-        n = len(src)
-        while True:
-            t0 = time.perf_counter()
-            f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1)
-            dt = time.perf_counter() - t0
-            if dt < budget_s / 3 or n <= 100_000:
-                break
-            n //= 2
-        reps, elapsed = 1, dt
-        while elapsed < budget_s / 3 and reps < 3:
-            t0 = time.perf_counter()
-            f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1)
-            elapsed += time.perf_counter() - t0
-            reps += 1
-    finally:
-        sys.stdout.flush()
-        os.dup2(saved, 1)
-        os.close(saved)
+    # bounded sample: whole transforms of the first n points of the
+    # workload, n halved until one call takes under budget_s / 3; at most
+    # 3 calls and budget_s / 3 seconds of repetitions
+    n = len(src)
+    while True:
+        t0 = time.perf_counter()
+        f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1)
+        dt = time.perf_counter() - t0
+        if dt < budget_s / 3 or n <= 100_000:
+            break
+        n //= 2
+    reps, elapsed = 1, dt
+    while elapsed < budget_s / 3 and reps < 3:
+        t0 = time.perf_counter()
+        f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1)
+        elapsed += time.perf_counter() - t0
+        reps += 1
     rate = 2 * n * reps / elapsed / 1e6
     return {"value": round(rate, 4), "unit": "Mpts/s", "cores": 1, "kind": kind,
             "sample": "%d full transforms of the first %d sources and %d targets of the workload, "
@@ -226,9 +239,14 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
                     help="c2: the 2D 2M+2M mixture (default, the headline metric); "
                          "c3: the 3D 1M uniform configuration")
+    ap.add_argument("--cpu-leg", metavar="OUT", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calib", action="store_true",
                     help="end with one 256 MiB device copy (the reference kernel for PMC byte counts)")
     args = ap.parse_args()
+    if args.cpu_leg:  # child process of cpu_baseline(): CPU only, no GPU
+        with open(args.cpu_leg, "w") as f:
+            json.dump(cpu_leg(*workload()), f)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
@@ -300,8 +318,10 @@ def main():
         "rel_l2_err_vs_direct": err,
         "direct_oracle_check": direct_check,
         "stages_ms": {k: round(v, 4) for k, v in st.items()},
-        "roofline": roofline(agg, stats, 1),
     }
+    rl = rooflines(agg, stats, 1)
+    rec["roofline"] = rl[0] if rl else None
+    rec["roofline_next"] = rl[1:]
     if args.copy_calib:
         a = torch.empty(32 << 20, dtype=torch.float64, device=dev).fill_(1.0)
         b = torch.empty_like(a)
@@ -309,7 +329,7 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(src, chg, tg)
+        rec["cpu_baseline"] = cpu_baseline()
     print(json.dumps(rec))
 
 
