@@ -43,23 +43,40 @@ def workload(ns=NS, nt=NT, seed=SEED):
 PEAK = {"mfma": (78.6, "TFLOP/s"), "hbm": (8000.0, "GB/s")}
 
 
-def per_unit(kernel, npw, nd):
-    """Algorithmic work of one unit of an instrumented kernel: (bound, amount,
-    what).  Formation and evaluation count the plane-wave contraction of a
-    point, 8*(npw/2)*npw flop to form (plus 8*npw to evaluate); translations
-    count 48 bytes per complex coefficient and box (16 B read, 32 B
-    read-modify-write); gathers and scatters their bytes per point."""
+def per_unit(kernel, npw, nd, p):
+    """Algorithmic work of one unit of an instrumented kernel on the route it
+    runs, (bound, amount, what), for expansion order p at its level.
+    Hermite formation accumulates one FMA per coefficient (j, k <= p) and
+    density per source, 2(p+1)^2 nd flop; Taylor evaluation the same per
+    evaluation point (potential); plane-wave evaluation contracts the
+    npw/2 x npw half-spectrum per point, 8 (npw/2) npw + 8 npw flop per
+    density; translations move 48 bytes per complex coefficient and box
+    (16 B read, 32 B read-modify-write); gathers and scatters their bytes
+    per point."""
     half = npw // 2
-    if kernel == "form_moments":
-        return "mfma", 8.0 * half * npw, "flop per source"
-    if kernel in ("taylor_eval", "pw_eval"):
-        return "mfma", 8.0 * half * npw + 8.0 * npw, "flop per evaluation point"
+    if kernel in ("form_moments", "form_valu"):
+        return "mfma", 2.0 * (p + 1) ** 2 * nd, "flop per source"
+    if kernel == "taylor_eval":
+        return "mfma", 2.0 * (p + 1) ** 2 * nd, "flop per evaluation point"
+    if kernel == "pw_eval":
+        return "mfma", (8.0 * half * npw + 8.0 * npw) * nd, "flop per evaluation point"
     if kernel in ("merge", "m2l", "split"):
         return "hbm", 48.0 * half * npw * nd, "bytes per box"
     if kernel == "gather":
         return "hbm", 2 * 8.0 * (2 + nd) + 4, "bytes per point"
     if kernel == "scatter":
         return "hbm", 2 * 8.0 * nd + 4, "bytes per point"
+    return None
+
+
+def pw_equivalent(kernel, npw, nd):
+    """The plane-wave-direct flop count of one source (formation) or one
+    evaluation point: the work the Hermite/Taylor route replaces."""
+    half = npw // 2
+    if kernel in ("form_moments", "form_valu"):
+        return 8.0 * half * npw * nd
+    if kernel in ("taylor_eval", "pw_eval"):
+        return (8.0 * half * npw + 8.0 * npw) * nd
     return None
 
 
@@ -114,21 +131,26 @@ def rooflines(agg, stats, nd, count=4):
     total = sum(v[0] for v in agg.values())
     out = []
     for (kernel, level), (ms, n, units) in ranked:
-        pu = per_unit(kernel, npw, nd)
+        nl = nlocal[level] if 0 <= level < len(nlocal) else 0
+        pu = per_unit(kernel, npw, nd, nl)
         if pu is None:
             continue
         bound, amount, what = pu
         launch_s = ms / n / 1e3
         work = amount * units / n
         peak, unit = PEAK[bound]
-        achieved = work / launch_s / (1e12 if bound == "mfma" else 1e9)
-        nl = nlocal[level] if 0 <= level < len(nlocal) else 0
+        scale = 1e12 if bound == "mfma" else 1e9
+        achieved = work / launch_s / scale
         name = rocprof_name(kernel, nl, 1)
-        out.append({"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                    "frac": round(achieved / peak, 4), "traffic": read_traffic(name),
-                    "kernel": name, "level": level, "launch_us": round(launch_s * 1e6, 2),
-                    "units_per_launch": units // n, "per_unit": "%g %s" % (amount, what),
-                    "share_of_kernel_time": round(ms / total, 3)})
+        rec = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+               "frac": round(achieved / peak, 4), "traffic": read_traffic(name),
+               "kernel": name, "level": level, "launch_us": round(launch_s * 1e6, 2),
+               "units_per_launch": units // n, "per_unit": "%g %s" % (amount, what),
+               "share_of_kernel_time": round(ms / total, 3)}
+        eq = pw_equivalent(kernel, npw, nd)
+        if eq is not None:
+            rec["pw_equivalent_rate"] = round(eq * units / n / launch_s / scale, 3)
+        out.append(rec)
         if len(out) == count:
             break
     return out
@@ -229,6 +251,91 @@ def bench3d(args, torch, fgt2d_amd):
     print(json.dumps(rec))
 
 
+POOL = 1_000_000
+
+
+def serve_stream(fgt2d_amd, nreq, rate=200.0, slots=4, seed=SEED, check=24):
+    """Serving stream: `nreq` requests with seed-fixed Poisson arrivals at
+    `rate` per second; dimension 2 or 3 (p = 1/2), N = round(10^U[3,6])
+    points uniform in the unit box with sources = targets and potentials at
+    the sources, charges N(0,1), delta = 10^U[-6,-2], eps from {1e-3, 1e-6,
+    1e-9}.  The points of a request are a random slice of a pool of 10^6
+    uniform points per dimension (no generation inside the timed stream).
+    Latency runs from the scheduled arrival to completion.  The rel. l2 error
+    of `check` requests of at most 2*10^5 points is taken against the GPU
+    direct sum on 1000 of their points."""
+    import queue
+    import threading
+    rng = np.random.default_rng(seed)
+    pools = {d: rng.random((POOL, d)) for d in (2, 3)}
+    qpool = rng.normal(size=(POOL, 1))
+    arrive = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    dims = rng.choice([2, 3], nreq)
+    npts = np.rint(10 ** rng.uniform(3, 6, nreq)).astype(np.int64)
+    deltas = 10 ** rng.uniform(-6, -2, nreq)
+    epss = rng.choice([1e-3, 1e-6, 1e-9], nreq)
+    offs = rng.integers(0, POOL - npts + 1)
+    small = np.flatnonzero(npts <= 200_000)
+    checked = set(rng.choice(small, min(check, len(small)), replace=False).tolist())
+
+    def req(i):
+        o, n = int(offs[i]), int(npts[i])
+        return pools[int(dims[i])][o:o + n], qpool[o:o + n]
+
+    srv = fgt2d_amd.Server(slots=slots)
+    tickets = queue.Queue()
+    submitted = np.zeros(nreq)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i in range(nreq):
+            dt = t0 + arrive[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            src, chg = req(i)
+            submitted[i] = time.perf_counter() - t0
+            tickets.put((i, srv.submit(int(dims[i]), deltas[i], epss[i], src, chg, None,
+                                       ifpgh=1, ifpghtarg=0)))
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat = np.zeros(nreq)
+    done = np.zeros(nreq)
+    errs = {}
+    try:
+        for _ in range(nreq):
+            i, t = tickets.get()
+            outs, ms = srv.wait(t)
+            lat[i] = submitted[i] - arrive[i] + ms[2] / 1e3
+            done[i] = submitted[i] + ms[2] / 1e3
+            if i in checked:
+                src, chg = req(i)
+                sel = np.random.default_rng(i).permutation(len(src))[:1000]
+                f = fgt2d_amd.fgt2dpart_direct if dims[i] == 2 else fgt2d_amd.fgt3dpart_direct
+                # no cutoff: eps 1e-300 (2D) and dmax 1e300 (3D)
+                ex = f(deltas[i], 1e-300 if dims[i] == 2 else 1e300, src, chg, src[sel])[0]
+                e = float(np.linalg.norm(ex - outs[0][sel]) / np.linalg.norm(ex))
+                k = "%g" % epss[i]
+                errs[k] = max(errs.get(k, 0.0), e)
+            del outs
+    finally:
+        th.join()
+        stats = srv.stats()
+        srv.close()
+    span = done.max()
+    pts = float(2 * npts.sum())
+    lat_ms = lat * 1e3
+    return {"requests": nreq, "rate_per_s": rate, "slots": slots,
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+            "mean_ms": round(float(lat_ms.mean()), 3), "max_ms": round(float(lat_ms.max()), 3),
+            "Mpts_s": round(pts / span / 1e6, 3),
+            "offered_Mpts_s": round(pts / arrive[-1] / 1e6, 3), "span_s": round(float(span), 3),
+            "peak_running": stats.get("peak_running"),
+            "max_rel_l2_err_by_eps": {k: float("%.3g" % v) for k, v in sorted(errs.items())},
+            "checked_requests": len(checked)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,9 +343,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--stages", action="store_true", help="print per-stage times to stderr")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
                     help="c2: the 2D 2M+2M mixture (default, the headline metric); "
-                         "c3: the 3D 1M uniform configuration")
+                         "c3: the 3D 1M uniform configuration; "
+                         "c4: the serving stream alone (10k requests at 200/s)")
+    ap.add_argument("--serve-requests", type=int, default=1000,
+                    help="requests of the serving stream reported beside c2 (0: none)")
+    ap.add_argument("--slots", type=int, default=4, help="server slots of the serving stream")
     ap.add_argument("--cpu-leg", metavar="OUT", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calib", action="store_true",
                     help="end with one 256 MiB device copy (the reference kernel for PMC byte counts)")
@@ -258,6 +369,15 @@ def main():
 
     if args.workload == "c3":
         return bench3d(args, torch, fgt2d_amd)
+    if args.workload == "c4":
+        rec = serve_stream(fgt2d_amd, 10_000, slots=args.slots)
+        print(json.dumps({"metric": "p50/p99 request latency, serving stream (one MI355X, FP64)",
+                          "value": rec["p50_ms"], "unit": "ms", "n_gpus": 1,
+                          "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+                          "config": {"workload": "serving: 10k requests, Poisson 200/s, 2D/3D, "
+                                                 "N 10^U[3,6], delta 10^U[-6,-2], eps 1e-3/6/9"},
+                          "serving": rec}))
+        return
     dev = torch.device("cuda:0")
     src, chg, tg = workload()
     d_src = torch.from_numpy(src).to(dev)
@@ -328,6 +448,8 @@ def main():
         torch.cuda.synchronize()
         b.copy_(a)
         torch.cuda.synchronize()
+    if args.serve_requests > 0:
+        rec["serving"] = serve_stream(fgt2d_amd, args.serve_requests, slots=args.slots)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline()
     print(json.dumps(rec))

@@ -16,8 +16,9 @@ import os as _os
 
 import numpy as _np
 
-__all__ = ["fgt2d", "fgt2dpart_direct", "FGTError", "last_stats", "last_tree2d",
-           "tree2d_from_leaves", "set_timing", "stage_times", "library_path", "load"]
+__all__ = ["fgt2d", "fgt3d", "fgt2dpart_direct", "fgt3dpart_direct", "FGTError", "Server",
+           "last_stats", "last_tree2d", "last_tree3d", "tree2d_from_leaves", "set_timing",
+           "stage_times", "kernel_launches", "library_path", "load"]
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
 _LIB = None
@@ -67,13 +68,30 @@ def load():
     lib.fgtpw_fgt3dpart_direct.argtypes = lib.fgtpw_fgt2dpart_direct.argtypes
     lib.fgtpw_last_tree3d.argtypes = [ip, i] + [dp] * 12
     lib.fgtpw_tree2d_from_leaves.argtypes = [i, i, d, dp, i, dp, dp, ip, i] + [dp] * 8
-    for f in ("fgtpw_init", "fgtpw_fgt2d", "fgtpw_fgt2d_device", "fgtpw_fgt2dpart_direct",
+    lib.fgtpw_serve_start.argtypes = [i, d]
+    lib.fgtpw_submit.argtypes = [_C.POINTER(_Call)]
+    lib.fgtpw_submit.restype = _C.c_longlong
+    lib.fgtpw_wait.argtypes = [_C.c_longlong, _C.POINTER(_C.c_double)]
+    lib.fgtpw_serve_stats.argtypes = [_C.POINTER(_C.c_longlong), i]
+    for f in ("fgtpw_serve_start", "fgtpw_wait", "fgtpw_serve_stats", "fgtpw_serve_stop",
+              "fgtpw_init", "fgtpw_fgt2d", "fgtpw_fgt2d_device", "fgtpw_fgt2dpart_direct",
               "fgtpw_last_stats", "fgtpw_set_timing", "fgtpw_stage_times", "fgtpw_last_tree2d",
               "fgtpw_tree2d_from_leaves", "fgtpw_fgt3d", "fgtpw_fgt3d_device",
               "fgtpw_fgt3dpart_direct", "fgtpw_last_tree3d"):
         getattr(lib, f).restype = _C.c_int
     _LIB = lib
     return lib
+
+
+class _Call(_C.Structure):
+    """struct fgtpw_call of include/fgtpw.h."""
+    _fields_ = [("dim", _C.c_int), ("nd", _C.c_int), ("delta", _C.c_double),
+                ("eps", _C.c_double), ("ns", _C.c_int), ("sources", _C.c_void_p),
+                ("ifcharge", _C.c_int), ("charge", _C.c_void_p), ("ifdipole", _C.c_int),
+                ("rnormal", _C.c_void_p), ("dipstr", _C.c_void_p), ("ifpgh", _C.c_int),
+                ("pot", _C.c_void_p), ("grad", _C.c_void_p), ("hess", _C.c_void_p),
+                ("nt", _C.c_int), ("targ", _C.c_void_p), ("ifpghtarg", _C.c_int),
+                ("pottarg", _C.c_void_p), ("gradtarg", _C.c_void_p), ("hesstarg", _C.c_void_p)]
 
 
 def _check(rc):
@@ -310,3 +328,76 @@ def last_tree3d(ns, nt):
     t["itarg"] = t["itarg"][:nt]
     t.update(nlevels=nlevels, nboxes=nb, nleaves=nleaves, nlmax=nlmax)
     return t
+
+
+class Server:
+    """Request server of the C-ABI (fgtpw_serve_start / fgtpw_submit /
+    fgtpw_wait): transforms of any dimension, size, delta and eps are queued,
+    admitted by estimated HBM footprint into `slots` concurrent slots, and
+    run with their kernels sharing the device.  Host (numpy) arrays; the
+    inputs are copied at submit, the outputs belong to the ticket.
+
+        with Server(slots=4) as srv:
+            t = srv.submit(2, delta, eps, src, chg, tg, ifpgh=1, ifpghtarg=1)
+            (pot, grad, hess, pt, gt, ht), ms = srv.wait(t)
+    """
+
+    def __init__(self, slots=4, hbm_budget_gib=0.0):
+        self._lib = load()
+        _check(self._lib.fgtpw_serve_start(int(slots), float(hbm_budget_gib)))
+        self._out = {}
+
+    def submit(self, dim, delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
+        """Queues one transform; returns its ticket."""
+        src = _np.ascontiguousarray(sources, dtype=_np.float64)
+        if src.ndim != 2 or src.shape[1] != dim:
+            raise ValueError("sources must have shape (ns, %d)" % dim)
+        ns = src.shape[0]
+        chg = _np.ascontiguousarray(charge, dtype=_np.float64).reshape(ns, -1)
+        nd = chg.shape[1]
+        tg = None if targ is None else _np.ascontiguousarray(targ, dtype=_np.float64)
+        nt = 0 if tg is None else tg.shape[0]
+        if tg is not None and (tg.ndim != 2 or tg.shape[1] != dim):
+            raise ValueError("targ must have shape (nt, %d)" % dim)
+        outs = (_outputs(None, ns, nd, ifpgh, None, dim) +
+                _outputs(None, max(nt, 1), nd, ifpghtarg if nt else 0, None, dim))
+        c = _Call(dim=dim, nd=nd, delta=float(delta), eps=float(eps), ns=ns,
+                  sources=src.ctypes.data, ifcharge=1, charge=chg.ctypes.data, ifdipole=0,
+                  ifpgh=ifpgh, nt=nt, targ=None if tg is None else tg.ctypes.data,
+                  ifpghtarg=ifpghtarg if nt else 0)
+        for name, o in zip(("pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"), outs):
+            setattr(c, name, None if o is None else o.ctypes.data)
+        ticket = self._lib.fgtpw_submit(_C.byref(c))
+        if ticket <= 0:
+            _check(int(ticket))
+        self._out[ticket] = outs
+        return ticket
+
+    def wait(self, ticket):
+        """Blocks until the ticket's transform is done; returns (outputs, ms)
+        with outputs as fgt2d returns them and ms = (queue, run, total)."""
+        ms = (_C.c_double * 3)()
+        rc = self._lib.fgtpw_wait(_C.c_longlong(ticket), ms)
+        outs = self._out.pop(ticket, None)
+        _check(rc)
+        return tuple(outs), (ms[0], ms[1], ms[2])
+
+    def stats(self):
+        """slots, budget bytes, queued, running, bytes in flight, peak
+        running, peak bytes in flight."""
+        buf = (_C.c_longlong * 7)()
+        k = self._lib.fgtpw_serve_stats(buf, 7)
+        keys = ["slots", "budget", "queued", "running", "in_flight", "peak_running",
+                "peak_in_flight"]
+        return dict(zip(keys, [int(v) for v in buf[:k]]))
+
+    def close(self):
+        if self._lib is not None:
+            lib, self._lib = self._lib, None
+            _check(lib.fgtpw_serve_stop())
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

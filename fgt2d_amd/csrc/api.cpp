@@ -16,8 +16,13 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <climits>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -110,6 +115,24 @@ struct Workspace {
         }
         return (T*)b.p;
     }
+    // frees every device and pinned buffer, the events and the stream
+    void release()
+    {
+        for (auto& kv : dev)
+            if (kv.second.p) (void)hipFree(kv.second.p);
+        dev.clear();
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        pin_cap = 0;
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : kev)
+            if (e) (void)hipEventDestroy(e);
+        kev.clear();
+        if (own) (void)hipStreamDestroy(own);
+        own = nullptr;
+        tab.valid = tab3.valid = false;
+    }
     void* pinned(size_t bytes)
     {
         if (pin_cap < bytes) {
@@ -123,17 +146,21 @@ struct Workspace {
     }
 };
 
+Workspace* new_workspace()
+{
+    Workspace* w = new Workspace();
+    HIPCHECK(hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking));
+    for (auto& e : w->ev) HIPCHECK(hipEventCreate(&e));
+    return w;
+}
+
 Workspace& workspace()
 {
     static std::map<int, Workspace*> all;
     int dev = 0;
     HIPCHECK(hipGetDevice(&dev));
     Workspace*& w = all[dev];
-    if (!w) {
-        w = new Workspace();
-        HIPCHECK(hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking));
-        for (auto& e : w->ev) HIPCHECK(hipEventCreate(&e));
-    }
+    if (!w) w = new_workspace();
     return *w;
 }
 
@@ -1818,6 +1845,45 @@ namespace fgtpw {
 namespace {
 // host-array entry (fgt2d / fgt3d): copies in, runs on the library's own
 // stream, copies the requested outputs back
+void host_transform(Workspace& W, int dim, int nd, double delta, double eps, int ns,
+                    const double* sources, const double* charge, int ifpgh, double* pot,
+                    double* grad, double* hess, int nt, const double* targ, int ifpghtarg,
+                    double* pottarg, double* gradtarg, double* hesstarg)
+{
+    hipStream_t s = W.own;
+    const size_t ng = dim, nh = dim == 2 ? 3 : 6;
+    const size_t nps = dim * (size_t)ns, npt = dim * (size_t)nt;
+    double* d_src = W.get<double>("in_src", nps);
+    double* d_chg = W.get<double>("in_chg", (size_t)nd * ns);
+    double* d_trg = W.get<double>("in_trg", std::max<size_t>(npt, 3));
+    HIPCHECK(hipMemcpyAsync(d_src, sources, nps * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(d_chg, charge, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+    if (nt > 0) HIPCHECK(hipMemcpyAsync(d_trg, targ, npt * 8, hipMemcpyHostToDevice, s));
+    const size_t wps = ifpgh > 0 ? (size_t)ns : 0, wpt = ifpghtarg > 0 ? (size_t)nt : 0;
+    double* o_p = W.get<double>("out_pot", std::max<size_t>(nd * wps, 1));
+    double* o_g = W.get<double>("out_grad", std::max<size_t>(ng * nd * wps, 1));
+    double* o_h = W.get<double>("out_hess", std::max<size_t>(nh * nd * wps, 1));
+    double* o_pt = W.get<double>("out_pott", std::max<size_t>(nd * wpt, 1));
+    double* o_gt = W.get<double>("out_gradt", std::max<size_t>(ng * nd * wpt, 1));
+    double* o_ht = W.get<double>("out_hesst", std::max<size_t>(nh * nd * wpt, 1));
+    Call q{nd, delta, eps, ns, d_src, d_chg, ifpgh, o_p, o_g, o_h,
+           nt, d_trg, ifpghtarg, o_pt, o_gt, o_ht};
+    if (dim == 2)
+        run2d(W, q, s);
+    else
+        run3d(W, q, s);
+    if (ifpgh >= 1) HIPCHECK(hipMemcpyAsync(pot, o_p, nd * wps * 8, hipMemcpyDeviceToHost, s));
+    if (ifpgh >= 2) HIPCHECK(hipMemcpyAsync(grad, o_g, ng * nd * wps * 8, hipMemcpyDeviceToHost, s));
+    if (ifpgh >= 3) HIPCHECK(hipMemcpyAsync(hess, o_h, nh * nd * wps * 8, hipMemcpyDeviceToHost, s));
+    if (ifpghtarg >= 1 && nt > 0)
+        HIPCHECK(hipMemcpyAsync(pottarg, o_pt, nd * wpt * 8, hipMemcpyDeviceToHost, s));
+    if (ifpghtarg >= 2 && nt > 0)
+        HIPCHECK(hipMemcpyAsync(gradtarg, o_gt, ng * nd * wpt * 8, hipMemcpyDeviceToHost, s));
+    if (ifpghtarg >= 3 && nt > 0)
+        HIPCHECK(hipMemcpyAsync(hesstarg, o_ht, nh * nd * wpt * 8, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
 int run_host(int dim, int nd, double delta, double eps, int ns, const double* sources,
              int ifcharge, const double* charge, int ifdipole, int ifpgh, double* pot,
              double* grad, double* hess, int nt, const double* targ, int ifpghtarg,
@@ -1827,39 +1893,8 @@ int run_host(int dim, int nd, double delta, double eps, int ns, const double* so
         validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
         std::lock_guard<std::mutex> lk(g_mutex);
         select_device();
-        Workspace& W = workspace();
-        hipStream_t s = W.own;
-        const size_t ng = dim, nh = dim == 2 ? 3 : 6;
-        const size_t nps = dim * (size_t)ns, npt = dim * (size_t)nt;
-        double* d_src = W.get<double>("in_src", nps);
-        double* d_chg = W.get<double>("in_chg", (size_t)nd * ns);
-        double* d_trg = W.get<double>("in_trg", std::max<size_t>(npt, 3));
-        HIPCHECK(hipMemcpyAsync(d_src, sources, nps * 8, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d_chg, charge, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
-        if (nt > 0) HIPCHECK(hipMemcpyAsync(d_trg, targ, npt * 8, hipMemcpyHostToDevice, s));
-        const size_t wps = ifpgh > 0 ? (size_t)ns : 0, wpt = ifpghtarg > 0 ? (size_t)nt : 0;
-        double* o_p = W.get<double>("out_pot", std::max<size_t>(nd * wps, 1));
-        double* o_g = W.get<double>("out_grad", std::max<size_t>(ng * nd * wps, 1));
-        double* o_h = W.get<double>("out_hess", std::max<size_t>(nh * nd * wps, 1));
-        double* o_pt = W.get<double>("out_pott", std::max<size_t>(nd * wpt, 1));
-        double* o_gt = W.get<double>("out_gradt", std::max<size_t>(ng * nd * wpt, 1));
-        double* o_ht = W.get<double>("out_hesst", std::max<size_t>(nh * nd * wpt, 1));
-        Call q{nd, delta, eps, ns, d_src, d_chg, ifpgh, o_p, o_g, o_h,
-               nt, d_trg, ifpghtarg, o_pt, o_gt, o_ht};
-        if (dim == 2)
-            run2d(W, q, s);
-        else
-            run3d(W, q, s);
-        if (ifpgh >= 1) HIPCHECK(hipMemcpyAsync(pot, o_p, nd * wps * 8, hipMemcpyDeviceToHost, s));
-        if (ifpgh >= 2) HIPCHECK(hipMemcpyAsync(grad, o_g, ng * nd * wps * 8, hipMemcpyDeviceToHost, s));
-        if (ifpgh >= 3) HIPCHECK(hipMemcpyAsync(hess, o_h, nh * nd * wps * 8, hipMemcpyDeviceToHost, s));
-        if (ifpghtarg >= 1 && nt > 0)
-            HIPCHECK(hipMemcpyAsync(pottarg, o_pt, nd * wpt * 8, hipMemcpyDeviceToHost, s));
-        if (ifpghtarg >= 2 && nt > 0)
-            HIPCHECK(hipMemcpyAsync(gradtarg, o_gt, ng * nd * wpt * 8, hipMemcpyDeviceToHost, s));
-        if (ifpghtarg >= 3 && nt > 0)
-            HIPCHECK(hipMemcpyAsync(hesstarg, o_ht, nh * nd * wpt * 8, hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
+        host_transform(workspace(), dim, nd, delta, eps, ns, sources, charge, ifpgh, pot, grad,
+                       hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg);
     });
 }
 
@@ -2158,6 +2193,290 @@ const char* fgtpw_stage_name(int i)
 {
     if (i < 0 || i >= (int)(sizeof(kStages) / sizeof(kStages[0]))) return "";
     return kStages[i];
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// serving: a queue of transforms run by a pool of slots, each slot a worker
+// thread with its own workspace (device buffers that stay allocated from one
+// request to the next) and its own stream, so the host tree of one request
+// overlaps the kernels of others and the kernels of several requests share
+// the device.  Requests are admitted in arrival order when a slot is free
+// and their estimated HBM footprint fits the budget beside those in flight.
+// ---------------------------------------------------------------------------
+namespace fgtpw {
+namespace {
+struct ServeRequest {
+    long long id = 0;
+    int dim = 2, nd = 1, ns = 0, nt = 0, ifpgh = 0, ifpghtarg = 0;
+    double delta = 0, eps = 0;
+    std::vector<double> src, chg, trg;  // copies made by submit
+    double *pot = nullptr, *grad = nullptr, *hess = nullptr;
+    double *pottarg = nullptr, *gradtarg = nullptr, *hesstarg = nullptr;
+    size_t footprint = 0;
+    int status = FGTPW_OK;
+    std::string err;
+    bool done = false;
+    std::chrono::steady_clock::time_point t_submit, t_start, t_end;
+};
+
+struct Server {
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServeRequest*> queue;
+    std::unordered_map<long long, ServeRequest*> tickets;
+    std::vector<std::thread> workers;
+    std::vector<Workspace*> slots;
+    size_t budget = 0, in_flight = 0, peak_in_flight = 0;
+    int running = 0, peak_running = 0, waiters = 0;
+    long long next_id = 1;
+    bool stopping = false;
+    int device = 0;
+};
+Server* g_server = nullptr;
+std::mutex g_server_mutex;  // start / stop
+
+int quantities(int dim, int ifpgh)
+{
+    return ifpgh <= 0 ? 0 : ifpgh == 1 ? 1 : ifpgh == 2 ? 1 + dim : 1 + dim + (dim == 2 ? 3 : 6);
+}
+
+// HBM bytes a request is admitted with (an estimate): inputs and their
+// sorted copies, outputs with their sorted-order accumulators and staging,
+// keys/permutations/sort scratch (48 B per point), and two plane-wave
+// expansions (npw^d/2 complex per density) per box at or below the cutoff
+// level, whose side is about dc = sqrt(delta ln(1/eps)) (fgt2d.f:124-127).
+// The tree reaches the cutoff level L only when boxes of level L-1 still
+// hold more than ndiv0 = 10 points on average (fgt2d.f:158); then the boxes
+// with expansions are the occupied cutoff cells plus those below, refined
+// to ndiv points (80 in 2D, 100 in 3D; fgt2d.f:162).
+size_t footprint(const ServeRequest& r, double extent)
+{
+    const double d = r.dim, n = (double)r.ns + r.nt, nch = std::pow(2.0, d);
+    double bytes = 16.0 * ((d + r.nd) * r.ns + d * r.nt) +
+                   24.0 * r.nd * ((double)quantities(r.dim, r.ifpgh) * r.ns +
+                                  (double)quantities(r.dim, r.ifpghtarg) * r.nt) +
+                   48.0 * n;
+    const double lne = std::log(1.0 / r.eps);
+    const double dc = std::sqrt(r.delta * lne);
+    const double lcut = std::max(0.0, std::ceil(std::log2(std::max(extent, 1e-300) / dc)));
+    const double ltree = std::ceil(std::log(std::max(n / 10.0, 1.0)) / std::log(nch)) + 1.0;
+    if (lcut <= ltree) {
+        const double npw = 2.0 * std::floor(3.0 * lne / M_PI) + 2.0;
+        const double nexp = std::pow(npw, d) / 2.0;
+        const double cells = std::pow(std::min(std::pow(2.0, lcut), 1e6), d);
+        const double ndiv = r.dim == 2 ? 80.0 : 100.0;
+        const double nbox = std::min(cells, n) + nch / (nch - 1.0) * 4.0 * n / ndiv;
+        bytes += 2.0 * 16.0 * nexp * r.nd * nbox;
+    }
+    return (size_t)std::min(bytes, 1e18);
+}
+
+void serve_worker(Server* S, int slot)
+{
+    (void)hipSetDevice(S->device);
+    Workspace* W = S->slots[slot];
+    for (;;) {
+        ServeRequest* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->m);
+            S->cv_work.wait(lk, [&] {
+                if (S->queue.empty()) return S->stopping;
+                const ServeRequest* f = S->queue.front();
+                return S->running == 0 || S->in_flight + f->footprint <= S->budget;
+            });
+            if (S->queue.empty()) return;  // stopping and drained
+            r = S->queue.front();
+            S->queue.pop_front();
+            S->in_flight += r->footprint;
+            S->running += 1;
+            S->peak_in_flight = std::max(S->peak_in_flight, S->in_flight);
+            S->peak_running = std::max(S->peak_running, S->running);
+        }
+        r->t_start = std::chrono::steady_clock::now();
+        r->status = run_guarded([&] {
+            host_transform(*W, r->dim, r->nd, r->delta, r->eps, r->ns, r->src.data(),
+                           r->chg.data(), r->ifpgh, r->pot, r->grad, r->hess, r->nt,
+                           r->nt > 0 ? r->trg.data() : nullptr, r->ifpghtarg, r->pottarg,
+                           r->gradtarg, r->hesstarg);
+        });
+        if (r->status != FGTPW_OK) r->err = t_err;
+        r->t_end = std::chrono::steady_clock::now();
+        // release the request's inputs before it is reported done
+        std::vector<double>().swap(r->src);
+        std::vector<double>().swap(r->chg);
+        std::vector<double>().swap(r->trg);
+        {
+            std::lock_guard<std::mutex> lk(S->m);
+            S->in_flight -= r->footprint;
+            S->running -= 1;
+            r->done = true;
+        }
+        S->cv_done.notify_all();
+        S->cv_work.notify_all();
+    }
+}
+}  // namespace
+}  // namespace fgtpw
+
+extern "C" {
+
+int fgtpw_serve_start(int slots, double hbm_budget_gib)
+{
+    return run_guarded([&] {
+        std::lock_guard<std::mutex> g(g_server_mutex);
+        if (g_server) fail(FGTPW_EINVAL, "the server is already running");
+        if (slots < 1 || slots > 16) fail(FGTPW_EINVAL, "slots must be in 1..16");
+        if (!(hbm_budget_gib >= 0)) fail(FGTPW_EINVAL, "the HBM budget must be >= 0");
+        select_device();
+        auto* S = new Server();
+        HIPCHECK(hipGetDevice(&S->device));
+        size_t free_b = 0, total_b = 0;
+        HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+        S->budget = hbm_budget_gib > 0 ? (size_t)(hbm_budget_gib * 1073741824.0)
+                                       : (size_t)(0.6 * (double)free_b);
+        try {
+            for (int i = 0; i < slots; ++i) S->slots.push_back(new_workspace());
+        } catch (...) {
+            for (auto* w : S->slots) { w->release(); delete w; }
+            delete S;
+            throw;
+        }
+        for (int i = 0; i < slots; ++i) S->workers.emplace_back(serve_worker, S, i);
+        g_server = S;
+    });
+}
+
+long long fgtpw_submit(const fgtpw_call* c)
+{
+    long long ticket = 0;
+    int rc = run_guarded([&] {
+        if (!c) fail(FGTPW_EINVAL, "no call");
+        if (c->dim != 2 && c->dim != 3) fail(FGTPW_EINVAL, "dim must be 2 or 3");
+        validate(c->nd, c->delta, c->eps, c->ns, c->ifcharge, c->ifdipole, c->ifpgh, c->nt,
+                 c->ifpghtarg);
+        if (!c->sources || !c->charge || (c->nt > 0 && !c->targ))
+            fail(FGTPW_EINVAL, "missing input array");
+        auto r = std::make_unique<ServeRequest>();
+        r->dim = c->dim;
+        r->nd = c->nd;
+        r->ns = c->ns;
+        r->nt = c->nt;
+        r->delta = c->delta;
+        r->eps = c->eps;
+        r->ifpgh = c->ifpgh;
+        r->ifpghtarg = c->nt > 0 ? c->ifpghtarg : 0;
+        r->src.assign(c->sources, c->sources + (size_t)c->dim * c->ns);
+        r->chg.assign(c->charge, c->charge + (size_t)c->nd * c->ns);
+        if (c->nt > 0) r->trg.assign(c->targ, c->targ + (size_t)c->dim * c->nt);
+        r->pot = c->pot; r->grad = c->grad; r->hess = c->hess;
+        r->pottarg = c->pottarg; r->gradtarg = c->gradtarg; r->hesstarg = c->hesstarg;
+        double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+        for (const auto* v : {&r->src, &r->trg})
+            for (size_t i = 0; i < v->size(); ++i) {
+                const int k = (int)(i % c->dim);
+                lo[k] = std::min(lo[k], (*v)[i]);
+                hi[k] = std::max(hi[k], (*v)[i]);
+            }
+        double ext = 0;
+        for (int k = 0; k < c->dim; ++k) ext = std::max(ext, hi[k] - lo[k]);
+        if (!std::isfinite(ext)) fail(FGTPW_EINVAL, "non-finite coordinates");
+        r->footprint = footprint(*r, ext);
+        r->t_submit = std::chrono::steady_clock::now();
+        std::lock_guard<std::mutex> g(g_server_mutex);
+        Server* S = g_server;
+        if (!S) fail(FGTPW_EINVAL, "the server is not running (fgtpw_serve_start)");
+        {
+            std::lock_guard<std::mutex> lk(S->m);
+            r->id = S->next_id++;
+            ticket = r->id;
+            S->tickets[r->id] = r.get();
+            S->queue.push_back(r.release());
+        }
+        S->cv_work.notify_all();
+    });
+    return rc == FGTPW_OK ? ticket : rc;
+}
+
+int fgtpw_wait(long long ticket, double* ms)
+{
+    int status = FGTPW_OK;
+    std::string err;
+    int rc = run_guarded([&] {
+        Server* S;
+        std::unique_lock<std::mutex> lk;
+        {
+            std::lock_guard<std::mutex> g(g_server_mutex);
+            S = g_server;
+            if (!S) fail(FGTPW_EINVAL, "the server is not running");
+            lk = std::unique_lock<std::mutex>(S->m);
+        }
+        auto it = S->tickets.find(ticket);
+        if (it == S->tickets.end()) fail(FGTPW_EINVAL, "unknown ticket");
+        ServeRequest* r = it->second;
+        S->tickets.erase(it);
+        S->waiters += 1;
+        S->cv_done.wait(lk, [&] { return r->done; });
+        S->waiters -= 1;
+        lk.unlock();
+        S->cv_done.notify_all();
+        if (ms) {
+            auto d = [](std::chrono::steady_clock::time_point a,
+                        std::chrono::steady_clock::time_point b) {
+                return std::chrono::duration<double, std::milli>(b - a).count();
+            };
+            ms[0] = d(r->t_submit, r->t_start);
+            ms[1] = d(r->t_start, r->t_end);
+            ms[2] = d(r->t_submit, r->t_end);
+        }
+        status = r->status;
+        err = r->err;
+        delete r;
+    });
+    if (rc != FGTPW_OK) return rc;
+    if (status != FGTPW_OK) t_err = err;
+    return status;
+}
+
+int fgtpw_serve_stats(long long* out, int n)
+{
+    std::lock_guard<std::mutex> g(g_server_mutex);
+    Server* S = g_server;
+    if (!S) return 0;
+    std::lock_guard<std::mutex> lk(S->m);
+    const long long v[] = {(long long)S->slots.size(), (long long)S->budget,
+                           (long long)S->queue.size(), (long long)S->running,
+                           (long long)S->in_flight, (long long)S->peak_running,
+                           (long long)S->peak_in_flight};
+    const int k = std::min(n, (int)(sizeof(v) / sizeof(v[0])));
+    for (int i = 0; i < k; ++i) out[i] = v[i];
+    return k;
+}
+
+int fgtpw_serve_stop(void)
+{
+    return run_guarded([&] {
+        std::lock_guard<std::mutex> g(g_server_mutex);
+        Server* S = g_server;
+        if (!S) return;
+        {
+            std::lock_guard<std::mutex> lk(S->m);
+            S->stopping = true;
+        }
+        S->cv_work.notify_all();
+        for (auto& t : S->workers) t.join();
+        {
+            // waits already blocked on a request return before the server goes
+            std::unique_lock<std::mutex> lk(S->m);
+            S->cv_done.wait(lk, [&] { return S->waiters == 0; });
+        }
+        // requests never waited for
+        for (auto& kv : S->tickets) delete kv.second;
+        for (auto* w : S->slots) { w->release(); delete w; }
+        delete S;
+        g_server = nullptr;
+    });
 }
 
 }  // extern "C"

@@ -159,6 +159,66 @@ int fgtpw_last_tree3d(int *info, int cap, int *laddr, int *lev, int *par, int *n
                       int *itargse);
 
 /*
+ * Serving.  One transform request: the arguments of fgt2d / fgt3d (dim 2 or
+ * 3), host arrays.  Replaces nothing in the reference, which has no request
+ * interface; the fields mean what the fgt2d arguments of the same names mean
+ * (2d/planewave/src/fgt2d.f:23-62; fgt3d.f:19-63 for dim 3).
+ */
+typedef struct fgtpw_call {
+    int dim;
+    int nd;
+    double delta, eps;
+    int ns;
+    const double *sources; /* (dim, ns) */
+    int ifcharge;
+    const double *charge;  /* (nd, ns) */
+    int ifdipole;
+    const double *rnormal, *dipstr;
+    int ifpgh;
+    double *pot, *grad, *hess;
+    int nt;
+    const double *targ;    /* (dim, nt) */
+    int ifpghtarg;
+    double *pottarg, *gradtarg, *hesstarg;
+} fgtpw_call;
+
+/*
+ * Starts the request server on the current device: `slots` worker slots
+ * (1..16), each with its own stream and device buffers that stay allocated
+ * between requests, and an HBM budget in GiB for the requests in flight
+ * (0: 60% of the free device memory).  Requests are admitted in submission
+ * order when a slot is free and their estimated footprint fits beside those
+ * in flight (one request is always admitted when none runs).
+ */
+int fgtpw_serve_start(int slots, double hbm_budget_gib);
+
+/*
+ * Queues one transform and returns a ticket (> 0) or a negative status.
+ * The input arrays are copied before it returns; the output arrays must stay
+ * valid until fgtpw_wait for the ticket returns, and are written by then.
+ * Thread-safe.
+ */
+long long fgtpw_submit(const fgtpw_call *call);
+
+/*
+ * Blocks until the request of `ticket` has finished and returns its status.
+ * When ms is non-NULL: ms[0] queue time, ms[1] run time, ms[2] submit to
+ * completion, in milliseconds.  Each ticket is waited for once.
+ */
+int fgtpw_wait(long long ticket, double *ms);
+
+/*
+ * Server counters: out[0] slots, out[1] budget bytes, out[2] queued,
+ * out[3] running, out[4] bytes in flight, out[5] peak running, out[6] peak
+ * bytes in flight.  Returns the number written (0 when no server runs).
+ */
+int fgtpw_serve_stats(long long *out, int n);
+
+/* Finishes every queued request, then stops the workers and frees their
+ * buffers.  Call fgtpw_wait for every ticket before. */
+int fgtpw_serve_stop(void);
+
+/*
  * Host-only: the box tree the runtime derives from a list of non-empty
  * leaves of the adaptive tree (Morton prefix keys[i] of level levels[i], in
  * Morton order) -- refinement, level restriction (pts_tree_fix_lr,

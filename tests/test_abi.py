@@ -50,5 +50,14 @@ def test_fortran_dropin_exports_reference_symbols():
     out = subprocess.run([nm, "-D", "--defined-only", os.path.join(ROOT, "fgt2d_amd", "libfgt2d.so")],
                          capture_output=True, text=True, check=True).stdout
     syms = {line.split()[-1] for line in out.splitlines() if line.strip()}
-    for s in ("fgt2d_", "fgt2dpart_direct_vec_", "prini_", "prin2_", "prinf_", "prinf_long_"):
+    for s in ("fgt2d_", "fgt2dpart_direct_vec_", "fgt3d_", "fgt3dpart_direct_vec_", "prini_",
+              "prin2_", "prinf_", "prinf_long_"):
         assert s in syms, s
+
+
+def test_submit_without_a_server_is_an_error():
+    import numpy as np
+    src = np.random.default_rng(0).random((10, 2))
+    with pytest.raises(fgt2d_amd.FGTError, match="not running"):
+        fgt2d_amd.Server.submit(type("S", (), {"_lib": fgt2d_amd.load(), "_out": {}})(), 2,
+                                1e-3, 1e-6, src, np.ones(10))
