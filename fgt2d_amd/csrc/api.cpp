@@ -882,32 +882,45 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         // segment numbering of every formation and evaluation list (one
         // launch): formation over sources in segments of SEGF, evaluation
         // over targets / sources in segments of SEGE
-        auto segf = [&](int l) { int tm_ = (nlocal[l] + 16) / 16; return tm_ <= 2 ? 128 : 64; };
+        auto segf = [&](int l) { int tm_ = (nlocal[l] + 16) / 16; return tm_ <= 2 ? 256 : 128; };
         const int SEGE = 256, GRIDMAX = 4096;
         std::vector<int*> ts_form(nlevels + 1, nullptr), ts_trg(nlevels + 1, nullptr),
-            ts_src(nlevels + 1, nullptr);
+            ts_src(nlevels + 1, nullptr), sb_form(nlevels + 1, nullptr),
+            sb_trg(nlevels + 1, nullptr), sb_src(nlevels + 1, nullptr);
         {
-            size_t ntot = 0;
-            for (int l = 0; l <= nlevels; ++l)
+            // tstart: one entry per box and list plus the total; segbox: one
+            // per segment, at most boxes + points / seg + 1 per list
+            size_t ntot = 0, nseg = 0;
+            for (int l = 0; l <= nlevels; ++l) {
                 ntot += form_l[l].size() + 2 * eval_l[l].size() + 3;
+                if (l >= std::max(npwlevel + 1, 0) && !form_l[l].empty())
+                    nseg += form_l[l].size() + ns / segf(l) + 1;
+                if (l >= nlevstart && !eval_l[l].empty())
+                    nseg += 2 * eval_l[l].size() + (size_t)(nt + ns) / SEGE + 2;
+            }
             int* pool = W.get<int>("tstart", ntot);
+            int* spool = W.get<int>("segbox", nseg);
             SegLists L{};
-            auto add = [&](const std::vector<int>& v, size_t off, const int2* se, int seg) {
+            auto add = [&](const std::vector<int>& v, size_t off, const int2* se, int seg,
+                           int npts, int** sb) {
                 int* t = pool;
                 pool += v.size() + 1;
+                *sb = spool;
+                spool += v.size() + npts / seg + 1;
                 if (L.count == 32) {
                     launch_seg_scan(L, s);
                     L.count = 0;
                 }
-                L.l[L.count++] = SegList{(const int*)D(off), (int)v.size(), se, seg, t};
+                L.l[L.count++] = SegList{(const int*)D(off), (int)v.size(), se, seg, t, *sb};
                 return t;
             };
             for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l)
-                if (!form_l[l].empty()) ts_form[l] = add(form_l[l], o_form[l], src_se, segf(l));
+                if (!form_l[l].empty())
+                    ts_form[l] = add(form_l[l], o_form[l], src_se, segf(l), ns, &sb_form[l]);
             for (int l = nlevstart; l <= nlevels; ++l) {
                 if (eval_l[l].empty()) continue;
-                if (use_trg) ts_trg[l] = add(eval_l[l], o_eval[l], trg_se, SEGE);
-                if (use_src) ts_src[l] = add(eval_l[l], o_eval[l], src_se, SEGE);
+                if (use_trg) ts_trg[l] = add(eval_l[l], o_eval[l], trg_se, SEGE, nt, &sb_trg[l]);
+                if (use_src) ts_src[l] = add(eval_l[l], o_eval[l], src_se, SEGE, ns, &sb_src[l]);
             }
             launch_seg_scan(L, s);
         }
@@ -924,7 +937,8 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                 const int nb_l = (int)form_l[l].size(), m = nlocal[l] + 1;
                 FormArgs fa{(const int*)D(o_form[l]), nb_l, d_cen, src_se, d_slot, xs, qs, nd, ind,
                             dsq, nlocal[l], d_h2x, d_h2xr, d_rsq, npw, mp, ts_form[l], segf(l),
-                            (int)std::min<long>(GRIDMAX, nb_l + ns / segf(l) + 1), partial, m * m};
+                            (int)std::min<long>(GRIDMAX, nb_l + ns / segf(l) + 1), partial, m * m,
+                            sb_form[l]};
                 tm.ctx(l, &form_l[l], 0);
                 launch_form(fa, s);
             }
@@ -972,6 +986,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                     if (side == 1 && !use_src) continue;
                     EvalArgs eb = ea;
                     eb.tstart = side == 0 ? ts_trg[l] : ts_src[l];
+                    eb.segbox = side == 0 ? sb_trg[l] : sb_src[l];
                     eb.seg = SEGE;
                     eb.grid = (int)std::min<long>(GRIDMAX, ea.nbox + (side == 0 ? nt : ns) / SEGE + 1);
                     eb.se = side == 0 ? trg_se : src_se;
@@ -1579,7 +1594,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
                     launch_seg_scan(L, s);
                     L.count = 0;
                 }
-                L.l[L.count++] = SegList{(const int*)D(off), (int)v.size(), se, SEGE, t};
+                L.l[L.count++] = SegList{(const int*)D(off), (int)v.size(), se, SEGE, t, nullptr};
                 return t;
             };
             for (int l = nlevstart; l <= nlevels; ++l) {

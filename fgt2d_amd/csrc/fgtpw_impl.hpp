@@ -153,6 +153,7 @@ struct FormArgs {
     int seg, grid;
     double* partial;
     int pstride;
+    const int* segbox;      // list index of the box of every segment (or nullptr)
 };
 void launch_form(const FormArgs& a, hipStream_t s);
 
@@ -165,6 +166,7 @@ struct SegList {
     const int2* se;
     int seg;
     int* tstart;
+    int* segbox;  // optional: list index of the box of every segment
 };
 struct SegLists {
     SegList l[32];
@@ -203,6 +205,7 @@ struct EvalArgs {
     // point segments (as FormArgs); tstart == nullptr: one block per box
     const int* tstart;
     int seg, grid;
+    const int* segbox;
 };
 void launch_pw2local(const EvalArgs& a, hipStream_t s);
 void launch_taylor_eval(const EvalArgs& a, hipStream_t s);

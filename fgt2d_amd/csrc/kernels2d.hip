@@ -500,7 +500,7 @@ __global__ __launch_bounds__(256) void k_form(FormArgs a, int chunk)
 // heavy steps run over fixed-size segments of each box's point range rather
 // than over boxes: tstart (an exclusive scan of the per-box segment counts)
 // numbers the segments, a persistent grid strides over them, and a block
-// finds its box by binary search in tstart.
+// finds its box in segbox (filled here) or by binary search in tstart.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_seg_scan(SegLists L)
 {
@@ -528,6 +528,8 @@ __global__ __launch_bounds__(1024) void k_seg_scan(SegLists L)
         }
         const int incl = buf[t];
         if (i <= d.n) d.tstart[i] = carry + incl - c;
+        if (d.segbox)
+            for (int j = 0; j < c; ++j) d.segbox[carry + incl - c + j] = i;
         __syncthreads();
         if (t == 1023) carry += incl;
         __syncthreads();
@@ -546,7 +548,8 @@ struct SegRange {
 
 // segment g of a list: its box index li and point range [p0, p1)
 __device__ __forceinline__ SegRange seg_at(const int* tstart, int nbox, const int* boxes,
-                                           const int2* se, int seg, int g)
+                                           const int2* se, int seg, int g,
+                                           const int* segbox = nullptr)
 {
     SegRange r;
     if (!tstart) {
@@ -556,10 +559,15 @@ __device__ __forceinline__ SegRange seg_at(const int* tstart, int nbox, const in
         r.p1 = x.y;
         return r;
     }
-    int lo = 0, hi = nbox - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (tstart[mid] <= g) lo = mid; else hi = mid - 1;
+    int lo = 0;
+    if (segbox) {
+        lo = segbox[g];
+    } else {
+        int hi = nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (tstart[mid] <= g) lo = mid; else hi = mid - 1;
+        }
     }
     int2 x = se[boxes[lo]];
     r.li = lo;
@@ -688,7 +696,90 @@ __global__ __launch_bounds__(256) void k_form_moments(FormArgs a)
             __syncthreads();
         }
         double* out = a.partial + (long)g * a.pstride;  // m x m, x order fastest
-        for (int o = t; o < m * m; o += 256) out[o] = Hf[(o % m) * MP + o / m];
+        for (int o = t; o < m * m; o += 256) out[o] = Hf[(o / m) * MP + o % m];
+    }
+}
+
+// Step 1a, one wavefront per segment (at most WSEG points), in chunks of 32
+// sources: lane u < 32 writes the powers xh^e of source u of the chunk to
+// the wave's slice of LDS and lane 32 + u the powers q yh^e, e < 16 TM,
+// while the next chunk's points are already being loaded.  Every MFMA then
+// takes 4 sources, lane l reading xh^e and q yh^e for e = l%16 (+16 per
+// tile) of source l/16, so D(j,k) = sum_s xh_s^j q_s yh_s^k accumulates in
+// registers.  Lane l holds x orders l/16 + 4r and y orders l%16 of each
+// tile and writes the segment's partial moments, y order fastest.
+constexpr int WSEG = 256;
+
+template <int TM>
+__global__ __launch_bounds__(256) void k_form_wave(FormArgs a)
+{
+    extern __shared__ double sm[];
+    constexpr int MP = 16 * TM, RS = MP + 1, CH = 32;  // odd row stride: no bank conflicts
+    const int m = a.p + 1, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s_l = lane >> 4, e_l = lane & 15, u = lane & 31, side = lane >> 5;
+    double* Px = sm + (size_t)w * 2 * CH * RS;  // CH x RS: xh^e
+    double* Py = Px + CH * RS;                   // CH x RS: q yh^e
+    double* mine = (side ? Py : Px) + u * RS;
+    const int total = seg_count(a.tstart, a.nbox);
+    for (int g = blockIdx.x * 4 + w; g < total; g += gridDim.x * 4) {
+        const SegRange r = seg_at(a.tstart, a.nbox, a.boxes, a.src_se, a.seg, g, a.segbox);
+        const int box = a.boxes[r.li], n = r.p1 - r.p0;
+        const double c = a.cen[2 * box + side], dsq = a.dsq;
+        d4_t acc[TM][TM];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TM; ++j) acc[i][j] = (d4_t){0, 0, 0, 0};
+        // this lane's coordinate (x for side 0, y for side 1) and factor of
+        // the first chunk
+        double v = 0, f = 0;
+        if (u < n) {
+            const long i = r.p0 + u;
+            v = (a.xs[2 * i + side] - c) * dsq;
+            f = side ? a.qs[(long)a.nd * i + a.ind] : 1.0;
+        }
+        for (int c0 = 0; c0 < n; c0 += CH) {
+            double vn = 0, fn = 0;  // next chunk, loaded ahead
+            if (c0 + CH + u < n) {
+                const long i = r.p0 + c0 + CH + u;
+                vn = (a.xs[2 * i + side] - c) * dsq;
+                fn = side ? a.qs[(long)a.nd * i + a.ind] : 1.0;
+            }
+            wave_lds_sync();  // the previous chunk's reads are done
+            double pw = f;
+#pragma unroll
+            for (int e = 0; e < MP; ++e) {
+                mine[e] = pw;
+                pw *= v;
+            }
+            wave_lds_sync();
+            const int nc = min(CH, n - c0);
+            for (int gg = 0; 4 * gg < nc; ++gg) {
+                const int src = 4 * gg + s_l;
+                double av[TM], bv[TM];
+#pragma unroll
+                for (int t = 0; t < TM; ++t) {
+                    av[t] = Px[src * RS + e_l + 16 * t];
+                    bv[t] = Py[src * RS + e_l + 16 * t];
+                }
+#pragma unroll
+                for (int ii = 0; ii < TM; ++ii)
+#pragma unroll
+                    for (int jj = 0; jj < TM; ++jj) acc[ii][jj] = mfma16x16x4(av[ii], bv[jj], acc[ii][jj]);
+            }
+            v = vn;
+            f = fn;
+        }
+        double* out = a.partial + (long)g * a.pstride;
+#pragma unroll
+        for (int ii = 0; ii < TM; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < TM; ++jj)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int xo = s_l + 4 * q + 16 * ii, yo = e_l + 16 * jj;
+                    if (xo < m && yo < m) out[yo + m * xo] = acc[ii][jj][q];
+                }
     }
 }
 
@@ -703,10 +794,10 @@ __global__ __launch_bounds__(256) void k_form_h2pw(FormArgs a)
     double* H = sm;                // m x m, x order fastest
     double* hx = H + m * m;        // complex (m, npw)
     double* T = hx + 2 * m * npw;  // complex (half, m)
-    for (int o = t; o < m * m; o += 256) {
+    for (int o = t; o < m * m; o += 256) {  // partials y order fastest, H x order fastest
         double v = 0;
         for (int g = g0; g < g1; ++g) v += a.partial[(long)g * a.pstride + o];
-        H[o] = v;
+        H[o / m + m * (o % m)] = v;
     }
     for (int o = t; o < 2 * m * npw; o += 256) hx[o] = a.h2xr[o];
     __syncthreads();
@@ -743,13 +834,24 @@ void launch_form(const FormArgs& a, hipStream_t s)
     int m = a.p + 1;
     if (m <= 64 && a.tstart) {
         const int TM = (m + 15) / 16, MP = 16 * TM;
-        size_t lds = std::max((size_t)2 * a.seg * (MP + 1), (size_t)MP * MP) * sizeof(double);
         kmark(KID_FORM_MOMENTS, false, s);
-        switch (TM) {
-        case 1: allow_lds(k_form_moments<1>, lds); k_form_moments<1><<<a.grid, 256, lds, s>>>(a); break;
-        case 2: allow_lds(k_form_moments<2>, lds); k_form_moments<2><<<a.grid, 256, lds, s>>>(a); break;
-        case 3: allow_lds(k_form_moments<3>, lds); k_form_moments<3><<<a.grid, 256, lds, s>>>(a); break;
-        default: allow_lds(k_form_moments<4>, lds); k_form_moments<4><<<a.grid, 256, lds, s>>>(a); break;
+        if (a.segbox && a.seg <= WSEG) {
+            // per wave: two 32 x (MP + 1) power tables
+            size_t lds = (size_t)4 * 2 * 32 * (MP + 1) * sizeof(double);
+            switch (TM) {
+            case 1: allow_lds(k_form_wave<1>, lds); k_form_wave<1><<<a.grid, 256, lds, s>>>(a); break;
+            case 2: allow_lds(k_form_wave<2>, lds); k_form_wave<2><<<a.grid, 256, lds, s>>>(a); break;
+            case 3: allow_lds(k_form_wave<3>, lds); k_form_wave<3><<<a.grid, 256, lds, s>>>(a); break;
+            default: allow_lds(k_form_wave<4>, lds); k_form_wave<4><<<a.grid, 256, lds, s>>>(a); break;
+            }
+        } else {
+            size_t lds = std::max((size_t)2 * a.seg * (MP + 1), (size_t)MP * MP) * sizeof(double);
+            switch (TM) {
+            case 1: allow_lds(k_form_moments<1>, lds); k_form_moments<1><<<a.grid, 256, lds, s>>>(a); break;
+            case 2: allow_lds(k_form_moments<2>, lds); k_form_moments<2><<<a.grid, 256, lds, s>>>(a); break;
+            case 3: allow_lds(k_form_moments<3>, lds); k_form_moments<3><<<a.grid, 256, lds, s>>>(a); break;
+            default: allow_lds(k_form_moments<4>, lds); k_form_moments<4><<<a.grid, 256, lds, s>>>(a); break;
+            }
         }
         kmark(KID_FORM_MOMENTS, true, s);
         size_t lds2 = ((size_t)m * m + 2 * (size_t)m * a.npw + (size_t)a.npw * m) * sizeof(double);
@@ -1185,6 +1287,156 @@ __global__ __launch_bounds__(256) void k_taylor_mfma(EvalArgs a)
     }
 }
 
+// Step 5b on matrix cores, one wavefront per segment (at most WSEG points):
+// the lane's A operands L(4 ks + l/16, l%16 + 16 j) (and the x-derivative
+// variants) are loaded once per segment into registers, the segment's
+// scaled points are staged in the wave's slice of LDS in one coalesced
+// pass, then every tile of 16 points runs as in k_taylor_mfma.
+template <int TM, int PGH>
+__global__ __launch_bounds__(256) void k_taylor_wave(EvalArgs a)
+{
+    constexpr int MP = 16 * TM, KS = MP / 4, NV = PGH >= 3 ? 3 : (PGH >= 2 ? 2 : 1);
+    __shared__ double stage[4][2][WSEG];
+    const int m = a.nl + 1, lane = threadIdx.x & 63;
+    double* sx = stage[threadIdx.x >> 6][0];
+    double* sy = stage[threadIdx.x >> 6][1];
+    const int col = lane & 15, kq = lane >> 4;
+    const int total = seg_count(a.tstart, a.nbox);
+    for (int g = blockIdx.x * 4 + (threadIdx.x >> 6); g < total; g += gridDim.x * 4) {
+        const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g, a.segbox);
+        const int b = a.boxes[sr.li];
+        const double* Lg = a.taylor + (long)sr.li * m * m;  // L(j,k) at j + m k
+        double A[NV][KS][TM];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int jt = 0; jt < TM; ++jt) {
+                const int j = 4 * ks + kq, k = col + 16 * jt;
+                const bool kin = k < m;
+                A[0][ks][jt] = (kin && j < m) ? Lg[j + m * k] : 0.0;
+                if (NV >= 2) A[NV >= 2 ? 1 : 0][ks][jt] = (kin && j + 1 < m) ? (j + 1) * Lg[(j + 1) + m * k] : 0.0;
+                if (NV >= 3)
+                    A[NV >= 3 ? 2 : 0][ks][jt] =
+                        (kin && j + 2 < m) ? (double)((j + 1) * (j + 2)) * Lg[(j + 2) + m * k] : 0.0;
+            }
+        const double cx = a.cen[2 * b], cy = a.cen[2 * b + 1], dsq = a.dsq;
+        const int npts = sr.p1 - sr.p0;
+        wave_lds_sync();  // the previous segment's reads are done
+#pragma unroll
+        for (int k = 0; k < WSEG / 64; ++k) {
+            const int u = lane + 64 * k;
+            if (u < npts) {
+                const long i = sr.p0 + u;
+                sx[u] = (a.pts[2 * i] - cx) * dsq;
+                sy[u] = (a.pts[2 * i + 1] - cy) * dsq;
+            }
+        }
+        wave_lds_sync();
+        for (int base = 0; base < npts; base += 16) {
+            const int ip = base + col;
+            const bool act = ip < npts;
+            const long i = sr.p0 + (act ? ip : 0);
+            const double x = act ? sx[ip] : 0.0;
+            const double y = act ? sy[ip] : 0.0;
+            double p = 1.0;
+            for (int r = 0; r < kq; ++r) p *= x;  // x^(l/16)
+            double x4 = x * x;
+            x4 *= x4;
+            d4_t acc[NV][TM];
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int j = 0; j < TM; ++j) acc[v][j] = (d4_t){0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+#pragma unroll
+                    for (int j = 0; j < TM; ++j) acc[v][j] = mfma16x16x4(A[v][ks][j], p, acc[v][j]);
+                p *= x4;
+            }
+            double yk = 1.0;  // y^kq
+            for (int r = 0; r < kq; ++r) yk *= y;
+            double y4 = y * y;
+            y4 *= y4;
+            double V = 0, Vy = 0, Vyy = 0, U = 0, Uy = 0, Z = 0;
+#pragma unroll
+            for (int j = 0; j < TM; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    V += acc[0][j][r] * yk;
+                    if (PGH >= 2) U += acc[NV >= 2 ? 1 : 0][j][r] * yk;
+                    if (PGH >= 3) Z += acc[NV >= 3 ? 2 : 0][j][r] * yk;
+                    yk *= y4;
+                }
+            if (PGH >= 2) {
+                const int f1 = kq >= 1 ? 0 : 1, f2 = kq >= 2 ? 0 : 1;
+                const int p1 = kq >= 1 ? kq - 1 : 3, p2 = kq >= 2 ? kq - 2 : kq + 2;
+                double e1 = 1.0, e2 = 1.0;
+                for (int r = 0; r < p1; ++r) e1 *= y;
+                for (int r = 0; r < p2; ++r) e2 *= y;
+                int st = 0;
+#pragma unroll
+                for (int j = 0; j < TM; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r, ++st) {
+                        const double k = kq + 4 * r + 16 * j;
+                        if (st >= f1) {
+                            Vy += k * acc[0][j][r] * e1;
+                            if (PGH >= 3) Uy += k * acc[NV >= 2 ? 1 : 0][j][r] * e1;
+                            e1 *= y4;
+                        }
+                        if (PGH >= 3 && st >= f2) {
+                            Vyy += k * (k - 1) * acc[0][j][r] * e2;
+                            e2 *= y4;
+                        }
+                    }
+            }
+            V += __shfl_xor(V, 16);
+            V += __shfl_xor(V, 32);
+            if (PGH >= 2) {
+                U += __shfl_xor(U, 16);
+                U += __shfl_xor(U, 32);
+                Vy += __shfl_xor(Vy, 16);
+                Vy += __shfl_xor(Vy, 32);
+            }
+            if (PGH >= 3) {
+                Z += __shfl_xor(Z, 16);
+                Z += __shfl_xor(Z, 32);
+                Uy += __shfl_xor(Uy, 16);
+                Uy += __shfl_xor(Uy, 32);
+                Vyy += __shfl_xor(Vyy, 16);
+                Vyy += __shfl_xor(Vyy, 32);
+            }
+            if (kq == 0 && act) {
+                a.pot[i * a.nd + a.ind] = V;
+                if (PGH >= 2) {
+                    const long gi = i * 2 * a.nd + a.ind;
+                    a.grad[gi] = U * dsq;
+                    a.grad[gi + a.nd] = Vy * dsq;
+                }
+                if (PGH >= 3) {
+                    const long h = i * 3 * a.nd + a.ind;
+                    a.hess[h] = Z * dsq * dsq;
+                    a.hess[h + a.nd] = Uy * dsq * dsq;
+                    a.hess[h + 2 * a.nd] = Vyy * dsq * dsq;
+                }
+            }
+        }
+    }
+}
+
+template <int TM>
+static void taylor_wave_launch(const EvalArgs& a, hipStream_t s)
+{
+    if (a.ifpgh >= 3)
+        k_taylor_wave<TM, 3><<<a.grid, 256, 0, s>>>(a);
+    else if (a.ifpgh == 2)
+        k_taylor_wave<TM, 2><<<a.grid, 256, 0, s>>>(a);
+    else
+        k_taylor_wave<TM, 1><<<a.grid, 256, 0, s>>>(a);
+}
+
 template <int TM>
 static void taylor_mfma_launch(const EvalArgs& a, hipStream_t s)
 {
@@ -1209,6 +1461,14 @@ void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
     if (a.nbox <= 0) return;
     int m = a.nl + 1;
     kmark(KID_TAYLOR_EVAL, false, s);
+    if (m <= 32 && a.tstart && a.segbox && a.seg <= WSEG) {
+        if (m <= 16)
+            taylor_wave_launch<1>(a, s);
+        else
+            taylor_wave_launch<2>(a, s);
+        kmark(KID_TAYLOR_EVAL, true, s);
+        return;
+    }
     if (m <= 64) {
         switch ((m + 15) / 16) {
         case 1: taylor_mfma_launch<1>(a, s); break;
