@@ -145,21 +145,37 @@ def _transform(dim, delta, eps, sources, charge, targ, ifpgh, ifpghtarg, ifcharg
                rnormal, dipstr, iper):
     lib = load()
     gpu = _is_torch_gpu(sources)
+    if charge is None:
+        ifcharge = 0
+    if rnormal is None or dipstr is None:
+        ifdipole = 0
     if gpu:
         import torch as mod
+
+        def arr(a, n):
+            return a.contiguous().to(mod.float64).reshape(n, -1)
         src = sources.contiguous().to(mod.float64)
-        ns = src.shape[0]
-        chg = charge.contiguous().to(mod.float64).reshape(ns, -1)
         tg = targ.contiguous().to(mod.float64) if targ is not None else None
     else:
         mod = None
+
+        def arr(a, n):
+            return _np.ascontiguousarray(a, dtype=_np.float64).reshape(n, -1)
         src = _np.ascontiguousarray(sources, dtype=_np.float64)
-        ns = src.shape[0]
-        chg = _np.ascontiguousarray(charge, dtype=_np.float64).reshape(ns, -1)
         tg = _np.ascontiguousarray(targ, dtype=_np.float64) if targ is not None else None
+    ns = src.shape[0]
+    chg = arr(charge, ns) if ifcharge else None
+    rn = arr(rnormal, ns) if ifdipole else None
+    ds = arr(dipstr, ns) if ifdipole else None
     if src.ndim != 2 or src.shape[1] != dim:
         raise ValueError("sources must have shape (ns, %d)" % dim)
-    nd = chg.shape[1]
+    if rn is not None and rn.shape[1] != dim:
+        raise ValueError("rnormal must have shape (ns, %d)" % dim)
+    if chg is None and ds is None:
+        raise ValueError("no charges and no dipoles")
+    nd = chg.shape[1] if chg is not None else ds.shape[1]
+    if ds is not None and ds.shape[1] != nd:
+        raise ValueError("dipstr must have shape (ns, nd)")
     nt = 0 if tg is None else tg.shape[0]
     if tg is not None and (tg.ndim != 2 or tg.shape[1] != dim):
         raise ValueError("targ must have shape (nt, %d)" % dim)
@@ -167,8 +183,9 @@ def _transform(dim, delta, eps, sources, charge, targ, ifpgh, ifpghtarg, ifcharg
             _outputs(mod, max(nt, 1), nd, ifpghtarg, src, dim))
     dummy = _np.zeros(8) if mod is None else mod.zeros(8, dtype=mod.float64, device=src.device)
     P = [_ptr(o if o is not None else dummy) for o in outs]
-    args = [nd, float(delta), float(eps), ns, _ptr(src), ifcharge, _ptr(chg), ifdipole,
-            _ptr(rnormal if rnormal is not None else dummy), _ptr(dipstr if dipstr is not None else dummy),
+    args = [nd, float(delta), float(eps), ns, _ptr(src), ifcharge,
+            _ptr(chg if chg is not None else dummy), ifdipole,
+            _ptr(rn if rn is not None else dummy), _ptr(ds if ds is not None else dummy),
             iper, ifpgh, P[0], P[1], P[2], nt, _ptr(tg if tg is not None else dummy), ifpghtarg,
             P[3], P[4], P[5]]
     if gpu:
@@ -203,22 +220,32 @@ def fgt3dpart_direct(delta, dmax, sources, charge, targ, ifpgh=1, pot=None, grad
     return pot, grad, hess
 
 
-def fgt2dpart_direct(delta, eps, sources, charge, targ, ifpgh=1, pot=None, grad=None, hess=None):
+def fgt2dpart_direct(delta, eps, sources, charge, targ, ifpgh=1, pot=None, grad=None, hess=None,
+                     rnormal=None, dipstr=None):
     """fgt2dpart_direct_vec (fgt2d.f:1354): adds the direct sum with cutoff
     r^2 < ln(1/eps) delta over all sources to pot (grad, hess) at all
-    targets, on the GPU; returns the arrays."""
+    targets, on the GPU; returns the arrays.  charge=None: no charges;
+    rnormal (ns, 2) and dipstr (ns, nd): dipoles."""
     lib = load()
     src = _np.ascontiguousarray(sources, dtype=_np.float64)
-    chg = _np.ascontiguousarray(charge, dtype=_np.float64).reshape(src.shape[0], -1)
+    ns = src.shape[0]
+    chg = None if charge is None else _np.ascontiguousarray(charge, dtype=_np.float64).reshape(ns, -1)
+    rn = ds = None
+    if rnormal is not None:
+        rn = _np.ascontiguousarray(rnormal, dtype=_np.float64).reshape(ns, 2)
+        ds = _np.ascontiguousarray(dipstr, dtype=_np.float64).reshape(ns, -1)
     tg = _np.ascontiguousarray(targ, dtype=_np.float64)
-    nd, ns, nt = chg.shape[1], src.shape[0], tg.shape[0]
+    nd = chg.shape[1] if chg is not None else ds.shape[1]
+    nt = tg.shape[0]
     pot = _np.zeros((nt, nd)) if pot is None else pot
     grad = _np.zeros((nt, 2, nd)) if grad is None else grad
     hess = _np.zeros((nt, 3, nd)) if hess is None else hess
     dummy = _np.zeros(8)
-    _check(lib.fgtpw_fgt2dpart_direct(nd, float(delta), float(eps), 1, ns, 1, nt, _ptr(src), 1,
-                                      _ptr(chg), 0, _ptr(dummy), _ptr(dummy), _ptr(tg), ifpgh,
-                                      _ptr(pot), _ptr(grad), _ptr(hess)))
+    _check(lib.fgtpw_fgt2dpart_direct(
+        nd, float(delta), float(eps), 1, ns, 1, nt, _ptr(src), int(chg is not None),
+        _ptr(chg if chg is not None else dummy), int(rn is not None),
+        _ptr(rn if rn is not None else dummy), _ptr(ds if ds is not None else dummy), _ptr(tg),
+        ifpgh, _ptr(pot), _ptr(grad), _ptr(hess)))
     return pot, grad, hess
 
 
@@ -347,14 +374,20 @@ class Server:
         _check(self._lib.fgtpw_serve_start(int(slots), float(hbm_budget_gib)))
         self._out = {}
 
-    def submit(self, dim, delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
-        """Queues one transform; returns its ticket."""
+    def submit(self, dim, delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0,
+               rnormal=None, dipstr=None):
+        """Queues one transform; returns its ticket.  charge=None: no
+        charges; rnormal (ns, dim) and dipstr (ns, nd): dipoles."""
         src = _np.ascontiguousarray(sources, dtype=_np.float64)
         if src.ndim != 2 or src.shape[1] != dim:
             raise ValueError("sources must have shape (ns, %d)" % dim)
         ns = src.shape[0]
-        chg = _np.ascontiguousarray(charge, dtype=_np.float64).reshape(ns, -1)
-        nd = chg.shape[1]
+        chg = None if charge is None else _np.ascontiguousarray(charge, dtype=_np.float64).reshape(ns, -1)
+        rn = ds = None
+        if rnormal is not None:
+            rn = _np.ascontiguousarray(rnormal, dtype=_np.float64).reshape(ns, dim)
+            ds = _np.ascontiguousarray(dipstr, dtype=_np.float64).reshape(ns, -1)
+        nd = chg.shape[1] if chg is not None else ds.shape[1]
         tg = None if targ is None else _np.ascontiguousarray(targ, dtype=_np.float64)
         nt = 0 if tg is None else tg.shape[0]
         if tg is not None and (tg.ndim != 2 or tg.shape[1] != dim):
@@ -362,7 +395,10 @@ class Server:
         outs = (_outputs(None, ns, nd, ifpgh, None, dim) +
                 _outputs(None, max(nt, 1), nd, ifpghtarg if nt else 0, None, dim))
         c = _Call(dim=dim, nd=nd, delta=float(delta), eps=float(eps), ns=ns,
-                  sources=src.ctypes.data, ifcharge=1, charge=chg.ctypes.data, ifdipole=0,
+                  sources=src.ctypes.data, ifcharge=int(chg is not None),
+                  charge=None if chg is None else chg.ctypes.data, ifdipole=int(rn is not None),
+                  rnormal=None if rn is None else rn.ctypes.data,
+                  dipstr=None if ds is None else ds.ctypes.data,
                   ifpgh=ifpgh, nt=nt, targ=None if tg is None else tg.ctypes.data,
                   ifpghtarg=ifpghtarg if nt else 0)
         for name, o in zip(("pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"), outs):

@@ -41,9 +41,10 @@ int g_device = -1;
 
 const char* const kStages[] = {"bbox",  "keys_sort", "leafdepth", "host_tree", "assign",
                                "ranges", "form",     "merge",     "m2l",       "split",
-                               "eval",  "direct",    "scatter"};
+                               "eval",  "direct",    "scatter",   "host_build"};
+// host_build: the box tree itself, part of host_tree (host clock)
 enum Stage { S_BBOX, S_KEYS, S_LEAF, S_HOST, S_ASSIGN, S_RANGES, S_FORM, S_MERGE, S_M2L, S_SPLIT,
-             S_EVAL, S_DIRECT, S_SCATTER, S_N };
+             S_EVAL, S_DIRECT, S_SCATTER, S_HOST_BUILD, S_N };
 }  // namespace
 
 thread_local KernelClock* t_kclock = nullptr;
@@ -87,6 +88,11 @@ struct Workspace {
     void* pin = nullptr;
     size_t pin_cap = 0;
     hipStream_t own = nullptr;
+    // side streams (default priority): coarse levels of formation and
+    // evaluation and the near field overlap the finest level's work
+    hipStream_t aux[2] = {};
+    std::vector<hipEvent_t> order_ev;  // stream-ordering events, reused from call to call
+    size_t n_order = 0;
     hipEvent_t ev[2 * S_N] = {};
     std::vector<hipEvent_t> kev;  // per-kernel timing events, grown on demand
     // host scratch of the plan, reused from call to call
@@ -115,7 +121,20 @@ struct Workspace {
         }
         return (T*)b.p;
     }
-    // frees every device and pinned buffer, the events and the stream
+    // work queued on `to` after this call waits for the work queued on
+    // `from` before it
+    void order(hipStream_t from, hipStream_t to)
+    {
+        if (n_order == order_ev.size()) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            order_ev.push_back(e);
+        }
+        hipEvent_t e = order_ev[n_order++];
+        HIPCHECK(hipEventRecord(e, from));
+        HIPCHECK(hipStreamWaitEvent(to, e, 0));
+    }
+    // frees every device and pinned buffer, the events and the streams
     void release()
     {
         for (auto& kv : dev)
@@ -129,6 +148,11 @@ struct Workspace {
         for (auto& e : kev)
             if (e) (void)hipEventDestroy(e);
         kev.clear();
+        for (auto& e : order_ev) (void)hipEventDestroy(e);
+        order_ev.clear();
+        for (auto& a : aux)
+            if (a) (void)hipStreamDestroy(a);
+        aux[0] = aux[1] = nullptr;
         if (own) (void)hipStreamDestroy(own);
         own = nullptr;
         tab.valid = tab3.valid = false;
@@ -150,6 +174,7 @@ Workspace* new_workspace()
 {
     Workspace* w = new Workspace();
     HIPCHECK(hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking));
+    for (auto& a : w->aux) HIPCHECK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
     for (auto& e : w->ev) HIPCHECK(hipEventCreate(&e));
     return w;
 }
@@ -220,7 +245,7 @@ struct Timer {
     Workspace& W;
     hipStream_t s;
     bool on;
-    double host_ms = 0;
+    double host_ms = 0, build_ms = 0;
     bool used[S_N] = {};
     LaunchClock clock;
     const int2 *src_se = nullptr, *trg_se = nullptr;
@@ -235,13 +260,13 @@ struct Timer {
     {
         if (on) clock.ctx(l, b, sd, u);
     }
-    void begin(int k)
+    void begin(int k, hipStream_t st = nullptr)
     {
-        if (on) { HIPCHECK(hipEventRecord(W.ev[2 * k], s)); used[k] = true; }
+        if (on) { HIPCHECK(hipEventRecord(W.ev[2 * k], st ? st : s)); used[k] = true; }
     }
-    void end(int k)
+    void end(int k, hipStream_t st = nullptr)
     {
-        if (on) HIPCHECK(hipEventRecord(W.ev[2 * k + 1], s));
+        if (on) HIPCHECK(hipEventRecord(W.ev[2 * k + 1], st ? st : s));
     }
     void finish()
     {
@@ -255,6 +280,7 @@ struct Timer {
             t_stage_ms[k] = ms;
         }
         t_stage_ms[S_HOST] = host_ms;
+        t_stage_ms[S_HOST_BUILD] = build_ms;
         std::vector<int2> hs, ht;
         if (nb > 0) {
             hs.resize(nb);
@@ -277,6 +303,8 @@ struct Timer {
     }
 };
 
+// one transform, device arrays; chg == nullptr: no charges, rn == nullptr:
+// no dipoles (rn (2 or 3, ns) normals, ds (nd, ns) strengths)
 struct Call {
     int nd;
     double delta, eps;
@@ -289,6 +317,8 @@ struct Call {
     const double* targ;
     int ifpghtarg;
     double *pottarg, *gradtarg, *hesstarg;
+    const double* rn = nullptr;
+    const double* ds = nullptr;
 };
 
 void copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t s)
@@ -540,10 +570,31 @@ const Tables3D& tables3d(Workspace& W, int npw, double pmax, int ntermmax, int n
 void run_large_delta(Workspace& W, const Call& q, hipStream_t s, int npwlevel, double bs0,
                      const double* cen0, Timer& tm);
 
+// outputs of a transform without charges or dipoles: zero (fgt2d.f:882-959
+// and fgt2dpart_direct_vec form and add nothing)
+void zero_outputs(const Call& q, int dim, hipStream_t s)
+{
+    const size_t nd = q.nd, ng = dim, nh = dim == 2 ? 3 : 6;
+    if (q.ifpgh >= 1) launch_zero(q.pot, nd * q.ns, s);
+    if (q.ifpgh >= 2) launch_zero(q.grad, ng * nd * q.ns, s);
+    if (q.ifpgh >= 3) launch_zero(q.hess, nh * nd * q.ns, s);
+    if (q.nt > 0) {
+        if (q.ifpghtarg >= 1) launch_zero(q.pottarg, nd * q.nt, s);
+        if (q.ifpghtarg >= 2) launch_zero(q.gradtarg, ng * nd * q.nt, s);
+        if (q.ifpghtarg >= 3) launch_zero(q.hesstarg, nh * nd * q.nt, s);
+    }
+}
+
 void run2d(Workspace& W, const Call& q, hipStream_t s)
 {
+    W.n_order = 0;
     Timer tm(W, s);
     const int ns = q.ns, nt = q.nt, nd = q.nd;
+    if (!q.chg && !q.rn) {
+        zero_outputs(q, 2, s);
+        return;
+    }
+    const int ifdipole = q.rn ? 1 : 0;
     // ---------------- bounding box (pts_tree_boxsize0) ----------------
     tm.begin(S_BBOX);
     int nblk = (int)std::min<long>(1024, std::max<long>(1, ((long)ns + nt + 255) / 256));
@@ -635,7 +686,17 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     double* xs = W.get<double>("xs", 2 * (size_t)ns);
     double* qs = W.get<double>("qs", (size_t)nd * ns);
     tm.ctx(-1, nullptr, -1, ns);
-    launch_gather(q.src, q.chg, nd, isrc, ns, xs, qs, s);
+    if (q.chg)
+        launch_gather(q.src, q.chg, nd, isrc, ns, xs, qs, s);
+    else {
+        launch_gather(q.src, nullptr, nd, isrc, ns, xs, nullptr, s);
+        launch_zero(qs, (size_t)nd * ns, s);
+    }
+    double* wsd = nullptr;  // dipole weights d n in sorted order
+    if (q.rn) {
+        wsd = W.get<double>("dip_w", 2 * (size_t)nd * ns);
+        launch_gather_dip(q.rn, q.ds, nd, isrc, ns, wsd, s);
+    }
     unsigned* itarg = nullptr;
     u64* kt2 = nullptr;
     double* xt = nullptr;
@@ -656,18 +717,19 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     W.last_valid = false;
     Tree2D& T = W.last;
     build_tree2d(T, leaves, nlmax, npwlevel, bs0, cen0);
+    tm.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
     W.last_ns = ns;
     W.last_nt = nt;
     W.last_dim = 2;
     const int nb = T.nboxes(), nlevels = T.nlevels, nleaves = (int)T.leaf_of_rank.size();
     const bool use_src = q.ifpgh > 0, use_trg = q.ifpghtarg > 0 && nt > 0;
     const int npg = std::max(q.ifpgh, q.ifpghtarg);
-    const int nadd = npg;  // ifdipole = 0
+    const int nadd = ifdipole + npg;  // fgt2d.f:340
     std::vector<int> nlocal(nlevels + 1, 0);
     int ntermmax = 0;
     for (int i = nlevstart; i <= nlevels; ++i) {
         nlocal[i] = hermite_terms(2, T.bsz[i], q.delta, q.eps);
-        if (q.ifpgh >= 2 || q.ifpghtarg >= 2) nlocal[i] += nadd;
+        if (ifdipole || q.ifpgh >= 2 || q.ifpghtarg >= 2) nlocal[i] += nadd;
         ntermmax = std::max(ntermmax, nlocal[i]);
     }
     double pmax = 0;
@@ -860,6 +922,47 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         if (q.ifpghtarg >= 3) { ht = W.get<double>("hess_t", 3 * (size_t)nd * wt_); launch_zero(ht, 3 * (size_t)nd * wt_, s); }
     }
 
+    // ---------------- step 6: near field, cutoff eps/90 (fgt2d.f:1316) ----------------
+    // on a side stream, concurrently with steps 1-4; it accumulates into the
+    // zeroed outputs before any evaluation does (the evaluations wait for it)
+    hipStream_t s_nf = W.aux[1], s_lv = W.aux[0];
+    W.order(s, s_nf);
+    tm.begin(S_DIRECT, s_nf);
+    if (!unit_ids.empty()) {
+        DirectArgs da{};
+        da.units = (const int*)D(o_units);
+        da.nunits = (int)unit_ids.size();
+        da.list_ptr = (const int*)D(o_uptr);
+        da.list = (const int*)D(o_ulist);
+        da.src_se = src_se;
+        da.xs = xs;
+        da.qs = qs;
+        da.nd = nd;
+        da.ws = wsd;
+        da.delta = q.delta;
+        double e90 = q.eps / 90;
+        da.dmax = std::log(1.0 / e90) * q.delta;
+        if (use_trg) {
+            DirectArgs d2 = da;
+            d2.tgt_se = trg_se;
+            d2.xt = xt;
+            d2.ifpgh = q.ifpghtarg;
+            d2.pot = pt; d2.grad = gt; d2.hess = ht;
+            tm.ctx(npwlevel, nullptr, -1, nt);
+            launch_direct(d2, s_nf);
+        }
+        if (use_src) {
+            DirectArgs d2 = da;
+            d2.tgt_se = src_se;
+            d2.xt = xs;
+            d2.ifpgh = q.ifpgh;
+            d2.pot = ps; d2.grad = gs; d2.hess = hs;
+            tm.ctx(npwlevel, nullptr, -1, ns);
+            launch_direct(d2, s_nf);
+        }
+    }
+    tm.end(S_DIRECT, s_nf);
+
     // ---------------- expansions: steps 1-5 ----------------
     if (expansions) {
         size_t per = 2 * (size_t)nexp * nd;
@@ -874,11 +977,12 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         const double* d_ts = tab.ts;
         const double* d_wms = tab.wms;
         const double* d_wm2l = tab.wm2l;
-        size_t maxeval = 0;
+        // Taylor coefficients of every evaluated box, one region per level
+        // (levels are evaluated concurrently)
+        std::vector<size_t> o_taylor(nlevels + 2, 0);
         for (int l = 0; l <= nlevels; ++l)
-            if (nlocal[l] <= 50)
-                maxeval = std::max(maxeval, eval_l[l].size() * (size_t)(nlocal[l] + 1) * (nlocal[l] + 1));
-        double* taylor = W.get<double>("taylor", maxeval);
+            o_taylor[l + 1] = o_taylor[l] + (nlocal[l] <= 50 ? eval_l[l].size() * (size_t)(nlocal[l] + 1) * (nlocal[l] + 1) : 0);
+        double* taylor = W.get<double>("taylor", o_taylor[nlevels + 1]);
         // segment numbering of every formation and evaluation list (one
         // launch): formation over sources in segments of SEGF, evaluation
         // over targets / sources in segments of SEGE
@@ -924,26 +1028,32 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             }
             launch_seg_scan(L, s);
         }
-        size_t maxpart = 0;
-        for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l) {
+        // partial moments of every segment, one region per level (the
+        // finest level is formed on s, the others concurrently on s_lv)
+        std::vector<size_t> o_part(nlevels + 2, 0);
+        for (int l = 0; l <= nlevels; ++l) {
             size_t m = nlocal[l] + 1;
-            maxpart = std::max(maxpart, (form_l[l].size() + ns / segf(l) + 1) * m * m);
+            o_part[l + 1] = o_part[l] + (l >= std::max(npwlevel + 1, 0) && !form_l[l].empty()
+                                             ? (form_l[l].size() + ns / segf(l) + 1) * m * m
+                                             : 0);
         }
-        double* partial = W.get<double>("form_partial", maxpart);
+        double* partial = W.get<double>("form_partial", o_part[nlevels + 1]);
+        tm.begin(S_FORM);
+        W.order(s, s_lv);
         for (int ind = 0; ind < nd; ++ind) {
-            tm.begin(S_FORM);
             for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l) {
                 if (form_l[l].empty()) continue;
                 const int nb_l = (int)form_l[l].size(), m = nlocal[l] + 1;
                 FormArgs fa{(const int*)D(o_form[l]), nb_l, d_cen, src_se, d_slot, xs, qs, nd, ind,
                             dsq, nlocal[l], d_h2x, d_h2xr, d_rsq, npw, mp, ts_form[l], segf(l),
-                            (int)std::min<long>(GRIDMAX, nb_l + ns / segf(l) + 1), partial, m * m,
-                            sb_form[l]};
+                            (int)std::min<long>(GRIDMAX, nb_l + ns / segf(l) + 1),
+                            partial + o_part[l], m * m, sb_form[l], wsd};
                 tm.ctx(l, &form_l[l], 0);
-                launch_form(fa, s);
+                launch_form(fa, l == nlevels ? s : s_lv);
             }
-            tm.end(S_FORM);
         }
+        W.order(s_lv, s);
+        tm.end(S_FORM);
         tm.begin(S_MERGE);
         for (int l = nlevels - 1; l >= nlevstart; --l)
             tm.ctx(l, &merge_l[l], 2), launch_merge((const int*)D(o_merge[l]), (int)merge_l[l].size(), d_child, d_cen, src_se,
@@ -955,14 +1065,13 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                    use_src, use_trg, d_slot, d_wm2l, npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0],
                    nlevstart == npwlevel, nd, nexp, mp, lc, s);
         tm.end(S_M2L);
-        tm.begin(S_SPLIT);
-        for (int l = nlevstart; l <= nlevels - 1; ++l)
-            tm.ctx(l, &split_l[l], 2), launch_split((const int*)D(o_split[l]), (int)split_l[l].size(), d_par, d_cen, d_slot,
-                         d_wms, 4 * (nlevels - l - 1), nd, nexp, lc, s);
-        tm.end(S_SPLIT);
-        tm.begin(S_EVAL);
-        for (int ind = 0; ind < nd; ++ind)
-            for (int l = nlevstart; l <= nlevels; ++l) {
+        // the evaluations accumulate after the near field
+        W.order(s_nf, s);
+        W.order(s_nf, s_lv);
+        // level l is evaluated (on s_lv, the finest level on s) once the
+        // split that completes its local expansions is queued
+        auto eval_level = [&](int l, hipStream_t st) {
+            for (int ind = 0; ind < nd; ++ind) {
                 if (eval_l[l].empty()) continue;
                 EvalArgs ea{};
                 ea.boxes = (const int*)D(o_eval[l]);
@@ -977,10 +1086,10 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                 ea.ts = d_ts;
                 ea.npw = npw;
                 ea.loc = lc;
-                ea.taylor = taylor;
+                ea.taylor = taylor + o_taylor[l];
                 bool pwroute = nlocal[l] > 50;  // nlocalswitch, fgt2d.f:855
                 tm.ctx(l, &eval_l[l], 2);
-                if (!pwroute) launch_pw2local(ea, s);
+                if (!pwroute) launch_pw2local(ea, st);
                 for (int side = 0; side < 2; ++side) {
                     if (side == 0 && !use_trg) continue;
                     if (side == 1 && !use_src) continue;
@@ -997,49 +1106,29 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                     eb.hess = side == 0 ? ht : hs;
                     tm.ctx(l, &eval_l[l], side == 0 ? 1 : 0);
                     if (pwroute)
-                        launch_pw_eval(eb, s);
+                        launch_pw_eval(eb, st);
                     else
-                        launch_taylor_eval(eb, s);
+                        launch_taylor_eval(eb, st);
                 }
             }
+        };
+        tm.begin(S_SPLIT);
+        for (int l = nlevstart; l <= nlevels - 1; ++l) {
+            W.order(s, s_lv);
+            eval_level(l, s_lv);
+            tm.ctx(l, &split_l[l], 2);
+            launch_split((const int*)D(o_split[l]), (int)split_l[l].size(), d_par, d_cen, d_slot,
+                         d_wms, 4 * (nlevels - l - 1), nd, nexp, lc, s);
+        }
+        tm.end(S_SPLIT);
+        tm.begin(S_EVAL);
+        eval_level(nlevels, s);
+        W.order(s_lv, s);
         tm.end(S_EVAL);
     }
 
-    // ---------------- step 6: near field, cutoff eps/90 (fgt2d.f:1316) ----------------
-    tm.begin(S_DIRECT);
-    if (!unit_ids.empty()) {
-        DirectArgs da{};
-        da.units = (const int*)D(o_units);
-        da.nunits = (int)unit_ids.size();
-        da.list_ptr = (const int*)D(o_uptr);
-        da.list = (const int*)D(o_ulist);
-        da.src_se = src_se;
-        da.xs = xs;
-        da.qs = qs;
-        da.nd = nd;
-        da.delta = q.delta;
-        double e90 = q.eps / 90;
-        da.dmax = std::log(1.0 / e90) * q.delta;
-        if (use_trg) {
-            DirectArgs d2 = da;
-            d2.tgt_se = trg_se;
-            d2.xt = xt;
-            d2.ifpgh = q.ifpghtarg;
-            d2.pot = pt; d2.grad = gt; d2.hess = ht;
-            tm.ctx(npwlevel, nullptr, -1, nt);
-            launch_direct(d2, s);
-        }
-        if (use_src) {
-            DirectArgs d2 = da;
-            d2.tgt_se = src_se;
-            d2.xt = xs;
-            d2.ifpgh = q.ifpgh;
-            d2.pot = ps; d2.grad = gs; d2.hess = hs;
-            tm.ctx(npwlevel, nullptr, -1, ns);
-            launch_direct(d2, s);
-        }
-    }
-    tm.end(S_DIRECT);
+    W.order(s_nf, s);
+    W.order(s_lv, s);
 
     // ---------------- back to input order ----------------
     tm.begin(S_SCATTER);
@@ -1101,8 +1190,15 @@ void run_large_delta(Workspace& W, const Call& q, hipStream_t s, int npwlevel, d
     HIPCHECK(hipMemcpyAsync(dint, hp, sizeof(hostarr), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemcpyAsync(dcen, hp + 64, 2 * sizeof(double), hipMemcpyHostToDevice, s));
     tm.begin(S_FORM);
+    double* wsd = nullptr;  // dipole weights d n in input order
+    if (q.rn) {
+        unsigned* id = W.get<unsigned>("ld_iota", q.ns);
+        launch_iota(id, q.ns, s);
+        wsd = W.get<double>("ld_dipw", 2 * (size_t)nd * q.ns);
+        launch_gather_dip(q.rn, q.ds, nd, id, q.ns, wsd, s);
+    }
     for (int ind = 0; ind < nd; ++ind)
-        launch_formlc(q.src, q.chg, nd, ind, q.ns, cen0[0], cen0[1], dsq, nl, part, nparts,
+        launch_formlc(q.src, q.chg, wsd, nd, ind, q.ns, cen0[0], cen0[1], dsq, nl, part, nparts,
                       L + (size_t)ind * m * m, s);
     tm.end(S_FORM);
     tm.begin(S_EVAL);
@@ -1242,6 +1338,10 @@ void run_large_delta3(Workspace& W, const Call& q, hipStream_t s, int npwlevel, 
 // dmax = ln(1/eps) delta and the 3D modified list1 (tree_routs3d.f:511-620).
 void run3d(Workspace& W, const Call& q, hipStream_t s)
 {
+    if (!q.chg) {  // no charges (dipoles are refused before): nothing to add
+        zero_outputs(q, 3, s);
+        return;
+    }
     Timer tm(W, s);
     const int ns = q.ns, nt = q.nt, nd = q.nd;
     tm.begin(S_BBOX);
@@ -1806,8 +1906,15 @@ void validate(int nd, double delta, double eps, int ns, int ifcharge, int ifdipo
     if (!(eps > 0) || !(eps < 1)) fail(FGTPW_EINVAL, "eps must be in (0,1)");
     if (ifpgh < 0 || ifpgh > 3 || ifpghtarg < 0 || ifpghtarg > 3)
         fail(FGTPW_EINVAL, "ifpgh and ifpghtarg must be in 0..3");
-    if (ifdipole == 1) fail(FGTPW_EUNSUPPORTED, "dipole sources are not supported in this build");
-    if (ifcharge != 1) fail(FGTPW_EINVAL, "ifcharge must be 1");
+    if (ifcharge != 0 && ifcharge != 1) fail(FGTPW_EINVAL, "ifcharge must be 0 or 1");
+    if (ifdipole != 0 && ifdipole != 1) fail(FGTPW_EINVAL, "ifdipole must be 0 or 1");
+}
+
+void check_strengths(int ifcharge, const double* charge, int ifdipole, const double* rnormal,
+                     const double* dipstr)
+{
+    if (ifcharge && !charge) fail(FGTPW_EINVAL, "charge is NULL");
+    if (ifdipole && (!rnormal || !dipstr)) fail(FGTPW_EINVAL, "rnormal or dipstr is NULL");
 }
 
 void select_device()
@@ -1840,14 +1947,16 @@ int fgtpw_fgt2d_device(int nd, double delta, double eps, int ns, const double* s
                        double* hess, int nt, const double* targ, int ifpghtarg, double* pottarg,
                        double* gradtarg, double* hesstarg, void* stream)
 {
-    (void)rnormal; (void)dipstr; (void)iper;
+    (void)iper;
     return run_guarded([&] {
         validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
+        check_strengths(ifcharge, charge, ifdipole, rnormal, dipstr);
         std::lock_guard<std::mutex> lk(g_mutex);
         select_device();
         Workspace& W = workspace();
-        Call q{nd, delta, eps, ns, sources, charge, ifpgh, pot, grad, hess,
+        Call q{nd, delta, eps, ns, sources, ifcharge ? charge : nullptr, ifpgh, pot, grad, hess,
                nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg};
+        if (ifdipole) { q.rn = rnormal; q.ds = dipstr; }
         hipStream_t s = (hipStream_t)stream;
         run2d(W, q, s);
         HIPCHECK(hipStreamSynchronize(s));
@@ -1860,19 +1969,28 @@ namespace fgtpw {
 namespace {
 // host-array entry (fgt2d / fgt3d): copies in, runs on the library's own
 // stream, copies the requested outputs back
+// charge == nullptr: no charges; rnormal == nullptr: no dipoles
 void host_transform(Workspace& W, int dim, int nd, double delta, double eps, int ns,
-                    const double* sources, const double* charge, int ifpgh, double* pot,
-                    double* grad, double* hess, int nt, const double* targ, int ifpghtarg,
-                    double* pottarg, double* gradtarg, double* hesstarg)
+                    const double* sources, const double* charge, const double* rnormal,
+                    const double* dipstr, int ifpgh, double* pot, double* grad, double* hess,
+                    int nt, const double* targ, int ifpghtarg, double* pottarg,
+                    double* gradtarg, double* hesstarg)
 {
     hipStream_t s = W.own;
     const size_t ng = dim, nh = dim == 2 ? 3 : 6;
     const size_t nps = dim * (size_t)ns, npt = dim * (size_t)nt;
     double* d_src = W.get<double>("in_src", nps);
-    double* d_chg = W.get<double>("in_chg", (size_t)nd * ns);
+    double* d_chg = charge ? W.get<double>("in_chg", (size_t)nd * ns) : nullptr;
+    double* d_rn = rnormal ? W.get<double>("in_rn", nps) : nullptr;
+    double* d_ds = rnormal ? W.get<double>("in_ds", (size_t)nd * ns) : nullptr;
     double* d_trg = W.get<double>("in_trg", std::max<size_t>(npt, 3));
     HIPCHECK(hipMemcpyAsync(d_src, sources, nps * 8, hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemcpyAsync(d_chg, charge, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+    if (charge)
+        HIPCHECK(hipMemcpyAsync(d_chg, charge, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+    if (rnormal) {
+        HIPCHECK(hipMemcpyAsync(d_rn, rnormal, nps * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d_ds, dipstr, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+    }
     if (nt > 0) HIPCHECK(hipMemcpyAsync(d_trg, targ, npt * 8, hipMemcpyHostToDevice, s));
     const size_t wps = ifpgh > 0 ? (size_t)ns : 0, wpt = ifpghtarg > 0 ? (size_t)nt : 0;
     double* o_p = W.get<double>("out_pot", std::max<size_t>(nd * wps, 1));
@@ -1883,6 +2001,8 @@ void host_transform(Workspace& W, int dim, int nd, double delta, double eps, int
     double* o_ht = W.get<double>("out_hesst", std::max<size_t>(nh * nd * wpt, 1));
     Call q{nd, delta, eps, ns, d_src, d_chg, ifpgh, o_p, o_g, o_h,
            nt, d_trg, ifpghtarg, o_pt, o_gt, o_ht};
+    q.rn = d_rn;
+    q.ds = d_ds;
     if (dim == 2)
         run2d(W, q, s);
     else
@@ -1900,16 +2020,21 @@ void host_transform(Workspace& W, int dim, int nd, double delta, double eps, int
 }
 
 int run_host(int dim, int nd, double delta, double eps, int ns, const double* sources,
-             int ifcharge, const double* charge, int ifdipole, int ifpgh, double* pot,
-             double* grad, double* hess, int nt, const double* targ, int ifpghtarg,
-             double* pottarg, double* gradtarg, double* hesstarg)
+             int ifcharge, const double* charge, int ifdipole, const double* rnormal,
+             const double* dipstr, int ifpgh, double* pot, double* grad, double* hess, int nt,
+             const double* targ, int ifpghtarg, double* pottarg, double* gradtarg,
+             double* hesstarg)
 {
     return run_guarded([&] {
         validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
+        check_strengths(ifcharge, charge, ifdipole, rnormal, dipstr);
+        if (dim == 3 && ifdipole)
+            fail(FGTPW_EUNSUPPORTED, "dipole sources in 3D are not supported in this build");
         std::lock_guard<std::mutex> lk(g_mutex);
         select_device();
-        host_transform(workspace(), dim, nd, delta, eps, ns, sources, charge, ifpgh, pot, grad,
-                       hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg);
+        host_transform(workspace(), dim, nd, delta, eps, ns, sources, ifcharge ? charge : nullptr,
+                       ifdipole ? rnormal : nullptr, dipstr, ifpgh, pot, grad, hess, nt, targ,
+                       ifpghtarg, pottarg, gradtarg, hesstarg);
     });
 }
 
@@ -1917,13 +2042,17 @@ int run_host(int dim, int nd, double delta, double eps, int ns, const double* so
 // a cutoff r^2 < dmax, accumulated into host arrays
 int run_part_direct(int dim, int nd, double delta, double dmax, int istart, int iend, int jstart,
                     int jend, const double* source, int ifcharge, const double* charge,
-                    int ifdipole, const double* targ, int ifpgh, double* pot, double* grad,
-                    double* hess)
+                    int ifdipole, const double* rnormal, const double* dipstr,
+                    const double* targ, int ifpgh, double* pot, double* grad, double* hess)
 {
     return run_guarded([&] {
         if (nd < 1) fail(FGTPW_EINVAL, "nd must be >= 1");
-        if (ifdipole == 1) fail(FGTPW_EUNSUPPORTED, "dipole sources are not supported in this build");
-        if (ifcharge != 1) fail(FGTPW_EINVAL, "ifcharge must be 1");
+        if (ifcharge != 0 && ifcharge != 1) fail(FGTPW_EINVAL, "ifcharge must be 0 or 1");
+        if (ifdipole != 0 && ifdipole != 1) fail(FGTPW_EINVAL, "ifdipole must be 0 or 1");
+        if (dim == 3 && ifdipole)
+            fail(FGTPW_EUNSUPPORTED, "dipole sources in 3D are not supported in this build");
+        check_strengths(ifcharge, charge, ifdipole, rnormal, dipstr);
+        if (!ifcharge && !ifdipole) return;  // fgt2dpart_direct_vec: no branch applies
         if (ifpgh < 1 || ifpgh > 3) return;  // the reference does nothing for other values
         if (!(delta > 0) || std::isnan(dmax)) fail(FGTPW_EINVAL, "delta must be positive");
         const int ns = iend - istart + 1, nt = jend - jstart + 1;
@@ -1935,16 +2064,30 @@ int run_part_direct(int dim, int nd, double delta, double dmax, int istart, int 
         hipStream_t s = W.own;
         const size_t ng = dim, nh = dim == 2 ? 3 : 6;
         const double* src = source + dim * (size_t)(istart - 1);
-        const double* chg = charge + (size_t)nd * (istart - 1);
         const double* tg = targ + dim * (size_t)(jstart - 1);
         double* d_src = W.get<double>("pd_src", dim * (size_t)ns);
-        double* d_chg = W.get<double>("pd_chg", (size_t)nd * ns);
+        double* d_chg = ifcharge ? W.get<double>("pd_chg", (size_t)nd * ns) : nullptr;
+        double* d_ws = nullptr;  // dipole weights d n, (ns, nd, 2)
         double* d_tg = W.get<double>("pd_tg", dim * (size_t)nt);
         double* d_p = W.get<double>("pd_pot", (size_t)nd * nt);
         double* d_g = W.get<double>("pd_grad", ng * nd * nt);
         double* d_h = W.get<double>("pd_hess", nh * nd * nt);
         HIPCHECK(hipMemcpyAsync(d_src, src, dim * (size_t)ns * 8, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d_chg, chg, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+        if (ifcharge)
+            HIPCHECK(hipMemcpyAsync(d_chg, charge + (size_t)nd * (istart - 1), (size_t)nd * ns * 8,
+                                    hipMemcpyHostToDevice, s));
+        if (ifdipole) {
+            double* d_rn = W.get<double>("pd_rn", dim * (size_t)ns);
+            double* d_ds = W.get<double>("pd_ds", (size_t)nd * ns);
+            unsigned* id = W.get<unsigned>("pd_iota", ns);
+            d_ws = W.get<double>("pd_ws", 2 * (size_t)nd * ns);
+            HIPCHECK(hipMemcpyAsync(d_rn, rnormal + dim * (size_t)(istart - 1), dim * (size_t)ns * 8,
+                                    hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(d_ds, dipstr + (size_t)nd * (istart - 1), (size_t)nd * ns * 8,
+                                    hipMemcpyHostToDevice, s));
+            launch_iota(id, ns, s);
+            launch_gather_dip(d_rn, d_ds, nd, id, ns, d_ws, s);
+        }
         HIPCHECK(hipMemcpyAsync(d_tg, tg, dim * (size_t)nt * 8, hipMemcpyHostToDevice, s));
         HIPCHECK(hipMemcpyAsync(d_p, pot + (size_t)nd * (jstart - 1), (size_t)nd * nt * 8,
                                 hipMemcpyHostToDevice, s));
@@ -1955,8 +2098,9 @@ int run_part_direct(int dim, int nd, double delta, double dmax, int istart, int 
             HIPCHECK(hipMemcpyAsync(d_h, hess + nh * nd * (jstart - 1), nh * nd * nt * 8,
                                     hipMemcpyHostToDevice, s));
         if (dim == 2)
-            launch_direct_all(d_src, d_chg, nd, ns, d_tg, nt, delta, dmax, ifpgh, d_p, d_g, d_h, s);
-        else
+            launch_direct_all(d_src, d_chg, d_ws, nd, ns, d_tg, nt, delta, dmax, ifpgh, d_p, d_g,
+                              d_h, s);
+        else if (d_chg)
             launch_direct_all3(d_src, d_chg, nd, ns, d_tg, nt, delta, dmax, ifpgh, d_p, d_g, d_h, s);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(pot + (size_t)nd * (jstart - 1), d_p, (size_t)nd * nt * 8,
@@ -1981,9 +2125,9 @@ int fgtpw_fgt2d(int nd, double delta, double eps, int ns, const double* sources,
                 const double* targ, int ifpghtarg, double* pottarg, double* gradtarg,
                 double* hesstarg)
 {
-    (void)rnormal; (void)dipstr; (void)iper;
-    return run_host(2, nd, delta, eps, ns, sources, ifcharge, charge, ifdipole, ifpgh, pot, grad,
-                    hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg);
+    (void)iper;
+    return run_host(2, nd, delta, eps, ns, sources, ifcharge, charge, ifdipole, rnormal, dipstr,
+                    ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg);
 }
 
 int fgtpw_fgt3d(int nd, double delta, double eps, int ns, const double* sources, int ifcharge,
@@ -1992,9 +2136,9 @@ int fgtpw_fgt3d(int nd, double delta, double eps, int ns, const double* sources,
                 const double* targ, int ifpghtarg, double* pottarg, double* gradtarg,
                 double* hesstarg)
 {
-    (void)rnormal; (void)dipstr; (void)iper;
-    return run_host(3, nd, delta, eps, ns, sources, ifcharge, charge, ifdipole, ifpgh, pot, grad,
-                    hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg);
+    (void)iper;
+    return run_host(3, nd, delta, eps, ns, sources, ifcharge, charge, ifdipole, rnormal, dipstr,
+                    ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg);
 }
 
 int fgtpw_fgt3d_device(int nd, double delta, double eps, int ns, const double* sources,
@@ -2006,10 +2150,13 @@ int fgtpw_fgt3d_device(int nd, double delta, double eps, int ns, const double* s
     (void)rnormal; (void)dipstr; (void)iper;
     return run_guarded([&] {
         validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
+        check_strengths(ifcharge, charge, ifdipole, rnormal, dipstr);
         std::lock_guard<std::mutex> lk(g_mutex);
         select_device();
         Workspace& W = workspace();
-        Call q{nd, delta, eps, ns, sources, charge, ifpgh, pot, grad, hess,
+        if (ifdipole)
+            fail(FGTPW_EUNSUPPORTED, "dipole sources in 3D are not supported in this build");
+        Call q{nd, delta, eps, ns, sources, ifcharge ? charge : nullptr, ifpgh, pot, grad, hess,
                nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg};
         hipStream_t s = (hipStream_t)stream;
         run3d(W, q, s);
@@ -2022,9 +2169,8 @@ int fgtpw_fgt3dpart_direct(int nd, double delta, double dmax, int istart, int ie
                            int ifdipole, const double* rnormal, const double* dipstr,
                            const double* targ, int ifpgh, double* pot, double* grad, double* hess)
 {
-    (void)rnormal; (void)dipstr;
     return run_part_direct(3, nd, delta, dmax, istart, iend, jstart, jend, source, ifcharge,
-                           charge, ifdipole, targ, ifpgh, pot, grad, hess);
+                           charge, ifdipole, rnormal, dipstr, targ, ifpgh, pot, grad, hess);
 }
 
 int fgtpw_fgt2dpart_direct(int nd, double delta, double eps, int istart, int iend, int jstart,
@@ -2032,13 +2178,13 @@ int fgtpw_fgt2dpart_direct(int nd, double delta, double eps, int istart, int ien
                            int ifdipole, const double* rnormal, const double* dipstr,
                            const double* targ, int ifpgh, double* pot, double* grad, double* hess)
 {
-    (void)rnormal; (void)dipstr;
     if (!(eps > 0)) {
         t_err = "eps must be positive";
         return FGTPW_EINVAL;
     }
     return run_part_direct(2, nd, delta, std::log(1.0 / eps) * delta, istart, iend, jstart, jend,
-                           source, ifcharge, charge, ifdipole, targ, ifpgh, pot, grad, hess);
+                           source, ifcharge, charge, ifdipole, rnormal, dipstr, targ, ifpgh, pot,
+                           grad, hess);
 }
 
 int fgtpw_last_tree2d(int* info, int cap, int* laddr, int* lev, int* par, int* nch, int* ch,
@@ -2226,7 +2372,7 @@ struct ServeRequest {
     long long id = 0;
     int dim = 2, nd = 1, ns = 0, nt = 0, ifpgh = 0, ifpghtarg = 0;
     double delta = 0, eps = 0;
-    std::vector<double> src, chg, trg;  // copies made by submit
+    std::vector<double> src, chg, trg, rn, ds;  // copies made by submit
     double *pot = nullptr, *grad = nullptr, *hess = nullptr;
     double *pottarg = nullptr, *gradtarg = nullptr, *hesstarg = nullptr;
     size_t footprint = 0;
@@ -2312,9 +2458,10 @@ void serve_worker(Server* S, int slot)
         r->t_start = std::chrono::steady_clock::now();
         r->status = run_guarded([&] {
             host_transform(*W, r->dim, r->nd, r->delta, r->eps, r->ns, r->src.data(),
-                           r->chg.data(), r->ifpgh, r->pot, r->grad, r->hess, r->nt,
-                           r->nt > 0 ? r->trg.data() : nullptr, r->ifpghtarg, r->pottarg,
-                           r->gradtarg, r->hesstarg);
+                           r->chg.empty() ? nullptr : r->chg.data(),
+                           r->rn.empty() ? nullptr : r->rn.data(), r->ds.data(), r->ifpgh,
+                           r->pot, r->grad, r->hess, r->nt, r->nt > 0 ? r->trg.data() : nullptr,
+                           r->ifpghtarg, r->pottarg, r->gradtarg, r->hesstarg);
         });
         if (r->status != FGTPW_OK) r->err = t_err;
         r->t_end = std::chrono::steady_clock::now();
@@ -2322,6 +2469,8 @@ void serve_worker(Server* S, int slot)
         std::vector<double>().swap(r->src);
         std::vector<double>().swap(r->chg);
         std::vector<double>().swap(r->trg);
+        std::vector<double>().swap(r->rn);
+        std::vector<double>().swap(r->ds);
         {
             std::lock_guard<std::mutex> lk(S->m);
             S->in_flight -= r->footprint;
@@ -2371,8 +2520,10 @@ long long fgtpw_submit(const fgtpw_call* c)
         if (c->dim != 2 && c->dim != 3) fail(FGTPW_EINVAL, "dim must be 2 or 3");
         validate(c->nd, c->delta, c->eps, c->ns, c->ifcharge, c->ifdipole, c->ifpgh, c->nt,
                  c->ifpghtarg);
-        if (!c->sources || !c->charge || (c->nt > 0 && !c->targ))
-            fail(FGTPW_EINVAL, "missing input array");
+        if (!c->sources || (c->nt > 0 && !c->targ)) fail(FGTPW_EINVAL, "missing input array");
+        check_strengths(c->ifcharge, c->charge, c->ifdipole, c->rnormal, c->dipstr);
+        if (c->dim == 3 && c->ifdipole)
+            fail(FGTPW_EUNSUPPORTED, "dipole sources in 3D are not supported in this build");
         auto r = std::make_unique<ServeRequest>();
         r->dim = c->dim;
         r->nd = c->nd;
@@ -2383,7 +2534,11 @@ long long fgtpw_submit(const fgtpw_call* c)
         r->ifpgh = c->ifpgh;
         r->ifpghtarg = c->nt > 0 ? c->ifpghtarg : 0;
         r->src.assign(c->sources, c->sources + (size_t)c->dim * c->ns);
-        r->chg.assign(c->charge, c->charge + (size_t)c->nd * c->ns);
+        if (c->ifcharge) r->chg.assign(c->charge, c->charge + (size_t)c->nd * c->ns);
+        if (c->ifdipole) {
+            r->rn.assign(c->rnormal, c->rnormal + (size_t)c->dim * c->ns);
+            r->ds.assign(c->dipstr, c->dipstr + (size_t)c->nd * c->ns);
+        }
         if (c->nt > 0) r->trg.assign(c->targ, c->targ + (size_t)c->dim * c->nt);
         r->pot = c->pot; r->grad = c->grad; r->hess = c->hess;
         r->pottarg = c->pottarg; r->gradtarg = c->gradtarg; r->hesstarg = c->hesstarg;

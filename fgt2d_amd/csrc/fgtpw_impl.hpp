@@ -154,6 +154,7 @@ struct FormArgs {
     double* partial;
     int pstride;
     const int* segbox;      // list index of the box of every segment (or nullptr)
+    const double* ws;       // dipole weights d n, (n, nd, 2) sorted, or nullptr
 };
 void launch_form(const FormArgs& a, hipStream_t s);
 
@@ -225,17 +226,22 @@ struct DirectArgs {
     double delta, dmax;
     int ifpgh;
     double *pot, *grad, *hess;
+    const double* ws;       // dipole weights d n, (n, nd, 2) sorted, or nullptr
 };
 void launch_direct(const DirectArgs& a, hipStream_t s);
 
 // Taylor expansion of all sources about one centre (large-delta branch)
-void launch_formlc(const double* xy, const double* q, int nd, int ind, int n, double cx,
-                   double cy, double dsq, int nl, double* part, int nparts, double* out,
-                   hipStream_t s);
-// all-pairs direct sum with a cutoff (fgt2dpart_direct_vec), accumulating
-void launch_direct_all(const double* src, const double* q, int nd, int ns, const double* tg,
-                       int nt, double delta, double dmax, int ifpgh, double* pot, double* grad,
-                       double* hess, hipStream_t s);
+void launch_formlc(const double* xy, const double* q, const double* ws, int nd, int ind, int n,
+                   double cx, double cy, double dsq, int nl, double* part, int nparts,
+                   double* out, hipStream_t s);
+// dipole weights w(c, ind, i) = rnormal(c, perm i) dipstr(ind, perm i), sorted
+void launch_gather_dip(const double* rn, const double* ds, int nd, const unsigned* perm, int n,
+                       double* ws, hipStream_t s);
+// all-pairs direct sum with a cutoff (fgt2dpart_direct_vec), accumulating;
+// q == nullptr: no charges; ws: dipole weights (n, nd, 2) or nullptr
+void launch_direct_all(const double* src, const double* q, const double* ws, int nd, int ns,
+                       const double* tg, int nt, double delta, double dmax, int ifpgh,
+                       double* pot, double* grad, double* hess, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // 3D kernels (kernels3d.hip).  Points (3,n), expansions complex

@@ -362,6 +362,31 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ xy,
         for (int d = 0; d < nd; ++d) qs[(long)nd * i + d] = q[(long)nd * j + d];
 }
 
+// dipole weights in sorted order: w(c, ind, i) = rnormal(c, perm i) dipstr(ind, perm i)
+// at ws[2 (nd i + ind) + c], the products the reference forms (g2drouts.f:239-240)
+__global__ __launch_bounds__(256) void k_gather_dip(const double* __restrict__ rn,
+                                                    const double* __restrict__ ds, int nd,
+                                                    const unsigned* __restrict__ perm, int n,
+                                                    double* __restrict__ ws)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    long j = perm[i];
+    const double n1 = rn[2 * j], n2 = rn[2 * j + 1];
+    for (int d = 0; d < nd; ++d) {
+        const double dd = ds[(long)nd * j + d];
+        ws[2 * ((long)nd * i + d)] = n1 * dd;
+        ws[2 * ((long)nd * i + d) + 1] = n2 * dd;
+    }
+}
+
+void launch_gather_dip(const double* rn, const double* ds, int nd, const unsigned* perm, int n,
+                       double* ws, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_gather_dip<<<grid_for(n, 256), 256, 0, s>>>(rn, ds, nd, perm, n, ws);
+}
+
 void launch_gather(const double* xy, const double* q, int nd, const unsigned* perm, int n,
                    double* xy_s, double* q_s, hipStream_t s)
 {
@@ -413,7 +438,11 @@ void launch_scatter(const double* v_s, const unsigned* perm, int n, int w, doubl
 //   PW(a,b) = sum_k h2x(k,b) T(a,k)          b over all y nodes
 // with h2x(j,a) = ws_a (-i ts_a)^j / sqrt(j!).
 // ---------------------------------------------------------------------------
-template <int MAXE>
+// With dipoles (w = d n per source and density) two more tables give
+//   H(j,k) = sum_i X_j (q Y_k + w2 Y'_k) + X'_j w1 Y_k,
+//   X'_j = dsq j xh^(j-1)/sqrt(j!), the source derivative of X_j
+// (g2dformhd_vec / g2dformhcd_vec, g2drouts.f:173-330).
+template <int MAXE, bool DIP>
 __global__ __launch_bounds__(256) void k_form(FormArgs a, int chunk)
 {
     extern __shared__ double sm[];
@@ -425,7 +454,9 @@ __global__ __launch_bounds__(256) void k_form(FormArgs a, int chunk)
     const int m = a.p + 1, mm = m * m, t = threadIdx.x;
     const double cx = a.cen[2 * box], cy = a.cen[2 * box + 1];
     double* X = sm;              // chunk x m : xh^j / sqrt(j!)
-    double* Y = sm + chunk * m;  // chunk x m : q yh^k / sqrt(k!)
+    double* Y = sm + chunk * m;  // chunk x m : q yh^k / sqrt(k!) (+ w2 Y'_k)
+    double* Xd = Y + chunk * m;  // chunk x m : X'_j           (dipoles)
+    double* Yw = Xd + chunk * m; // chunk x m : w1 yh^k/sqrt(k!) (dipoles)
     double acc[MAXE];
 #pragma unroll
     for (int r = 0; r < MAXE; ++r) acc[r] = 0;
@@ -442,21 +473,45 @@ __global__ __launch_bounds__(256) void k_form(FormArgs a, int chunk)
             long i = se.x + s0 + u;
             double x = (a.xs[2 * i] - cx) * a.dsq;
             double y = (a.xs[2 * i + 1] - cy) * a.dsq;
-            double px = 1.0, py = a.qs[(long)a.nd * i + a.ind];
-            X[u * m] = px;
-            Y[u * m] = py;
-            for (int j = 1; j < m; ++j) {
-                px = px * x * a.rsq[j];
-                py = py * y * a.rsq[j];
-                X[u * m + j] = px;
-                Y[u * m + j] = py;
+            const double q = a.qs[(long)a.nd * i + a.ind];
+            if (!DIP) {
+                double px = 1.0, py = q;
+                X[u * m] = px;
+                Y[u * m] = py;
+                for (int j = 1; j < m; ++j) {
+                    px = px * x * a.rsq[j];
+                    py = py * y * a.rsq[j];
+                    X[u * m + j] = px;
+                    Y[u * m + j] = py;
+                }
+            } else {
+                const double w1 = a.ws[2 * ((long)a.nd * i + a.ind)];
+                const double w2 = a.ws[2 * ((long)a.nd * i + a.ind) + 1];
+                double px = 1.0, py = 1.0;  // xh^j/sqrt(j!), yh^j/sqrt(j!)
+                X[u * m] = 1.0;
+                Y[u * m] = q;
+                Xd[u * m] = 0.0;
+                Yw[u * m] = w1;
+                for (int j = 1; j < m; ++j) {
+                    const double f = a.dsq * j * a.rsq[j];
+                    Xd[u * m + j] = f * px;
+                    const double dy = f * py;
+                    px = px * x * a.rsq[j];
+                    py = py * y * a.rsq[j];
+                    X[u * m + j] = px;
+                    Y[u * m + j] = q * py + w2 * dy;
+                    Yw[u * m + j] = w1 * py;
+                }
             }
         }
         __syncthreads();
         for (int s = 0; s < cn; ++s) {
 #pragma unroll
             for (int r = 0; r < MAXE; ++r)
-                if (t + 256 * r < mm) acc[r] += Y[s * m + kk[r]] * X[s * m + jj[r]];
+                if (t + 256 * r < mm) {
+                    acc[r] += Y[s * m + kk[r]] * X[s * m + jj[r]];
+                    if (DIP) acc[r] += Yw[s * m + kk[r]] * Xd[s * m + jj[r]];
+                }
         }
         __syncthreads();
     }
@@ -710,18 +765,30 @@ __global__ __launch_bounds__(256) void k_form_moments(FormArgs a)
 // tile and writes the segment's partial moments, y order fastest.
 constexpr int WSEG = 256;
 
-template <int TM>
+//
+// With dipoles (w = d n per source) the raw moments are
+//   M(j,k) = sum_s xh^j (q yh^k + w2 dsq k yh^(k-1)) + (dsq j xh^(j-1)) w1 yh^k,
+// the source derivatives of g2dformhd_vec / g2dformhcd_vec
+// (g2drouts.f:173-330): two more tables and a second MFMA per step.  The
+// block then holds blockDim.x / 64 waves (128 threads for TM >= 3, to fit
+// the tables in LDS).
+template <int TM, bool DIP>
 __global__ __launch_bounds__(256) void k_form_wave(FormArgs a)
 {
     extern __shared__ double sm[];
     constexpr int MP = 16 * TM, RS = MP + 1, CH = 32;  // odd row stride: no bank conflicts
+    constexpr int NT = DIP ? 4 : 2;                     // tables per wave
     const int m = a.p + 1, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
     const int s_l = lane >> 4, e_l = lane & 15, u = lane & 31, side = lane >> 5;
-    double* Px = sm + (size_t)w * 2 * CH * RS;  // CH x RS: xh^e
-    double* Py = Px + CH * RS;                   // CH x RS: q yh^e
+    double* Px = sm + (size_t)w * NT * CH * RS;  // CH x RS: xh^e
+    double* Py = Px + CH * RS;                    // CH x RS: q yh^e (+ w2 dsq e yh^(e-1))
+    double* Pxd = Py + CH * RS;                   // CH x RS: dsq e xh^(e-1)   (dipoles)
+    double* Pyw = Pxd + CH * RS;                  // CH x RS: w1 yh^e          (dipoles)
     double* mine = (side ? Py : Px) + u * RS;
+    double* mine2 = (side ? Pyw : Pxd) + u * RS;
     const int total = seg_count(a.tstart, a.nbox);
-    for (int g = blockIdx.x * 4 + w; g < total; g += gridDim.x * 4) {
+    for (int g = blockIdx.x * nw + w; g < total; g += gridDim.x * nw) {
         const SegRange r = seg_at(a.tstart, a.nbox, a.boxes, a.src_se, a.seg, g, a.segbox);
         const int box = a.boxes[r.li], n = r.p1 - r.p0;
         const double c = a.cen[2 * box + side], dsq = a.dsq;
@@ -730,27 +797,45 @@ __global__ __launch_bounds__(256) void k_form_wave(FormArgs a)
         for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int j = 0; j < TM; ++j) acc[i][j] = (d4_t){0, 0, 0, 0};
-        // this lane's coordinate (x for side 0, y for side 1) and factor of
-        // the first chunk
-        double v = 0, f = 0;
-        if (u < n) {
-            const long i = r.p0 + u;
-            v = (a.xs[2 * i + side] - c) * dsq;
-            f = side ? a.qs[(long)a.nd * i + a.ind] : 1.0;
-        }
-        for (int c0 = 0; c0 < n; c0 += CH) {
-            double vn = 0, fn = 0;  // next chunk, loaded ahead
-            if (c0 + CH + u < n) {
-                const long i = r.p0 + c0 + CH + u;
-                vn = (a.xs[2 * i + side] - c) * dsq;
-                fn = side ? a.qs[(long)a.nd * i + a.ind] : 1.0;
+        // this lane's coordinate (x for side 0, y for side 1) and factors of
+        // the first chunk: table 1 gets f v^e + g e v^(e-1), table 2
+        // f2 v^e + g2 e v^(e-1)
+        double v = 0, f = 0, g1 = 0, f2 = 0, g2 = 0;
+        auto load = [&](int k, double& v_, double& f_, double& g_, double& f2_, double& g2_) {
+            v_ = f_ = g_ = f2_ = g2_ = 0;
+            if (k < n) {
+                const long i = r.p0 + k;
+                v_ = (a.xs[2 * i + side] - c) * dsq;
+                f_ = side ? a.qs[(long)a.nd * i + a.ind] : 1.0;
+                if (DIP) {
+                    const double* wv = a.ws + 2 * ((long)a.nd * i + a.ind);
+                    g_ = side ? dsq * wv[1] : 0.0;
+                    f2_ = side ? wv[0] : 0.0;
+                    g2_ = side ? 0.0 : dsq;
+                }
             }
+        };
+        load(u, v, f, g1, f2, g2);
+        for (int c0 = 0; c0 < n; c0 += CH) {
+            double vn, fn, gn, f2n, g2n;  // next chunk, loaded ahead
+            load(c0 + CH + u, vn, fn, gn, f2n, g2n);
             wave_lds_sync();  // the previous chunk's reads are done
-            double pw = f;
+            if (!DIP) {
+                double pw = f;
 #pragma unroll
-            for (int e = 0; e < MP; ++e) {
-                mine[e] = pw;
-                pw *= v;
+                for (int e = 0; e < MP; ++e) {
+                    mine[e] = pw;
+                    pw *= v;
+                }
+            } else {
+                double pw = 1.0, prev = 0.0;  // v^e, v^(e-1)
+#pragma unroll
+                for (int e = 0; e < MP; ++e) {
+                    mine[e] = f * pw + g1 * e * prev;
+                    mine2[e] = f2 * pw + g2 * e * prev;
+                    prev = pw;
+                    pw *= v;
+                }
             }
             wave_lds_sync();
             const int nc = min(CH, n - c0);
@@ -766,9 +851,24 @@ __global__ __launch_bounds__(256) void k_form_wave(FormArgs a)
                 for (int ii = 0; ii < TM; ++ii)
 #pragma unroll
                     for (int jj = 0; jj < TM; ++jj) acc[ii][jj] = mfma16x16x4(av[ii], bv[jj], acc[ii][jj]);
+                if (DIP) {
+#pragma unroll
+                    for (int t = 0; t < TM; ++t) {
+                        av[t] = Pxd[src * RS + e_l + 16 * t];
+                        bv[t] = Pyw[src * RS + e_l + 16 * t];
+                    }
+#pragma unroll
+                    for (int ii = 0; ii < TM; ++ii)
+#pragma unroll
+                        for (int jj = 0; jj < TM; ++jj)
+                            acc[ii][jj] = mfma16x16x4(av[ii], bv[jj], acc[ii][jj]);
+                }
             }
             v = vn;
             f = fn;
+            g1 = gn;
+            f2 = f2n;
+            g2 = g2n;
         }
         double* out = a.partial + (long)g * a.pstride;
 #pragma unroll
@@ -836,14 +936,30 @@ void launch_form(const FormArgs& a, hipStream_t s)
         const int TM = (m + 15) / 16, MP = 16 * TM;
         kmark(KID_FORM_MOMENTS, false, s);
         if (a.segbox && a.seg <= WSEG) {
-            // per wave: two 32 x (MP + 1) power tables
-            size_t lds = (size_t)4 * 2 * 32 * (MP + 1) * sizeof(double);
-            switch (TM) {
-            case 1: allow_lds(k_form_wave<1>, lds); k_form_wave<1><<<a.grid, 256, lds, s>>>(a); break;
-            case 2: allow_lds(k_form_wave<2>, lds); k_form_wave<2><<<a.grid, 256, lds, s>>>(a); break;
-            case 3: allow_lds(k_form_wave<3>, lds); k_form_wave<3><<<a.grid, 256, lds, s>>>(a); break;
-            default: allow_lds(k_form_wave<4>, lds); k_form_wave<4><<<a.grid, 256, lds, s>>>(a); break;
+            // per wave: two (four with dipoles) 32 x (MP + 1) power tables
+            const bool dip = a.ws != nullptr;
+            const int nwave = dip && TM >= 3 ? 2 : 4;
+            size_t lds = (size_t)nwave * (dip ? 4 : 2) * 32 * (MP + 1) * sizeof(double);
+            const int grid = (int)std::min<long>(2L * a.grid, (long)a.grid * 4 / nwave);
+#define FGT_FORM_WAVE(T_, D_)                                                                  \
+    allow_lds(k_form_wave<T_, D_>, lds);                                                       \
+    k_form_wave<T_, D_><<<grid, 64 * nwave, lds, s>>>(a)
+            if (!dip) {
+                switch (TM) {
+                case 1: FGT_FORM_WAVE(1, false); break;
+                case 2: FGT_FORM_WAVE(2, false); break;
+                case 3: FGT_FORM_WAVE(3, false); break;
+                default: FGT_FORM_WAVE(4, false); break;
+                }
+            } else {
+                switch (TM) {
+                case 1: FGT_FORM_WAVE(1, true); break;
+                case 2: FGT_FORM_WAVE(2, true); break;
+                case 3: FGT_FORM_WAVE(3, true); break;
+                default: FGT_FORM_WAVE(4, true); break;
+                }
             }
+#undef FGT_FORM_WAVE
         } else {
             size_t lds = std::max((size_t)2 * a.seg * (MP + 1), (size_t)MP * MP) * sizeof(double);
             switch (TM) {
@@ -861,26 +977,37 @@ void launch_form(const FormArgs& a, hipStream_t s)
     kmark(KID_FORM_H2PW, true, s);
         return;
     }
-    int chunk = 4096 / m;
+    const bool dip = a.ws != nullptr;
+    int chunk = (dip ? 2048 : 4096) / m;
     if (chunk > 256) chunk = 256;
     if (chunk < 16) chunk = 16;
-    size_t lds = (size_t)2 * chunk * m * sizeof(double);
+    size_t lds = (size_t)(dip ? 4 : 2) * chunk * m * sizeof(double);
     size_t lds2 = ((size_t)m * m + (size_t)a.npw * m) * sizeof(double);
     if (lds2 > lds) lds = lds2;
     int ne = (m * m + 255) / 256;
     kmark(KID_FORM_VALU, false, s);
-    if (ne <= 1)
-        k_form<1><<<a.nbox, 256, lds, s>>>(a, chunk);
-    else if (ne <= 2)
-        k_form<2><<<a.nbox, 256, lds, s>>>(a, chunk);
-    else if (ne <= 4)
-        k_form<4><<<a.nbox, 256, lds, s>>>(a, chunk);
-    else if (ne <= 8)
-        k_form<8><<<a.nbox, 256, lds, s>>>(a, chunk);
-    else if (ne <= 16)
-        k_form<16><<<a.nbox, 256, lds, s>>>(a, chunk);
-    else
+#define FGT_FORM_VALU(E_)                                                                      \
+    if (dip) {                                                                                 \
+        allow_lds(k_form<E_, true>, lds);                                                      \
+        k_form<E_, true><<<a.nbox, 256, lds, s>>>(a, chunk);                                   \
+    } else {                                                                                   \
+        allow_lds(k_form<E_, false>, lds);                                                     \
+        k_form<E_, false><<<a.nbox, 256, lds, s>>>(a, chunk);                                  \
+    }
+    if (ne <= 1) {
+        FGT_FORM_VALU(1)
+    } else if (ne <= 2) {
+        FGT_FORM_VALU(2)
+    } else if (ne <= 4) {
+        FGT_FORM_VALU(4)
+    } else if (ne <= 8) {
+        FGT_FORM_VALU(8)
+    } else if (ne <= 16) {
+        FGT_FORM_VALU(16)
+    } else {
         fail(-3, "Hermite order too large for the formation kernel");
+    }
+#undef FGT_FORM_VALU
     kmark(KID_FORM_VALU, true, s);
 }
 
@@ -1160,8 +1287,8 @@ __global__ __launch_bounds__(256) void k_taylor_eval(EvalArgs a)
 // products in registers).  Lane l then holds y orders k = l/16 + 4r (+16 per
 // column tile) of point t = l%16, so sum_k D(k,t) yh_t^k is a running
 // product in registers plus a reduction over the 4 lanes of the point.
-// Outputs are stored, not accumulated: each point is evaluated by exactly
-// one leaf and the buffers start at zero.
+// Outputs are accumulated: the near field of a point may already be in
+// them (each point is evaluated by exactly one leaf).
 template <int TM, int PGH>
 __global__ __launch_bounds__(256) void k_taylor_mfma(EvalArgs a)
 {
@@ -1270,17 +1397,17 @@ __global__ __launch_bounds__(256) void k_taylor_mfma(EvalArgs a)
             Vyy += __shfl_xor(Vyy, 32);
         }
         if (kq == 0 && act) {
-            a.pot[i * a.nd + a.ind] = V;
+            a.pot[i * a.nd + a.ind] += V;
             if (PGH >= 2) {
                 const long g = i * 2 * a.nd + a.ind;
-                a.grad[g] = U * dsq;
-                a.grad[g + a.nd] = Vy * dsq;
+                a.grad[g] += U * dsq;
+                a.grad[g + a.nd] += Vy * dsq;
             }
             if (PGH >= 3) {
                 const long h = i * 3 * a.nd + a.ind;
-                a.hess[h] = Z * dsq * dsq;
-                a.hess[h + a.nd] = Uy * dsq * dsq;
-                a.hess[h + 2 * a.nd] = Vyy * dsq * dsq;
+                a.hess[h] += Z * dsq * dsq;
+                a.hess[h + a.nd] += Uy * dsq * dsq;
+                a.hess[h + 2 * a.nd] += Vyy * dsq * dsq;
             }
         }
     }
@@ -1409,17 +1536,17 @@ __global__ __launch_bounds__(256) void k_taylor_wave(EvalArgs a)
                 Vyy += __shfl_xor(Vyy, 32);
             }
             if (kq == 0 && act) {
-                a.pot[i * a.nd + a.ind] = V;
+                a.pot[i * a.nd + a.ind] += V;
                 if (PGH >= 2) {
                     const long gi = i * 2 * a.nd + a.ind;
-                    a.grad[gi] = U * dsq;
-                    a.grad[gi + a.nd] = Vy * dsq;
+                    a.grad[gi] += U * dsq;
+                    a.grad[gi + a.nd] += Vy * dsq;
                 }
                 if (PGH >= 3) {
                     const long h = i * 3 * a.nd + a.ind;
-                    a.hess[h] = Z * dsq * dsq;
-                    a.hess[h + a.nd] = Uy * dsq * dsq;
-                    a.hess[h + 2 * a.nd] = Vyy * dsq * dsq;
+                    a.hess[h] += Z * dsq * dsq;
+                    a.hess[h + a.nd] += Uy * dsq * dsq;
+                    a.hess[h + 2 * a.nd] += Vyy * dsq * dsq;
                 }
             }
         }
@@ -1607,14 +1734,45 @@ void launch_pw_eval(const EvalArgs& a, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// Step 6, near field (fgt2dpart_direct_vec fgt2d.f:1354 -> g2d_directc*_vec,
-// g2ddirect.f:32-210), target-owned: one block per target unit, looping over
+// One source's contribution at a target, r = t - s, e = exp(-|r|^2/delta):
+// a charge q gives q e; a dipole w = d n gives d (n . grad_s) e =
+// (2/delta) (w . r) e (g2d_directd*_vec, g2ddirect.f:217-420).  With
+// u = -2 r / delta and A = q - w . u the sum is e A, its gradient
+// e (u A - rfac w) and its Hessian e ((u_a u_b + rfac d_ab) A
+// - rfac (u_a w_b + u_b w_a)), rfac = -2/delta.
+// ---------------------------------------------------------------------------
+template <int PGH, bool DIP>
+__device__ __forceinline__ void pair_add(double dx, double dy, double q, double w1, double w2,
+                                         double inv, double rfac, double dmax, double& p,
+                                         double& gx, double& gy, double& hxx, double& hxy,
+                                         double& hyy)
+{
+    const double rr = dx * dx + dy * dy;
+    if (!(rr < dmax)) return;
+    const double e = exp(-rr * inv);
+    const double ux = rfac * dx, uy = rfac * dy;
+    const double A = DIP ? q - (w1 * ux + w2 * uy) : q;
+    p += e * A;
+    if (PGH >= 2) {
+        gx += e * (ux * A - (DIP ? rfac * w1 : 0.0));
+        gy += e * (uy * A - (DIP ? rfac * w2 : 0.0));
+        if (PGH >= 3) {
+            hxx += e * ((rfac + ux * ux) * A - (DIP ? 2.0 * rfac * ux * w1 : 0.0));
+            hxy += e * (ux * uy * A - (DIP ? rfac * (ux * w2 + uy * w1) : 0.0));
+            hyy += e * ((rfac + uy * uy) * A - (DIP ? 2.0 * rfac * uy * w2 : 0.0));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Step 6, near field (fgt2dpart_direct_vec fgt2d.f:1354 -> g2d_direct*_vec,
+// g2ddirect.f:32-655), target-owned: one block per target unit, looping over
 // its source boxes in LDS tiles; pairs with r^2 >= dmax are skipped.
 // ---------------------------------------------------------------------------
-template <int PGH>
+template <int PGH, bool DIP>
 __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
 {
-    __shared__ double sx[256], sy[256], sq[256];
+    __shared__ double sx[256], sy[256], sq[256], sw1[DIP ? 256 : 1], sw2[DIP ? 256 : 1];
     const int u = a.units[blockIdx.x];
     const int2 tr = a.tgt_se[u];
     const int l0 = a.list_ptr[blockIdx.x], l1 = a.list_ptr[blockIdx.x + 1];
@@ -1636,26 +1794,16 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
                         sx[t] = a.xs[2 * j];
                         sy[t] = a.xs[2 * j + 1];
                         sq[t] = a.qs[j * a.nd + ind];
-                    }
-                    __syncthreads();
-                    for (int s = 0; s < cn; ++s) {
-                        double dx = x - sx[s], dy = y - sy[s];
-                        double rr = dx * dx + dy * dy;
-                        if (rr < a.dmax) {
-                            double e = exp(-rr * inv) * sq[s];
-                            p += e;
-                            if (PGH >= 2) {
-                                double ux = rfac * dx, uy = rfac * dy;
-                                gx += ux * e;
-                                gy += uy * e;
-                                if (PGH >= 3) {
-                                    hxx += (rfac + ux * ux) * e;
-                                    hxy += ux * uy * e;
-                                    hyy += (rfac + uy * uy) * e;
-                                }
-                            }
+                        if (DIP) {
+                            sw1[t] = a.ws[2 * (j * a.nd + ind)];
+                            sw2[t] = a.ws[2 * (j * a.nd + ind) + 1];
                         }
                     }
+                    __syncthreads();
+                    for (int s = 0; s < cn; ++s)
+                        pair_add<PGH, DIP>(x - sx[s], y - sy[s], sq[s], DIP ? sw1[s] : 0.0,
+                                           DIP ? sw2[s] : 0.0, inv, rfac, a.dmax, p, gx, gy, hxx,
+                                           hxy, hyy);
                 }
             }
             if (act) {
@@ -1679,12 +1827,13 @@ void launch_direct(const DirectArgs& a, hipStream_t s)
 {
     if (a.nunits <= 0) return;
     kmark(KID_DIRECT, false, s);
+    const bool dip = a.ws != nullptr;
     if (a.ifpgh >= 3)
-        k_direct<3><<<a.nunits, 256, 0, s>>>(a);
+        dip ? k_direct<3, true><<<a.nunits, 256, 0, s>>>(a) : k_direct<3, false><<<a.nunits, 256, 0, s>>>(a);
     else if (a.ifpgh == 2)
-        k_direct<2><<<a.nunits, 256, 0, s>>>(a);
+        dip ? k_direct<2, true><<<a.nunits, 256, 0, s>>>(a) : k_direct<2, false><<<a.nunits, 256, 0, s>>>(a);
     else
-        k_direct<1><<<a.nunits, 256, 0, s>>>(a);
+        dip ? k_direct<1, true><<<a.nunits, 256, 0, s>>>(a) : k_direct<1, false><<<a.nunits, 256, 0, s>>>(a);
     kmark(KID_DIRECT, true, s);
 }
 
@@ -1695,8 +1844,13 @@ void launch_direct(const DirectArgs& a, hipStream_t s)
 //   h_0 = e^{-u^2}, h_1 = 2u e^{-u^2}, h_{j+1} = 2 (u h_j - h_{j-1})/(j+1).
 // Each block sums a slice of the sources; a second kernel adds the slices.
 // ---------------------------------------------------------------------------
+//
+// Dipoles (g2dformld_vec / g2dformlcd_vec, g2drouts.f:685-845) add
+// dh_j = -dsq (j+1) h_{j+1}: L(j,k) += dh_j(xh) w1 h_k(yh) + h_j(xh) w2 dh_k(yh).
+template <bool DIP>
 __global__ __launch_bounds__(256) void k_formlc_part(const double* __restrict__ xy,
-                                                     const double* __restrict__ q, int nd,
+                                                     const double* __restrict__ q,
+                                                     const double* __restrict__ ws, int nd,
                                                      int ind, int n, double cx, double cy,
                                                      double dsq, int nl, int nparts,
                                                      double* __restrict__ part)
@@ -1706,6 +1860,8 @@ __global__ __launch_bounds__(256) void k_formlc_part(const double* __restrict__ 
     const int chunk = 64;
     double* X = sm;
     double* Y = sm + chunk * m;
+    double* Xd = Y + chunk * m;   // dipoles: dh_j(xh)
+    double* Yw = Xd + chunk * m;  // dipoles: w1 h_k(yh)
     long per = (n + nparts - 1) / nparts;
     long i0 = blockIdx.x * per, i1 = min((long)n, i0 + per);
     double* out = part + (long)blockIdx.x * mm;
@@ -1716,15 +1872,23 @@ __global__ __launch_bounds__(256) void k_formlc_part(const double* __restrict__ 
         if (t < cn) {
             long i = s0 + t;
             double x = (xy[2 * i] - cx) * dsq, y = (xy[2 * i + 1] - cy) * dsq;
-            double fx = exp(-x * x), fy = exp(-y * y), qq = q[(long)nd * i + ind];
+            double fx = exp(-x * x), fy = exp(-y * y), qq = q ? q[(long)nd * i + ind] : 0.0;
+            double w1 = DIP ? ws[2 * ((long)nd * i + ind)] : 0.0;
+            double w2 = DIP ? ws[2 * ((long)nd * i + ind) + 1] : 0.0;
+            // h_j for j = 0..m (one past the table for the derivatives)
             double a0 = fx, a1 = 2.0 * x * fx, b0 = fy, b1 = 2.0 * y * fy;
-            X[t * m] = a0;
-            Y[t * m] = qq * b0;
-            if (m > 1) { X[t * m + 1] = a1; Y[t * m + 1] = qq * b1; }
-            for (int j = 1; j + 1 < m; ++j) {
-                double a2 = 2.0 * (x * a1 - a0) / (j + 1), b2 = 2.0 * (y * b1 - b0) / (j + 1);
-                X[t * m + j + 1] = a2;
-                Y[t * m + j + 1] = qq * b2;
+            for (int j = 0; j < m; ++j) {
+                // here a0 = h_j(x), a1 = h_{j+1}(x) (same for b, y)
+                X[t * m + j] = a0;
+                if (DIP) {
+                    const double dha = -(a1 * dsq * (j + 1.0)), dhb = -(b1 * dsq * (j + 1.0));
+                    Xd[t * m + j] = dha;
+                    Y[t * m + j] = qq * b0 + w2 * dhb;
+                    Yw[t * m + j] = w1 * b0;
+                } else {
+                    Y[t * m + j] = qq * b0;
+                }
+                const double a2 = 2.0 * (x * a1 - a0) / (j + 2), b2 = 2.0 * (y * b1 - b0) / (j + 2);
                 a0 = a1; a1 = a2; b0 = b1; b1 = b2;
             }
         }
@@ -1732,7 +1896,10 @@ __global__ __launch_bounds__(256) void k_formlc_part(const double* __restrict__ 
         for (int e = t; e < mm; e += 256) {
             int j = e % m, k = e / m;
             double acc = 0;
-            for (int s = 0; s < cn; ++s) acc += Y[s * m + k] * X[s * m + j];
+            for (int s = 0; s < cn; ++s) {
+                acc += Y[s * m + k] * X[s * m + j];
+                if (DIP) acc += Yw[s * m + k] * Xd[s * m + j];
+            }
             out[e] += acc;
         }
     }
@@ -1748,28 +1915,37 @@ __global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ pa
     out[e] = acc;
 }
 
-void launch_formlc(const double* xy, const double* q, int nd, int ind, int n, double cx,
-                   double cy, double dsq, int nl, double* part, int nparts, double* out,
-                   hipStream_t s)
+void launch_formlc(const double* xy, const double* q, const double* ws, int nd, int ind, int n,
+                   double cx, double cy, double dsq, int nl, double* part, int nparts,
+                   double* out, hipStream_t s)
 {
     int m = nl + 1;
-    size_t lds = (size_t)2 * 64 * m * sizeof(double);
-    k_formlc_part<<<nparts, 256, lds, s>>>(xy, q, nd, ind, n, cx, cy, dsq, nl, nparts, part);
+    size_t lds = (size_t)(ws ? 4 : 2) * 64 * m * sizeof(double);
+    if (ws) {
+        allow_lds(k_formlc_part<true>, lds);
+        k_formlc_part<true><<<nparts, 256, lds, s>>>(xy, q, ws, nd, ind, n, cx, cy, dsq, nl, nparts,
+                                                     part);
+    } else {
+        allow_lds(k_formlc_part<false>, lds);
+        k_formlc_part<false><<<nparts, 256, lds, s>>>(xy, q, ws, nd, ind, n, cx, cy, dsq, nl,
+                                                      nparts, part);
+    }
     k_sum_parts<<<grid_for(m * m, 256), 256, 0, s>>>(part, nparts, m * m, out);
 }
 
 // fgt2dpart_direct_vec (fgt2d.f:1354): every target against every source
 // with the r^2 < dmax cutoff, accumulating.  Source tiles in LDS.
-template <int PGH>
+template <int PGH, bool DIP>
 __global__ __launch_bounds__(256) void k_direct_all(const double* __restrict__ src,
-                                                    const double* __restrict__ q, int nd, int ns,
+                                                    const double* __restrict__ q,
+                                                    const double* __restrict__ ws, int nd, int ns,
                                                     const double* __restrict__ tg, int nt,
                                                     double delta, double dmax,
                                                     double* __restrict__ pot,
                                                     double* __restrict__ grad,
                                                     double* __restrict__ hess)
 {
-    __shared__ double sx[256], sy[256], sq[256];
+    __shared__ double sx[256], sy[256], sq[256], sw1[DIP ? 256 : 1], sw2[DIP ? 256 : 1];
     const int t = threadIdx.x;
     const int i = blockIdx.x * 256 + t;
     const bool act = i < nt;
@@ -1784,27 +1960,16 @@ __global__ __launch_bounds__(256) void k_direct_all(const double* __restrict__ s
                 long j = s0 + t;
                 sx[t] = src[2 * j];
                 sy[t] = src[2 * j + 1];
-                sq[t] = q[j * nd + ind];
-            }
-            __syncthreads();
-            for (int s = 0; s < cn; ++s) {
-                double dx = x - sx[s], dy = y - sy[s];
-                double rr = dx * dx + dy * dy;
-                if (rr < dmax) {
-                    double e = exp(-rr * inv) * sq[s];
-                    p += e;
-                    if (PGH >= 2) {
-                        double ux = rfac * dx, uy = rfac * dy;
-                        gx += ux * e;
-                        gy += uy * e;
-                        if (PGH >= 3) {
-                            hxx += (rfac + ux * ux) * e;
-                            hxy += ux * uy * e;
-                            hyy += (rfac + uy * uy) * e;
-                        }
-                    }
+                sq[t] = q ? q[j * nd + ind] : 0.0;
+                if (DIP) {
+                    sw1[t] = ws[2 * (j * nd + ind)];
+                    sw2[t] = ws[2 * (j * nd + ind) + 1];
                 }
             }
+            __syncthreads();
+            for (int s = 0; s < cn; ++s)
+                pair_add<PGH, DIP>(x - sx[s], y - sy[s], sq[s], DIP ? sw1[s] : 0.0,
+                                   DIP ? sw2[s] : 0.0, inv, rfac, dmax, p, gx, gy, hxx, hxy, hyy);
         }
         if (act) {
             pot[(long)i * nd + ind] += p;
@@ -1821,18 +1986,22 @@ __global__ __launch_bounds__(256) void k_direct_all(const double* __restrict__ s
     }
 }
 
-void launch_direct_all(const double* src, const double* q, int nd, int ns, const double* tg,
-                       int nt, double delta, double dmax, int ifpgh, double* pot, double* grad,
-                       double* hess, hipStream_t s)
+void launch_direct_all(const double* src, const double* q, const double* ws, int nd, int ns,
+                       const double* tg, int nt, double delta, double dmax, int ifpgh,
+                       double* pot, double* grad, double* hess, hipStream_t s)
 {
     if (nt <= 0 || ns <= 0) return;
     unsigned g = grid_for(nt, 256);
-    if (ifpgh >= 3)
-        k_direct_all<3><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
-    else if (ifpgh == 2)
-        k_direct_all<2><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
-    else
-        k_direct_all<1><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
+#define FGT_DALL(P_, D_)                                                                       \
+    k_direct_all<P_, D_><<<g, 256, 0, s>>>(src, q, ws, nd, ns, tg, nt, delta, dmax, pot, grad, hess)
+    if (ifpgh >= 3) {
+        if (ws) FGT_DALL(3, true); else FGT_DALL(3, false);
+    } else if (ifpgh == 2) {
+        if (ws) FGT_DALL(2, true); else FGT_DALL(2, false);
+    } else {
+        if (ws) FGT_DALL(1, true); else FGT_DALL(1, false);
+    }
+#undef FGT_DALL
 }
 
 }  // namespace fgtpw

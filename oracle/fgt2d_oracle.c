@@ -545,10 +545,14 @@ void orc_tree2d_free(void *h)
 /* complex numbers are stored as interleaved (re, im) doubles          */
 /* ------------------------------------------------------------------ */
 
-/* g2dformhc_vec, g2drouts.f:100-168: H(j,k) += q x^j/sqrt(j!) y^k/sqrt(k!),
- * x = (s - c)/sqrt(delta). h is (nt+1)x(nt+1), j fastest. */
-static void orc_formhc(int nd, double delta, const double *src, int ns, const double *chg,
-                       const double *c, int nt, double *h, double *xp, double *yp)
+/* g2dformhc_vec / g2dformhd_vec / g2dformhcd_vec, g2drouts.f:100-330:
+ * H(j,k) += q hx_j hy_k + d (n1 dhx_j hy_k + n2 hx_j dhy_k), with
+ * hx_j = x^j/sqrt(j!), dhx_j = j dsq hx_{j-1}/sqrt(j), x = (s - c)/sqrt(delta),
+ * in the reference's order of operations.  chg == NULL: no charges;
+ * rn == NULL: no dipoles.  h is (nt+1)x(nt+1), j fastest. */
+static void orc_formh(int nd, double delta, const double *src, int ns, const double *chg,
+                      const double *rn, const double *ds, const double *c, int nt, double *h,
+                      double *xp, double *yp, double *dxp, double *dyp)
 {
     int m = nt + 1;
     memset(h, 0, sizeof(double) * m * m * nd);
@@ -556,17 +560,35 @@ static void orc_formhc(int nd, double delta, const double *src, int ns, const do
     for (int i = 0; i < ns; i++) {
         double x = (src[2 * i] - c[0]) * dsq, y = (src[2 * i + 1] - c[1]) * dsq;
         xp[0] = 1.0; yp[0] = 1.0;
+        dxp[0] = 0.0; dyp[0] = 0.0;
         for (int k = 1; k <= nt; k++) {
             double t = sqrt((double)k);
-            xp[k] = xp[k - 1] * x / t;
             yp[k] = yp[k - 1] * y / t;
+            xp[k] = xp[k - 1] * x / t;
+            dxp[k] = k * dsq * xp[k - 1] / t;
+            dyp[k] = k * dsq * yp[k - 1] / t;
         }
         for (int ind = 0; ind < nd; ind++) {
-            double q = chg[(size_t)nd * i + ind];
             double *hh = h + (size_t)ind * m * m;
-            for (int k = 0; k <= nt; k++) {
-                double yt = q * yp[k];
-                for (int j = 0; j <= nt; j++) hh[j + m * k] += yt * xp[j];
+            if (!rn) {
+                double q = chg[(size_t)nd * i + ind];
+                for (int k = 0; k <= nt; k++) {
+                    double yt = q * yp[k];
+                    for (int j = 0; j <= nt; j++) hh[j + m * k] += yt * xp[j];
+                }
+            } else {
+                double dd = ds[(size_t)nd * i + ind];
+                for (int k = 0; k <= nt; k++) {
+                    double d1 = yp[k] * rn[2 * i] * dd, d2 = dyp[k] * rn[2 * i + 1] * dd;
+                    if (chg) {
+                        double yt = chg[(size_t)nd * i + ind] * yp[k];
+                        for (int j = 0; j <= nt; j++)
+                            hh[j + m * k] = hh[j + m * k] + dxp[j] * d1 + xp[j] * d2 + xp[j] * yt;
+                    } else {
+                        for (int j = 0; j <= nt; j++)
+                            hh[j + m * k] = hh[j + m * k] + dxp[j] * d1 + xp[j] * d2;
+                    }
+                }
             }
         }
     }
@@ -820,67 +842,195 @@ static void orc_pweval(int nd, double delta, const double *c, int npw, const dou
 
 /* g2d_directcp_vec / cg / ch, g2ddirect.f:32-210: for each target,
  * sum over sources with r^2 < ln(1/eps) delta of q e^{-r^2/delta}
- * (and derivatives: grad = -2/delta (t-s) e, hess as in g2ddirect.f:146). */
-void orc_direct2d(int nd, double delta, double eps, const double *src, int ns,
-                  const double *chg, const double *tg, int ntg, int ifpgh,
-                  double *pot, double *grad, double *hess)
+ * (and derivatives: grad = -2/delta (t-s) e, hess as in g2ddirect.f:146).
+ * With dipoles (rn != NULL) the six variants g2d_directdp/dg/dh and
+ * g2d_directcdp/cdg/cdh, g2ddirect.f:217-655, in their order of
+ * operations: the dipole kernel is d (n . grad_s) e^{-r^2/delta}. */
+void orc_direct2d_cd(int nd, double delta, double eps, const double *src, int ns,
+                     const double *chg, const double *rn, const double *ds, const double *tg,
+                     int ntg, int ifpgh, double *pot, double *grad, double *hess)
 {
     double dmax = log(1.0 / eps) * delta;
-    double rfac = -2.0 / delta;
     for (int it = 0; it < ntg; it++) {
+        size_t t = (size_t)it;
+        double *P = pot + nd * t, *G = grad + nd * 2 * t, *H = hess + nd * 3 * t;
         for (int i = 0; i < ns; i++) {
             double xd = tg[2 * it] - src[2 * i], yd = tg[2 * it + 1] - src[2 * i + 1];
             double rr = xd * xd + yd * yd;
             if (!(rr < dmax)) continue;
-            double e = exp(-rr / delta);
-            size_t t = (size_t)it;
-            for (int ind = 0; ind < nd; ind++) {
-                double q = chg[(size_t)nd * i + ind];
-                pot[nd * t + ind] += e * q;
-                if (ifpgh >= 2) {
-                    double dx = rfac * xd, dy = rfac * yd;
-                    grad[nd * 2 * t + ind] += e * q * dx;
-                    grad[nd * 2 * t + nd + ind] += e * q * dy;
-                    if (ifpgh >= 3) {
-                        hess[nd * 3 * t + ind] += e * q * (dx * dx + rfac);
-                        hess[nd * 3 * t + nd + ind] += e * q * dx * dy;
-                        hess[nd * 3 * t + 2 * nd + ind] += e * q * (dy * dy + rfac);
+            const double *q = chg ? chg + (size_t)nd * i : NULL;
+            const double *d = rn ? ds + (size_t)nd * i : NULL;
+            double n1 = rn ? rn[2 * i] : 0, n2 = rn ? rn[2 * i + 1] : 0;
+            if (!rn) {
+                double rfac = -2.0 / delta;
+                double e = exp(-rr / delta);
+                for (int ind = 0; ind < nd; ind++) {
+                    P[ind] += e * q[ind];
+                    if (ifpgh >= 2) {
+                        double dx = rfac * xd, dy = rfac * yd;
+                        G[ind] += e * q[ind] * dx;
+                        G[nd + ind] += e * q[ind] * dy;
+                        if (ifpgh >= 3) {
+                            H[ind] += e * q[ind] * (dx * dx + rfac);
+                            H[nd + ind] += e * q[ind] * dx * dy;
+                            H[2 * nd + ind] += e * q[ind] * (dy * dy + rfac);
+                        }
                     }
+                }
+            } else if (ifpgh == 1 && !q) { /* g2d_directdp_vec */
+                double rfac = 2.0 / delta;
+                double rtmp = exp(-rr / delta);
+                double dx = rfac * xd, dy = rfac * yd;
+                double rinc = (dx * n1 + dy * n2) * rtmp;
+                for (int ind = 0; ind < nd; ind++) P[ind] = P[ind] + rinc * d[ind];
+            } else if (ifpgh == 2 && !q) { /* g2d_directdg_vec */
+                double rfac = 2.0 / delta;
+                double rtmp = exp(-rr / delta);
+                double dx = rfac * xd, dy = rfac * yd;
+                double rinc = (dx * n1 + dy * n2);
+                double rincx = n1 * rfac - rinc * dx, rincy = n2 * rfac - rinc * dy;
+                for (int ind = 0; ind < nd; ind++) {
+                    double dtmp = rtmp * d[ind];
+                    P[ind] = P[ind] + rinc * dtmp;
+                    G[ind] = G[ind] + rincx * dtmp;
+                    G[nd + ind] = G[nd + ind] + rincy * dtmp;
+                }
+            } else if (!q) { /* g2d_directdh_vec */
+                double rfac = 2.0 / delta;
+                double dtmp = exp(-rr / delta);
+                double dx = rfac * xd, dy = rfac * yd;
+                double rinc = (dx * n1 + dy * n2);
+                double rincx = n1 * rfac - rinc * dx, rincy = n2 * rfac - rinc * dy;
+                double rincxx = -(rfac * (dx * n1 + rinc)) - rincx * dx;
+                double rincyy = -(rfac * (dy * n2 + rinc)) - rincy * dy;
+                double rincxy = -(rfac * dx * n2) - rincx * dy;
+                for (int ind = 0; ind < nd; ind++) {
+                    double rt = dtmp * d[ind];
+                    P[ind] = P[ind] + rinc * rt;
+                    G[ind] = G[ind] + rincx * rt;
+                    G[nd + ind] = G[nd + ind] + rincy * rt;
+                    H[ind] = H[ind] + rincxx * rt;
+                    H[nd + ind] = H[nd + ind] + rincxy * rt;
+                    H[2 * nd + ind] = H[2 * nd + ind] + rincyy * rt;
+                }
+            } else if (ifpgh == 1) { /* g2d_directcdp_vec */
+                double rfac = 2.0 / delta;
+                double dx = rfac * xd, dy = rfac * yd;
+                double rtmp = exp(-rr / delta);
+                for (int ind = 0; ind < nd; ind++) P[ind] = P[ind] + rtmp * q[ind];
+                double rinc = (dx * n1 + dy * n2) * rtmp;
+                for (int ind = 0; ind < nd; ind++) P[ind] = P[ind] + rinc * d[ind];
+            } else if (ifpgh == 2) { /* g2d_directcdg_vec */
+                double rfac = -2.0 / delta;
+                double dtmp = exp(-rr / delta);
+                double dx = rfac * xd, dy = rfac * yd;
+                for (int ind = 0; ind < nd; ind++) {
+                    double rinc = dtmp * q[ind];
+                    P[ind] = P[ind] + rinc;
+                    G[ind] = G[ind] + dx * rinc;
+                    G[nd + ind] = G[nd + ind] + dy * rinc;
+                }
+                double rinc = -(dx * n1) - dy * n2;
+                double rincx = rinc * dx - rfac * n1, rincy = rinc * dy - rfac * n2;
+                for (int ind = 0; ind < nd; ind++) {
+                    double rt = dtmp * d[ind];
+                    P[ind] = P[ind] + rinc * rt;
+                    G[ind] = G[ind] + rincx * rt;
+                    G[nd + ind] = G[nd + ind] + rincy * rt;
+                }
+            } else { /* g2d_directcdh_vec */
+                double rfac = -2.0 / delta;
+                double dtmp = exp(-rr / delta);
+                double dx = rfac * xd, dy = rfac * yd;
+                double dxx = rfac + dx * dx, dyy = rfac + dy * dy, dxy = dx * dy;
+                for (int ind = 0; ind < nd; ind++) {
+                    double rinc = dtmp * q[ind];
+                    P[ind] = P[ind] + rinc;
+                    G[ind] = G[ind] + dx * rinc;
+                    G[nd + ind] = G[nd + ind] + dy * rinc;
+                    H[ind] = H[ind] + dxx * rinc;
+                    H[nd + ind] = H[nd + ind] + dxy * rinc;
+                    H[2 * nd + ind] = H[2 * nd + ind] + dyy * rinc;
+                }
+                double rinc = -(dx * n1) - dy * n2;
+                double rincx = rinc * dx - n1 * rfac, rincy = rinc * dy - n2 * rfac;
+                double rincxx = rfac * (rinc - dx * n1) + rincx * dx;
+                double rincyy = rfac * (rinc - dy * n2) + rincy * dy;
+                double rincxy = rincx * dy - rfac * dx * n2;
+                for (int ind = 0; ind < nd; ind++) {
+                    double rt = dtmp * d[ind];
+                    P[ind] = P[ind] + rinc * rt;
+                    G[ind] = G[ind] + rincx * rt;
+                    G[nd + ind] = G[nd + ind] + rincy * rt;
+                    H[ind] = H[ind] + rincxx * rt;
+                    H[nd + ind] = H[nd + ind] + rincxy * rt;
+                    H[2 * nd + ind] = H[2 * nd + ind] + rincyy * rt;
                 }
             }
         }
     }
 }
 
-/* g2dformlc_vec, g2drouts.f:621-680: Taylor coefficients of
- * sum q e^{-|t-s|^2/delta} about c, via the recurrence
- * h_{j+1} = 2 (x h_j - h_{j-1})/(j+1), h_0 = e^{-x^2}, h_1 = 2x e^{-x^2}. */
+void orc_direct2d(int nd, double delta, double eps, const double *src, int ns,
+                  const double *chg, const double *tg, int ntg, int ifpgh,
+                  double *pot, double *grad, double *hess)
+{
+    orc_direct2d_cd(nd, delta, eps, src, ns, chg, NULL, NULL, tg, ntg, ifpgh, pot, grad, hess);
+}
+
+/* g2dformlc_vec / g2dformld_vec / g2dformlcd_vec, g2drouts.f:621-845:
+ * Taylor coefficients of the sources' field about c, via the recurrence
+ * h_{j+1} = 2 (x h_j - h_{j-1})/(j+1), h_0 = e^{-x^2}, h_1 = 2x e^{-x^2};
+ * dipoles use dh_j = -h_{j+1} dsq (j+1), in the reference's order of
+ * operations.  Accumulates into loc (the reference does not clear it). */
 static void orc_formlc(int nd, double delta, const double *src, int ns, const double *chg,
-                       const double *c, int nl, double *loc)
+                       const double *rn, const double *ds, const double *c, int nl, double *loc)
 {
     int m = nl + 1;
-    memset(loc, 0, sizeof(double) * m * m * nd);
     double dsq = 1.0 / sqrt(delta);
     double *hx = xcalloc(m + 1, sizeof(double)), *hy = xcalloc(m + 1, sizeof(double));
+    double *dhx = xcalloc(m, sizeof(double)), *dhy = xcalloc(m, sizeof(double));
     for (int i = 0; i < ns; i++) {
         double x = (src[2 * i] - c[0]) * dsq, y = (src[2 * i + 1] - c[1]) * dsq;
         double fx = exp(-x * x), fy = exp(-y * y);
         hx[0] = fx; hy[0] = fy;
         hx[1] = 2.0 * x * fx; hy[1] = 2.0 * y * fy;
-        for (int j = 1; j <= nl - 1; j++) {
+        int jtop = rn ? nl : nl - 1;
+        for (int j = 1; j <= jtop; j++) {
             hx[j + 1] = 2.0 * (x * hx[j] - hx[j - 1]) / (j + 1);
             hy[j + 1] = 2.0 * (y * hy[j] - hy[j - 1]) / (j + 1);
         }
+        if (rn)
+            for (int j = 0; j <= nl; j++) {
+                dhx[j] = -(hx[j + 1] * dsq * (j + 1.0));
+                dhy[j] = -(hy[j + 1] * dsq * (j + 1.0));
+            }
         for (int ind = 0; ind < nd; ind++) {
-            double q = chg[(size_t)nd * i + ind];
             double *L = loc + (size_t)ind * m * m;
+            if (!rn) {
+                double q = chg[(size_t)nd * i + ind];
+                for (int j1 = 0; j1 <= nl; j1++) {
+                    double yt = q * hy[j1];
+                    for (int j2 = 0; j2 <= nl; j2++) L[j2 + m * j1] += yt * hx[j2];
+                }
+                continue;
+            }
+            double r1 = rn[2 * i] * ds[(size_t)nd * i + ind];
+            double r2 = rn[2 * i + 1] * ds[(size_t)nd * i + ind];
             for (int j1 = 0; j1 <= nl; j1++) {
-                double yt = q * hy[j1];
-                for (int j2 = 0; j2 <= nl; j2++) L[j2 + m * j1] += yt * hx[j2];
+                double dy1 = hy[j1] * r1, dy2 = dhy[j1] * r2;
+                if (chg) {
+                    double yt = chg[(size_t)nd * i + ind] * hy[j1];
+                    for (int j2 = 0; j2 <= nl; j2++)
+                        L[j2 + m * j1] = L[j2 + m * j1] + dhx[j2] * dy1 + hx[j2] * dy2 + yt * hx[j2];
+                } else {
+                    for (int j2 = 0; j2 <= nl; j2++)
+                        L[j2 + m * j1] = L[j2 + m * j1] + dhx[j2] * dy1 + hx[j2] * dy2;
+                }
             }
         }
     }
-    free(hx); free(hy);
+    free(hx); free(hy); free(dhx); free(dhy);
 }
 
 /* ------------------------------------------------------------------ */
@@ -969,14 +1119,16 @@ static double wall_now(void)
  * argument, -3 unsupported (dipoles). Outputs are overwritten for the
  * requested quantities. */
 int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
-              int ifcharge, const double *charge, int ifdipole,
-              int ifpgh, double *pot, double *grad, double *hess,
+              int ifcharge, const double *charge, int ifdipole, const double *rnormal,
+              const double *dipstr, int ifpgh, double *pot, double *grad, double *hess,
               int nt, const double *targ, int ifpghtarg, double *pottarg,
               double *gradtarg, double *hesstarg)
 {
-    if (nd < 1 || ns < 1 || nt < 0 || ifcharge != 1) return -1;
-    if (ifdipole != 0) return -3;
+    if (nd < 1 || ns < 1 || nt < 0) return -1;
+    if (ifcharge < 0 || ifcharge > 1 || ifdipole < 0 || ifdipole > 1) return -1;
     if (ifpgh < 0 || ifpgh > 3 || ifpghtarg < 0 || ifpghtarg > 3) return -1;
+    if (!ifcharge) charge = NULL;
+    if (!ifdipole) rnormal = dipstr = NULL;
     memset(&g_last_stats, 0, sizeof(g_last_stats));
     double t_start = wall_now();
     int npwlevel;
@@ -992,7 +1144,8 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
         int nl = orc_hlterms(2, bs0, delta, eps) + npg - 1;
         int m = nl + 1;
         double *loc = xcalloc((size_t)m * m * nd, sizeof(double));
-        orc_formlc(nd, delta, sources, ns, charge, cen0, nl, loc);
+        if (charge || rnormal)
+            orc_formlc(nd, delta, sources, ns, charge, rnormal, dipstr, cen0, nl, loc);
         if (ifpghtarg > 0) {
             orc_leval(nd, delta, cen0, nl, loc, targ, nt, ifpghtarg, pottarg, gradtarg, hesstarg);
         }
@@ -1051,11 +1204,19 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
     /* sorted copies, dreorderf fmmcommon2d.f:3-27 */
     double *ss = xcalloc(2 * (size_t)ns, sizeof(double)), *cs = xcalloc((size_t)nd * ns, sizeof(double));
     double *tsrt = xcalloc(2 * (size_t)(nt ? nt : 1), sizeof(double));
+    double *rs = xcalloc(2 * (size_t)ns, sizeof(double)), *dss = xcalloc((size_t)nd * ns, sizeof(double));
     for (int i = 0; i < ns; i++) {
         ss[2 * i] = sources[2 * (size_t)isrc[i]];
         ss[2 * i + 1] = sources[2 * (size_t)isrc[i] + 1];
-        for (int d = 0; d < nd; d++) cs[(size_t)nd * i + d] = charge[(size_t)nd * isrc[i] + d];
+        if (charge)
+            for (int d = 0; d < nd; d++) cs[(size_t)nd * i + d] = charge[(size_t)nd * isrc[i] + d];
+        if (rnormal) {
+            rs[2 * i] = rnormal[2 * (size_t)isrc[i]];
+            rs[2 * i + 1] = rnormal[2 * (size_t)isrc[i] + 1];
+            for (int d = 0; d < nd; d++) dss[(size_t)nd * i + d] = dipstr[(size_t)nd * isrc[i] + d];
+        }
     }
+    const double *cq = charge ? cs : NULL, *crn = rnormal ? rs : NULL;
     for (int i = 0; i < nt; i++) {
         tsrt[2 * i] = targ[2 * (size_t)itarg[i]];
         tsrt[2 * i + 1] = targ[2 * (size_t)itarg[i] + 1];
@@ -1133,14 +1294,17 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
         double *hexp = xcalloc((size_t)mt * mt * nd, sizeof(double));
         double *pxhy = xcalloc(2 * (size_t)mt * npw, sizeof(double));
         double *xp = xcalloc(mt + 1, sizeof(double)), *yp = xcalloc(mt + 1, sizeof(double));
+        double *dxp = xcalloc(mt + 1, sizeof(double)), *dyp = xcalloc(mt + 1, sizeof(double));
         /* step 1: leaf Hermite expansions -> PW, fgt2d.f:871-959 */
         int lstop = npwlevel + 1 > 0 ? npwlevel + 1 : 0;
         for (int l = nlevels; l >= lstop; l--)
             for (int ib = T.laddr[2 * l]; ib <= T.laddr[2 * l + 1]; ib++) {
                 int s = isrcse[2 * ib], n = isrcse[2 * ib + 1] - s;
                 if (T.nch[ib] != 0 || n <= 0) continue;
-                orc_formhc(nd, delta, ss + 2 * (size_t)s, n, cs + (size_t)nd * s, T.cen + 2 * ib,
-                           nlocal[l], hexp, xp, yp);
+                if (!cq && !crn) continue; /* fgt2d.f:882-959: no branch forms anything */
+                orc_formh(nd, delta, ss + 2 * (size_t)s, n, cq ? cs + (size_t)nd * s : NULL,
+                          crn ? rs + 2 * (size_t)s : NULL, dss + (size_t)nd * s, T.cen + 2 * ib,
+                          nlocal[l], hexp, xp, yp, dxp, dyp);
                 orc_h2pw(nd, nlocal[l], npw, rh2x, hexp, mp + per * ib, pxhy);
             }
         /* step 2: merge children into parents, fgt2d.f:976-1009 */
@@ -1226,7 +1390,7 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
                                   hs + (size_t)3 * nd * s0);
                 }
             }
-        free(loc); free(lxpy); free(hexp); free(pxhy); free(xp); free(yp);
+        free(loc); free(lxpy); free(hexp); free(pxhy); free(xp); free(yp); free(dxp); free(dyp);
         free(iflocal); free(wms); free(wm2l); free(rh2x); free(rx2l);
     }
 
@@ -1240,14 +1404,17 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
                 int ib = list1[13 * jb + i];
                 int st = itargse[2 * ib], ntg = itargse[2 * ib + 1] - st;
                 int s0 = isrcse[2 * ib], nsr = isrcse[2 * ib + 1] - s0;
+                const double *q = cq ? cs + (size_t)nd * js : NULL;
+                const double *r = crn ? rs + 2 * (size_t)js : NULL, *dd = dss + (size_t)nd * js;
+                if (!q && !r) continue; /* fgt2dpart_direct_vec: no branch applies */
                 if (ntg > 0 && ifpghtarg > 0)
-                    orc_direct2d(nd, delta, eps / 90, ss + 2 * (size_t)js, jn, cs + (size_t)nd * js,
-                                 tsrt + 2 * (size_t)st, ntg, ifpghtarg, pt + (size_t)nd * st,
-                                 gt + (size_t)2 * nd * st, ht + (size_t)3 * nd * st);
+                    orc_direct2d_cd(nd, delta, eps / 90, ss + 2 * (size_t)js, jn, q, r, dd,
+                                    tsrt + 2 * (size_t)st, ntg, ifpghtarg, pt + (size_t)nd * st,
+                                    gt + (size_t)2 * nd * st, ht + (size_t)3 * nd * st);
                 if (nsr > 0 && ifpgh > 0)
-                    orc_direct2d(nd, delta, eps / 90, ss + 2 * (size_t)js, jn, cs + (size_t)nd * js,
-                                 ss + 2 * (size_t)s0, nsr, ifpgh, ps + (size_t)nd * s0,
-                                 gs + (size_t)2 * nd * s0, hs + (size_t)3 * nd * s0);
+                    orc_direct2d_cd(nd, delta, eps / 90, ss + 2 * (size_t)js, jn, q, r, dd,
+                                    ss + 2 * (size_t)s0, nsr, ifpgh, ps + (size_t)nd * s0,
+                                    gs + (size_t)2 * nd * s0, hs + (size_t)3 * nd * s0);
             }
         }
 
@@ -1272,7 +1439,7 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
         }
 
     free(mp); free(lc); free(ifhung); free(nlist1); free(list1);
-    free(ss); free(cs); free(tsrt); free(ps); free(gs); free(hs); free(pt); free(gt); free(ht);
+    free(ss); free(cs); free(rs); free(dss); free(tsrt); free(ps); free(gs); free(hs); free(pt); free(gt); free(ht);
     free(isrc); free(itarg); free(isrcse); free(itargse);
     ot_free(&T);
     g_last_stats.time_main = wall_now() - t_start;

@@ -36,10 +36,13 @@ def _orc():
         dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
         lib.orc_fgt2d.restype = C.c_int
         lib.orc_fgt2d.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, dp, C.c_int, dp, C.c_int,
-                                  C.c_int, dp, dp, dp, C.c_int, dp, C.c_int, dp, dp, dp]
+                                  dp, dp, C.c_int, dp, dp, dp, C.c_int, dp, C.c_int, dp, dp, dp]
         lib.orc_direct2d.restype = None
         lib.orc_direct2d.argtypes = [C.c_int, C.c_double, C.c_double, dp, C.c_int, dp, dp, C.c_int,
                                      C.c_int, dp, dp, dp]
+        lib.orc_direct2d_cd.restype = None
+        lib.orc_direct2d_cd.argtypes = [C.c_int, C.c_double, C.c_double, dp, C.c_int, dp, dp, dp,
+                                        dp, C.c_int, C.c_int, dp, dp, dp]
         lib.orc_hlterms.restype = C.c_int
         lib.orc_hlterms.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         lib.orc_pwterms.restype = None
@@ -84,21 +87,39 @@ def _outs(nd, n, ifp):
            (np.zeros(nd), np.zeros(2 * nd), np.zeros(3 * nd))
 
 
-def oracle_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
-    """The C restatement of fgt2d (fgt2d.f:19) for charges.  Returns
-    (pot, grad, hess, pottarg, gradtarg, hesstarg) shaped (n, nd), (n, 2, nd),
-    (n, 3, nd) when requested."""
+def _strengths(ns, charge, rnormal, dipstr):
+    """(nd, charge (ns, nd) or None, rnormal (ns, 2) or None, dipstr
+    (ns, nd) or None) from the arguments of a transform."""
+    chg = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64).reshape(ns, -1)
+    rn = ds = None
+    if rnormal is not None:
+        rn = np.ascontiguousarray(rnormal, dtype=np.float64).reshape(ns, -1)
+        ds = np.ascontiguousarray(dipstr, dtype=np.float64).reshape(ns, -1)
+    nd = chg.shape[1] if chg is not None else ds.shape[1]
+    return nd, chg, rn, ds
+
+
+def _ptr_or(a, n):
+    return _d(a) if a is not None else _d(np.zeros(max(n, 1)))
+
+
+def oracle_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, rnormal=None,
+                 dipstr=None):
+    """The C restatement of fgt2d (fgt2d.f:19).  charge=None: no charges;
+    rnormal/dipstr (ns, 2) and (ns, nd): dipoles.  Returns (pot, grad, hess,
+    pottarg, gradtarg, hesstarg) shaped (n, nd), (n, 2, nd), (n, 3, nd) when
+    requested."""
     lib = _orc()
     src = _pts(sources)
     ns = src.shape[0]
-    chg = np.ascontiguousarray(charge, dtype=np.float64).reshape(ns, -1)
-    nd = chg.shape[1]
+    nd, chg, rn, ds = _strengths(ns, charge, rnormal, dipstr)
     tg = _pts(targ) if targ is not None else np.zeros((0, 2))
     nt = tg.shape[0]
     pot, grad, hess = _outs(nd, ns, ifpgh)
     pt, gt, ht = _outs(nd, max(nt, 1), ifpghtarg)
-    rc = lib.orc_fgt2d(nd, delta, eps, ns, _d(src), 1, _d(chg), 0, ifpgh, _d(pot), _d(grad), _d(hess),
-                       nt, _d(tg), ifpghtarg, _d(pt), _d(gt), _d(ht))
+    rc = lib.orc_fgt2d(nd, delta, eps, ns, _d(src), int(chg is not None), _ptr_or(chg, nd * ns),
+                       int(rn is not None), _ptr_or(rn, 2 * ns), _ptr_or(ds, nd * ns), ifpgh,
+                       _d(pot), _d(grad), _d(hess), nt, _d(tg), ifpghtarg, _d(pt), _d(gt), _d(ht))
     if rc != 0:
         raise RuntimeError("orc_fgt2d returned %d" % rc)
     return pot, grad, hess, pt[:nt], gt[:nt], ht[:nt]
@@ -129,6 +150,22 @@ def direct2d(delta, eps, sources, charge, targ, ifpgh=1):
     return pot, grad, hess
 
 
+def direct2d_cd(delta, eps, sources, charge, rnormal, dipstr, targ, ifpgh=1):
+    """The reference's near-field sums with dipoles (g2d_directdp/dg/dh and
+    g2d_directcdp/cdg/cdh, g2ddirect.f:217-655); charge=None: dipoles only."""
+    lib = _orc()
+    src = _pts(sources)
+    ns = src.shape[0]
+    nd, chg, rn, ds = _strengths(ns, charge, rnormal, dipstr)
+    tg = _pts(targ)
+    nt = tg.shape[0]
+    pot, grad, hess = np.zeros((nt, nd)), np.zeros((nt, 2, nd)), np.zeros((nt, 3, nd))
+    lib.orc_direct2d_cd(nd, delta, eps, _d(src), ns, _d(chg) if chg is not None else None,
+                        _ptr_or(rn, 2 * ns), _ptr_or(ds, nd * ns), _d(tg), nt, ifpgh,
+                        _d(pot), _d(grad), _d(hess))
+    return pot, grad, hess
+
+
 def derr(vref, vcomp):
     """Relative l2 error of the reference test (test_fgt2d.f:204-225)."""
     a = np.ascontiguousarray(vref, dtype=np.float64).ravel()
@@ -148,12 +185,13 @@ class Hkrand:
         return out
 
 
-def reftest_inputs(nsrc, ntarg):
+def reftest_inputs(nsrc, ntarg, dipoles=False):
     """Inputs of the reference's own test driver (2d/planewave/test/
     test_fgt2d.f:81-120) for nsrc sources and ntarg targets: per source seven
     draws (angle, x, y, two normal components, charge, dipole strength), the
     corner points (0,0) and (1,1) with zero charge and a unit charge at
-    (0.05, 0.05), then two draws per target."""
+    (0.05, 0.05), then two draws per target.  dipoles=True also returns the
+    normals and dipole strengths of those draws."""
     r = Hkrand()
     d = r.draw(7 * nsrc).reshape(nsrc, 7)
     src = d[:, 1:3].copy()
@@ -164,6 +202,8 @@ def reftest_inputs(nsrc, ntarg):
     src[1] = (1.0, 1.0)
     src[2] = (0.05, 0.05)
     tg = r.draw(2 * ntarg).reshape(ntarg, 2)
+    if dipoles:
+        return src, chg, tg, d[:, 3:5].copy(), d[:, 6:7].copy()
     return src, chg, tg
 
 
@@ -264,7 +304,8 @@ def ref_defined(delta, eps, sources, targ):
     return t["npwlevel"] != t["nlevels"]
 
 
-def ref_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
+def ref_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, rnormal=None,
+              dipstr=None):
     """Calls the reference's own fgt2d_ (fgt2d.f:19) through its Fortran ABI.
     Raises RefUndefined where the reference reads uninitialised memory."""
     lib = _ref()
@@ -272,8 +313,12 @@ def ref_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
         raise RefUndefined("npwlevel == nlevels: the reference's lmptot is uninitialised")
     src = _pts(sources)
     ns = src.shape[0]
-    chg = np.ascontiguousarray(charge, dtype=np.float64).reshape(ns, -1)
-    nd = chg.shape[1]
+    nd, chg, rn, ds = _strengths(ns, charge, rnormal, dipstr)
+    ifc, ifd = int(chg is not None), int(rn is not None)
+    if chg is None:
+        chg = np.zeros((ns, nd))
+    if rn is None:
+        rn, ds = np.zeros((ns, 2)), np.zeros((ns, nd))
     tg = _pts(targ) if targ is not None else np.zeros((1, 2))
     nt = tg.shape[0] if targ is not None else 0
     pot, grad, hess = _outs(nd, ns, ifpgh)
@@ -292,8 +337,15 @@ def ref_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
         nloc = nl.value + max(ifpgh, ifpghtarg) - 1          # fgt2d.f:134-135
         local = np.zeros(nd * (nloc + 1) ** 2)
         chg_f = np.ascontiguousarray(chg)                    # (ns, nd) = charge(nd, ns)
-        lib.g2dformlc_vec_(_ri(nd), _rd(delta), _d(src), _ri(ns), _d(chg_f), _d(cen0), _ri(nloc),
-                           _d(local))
+        if ifc and not ifd:                                  # fgt2d.f:1688-1696
+            lib.g2dformlc_vec_(_ri(nd), _rd(delta), _d(src), _ri(ns), _d(chg_f), _d(cen0),
+                               _ri(nloc), _d(local))
+        elif ifd and not ifc:
+            lib.g2dformld_vec_(_ri(nd), _rd(delta), _d(src), _ri(ns), _d(rn), _d(ds), _d(cen0),
+                               _ri(nloc), _d(local))
+        elif ifd and ifc:
+            lib.g2dformlcd_vec_(_ri(nd), _rd(delta), _d(src), _ri(ns), _d(chg_f), _d(rn), _d(ds),
+                                _d(cen0), _ri(nloc), _d(local))
         legs = [(ifpghtarg, tg, nt, pt, gt, ht), (ifpgh, src, ns, pot, grad, hess)]
         for ifp, x, n, p, g, h in legs:                      # targets first, fgt2d.f:1697-1732
             if ifp == 1:
@@ -306,9 +358,8 @@ def ref_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
                 lib.g2dlevalh_vec_(_ri(nd), _rd(delta), _d(cen0), _ri(nloc), _d(local), _d(x),
                                    _ri(n), _d(p), _d(g), _d(h))
         return pot, grad, hess, pt[:nt], gt[:nt], ht[:nt]
-    dummy = np.zeros(2 * max(ns, nt, 1))
-    lib.fgt2d_(_ri(nd), _rd(delta), _rd(eps), _ri(ns), _d(src), _ri(1), _d(chg), _ri(0), _d(dummy),
-               _d(dummy), _ri(0), _ri(ifpgh), _d(pot), _d(grad), _d(hess), _ri(nt), _d(tg),
+    lib.fgt2d_(_ri(nd), _rd(delta), _rd(eps), _ri(ns), _d(src), _ri(ifc), _d(chg), _ri(ifd), _d(rn),
+               _d(ds), _ri(0), _ri(ifpgh), _d(pot), _d(grad), _d(hess), _ri(nt), _d(tg),
                _ri(ifpghtarg), _d(pt), _d(gt), _d(ht))
     return pot, grad, hess, pt[:nt], gt[:nt], ht[:nt]
 

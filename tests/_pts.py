@@ -62,16 +62,23 @@ def leaf_runs(keys_sorted, nlmax, levcut, ndiv0=10, ndiv=80):
 # inputs regenerable bit-for-bit on any host: DLARAN uniforms (the reference
 # test's generator) combined with exact IEEE arithmetic only
 # ---------------------------------------------------------------------------
-def dlaran_points(kind, ns, nt, nd=1):
+def dlaran_points(kind, ns, nt, nd=1, dipoles=False):
     """Sources, charges (ns, nd) and targets drawn from one DLARAN stream.
 
     uniform  : points uniform in [0,1)^2, charges 2u-1.
     cluster  : 8 centres in [0.2,0.8]^2; offsets 0.01*(u1+u2+u3+u4-2) per axis
                (a bell of width ~0.01); charges 2u-1.
     reftest  : the reference test driver's own inputs (test_fgt2d.f:81-120).
+
+    dipoles=True also returns normals (ns, 2) and dipole strengths (ns, nd):
+    for reftest the driver's own draws, otherwise 2u-1 drawn after the
+    targets from the same stream.
     """
     from oracle import oracle as O
     if kind == "reftest":
+        if dipoles:
+            src, chg, tg, rn, ds = O.reftest_inputs(ns, nt, dipoles=True)
+            return src, chg, tg, rn, np.repeat(ds, nd, axis=1) if nd > 1 else ds
         src, chg, tg = O.reftest_inputs(ns, nt)
         return src, chg, tg
     r = O.Hkrand()
@@ -87,7 +94,24 @@ def dlaran_points(kind, ns, nt, nd=1):
     src = pts(ns)
     chg = 2.0 * r.draw(ns * nd).reshape(ns, nd) - 1.0
     tg = pts(nt)
+    if dipoles:
+        rn = 2.0 * r.draw(2 * ns).reshape(ns, 2) - 1.0
+        ds = 2.0 * r.draw(ns * nd).reshape(ns, nd) - 1.0
+        return src, chg, tg, rn, ds
     return src, chg, tg
+
+
+def golden_inputs(g):
+    """(sources, charge or None, targets, rnormal or None, dipstr or None)
+    of a 2D golden fixture."""
+    ns, nt, nd = int(g["ns"]), int(g["nt"]), int(g["nd"])
+    ifc = int(g["ifcharge"]) if "ifcharge" in g else 1
+    ifd = int(g["ifdipole"]) if "ifdipole" in g else 0
+    if not ifd:
+        src, chg, tg = dlaran_points(str(g["recipe"]), ns, nt, nd)
+        return src, chg, tg, None, None
+    src, chg, tg, rn, ds = dlaran_points(str(g["recipe"]), ns, nt, nd, dipoles=True)
+    return src, chg if ifc else None, tg, rn, ds
 
 
 def dlaran_points3d(kind, ns, nt, nd=1):

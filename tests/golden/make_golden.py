@@ -5,6 +5,7 @@ Run in the container that holds the reference sources, after
 them:
 
     python tests/golden/make_golden.py        # 2D fixtures
+    python tests/golden/make_golden.py dip    # 2D fixtures with dipole sources
     python tests/golden/make_golden.py 3d     # 3D fixtures (dim = 3)
 
 Every fixture is data only: the name of its input recipe
@@ -48,6 +49,14 @@ CASES3D = {
     "uniform3d_eps9": ("uniform", 50_000, 3_000, 1, 3e-3, 1e-9, 2, 3),
     "shell3d": ("shell", 60_000, 20_000, 1, 2e-3, 1e-6, 1, 1),
 }
+# dipole sources (fgt2d.f:35, ifdipole = 1):
+# name: (recipe, ns, nt, nd, delta, eps, ifpgh, ifpghtarg, ifcharge, ifdipole)
+CASES_DIP = {
+    "reftest_dip_200k": ("reftest", 200_000, 100_000, 1, 2e-5, 1e-9, 1, 1, 1, 1),
+    "cluster_dip_only_gh_nd2": ("cluster", 60_000, 30_000, 2, 1e-4, 1e-6, 3, 2, 0, 1),
+    "uniform_dip_eps3": ("uniform", 50_000, 20_000, 1, 1e-3, 1e-3, 2, 3, 1, 1),
+    "large_delta_dip": ("uniform", 5_000, 2_000, 2, 5.0, 1e-6, 3, 3, 1, 1),
+}
 NSAMPLE = 1000
 
 
@@ -84,9 +93,39 @@ def main3d(only=None):
         print("wrote", name, rec["ints"])
 
 
+def main_dip(only=None):
+    if not O.have_ref():
+        sys.exit("oracle/_ref/libref2d.so missing: run make -C oracle ref first")
+    for name, (recipe, ns, nt, nd, delta, eps, ifs, ift, ifc, ifd) in CASES_DIP.items():
+        if only and name not in only:
+            continue
+        src, chg, tg, rn, ds = _pts.dlaran_points(recipe, ns, nt, nd, dipoles=True)
+        chg = chg if ifc else None
+        if not O.ref_defined(delta, eps, src, tg):
+            raise SystemExit("%s: the reference is undefined for this input" % name)
+        out = O.ref_fgt2d(delta, eps, src, chg, tg, ifpgh=ifs, ifpghtarg=ift, rnormal=rn,
+                          dipstr=ds)
+        t = O.oracle_tree2d(delta, eps, src, tg)
+        isrc, itrg = sample(ns, 1), sample(nt, 2)
+        rec = dict(recipe=recipe, ns=ns, nt=nt, nd=nd, delta=delta, eps=eps, ifpgh=ifs,
+                   ifpghtarg=ift, ifcharge=ifc, ifdipole=ifd, isample_src=isrc,
+                   isample_trg=itrg,
+                   ints=np.array([t["npwlevel"], t["nlmax"], t["nlevels"], t["nboxes"]]))
+        names = ["pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"]
+        for k, v in enumerate(out):
+            if (k < 3 and k < ifs) or (k >= 3 and k - 3 < ift):
+                rec[names[k]] = v[isrc if k < 3 else itrg]
+        # direct sum without cutoff at the first 20 sources
+        rec["pot_direct20"] = O.direct2d_cd(delta, 1e-100, src, chg, rn, ds, src[:20])[0]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+        print("wrote", name, rec["ints"])
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "3d":
         return main3d(sys.argv[2:])
+    if len(sys.argv) > 1 and sys.argv[1] == "dip":
+        return main_dip(sys.argv[2:])
     if not O.have_ref():
         sys.exit("oracle/_ref/libref2d.so missing: run make -C oracle ref first")
     for name, (recipe, ns, nt, nd, delta, eps, ifs, ift) in CASES.items():

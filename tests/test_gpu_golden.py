@@ -32,10 +32,10 @@ def test_gpu_matches_reference_golden(path):
         g = {k: z[k] for k in z.files}
     ns, nt, nd = int(g["ns"]), int(g["nt"]), int(g["nd"])
     print("golden", os.path.basename(path), ns, nt, nd, flush=True)
-    src, chg, tg = _pts.dlaran_points(str(g["recipe"]), ns, nt, nd)
+    src, chg, tg, rn, ds = _pts.golden_inputs(g)
     delta, eps = float(g["delta"]), float(g["eps"])
     out = fgt2d_amd.fgt2d(delta, eps, src, chg, tg, ifpgh=int(g["ifpgh"]),
-                          ifpghtarg=int(g["ifpghtarg"]))
+                          ifpghtarg=int(g["ifpghtarg"]), rnormal=rn, dipstr=ds, ifdipole=int(rn is not None))
     for k, name in enumerate(NAMES):
         if name in g:
             sel = g["isample_src"] if k < 3 else g["isample_trg"]
@@ -45,4 +45,6 @@ def test_gpu_matches_reference_golden(path):
     if st["npwlevel"] >= -2:
         assert [st["nlmax"], st["nlevels"], st["nboxes"]] == list(g["ints"][1:])
     if "pot" in g:
-        assert O.derr(g["pot_direct20"], out[0][:20]) < 30 * eps
+        # dipoles: the reference's own error is ~100 eps (term counts of the
+        # charge kernel plus one, fgt2d.f:340-345)
+        assert O.derr(g["pot_direct20"], out[0][:20]) < (300 if rn is not None else 30) * eps

@@ -116,7 +116,14 @@ def test_invalid_arguments_are_rejected():
         fgt2d_amd.fgt2d(1e-3, 1e-6, bad, chg, None, ifpgh=1)
     assert e.value.code == fgt2d_amd.FGTPW_EINVAL
     with pytest.raises(fgt2d_amd.FGTError) as e:
-        fgt2d_amd.fgt2d(1e-3, 1e-6, src, chg, None, ifpgh=1, ifdipole=1)
-    assert e.value.code == fgt2d_amd.FGTPW_EUNSUPPORTED
+        fgt2d_amd.fgt2d(1e-3, 1e-6, src, chg, None, ifpgh=1, ifcharge=2)
+    assert e.value.code == fgt2d_amd.FGTPW_EINVAL
+    # a dipole flag without dipole arrays, through the C-ABI itself
+    lib = fgt2d_amd.load()
+    z = np.zeros(8)
+    p = fgt2d_amd._ptr(z)
+    rc = lib.fgtpw_fgt2d(1, 1e-3, 1e-6, 100, fgt2d_amd._ptr(src), 1, fgt2d_amd._ptr(chg), 1, None,
+                         None, 0, 1, fgt2d_amd._ptr(np.zeros(100)), p, p, 0, p, 0, p, p, p)
+    assert rc == fgt2d_amd.FGTPW_EINVAL
     # the library still works after errors
     fgt2d_amd.fgt2d(1e-3, 1e-6, src, chg, None, ifpgh=1)

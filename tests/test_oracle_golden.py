@@ -29,9 +29,10 @@ def _rel(a, b):
 def test_oracle_matches_golden(path):
     g = load(path)
     ns, nt, nd = int(g["ns"]), int(g["nt"]), int(g["nd"])
-    src, chg, tg = _pts.dlaran_points(str(g["recipe"]), ns, nt, nd)
+    src, chg, tg, rn, ds = _pts.golden_inputs(g)
     delta, eps = float(g["delta"]), float(g["eps"])
-    out = O.oracle_fgt2d(delta, eps, src, chg, tg, int(g["ifpgh"]), int(g["ifpghtarg"]))
+    out = O.oracle_fgt2d(delta, eps, src, chg, tg, int(g["ifpgh"]), int(g["ifpghtarg"]),
+                         rnormal=rn, dipstr=ds)
     for k, name in enumerate(NAMES):
         if name in g:
             sel = g["isample_src"] if k < 3 else g["isample_trg"]
@@ -43,9 +44,11 @@ def test_oracle_matches_golden(path):
         assert [st["nlmax"], st["nlevels"], st["nboxes"]] == list(g["ints"][1:])
     # the reference test's accuracy check: direct sum at the first 20 sources
     if "pot" in g:
-        ex = O.direct2d(delta, 1e-100, src, chg, src[:20])[0]
+        ex = O.direct2d_cd(delta, 1e-100, src, chg, rn, ds, src[:20])[0]
         assert _rel(ex, g["pot_direct20"]) < 1e-13
-        assert O.derr(g["pot_direct20"], g["pot"][:20]) < 30 * eps
+        # with dipoles the reference keeps the charge kernel's term counts
+        # (nlocal + 1, fgt2d.f:340-345) and its error grows to ~100 eps
+        assert O.derr(g["pot_direct20"], g["pot"][:20]) < (300 if rn is not None else 30) * eps
 
 
 def test_dlaran_closed_form():
