@@ -1273,8 +1273,16 @@ void run_large_delta3(Workspace& W, const Call& q, hipStream_t s, int npwlevel, 
     HIPCHECK(hipMemcpyAsync(dint, hp, sizeof(hostarr), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemcpyAsync(dcen, hp + 64, 3 * sizeof(double), hipMemcpyHostToDevice, s));
     tm.begin(S_FORM);
+    double* wsd = nullptr;  // dipole weights d n in input order
+    if (q.rn) {
+        unsigned* id = W.get<unsigned>("ld_iota", q.ns);
+        launch_iota(id, q.ns, s);
+        wsd = W.get<double>("ld_dipw", 3 * (size_t)nd * q.ns);
+        launch_gather_dip3(q.rn, q.ds, nd, id, q.ns, wsd, s);
+    }
     for (int ind = 0; ind < nd; ++ind)
-        launch_formlc3(q.src, q.chg, nd, ind, q.ns, cen0, dsq, nl, part, nparts, L + ind * m3, s);
+        launch_formlc3(q.src, q.chg, wsd, nd, ind, q.ns, cen0, dsq, nl, part, nparts, L + ind * m3,
+                       s);
     tm.end(S_FORM);
     tm.begin(S_EVAL);
     if (q.ifpgh > 0) {
@@ -1338,10 +1346,11 @@ void run_large_delta3(Workspace& W, const Call& q, hipStream_t s, int npwlevel, 
 // dmax = ln(1/eps) delta and the 3D modified list1 (tree_routs3d.f:511-620).
 void run3d(Workspace& W, const Call& q, hipStream_t s)
 {
-    if (!q.chg) {  // no charges (dipoles are refused before): nothing to add
+    if (!q.chg && !q.rn) {
         zero_outputs(q, 3, s);
         return;
     }
+    const int ifdipole = q.rn ? 1 : 0;
     Timer tm(W, s);
     const int ns = q.ns, nt = q.nt, nd = q.nd;
     tm.begin(S_BBOX);
@@ -1427,7 +1436,17 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     double* xs = W.get<double>("xs", 3 * (size_t)ns);
     double* qs = W.get<double>("qs", (size_t)nd * ns);
     tm.ctx(-1, nullptr, -1, ns);
-    launch_gather3(q.src, q.chg, nd, isrc, ns, xs, qs, s);
+    if (q.chg)
+        launch_gather3(q.src, q.chg, nd, isrc, ns, xs, qs, s);
+    else {
+        launch_gather3(q.src, nullptr, nd, isrc, ns, xs, nullptr, s);
+        launch_zero(qs, (size_t)nd * ns, s);
+    }
+    double* wsd = nullptr;  // dipole weights d n in sorted order
+    if (q.rn) {
+        wsd = W.get<double>("dip_w", 3 * (size_t)nd * ns);
+        launch_gather_dip3(q.rn, q.ds, nd, isrc, ns, wsd, s);
+    }
     unsigned* itarg = nullptr;
     u64* kt2 = nullptr;
     double* xt = nullptr;
@@ -1453,12 +1472,12 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     W.last_dim = 3;
     const int nb = T.nboxes(), nlevels = T.nlevels, nleaves = (int)T.leaf_of_rank.size();
     const bool use_src = q.ifpgh > 0, use_trg = q.ifpghtarg > 0 && nt > 0;
-    const int nadd = std::max(q.ifpgh, q.ifpghtarg);
+    const int nadd = ifdipole + std::max(q.ifpgh, q.ifpghtarg);  // fgt3d.f:355
     std::vector<int> nlocal(nlevels + 1, 0);
     int ntermmax = 0;
     for (int i = nlevstart; i <= nlevels; ++i) {
         nlocal[i] = hermite_terms(3, T.bsz[i], q.delta, q.eps);
-        if (q.ifpgh >= 2 || q.ifpghtarg >= 2) nlocal[i] += nadd;
+        if (ifdipole || q.ifpgh >= 2 || q.ifpghtarg >= 2) nlocal[i] += nadd;
         ntermmax = std::max(ntermmax, nlocal[i]);
     }
     double pmax = 0;
@@ -1745,6 +1764,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
                 fa.sstride = sstride;
                 fa.grid = std::min(GRID3, fa.nbox);
                 fa.thresh = form_thresh[l];
+                fa.dipw = wsd;
                 tm.ctx(l, &form_l[l], 0);
                 if (fa.thresh != INT_MAX) launch_form3_hermite(fa, s);
                 if (fa.thresh > 0) launch_form3_pw(fa, s);
@@ -1827,6 +1847,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         da.xs = xs;
         da.qs = qs;
         da.nd = nd;
+        da.ws = wsd;
         da.delta = q.delta;
         da.dmax = std::log(1.0 / q.eps) * q.delta;
         if (use_trg) {
@@ -2028,8 +2049,6 @@ int run_host(int dim, int nd, double delta, double eps, int ns, const double* so
     return run_guarded([&] {
         validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
         check_strengths(ifcharge, charge, ifdipole, rnormal, dipstr);
-        if (dim == 3 && ifdipole)
-            fail(FGTPW_EUNSUPPORTED, "dipole sources in 3D are not supported in this build");
         std::lock_guard<std::mutex> lk(g_mutex);
         select_device();
         host_transform(workspace(), dim, nd, delta, eps, ns, sources, ifcharge ? charge : nullptr,
@@ -2049,8 +2068,6 @@ int run_part_direct(int dim, int nd, double delta, double dmax, int istart, int 
         if (nd < 1) fail(FGTPW_EINVAL, "nd must be >= 1");
         if (ifcharge != 0 && ifcharge != 1) fail(FGTPW_EINVAL, "ifcharge must be 0 or 1");
         if (ifdipole != 0 && ifdipole != 1) fail(FGTPW_EINVAL, "ifdipole must be 0 or 1");
-        if (dim == 3 && ifdipole)
-            fail(FGTPW_EUNSUPPORTED, "dipole sources in 3D are not supported in this build");
         check_strengths(ifcharge, charge, ifdipole, rnormal, dipstr);
         if (!ifcharge && !ifdipole) return;  // fgt2dpart_direct_vec: no branch applies
         if (ifpgh < 1 || ifpgh > 3) return;  // the reference does nothing for other values
@@ -2080,13 +2097,16 @@ int run_part_direct(int dim, int nd, double delta, double dmax, int istart, int 
             double* d_rn = W.get<double>("pd_rn", dim * (size_t)ns);
             double* d_ds = W.get<double>("pd_ds", (size_t)nd * ns);
             unsigned* id = W.get<unsigned>("pd_iota", ns);
-            d_ws = W.get<double>("pd_ws", 2 * (size_t)nd * ns);
+            d_ws = W.get<double>("pd_ws", dim * (size_t)nd * ns);
             HIPCHECK(hipMemcpyAsync(d_rn, rnormal + dim * (size_t)(istart - 1), dim * (size_t)ns * 8,
                                     hipMemcpyHostToDevice, s));
             HIPCHECK(hipMemcpyAsync(d_ds, dipstr + (size_t)nd * (istart - 1), (size_t)nd * ns * 8,
                                     hipMemcpyHostToDevice, s));
             launch_iota(id, ns, s);
-            launch_gather_dip(d_rn, d_ds, nd, id, ns, d_ws, s);
+            if (dim == 2)
+                launch_gather_dip(d_rn, d_ds, nd, id, ns, d_ws, s);
+            else
+                launch_gather_dip3(d_rn, d_ds, nd, id, ns, d_ws, s);
         }
         HIPCHECK(hipMemcpyAsync(d_tg, tg, dim * (size_t)nt * 8, hipMemcpyHostToDevice, s));
         HIPCHECK(hipMemcpyAsync(d_p, pot + (size_t)nd * (jstart - 1), (size_t)nd * nt * 8,
@@ -2100,8 +2120,9 @@ int run_part_direct(int dim, int nd, double delta, double dmax, int istart, int 
         if (dim == 2)
             launch_direct_all(d_src, d_chg, d_ws, nd, ns, d_tg, nt, delta, dmax, ifpgh, d_p, d_g,
                               d_h, s);
-        else if (d_chg)
-            launch_direct_all3(d_src, d_chg, nd, ns, d_tg, nt, delta, dmax, ifpgh, d_p, d_g, d_h, s);
+        else
+            launch_direct_all3(d_src, d_chg, d_ws, nd, ns, d_tg, nt, delta, dmax, ifpgh, d_p, d_g,
+                               d_h, s);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(pot + (size_t)nd * (jstart - 1), d_p, (size_t)nd * nt * 8,
                                 hipMemcpyDeviceToHost, s));
@@ -2147,17 +2168,16 @@ int fgtpw_fgt3d_device(int nd, double delta, double eps, int ns, const double* s
                        double* hess, int nt, const double* targ, int ifpghtarg, double* pottarg,
                        double* gradtarg, double* hesstarg, void* stream)
 {
-    (void)rnormal; (void)dipstr; (void)iper;
+    (void)iper;
     return run_guarded([&] {
         validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
         check_strengths(ifcharge, charge, ifdipole, rnormal, dipstr);
         std::lock_guard<std::mutex> lk(g_mutex);
         select_device();
         Workspace& W = workspace();
-        if (ifdipole)
-            fail(FGTPW_EUNSUPPORTED, "dipole sources in 3D are not supported in this build");
         Call q{nd, delta, eps, ns, sources, ifcharge ? charge : nullptr, ifpgh, pot, grad, hess,
                nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg};
+        if (ifdipole) { q.rn = rnormal; q.ds = dipstr; }
         hipStream_t s = (hipStream_t)stream;
         run3d(W, q, s);
         HIPCHECK(hipStreamSynchronize(s));
@@ -2522,8 +2542,7 @@ long long fgtpw_submit(const fgtpw_call* c)
                  c->ifpghtarg);
         if (!c->sources || (c->nt > 0 && !c->targ)) fail(FGTPW_EINVAL, "missing input array");
         check_strengths(c->ifcharge, c->charge, c->ifdipole, c->rnormal, c->dipstr);
-        if (c->dim == 3 && c->ifdipole)
-            fail(FGTPW_EUNSUPPORTED, "dipole sources in 3D are not supported in this build");
+
         auto r = std::make_unique<ServeRequest>();
         r->dim = c->dim;
         r->nd = c->nd;

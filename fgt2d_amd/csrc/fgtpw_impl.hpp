@@ -226,7 +226,7 @@ struct DirectArgs {
     double delta, dmax;
     int ifpgh;
     double *pot, *grad, *hess;
-    const double* ws;       // dipole weights d n, (n, nd, 2) sorted, or nullptr
+    const double* ws;       // dipole weights d n, (n, nd, dim) sorted, or nullptr
 };
 void launch_direct(const DirectArgs& a, hipStream_t s);
 
@@ -277,6 +277,7 @@ struct Form3Args {
     int grid;
     int thresh;             // Hermite route for boxes with more than thresh
                             // sources, direct plane-wave route for the others
+    const double* dipw = nullptr;  // dipole weights d n, (n, nd, 3) sorted, or nullptr
 };
 void launch_form3_hermite(const Form3Args& a, hipStream_t s);
 void launch_form3_pw(const Form3Args& a, hipStream_t s);
@@ -321,10 +322,14 @@ void launch_pw2local3(const Eval3Args& a, hipStream_t s);
 void launch_taylor_eval3(const Eval3Args& a, hipStream_t s);
 void launch_pw_eval3(const Eval3Args& a, hipStream_t s);
 void launch_direct3(const DirectArgs& a, hipStream_t s);  // points (3,n)
-void launch_direct_all3(const double* src, const double* q, int nd, int ns, const double* tg,
-                        int nt, double delta, double dmax, int ifpgh, double* pot, double* grad,
-                        double* hess, hipStream_t s);
-void launch_formlc3(const double* p, const double* q, int nd, int ind, int n, const double* c,
-                    double dsq, int nl, double* part, int nparts, double* out, hipStream_t s);
+void launch_direct_all3(const double* src, const double* q, const double* ws, int nd, int ns,
+                        const double* tg, int nt, double delta, double dmax, int ifpgh,
+                        double* pot, double* grad, double* hess, hipStream_t s);
+void launch_formlc3(const double* p, const double* q, const double* ws, int nd, int ind, int n,
+                    const double* c, double dsq, int nl, double* part, int nparts, double* out,
+                    hipStream_t s);
+// dipole weights w(c, ind, i) = rnormal(c, perm i) dipstr(ind, perm i), sorted, (n, nd, 3)
+void launch_gather_dip3(const double* rn, const double* ds, int nd, const unsigned* perm, int n,
+                        double* ws, hipStream_t s);
 
 }  // namespace fgtpw

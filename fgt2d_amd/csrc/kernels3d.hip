@@ -162,6 +162,33 @@ __global__ __launch_bounds__(256) void k_gather3(const double* __restrict__ p,
         for (int d = 0; d < nd; ++d) qs[(long)nd * i + d] = q[(long)nd * j + d];
 }
 
+// dipole weights in sorted order: w(c, ind, i) = rnormal(c, perm i) dipstr(ind, perm i)
+// at ws[3 (nd i + ind) + c] (the products of g3drouts.f:273-275)
+__global__ __launch_bounds__(256) void k_gather_dip3(const double* __restrict__ rn,
+                                                     const double* __restrict__ ds, int nd,
+                                                     const unsigned* __restrict__ perm, int n,
+                                                     double* __restrict__ ws)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    long j = perm[i];
+    const double n1 = rn[3 * j], n2 = rn[3 * j + 1], n3 = rn[3 * j + 2];
+    for (int d = 0; d < nd; ++d) {
+        const double dd = ds[(long)nd * j + d];
+        double* w = ws + 3 * ((long)nd * i + d);
+        w[0] = n1 * dd;
+        w[1] = n2 * dd;
+        w[2] = n3 * dd;
+    }
+}
+
+void launch_gather_dip3(const double* rn, const double* ds, int nd, const unsigned* perm, int n,
+                        double* ws, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_gather_dip3<<<gridn(n, 256), 256, 0, s>>>(rn, ds, nd, perm, n, ws);
+}
+
 void launch_gather3(const double* p, const double* q, int nd, const unsigned* perm, int n,
                     double* ps, double* qs, hipStream_t s)
 {
@@ -194,8 +221,12 @@ static void allow_lds3(F* kernel, size_t bytes)
 }
 
 // F3_MAXM: register tile of the moment accumulators (Hermite orders up to
-// F3_MAXM - 1); instantiated for 32 and 64
-template <int F3_MAXM>
+// F3_MAXM - 1); instantiated for 32 and 64.  With dipoles (w = d n per
+// source, g3dformhd_vec / g3dformhcd_vec, g3drouts.f:197-409) the raw moments
+// gain the source derivatives:
+//   M(j,k,l) += x^j [q y^k z^l + dsq (w2 k y^(k-1) z^l + w3 l y^k z^(l-1))]
+//             + dsq j x^(j-1) w1 y^k z^l.
+template <int F3_MAXM, bool DIP>
 __global__ __launch_bounds__(256) void k_form3_hermite(Form3Args a)
 {
     extern __shared__ double sm[];
@@ -211,7 +242,11 @@ __global__ __launch_bounds__(256) void k_form3_hermite(Form3Args a)
         // ---- moments: thread per (k,l), all j in registers ----
         double* xp = sm;                          // [F3_CHUNK][m]
         double* yp = sm + F3_CHUNK * m;
-        double* zq = sm + 2 * F3_CHUNK * m;       // q z^l
+        double* zq = sm + 2 * F3_CHUNK * m;       // q z^l (dipoles: z^l)
+        double* xd = sm + 3 * F3_CHUNK * m;       // dipoles: j x^(j-1)
+        double* yd = sm + 4 * F3_CHUNK * m;       //          k y^(k-1)
+        double* zd = sm + 5 * F3_CHUNK * m;       //          l z^(l-1)
+        double* sw = sm + 6 * F3_CHUNK * m;       //          q, dsq w1, dsq w2, dsq w3
         for (int p0 = 0; p0 < m * m; p0 += 256) {
             const int pr = p0 + t, k = pr % m, l = pr / m;
             const bool act = pr < m * m;
@@ -225,24 +260,52 @@ __global__ __launch_bounds__(256) void k_form3_hermite(Form3Args a)
                     long i = s0 + t;
                     double x = (a.xs[3 * i] - cx) * a.dsq, y = (a.xs[3 * i + 1] - cy) * a.dsq,
                            z = (a.xs[3 * i + 2] - cz) * a.dsq;
-                    double px = 1, py = 1, pz = a.qs[(long)a.nd * i + a.ind];
+                    const double q = a.qs[(long)a.nd * i + a.ind];
+                    double px = 1, py = 1, pz = DIP ? 1.0 : q;
+                    double qx = 0, qy = 0, qz = 0;  // previous powers
                     for (int j = 0; j < m; ++j) {
                         xp[t * m + j] = px;
                         yp[t * m + j] = py;
                         zq[t * m + j] = pz;
+                        if (DIP) {
+                            xd[t * m + j] = j * qx;
+                            yd[t * m + j] = j * qy;
+                            zd[t * m + j] = j * qz;
+                            qx = px;
+                            qy = py;
+                            qz = pz;
+                        }
                         px *= x;
                         py *= y;
                         pz *= z;
+                    }
+                    if (DIP) {
+                        const double* wv = a.dipw + 3 * ((long)a.nd * i + a.ind);
+                        sw[4 * t] = q;
+                        sw[4 * t + 1] = a.dsq * wv[0];
+                        sw[4 * t + 2] = a.dsq * wv[1];
+                        sw[4 * t + 3] = a.dsq * wv[2];
                     }
                 }
                 __syncthreads();
                 if (act)
                     for (int s = 0; s < cn; ++s) {
-                        const double w = yp[s * m + k] * zq[s * m + l];
                         const double* xr = xp + s * m;
+                        if (!DIP) {
+                            const double w = yp[s * m + k] * zq[s * m + l];
 #pragma unroll
-                        for (int j = 0; j < F3_MAXM; ++j)
-                            if (j < m) acc[j] += w * xr[j];
+                            for (int j = 0; j < F3_MAXM; ++j)
+                                if (j < m) acc[j] += w * xr[j];
+                        } else {
+                            const double yk = yp[s * m + k], zl = zq[s * m + l];
+                            const double F = sw[4 * s] * yk * zl + sw[4 * s + 2] * yd[s * m + k] * zl +
+                                             sw[4 * s + 3] * yk * zd[s * m + l];
+                            const double G = sw[4 * s + 1] * yk * zl;
+                            const double* xdr = xd + s * m;
+#pragma unroll
+                            for (int j = 0; j < F3_MAXM; ++j)
+                                if (j < m) acc[j] += F * xr[j] + G * xdr[j];
+                        }
                     }
             }
             if (act)
@@ -323,23 +386,32 @@ void launch_form3_hermite(const Form3Args& a, hipStream_t s)
     if (a.nbox <= 0) return;
     const int m = a.p + 1;
     if (m > 64) fail(-3, "3D Hermite order above 63 is not supported");
-    size_t lds = std::max((size_t)3 * F3_CHUNK * m, (size_t)4 * a.npw * m) * sizeof(double);
+    const bool dip = a.dipw != nullptr;
+    size_t lds = std::max((size_t)(dip ? 6 : 3) * F3_CHUNK * m + (dip ? 4 * F3_CHUNK : 0),
+                          (size_t)4 * a.npw * m) * sizeof(double);
     kmark(KID_FORM_MOMENTS, false, s);
+#define FGT_F3H(M_, D_)                                                                        \
+    allow_lds3(k_form3_hermite<M_, D_>, lds);                                                  \
+    k_form3_hermite<M_, D_><<<a.grid, 256, lds, s>>>(a)
     if (m <= 32) {
-        allow_lds3(k_form3_hermite<32>, lds);
-        k_form3_hermite<32><<<a.grid, 256, lds, s>>>(a);
+        if (dip) { FGT_F3H(32, true); } else { FGT_F3H(32, false); }
     } else {
-        allow_lds3(k_form3_hermite<64>, lds);
-        k_form3_hermite<64><<<a.grid, 256, lds, s>>>(a);
+        if (dip) { FGT_F3H(64, true); } else { FGT_F3H(64, false); }
     }
+#undef FGT_F3H
     kmark(KID_FORM_MOMENTS, true, s);
 }
 
 // Step 1, direct plane-wave route (g3dformpwc_vec, g3drouts.f:1950-2040):
 // PW(a,b,c) = sum_s q ws_a ws_b ws_c e^{-i(ts_a x + ts_b y + ts_c z)}.  One
 // block per leaf; thread per (a,b), z nodes of the first half in chunks of 8.
+// With dipoles (g3dformpwd_vec / g3dformpwcd_vec, g3drouts.f:2042-2266) the
+// factor q becomes q - i dsq (w1 ts_a + w2 ts_b + w3 ts_c): ez carries
+// q - i dsq w3 ts_c, ez2 the bare z factor, and each thread adds
+// -i dsq (w1 ts_a + w2 ts_b) ex ey ez2.
 constexpr int P3_CHUNK = 16;
 
+template <bool DIP>
 __global__ __launch_bounds__(256) void k_form3_pw(Form3Args a)
 {
     extern __shared__ double sm[];
@@ -348,6 +420,8 @@ __global__ __launch_bounds__(256) void k_form3_pw(Form3Args a)
     double* ex = sm;                       // [chunk][npw] complex
     double* ey = sm + 2 * P3_CHUNK * npw;
     double* ez = sm + 4 * P3_CHUNK * npw;  // [chunk][half] complex, q folded in
+    double* ez2 = ez + 2 * P3_CHUNK * half;   // dipoles: [chunk][half] complex, bare
+    double* sw = ez2 + 2 * P3_CHUNK * half;   // dipoles: dsq w1, dsq w2 per source
     for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
         const int box = a.boxes[li];
         const int2 r = a.src_se[box];
@@ -368,16 +442,33 @@ __global__ __launch_bounds__(256) void k_form3_pw(Form3Args a)
                         const int s = o / npw, node = o % npw;
                         const long i = s0 + s;
                         const double q = a.qs[(long)a.nd * i + a.ind];
+                        const double* wv = DIP ? a.dipw + 3 * ((long)a.nd * i + a.ind) : nullptr;
                         for (int d = 0; d < 3; ++d) {
                             if (d == 2 && node >= half) continue;
                             const double u = (a.xs[3 * i + d] - c3[d]) * a.dsq;
                             double sn, cs;
                             sincos(-a.ts[node] * u, &sn, &cs);
-                            const double w = a.ws[node] * (d == 2 ? q : 1.0);
-                            double* e = d == 0 ? ex + 2 * (s * npw + node)
-                                      : d == 1 ? ey + 2 * (s * npw + node) : ez + 2 * (s * half + node);
-                            e[0] = cs * w;
-                            e[1] = sn * w;
+                            if (d < 2) {
+                                double* e = (d == 0 ? ex : ey) + 2 * (s * npw + node);
+                                e[0] = cs * a.ws[node];
+                                e[1] = sn * a.ws[node];
+                            } else if (!DIP) {
+                                const double w = a.ws[node] * q;
+                                ez[2 * (s * half + node)] = cs * w;
+                                ez[2 * (s * half + node) + 1] = sn * w;
+                            } else {
+                                // (cs + i sn) ws (q - i g), g = dsq w3 ts_c
+                                const double er = cs * a.ws[node], ei = sn * a.ws[node];
+                                const double g = a.dsq * wv[2] * a.ts[node];
+                                ez[2 * (s * half + node)] = er * q + ei * g;
+                                ez[2 * (s * half + node) + 1] = ei * q - er * g;
+                                ez2[2 * (s * half + node)] = er;
+                                ez2[2 * (s * half + node) + 1] = ei;
+                            }
+                        }
+                        if (DIP && node == 0) {
+                            sw[2 * s] = a.dsq * wv[0];
+                            sw[2 * s + 1] = a.dsq * wv[1];
                         }
                     }
                     __syncthreads();
@@ -386,6 +477,13 @@ __global__ __launch_bounds__(256) void k_form3_pw(Form3Args a)
                             const double xr = ex[2 * (s * npw + aa)], xi = ex[2 * (s * npw + aa) + 1];
                             const double yr = ey[2 * (s * npw + b)], yi = ey[2 * (s * npw + b) + 1];
                             const double tr = xr * yr - xi * yi, ti = xr * yi + xi * yr;
+                            // dipoles: tg = t (-i gam), gam = dsq (w1 ts_a + w2 ts_b)
+                            double gr = 0, gi = 0;
+                            if (DIP) {
+                                const double gam = sw[2 * s] * a.ts[aa] + sw[2 * s + 1] * a.ts[b];
+                                gr = ti * gam;
+                                gi = -tr * gam;
+                            }
 #pragma unroll
                             for (int cc = 0; cc < 8; ++cc)
                                 if (c0 + cc < half) {
@@ -393,6 +491,12 @@ __global__ __launch_bounds__(256) void k_form3_pw(Form3Args a)
                                                  zi = ez[2 * (s * half + c0 + cc) + 1];
                                     vr[cc] += tr * zr - ti * zi;
                                     vi[cc] += tr * zi + ti * zr;
+                                    if (DIP) {
+                                        const double z2r = ez2[2 * (s * half + c0 + cc)],
+                                                     z2i = ez2[2 * (s * half + c0 + cc) + 1];
+                                        vr[cc] += gr * z2r - gi * z2i;
+                                        vi[cc] += gr * z2i + gi * z2r;
+                                    }
                                 }
                         }
                 }
@@ -412,9 +516,17 @@ __global__ __launch_bounds__(256) void k_form3_pw(Form3Args a)
 void launch_form3_pw(const Form3Args& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
-    size_t lds = (size_t)2 * P3_CHUNK * (2 * a.npw + a.npw / 2) * sizeof(double);
+    const bool dip = a.dipw != nullptr;
+    size_t lds = ((size_t)2 * P3_CHUNK * (2 * a.npw + (dip ? 2 : 1) * (a.npw / 2)) +
+                  (dip ? 2 * P3_CHUNK : 0)) * sizeof(double);
     kmark(KID_FORM_VALU, false, s);
-    k_form3_pw<<<a.grid, 256, lds, s>>>(a);
+    if (dip) {
+        allow_lds3(k_form3_pw<true>, lds);
+        k_form3_pw<true><<<a.grid, 256, lds, s>>>(a);
+    } else {
+        allow_lds3(k_form3_pw<false>, lds);
+        k_form3_pw<false><<<a.grid, 256, lds, s>>>(a);
+    }
     kmark(KID_FORM_VALU, true, s);
 }
 
@@ -901,10 +1013,43 @@ void launch_pw_eval3(const Eval3Args& a, hipStream_t s)
 // fgt3d.f:1460-1512): one block per target unit, its source boxes staged
 // in LDS; pairs with r^2 < dmax; accumulated.
 // ---------------------------------------------------------------------------
-template <int PGH>
+// One source's contribution at a target, r = t - s, e = exp(-|r|^2/delta),
+// u = -2 r/delta, rfac = -2/delta: a charge q and a dipole w = d n
+// (g3d_directd*_vec, g3ddirect.f:207-435) give A = q - w . u, the sum e A,
+// its gradient e (u A - rfac w) and Hessian e ((u_a u_b + rfac d_ab) A -
+// rfac (u_a w_b + u_b w_a)), ordered xx, yy, zz, xy, xz, yz.
+template <int PGH, bool DIP>
+__device__ __forceinline__ void pair_add3(double dx, double dy, double dz, double q,
+                                          const double* w, double inv, double rfac, double dmax,
+                                          double& p, double& gx, double& gy, double& gz,
+                                          double* h)
+{
+    const double rr = dx * dx + dy * dy + dz * dz;
+    if (!(rr < dmax)) return;
+    const double e = exp(-rr * inv);
+    const double ux = rfac * dx, uy = rfac * dy, uz = rfac * dz;
+    const double w1 = DIP ? w[0] : 0.0, w2 = DIP ? w[1] : 0.0, w3 = DIP ? w[2] : 0.0;
+    const double A = DIP ? q - (w1 * ux + w2 * uy + w3 * uz) : q;
+    p += e * A;
+    if (PGH >= 2) {
+        gx += e * (ux * A - rfac * w1);
+        gy += e * (uy * A - rfac * w2);
+        gz += e * (uz * A - rfac * w3);
+        if (PGH >= 3) {
+            h[0] += e * ((rfac + ux * ux) * A - 2.0 * rfac * ux * w1);
+            h[1] += e * ((rfac + uy * uy) * A - 2.0 * rfac * uy * w2);
+            h[2] += e * ((rfac + uz * uz) * A - 2.0 * rfac * uz * w3);
+            h[3] += e * (ux * uy * A - rfac * (ux * w2 + uy * w1));
+            h[4] += e * (ux * uz * A - rfac * (ux * w3 + uz * w1));
+            h[5] += e * (uy * uz * A - rfac * (uy * w3 + uz * w2));
+        }
+    }
+}
+
+template <int PGH, bool DIP>
 __global__ __launch_bounds__(256) void k_direct3(DirectArgs a)
 {
-    __shared__ double sx[256], sy[256], sz[256], sq[256];
+    __shared__ double sx[256], sy[256], sz[256], sq[256], sw[DIP ? 3 * 256 : 1];
     const int u = a.units[blockIdx.x];
     const int2 tr = a.tgt_se[u];
     const int l0 = a.list_ptr[blockIdx.x], l1 = a.list_ptr[blockIdx.x + 1];
@@ -928,30 +1073,14 @@ __global__ __launch_bounds__(256) void k_direct3(DirectArgs a)
                         sy[t] = a.xs[3 * j + 1];
                         sz[t] = a.xs[3 * j + 2];
                         sq[t] = a.qs[j * a.nd + ind];
+                        if (DIP)
+                            for (int c = 0; c < 3; ++c) sw[3 * t + c] = a.ws[3 * (j * a.nd + ind) + c];
                     }
                     __syncthreads();
-                    for (int s = 0; s < cn; ++s) {
-                        const double dx = x - sx[s], dy = y - sy[s], dz = z - sz[s];
-                        const double rr = dx * dx + dy * dy + dz * dz;
-                        if (rr < a.dmax) {
-                            const double e = exp(-rr * inv) * sq[s];
-                            p += e;
-                            if (PGH >= 2) {
-                                const double ux = rfac * dx, uy = rfac * dy, uz = rfac * dz;
-                                gx += ux * e;
-                                gy += uy * e;
-                                gz += uz * e;
-                                if (PGH >= 3) {
-                                    h[0] += (rfac + ux * ux) * e;
-                                    h[1] += (rfac + uy * uy) * e;
-                                    h[2] += (rfac + uz * uz) * e;
-                                    h[3] += ux * uy * e;
-                                    h[4] += ux * uz * e;
-                                    h[5] += uy * uz * e;
-                                }
-                            }
-                        }
-                    }
+                    for (int s = 0; s < cn; ++s)
+                        pair_add3<PGH, DIP>(x - sx[s], y - sy[s], z - sz[s], sq[s],
+                                            DIP ? sw + 3 * s : nullptr, inv, rfac, a.dmax, p, gx,
+                                            gy, gz, h);
                 }
             }
             if (act) {
@@ -974,27 +1103,29 @@ void launch_direct3(const DirectArgs& a, hipStream_t s)
 {
     if (a.nunits <= 0) return;
     kmark(KID_DIRECT, false, s);
+    const bool dip = a.ws != nullptr;
     if (a.ifpgh >= 3)
-        k_direct3<3><<<a.nunits, 256, 0, s>>>(a);
+        dip ? k_direct3<3, true><<<a.nunits, 256, 0, s>>>(a) : k_direct3<3, false><<<a.nunits, 256, 0, s>>>(a);
     else if (a.ifpgh == 2)
-        k_direct3<2><<<a.nunits, 256, 0, s>>>(a);
+        dip ? k_direct3<2, true><<<a.nunits, 256, 0, s>>>(a) : k_direct3<2, false><<<a.nunits, 256, 0, s>>>(a);
     else
-        k_direct3<1><<<a.nunits, 256, 0, s>>>(a);
+        dip ? k_direct3<1, true><<<a.nunits, 256, 0, s>>>(a) : k_direct3<1, false><<<a.nunits, 256, 0, s>>>(a);
     kmark(KID_DIRECT, true, s);
 }
 
 // all-pairs 3D direct sum with a cutoff (fgt3dpart_direct_vec,
 // fgt3d.f:1533-1684), accumulating; thread per target, sources through LDS
-template <int PGH>
+template <int PGH, bool DIP>
 __global__ __launch_bounds__(256) void k_direct_all3(const double* __restrict__ src,
-                                                     const double* __restrict__ q, int nd, int ns,
-                                                     const double* __restrict__ tg, int nt,
+                                                     const double* __restrict__ q,
+                                                     const double* __restrict__ ws, int nd,
+                                                     int ns, const double* __restrict__ tg, int nt,
                                                      double delta, double dmax,
                                                      double* __restrict__ pot,
                                                      double* __restrict__ grad,
                                                      double* __restrict__ hess)
 {
-    __shared__ double sx[256], sy[256], sz[256], sq[256];
+    __shared__ double sx[256], sy[256], sz[256], sq[256], sw[DIP ? 3 * 256 : 1];
     const int t = threadIdx.x, i = blockIdx.x * 256 + t;
     const bool act = i < nt;
     const double x = act ? tg[3 * (long)i] : 0, y = act ? tg[3 * (long)i + 1] : 0,
@@ -1010,31 +1141,14 @@ __global__ __launch_bounds__(256) void k_direct_all3(const double* __restrict__ 
                 sx[t] = src[3 * j];
                 sy[t] = src[3 * j + 1];
                 sz[t] = src[3 * j + 2];
-                sq[t] = q[j * nd + ind];
+                sq[t] = q ? q[j * nd + ind] : 0.0;
+                if (DIP)
+                    for (int c = 0; c < 3; ++c) sw[3 * t + c] = ws[3 * (j * nd + ind) + c];
             }
             __syncthreads();
-            for (int s = 0; s < cn; ++s) {
-                const double dx = x - sx[s], dy = y - sy[s], dz = z - sz[s];
-                const double rr = dx * dx + dy * dy + dz * dz;
-                if (rr < dmax) {
-                    const double e = exp(-rr * inv) * sq[s];
-                    p += e;
-                    if (PGH >= 2) {
-                        const double ux = rfac * dx, uy = rfac * dy, uz = rfac * dz;
-                        gx += ux * e;
-                        gy += uy * e;
-                        gz += uz * e;
-                        if (PGH >= 3) {
-                            h[0] += (rfac + ux * ux) * e;
-                            h[1] += (rfac + uy * uy) * e;
-                            h[2] += (rfac + uz * uz) * e;
-                            h[3] += ux * uy * e;
-                            h[4] += ux * uz * e;
-                            h[5] += uy * uz * e;
-                        }
-                    }
-                }
-            }
+            for (int s = 0; s < cn; ++s)
+                pair_add3<PGH, DIP>(x - sx[s], y - sy[s], z - sz[s], sq[s], DIP ? sw + 3 * s : nullptr,
+                                    inv, rfac, dmax, p, gx, gy, gz, h);
         }
         if (act) {
             pot[(long)i * nd + ind] += p;
@@ -1052,18 +1166,22 @@ __global__ __launch_bounds__(256) void k_direct_all3(const double* __restrict__ 
     }
 }
 
-void launch_direct_all3(const double* src, const double* q, int nd, int ns, const double* tg,
-                        int nt, double delta, double dmax, int ifpgh, double* pot, double* grad,
-                        double* hess, hipStream_t s)
+void launch_direct_all3(const double* src, const double* q, const double* ws, int nd, int ns,
+                        const double* tg, int nt, double delta, double dmax, int ifpgh,
+                        double* pot, double* grad, double* hess, hipStream_t s)
 {
     if (nt <= 0 || ns <= 0) return;
     const unsigned g = gridn(nt, 256);
-    if (ifpgh >= 3)
-        k_direct_all3<3><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
-    else if (ifpgh == 2)
-        k_direct_all3<2><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
-    else
-        k_direct_all3<1><<<g, 256, 0, s>>>(src, q, nd, ns, tg, nt, delta, dmax, pot, grad, hess);
+#define FGT_DALL3(P_, D_)                                                                      \
+    k_direct_all3<P_, D_><<<g, 256, 0, s>>>(src, q, ws, nd, ns, tg, nt, delta, dmax, pot, grad, hess)
+    if (ifpgh >= 3) {
+        if (ws) FGT_DALL3(3, true); else FGT_DALL3(3, false);
+    } else if (ifpgh == 2) {
+        if (ws) FGT_DALL3(2, true); else FGT_DALL3(2, false);
+    } else {
+        if (ws) FGT_DALL3(1, true); else FGT_DALL3(1, false);
+    }
+#undef FGT_DALL3
 }
 
 // ---------------------------------------------------------------------------
@@ -1074,9 +1192,12 @@ void launch_direct_all3(const double* src, const double* q, int nd, int ns, cons
 // the z offset used for z).  Partial sums per block in a fixed order, then a
 // fixed-order reduction: deterministic.
 // ---------------------------------------------------------------------------
-template <int MM>
+// Dipoles (the intended g3dformld_vec / g3dformlcd_vec, g3drouts.f:857-1040)
+// add w . (dh_x h_y h_z, h_x dh_y h_z, h_x h_y dh_z), dh_j = -dsq (j+1) h_{j+1}.
+template <int MM, bool DIP>
 __global__ __launch_bounds__(256) void k_formlc3_part(const double* __restrict__ p,
-                                                      const double* __restrict__ q, int nd,
+                                                      const double* __restrict__ q,
+                                                      const double* __restrict__ ws, int nd,
                                                       int ind, int n, double cx, double cy,
                                                       double cz, double dsq, int nl, int nparts,
                                                       double* __restrict__ part)
@@ -1087,7 +1208,11 @@ __global__ __launch_bounds__(256) void k_formlc3_part(const double* __restrict__
     const long s0 = blockIdx.x * per, s1 = min((long)n, s0 + per);
     double* hx = sm;                 // [32][m]
     double* hy = sm + 32 * m;
-    double* hz = sm + 64 * m;        // q folded in
+    double* hz = sm + 64 * m;        // q folded in (without dipoles)
+    double* dhx = sm + 96 * m;       // dipoles: dh_j per axis
+    double* dhy = sm + 128 * m;
+    double* dhz = sm + 160 * m;
+    double* sw = sm + 192 * m;       // dipoles: q, w1, w2, w3 per source
     double* out = part + (long)blockIdx.x * m * m * m;
     for (int p0 = 0; p0 < m * m; p0 += 256) {
         const int pr = p0 + t, j2 = pr % m, j1 = pr / m;
@@ -1102,24 +1227,46 @@ __global__ __launch_bounds__(256) void k_formlc3_part(const double* __restrict__
                 const long i = c0 + t;
                 const double c3[3] = {cx, cy, cz};
                 double* hh[3] = {hx + t * m, hy + t * m, hz + t * m};
+                double* dh[3] = {dhx + t * m, dhy + t * m, dhz + t * m};
                 for (int d = 0; d < 3; ++d) {
                     const double u = (p[3 * i + d] - c3[d]) * dsq;
                     const double f = exp(-u * u);
                     double* H = hh[d];
-                    H[0] = f;
-                    if (m > 1) H[1] = 2.0 * u * f;
-                    for (int j = 1; j < nl; ++j) H[j + 1] = 2.0 * (u * H[j] - H[j - 1]) / (j + 1);
+                    // h_j for j < m; with dipoles one more for dh_{m-1}
+                    double h0 = f, h1 = 2.0 * u * f;
+                    for (int j = 0; j < m; ++j) {
+                        H[j] = h0;
+                        if (DIP) dh[d][j] = -(h1 * dsq * (j + 1.0));
+                        const double h2 = 2.0 * (u * h1 - h0) / (j + 2);
+                        h0 = h1;
+                        h1 = h2;
+                    }
                 }
-                const double qq = q[(long)nd * i + ind];
-                for (int j = 0; j < m; ++j) hz[t * m + j] *= qq;
+                const double qq = q ? q[(long)nd * i + ind] : 0.0;
+                if (!DIP) {
+                    for (int j = 0; j < m; ++j) hz[t * m + j] *= qq;
+                } else {
+                    sw[4 * t] = qq;
+                    for (int c = 0; c < 3; ++c) sw[4 * t + 1 + c] = ws[3 * ((long)nd * i + ind) + c];
+                }
             }
             __syncthreads();
             if (act)
                 for (int s = 0; s < cn; ++s) {
-                    const double w = hz[s * m + j1] * hy[s * m + j2];
+                    if (!DIP) {
+                        const double w = hz[s * m + j1] * hy[s * m + j2];
 #pragma unroll
-                    for (int j = 0; j < MM; ++j)
-                        if (j < m) acc[j] += w * hx[s * m + j];
+                        for (int j = 0; j < MM; ++j)
+                            if (j < m) acc[j] += w * hx[s * m + j];
+                    } else {
+                        const double yv = hy[s * m + j2], zv = hz[s * m + j1];
+                        const double F = sw[4 * s] * yv * zv + sw[4 * s + 2] * dhy[s * m + j2] * zv +
+                                         sw[4 * s + 3] * yv * dhz[s * m + j1];
+                        const double G = sw[4 * s + 1] * yv * zv;
+#pragma unroll
+                        for (int j = 0; j < MM; ++j)
+                            if (j < m) acc[j] += F * hx[s * m + j] + G * dhx[s * m + j];
+                    }
                 }
         }
         if (act)
@@ -1139,18 +1286,23 @@ __global__ __launch_bounds__(256) void k_sum_parts3(const double* __restrict__ p
     out[o] = v;
 }
 
-void launch_formlc3(const double* p, const double* q, int nd, int ind, int n, const double* c,
-                    double dsq, int nl, double* part, int nparts, double* out, hipStream_t s)
+void launch_formlc3(const double* p, const double* q, const double* ws, int nd, int ind, int n,
+                    const double* c, double dsq, int nl, double* part, int nparts, double* out,
+                    hipStream_t s)
 {
     const int m = nl + 1;
     if (m > L3_MAXM) fail(-3, "3D Taylor order above 63 is not supported");
-    size_t lds = (size_t)3 * 32 * m * sizeof(double);
-    if (m <= 32)
-        k_formlc3_part<32><<<nparts, 256, lds, s>>>(p, q, nd, ind, n, c[0], c[1], c[2], dsq, nl,
-                                                    nparts, part);
-    else
-        k_formlc3_part<64><<<nparts, 256, lds, s>>>(p, q, nd, ind, n, c[0], c[1], c[2], dsq, nl,
-                                                    nparts, part);
+    size_t lds = ((size_t)(ws ? 6 : 3) * 32 * m + (ws ? 4 * 32 : 0)) * sizeof(double);
+#define FGT_LC3(M_, D_)                                                                        \
+    allow_lds3(k_formlc3_part<M_, D_>, lds);                                                   \
+    k_formlc3_part<M_, D_><<<nparts, 256, lds, s>>>(p, q, ws, nd, ind, n, c[0], c[1], c[2], dsq, \
+                                                    nl, nparts, part)
+    if (m <= 32) {
+        if (ws) { FGT_LC3(32, true); } else { FGT_LC3(32, false); }
+    } else {
+        if (ws) { FGT_LC3(64, true); } else { FGT_LC3(64, false); }
+    }
+#undef FGT_LC3
     const long len = (long)m * m * m;
     k_sum_parts3<<<gridn(len, 256), 256, 0, s>>>(part, nparts, len, out);
 }

@@ -481,32 +481,55 @@ void orc_tree3d_free(void *h)
 
 /* g3dformhc_vec, g3drouts.f:109-195: H(j,k,l) += q x^j/sqrt(j!) y^k/sqrt(k!)
  * z^l/sqrt(l!) (each power divided by sqrt(k) step by step). */
+/* g3dformhc_vec / g3dformhd_vec / g3dformhcd_vec, g3drouts.f:109-409: with
+ * dipoles (rn != NULL) H += dxhx_j hy_k hz_l w1 + hx_j (dyhy_k hz_l w2 +
+ * hy_k (dzhz_l w3 + q hz_l)), dxhx_j = j dsq hx_{j-1}/sqrt(j), in the
+ * reference's order of operations.  chg == NULL: no charges. */
 static void o3_formhc(int nd, double delta, const double *src, int ns, const double *chg,
-                      const double *c, int nt, double *h)
+                      const double *rn, const double *ds, const double *c, int nt, double *h)
 {
     int m = nt + 1;
     size_t m3 = (size_t)m * m * m;
     memset(h, 0, sizeof(double) * m3 * nd);
-    double xp[301], yp[301], zp[301], sqk[301];
+    double xp[301], yp[301], zp[301], sqk[301], dx[301], dy[301], dz[301];
     double dsq = 1.0 / sqrt(delta);
     for (int k = 1; k <= nt; k++) sqk[k] = sqrt((double)k);
     for (int i = 0; i < ns; i++) {
         double x = (src[3 * i] - c[0]) * dsq, y = (src[3 * i + 1] - c[1]) * dsq,
                z = (src[3 * i + 2] - c[2]) * dsq;
         xp[0] = yp[0] = zp[0] = 1.0;
+        dx[0] = dy[0] = dz[0] = 0.0;
         for (int k = 1; k <= nt; k++) {
             xp[k] = xp[k - 1] * x / sqk[k];
             yp[k] = yp[k - 1] * y / sqk[k];
             zp[k] = zp[k - 1] * z / sqk[k];
+            dx[k] = k * dsq * xp[k - 1] / sqk[k];
+            dy[k] = k * dsq * yp[k - 1] / sqk[k];
+            dz[k] = k * dsq * zp[k - 1] / sqk[k];
         }
         for (int ind = 0; ind < nd; ind++) {
-            double q = chg[(size_t)nd * i + ind];
             double *hh = h + (size_t)ind * m3;
+            if (!rn) {
+                double q = chg[(size_t)nd * i + ind];
+                for (int l = 0; l <= nt; l++) {
+                    double zt = q * zp[l];
+                    for (int k = 0; k <= nt; k++) {
+                        double rt = zt * yp[k];
+                        for (int j = 0; j <= nt; j++) hh[j + m * (k + (size_t)m * l)] += rt * xp[j];
+                    }
+                }
+                continue;
+            }
+            double dd = ds[(size_t)nd * i + ind];
+            double r1 = rn[3 * i] * dd, r2 = rn[3 * i + 1] * dd, r3 = rn[3 * i + 2] * dd;
             for (int l = 0; l <= nt; l++) {
-                double zt = q * zp[l];
+                double d1 = zp[l] * r1, d2 = zp[l] * r2, d3 = dz[l] * r3;
+                double zt = chg ? chg[(size_t)nd * i + ind] * zp[l] : 0.0;
                 for (int k = 0; k <= nt; k++) {
-                    double rt = zt * yp[k];
-                    for (int j = 0; j <= nt; j++) hh[j + m * (k + (size_t)m * l)] += rt * xp[j];
+                    double c1 = yp[k] * d1;
+                    double c2 = chg ? dy[k] * d2 + yp[k] * (d3 + zt) : dy[k] * d2 + yp[k] * d3;
+                    double *hr = hh + m * (k + (size_t)m * l);
+                    for (int j = 0; j <= nt; j++) hr[j] = hr[j] + dx[j] * c1 + xp[j] * c2;
                 }
             }
         }
@@ -605,12 +628,22 @@ static void o3_h2pw(int nd, int nt, int npw, const double *h2x, const double *h,
 /* g3dformpwc_vec, g3drouts.f:1950-2040: PW(j1,j2,j3) += q ws ws ws
  * e^{-i ts x} e^{-i ts y} e^{-i ts z}, phases by recurrence from the first
  * positive node, negative nodes as conjugates. */
+static inline void o3_cmul(double ar, double ai, double br, double bi, double *r, double *i)
+{
+    *r = ar * br - ai * bi;
+    *i = ar * bi + ai * br;
+}
+
+/* with dipoles: g3dformpwd_vec / g3dformpwcd_vec, g3drouts.f:2042-2266,
+ * PW += ww1x ry1 + ww1 ry2 with wwx = ww (-i ts dsq), ry1 = ww3 w1 ww2,
+ * ry2 = ww3 w2 ww2y + (ww3z w3 + ww3 q) ww2, in the reference's order. */
 static void o3_formpwc(int nd, double delta, const double *src, int ns, const double *chg,
-                       const double *c, int npw, const double *ws, const double *ts, double *pw)
+                       const double *rn, const double *ds, const double *c, int npw,
+                       const double *ws, const double *ts, double *pw)
 {
     int npw2 = npw / 2;
     double dsq = 1.0 / sqrt(delta);
-    double w[3][100][2];
+    double w[3][100][2], wd[3][100][2];
     for (int i = 0; i < ns; i++) {
         double u[3];
         for (int d = 0; d < 3; d++) u[d] = (src[3 * i + d] - c[d]) * dsq;
@@ -626,19 +659,56 @@ static void o3_formpwc(int nd, double delta, const double *src, int ns, const do
                 w[d][npw - 1 - j][0] = w[d][j][0];
                 w[d][npw - 1 - j][1] = -w[d][j][1];
             }
+            if (rn)
+                for (int j = 0; j < npw; j++) {
+                    /* ztmp = -i ts dsq; ww * ztmp with ztmp = (-0, -(ts dsq)) */
+                    double zi = -(ts[j] * dsq);
+                    wd[d][j][0] = -(w[d][j][1] * zi);
+                    wd[d][j][1] = w[d][j][0] * zi;
+                }
         }
         for (int ind = 0; ind < nd; ind++) {
-            double q = chg[(size_t)nd * i + ind];
             double *p = pw + (size_t)ind * 2 * npw * npw * npw2;
+            if (!rn) {
+                double q = chg[(size_t)nd * i + ind];
+                for (int j3 = 0; j3 < npw2; j3++) {
+                    double zr = q * w[2][j3][0], zi = q * w[2][j3][1];
+                    for (int j2 = 0; j2 < npw; j2++) {
+                        double yr = zr * w[1][j2][0] - zi * w[1][j2][1];
+                        double yi = zr * w[1][j2][1] + zi * w[1][j2][0];
+                        for (int j1 = 0; j1 < npw; j1++) {
+                            double *d = p + 2 * (j1 + (size_t)npw * (j2 + (size_t)npw * j3));
+                            d[0] += yr * w[0][j1][0] - yi * w[0][j1][1];
+                            d[1] += yr * w[0][j1][1] + yi * w[0][j1][0];
+                        }
+                    }
+                }
+                continue;
+            }
+            double dd = ds[(size_t)nd * i + ind];
+            double r1 = rn[3 * i] * dd, r2 = rn[3 * i + 1] * dd, r3 = rn[3 * i + 2] * dd;
+            double q = chg ? chg[(size_t)nd * i + ind] : 0.0;
             for (int j3 = 0; j3 < npw2; j3++) {
-                double zr = q * w[2][j3][0], zi = q * w[2][j3][1];
+                double z1r = w[2][j3][0] * r1, z1i = w[2][j3][1] * r1;
+                double z2r = w[2][j3][0] * r2, z2i = w[2][j3][1] * r2;
+                double z3r = wd[2][j3][0] * r3, z3i = wd[2][j3][1] * r3;
+                if (chg) {  /* rz3 + ztmp, ztmp = ww3 q */
+                    z3r = z3r + w[2][j3][0] * q;
+                    z3i = z3i + w[2][j3][1] * q;
+                }
                 for (int j2 = 0; j2 < npw; j2++) {
-                    double yr = zr * w[1][j2][0] - zi * w[1][j2][1];
-                    double yi = zr * w[1][j2][1] + zi * w[1][j2][0];
+                    double y1r, y1i, ar, ai, br, bi;
+                    o3_cmul(z1r, z1i, w[1][j2][0], w[1][j2][1], &y1r, &y1i);
+                    o3_cmul(z2r, z2i, wd[1][j2][0], wd[1][j2][1], &ar, &ai);
+                    o3_cmul(z3r, z3i, w[1][j2][0], w[1][j2][1], &br, &bi);
+                    double y2r = ar + br, y2i = ai + bi;
                     for (int j1 = 0; j1 < npw; j1++) {
                         double *d = p + 2 * (j1 + (size_t)npw * (j2 + (size_t)npw * j3));
-                        d[0] += yr * w[0][j1][0] - yi * w[0][j1][1];
-                        d[1] += yr * w[0][j1][1] + yi * w[0][j1][0];
+                        double t1r, t1i, t2r, t2i;
+                        o3_cmul(wd[0][j1][0], wd[0][j1][1], y1r, y1i, &t1r, &t1i);
+                        o3_cmul(w[0][j1][0], w[0][j1][1], y2r, y2i, &t2r, &t2i);
+                        d[0] = d[0] + t1r + t2r;
+                        d[1] = d[1] + t1i + t2i;
                     }
                 }
             }
@@ -894,35 +964,127 @@ static void o3_pweval(int nd, double delta, const double *c, int npw, const doub
 }
 
 /* g3d_directcp/cg/ch_vec, g3ddirect.f:3-206: pairs with r^2 < dmax,
- * pot += q e^{-r^2/delta}, grad = -2/delta d * (.), hess likewise. */
+ * pot += q e^{-r^2/delta}, grad = -2/delta d * (.), hess likewise.  With
+ * dipoles (rn != NULL) the six variants g3d_directdp/dg/dh and
+ * g3d_directcdp/cdg/cdh, g3ddirect.f:207-700, in their order of operations;
+ * Hessians ordered xx, yy, zz, xy, xz, yz. */
 static void o3_direct(int nd, double delta, double dmax, const double *src, int ns,
-                      const double *chg, const double *tg, int nt, int ifpgh, double *pot,
-                      double *grad, double *hess)
+                      const double *chg, const double *rn, const double *ds, const double *tg,
+                      int nt, int ifpgh, double *pot, double *grad, double *hess)
 {
-    double rfac = -2.0 / delta;
     for (int it = 0; it < nt; it++)
         for (int i = 0; i < ns; i++) {
             double xd = tg[3 * it] - src[3 * i], yd = tg[3 * it + 1] - src[3 * i + 1],
                    zd = tg[3 * it + 2] - src[3 * i + 2];
             double rr = xd * xd + yd * yd + zd * zd;
             if (!(rr < dmax)) continue;
-            double rt = exp(-rr / delta);
-            double dx = rfac * xd, dy = rfac * yd, dz = rfac * zd;
-            for (int ii = 0; ii < nd; ii++) {
-                double rinc = rt * chg[(size_t)nd * i + ii];
-                pot[(size_t)nd * it + ii] += rinc;
-                if (ifpgh >= 2) {
-                    double *g = grad + (size_t)3 * nd * it;
-                    g[ii] += dx * rinc; g[nd + ii] += dy * rinc; g[2 * nd + ii] += dz * rinc;
+            double *P = pot + (size_t)nd * it, *G = grad + (size_t)3 * nd * it,
+                   *H = hess + (size_t)6 * nd * it;
+            const double *q = chg ? chg + (size_t)nd * i : NULL;
+            const double *d = rn ? ds + (size_t)nd * i : NULL;
+            double n1 = rn ? rn[3 * i] : 0, n2 = rn ? rn[3 * i + 1] : 0, n3 = rn ? rn[3 * i + 2] : 0;
+            if (!rn) {
+                double rfac = -2.0 / delta;
+                double rt = exp(-rr / delta);
+                double dx = rfac * xd, dy = rfac * yd, dz = rfac * zd;
+                for (int ii = 0; ii < nd; ii++) {
+                    double rinc = rt * q[ii];
+                    P[ii] += rinc;
+                    if (ifpgh >= 2) { G[ii] += dx * rinc; G[nd + ii] += dy * rinc; G[2 * nd + ii] += dz * rinc; }
+                    if (ifpgh >= 3) {
+                        H[ii] += (rfac + dx * dx) * rinc;
+                        H[nd + ii] += (rfac + dy * dy) * rinc;
+                        H[2 * nd + ii] += (rfac + dz * dz) * rinc;
+                        H[3 * nd + ii] += dx * dy * rinc;
+                        H[4 * nd + ii] += dx * dz * rinc;
+                        H[5 * nd + ii] += dy * dz * rinc;
+                    }
                 }
-                if (ifpgh >= 3) {
-                    double *h = hess + (size_t)6 * nd * it;
-                    h[ii] += (rfac + dx * dx) * rinc;
-                    h[nd + ii] += (rfac + dy * dy) * rinc;
-                    h[2 * nd + ii] += (rfac + dz * dz) * rinc;
-                    h[3 * nd + ii] += dx * dy * rinc;
-                    h[4 * nd + ii] += dx * dz * rinc;
-                    h[5 * nd + ii] += dy * dz * rinc;
+                continue;
+            }
+            if (!q && ifpgh == 1) { /* g3d_directdp_vec */
+                double rfac = 2.0 / delta;
+                double rtmp = exp(-rr / delta);
+                double dx = rfac * xd, dy = rfac * yd, dz = rfac * zd;
+                double rinc = (dx * n1 + dy * n2 + dz * n3) * rtmp;
+                for (int ii = 0; ii < nd; ii++) P[ii] = P[ii] + rinc * d[ii];
+            } else if (!q) { /* g3d_directdg_vec / g3d_directdh_vec */
+                double rfac = 2.0 / delta;
+                double dtmp = exp(-rr / delta);
+                double dx = rfac * xd, dy = rfac * yd, dz = rfac * zd;
+                double rinc = (dx * n1 + dy * n2 + dz * n3);
+                double rx = n1 * rfac - rinc * dx, ry = n2 * rfac - rinc * dy, rz = n3 * rfac - rinc * dz;
+                double rxx = -(rfac * (dx * n1 + rinc)) - rx * dx;
+                double ryy = -(rfac * (dy * n2 + rinc)) - ry * dy;
+                double rzz = -(rfac * (dz * n3 + rinc)) - rz * dz;
+                double rxy = -(rfac * dx * n2) - rx * dy;
+                double rxz = -(rfac * dx * n3) - rx * dz;
+                double ryz = -(rfac * dy * n3) - ry * dz;
+                for (int ii = 0; ii < nd; ii++) {
+                    double rt = dtmp * d[ii];
+                    P[ii] = P[ii] + rinc * rt;
+                    G[ii] = G[ii] + rx * rt;
+                    G[nd + ii] = G[nd + ii] + ry * rt;
+                    G[2 * nd + ii] = G[2 * nd + ii] + rz * rt;
+                    if (ifpgh >= 3) {
+                        H[ii] = H[ii] + rxx * rt;
+                        H[nd + ii] = H[nd + ii] + ryy * rt;
+                        H[2 * nd + ii] = H[2 * nd + ii] + rzz * rt;
+                        H[3 * nd + ii] = H[3 * nd + ii] + rxy * rt;
+                        H[4 * nd + ii] = H[4 * nd + ii] + rxz * rt;
+                        H[5 * nd + ii] = H[5 * nd + ii] + ryz * rt;
+                    }
+                }
+            } else if (ifpgh == 1) { /* g3d_directcdp_vec */
+                double rfac = 2.0 / delta;
+                double rtmp = exp(-rr / delta);
+                for (int ii = 0; ii < nd; ii++) P[ii] = P[ii] + rtmp * q[ii];
+                double dx = rfac * xd, dy = rfac * yd, dz = rfac * zd;
+                double rinc = (dx * n1 + dy * n2 + dz * n3) * rtmp;
+                for (int ii = 0; ii < nd; ii++) P[ii] = P[ii] + rinc * d[ii];
+            } else { /* g3d_directcdg_vec / g3d_directcdh_vec */
+                double rfac = -2.0 / delta;
+                double dtmp = exp(-rr / delta);
+                double dx = rfac * xd, dy = rfac * yd, dz = rfac * zd;
+                double dxx = rfac + dx * dx, dyy = rfac + dy * dy, dzz = rfac + dz * dz;
+                double dxy = dx * dy, dxz = dx * dz, dyz = dy * dz;
+                for (int ii = 0; ii < nd; ii++) {
+                    double rinc = dtmp * q[ii];
+                    P[ii] = P[ii] + rinc;
+                    G[ii] = G[ii] + dx * rinc;
+                    G[nd + ii] = G[nd + ii] + dy * rinc;
+                    G[2 * nd + ii] = G[2 * nd + ii] + dz * rinc;
+                    if (ifpgh >= 3) {
+                        H[ii] = H[ii] + dxx * rinc;
+                        H[nd + ii] = H[nd + ii] + dyy * rinc;
+                        H[2 * nd + ii] = H[2 * nd + ii] + dzz * rinc;
+                        H[3 * nd + ii] = H[3 * nd + ii] + dxy * rinc;
+                        H[4 * nd + ii] = H[4 * nd + ii] + dxz * rinc;
+                        H[5 * nd + ii] = H[5 * nd + ii] + dyz * rinc;
+                    }
+                }
+                double rinc = -(dx * n1) - dy * n2 - dz * n3;
+                double rx = rinc * dx - n1 * rfac, ry = rinc * dy - n2 * rfac, rz = rinc * dz - n3 * rfac;
+                double rxx = rfac * (rinc - dx * n1) + rx * dx;
+                double ryy = rfac * (rinc - dy * n2) + ry * dy;
+                double rzz = rfac * (rinc - dz * n3) + rz * dz;
+                double rxy = rx * dy - rfac * dx * n2;
+                double rxz = rx * dz - rfac * dx * n3;
+                double ryz = ry * dz - rfac * dy * n3;
+                for (int ii = 0; ii < nd; ii++) {
+                    double rt = dtmp * d[ii];
+                    P[ii] = P[ii] + rinc * rt;
+                    G[ii] = G[ii] + rx * rt;
+                    G[nd + ii] = G[nd + ii] + ry * rt;
+                    G[2 * nd + ii] = G[2 * nd + ii] + rz * rt;
+                    if (ifpgh >= 3) {
+                        H[ii] = H[ii] + rxx * rt;
+                        H[nd + ii] = H[nd + ii] + ryy * rt;
+                        H[2 * nd + ii] = H[2 * nd + ii] + rzz * rt;
+                        H[3 * nd + ii] = H[3 * nd + ii] + rxy * rt;
+                        H[4 * nd + ii] = H[4 * nd + ii] + rxz * rt;
+                        H[5 * nd + ii] = H[5 * nd + ii] + ryz * rt;
+                    }
                 }
             }
         }
@@ -933,39 +1095,66 @@ void orc_direct3d(int nd, double delta, double dmax, const double *src, int ns,
                   const double *chg, const double *tg, int nt, int ifpgh, double *pot,
                   double *grad, double *hess)
 {
-    o3_direct(nd, delta, dmax, src, ns, chg, tg, nt, ifpgh, pot, grad, hess);
+    o3_direct(nd, delta, dmax, src, ns, chg, NULL, NULL, tg, nt, ifpgh, pot, grad, hess);
+}
+
+/* the same with dipoles: chg == NULL no charges, rn == NULL no dipoles */
+void orc_direct3d_cd(int nd, double delta, double dmax, const double *src, int ns,
+                     const double *chg, const double *rn, const double *ds, const double *tg,
+                     int nt, int ifpgh, double *pot, double *grad, double *hess)
+{
+    o3_direct(nd, delta, dmax, src, ns, chg, rn, ds, tg, nt, ifpgh, pot, grad, hess);
 }
 
 /* the intended g3dformlc_vec (g3drouts.f:784-856): Taylor coefficients of
  * sum q e^{-|t-s|^2/delta} about c from the Hermite-function recurrence
- * h_{j+1} = 2 (u h_j - h_{j-1})/(j+1), h_0 = e^{-u^2}, per axis. */
+ * h_{j+1} = 2 (u h_j - h_{j-1})/(j+1), h_0 = e^{-u^2}, per axis.  Dipoles
+ * (the intended g3dformld_vec / g3dformlcd_vec, g3drouts.f:857-1040, whose
+ * x factors are indexed by the y order j2 where the x order j3 is meant)
+ * add w . (dh_x h_y h_z, h_x dh_y h_z, h_x h_y dh_z), dh_j = -dsq (j+1)
+ * h_{j+1}. */
 static void o3_formlc(int nd, double delta, const double *src, int ns, const double *chg,
-                      const double *c, int nl, double *loc)
+                      const double *rn, const double *ds, const double *c, int nl, double *loc)
 {
     int m = nl + 1;
     size_t m3 = (size_t)m * m * m;
-    double hx[301], hy[301], hz[301];
+    double hx[302], hy[302], hz[302], dhx[301], dhy[301], dhz[301];
     double dsq = 1.0 / sqrt(delta);
     for (int i = 0; i < ns; i++) {
         double u[3];
         for (int d = 0; d < 3; d++) u[d] = (src[3 * i + d] - c[d]) * dsq;
-        double *hh[3] = {hx, hy, hz};
+        double *hh[3] = {hx, hy, hz}, *dh[3] = {dhx, dhy, dhz};
+        int top = rn ? nl : nl - 1;
         for (int d = 0; d < 3; d++) {
             double f = exp(-u[d] * u[d]);
             hh[d][0] = 1.0 * f;
-            if (nl >= 1) hh[d][1] = 2.0 * u[d] * f;
-            for (int j = 1; j <= nl - 1; j++) hh[d][j + 1] = 2.0 * (u[d] * hh[d][j] - hh[d][j - 1]) / (j + 1);
+            hh[d][1] = 2.0 * u[d] * f;
+            for (int j = 1; j <= top; j++) hh[d][j + 1] = 2.0 * (u[d] * hh[d][j] - hh[d][j - 1]) / (j + 1);
+            if (rn)
+                for (int j = 0; j <= nl; j++) dh[d][j] = -(hh[d][j + 1] * dsq * (j + 1.0));
         }
         for (int ind = 0; ind < nd; ind++) {
-            double q = chg[(size_t)nd * i + ind];
+            double q = chg ? chg[(size_t)nd * i + ind] : 0.0;
             double *L = loc + (size_t)ind * m3;
-            for (int j1 = 0; j1 <= nl; j1++) {
-                double zt = q * hz[j1];
-                for (int j2 = 0; j2 <= nl; j2++) {
-                    double yt = zt * hy[j2];
-                    for (int j3 = 0; j3 <= nl; j3++) L[j3 + m * (j2 + (size_t)m * j1)] += yt * hx[j3];
+            if (!rn) {
+                for (int j1 = 0; j1 <= nl; j1++) {
+                    double zt = q * hz[j1];
+                    for (int j2 = 0; j2 <= nl; j2++) {
+                        double yt = zt * hy[j2];
+                        for (int j3 = 0; j3 <= nl; j3++) L[j3 + m * (j2 + (size_t)m * j1)] += yt * hx[j3];
+                    }
                 }
+                continue;
             }
+            double dd = ds[(size_t)nd * i + ind];
+            double w1 = rn[3 * i] * dd, w2 = rn[3 * i + 1] * dd, w3 = rn[3 * i + 2] * dd;
+            for (int j1 = 0; j1 <= nl; j1++)
+                for (int j2 = 0; j2 <= nl; j2++) {
+                    double a = hy[j2] * hz[j1];                                  /* x-derivative term */
+                    double b = (q * hy[j2] + w2 * dhy[j2]) * hz[j1] + w3 * hy[j2] * dhz[j1];
+                    for (int j3 = 0; j3 <= nl; j3++)
+                        L[j3 + m * (j2 + (size_t)m * j1)] += w1 * a * dhx[j3] + b * hx[j3];
+                }
         }
     }
 }
@@ -1064,11 +1253,14 @@ void orc3_last_stats(int *out)
 
 /* fgt3d (fgt3d.f:19-480) and fgt3dmain (:483-1530) for charges. */
 int orc_fgt3d(int nd, double delta, double eps, int ns, const double *sources,
-              int ifcharge, const double *charge, int ifdipole, int ifpgh, double *pot,
-              double *grad, double *hess, int nt, const double *targ, int ifpghtarg,
-              double *pottarg, double *gradtarg, double *hesstarg)
+              int ifcharge, const double *charge, int ifdipole, const double *rnormal,
+              const double *dipstr, int ifpgh, double *pot, double *grad, double *hess, int nt,
+              const double *targ, int ifpghtarg, double *pottarg, double *gradtarg,
+              double *hesstarg)
 {
-    if (ifcharge != 1 || ifdipole != 0) return -3;
+    if (ifcharge < 0 || ifcharge > 1 || ifdipole < 0 || ifdipole > 1) return -1;
+    if (!ifcharge) charge = NULL;
+    if (!ifdipole) rnormal = dipstr = NULL;
     int npwlevel;
     double bs0, cen0[3];
     orc3_boxsize0(delta, eps, sources, ns, targ, nt, &npwlevel, &bs0, cen0);
@@ -1079,7 +1271,7 @@ int orc_fgt3d(int nd, double delta, double eps, int ns, const double *sources,
         int nl0 = orc_hlterms(3, bs0, delta, eps) + (ifpgh > ifpghtarg ? ifpgh : ifpghtarg) - 1;
         size_t m3 = (size_t)(nl0 + 1) * (nl0 + 1) * (nl0 + 1);
         double *loc = xc(m3 * nd, sizeof(double));
-        o3_formlc(nd, delta, sources, ns, charge, cen0, nl0, loc);
+        if (charge || rnormal) o3_formlc(nd, delta, sources, ns, charge, rnormal, dipstr, cen0, nl0, loc);
         if (ifpghtarg >= 1) o3_leval(nd, delta, cen0, nl0, loc, targ, nt, ifpghtarg, pottarg, gradtarg, hesstarg);
         if (ifpgh >= 1) o3_leval(nd, delta, cen0, nl0, loc, sources, ns, ifpgh, pot, grad, hess);
         free(loc);
@@ -1099,10 +1291,17 @@ int orc_fgt3d(int nd, double delta, double eps, int ns, const double *sources,
     /* sorted copies (dreorderf) and zeroed sorted outputs */
     double *ss = xc(3 * (size_t)ns, sizeof(double)), *ts3 = xc(3 * (size_t)nt + 3, sizeof(double));
     double *cs = xc((size_t)nd * ns, sizeof(double));
+    double *rs = xc(3 * (size_t)ns, sizeof(double)), *dss = xc((size_t)nd * ns, sizeof(double));
     for (int i = 0; i < ns; i++) {
         for (int d = 0; d < 3; d++) ss[3 * i + d] = sources[3 * (size_t)isrc[i] + d];
-        for (int k = 0; k < nd; k++) cs[(size_t)nd * i + k] = charge[(size_t)nd * isrc[i] + k];
+        if (charge)
+            for (int k = 0; k < nd; k++) cs[(size_t)nd * i + k] = charge[(size_t)nd * isrc[i] + k];
+        if (rnormal) {
+            for (int d = 0; d < 3; d++) rs[3 * i + d] = rnormal[3 * (size_t)isrc[i] + d];
+            for (int k = 0; k < nd; k++) dss[(size_t)nd * i + k] = dipstr[(size_t)nd * isrc[i] + k];
+        }
     }
+    const double *cq = charge ? cs : NULL, *crn = rnormal ? rs : NULL;
     for (int i = 0; i < nt; i++)
         for (int d = 0; d < 3; d++) ts3[3 * i + d] = targ[3 * (size_t)itarg[i] + d];
     double *ps = xc((size_t)nd * ns + 1, sizeof(double)), *gs = xc((size_t)3 * nd * ns + 1, sizeof(double)),
@@ -1113,10 +1312,10 @@ int orc_fgt3d(int nd, double delta, double eps, int ns, const double *sources,
     int nlevstart = npwlevel > 0 ? npwlevel : 0;
     int nlocal[O3_MAXLEV + 1];
     memset(nlocal, 0, sizeof(nlocal));
-    int nadd = (ifpgh > ifpghtarg ? ifpgh : ifpghtarg);
+    int nadd = ifdipole + (ifpgh > ifpghtarg ? ifpgh : ifpghtarg); /* fgt3d.f:355-360 */
     for (int l = nlevstart; l <= nlevels; l++) {
         nlocal[l] = orc_hlterms(3, T.bsz[l], delta, eps);
-        if (ifpgh >= 2 || ifpghtarg >= 2) nlocal[l] += nadd;
+        if (ifdipole == 1 || ifpgh >= 2 || ifpghtarg >= 2) nlocal[l] += nadd;
     }
     double pmax;
     int npw;
@@ -1211,13 +1410,16 @@ int orc_fgt3d(int nd, double delta, double eps, int ns, const double *sources,
             for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; b++) {
                 int np = sse[2 * b + 1] - sse[2 * b];
                 if (T.nch[b] != 0 || np <= 0) continue;
+                if (!cq && !crn) continue; /* fgt3d.f:952-1090: no branch forms anything */
                 const double *s = ss + 3 * (size_t)sse[2 * b];
-                const double *q = cs + (size_t)nd * sse[2 * b];
+                const double *q = cq ? cs + (size_t)nd * sse[2 * b] : NULL;
+                const double *r = crn ? rs + 3 * (size_t)sse[2 * b] : NULL;
+                const double *dd = dss + (size_t)nd * sse[2 * b];
                 if (np > nptsswitch) {
-                    o3_formhc(nd, delta, s, np, q, T.cen + 3 * b, p, hexp);
+                    o3_formhc(nd, delta, s, np, q, r, dd, T.cen + 3 * b, p, hexp);
                     o3_h2pw(nd, p, npw, h2xt, hexp, MP(b));
                 } else {
-                    o3_formpwc(nd, delta, s, np, q, T.cen + 3 * b, npw, ws, tsn, MP(b));
+                    o3_formpwc(nd, delta, s, np, q, r, dd, T.cen + 3 * b, npw, ws, tsn, MP(b));
                 }
             }
             free(hexp);
@@ -1327,12 +1529,15 @@ int orc_fgt3d(int nd, double delta, double eps, int ns, const double *sources,
             for (int i = 0; i < nl1[jb]; i++) {
                 int ib = l1[(size_t)mnl1 * jb + i];
                 int t0 = tse[2 * ib], t1 = tse[2 * ib + 1], s0 = sse[2 * ib], s1 = sse[2 * ib + 1];
+                const double *q = cq ? cs + (size_t)nd * js : NULL;
+                const double *r = crn ? rs + 3 * (size_t)js : NULL, *dd = dss + (size_t)nd * js;
+                if (!q && !r) continue; /* fgt3dpart_direct_vec: no branch applies */
                 if (t1 > t0 && ifpghtarg > 0)
-                    o3_direct(nd, delta, dmax, ss + 3 * (size_t)js, je - js, cs + (size_t)nd * js,
+                    o3_direct(nd, delta, dmax, ss + 3 * (size_t)js, je - js, q, r, dd,
                               ts3 + 3 * (size_t)t0, t1 - t0, ifpghtarg, pt + (size_t)nd * t0,
                               gt + (size_t)3 * nd * t0, ht + (size_t)6 * nd * t0);
                 if (s1 > s0 && ifpgh > 0)
-                    o3_direct(nd, delta, dmax, ss + 3 * (size_t)js, je - js, cs + (size_t)nd * js,
+                    o3_direct(nd, delta, dmax, ss + 3 * (size_t)js, je - js, q, r, dd,
                               ss + 3 * (size_t)s0, s1 - s0, ifpgh, ps + (size_t)nd * s0,
                               gs + (size_t)3 * nd * s0, hs + (size_t)6 * nd * s0);
             }
@@ -1355,7 +1560,7 @@ int orc_fgt3d(int nd, double delta, double eps, int ns, const double *sources,
     g3_last[5] = 0;
     for (int b = 0; b < nb; b++) g3_last[5] += T.nch[b] == 0;
     free(ifhung); free(nl1); free(l1);
-    free(isrc); free(itarg); free(sse); free(tse); free(ss); free(ts3); free(cs);
+    free(isrc); free(itarg); free(sse); free(tse); free(ss); free(ts3); free(cs); free(rs); free(dss);
     free(ps); free(gs); free(hs); free(pt); free(gt); free(ht);
     t3_free(&T);
     return 0;

@@ -88,7 +88,7 @@ def _outs(nd, n, ifp):
 
 
 def _strengths(ns, charge, rnormal, dipstr):
-    """(nd, charge (ns, nd) or None, rnormal (ns, 2) or None, dipstr
+    """(nd, charge (ns, nd) or None, rnormal (ns, dim) or None, dipstr
     (ns, nd) or None) from the arguments of a transform."""
     chg = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64).reshape(ns, -1)
     rn = ds = None
@@ -428,12 +428,13 @@ def ref_tree2d(delta, eps, sources, targ):
 # 3D library (oracle/_ref/libref3d.so)
 # --------------------------------------------------------------------------
 
-def reftest3_inputs(nsrc, ntarg):
+def reftest3_inputs(nsrc, ntarg, dipoles=False):
     """Inputs of the reference's own 3D test driver (3d/planewave/test/
     test_fgt3d.f:48-98): per source ten draws (theta, phi, x, y, z, three
     normal components, charge, dipole strength) with (x, y, z) kept, the
     corners (0,0,0) and (1,1,1) with zero charge and a unit charge at
-    (0.05,0.05,0.05), then three draws per target."""
+    (0.05,0.05,0.05), then three draws per target.  dipoles=True also
+    returns the normals and dipole strengths of those draws."""
     r = Hkrand()
     d = r.draw(10 * nsrc).reshape(nsrc, 10)
     src = d[:, 2:5].copy()
@@ -444,6 +445,8 @@ def reftest3_inputs(nsrc, ntarg):
     src[1] = (1.0, 1.0, 1.0)
     src[2] = (0.05, 0.05, 0.05)
     tg = r.draw(3 * ntarg).reshape(ntarg, 3)
+    if dipoles:
+        return src, chg, tg, d[:, 5:8].copy(), d[:, 9:10].copy()
     return src, chg, tg
 
 
@@ -468,10 +471,13 @@ def _orc3():
         dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
         lib.orc_fgt3d.restype = C.c_int
         lib.orc_fgt3d.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, dp, C.c_int, dp, C.c_int,
-                                  C.c_int, dp, dp, dp, C.c_int, dp, C.c_int, dp, dp, dp]
+                                  dp, dp, C.c_int, dp, dp, dp, C.c_int, dp, C.c_int, dp, dp, dp]
         lib.orc_direct3d.restype = None
         lib.orc_direct3d.argtypes = [C.c_int, C.c_double, C.c_double, dp, C.c_int, dp, dp, C.c_int,
                                      C.c_int, dp, dp, dp]
+        lib.orc_direct3d_cd.restype = None
+        lib.orc_direct3d_cd.argtypes = [C.c_int, C.c_double, C.c_double, dp, C.c_int, dp, dp, dp,
+                                        dp, C.c_int, C.c_int, dp, dp, dp]
         lib.orc3_boxsize0.restype = None
         lib.orc3_boxsize0.argtypes = [C.c_double, C.c_double, dp, C.c_int, dp, C.c_int, ip, dp, dp]
         lib.orc3_pwnodes.restype = None
@@ -488,21 +494,23 @@ def _orc3():
     return lib
 
 
-def oracle_fgt3d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
-    """The C restatement of fgt3d (fgt3d.f:19) for charges.  Returns
-    (pot, grad, hess, pottarg, gradtarg, hesstarg) shaped (n, nd), (n, 3, nd),
-    (n, 6, nd); Hessians ordered xx, yy, zz, xy, xz, yz."""
+def oracle_fgt3d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, rnormal=None,
+                 dipstr=None):
+    """The C restatement of fgt3d (fgt3d.f:19).  charge=None: no charges;
+    rnormal/dipstr (ns, 3) and (ns, nd): dipoles.  Returns (pot, grad, hess,
+    pottarg, gradtarg, hesstarg) shaped (n, nd), (n, 3, nd), (n, 6, nd);
+    Hessians ordered xx, yy, zz, xy, xz, yz."""
     lib = _orc3()
     src = _pts3(sources)
     ns = src.shape[0]
-    chg = np.ascontiguousarray(charge, dtype=np.float64).reshape(ns, -1)
-    nd = chg.shape[1]
+    nd, chg, rn, ds = _strengths(ns, charge, rnormal, dipstr)
     tg = _pts3(targ) if targ is not None else np.zeros((0, 3))
     nt = tg.shape[0]
     pot, grad, hess = _outs3(nd, ns, ifpgh)
     pt, gt, ht = _outs3(nd, max(nt, 1), ifpghtarg)
-    rc = lib.orc_fgt3d(nd, delta, eps, ns, _d(src), 1, _d(chg), 0, ifpgh, _d(pot), _d(grad),
-                       _d(hess), nt, _d(tg), ifpghtarg, _d(pt), _d(gt), _d(ht))
+    rc = lib.orc_fgt3d(nd, delta, eps, ns, _d(src), int(chg is not None), _ptr_or(chg, nd * ns),
+                       int(rn is not None), _ptr_or(rn, 3 * ns), _ptr_or(ds, nd * ns), ifpgh,
+                       _d(pot), _d(grad), _d(hess), nt, _d(tg), ifpghtarg, _d(pt), _d(gt), _d(ht))
     if rc != 0:
         raise RuntimeError("orc_fgt3d returned %d" % rc)
     return pot, grad, hess, pt[:nt], gt[:nt], ht[:nt]
@@ -528,6 +536,23 @@ def direct3d(delta, dmax, sources, charge, targ, ifpgh=1):
     pot, grad, hess = np.zeros((nt, nd)), np.zeros((nt, 3, nd)), np.zeros((nt, 6, nd))
     lib.orc_direct3d(nd, delta, dmax, _d(src), src.shape[0], _d(chg), _d(tg), nt, ifpgh,
                      _d(pot), _d(grad), _d(hess))
+    return pot, grad, hess
+
+
+def direct3d_cd(delta, dmax, sources, charge, rnormal, dipstr, targ, ifpgh=1):
+    """The 3D reference's near-field sums with dipoles (g3d_directdp/dg/dh
+    and g3d_directcdp/cdg/cdh, g3ddirect.f:207-700); charge=None: dipoles
+    only."""
+    lib = _orc3()
+    src = _pts3(sources)
+    ns = src.shape[0]
+    nd, chg, rn, ds = _strengths(ns, charge, rnormal, dipstr)
+    tg = _pts3(targ)
+    nt = tg.shape[0]
+    pot, grad, hess = np.zeros((nt, nd)), np.zeros((nt, 3, nd)), np.zeros((nt, 6, nd))
+    lib.orc_direct3d_cd(nd, delta, dmax, _d(src), ns, _d(chg) if chg is not None else None,
+                        _ptr_or(rn, 3 * ns), _ptr_or(ds, nd * ns), _d(tg), nt, ifpgh,
+                        _d(pot), _d(grad), _d(hess))
     return pot, grad, hess
 
 
@@ -600,22 +625,26 @@ def ref3_defined(delta, eps, sources, targ):
     return t["npwlevel"] != t["nlevels"]
 
 
-def ref_fgt3d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0):
+def ref_fgt3d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, rnormal=None,
+              dipstr=None):
     """Calls the reference's own fgt3d_ (fgt3d.f:19) through its Fortran ABI."""
     lib = _ref3()
     if not ref3_defined(delta, eps, sources, targ):
         raise RefUndefined("the 3D reference reads uninitialised memory for this input")
     src = _pts3(sources)
     ns = src.shape[0]
-    chg = np.ascontiguousarray(charge, dtype=np.float64).reshape(ns, -1)
-    nd = chg.shape[1]
+    nd, chg, rn, ds = _strengths(ns, charge, rnormal, dipstr)
+    ifc, ifd = int(chg is not None), int(rn is not None)
+    if chg is None:
+        chg = np.zeros((ns, nd))
+    if rn is None:
+        rn, ds = np.zeros((ns, 3)), np.zeros((ns, nd))
     tg = _pts3(targ) if targ is not None else np.zeros((1, 3))
     nt = tg.shape[0] if targ is not None else 0
     pot, grad, hess = _outs3(nd, ns, ifpgh)
     pt, gt, ht = _outs3(nd, max(nt, 1), ifpghtarg)
-    dummy = np.zeros(3 * max(ns, nt, 1))
-    lib.fgt3d_(_ri(nd), _rd(delta), _rd(eps), _ri(ns), _d(src), _ri(1), _d(chg), _ri(0), _d(dummy),
-               _d(dummy), _ri(0), _ri(ifpgh), _d(pot), _d(grad), _d(hess), _ri(nt), _d(tg),
+    lib.fgt3d_(_ri(nd), _rd(delta), _rd(eps), _ri(ns), _d(src), _ri(ifc), _d(chg), _ri(ifd), _d(rn),
+               _d(ds), _ri(0), _ri(ifpgh), _d(pot), _d(grad), _d(hess), _ri(nt), _d(tg),
                _ri(ifpghtarg), _d(pt), _d(gt), _d(ht))
     return pot, grad, hess, pt[:nt], gt[:nt], ht[:nt]
 

@@ -114,7 +114,7 @@ def golden_inputs(g):
     return src, chg if ifc else None, tg, rn, ds
 
 
-def dlaran_points3d(kind, ns, nt, nd=1):
+def dlaran_points3d(kind, ns, nt, nd=1, dipoles=False):
     """3D sources, charges (ns, nd) and targets from one DLARAN stream.
 
     uniform : points uniform in [0,1)^3, charges 2u-1.
@@ -123,9 +123,16 @@ def dlaran_points3d(kind, ns, nt, nd=1):
               centred sums of four uniforms, normalised); targets uniform;
               charges 2u-1.
     reftest : the reference 3D test driver's own inputs (test_fgt3d.f:48-98).
+
+    dipoles=True also returns normals (ns, 3) and dipole strengths (ns, nd):
+    for reftest the driver's own draws, otherwise 2u-1 drawn after the
+    targets from the same stream.
     """
     from oracle import oracle as O
     if kind == "reftest":
+        if dipoles:
+            src, chg, tg, rn, ds = O.reftest3_inputs(ns, nt, dipoles=True)
+            return src, chg, tg, rn, np.repeat(ds, nd, axis=1) if nd > 1 else ds
         return O.reftest3_inputs(ns, nt)
     r = O.Hkrand()
     if kind == "uniform":
@@ -137,4 +144,21 @@ def dlaran_points3d(kind, ns, nt, nd=1):
         src = 0.5 + rad[:, None] * v
     chg = 2.0 * r.draw(ns * nd).reshape(ns, nd) - 1.0
     tg = r.draw(3 * nt).reshape(nt, 3)
+    if dipoles:
+        rn = 2.0 * r.draw(3 * ns).reshape(ns, 3) - 1.0
+        ds = 2.0 * r.draw(ns * nd).reshape(ns, nd) - 1.0
+        return src, chg, tg, rn, ds
     return src, chg, tg
+
+
+def golden_inputs3(g):
+    """(sources, charge or None, targets, rnormal or None, dipstr or None)
+    of a 3D golden fixture."""
+    ns, nt, nd = int(g["ns"]), int(g["nt"]), int(g["nd"])
+    ifc = int(g["ifcharge"]) if "ifcharge" in g else 1
+    ifd = int(g["ifdipole"]) if "ifdipole" in g else 0
+    if not ifd:
+        src, chg, tg = dlaran_points3d(str(g["recipe"]), ns, nt, nd)
+        return src, chg, tg, None, None
+    src, chg, tg, rn, ds = dlaran_points3d(str(g["recipe"]), ns, nt, nd, dipoles=True)
+    return src, chg if ifc else None, tg, rn, ds

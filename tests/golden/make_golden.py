@@ -7,6 +7,7 @@ them:
     python tests/golden/make_golden.py        # 2D fixtures
     python tests/golden/make_golden.py dip    # 2D fixtures with dipole sources
     python tests/golden/make_golden.py 3d     # 3D fixtures (dim = 3)
+    python tests/golden/make_golden.py 3ddip  # 3D fixtures with dipole sources
 
 Every fixture is data only: the name of its input recipe
 (tests/_pts.dlaran_points, DLARAN uniforms and exact arithmetic, so the
@@ -57,6 +58,11 @@ CASES_DIP = {
     "uniform_dip_eps3": ("uniform", 50_000, 20_000, 1, 1e-3, 1e-3, 2, 3, 1, 1),
     "large_delta_dip": ("uniform", 5_000, 2_000, 2, 5.0, 1e-6, 3, 3, 1, 1),
 }
+CASES3D_DIP = {
+    "uniform3d_dip_only_gh_nd2": ("uniform", 40_000, 5_000, 2, 1e-2, 1e-3, 3, 2, 0, 1),
+    "uniform3d_dip_eps9": ("uniform", 50_000, 3_000, 1, 3e-3, 1e-9, 1, 3, 1, 1),
+    "reftest3d_dip_300k": ("reftest", 300_000, 100_000, 1, 1.2e-3, 1e-6, 1, 1, 1, 1),
+}
 NSAMPLE = 1000
 
 
@@ -89,6 +95,33 @@ def main3d(only=None):
         # the 3D reference test's oracle: direct sum with dmax = 1e30 at the
         # first 20 sources (test_fgt3d.f:142-152)
         rec["pot_direct20"] = O.direct3d(delta, 1e30, src, chg, src[:20])[0]
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+        print("wrote", name, rec["ints"])
+
+
+def main3d_dip(only=None):
+    if not O.have_ref3():
+        sys.exit("oracle/_ref/libref3d.so missing: run make -C oracle ref first")
+    for name, (recipe, ns, nt, nd, delta, eps, ifs, ift, ifc, ifd) in CASES3D_DIP.items():
+        if only and name not in only:
+            continue
+        src, chg, tg, rn, ds = _pts.dlaran_points3d(recipe, ns, nt, nd, dipoles=True)
+        chg = chg if ifc else None
+        if not O.ref3_defined(delta, eps, src, tg):
+            raise SystemExit("%s: the 3D reference is undefined for this input" % name)
+        out = O.ref_fgt3d(delta, eps, src, chg, tg, ifpgh=ifs, ifpghtarg=ift, rnormal=rn,
+                          dipstr=ds)
+        t = O.oracle_tree3d(delta, eps, src, tg)
+        isrc, itrg = sample(ns, 1), sample(nt, 2)
+        rec = dict(dim=3, recipe=recipe, ns=ns, nt=nt, nd=nd, delta=delta, eps=eps, ifpgh=ifs,
+                   ifpghtarg=ift, ifcharge=ifc, ifdipole=ifd, isample_src=isrc,
+                   isample_trg=itrg,
+                   ints=np.array([t["npwlevel"], t["nlmax"], t["nlevels"], t["nboxes"]]))
+        names = ["pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"]
+        for k, v in enumerate(out):
+            if (k < 3 and k < ifs) or (k >= 3 and k - 3 < ift):
+                rec[names[k]] = v[isrc if k < 3 else itrg]
+        rec["pot_direct20"] = O.direct3d_cd(delta, 1e30, src, chg, rn, ds, src[:20])[0]
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
         print("wrote", name, rec["ints"])
 
@@ -126,6 +159,8 @@ def main():
         return main3d(sys.argv[2:])
     if len(sys.argv) > 1 and sys.argv[1] == "dip":
         return main_dip(sys.argv[2:])
+    if len(sys.argv) > 1 and sys.argv[1] == "3ddip":
+        return main3d_dip(sys.argv[2:])
     if not O.have_ref():
         sys.exit("oracle/_ref/libref2d.so missing: run make -C oracle ref first")
     for name, (recipe, ns, nt, nd, delta, eps, ifs, ift) in CASES.items():

@@ -73,10 +73,11 @@ def test_fgt3d_matches_reference_golden(path):
         g = {k: z[k] for k in z.files}
     ns, nt, nd = int(g["ns"]), int(g["nt"]), int(g["nd"])
     print("golden", os.path.basename(path), ns, nt, nd, flush=True)
-    src, chg, tg = _pts.dlaran_points3d(str(g["recipe"]), ns, nt, nd)
+    src, chg, tg, rn, ds = _pts.golden_inputs3(g)
     delta, eps = float(g["delta"]), float(g["eps"])
     out = fgt2d_amd.fgt3d(delta, eps, src, chg, tg, ifpgh=int(g["ifpgh"]),
-                          ifpghtarg=int(g["ifpghtarg"]))
+                          ifpghtarg=int(g["ifpghtarg"]), ifdipole=int(rn is not None),
+                          rnormal=rn, dipstr=ds)
     for k, name in enumerate(NAMES):
         if name in g:
             sel = g["isample_src"] if k < 3 else g["isample_trg"]
@@ -84,7 +85,8 @@ def test_fgt3d_matches_reference_golden(path):
     st = fgt2d_amd.last_stats()
     assert [st["npwlevel"], st["nlmax"], st["nlevels"], st["nboxes"]] == list(g["ints"])
     if "pot" in g:
-        assert O.derr(g["pot_direct20"], out[0][:20]) < 30 * eps
+        # dipoles: the reference's own error reaches ~100 eps
+        assert O.derr(g["pot_direct20"], out[0][:20]) < (300 if rn is not None else 30) * eps
 
 
 def test_fgt3d_accuracy_against_direct_sum():

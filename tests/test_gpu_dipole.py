@@ -1,9 +1,11 @@
 """Dipole sources (ifdipole = 1) on the GPU, through the C-ABI, against the
-oracle (itself bit-for-bit with the reference library on dipoles): the
-whole transform with charges+dipoles and dipoles only, gradients and
-Hessians, nd = 2, the near-field-only, high-order (VALU formation) and
-large-delta branches; the all-pairs part_direct entry; the device entry
-against the host entry; a dipole request in the request server."""
+oracle (itself bit-for-bit with the reference libraries on dipoles, apart
+from the 3D large-delta branch, pinned by the direct sum): the whole
+transform with charges+dipoles and dipoles only, gradients and Hessians,
+nd = 2, the near-field-only, high-order (VALU formation) and large-delta
+branches, in 2D and 3D (Hermite and direct plane-wave formation routes);
+the all-pairs part_direct entries; the device entry against the host
+entry; a dipole request in the request server."""
 import numpy as np
 import pytest
 
@@ -110,10 +112,50 @@ def test_dipole_request_in_the_server():
             assert np.array_equal(served[k], alone[k]), k
 
 
-def test_dipoles_in_3d_are_refused():
-    rng = np.random.default_rng(1)
-    src = rng.random((100, 3))
-    with pytest.raises(fgt2d_amd.FGTError) as e:
-        fgt2d_amd.fgt3d(1e-3, 1e-6, src, None, None, ifpgh=1, ifdipole=1,
-                        rnormal=rng.normal(size=(100, 3)), dipstr=np.ones((100, 1)))
-    assert e.value.code == fgt2d_amd.FGTPW_EUNSUPPORTED
+# 3D: (recipe, ns, nt, delta, eps, nd, ifpgh, ifpghtarg, ifcharge)
+CASES3 = [
+    ("uniform", 40000, 5000, 1e-2, 1e-3, 2, 3, 2, 0),     # Hermite route, dipoles only
+    ("uniform", 30000, 3000, 3e-3, 1e-9, 1, 2, 3, 1),     # direct plane-wave route
+    ("uniform", 20000, 4000, 1e-3, 1e-6, 1, 1, 1, 1),
+    ("uniform", 4000, 1000, 1e-4, 1e-6, 1, 3, 3, 1),      # near field only
+    ("uniform", 600, 300, 5.0, 1e-6, 2, 3, 3, 1),         # large delta
+]
+
+
+@pytest.mark.parametrize("case", CASES3, ids=[str(i) for i in range(len(CASES3))])
+def test_dipole_transform_3d_matches_oracle(case):
+    recipe, ns, nt, delta, eps, nd, ifs, ift, ifc = case
+    print("case3d", case, flush=True)
+    src, chg, tg, rn, ds = _pts.dlaran_points3d(recipe, ns, nt, nd, dipoles=True)
+    chg = chg if ifc else None
+    got = fgt2d_amd.fgt3d(delta, eps, src, chg, tg, ifpgh=ifs, ifpghtarg=ift, ifdipole=1,
+                          rnormal=rn, dipstr=ds)
+    want = O.oracle_fgt3d(delta, eps, src, chg, tg, ifpgh=ifs, ifpghtarg=ift, rnormal=rn,
+                          dipstr=ds)
+    for k, (g, w) in enumerate(zip(got, want)):
+        if g is None or (k < 3 and k >= ifs) or (k >= 3 and k - 3 >= ift):
+            continue
+        assert _rel(g, w) < 1e-10, (k, _rel(g, w))
+
+
+def test_dipole_part_direct_3d_matches_oracle():
+    rng = np.random.default_rng(4)
+    src = rng.random((2000, 3))
+    tg = rng.random((500, 3))
+    chg = rng.normal(size=(2000, 1))
+    rn = rng.normal(size=(2000, 3))
+    ds = rng.normal(size=(2000, 1))
+    lib = fgt2d_amd.load()
+    for c in (chg, None):
+        for ifp in (1, 2, 3):
+            pot, grad, hess = np.zeros((500, 1)), np.zeros((500, 3, 1)), np.zeros((500, 6, 1))
+            z = np.zeros(8)
+            rc = lib.fgtpw_fgt3dpart_direct(
+                1, 1e-2, 0.05, 1, 2000, 1, 500, fgt2d_amd._ptr(src), int(c is not None),
+                fgt2d_amd._ptr(c if c is not None else z), 1, fgt2d_amd._ptr(rn),
+                fgt2d_amd._ptr(ds), fgt2d_amd._ptr(tg), ifp, fgt2d_amd._ptr(pot),
+                fgt2d_amd._ptr(grad), fgt2d_amd._ptr(hess))
+            assert rc == 0
+            want = O.direct3d_cd(1e-2, 0.05, src, c, rn, ds, tg, ifp)
+            for k, g in enumerate((pot, grad, hess)[:ifp]):
+                assert _rel(g, want[k]) < 1e-12, (ifp, k)

@@ -28,9 +28,10 @@ def test_oracle3d_matches_golden(path):
     ns, nt, nd = int(g["ns"]), int(g["nt"]), int(g["nd"])
     if ns > 100_000:
         pytest.skip("large fixture: checked on the GPU path")
-    src, chg, tg = _pts.dlaran_points3d(str(g["recipe"]), ns, nt, nd)
+    src, chg, tg, rn, ds = _pts.golden_inputs3(g)
     delta, eps = float(g["delta"]), float(g["eps"])
-    out = O.oracle_fgt3d(delta, eps, src, chg, tg, int(g["ifpgh"]), int(g["ifpghtarg"]))
+    out = O.oracle_fgt3d(delta, eps, src, chg, tg, int(g["ifpgh"]), int(g["ifpghtarg"]),
+                         rnormal=rn, dipstr=ds)
     for k, name in enumerate(NAMES):
         if name in g:
             sel = g["isample_src"] if k < 3 else g["isample_trg"]
@@ -38,9 +39,11 @@ def test_oracle3d_matches_golden(path):
     st = O.oracle_stats3d()
     assert [st["npwlevel"], st["nlmax"], st["nlevels"], st["nboxes"]] == list(g["ints"])
     if "pot" in g:
-        ex = O.direct3d(delta, 1e30, src, chg, src[:20])[0]
+        ex = O.direct3d_cd(delta, 1e30, src, chg, rn, ds, src[:20])[0]
         assert _rel(ex, g["pot_direct20"]) < 1e-13
-        assert O.derr(g["pot_direct20"], g["pot"][:20]) < 30 * eps
+        # with dipoles the reference keeps the charge kernel's term counts plus
+        # one (fgt3d.f:355-360): its own error reaches ~100 eps
+        assert O.derr(g["pot_direct20"], g["pot"][:20]) < (300 if rn is not None else 30) * eps
 
 
 @pytest.mark.skipif(not O.have_ref3(), reason="3D reference build (oracle/_ref) not present")

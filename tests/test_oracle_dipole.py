@@ -1,4 +1,4 @@
-"""Dipole sources (ifdipole = 1) in the 2D oracle.
+"""Dipole sources (ifdipole = 1) in the 2D and 3D oracles.
 
 * The near-field kernels (g2d_directd*/directcd*_vec, g2ddirect.f:217-655)
   against what the mathematics fixes: a dipole of strength d and direction n
@@ -84,4 +84,67 @@ def test_oracle_with_dipoles_matches_reference_library(recipe, ns, nt, nd, delta
     for k in range(6):
         if (k < 3 and k < ifs) or (k >= 3 and k - 3 < ift):
             # same algorithm, same operation order
+            assert _rel(o[k], r[k]) < 1e-13, NAMES[k]
+
+
+@pytest.mark.parametrize("ifpgh", [1, 2, 3])
+def test_dipole_kernel_3d_is_the_source_derivative(ifpgh):
+    rng = np.random.default_rng(10 + ifpgh)
+    delta = 5e-3
+    src = rng.random((30, 3)) * 0.2 + 0.4
+    tg = rng.random((20, 3)) * 0.2 + 0.4
+    rn = rng.normal(size=(30, 3))
+    ds = rng.normal(size=(30, 1))
+    chg = rng.normal(size=(30, 1))
+    got = O.direct3d_cd(delta, 1e30, src, None, rn, ds, tg, ifpgh)
+    h = 1e-5 * np.sqrt(delta)
+    want = [np.zeros_like(v) for v in got]
+    for i in range(30):
+        for sgn in (1.0, -1.0):
+            part = O.direct3d(delta, 1e30, src[i:i + 1] + sgn * h * rn[i], ds[i:i + 1], tg, ifpgh)
+            for k in range(3):
+                want[k] += sgn * part[k] / (2 * h)
+    for k in range(ifpgh):
+        assert _rel(got[k], want[k]) < 1e-7, NAMES[k]
+    both = O.direct3d_cd(delta, 1e30, src, chg, rn, ds, tg, ifpgh)
+    only_c = O.direct3d(delta, 1e30, src, chg, tg, ifpgh)
+    for k in range(ifpgh):
+        assert _rel(both[k], only_c[k] + got[k]) < 1e-13, NAMES[k]
+
+
+@pytest.mark.parametrize("ifc", [1, 0])
+@pytest.mark.parametrize("n,delta,eps,tol", [(3000, 2e-2, 1e-6, 5e-5), (600, 5.0, 1e-6, 3e-5),
+                                             (600, 5.0, 1e-9, 3e-8)])
+def test_transform_3d_with_dipoles_against_direct_sum(ifc, n, delta, eps, tol):
+    """Includes the large-delta branch, whose dipole terms this build
+    computes as the mathematics fixes (the reference's g3dformld_vec indexes
+    the x factors by the y order)."""
+    rng = np.random.default_rng(8)
+    src = rng.random((n, 3))
+    tg = rng.random((400, 3))
+    chg = rng.uniform(-1, 1, (n, 1)) if ifc else None
+    rn = rng.normal(size=(n, 3))
+    ds = rng.uniform(-1, 1, (n, 1))
+    out = O.oracle_fgt3d(delta, eps, src, chg, tg, ifpgh=1, ifpghtarg=3, rnormal=rn, dipstr=ds)
+    ex = O.direct3d_cd(delta, 1e30, src, chg, rn, ds, tg[:200], ifpgh=3)
+    assert O.derr(ex[0], out[3][:200]) < tol
+    assert O.derr(ex[1], out[4][:200]) < 10 * tol
+    assert O.derr(ex[2], out[5][:200]) < 100 * tol
+
+
+@pytest.mark.skipif(not O.have_ref3(), reason="3D reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("n,delta,eps,ifs,ift,ifc", [
+    (40000, 1e-2, 1e-3, 3, 2, 1),
+    (30000, 8e-3, 1e-6, 1, 1, 0),
+    (30000, 3e-3, 1e-9, 2, 3, 1),
+])
+def test_oracle3d_with_dipoles_matches_reference_library(n, delta, eps, ifs, ift, ifc):
+    src, chg, tg, rn, ds = _pts.dlaran_points3d("uniform", n, 3000, 2, dipoles=True)
+    chg = chg if ifc else None
+    if not O.ref3_defined(delta, eps, src, tg):
+        pytest.skip("3D reference undefined for this input")
+    r = O.ref_fgt3d(delta, eps, src, chg, tg, ifs, ift, rnormal=rn, dipstr=ds)
+    o = O.oracle_fgt3d(delta, eps, src, chg, tg, ifs, ift, rnormal=rn, dipstr=ds)
+    for k in range(6):
+        if (k < 3 and k < ifs) or (k >= 3 and k - 3 < ift):
             assert _rel(o[k], r[k]) < 1e-13, NAMES[k]
