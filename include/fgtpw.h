@@ -11,8 +11,9 @@
  * ifpgh / ifpghtarg (1 pot, 2 pot+grad, 3 pot+grad+hess); arrays of
  * quantities not requested are not touched and may be dummies.
  *
- * Supported in this build: charges (ifcharge = 1, ifdipole = 0), any nd >= 1,
- * in 2D (fgt2d) and 3D (fgt3d).  Dipoles return FGTPW_EUNSUPPORTED.
+ * Supported in this build: charges (ifcharge = 1) and dipoles (ifdipole = 1,
+ * rnormal (dim,n), dipstr (nd,n)), alone or together, any nd >= 1, in 2D
+ * (fgt2d) and 3D (fgt3d).  With neither, the outputs are zero.
  */
 #ifndef FGTPW_H
 #define FGTPW_H
