@@ -760,11 +760,146 @@ __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
     }
 }
 
+// FP64 matrix cores (v_mfma_f64_16x16x4f64): D(16x16) += A(16x4) B(4x16) per
+// wavefront; lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
+// D[l/16 + 4r][l%16] in element r of the result.
+typedef double d4_t3 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4_t3 mfma3(double a, double b, d4_t3 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ void wave_sync3()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// PW -> Taylor on matrix cores, one wavefront (block of 64) per leaf, for
+// Taylor orders nl < 16 and npw <= 32:
+//   for each z node c < npw/2:
+//     S1(j3, b) = sum_a D(a, j3) PW(a, b, c)      MFMA, a padded to 32
+//     S2(j3, j2) = sum_b S1(j3, b) D(b, j2)       MFMA
+//     L(j3, j2, j1) += 2 Re(D(c, j1) S2(j3, j2))  registers
+// with D(a, j) = (i ts_a)^j / j!.  The slab PW(., ., c) is staged in the
+// wave's LDS (zero padded); S1 then reuses that space.  Lane l holds
+// L(j3 = l/16 + 4r, j2 = l%16, j1) for r < 4, j1 < 16.
+constexpr int PW3_N = 32;  // padded npw
+
+__global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
+{
+    extern __shared__ double sm[];
+    const int m = a.nl + 1, npw = a.npw, half = npw / 2, lane = threadIdx.x;
+    const long nexp = (long)npw * npw * half;
+    double* P = sm;                  // complex (PW3_N a) x (PW3_N b), a fastest
+    double* Dt = sm + 2 * PW3_N * PW3_N;  // complex D(c, j1), (half, 16)
+    const int i16 = lane & 15, k4 = lane >> 4;
+    // this lane's D(4 ks + l/16, l%16): A operand of S1 and B operand of S2
+    double Dr[PW3_N / 4], Di[PW3_N / 4];
+#pragma unroll
+    for (int ks = 0; ks < PW3_N / 4; ++ks) {
+        const int aa = 4 * ks + k4;
+        const bool in = aa < npw && i16 < m;
+        Dr[ks] = in ? a.dcoef[2 * (aa + a.cstride * i16)] : 0.0;
+        Di[ks] = in ? a.dcoef[2 * (aa + a.cstride * i16) + 1] : 0.0;
+    }
+    for (int o = lane; o < half * 16; o += 64) {
+        const int c = o / 16, j = o % 16;
+        const bool in = j < m;
+        Dt[2 * o] = in ? a.dcoef[2 * (c + a.cstride * j)] : 0.0;
+        Dt[2 * o + 1] = in ? a.dcoef[2 * (c + a.cstride * j) + 1] : 0.0;
+    }
+    for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
+        const int box = a.boxes[li];
+        if (pw_box(a, box)) continue;
+        const double* pw = a.loc + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+        double L[4][16];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) L[r][j] = 0;
+        for (int c = 0; c < half; ++c) {
+            wave_sync3();  // previous readers of P / S1 are done
+            const double* slab = pw + 2L * npw * npw * c;
+            for (int o = lane; o < PW3_N * PW3_N; o += 64) {
+                const int aa = o % PW3_N, b = o / PW3_N;
+                const bool in = aa < npw && b < npw;
+                P[2 * o] = in ? slab[2 * (aa + npw * b)] : 0.0;
+                P[2 * o + 1] = in ? slab[2 * (aa + npw * b) + 1] : 0.0;
+            }
+            wave_sync3();
+            // S1 = D^T PW: two 16-column tiles over b
+            d4_t3 s1r[2], s1i[2];
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                s1r[nt] = (d4_t3){0, 0, 0, 0};
+                s1i[nt] = (d4_t3){0, 0, 0, 0};
+#pragma unroll
+                for (int ks = 0; ks < PW3_N / 4; ++ks) {
+                    const int o = (4 * ks + k4) + PW3_N * (16 * nt + i16);
+                    const double pr = P[2 * o], pi = P[2 * o + 1];
+                    s1r[nt] = mfma3(Dr[ks], pr, s1r[nt]);
+                    s1r[nt] = mfma3(-Di[ks], pi, s1r[nt]);
+                    s1i[nt] = mfma3(Dr[ks], pi, s1i[nt]);
+                    s1i[nt] = mfma3(Di[ks], pr, s1i[nt]);
+                }
+            }
+            wave_sync3();  // all reads of P done: S1 overwrites it
+            double* S1 = P;  // complex (16 j3) x (PW3_N b), j3 fastest
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int o = (k4 + 4 * r) + 16 * (16 * nt + i16);
+                    S1[2 * o] = s1r[nt][r];
+                    S1[2 * o + 1] = s1i[nt][r];
+                }
+            wave_sync3();
+            // S2 = S1 D
+            d4_t3 s2r = {0, 0, 0, 0}, s2i = {0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < PW3_N / 4; ++ks) {
+                const int o = i16 + 16 * (4 * ks + k4);
+                const double xr = S1[2 * o], xi = S1[2 * o + 1];
+                s2r = mfma3(xr, Dr[ks], s2r);
+                s2r = mfma3(-xi, Di[ks], s2r);
+                s2i = mfma3(xr, Di[ks], s2i);
+                s2i = mfma3(xi, Dr[ks], s2i);
+            }
+            // L += 2 Re(D(c, j1) S2)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const double dr = Dt[2 * (16 * c + j)], di = Dt[2 * (16 * c + j) + 1];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) L[r][j] += 2.0 * (dr * s2r[r] - di * s2i[r]);
+            }
+        }
+        double* out = a.taylor + (long)li * m * m * m;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j3 = k4 + 4 * r, j2 = i16;
+            if (j3 < m && j2 < m)
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (j < m) out[j3 + m * (j2 + (long)m * j)] = L[r][j];
+        }
+    }
+}
+
 void launch_pw2local3(const Eval3Args& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
     const int m = a.nl + 1;
     if (m > L3_MAXM) fail(-3, "3D Taylor order above 63 is not supported");
+    if (m <= 16 && a.npw <= PW3_N) {
+        size_t lds = ((size_t)2 * PW3_N * PW3_N + (size_t)2 * (a.npw / 2) * 16) * sizeof(double);
+        kmark(KID_PW2LOCAL, false, s);
+        k_pw2local3_wave<<<std::min(a.nbox, 65536), 64, lds, s>>>(a);
+        kmark(KID_PW2LOCAL, true, s);
+        return;
+    }
     size_t lds = (size_t)4 * a.npw * m * sizeof(double);
     kmark(KID_PW2LOCAL, false, s);
     if (m <= 32) {
