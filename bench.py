@@ -204,21 +204,50 @@ def cpu_leg(src, chg, tg, budget_s=30.0):
 
 N3 = 1_000_000
 DELTA3, EPS3 = 1e-3, 1e-6
+N5 = 8_000_000
+DELTA5, EPS5 = 1e-5, 1e-9
+
+
+def shell_workload(n=N5, seed=SEED):
+    """8M sources on the sphere shell of radius 0.4 about (0.5,0.5,0.5) with
+    radial noise of standard deviation 1e-3 (directions uniform on the
+    sphere), 8M targets uniform in [0,1)^3, charges U[-1,1]."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    src = 0.5 + d * (0.4 + rng.normal(0.0, 1e-3, n))[:, None]
+    tg = rng.random((n, 3))
+    chg = rng.uniform(-1, 1, (n, 1))
+    return src, chg, tg
 
 
 def bench3d(args, torch, fgt2d_amd):
-    """3D: 1M sources = targets uniform in [0,1)^3, charges U[-1,1], delta
-    1e-3, eps 1e-6, potentials at the sources (ifpgh = 1)."""
-    rng = np.random.default_rng(SEED)
-    src = rng.random((N3, 3))
-    chg = rng.uniform(-1, 1, (N3, 1))
+    """3D.  c3: 1M sources = targets uniform in [0,1)^3, charges U[-1,1],
+    delta 1e-3, eps 1e-6, potentials at the sources (ifpgh = 1).  c5: the
+    8M + 8M shell workload, delta 1e-5, eps 1e-9, potentials at the targets
+    (ifpghtarg = 1)."""
+    c5 = args.workload == "c5"
+    if c5:
+        src, chg, tg = shell_workload()
+        delta, eps, nsel = DELTA5, EPS5, 10000
+        npts = 2 * N5
+    else:
+        rng = np.random.default_rng(SEED)
+        src = rng.random((N3, 3))
+        chg = rng.uniform(-1, 1, (N3, 1))
+        tg = None
+        delta, eps, nsel = DELTA3, EPS3, 5000
+        npts = N3
     dev = torch.device("cuda:0")
     d_src = torch.from_numpy(src).to(dev)
     d_chg = torch.from_numpy(chg).to(dev)
+    d_tg = torch.from_numpy(tg).to(dev) if c5 else None
     torch.cuda.synchronize()
 
     def step():
-        return fgt2d_amd.fgt3d(DELTA3, EPS3, d_src, d_chg, None, ifpgh=1, ifpghtarg=0)
+        if c5:
+            return fgt2d_amd.fgt3d(delta, eps, d_src, d_chg, d_tg, ifpgh=0, ifpghtarg=1)
+        return fgt2d_amd.fgt3d(delta, eps, d_src, d_chg, None, ifpgh=1, ifpghtarg=0)
 
     for _ in range(args.warmup):
         step()
@@ -235,16 +264,24 @@ def bench3d(args, torch, fgt2d_amd):
               for k, v in sorted(agg.items(), key=lambda kv: -kv[1][0])}
         print(json.dumps({"stages_ms": st, "stats": stats, "kernels_us_n_units": ks}),
               file=sys.stderr)
-    sel = np.random.default_rng(1).permutation(N3)[:5000]
-    ex = fgt2d_amd.fgt3dpart_direct(DELTA3, 1e30, src, chg, src[sel])[0]
+    pts = tg if c5 else src
+    sel = np.random.default_rng(1).permutation(len(pts))[:nsel]
+    ex = fgt2d_amd.fgt3dpart_direct(delta, 1e30, src, chg, pts[sel])[0]
     from oracle import oracle as O
-    err = O.derr(ex, out[0].cpu().numpy()[sel])
-    rec = {"metric": "Mpts/s, 3D N=1M eps=1e-6 plane-wave FGT (one MI355X, FP64)",
-           "value": round(N3 / t / 1e6, 3), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
+    err = O.derr(ex, out[3 if c5 else 0].cpu().numpy()[sel])
+    if c5:
+        metric = "Mpts/s, 3D 8M shell sources + 8M targets eps=1e-9 plane-wave FGT (one MI355X, FP64)"
+        wl = ("fgt3d 8M sources on a sphere shell (r=0.4, radial sd 1e-3) + 8M uniform targets, "
+              "delta=1e-5, eps=1e-9, pot at targets")
+    else:
+        metric = "Mpts/s, 3D N=1M eps=1e-6 plane-wave FGT (one MI355X, FP64)"
+        wl = "fgt3d uniform 1M sources = targets, delta=1e-3, eps=1e-6, pot at sources"
+    rec = {"metric": metric,
+           "value": round(npts / t / 1e6, 3), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(t * 1e3, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "fgt3d uniform 1M sources = targets, delta=1e-3, eps=1e-6, pot at sources",
-                      "ns": N3, "delta": DELTA3, "eps": EPS3, "nlevels": stats.get("nlevels"),
+           "config": {"workload": wl, "ns": len(src), "nt": 0 if tg is None else len(tg),
+                      "delta": delta, "eps": eps, "nlevels": stats.get("nlevels"),
                       "nboxes": stats.get("nboxes"), "npw": stats.get("npw")},
            "rel_l2_err_vs_direct": err,
            "stages_ms": {k: round(v, 4) for k, v in st.items()}}
@@ -343,10 +380,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--stages", action="store_true", help="print per-stage times to stderr")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: the 2D 2M+2M mixture (default, the headline metric); "
                          "c3: the 3D 1M uniform configuration; "
-                         "c4: the serving stream alone (10k requests at 200/s)")
+                         "c4: the serving stream alone (10k requests at 200/s); "
+                         "c5: the 3D 8M shell + 8M targets at eps 1e-9")
     ap.add_argument("--serve-requests", type=int, default=1000,
                     help="requests of the serving stream reported beside c2 (0: none)")
     ap.add_argument("--slots", type=int, default=4, help="server slots of the serving stream")
@@ -367,7 +405,7 @@ def main():
 
     import fgt2d_amd
 
-    if args.workload == "c3":
+    if args.workload in ("c3", "c5"):
         return bench3d(args, torch, fgt2d_amd)
     if args.workload == "c4":
         rec = serve_stream(fgt2d_amd, 10_000, slots=args.slots)

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -93,10 +94,10 @@ struct Workspace {
     hipStream_t aux[2] = {};
     std::vector<hipEvent_t> order_ev;  // stream-ordering events, reused from call to call
     size_t n_order = 0;
-    hipEvent_t ev[2 * S_N] = {};
+    std::vector<hipEvent_t> sev;  // stage timing events, grown on demand
     std::vector<hipEvent_t> kev;  // per-kernel timing events, grown on demand
     // host scratch of the plan, reused from call to call
-    std::vector<int> h_units, h_uptr, h_ulist, h_nl1, h_l1, h_sp, h_sl, h_fill, h_slot;
+    std::vector<int> h_units, h_uptr, h_ulist, h_nl1, h_l1, h_sp, h_sl, h_fill, h_slot, h_slot2;
     std::vector<char> h_plan;
     Tables2D tab;
     Tables3D tab3;
@@ -115,11 +116,17 @@ struct Workspace {
         if (b.cap < bytes) {
             if (b.p) HIPCHECK(hipFree(b.p));
             b.p = nullptr;
-            size_t want = bytes + bytes / 8;
+            b.cap = 0;
+            size_t want = bytes + std::min<size_t>(bytes / 8, (size_t)256 << 20);
             HIPCHECK(hipMalloc(&b.p, want));
             b.cap = want;
         }
         return (T*)b.p;
+    }
+    size_t held(const std::string& name) const
+    {
+        auto it = dev.find(name);
+        return it == dev.end() ? 0 : it->second.cap;
     }
     // work queued on `to` after this call waits for the work queued on
     // `from` before it
@@ -143,8 +150,9 @@ struct Workspace {
         if (pin) (void)hipHostFree(pin);
         pin = nullptr;
         pin_cap = 0;
-        for (auto& e : ev)
+        for (auto& e : sev)
             if (e) (void)hipEventDestroy(e);
+        sev.clear();
         for (auto& e : kev)
             if (e) (void)hipEventDestroy(e);
         kev.clear();
@@ -175,7 +183,6 @@ Workspace* new_workspace()
     Workspace* w = new Workspace();
     HIPCHECK(hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking));
     for (auto& a : w->aux) HIPCHECK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
-    for (auto& e : w->ev) HIPCHECK(hipEventCreate(&e));
     return w;
 }
 
@@ -207,6 +214,7 @@ struct LaunchClock : KernelClock {
     struct Rec {
         int id, level, side;
         const std::vector<int>* boxes;
+        int r0, r1;
         long long units;
         hipEvent_t e0, e1;
     };
@@ -214,11 +222,12 @@ struct LaunchClock : KernelClock {
     std::vector<Rec> recs;
     int level = -1, side = -1;
     const std::vector<int>* boxes = nullptr;
+    int r0 = 0, r1 = -1;  // the launch covers (*boxes)[r0, r1), r1 < 0: all
     long long units = 0;
     explicit LaunchClock(std::vector<hipEvent_t>& p) : pool(p) {}
-    void ctx(int l, const std::vector<int>* b, int sd, long long u = 0)
+    void ctx(int l, const std::vector<int>* b, int sd, long long u = 0, int b0 = 0, int b1 = -1)
     {
-        level = l, boxes = b, side = sd, units = u;
+        level = l, boxes = b, side = sd, units = u, r0 = b0, r1 = b1;
     }
     hipEvent_t ev(size_t i)
     {
@@ -232,7 +241,7 @@ struct LaunchClock : KernelClock {
     void mark(int id, bool end, hipStream_t st) override
     {
         if (!end) {
-            Rec r{id, level, side, boxes, units, ev(2 * recs.size()), ev(2 * recs.size() + 1)};
+            Rec r{id, level, side, boxes, r0, r1, units, ev(2 * recs.size()), ev(2 * recs.size() + 1)};
             HIPCHECK(hipEventRecord(r.e0, st));
             recs.push_back(r);
         } else {
@@ -246,38 +255,57 @@ struct Timer {
     hipStream_t s;
     bool on;
     double host_ms = 0, build_ms = 0;
-    bool used[S_N] = {};
+    // stage spans: a stage may be entered several times per call (chunked
+    // work); its time is the sum of its spans
+    struct Span { int k; hipEvent_t e0, e1; };
+    std::vector<Span> spans;
+    int open[S_N];
     LaunchClock clock;
     const int2 *src_se = nullptr, *trg_se = nullptr;
     int nb = 0;
     Timer(Workspace& w, hipStream_t st) : W(w), s(st), on(g_timing), clock(w.kev)
     {
         t_klaunch.clear();
+        for (int& o : open) o = -1;
         if (on) t_kclock = &clock;
     }
-    ~Timer() { t_kclock = nullptr; }
-    void ctx(int l, const std::vector<int>* b, int sd, long long u = 0)
+    hipEvent_t sev(size_t i)
     {
-        if (on) clock.ctx(l, b, sd, u);
+        while (W.sev.size() <= i) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            W.sev.push_back(e);
+        }
+        return W.sev[i];
+    }
+    ~Timer() { t_kclock = nullptr; }
+    void ctx(int l, const std::vector<int>* b, int sd, long long u = 0, int b0 = 0, int b1 = -1)
+    {
+        if (on) clock.ctx(l, b, sd, u, b0, b1);
     }
     void begin(int k, hipStream_t st = nullptr)
     {
-        if (on) { HIPCHECK(hipEventRecord(W.ev[2 * k], st ? st : s)); used[k] = true; }
+        if (!on) return;
+        const size_t n = spans.size();
+        spans.push_back({k, sev(2 * n), sev(2 * n + 1)});
+        HIPCHECK(hipEventRecord(spans[n].e0, st ? st : s));
+        open[k] = (int)n;
     }
     void end(int k, hipStream_t st = nullptr)
     {
-        if (on) HIPCHECK(hipEventRecord(W.ev[2 * k + 1], st ? st : s));
+        if (!on || open[k] < 0) return;
+        HIPCHECK(hipEventRecord(spans[open[k]].e1, st ? st : s));
+        open[k] = -1;
     }
     void finish()
     {
         t_stage_ms.assign(S_N, 0.0);
         if (!on) return;
         HIPCHECK(hipStreamSynchronize(s));
-        for (int k = 0; k < S_N; ++k) {
-            if (!used[k]) continue;
+        for (const Span& sp : spans) {
             float ms = 0;
-            HIPCHECK(hipEventElapsedTime(&ms, W.ev[2 * k], W.ev[2 * k + 1]));
-            t_stage_ms[k] = ms;
+            HIPCHECK(hipEventElapsedTime(&ms, sp.e0, sp.e1));
+            t_stage_ms[sp.k] += ms;
         }
         t_stage_ms[S_HOST] = host_ms;
         t_stage_ms[S_HOST_BUILD] = build_ms;
@@ -292,11 +320,12 @@ struct Timer {
             float ms = 0;
             HIPCHECK(hipEventElapsedTime(&ms, r.e0, r.e1));
             long long u = r.units;
-            if (r.boxes && r.side == 2) u = (long long)r.boxes->size();
+            const int b1 = r.boxes ? (r.r1 < 0 ? (int)r.boxes->size() : r.r1) : 0;
+            if (r.boxes && r.side == 2) u = b1 - r.r0;
             if (r.boxes && (r.side == 0 || r.side == 1) && nb > 0) {
                 const auto& se = r.side == 0 ? hs : ht;
                 u = 0;
-                for (int b : *r.boxes) u += se[b].y - se[b].x;
+                for (int i = r.r0; i < b1; ++i) u += se[(*r.boxes)[i]].y - se[(*r.boxes)[i]].x;
             }
             t_klaunch.push_back(KLaunch{r.id, r.level, ms, u});
         }
@@ -1511,24 +1540,35 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
             form_thresh[l] = eval_thresh[l] = pw ? (big ? INT_MAX : 1000000) : 0;
         }
 
-    std::vector<int>& slot = W.h_slot;
-    slot.assign(nb, -1);
-    int nslot = 0;
-    if (expansions)
-        for (int b = 0; b < nb; ++b)
-            if (T.lev[b] >= nlevstart) slot[b] = nslot++;
-    Pack pk(W.h_plan);
-    size_t o_child = pk.add(T.ch), o_par = pk.add(T.par), o_cen = pk.add(T.cen);
-    size_t o_ncol = pk.add(T.ncol), o_col = pk.add(T.col), o_rank = pk.add(T.rank_of_box);
-    size_t o_rlo = pk.add(T.rank_lo), o_rhi = pk.add(T.rank_hi), o_slot = pk.add(slot);
-    size_t o_nch = pk.add(T.nch);
-
+    // Expansion storage.  The reference keeps a multipole and a local
+    // expansion for every box from the cutoff level down (fgt3d.f:388-400,
+    // g3dmpalloc fgt3d.f:1685).  Here persistent slots hold only what lives
+    // across levels -- multipole expansions of the merge parents (and of
+    // leaves formed at the top level), local expansions of the top level and
+    // of non-leaf boxes -- and one chunk region is reused by the leaves: on
+    // the way up a chunk of leaves is formed there and merged into its
+    // parents, on the way down a chunk is split into there and evaluated.
+    // All of it lies in one buffer: the multipole table addresses it from
+    // slot 0, the local table from slot n_mp, the chunk region starts at slot
+    // n_mp + n_loc (multipole numbering) = n_loc (local numbering).
+    std::vector<int>& slot_mp = W.h_slot;
+    std::vector<int>& slot_loc = W.h_slot2;
+    slot_mp.assign(nb, -1);
+    slot_loc.assign(nb, -1);
+    int n_mp = 0, n_loc = 0;
     std::vector<std::vector<int>> form_l(nlevels + 1), merge_l(nlevels + 1), split_l(nlevels + 1),
         eval_l(nlevels + 1);
     std::vector<int> m2l_l;
-    std::vector<size_t> o_form(nlevels + 1), o_merge(nlevels + 1), o_split(nlevels + 1),
-        o_eval(nlevels + 1);
-    size_t o_m2l = 0;
+    // chunks: upward (level of the formed leaves, first and last parent in
+    // merge_l[level - 1], first and last leaf in form_l[level]); downward
+    // (level, first and last box in eval_l[level])
+    struct UpChunk { int l, p0, p1, f0, f1; };
+    struct DownChunk { int l, e0, e1; };
+    std::vector<UpChunk> up;
+    std::vector<DownChunk> down;
+    const size_t per = 2 * (size_t)nexp * nd;  // doubles per slot
+    size_t ntay = 1;                             // Taylor buffer, doubles
+    int ncap = 0;                                // chunk region, slots
     if (expansions) {
         std::vector<char> iflocal(nb, 0);
         if (npwlevel >= 0 && npwlevel <= nlevels)
@@ -1537,17 +1577,118 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
                 for (int i = 0; i < T.ncol[b]; ++i)
                     if (T.nch[T.col[27 * b + i]] > 0) { iflocal[b] = 1; break; }
             }
-        for (int l = 0; l <= nlevels; ++l)
+        const int lform = std::max(npwlevel + 1, 0);
+        for (int l = nlevstart; l <= nlevels; ++l)
             for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
-                if (T.nch[b] == 0 && l >= std::max(npwlevel + 1, 0)) form_l[l].push_back(b);
-                if (T.nch[b] > 0 && l >= nlevstart && l <= nlevels - 1) {
+                if (T.nch[b] > 0 && l <= nlevels - 1) {
                     merge_l[l].push_back(b);
-                    for (int j = 0; j < 8; ++j) split_l[l].push_back(T.ch[8 * b + j]);
+                    slot_mp[b] = n_mp++;
+                    for (int j = 0; j < 8; ++j) {
+                        const int c = T.ch[8 * b + j];
+                        if (c >= 0 && T.nch[c] > 0) split_l[l].push_back(c);
+                    }
                 }
-                if (l == nlevstart) m2l_l.push_back(b);
-                if (T.nch[b] == 0 && l >= nlevstart && !(l == npwlevel && !iflocal[b]))
-                    eval_l[l].push_back(b);
+                if (l == nlevstart) {
+                    m2l_l.push_back(b);
+                    if (T.nch[b] == 0 && l >= lform) {
+                        form_l[l].push_back(b);
+                        slot_mp[b] = n_mp++;
+                    }
+                }
+                if (l == nlevstart || T.nch[b] > 0) slot_loc[b] = n_loc++;
+                if (T.nch[b] == 0 && !(l == npwlevel && !iflocal[b])) eval_l[l].push_back(b);
             }
+        // leaves below the top level, grouped by parent
+        for (int l = nlevstart + 1; l <= nlevels; ++l)
+            for (int p : merge_l[l - 1])
+                for (int j = 0; j < 8; ++j) {
+                    const int c = T.ch[8 * p + j];
+                    if (c >= 0 && T.nch[c] == 0) form_l[l].push_back(c);
+                }
+        // memory: persistent slots, then the chunk region and the Taylor
+        // buffer as large as the largest level needs, or as the device allows
+        size_t need_c = 0, need_tay = 1, m3max = 1;
+        for (int l = nlevstart + 1; l <= nlevels; ++l)
+            need_c = std::max({need_c, form_l[l].size(), eval_l[l].size()});
+        for (int l = nlevstart; l <= nlevels; ++l) {
+            if (eval_thresh[l] == INT_MAX || eval_l[l].empty()) continue;
+            const size_t m = nlocal[l] + 1;
+            m3max = std::max(m3max, m * m * m);
+            need_tay = std::max(need_tay, eval_l[l].size() * m * m * m);
+        }
+        size_t fr = 0, tot = 0;
+        HIPCHECK(hipMemGetInfo(&fr, &tot));
+        const double G = 1024.0 * 1024.0 * 1024.0;
+        const double avail = (double)fr + W.held("exp3") + W.held("taylor3") - 1.0 * G - 0.02 * tot;
+        const double pers = 8.0 * per * ((double)n_mp + n_loc);
+        const double full = pers + 8.0 * per * need_c + 8.0 * need_tay;
+        size_t cap_c = need_c, cap_tay = need_tay;
+        if (full > avail) {
+            const double rest = avail - pers;
+            if (rest < 8.0 * per * std::min<size_t>(need_c, 8) + 8.0 * m3max) {
+                char msg[200];
+                std::snprintf(msg, sizeof msg,
+                              "3D expansions need %.1f GiB of device memory at least, %.1f GiB free",
+                              (pers + 8.0 * per * 8 + 8.0 * m3max) / G, avail / G);
+                fail(FGTPW_ENOMEM, msg);
+            }
+            cap_tay = std::min(need_tay, std::max(m3max, (size_t)(0.1 * rest / 8)));
+            cap_c = std::min(need_c, (size_t)((rest - 8.0 * cap_tay) / (8.0 * per)));
+            cap_c = std::max(cap_c, std::min<size_t>(need_c, 8));
+        }
+        // FGTPW_MAX_CHUNK caps the boxes per chunk (at least 8 on the way up,
+        // where chunks hold whole parents): less device memory for more launches
+        size_t cap_env = 0;
+        if (const char* e = std::getenv("FGTPW_MAX_CHUNK")) cap_env = std::max(0L, std::atol(e));
+        if (cap_env > 0) cap_c = std::min(cap_c, std::max<size_t>(cap_env, std::min<size_t>(need_c, 8)));
+        ncap = (int)cap_c;
+        ntay = cap_tay;
+        // upward chunks: whole parents, at most ncap leaves
+        for (int l = nlevstart + 1; l <= nlevels; ++l) {
+            const std::vector<int>& P = merge_l[l - 1];
+            int p0 = 0, f0 = 0, f = 0;
+            for (int i = 0; i < (int)P.size(); ++i) {
+                int k = 0;
+                for (int j = 0; j < 8; ++j) {
+                    const int c = T.ch[8 * P[i] + j];
+                    k += c >= 0 && T.nch[c] == 0;
+                }
+                if (f - f0 + k > ncap) {
+                    up.push_back({l, p0, i, f0, f});
+                    p0 = i;
+                    f0 = f;
+                }
+                for (int j = 0; j < k; ++j, ++f) slot_mp[form_l[l][f]] = n_mp + n_loc + (f - f0);
+            }
+            if (!P.empty()) up.push_back({l, p0, (int)P.size(), f0, f});
+        }
+        // downward chunks: at most ncap leaves below the top level, and at
+        // most the Taylor buffer's worth of boxes
+        for (int l = nlevstart; l <= nlevels; ++l) {
+            const size_t n = eval_l[l].size();
+            if (n == 0) continue;
+            const size_t m = nlocal[l] + 1;
+            size_t cmax = eval_thresh[l] == INT_MAX ? n : std::max<size_t>(1, ntay / (m * m * m));
+            if (l > nlevstart) cmax = std::min<size_t>(cmax, ncap);
+            if (cap_env > 0) cmax = std::min(cmax, cap_env);
+            for (size_t e0 = 0; e0 < n; e0 += cmax) {
+                const size_t e1 = std::min(n, e0 + cmax);
+                down.push_back({l, (int)e0, (int)e1});
+                if (l > nlevstart)
+                    for (size_t i = e0; i < e1; ++i) slot_loc[eval_l[l][i]] = n_loc + (int)(i - e0);
+            }
+        }
+    }
+    Pack pk(W.h_plan);
+    size_t o_child = pk.add(T.ch), o_par = pk.add(T.par), o_cen = pk.add(T.cen);
+    size_t o_ncol = pk.add(T.ncol), o_col = pk.add(T.col), o_rank = pk.add(T.rank_of_box);
+    size_t o_rlo = pk.add(T.rank_lo), o_rhi = pk.add(T.rank_hi);
+    size_t o_smp = pk.add(slot_mp), o_sloc = pk.add(slot_loc);
+    size_t o_nch = pk.add(T.nch);
+    std::vector<size_t> o_form(nlevels + 1), o_merge(nlevels + 1), o_split(nlevels + 1),
+        o_eval(nlevels + 1);
+    size_t o_m2l = 0;
+    if (expansions) {
         for (int l = 0; l <= nlevels; ++l) {
             o_form[l] = pk.add(form_l[l]);
             o_merge[l] = pk.add(merge_l[l]);
@@ -1654,7 +1795,8 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     const int* d_rank = (const int*)D(o_rank);
     const int* d_rlo = (const int*)D(o_rlo);
     const int* d_rhi = (const int*)D(o_rhi);
-    const int* d_slot = (const int*)D(o_slot);
+    const int* d_smp = (const int*)D(o_smp);
+    const int* d_sloc = (const int*)D(o_sloc);
     const int* d_nch = (const int*)D(o_nch);
 
     // ---------------- point ranges of the boxes ----------------
@@ -1693,11 +1835,9 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
 
     // ---------------- expansions: steps 1-5 ----------------
     if (expansions) {
-        const size_t per = 2 * (size_t)nexp * nd;
-        double* mp = W.get<double>("mp", per * nslot);
-        double* lc = W.get<double>("loc", per * nslot);
-        launch_zero(mp, per * nslot, s);
-        launch_zero(lc, per * nslot, s);
+        double* xb = W.get<double>("exp3", per * ((size_t)n_mp + n_loc + ncap));
+        double* mp = xb;                       // multipole slots
+        double* lc = xb + per * (size_t)n_mp;  // local slots (the chunk region follows)
         const int GRID3 = 1024, SEGE = 256, GRIDMAX = 4096;
         // segment scans of the evaluation lists
         std::vector<int*> ts_trg(nlevels + 1, nullptr), ts_src(nlevels + 1, nullptr);
@@ -1731,23 +1871,15 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
             sstride = std::max(sstride, m * m * m + 2 * m * m * half);
         }
         double* fscr = W.get<double>("form3_scratch", (size_t)GRID3 * std::max(sstride, 1L));
-        size_t maxtay = 0;
-        for (int l = nlevstart; l <= nlevels; ++l) {
-            if (eval_thresh[l] == INT_MAX) continue;
-            size_t m = nlocal[l] + 1;
-            maxtay = std::max(maxtay, eval_l[l].size() * m * m * m);
-        }
-        double* taylor = W.get<double>("taylor3", std::max<size_t>(maxtay, 1));
-        for (int ind = 0; ind < nd; ++ind) {
-            tm.begin(S_FORM);
-            for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l) {
-                if (form_l[l].empty()) continue;
+        double* taylor = W.get<double>("taylor3", ntay);
+        auto form_level = [&](int l, int f0, int f1) {
+            for (int ind = 0; ind < nd; ++ind) {
                 Form3Args fa{};
-                fa.boxes = (const int*)D(o_form[l]);
-                fa.nbox = (int)form_l[l].size();
+                fa.boxes = (const int*)D(o_form[l]) + f0;
+                fa.nbox = f1 - f0;
                 fa.cen = d_cen;
                 fa.src_se = src_se;
-                fa.slot = d_slot;
+                fa.slot = d_smp;
                 fa.xs = xs;
                 fa.qs = qs;
                 fa.nd = nd;
@@ -1765,74 +1897,111 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
                 fa.grid = std::min(GRID3, fa.nbox);
                 fa.thresh = form_thresh[l];
                 fa.dipw = wsd;
-                tm.ctx(l, &form_l[l], 0);
+                tm.ctx(l, &form_l[l], 0, 0, f0, f1);
+                if (fa.nbox <= 0) continue;
                 if (fa.thresh != INT_MAX) launch_form3_hermite(fa, s);
                 if (fa.thresh > 0) launch_form3_pw(fa, s);
             }
-            tm.end(S_FORM);
+        };
+        // steps 1-2: leaves of the top level, then chunks of leaves formed
+        // and merged into their parents, finest level first
+        tm.begin(S_FORM);
+        if (!form_l[nlevstart].empty()) form_level(nlevstart, 0, (int)form_l[nlevstart].size());
+        tm.end(S_FORM);
+        for (int ci = (int)up.size() - 1; ci >= 0;) {
+            const int l = up[ci].l;
+            int cj = ci;
+            while (cj >= 0 && up[cj].l == l) --cj;
+            // chunks of level l are up[cj+1 .. ci]
+            for (int k = cj + 1; k <= ci; ++k) {
+                const UpChunk& u = up[k];
+                tm.begin(S_FORM);
+                form_level(l, u.f0, u.f1);
+                tm.end(S_FORM);
+                tm.begin(S_MERGE);
+                tm.ctx(l - 1, &merge_l[l - 1], 2, 0, u.p0, u.p1);
+                launch_merge3((const int*)D(o_merge[l - 1]) + u.p0, u.p1 - u.p0, d_child, d_cen,
+                              src_se, d_smp, tab.wms, 8 * (nlevels - l), nd, nexp, mp, s);
+                tm.end(S_MERGE);
+            }
+            ci = cj;
         }
-        tm.begin(S_MERGE);
-        for (int l = nlevels - 1; l >= nlevstart; --l)
-            tm.ctx(l, &merge_l[l], 2), launch_merge3((const int*)D(o_merge[l]), (int)merge_l[l].size(),
-                                                     d_child, d_cen, src_se, d_slot, tab.wms,
-                                                     8 * (nlevels - l - 1), nd, nexp, mp, s);
-        tm.end(S_MERGE);
         tm.begin(S_M2L);
         tm.ctx(nlevstart, &m2l_l, 2);
         launch_m2l3((const int*)D(o_m2l), (int)m2l_l.size(), d_cen, d_ncol, d_col, d_nch, src_se,
-                    trg_se, use_src, use_trg, d_slot, tab.wm2l, T.bsz[nlevstart],
+                    trg_se, use_src, use_trg, d_smp, d_sloc, tab.wm2l, T.bsz[nlevstart],
                     nlevstart == npwlevel, nd, nexp, mp, lc, s);
         tm.end(S_M2L);
-        tm.begin(S_SPLIT);
-        for (int l = nlevstart; l <= nlevels - 1; ++l)
-            tm.ctx(l, &split_l[l], 2), launch_split3((const int*)D(o_split[l]), (int)split_l[l].size(),
-                                                     d_par, d_cen, d_slot, tab.wms,
-                                                     8 * (nlevels - l - 1), nd, nexp, lc, s);
-        tm.end(S_SPLIT);
-        tm.begin(S_EVAL);
-        for (int ind = 0; ind < nd; ++ind)
-            for (int l = nlevstart; l <= nlevels; ++l) {
-                if (eval_l[l].empty()) continue;
-                Eval3Args ea{};
-                ea.boxes = (const int*)D(o_eval[l]);
-                ea.nbox = (int)eval_l[l].size();
-                ea.cen = d_cen;
-                ea.slot = d_slot;
-                ea.nd = nd;
-                ea.ind = ind;
-                ea.dsq = dsq;
-                ea.nl = nlocal[l];
-                ea.dcoef = tab.dcoef;
-                ea.cstride = npw;
-                ea.ts = tab.ts;
-                ea.npw = npw;
-                ea.loc = lc;
-                ea.taylor = taylor;
-                ea.rs = src_se;
-                ea.rt = trg_se;
-                ea.thresh = eval_thresh[l];
-                ea.grid = std::min(2048, ea.nbox);
-                tm.ctx(l, &eval_l[l], 2);
-                if (ea.thresh != INT_MAX) launch_pw2local3(ea, s);
-                for (int side = 0; side < 2; ++side) {
-                    if (side == 0 && !use_trg) continue;
-                    if (side == 1 && !use_src) continue;
-                    Eval3Args eb = ea;
-                    eb.tstart = side == 0 ? ts_trg[l] : ts_src[l];
-                    eb.seg = SEGE;
-                    eb.grid = (int)std::min<long>(GRIDMAX, ea.nbox + (side == 0 ? nt : ns) / SEGE + 1);
-                    eb.se = side == 0 ? trg_se : src_se;
-                    eb.pts = side == 0 ? xt : xs;
-                    eb.ifpgh = side == 0 ? q.ifpghtarg : q.ifpgh;
-                    eb.pot = side == 0 ? pt : ps;
-                    eb.grad = side == 0 ? gt : gs;
-                    eb.hess = side == 0 ? ht : hs;
-                    tm.ctx(l, &eval_l[l], side == 0 ? 1 : 0);
-                    if (eb.thresh != INT_MAX) launch_taylor_eval3(eb, s);
-                    if (eb.thresh > 0) launch_pw_eval3(eb, s);
-                }
+        // steps 4-5 by level: non-leaf children get their persistent local
+        // expansions, then chunks of leaves are split into the chunk region
+        // (below the top level) and evaluated
+        size_t di = 0;
+        for (int l = nlevstart; l <= nlevels; ++l) {
+            if (l > nlevstart) {
+                tm.begin(S_SPLIT);
+                tm.ctx(l - 1, &split_l[l - 1], 2);
+                launch_split3((const int*)D(o_split[l - 1]), (int)split_l[l - 1].size(), d_par,
+                              d_cen, d_sloc, tab.wms, 8 * (nlevels - l), nd, nexp, lc, src_se,
+                              trg_se, use_src, use_trg, s);
+                tm.end(S_SPLIT);
             }
-        tm.end(S_EVAL);
+            for (; di < down.size() && down[di].l == l; ++di) {
+                const DownChunk& dc = down[di];
+                const int* boxes = (const int*)D(o_eval[l]) + dc.e0;
+                const int nbox = dc.e1 - dc.e0;
+                if (l > nlevstart) {
+                    tm.begin(S_SPLIT);
+                    tm.ctx(l - 1, &eval_l[l], 2, 0, dc.e0, dc.e1);
+                    launch_split3(boxes, nbox, d_par, d_cen, d_sloc, tab.wms, 8 * (nlevels - l), nd,
+                                  nexp, lc, src_se, trg_se, use_src, use_trg, s);
+                    tm.end(S_SPLIT);
+                }
+                tm.begin(S_EVAL);
+                for (int ind = 0; ind < nd; ++ind) {
+                    Eval3Args ea{};
+                    ea.boxes = boxes;
+                    ea.nbox = nbox;
+                    ea.cen = d_cen;
+                    ea.slot = d_sloc;
+                    ea.nd = nd;
+                    ea.ind = ind;
+                    ea.dsq = dsq;
+                    ea.nl = nlocal[l];
+                    ea.dcoef = tab.dcoef;
+                    ea.cstride = npw;
+                    ea.ts = tab.ts;
+                    ea.npw = npw;
+                    ea.loc = lc;
+                    ea.taylor = taylor;
+                    ea.rs = src_se;
+                    ea.rt = trg_se;
+                    ea.use_src = use_src;
+                    ea.use_trg = use_trg;
+                    ea.thresh = eval_thresh[l];
+                    ea.grid = std::min(2048, ea.nbox);
+                    tm.ctx(l, &eval_l[l], 2, 0, dc.e0, dc.e1);
+                    if (ea.thresh != INT_MAX) launch_pw2local3(ea, s);
+                    for (int side = 0; side < 2; ++side) {
+                        if (side == 0 && !use_trg) continue;
+                        if (side == 1 && !use_src) continue;
+                        Eval3Args eb = ea;
+                        eb.tstart = (side == 0 ? ts_trg[l] : ts_src[l]) + dc.e0;
+                        eb.seg = SEGE;
+                        eb.grid = (int)std::min<long>(GRIDMAX, ea.nbox + (side == 0 ? nt : ns) / SEGE + 1);
+                        eb.se = side == 0 ? trg_se : src_se;
+                        eb.pts = side == 0 ? xt : xs;
+                        eb.ifpgh = side == 0 ? q.ifpghtarg : q.ifpgh;
+                        eb.pot = side == 0 ? pt : ps;
+                        eb.grad = side == 0 ? gt : gs;
+                        eb.hess = side == 0 ? ht : hs;
+                        tm.ctx(l, &eval_l[l], side == 0 ? 1 : 0, 0, dc.e0, dc.e1);
+                        if (eb.thresh != INT_MAX) launch_taylor_eval3(eb, s);
+                        if (eb.thresh > 0) launch_pw_eval3(eb, s);
+                    }
+                }
+                tm.end(S_EVAL);
+            }
+        }
     }
 
     // ---------------- step 6: near field, dmax = ln(1/eps) delta (fgt3d.f:1466) ----------------

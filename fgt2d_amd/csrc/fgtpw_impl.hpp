@@ -287,10 +287,11 @@ void launch_merge3(const int* parents, int np, const int* child8, const double* 
                    long nexp, double* mp, hipStream_t s);
 void launch_m2l3(const int* boxes, int nb, const double* cen, const int* ncol, const int* col27,
                  const int* nch, const int2* src_se, const int2* trg_se, int use_src, int use_trg,
-                 const int* slot, const double* wm2l, double xmin, int do_list, int nd, long nexp,
-                 const double* mp, double* loc, hipStream_t s);
+                 const int* slot_mp, const int* slot_loc, const double* wm2l, double xmin,
+                 int do_list, int nd, long nexp, const double* mp, double* loc, hipStream_t s);
 void launch_split3(const int* children, int nc, const int* par, const double* cen,
                    const int* slot, const double* wms, int kbase, int nd, long nexp, double* loc,
+                   const int2* src_se, const int2* trg_se, int use_src, int use_trg,
                    hipStream_t s);
 
 struct Eval3Args {
@@ -311,12 +312,16 @@ struct Eval3Args {
     const double* loc;      // PW local expansions per slot
     double* taylor;         // (nbox, (nl+1)^3)
     double *pot, *grad, *hess;
-    const int* tstart;      // segments of the point ranges (evaluation kernels)
+    // segments of the point ranges (evaluation kernels): box i of the list
+    // owns segments tstart[i] .. tstart[i+1]-1; tstart[0] need not be 0, so a
+    // sub-list of a scanned list is passed by offsetting boxes and tstart
+    const int* tstart;
     int seg, grid;
     // route: boxes whose source + target count is below thresh are evaluated
     // from the plane waves directly, the others through Taylor
     const int2 *rs, *rt;
     int thresh;
+    int use_src, use_trg;   // which points are evaluated (PW -> Taylor skips boxes with none)
 };
 void launch_pw2local3(const Eval3Args& a, hipStream_t s);
 void launch_taylor_eval3(const Eval3Args& a, hipStream_t s);

@@ -27,6 +27,13 @@ static inline unsigned gridn(long n, int bs)
     return (unsigned)(g < 1 ? 1 : g);
 }
 
+// grid of a grid-stride kernel over n items: at most 2^20 blocks
+static inline unsigned gridcap(long n, int bs)
+{
+    long g = (n + bs - 1) / bs;
+    return (unsigned)(g < 1 ? 1 : (g > (1L << 20) ? (1L << 20) : g));
+}
+
 // ---------------------------------------------------------------------------
 // bounding box (pts_tree_boxsize0, pts_tree3d.f:1597-1680): mins and maxes
 // of x, y, z over sources and targets, plus a non-finite flag
@@ -549,27 +556,29 @@ __global__ __launch_bounds__(256) void k_merge3(const int* __restrict__ parents,
                                                 const double* __restrict__ wms, int kbase, int nd,
                                                 long nexp, double* __restrict__ mp)
 {
-    long t = blockIdx.x * 256L + threadIdx.x;
-    long pi = t / nexp, e = t - pi * nexp;
-    if (pi >= np) return;
-    int p = parents[pi];
-    for (int ind = 0; ind < nd; ++ind) {
-        double* dst = mp + 2 * (((long)slot[p] * nd + ind) * nexp + e);
-        double re = dst[0], im = dst[1];
-        for (int j = 0; j < 8; ++j) {
-            int c = child[8 * p + j];
-            if (c < 0) continue;
-            int2 r = src_se[c];
-            if (r.y - r.x <= 0) continue;
-            int k = octant(cen[3 * p] - cen[3 * c], cen[3 * p + 1] - cen[3 * c + 1],
-                           cen[3 * p + 2] - cen[3 * c + 2]);
-            const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
-            const double* v = mp + 2 * (((long)slot[c] * nd + ind) * nexp + e);
-            re += v[0] * w[0] - v[1] * w[1];
-            im += v[0] * w[1] + v[1] * w[0];
+    for (long t = blockIdx.x * 256L + threadIdx.x; t < (long)np * nexp; t += gridDim.x * 256L) {
+        const long pi = t / nexp, e = t - pi * nexp;
+        int p = parents[pi];
+        for (int ind = 0; ind < nd; ++ind) {
+            // a parent's expansion is the sum of its children's: written, not
+            // accumulated, so the slot needs no clearing beforehand
+            double* dst = mp + 2 * (((long)slot[p] * nd + ind) * nexp + e);
+            double re = 0, im = 0;
+            for (int j = 0; j < 8; ++j) {
+                int c = child[8 * p + j];
+                if (c < 0) continue;
+                int2 r = src_se[c];
+                if (r.y - r.x <= 0) continue;
+                int k = octant(cen[3 * p] - cen[3 * c], cen[3 * p + 1] - cen[3 * c + 1],
+                               cen[3 * p + 2] - cen[3 * c + 2]);
+                const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
+                const double* v = mp + 2 * (((long)slot[c] * nd + ind) * nexp + e);
+                re += v[0] * w[0] - v[1] * w[1];
+                im += v[0] * w[1] + v[1] * w[0];
+            }
+            dst[0] = re;
+            dst[1] = im;
         }
-        dst[0] = re;
-        dst[1] = im;
     }
 }
 
@@ -579,15 +588,17 @@ void launch_merge3(const int* parents, int np, const int* child8, const double* 
 {
     if (np <= 0) return;
     kmark(KID_MERGE, false, s);
-    k_merge3<<<gridn((long)np * nexp, 256), 256, 0, s>>>(parents, np, child8, cen, src_se, slot,
+    k_merge3<<<gridcap((long)np * nexp, 256), 256, 0, s>>>(parents, np, child8, cen, src_se, slot,
                                                          wms, kbase, nd, nexp, mp);
     kmark(KID_MERGE, true, s);
 }
 
 // Step 3 (fgt3d.f:1162-1210): at the cutoff level, loc = sum over colleagues
 // j != self that have children (gt3d_computelistpw, tree_routs3d.f:695) of
-// mp_j shifted by (jx, jy, jz) boxes, then + mp_self; only boxes holding
-// evaluation points.
+// mp_j shifted by (jx, jy, jz) boxes, then + mp_self; boxes without
+// evaluation points get a zero expansion.  Multipole and local expansions
+// have separate slot tables; a box without a multipole slot (slot_mp < 0)
+// has a zero multipole expansion.
 __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int nb,
                                               const double* __restrict__ cen,
                                               const int* __restrict__ ncol,
@@ -595,85 +606,101 @@ __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int
                                               const int* __restrict__ nch,
                                               const int2* __restrict__ src_se,
                                               const int2* __restrict__ trg_se, int use_src,
-                                              int use_trg, const int* __restrict__ slot,
+                                              int use_trg, const int* __restrict__ slot_mp,
+                                              const int* __restrict__ slot_loc,
                                               const double* __restrict__ wm2l, double xmin,
                                               int do_list, int nd, long nexp,
                                               const double* __restrict__ mp,
                                               double* __restrict__ loc)
 {
-    long t = blockIdx.x * 256L + threadIdx.x;
-    long bi = t / nexp, e = t - bi * nexp;
-    if (bi >= nb) return;
-    int b = boxes[bi];
-    int npts = 0;
-    if (use_trg) npts += trg_se[b].y - trg_se[b].x;
-    if (use_src) npts += src_se[b].y - src_se[b].x;
-    if (npts <= 0) return;
-    for (int ind = 0; ind < nd; ++ind) {
-        double* d = loc + 2 * (((long)slot[b] * nd + ind) * nexp + e);
-        double re = d[0], im = d[1];
-        if (do_list)
-            for (int i = 0; i < ncol[b]; ++i) {
-                int j = col[27 * b + i];
-                if (j == b || nch[j] <= 0) continue;
-                int jx = (int)rint((cen[3 * b] - cen[3 * j]) / xmin);
-                int jy = (int)rint((cen[3 * b + 1] - cen[3 * j + 1]) / xmin);
-                int jz = (int)rint((cen[3 * b + 2] - cen[3 * j + 2]) / xmin);
-                const double* w = wm2l + 2 * ((long)((jx + 1) + 3 * (jy + 1) + 9 * (jz + 1)) * nexp + e);
-                const double* v = mp + 2 * (((long)slot[j] * nd + ind) * nexp + e);
-                re += v[0] * w[0] - v[1] * w[1];
-                im += v[0] * w[1] + v[1] * w[0];
+    for (long t = blockIdx.x * 256L + threadIdx.x; t < (long)nb * nexp; t += gridDim.x * 256L) {
+        const long bi = t / nexp, e = t - bi * nexp;
+        int b = boxes[bi];
+        int npts = 0;
+        if (use_trg) npts += trg_se[b].y - trg_se[b].x;
+        if (use_src) npts += src_se[b].y - src_se[b].x;
+        for (int ind = 0; ind < nd; ++ind) {
+            double* d = loc + 2 * (((long)slot_loc[b] * nd + ind) * nexp + e);
+            double re = 0, im = 0;
+            if (npts > 0 && do_list)
+                for (int i = 0; i < ncol[b]; ++i) {
+                    int j = col[27 * b + i];
+                    if (j == b || nch[j] <= 0 || slot_mp[j] < 0) continue;
+                    int jx = (int)rint((cen[3 * b] - cen[3 * j]) / xmin);
+                    int jy = (int)rint((cen[3 * b + 1] - cen[3 * j + 1]) / xmin);
+                    int jz = (int)rint((cen[3 * b + 2] - cen[3 * j + 2]) / xmin);
+                    const double* w = wm2l + 2 * ((long)((jx + 1) + 3 * (jy + 1) + 9 * (jz + 1)) * nexp + e);
+                    const double* v = mp + 2 * (((long)slot_mp[j] * nd + ind) * nexp + e);
+                    re += v[0] * w[0] - v[1] * w[1];
+                    im += v[0] * w[1] + v[1] * w[0];
+                }
+            if (npts > 0 && slot_mp[b] >= 0) {
+                const double* v = mp + 2 * (((long)slot_mp[b] * nd + ind) * nexp + e);
+                re += v[0];
+                im += v[1];
             }
-        const double* v = mp + 2 * (((long)slot[b] * nd + ind) * nexp + e);
-        d[0] = re + v[0];
-        d[1] = im + v[1];
+            d[0] = re;
+            d[1] = im;
+        }
     }
 }
 
 void launch_m2l3(const int* boxes, int nb, const double* cen, const int* ncol, const int* col27,
                  const int* nch, const int2* src_se, const int2* trg_se, int use_src, int use_trg,
-                 const int* slot, const double* wm2l, double xmin, int do_list, int nd, long nexp,
-                 const double* mp, double* loc, hipStream_t s)
+                 const int* slot_mp, const int* slot_loc, const double* wm2l, double xmin,
+                 int do_list, int nd, long nexp, const double* mp, double* loc, hipStream_t s)
 {
     if (nb <= 0) return;
     kmark(KID_M2L, false, s);
-    k_m2l3<<<gridn((long)nb * nexp, 256), 256, 0, s>>>(boxes, nb, cen, ncol, col27, nch, src_se,
-                                                       trg_se, use_src, use_trg, slot, wm2l, xmin,
-                                                       do_list, nd, nexp, mp, loc);
+    k_m2l3<<<gridcap((long)nb * nexp, 256), 256, 0, s>>>(boxes, nb, cen, ncol, col27, nch, src_se,
+                                                       trg_se, use_src, use_trg, slot_mp, slot_loc,
+                                                       wm2l, xmin, do_list, nd, nexp, mp, loc);
     kmark(KID_M2L, true, s);
 }
 
-// Step 4 (fgt3d.f:1215-1276): loc(child) += loc(parent) shifted to the child.
+// Step 4 (fgt3d.f:1215-1276): loc(child) = loc(parent) shifted to the child
+// (the only contribution below the cutoff level, so written, not
+// accumulated).  Children without evaluation points in their box -- and so
+// in their subtree -- are skipped: nothing reads their expansion.
 __global__ __launch_bounds__(256) void k_split3(const int* __restrict__ children, int nc,
                                                 const int* __restrict__ par,
                                                 const double* __restrict__ cen,
                                                 const int* __restrict__ slot,
                                                 const double* __restrict__ wms, int kbase, int nd,
-                                                long nexp, double* __restrict__ loc)
+                                                long nexp, double* __restrict__ loc,
+                                                const int2* __restrict__ src_se,
+                                                const int2* __restrict__ trg_se, int use_src,
+                                                int use_trg)
 {
-    long t = blockIdx.x * 256L + threadIdx.x;
-    long ci = t / nexp, e = t - ci * nexp;
-    if (ci >= nc) return;
-    int c = children[ci], p = par[c];
-    int k = octant(cen[3 * c] - cen[3 * p], cen[3 * c + 1] - cen[3 * p + 1],
-                   cen[3 * c + 2] - cen[3 * p + 2]);
-    const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
-    for (int ind = 0; ind < nd; ++ind) {
-        const double* v = loc + 2 * (((long)slot[p] * nd + ind) * nexp + e);
-        double* d = loc + 2 * (((long)slot[c] * nd + ind) * nexp + e);
-        d[0] += v[0] * w[0] - v[1] * w[1];
-        d[1] += v[0] * w[1] + v[1] * w[0];
+    for (long t = blockIdx.x * 256L + threadIdx.x; t < (long)nc * nexp; t += gridDim.x * 256L) {
+        const long ci = t / nexp, e = t - ci * nexp;
+        int c = children[ci], p = par[c];
+        int npts = 0;
+        if (use_trg) npts += trg_se[c].y - trg_se[c].x;
+        if (use_src) npts += src_se[c].y - src_se[c].x;
+        if (npts <= 0) continue;
+        int k = octant(cen[3 * c] - cen[3 * p], cen[3 * c + 1] - cen[3 * p + 1],
+                       cen[3 * c + 2] - cen[3 * p + 2]);
+        const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
+        for (int ind = 0; ind < nd; ++ind) {
+            const double* v = loc + 2 * (((long)slot[p] * nd + ind) * nexp + e);
+            double* d = loc + 2 * (((long)slot[c] * nd + ind) * nexp + e);
+            d[0] = v[0] * w[0] - v[1] * w[1];
+            d[1] = v[0] * w[1] + v[1] * w[0];
+        }
     }
 }
 
 void launch_split3(const int* children, int nc, const int* par, const double* cen,
                    const int* slot, const double* wms, int kbase, int nd, long nexp, double* loc,
+                   const int2* src_se, const int2* trg_se, int use_src, int use_trg,
                    hipStream_t s)
 {
     if (nc <= 0) return;
     kmark(KID_SPLIT, false, s);
-    k_split3<<<gridn((long)nc * nexp, 256), 256, 0, s>>>(children, nc, par, cen, slot, wms, kbase,
-                                                         nd, nexp, loc);
+    k_split3<<<gridcap((long)nc * nexp, 256), 256, 0, s>>>(children, nc, par, cen, slot, wms, kbase,
+                                                         nd, nexp, loc, src_se, trg_se, use_src,
+                                                         use_trg);
     kmark(KID_SPLIT, true, s);
 }
 
@@ -695,6 +722,15 @@ __device__ __forceinline__ bool pw_box(const Eval3Args& a, int b)
     return n < a.thresh;
 }
 
+// true when box b holds points where the potential is wanted
+__device__ __forceinline__ bool has_eval(const Eval3Args& a, int b)
+{
+    int n = 0;
+    if (a.use_src) n += a.rs[b].y - a.rs[b].x;
+    if (a.use_trg) n += a.rt[b].y - a.rt[b].x;
+    return n > 0;
+}
+
 template <int MM>
 __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
 {
@@ -709,7 +745,7 @@ __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
     }
     for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
         const int box = a.boxes[li];
-        if (pw_box(a, box)) continue;
+        if (pw_box(a, box) || !has_eval(a, box)) continue;
         const double* pw = a.loc + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
         double* L = a.taylor + (long)li * m * m * m;
         for (int p0 = 0; p0 < m * m; p0 += 256) {
@@ -813,7 +849,7 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
     }
     for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
         const int box = a.boxes[li];
-        if (pw_box(a, box)) continue;
+        if (pw_box(a, box) || !has_eval(a, box)) continue;
         const double* pw = a.loc + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
         double L[4][16];
 #pragma unroll
@@ -920,7 +956,7 @@ __global__ __launch_bounds__(256) void k_taylor_eval3(Eval3Args a)
 {
     const int nl = a.nl, m = nl + 1, t = threadIdx.x;
     const int total = a.tstart[a.nbox];
-    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    for (int g = a.tstart[0] + blockIdx.x; g < total; g += gridDim.x) {
         int lo = 0, hi = a.nbox - 1;
         while (lo < hi) {
             int mid = (lo + hi + 1) >> 1;
@@ -1027,7 +1063,7 @@ __global__ __launch_bounds__(256) void k_pw_eval3(Eval3Args a)
     const long nexp = (long)npw * npw * half;
     const double h = a.ts[1] - a.ts[0];
     const int total = a.tstart[a.nbox];
-    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    for (int g = a.tstart[0] + blockIdx.x; g < total; g += gridDim.x) {
         int lo = 0, hi = a.nbox - 1;
         while (lo < hi) {
             int mid = (lo + hi + 1) >> 1;

@@ -118,3 +118,23 @@ def test_fgt3d_device_entry_point_matches_host_entry_point():
     torch.cuda.synchronize()
     assert np.array_equal(h[0], d[0].cpu().numpy())
     assert np.array_equal(h[3], d[3].cpu().numpy())
+
+
+@pytest.mark.parametrize("cap", ["8", "37"])
+def test_chunked_expansions_match_one_chunk(cap, monkeypatch):
+    """Leaves formed, merged, split and evaluated in chunks of at most `cap`
+    boxes (FGTPW_MAX_CHUNK) give the same bits as one chunk per level: the
+    chunk region only changes where a leaf's expansion lives, not the order
+    of any sum."""
+    src, chg, tg = _pts.dlaran_points3d("shell", 50000, 4000, 2)
+    for ifs, ift in ((2, 1), (1, 3)):
+        monkeypatch.delenv("FGTPW_MAX_CHUNK", raising=False)
+        one = fgt2d_amd.fgt3d(5e-3, 1e-6, src, chg, tg, ifpgh=ifs, ifpghtarg=ift)
+        st = fgt2d_amd.last_stats()
+        assert st["nlevels"] >= st["npwlevel"] + 2, st  # leaves below the top level
+        monkeypatch.setenv("FGTPW_MAX_CHUNK", cap)
+        print("cap", cap, "ifpgh", ifs, "ifpghtarg", ift, flush=True)
+        got = fgt2d_amd.fgt3d(5e-3, 1e-6, src, chg, tg, ifpgh=ifs, ifpghtarg=ift)
+        for k in range(6):
+            if (k < 3 and k < ifs) or (k >= 3 and k - 3 < ift):
+                assert np.array_equal(got[k], one[k]), NAMES[k]
