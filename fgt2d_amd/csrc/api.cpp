@@ -1612,9 +1612,10 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
             need_c = std::max({need_c, form_l[l].size(), eval_l[l].size()});
         for (int l = nlevstart; l <= nlevels; ++l) {
             if (eval_thresh[l] == INT_MAX || eval_l[l].empty()) continue;
-            const size_t m = nlocal[l] + 1;
-            m3max = std::max(m3max, m * m * m);
-            need_tay = std::max(need_tay, eval_l[l].size() * m * m * m);
+            const int m = nlocal[l] + 1;
+            const size_t st = taylor3_stride(m, m <= 32);
+            m3max = std::max(m3max, st);
+            need_tay = std::max(need_tay, eval_l[l].size() * st);
         }
         size_t fr = 0, tot = 0;
         HIPCHECK(hipMemGetInfo(&fr, &tot));
@@ -1667,8 +1668,9 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         for (int l = nlevstart; l <= nlevels; ++l) {
             const size_t n = eval_l[l].size();
             if (n == 0) continue;
-            const size_t m = nlocal[l] + 1;
-            size_t cmax = eval_thresh[l] == INT_MAX ? n : std::max<size_t>(1, ntay / (m * m * m));
+            const int m = nlocal[l] + 1;
+            const size_t st = taylor3_stride(m, m <= 32);
+            size_t cmax = eval_thresh[l] == INT_MAX ? n : std::max<size_t>(1, ntay / st);
             if (l > nlevstart) cmax = std::min<size_t>(cmax, ncap);
             if (cap_env > 0) cmax = std::min(cmax, cap_env);
             for (size_t e0 = 0; e0 < n; e0 += cmax) {
@@ -1978,6 +1980,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
                     ea.use_src = use_src;
                     ea.use_trg = use_trg;
                     ea.thresh = eval_thresh[l];
+                    ea.tlayout = nlocal[l] + 1 <= 32;
                     ea.grid = std::min(2048, ea.nbox);
                     tm.ctx(l, &eval_l[l], 2, 0, dc.e0, dc.e1);
                     if (ea.thresh != INT_MAX) launch_pw2local3(ea, s);

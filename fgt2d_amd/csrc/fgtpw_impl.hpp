@@ -322,7 +322,15 @@ struct Eval3Args {
     const int2 *rs, *rt;
     int thresh;
     int use_src, use_trg;   // which points are evaluated (PW -> Taylor skips boxes with none)
+    // Taylor coefficient layout: 0 L[j3 + m (j2 + m j1)]; 1 (m <= 32) the
+    // transposed Lt[j2 + MP (j3 + m j1)], MP = m rounded up to 16, zero columns
+    int tlayout;
 };
+// doubles per box of the Taylor buffer
+__host__ __device__ inline long taylor3_stride(int m, int tlayout)
+{
+    return tlayout ? (long)(16 * ((m + 15) / 16)) * m * m : (long)m * m * m;
+}
 void launch_pw2local3(const Eval3Args& a, hipStream_t s);
 void launch_taylor_eval3(const Eval3Args& a, hipStream_t s);
 void launch_pw_eval3(const Eval3Args& a, hipStream_t s);

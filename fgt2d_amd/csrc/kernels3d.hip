@@ -747,7 +747,13 @@ __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
         const int box = a.boxes[li];
         if (pw_box(a, box) || !has_eval(a, box)) continue;
         const double* pw = a.loc + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
-        double* L = a.taylor + (long)li * m * m * m;
+        double* L = a.taylor + (long)li * taylor3_stride(m, a.tlayout);
+        const int MP = 16 * ((m + 15) / 16);
+        if (a.tlayout)  // padding columns j2 = m..MP-1 of the transposed layout
+            for (int o = t; o < (MP - m) * m * m; o += 256) {
+                const int j2 = m + o % (MP - m), r = o / (MP - m);
+                L[j2 + (long)MP * r] = 0;
+            }
         for (int p0 = 0; p0 < m * m; p0 += 256) {
             const int pr = p0 + t, j3 = pr % m, j2 = pr / m;
             const bool act = pr < m * m;
@@ -790,7 +796,12 @@ __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
             if (act)
 #pragma unroll
                 for (int j1 = 0; j1 < MM; ++j1)
-                    if (j1 < m) L[j3 + m * (j2 + (long)m * j1)] = acc[j1];
+                    if (j1 < m) {
+                        if (a.tlayout)
+                            L[j2 + (long)MP * (j3 + (long)m * j1)] = acc[j1];
+                        else
+                            L[j3 + m * (j2 + (long)m * j1)] = acc[j1];
+                    }
         }
         __syncthreads();
     }
@@ -912,16 +923,132 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
                 for (int r = 0; r < 4; ++r) L[r][j] += 2.0 * (dr * s2r[r] - di * s2i[r]);
             }
         }
-        double* out = a.taylor + (long)li * m * m * m;
+        // transposed layout Lt[j2 + 16 (j3 + m j1)]; columns j2 >= m hold
+        // the zeros computed for them
+        double* out = a.taylor + (long)li * taylor3_stride(m, 1);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int j3 = k4 + 4 * r, j2 = i16;
-            if (j3 < m && j2 < m)
+            const int j3 = k4 + 4 * r;
+            if (j3 < m)
 #pragma unroll
                 for (int j = 0; j < 16; ++j)
-                    if (j < m) out[j3 + m * (j2 + (long)m * j)] = L[r][j];
+                    if (j < m) out[i16 + 16 * (j3 + (long)m * j)] = L[r][j];
         }
     }
+}
+
+// PW -> Taylor on matrix cores for 16 < m <= 32 (m = nl + 1) and npw <= 4 KA,
+// one block of 8 wavefronts per leaf.  Per z node c:
+//   the slab PW(., ., c) -> LDS P (a padded to 4 KA, b to 16 NB, zeros);
+//   S1(j3, b) = sum_a D(a, j3) P(a, b): waves w < 2 NB, tile (j3: w % 2,
+//     b: w / 2), into LDS;
+//   S2(j3, j2) = sum_b S1(j3, b) D(b, j2): wave w, tile (j3: w % 2,
+//     j2: (w / 2) % 2) (waves w and w + 4 compute the same tile);
+//   L(j3, j2, j1) += 2 Re(D(c, j1) S2(j3, j2)) for the wave's half of j1
+//     (w / 4), in registers.
+// D(a, j) sits in LDS for every a < 4 KA, j < 32 (zero outside a < npw,
+// j < m).  Writes the transposed layout Lt[j2 + 32 (j3 + m j1)].
+template <int KA, int NB>
+__global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
+{
+    constexpr int A4 = 4 * KA, BP = 16 * NB;
+    extern __shared__ double sm[];
+    double* P = sm;                     // complex (A4 a) x (BP b), a fastest
+    double* S1 = P + 2 * A4 * BP;       // complex (32 j3) x (BP b), j3 fastest
+    double* Dt = S1 + 2 * 32 * BP;      // complex D(a, j) at j + 32 a, a < A4
+    const int m = a.nl + 1, npw = a.npw, half = npw / 2, t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63, i16 = lane & 15, k4 = lane >> 4;
+    const int mt = w & 1, jt = (w >> 1) & 1, jh = w >> 2;
+    const long nexp = (long)npw * npw * half;
+    for (int o = t; o < A4 * 32; o += 512) {
+        const int aa = o / 32, j = o % 32;
+        const bool in = aa < npw && j < m;
+        Dt[2 * o] = in ? a.dcoef[2 * (aa + a.cstride * j)] : 0.0;
+        Dt[2 * o + 1] = in ? a.dcoef[2 * (aa + a.cstride * j) + 1] : 0.0;
+    }
+    for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
+        const int box = a.boxes[li];
+        if (pw_box(a, box) || !has_eval(a, box)) continue;
+        const double* pw = a.loc + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+        double L[4][16];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) L[r][j] = 0;
+        for (int c = 0; c < half; ++c) {
+            __syncthreads();  // readers of P, S1 of the previous node are done
+            const double* slab = pw + 2L * npw * npw * c;
+            for (int o = t; o < A4 * BP; o += 512) {
+                const int aa = o % A4, b = o / A4;
+                const bool in = aa < npw && b < npw;
+                P[2 * o] = in ? slab[2 * (aa + npw * b)] : 0.0;
+                P[2 * o + 1] = in ? slab[2 * (aa + npw * b) + 1] : 0.0;
+            }
+            __syncthreads();
+            if (w < 2 * NB) {
+                const int nb = w >> 1;
+                d4_t3 sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
+#pragma unroll 2
+                for (int ks = 0; ks < KA; ++ks) {
+                    const int o = (4 * ks + k4) + A4 * (16 * nb + i16);
+                    const int od = (16 * mt + i16) + 32 * (4 * ks + k4);
+                    const double pr = P[2 * o], pi = P[2 * o + 1];
+                    const double ar = Dt[2 * od], ai = Dt[2 * od + 1];
+                    sr = mfma3(ar, pr, sr);
+                    sr = mfma3(-ai, pi, sr);
+                    si = mfma3(ar, pi, si);
+                    si = mfma3(ai, pr, si);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int o = (16 * mt + k4 + 4 * r) + 32 * (16 * nb + i16);
+                    S1[2 * o] = sr[r];
+                    S1[2 * o + 1] = si[r];
+                }
+            }
+            __syncthreads();
+            d4_t3 s2r = {0, 0, 0, 0}, s2i = {0, 0, 0, 0};
+#pragma unroll 2
+            for (int ks = 0; ks < KA; ++ks) {
+                const int o = (16 * mt + i16) + 32 * (4 * ks + k4);
+                const int od = (16 * jt + i16) + 32 * (4 * ks + k4);
+                const double xr = S1[2 * o], xi = S1[2 * o + 1];
+                const double br = Dt[2 * od], bi = Dt[2 * od + 1];
+                s2r = mfma3(xr, br, s2r);
+                s2r = mfma3(-xi, bi, s2r);
+                s2i = mfma3(xr, bi, s2i);
+                s2i = mfma3(xi, br, s2i);
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int o = 32 * c + 16 * jh + j;
+                const double dr = Dt[2 * o], di = Dt[2 * o + 1];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) L[r][j] += 2.0 * (dr * s2r[r] - di * s2i[r]);
+                if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // bounds the loads in flight
+            }
+        }
+        double* out = a.taylor + (long)li * taylor3_stride(m, 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j3 = 16 * mt + k4 + 4 * r, j2 = 16 * jt + i16;
+            if (j3 < m)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int j1 = 16 * jh + j;
+                    if (j1 < m) out[j2 + 32 * (j3 + (long)m * j1)] = L[r][j];
+                }
+        }
+    }
+}
+
+template <int KA, int NB>
+static void pw2local3_blk_launch(const Eval3Args& a, hipStream_t s)
+{
+    const size_t lds = ((size_t)2 * 4 * KA * 16 * NB + (size_t)2 * 32 * 16 * NB +
+                        (size_t)2 * 4 * KA * 32) * sizeof(double);
+    allow_lds3(k_pw2local3_blk<KA, NB>, lds);
+    k_pw2local3_blk<KA, NB><<<std::min(a.nbox, 8192), 512, lds, s>>>(a);
 }
 
 void launch_pw2local3(const Eval3Args& a, hipStream_t s)
@@ -929,10 +1056,21 @@ void launch_pw2local3(const Eval3Args& a, hipStream_t s)
     if (a.nbox <= 0) return;
     const int m = a.nl + 1;
     if (m > L3_MAXM) fail(-3, "3D Taylor order above 63 is not supported");
-    if (m <= 16 && a.npw <= PW3_N) {
+    if (m <= 16 && a.npw <= PW3_N && a.tlayout) {
         size_t lds = ((size_t)2 * PW3_N * PW3_N + (size_t)2 * (a.npw / 2) * 16) * sizeof(double);
         kmark(KID_PW2LOCAL, false, s);
         k_pw2local3_wave<<<std::min(a.nbox, 65536), 64, lds, s>>>(a);
+        kmark(KID_PW2LOCAL, true, s);
+        return;
+    }
+    if (m > 16 && m <= 32 && a.npw <= 48 && a.tlayout) {
+        kmark(KID_PW2LOCAL, false, s);
+        if (a.npw <= 32)
+            pw2local3_blk_launch<8, 2>(a, s);
+        else if (a.npw <= 40)
+            pw2local3_blk_launch<10, 3>(a, s);
+        else
+            pw2local3_blk_launch<12, 3>(a, s);
         kmark(KID_PW2LOCAL, true, s);
         return;
     }
@@ -1039,11 +1177,199 @@ __global__ __launch_bounds__(256) void k_taylor_eval3(Eval3Args a)
     }
 }
 
+// Taylor evaluation on FP64 matrix cores (Taylor order nl < 32), the
+// coefficients in the transposed layout Lt[j2 + MP (j3 + m j1)], MP = 16 NT,
+// columns j2 >= m zero:
+//   T(p, j2) = sum_j3 x_p^j3 Lt(j2, j3, j1)          MFMA, 16-point tiles
+//   pot(p) += sum_j1 z_p^j1 sum_j2 y_p^j2 T(p, j2)     registers, 16-lane sum
+// x derivatives come from the rows j3 x^(j3-1) and j3 (j3-1) x^(j3-2) (more
+// MFMAs), y and z derivatives from the weights.  One wavefront per segment of
+// at most 256 points of one leaf; lane l supplies the powers of point l%16
+// of the tile and holds T(l/16 + 4r, 16 nt + l%16).
+template <int NT, int PGH>
+__global__ __launch_bounds__(256) void k_taylor3_mfma(Eval3Args a)
+{
+    constexpr int MP = 16 * NT, KS = 4 * NT;
+    constexpr int NX = PGH >= 3 ? 3 : (PGH == 2 ? 2 : 1);
+    constexpr int NACC = PGH >= 3 ? 10 : (PGH == 2 ? 4 : 1);
+    const int m = a.nl + 1, lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+    const long tst = taylor3_stride(m, 1);
+    const int total = a.tstart[a.nbox];
+    for (int g = a.tstart[0] + blockIdx.x * 4 + (threadIdx.x >> 6); g < total;
+         g += gridDim.x * 4) {
+        int lo = 0, hi = a.nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (a.tstart[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        const int b = a.boxes[lo];
+        if (pw_box(a, b)) continue;
+        const int2 rr = a.se[b];
+        const int p0 = rr.x + (g - a.tstart[lo]) * a.seg, p1 = min(rr.y, p0 + a.seg);
+        const double* Lt = a.taylor + (long)lo * tst;
+        const double cx = a.cen[3 * b], cy = a.cen[3 * b + 1], cz = a.cen[3 * b + 2], dsq = a.dsq;
+        for (int t0 = p0; t0 < p1; t0 += 16) {
+            // A operands: powers of x of point t0 + l%16, orders 4 ks + l/16
+            double X[NX][KS];
+            {
+                const int pa = t0 + i16;
+                const double x = pa < p1 ? (a.pts[3 * (long)pa] - cx) * dsq : 0.0;
+                double q0 = 1, q1 = 0, q2 = 0;  // x^j, j x^(j-1), j (j-1) x^(j-2)
+#pragma unroll
+                for (int j = 0; j < 4 * KS; ++j) {
+                    if ((j & 3) == k4) {
+                        const bool in = j < m;
+                        X[0][j >> 2] = in ? q0 : 0.0;
+                        if (NX > 1) X[1 % NX][j >> 2] = in ? q1 : 0.0;
+                        if (NX > 2) X[2 % NX][j >> 2] = in ? q2 : 0.0;
+                    }
+                    q2 = (j + 1) * q1;
+                    q1 = (j + 1) * q0;
+                    q0 *= x;
+                }
+            }
+            // result rows: points t0 + l/16 + 4r; y weights of column 16 nt + l%16
+            double zv[4], Y[3][4][NT];
+            bool in_r[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int p = t0 + k4 + 4 * r;
+                in_r[r] = p < p1;
+                const double y = in_r[r] ? (a.pts[3 * (long)p + 1] - cy) * dsq : 0.0;
+                zv[r] = in_r[r] ? (a.pts[3 * (long)p + 2] - cz) * dsq : 0.0;
+                double q0 = 1, q1 = 0, q2 = 0;
+#pragma unroll
+                for (int j = 0; j < MP; ++j) {
+                    if ((j & 15) == i16) {
+                        const bool in = j < m;
+                        Y[0][r][j >> 4] = in ? q0 : 0.0;
+                        Y[1][r][j >> 4] = in ? q1 : 0.0;
+                        Y[2][r][j >> 4] = in ? q2 : 0.0;
+                    }
+                    q2 = (j + 1) * q1;
+                    q1 = (j + 1) * q0;
+                    q0 *= y;
+                }
+            }
+            // acc: pot; gx gy gz; hxx hyy hzz hxy hxz hyz
+            double acc[NACC][4];
+#pragma unroll
+            for (int k = 0; k < NACC; ++k)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[k][r] = 0;
+            double z0[4], z1[4], z2[4];  // z^j1, j1 z^(j1-1), j1 (j1-1) z^(j1-2)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) z0[r] = 1, z1[r] = 0, z2[r] = 0;
+            for (int j1 = 0; j1 < m; ++j1) {
+                d4_t3 T[NX][NT];
+#pragma unroll
+                for (int ix = 0; ix < NX; ++ix)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) T[ix][nt] = (d4_t3){0, 0, 0, 0};
+                const double* Lj = Lt + (long)MP * m * j1;
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) {
+                    const int j3 = 4 * ks + k4;
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) {
+                        const double lb = j3 < m ? Lj[(16 * nt + i16) + (long)MP * j3] : 0.0;
+#pragma unroll
+                        for (int ix = 0; ix < NX; ++ix) T[ix][nt] = mfma3(X[ix][ks], lb, T[ix][nt]);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    double s00 = 0, s01 = 0, s10 = 0, s02 = 0, s11 = 0, s20 = 0;
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) {
+                        s00 += T[0][nt][r] * Y[0][r][nt];
+                        if (PGH >= 2) {
+                            s01 += T[0][nt][r] * Y[1][r][nt];
+                            s10 += T[1 % NX][nt][r] * Y[0][r][nt];
+                        }
+                        if (PGH >= 3) {
+                            s02 += T[0][nt][r] * Y[2][r][nt];
+                            s11 += T[1 % NX][nt][r] * Y[1][r][nt];
+                            s20 += T[2 % NX][nt][r] * Y[0][r][nt];
+                        }
+                    }
+                    acc[0][r] += z0[r] * s00;
+                    if (PGH >= 2) {
+                        acc[1 % NACC][r] += z0[r] * s10;
+                        acc[2 % NACC][r] += z0[r] * s01;
+                        acc[3 % NACC][r] += z1[r] * s00;
+                    }
+                    if (PGH >= 3) {
+                        acc[4 % NACC][r] += z0[r] * s20;
+                        acc[5 % NACC][r] += z0[r] * s02;
+                        acc[6 % NACC][r] += z2[r] * s00;
+                        acc[7 % NACC][r] += z0[r] * s11;
+                        acc[8 % NACC][r] += z1[r] * s10;
+                        acc[9 % NACC][r] += z1[r] * s01;
+                    }
+                    z2[r] = (j1 + 1) * z1[r];
+                    z1[r] = (j1 + 1) * z0[r];
+                    z0[r] *= zv[r];
+                }
+            }
+            // sum over the 16 lanes that hold the same points
+#pragma unroll
+            for (int k = 0; k < NACC; ++k)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int o = 1; o < 16; o <<= 1) acc[k][r] += __shfl_xor(acc[k][r], o, 64);
+            if (i16 == 0)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (!in_r[r]) continue;
+                    const long i = t0 + k4 + 4 * r;
+                    a.pot[i * a.nd + a.ind] += acc[0][r];
+                    if (PGH >= 2) {
+                        const long gi = i * 3 * a.nd + a.ind;
+                        a.grad[gi] += acc[1 % NACC][r] * dsq;
+                        a.grad[gi + a.nd] += acc[2 % NACC][r] * dsq;
+                        a.grad[gi + 2 * a.nd] += acc[3 % NACC][r] * dsq;
+                    }
+                    if (PGH >= 3) {
+                        const long hi2 = i * 6 * a.nd + a.ind;
+                        const double d2 = dsq * dsq;
+                        a.hess[hi2] += acc[4 % NACC][r] * d2;
+                        a.hess[hi2 + a.nd] += acc[5 % NACC][r] * d2;
+                        a.hess[hi2 + 2 * a.nd] += acc[6 % NACC][r] * d2;
+                        a.hess[hi2 + 3 * a.nd] += acc[7 % NACC][r] * d2;
+                        a.hess[hi2 + 4 * a.nd] += acc[8 % NACC][r] * d2;
+                        a.hess[hi2 + 5 * a.nd] += acc[9 % NACC][r] * d2;
+                    }
+                }
+        }
+    }
+}
+
+template <int NT>
+static void taylor3_mfma_launch(const Eval3Args& a, hipStream_t s)
+{
+    const int grid = (a.grid + 3) / 4;
+    if (a.ifpgh >= 3)
+        k_taylor3_mfma<NT, 3><<<grid, 256, 0, s>>>(a);
+    else if (a.ifpgh == 2)
+        k_taylor3_mfma<NT, 2><<<grid, 256, 0, s>>>(a);
+    else
+        k_taylor3_mfma<NT, 1><<<grid, 256, 0, s>>>(a);
+}
+
 void launch_taylor_eval3(const Eval3Args& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
+    const int m = a.nl + 1;
     kmark(KID_TAYLOR_EVAL, false, s);
-    if (a.ifpgh >= 3)
+    if (a.tlayout) {
+        if (m > 32) fail(-3, "transposed Taylor layout above order 31");
+        if (m <= 16)
+            taylor3_mfma_launch<1>(a, s);
+        else
+            taylor3_mfma_launch<2>(a, s);
+    } else if (a.ifpgh >= 3)
         k_taylor_eval3<3><<<a.grid, 256, 0, s>>>(a);
     else if (a.ifpgh == 2)
         k_taylor_eval3<2><<<a.grid, 256, 0, s>>>(a);
