@@ -1865,12 +1865,14 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
             }
             if (L.count) launch_seg_scan(L, s);
         }
-        // formation scratch: per block m^3 + 2 m^2 (npw/2)
+        // formation scratch per block: m^3 + 2 m^2 (npw/2), or the
+        // matrix-core route's padded moments and z contractions
         long sstride = 0;
         for (int l = std::max(npwlevel + 1, 0); l <= nlevels; ++l) {
             if (form_thresh[l] == INT_MAX) continue;
             long m = nlocal[l] + 1;
             sstride = std::max(sstride, m * m * m + 2 * m * m * half);
+            if (m <= 32 && npw <= 48) sstride = std::max(sstride, form3_mfma_scratch((int)m, npw));
         }
         double* fscr = W.get<double>("form3_scratch", (size_t)GRID3 * std::max(sstride, 1L));
         double* taylor = W.get<double>("taylor3", ntay);

@@ -280,6 +280,7 @@ struct Form3Args {
     const double* dipw = nullptr;  // dipole weights d n, (n, nd, 3) sorted, or nullptr
 };
 void launch_form3_hermite(const Form3Args& a, hipStream_t s);
+long form3_mfma_scratch(int m, int npw);  // per-block scratch of the matrix-core route
 void launch_form3_pw(const Form3Args& a, hipStream_t s);
 
 void launch_merge3(const int* parents, int np, const int* child8, const double* cen,

@@ -27,6 +27,23 @@ static inline unsigned gridn(long n, int bs)
     return (unsigned)(g < 1 ? 1 : g);
 }
 
+// FP64 matrix cores (v_mfma_f64_16x16x4f64): D(16x16) += A(16x4) B(4x16) per
+// wavefront; lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
+// D[l/16 + 4r][l%16] in element r of the result.
+typedef double d4_t3 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4_t3 mfma3(double a, double b, d4_t3 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ void wave_sync3()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // grid of a grid-stride kernel over n items: at most 2^20 blocks
 static inline unsigned gridcap(long n, int bs)
 {
@@ -388,12 +405,264 @@ __global__ __launch_bounds__(256) void k_form3_hermite(Form3Args a)
     }
 }
 
+// Hermite-route formation on matrix cores (m = p + 1 <= 32, npw <= 16 NB),
+// one block of 8 wavefronts per leaf, in four contractions:
+//   1. M(j, k, l) = sum_s x^j W_s(k, l), W = q y^k z^l (+ dipole terms): rows
+//      j, columns (k, l) at k + KP l, sources as K; to per-block scratch
+//      M[(k + KP l) + KP m j];
+//   2. N(j, k, c) = sum_l M(j, k, l) C(l, c), computed transposed (rows c,
+//      columns (j, k) at k + KP j) into scratch N[(k + KP j) + KP m c];
+//   3. per z node c: P_c(a, k) = sum_j C(j, a) N(j, k, c), into LDS;
+//   4. PW(a, b, c) = sum_k C(k, b) P_c(a, k), computed transposed (rows b,
+//      columns a) so that the store runs along a.
+// C(j, a) = ws_a (-i ts_a)^j / j! sits in LDS at a + CW j; KP = 16 MT is m
+// rounded up to 16; all padding is zero.
+template <int NB, bool DIP>
+__global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
+{
+    constexpr int CW = 16 * NB + 1;   // LDS row stride of the C table
+    extern __shared__ double sm[];
+    const int m = a.p + 1, npw = a.npw, half = npw / 2, t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63, i16 = lane & 15, k4 = lane >> 4;
+    const int MT = (m + 15) / 16, KP = 16 * MT, KS = (m + 3) / 4, PK = KP + 1;
+    const int KLP = KP * m;           // columns (k, l) of M, rows (j, k) of N
+    const int NC = (half + 15) / 16;
+    const long nexp = (long)npw * npw * half;
+    double* Mg = a.scratch + (long)blockIdx.x * a.sstride;  // KP rows j x KLP
+    double* Ng = Mg + (long)KP * KLP;                        // complex KLP x half
+    double* Ct = sm;                                         // complex C(j, a), j < 4 KS
+    double* work = Ct + 2 * CW * 4 * KS;                     // sources, then P_c
+    for (int o = t; o < CW * 4 * KS; o += 512) {
+        const int aa = o % CW, j = o / CW;
+        const bool in = aa < npw && j < m;
+        Ct[2 * o] = in ? a.coef[2 * (aa + a.cstride * j)] : 0.0;
+        Ct[2 * o + 1] = in ? a.coef[2 * (aa + a.cstride * j) + 1] : 0.0;
+    }
+    for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
+        const int box = a.boxes[li];
+        const int2 r = a.src_se[box];
+        if (r.y - r.x <= a.thresh) continue;  // empty, or on the direct plane-wave route
+        const double cx = a.cen[3 * box], cy = a.cen[3 * box + 1], cz = a.cen[3 * box + 2];
+        // ---- 1. moments: tiles (j tile mt, column tile nk) over the waves ----
+        {
+            double* xp = work;                 // [32 sources][KP] x^j
+            double* yp = xp + 32 * KP;         // y^k
+            double* zq = yp + 32 * KP;         // q z^l (dipoles: z^l)
+            double* xd = zq + 32 * KP;         // dipoles: j x^(j-1), k y^(k-1), l z^(l-1)
+            double* yd = xd + 32 * KP;
+            double* zd = yd + 32 * KP;
+            double* sw = zd + 32 * KP;         // dipoles: q, dsq w1, dsq w2, dsq w3
+            const int ntile = MT * (KLP / 16);
+            constexpr int TPW = 4;  // tiles per wave and pass
+            for (int tp = 0; tp < ntile; tp += 8 * TPW) {
+                d4_t3 acc[TPW];
+#pragma unroll
+                for (int i = 0; i < TPW; ++i) acc[i] = (d4_t3){0, 0, 0, 0};
+                for (int s0 = r.x; s0 < r.y; s0 += 32) {
+                    const int cn = min(32, r.y - s0);
+                    __syncthreads();
+                    if (t < 32) {
+                        const bool in = t < cn;
+                        const long i = s0 + t;
+                        double x = 0, y = 0, z = 0, q = 0;
+                        if (in) {
+                            x = (a.xs[3 * i] - cx) * a.dsq;
+                            y = (a.xs[3 * i + 1] - cy) * a.dsq;
+                            z = (a.xs[3 * i + 2] - cz) * a.dsq;
+                            q = a.qs[(long)a.nd * i + a.ind];
+                        }
+                        double px = 1, py = 1, pz = DIP ? 1.0 : q, qx = 0, qy = 0, qz = 0;
+                        for (int j = 0; j < KP; ++j) {
+                            const bool jm = in && j < m;
+                            xp[t * KP + j] = jm ? px : 0.0;
+                            yp[t * KP + j] = jm ? py : 0.0;
+                            zq[t * KP + j] = jm ? pz : 0.0;
+                            if (DIP) {
+                                xd[t * KP + j] = jm ? j * qx : 0.0;
+                                yd[t * KP + j] = jm ? j * qy : 0.0;
+                                zd[t * KP + j] = jm ? j * qz : 0.0;
+                                qx = px;
+                                qy = py;
+                                qz = pz;
+                            }
+                            px *= x;
+                            py *= y;
+                            pz *= z;
+                        }
+                        if (DIP) {
+                            double w1 = 0, w2 = 0, w3 = 0;
+                            if (in) {
+                                const double* wv = a.dipw + 3 * ((long)a.nd * i + a.ind);
+                                w1 = wv[0], w2 = wv[1], w3 = wv[2];
+                            }
+                            sw[4 * t] = q;
+                            sw[4 * t + 1] = a.dsq * w1;
+                            sw[4 * t + 2] = a.dsq * w2;
+                            sw[4 * t + 3] = a.dsq * w3;
+                        }
+                    }
+                    __syncthreads();
+                    const int nks = (cn + 3) / 4;
+#pragma unroll
+                    for (int i = 0; i < TPW; ++i) {
+                        const int tile = tp + w + 8 * i;
+                        if (tile >= ntile) break;
+                        const int mt = tile % MT, nk = tile / MT;
+                        const int j = 16 * mt + i16, kl = 16 * nk + i16, k = kl % KP, l = kl / KP;
+                        for (int ks = 0; ks < nks; ++ks) {
+                            const int sa = 4 * ks + k4;  // this lane's source in the chunk
+                            const double av = xp[sa * KP + j];
+                            if (!DIP) {
+                                const double bv = yp[sa * KP + k] * zq[sa * KP + l];
+                                acc[i] = mfma3(av, bv, acc[i]);
+                            } else {
+                                const double yk = yp[sa * KP + k], zl = zq[sa * KP + l];
+                                const double F = sw[4 * sa] * yk * zl +
+                                                 sw[4 * sa + 2] * yd[sa * KP + k] * zl +
+                                                 sw[4 * sa + 3] * yk * zd[sa * KP + l];
+                                const double G = sw[4 * sa + 1] * yk * zl;
+                                acc[i] = mfma3(av, F, acc[i]);
+                                acc[i] = mfma3(xd[sa * KP + j], G, acc[i]);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < TPW; ++i) {
+                    const int tile = tp + w + 8 * i;
+                    if (tile >= ntile) break;
+                    const int mt = tile % MT, nk = tile / MT;
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr)
+                        Mg[(16 * nk + i16) + (long)KLP * (16 * mt + k4 + 4 * rr)] = acc[i][rr];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- 2. N^T(c, (j, k)) = sum_l C(l, c) M(j, k, l) ----
+        {
+            const int ntile = NC * (KLP / 16);
+            for (int tile = w; tile < ntile; tile += 8) {
+                const int nc = tile % NC, nr = tile / NC;
+                const int rho = 16 * nr + i16, j = rho / KP, k = rho % KP;
+                d4_t3 nr_ = {0, 0, 0, 0}, ni_ = {0, 0, 0, 0};
+                for (int ks = 0; ks < KS; ++ks) {
+                    const int l = 4 * ks + k4;
+                    const int oc = (16 * nc + i16) + CW * l;
+                    const double cr = Ct[2 * oc], ci = Ct[2 * oc + 1];
+                    const double mv = l < m ? Mg[(k + KP * l) + (long)KLP * j] : 0.0;
+                    nr_ = mfma3(cr, mv, nr_);
+                    ni_ = mfma3(ci, mv, ni_);
+                }
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int c = 16 * nc + k4 + 4 * rr;
+                    if (c < half) {
+                        double* d = Ng + 2 * ((16 * nr + i16) + (long)KLP * c);
+                        d[0] = nr_[rr];
+                        d[1] = ni_[rr];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // ---- 3, 4 per z node; P_c double-buffered in LDS at k + PK a ----
+        double* out = a.mp + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+        for (int c = 0; c < half; ++c) {
+            double* Pc = work + (c & 1) * 2 * PK * 16 * NB;
+            const double* Nc = Ng + 2L * KLP * c;
+            for (int tile = w; tile < NB * MT; tile += 8) {
+                const int na = tile % NB, mt = tile / NB;
+                d4_t3 pr = {0, 0, 0, 0}, pi = {0, 0, 0, 0};
+                for (int ks = 0; ks < KS; ++ks) {
+                    const int j = 4 * ks + k4;
+                    const int oc = (16 * na + i16) + CW * j;
+                    const double cr = Ct[2 * oc], ci = Ct[2 * oc + 1];
+                    double nvr = 0, nvi = 0;
+                    if (j < m) {
+                        const double* nv = Nc + 2 * ((16 * mt + i16) + KP * j);
+                        nvr = nv[0];
+                        nvi = nv[1];
+                    }
+                    pr = mfma3(cr, nvr, pr);
+                    pr = mfma3(-ci, nvi, pr);
+                    pi = mfma3(cr, nvi, pi);
+                    pi = mfma3(ci, nvr, pi);
+                }
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int o = (16 * mt + i16) + PK * (16 * na + k4 + 4 * rr);
+                    Pc[2 * o] = pr[rr];
+                    Pc[2 * o + 1] = pi[rr];
+                }
+            }
+            __syncthreads();
+            for (int tile = w; tile < NB * NB; tile += 8) {
+                const int nb = tile % NB, na = tile / NB;
+                d4_t3 vr = {0, 0, 0, 0}, vi = {0, 0, 0, 0};
+                for (int ks = 0; ks < KS; ++ks) {
+                    const int k = 4 * ks + k4;
+                    const int oc = (16 * nb + i16) + CW * k;
+                    const int op = k + PK * (16 * na + i16);
+                    const double cr = Ct[2 * oc], ci = Ct[2 * oc + 1];
+                    const double xr = Pc[2 * op], xi = Pc[2 * op + 1];
+                    vr = mfma3(cr, xr, vr);
+                    vr = mfma3(-ci, xi, vr);
+                    vi = mfma3(cr, xi, vi);
+                    vi = mfma3(ci, xr, vi);
+                }
+                const int aa = 16 * na + i16;
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int b = 16 * nb + k4 + 4 * rr;
+                    if (aa < npw && b < npw) {
+                        double* d = out + 2 * (aa + (long)npw * (b + (long)npw * c));
+                        d[0] = vr[rr];
+                        d[1] = vi[rr];
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the next leaf's staging reuses the P_c space
+    }
+}
+
+// per-block scratch of the matrix-core formation, doubles
+long form3_mfma_scratch(int m, int npw)
+{
+    const long KP = 16 * ((m + 15) / 16);
+    return KP * KP * m + 2 * KP * m * (npw / 2);
+}
+
+template <int NB, bool DIP>
+static void form3_mfma_launch(const Form3Args& a, hipStream_t s)
+{
+    const int m = a.p + 1, KP = 16 * ((m + 15) / 16), KS = (m + 3) / 4;
+    const size_t ct = (size_t)2 * (16 * NB + 1) * 4 * KS;
+    const size_t src = (size_t)(DIP ? 6 : 3) * 32 * KP + (DIP ? 4 * 32 : 0);
+    const size_t pc = (size_t)2 * 2 * (KP + 1) * 16 * NB;
+    const size_t lds = (ct + std::max(src, pc)) * sizeof(double);
+    allow_lds3(k_form3_mfma<NB, DIP>, lds);
+    k_form3_mfma<NB, DIP><<<a.grid, 512, lds, s>>>(a);
+}
+
 void launch_form3_hermite(const Form3Args& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
     const int m = a.p + 1;
     if (m > 64) fail(-3, "3D Hermite order above 63 is not supported");
     const bool dip = a.dipw != nullptr;
+    if (m <= 32 && a.npw <= 48) {
+        if (a.sstride < form3_mfma_scratch(m, a.npw)) fail(-3, "formation scratch too small");
+        kmark(KID_FORM_MOMENTS, false, s);
+        if (a.npw <= 32) {
+            if (dip) form3_mfma_launch<2, true>(a, s); else form3_mfma_launch<2, false>(a, s);
+        } else {
+            if (dip) form3_mfma_launch<3, true>(a, s); else form3_mfma_launch<3, false>(a, s);
+        }
+        kmark(KID_FORM_MOMENTS, true, s);
+        return;
+    }
     size_t lds = std::max((size_t)(dip ? 6 : 3) * F3_CHUNK * m + (dip ? 4 * F3_CHUNK : 0),
                           (size_t)4 * a.npw * m) * sizeof(double);
     kmark(KID_FORM_MOMENTS, false, s);
@@ -807,22 +1076,6 @@ __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
     }
 }
 
-// FP64 matrix cores (v_mfma_f64_16x16x4f64): D(16x16) += A(16x4) B(4x16) per
-// wavefront; lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
-// D[l/16 + 4r][l%16] in element r of the result.
-typedef double d4_t3 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ d4_t3 mfma3(double a, double b, d4_t3 c)
-{
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ void wave_sync3()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
 
 // PW -> Taylor on matrix cores, one wavefront (block of 64) per leaf, for
 // Taylor orders nl < 16 and npw <= 32:
@@ -834,14 +1087,15 @@ __device__ __forceinline__ void wave_sync3()
 // wave's LDS (zero padded); S1 then reuses that space.  Lane l holds
 // L(j3 = l/16 + 4r, j2 = l%16, j1) for r < 4, j1 < 16.
 constexpr int PW3_N = 32;  // padded npw
+constexpr int PW3_S = 33;  // LDS stride of a slab column (bank-conflict-free lane stride)
 
 __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
 {
     extern __shared__ double sm[];
     const int m = a.nl + 1, npw = a.npw, half = npw / 2, lane = threadIdx.x;
     const long nexp = (long)npw * npw * half;
-    double* P = sm;                  // complex (PW3_N a) x (PW3_N b), a fastest
-    double* Dt = sm + 2 * PW3_N * PW3_N;  // complex D(c, j1), (half, 16)
+    double* P = sm;                  // complex (PW3_N a) x (PW3_N b) at a + PW3_S b
+    double* Dt = sm + 2 * PW3_S * PW3_N;  // complex D(c, j1), (half, 16)
     const int i16 = lane & 15, k4 = lane >> 4;
     // this lane's D(4 ks + l/16, l%16): A operand of S1 and B operand of S2
     double Dr[PW3_N / 4], Di[PW3_N / 4];
@@ -871,10 +1125,10 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
             wave_sync3();  // previous readers of P / S1 are done
             const double* slab = pw + 2L * npw * npw * c;
             for (int o = lane; o < PW3_N * PW3_N; o += 64) {
-                const int aa = o % PW3_N, b = o / PW3_N;
+                const int aa = o % PW3_N, b = o / PW3_N, op = aa + PW3_S * b;
                 const bool in = aa < npw && b < npw;
-                P[2 * o] = in ? slab[2 * (aa + npw * b)] : 0.0;
-                P[2 * o + 1] = in ? slab[2 * (aa + npw * b) + 1] : 0.0;
+                P[2 * op] = in ? slab[2 * (aa + npw * b)] : 0.0;
+                P[2 * op + 1] = in ? slab[2 * (aa + npw * b) + 1] : 0.0;
             }
             wave_sync3();
             // S1 = D^T PW: two 16-column tiles over b
@@ -885,7 +1139,7 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
                 s1i[nt] = (d4_t3){0, 0, 0, 0};
 #pragma unroll
                 for (int ks = 0; ks < PW3_N / 4; ++ks) {
-                    const int o = (4 * ks + k4) + PW3_N * (16 * nt + i16);
+                    const int o = (4 * ks + k4) + PW3_S * (16 * nt + i16);
                     const double pr = P[2 * o], pi = P[2 * o + 1];
                     s1r[nt] = mfma3(Dr[ks], pr, s1r[nt]);
                     s1r[nt] = mfma3(-Di[ks], pi, s1r[nt]);
@@ -894,12 +1148,12 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
                 }
             }
             wave_sync3();  // all reads of P done: S1 overwrites it
-            double* S1 = P;  // complex (16 j3) x (PW3_N b), j3 fastest
+            double* S1 = P;  // complex (16 j3) x (PW3_N b) at j3 + 17 b
 #pragma unroll
             for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int o = (k4 + 4 * r) + 16 * (16 * nt + i16);
+                    const int o = (k4 + 4 * r) + 17 * (16 * nt + i16);
                     S1[2 * o] = s1r[nt][r];
                     S1[2 * o + 1] = s1i[nt][r];
                 }
@@ -908,7 +1162,7 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
             d4_t3 s2r = {0, 0, 0, 0}, s2i = {0, 0, 0, 0};
 #pragma unroll
             for (int ks = 0; ks < PW3_N / 4; ++ks) {
-                const int o = i16 + 16 * (4 * ks + k4);
+                const int o = i16 + 17 * (4 * ks + k4);
                 const double xr = S1[2 * o], xi = S1[2 * o + 1];
                 s2r = mfma3(xr, Dr[ks], s2r);
                 s2r = mfma3(-xi, Di[ks], s2r);
@@ -951,11 +1205,11 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
 template <int KA, int NB>
 __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
 {
-    constexpr int A4 = 4 * KA, BP = 16 * NB;
+    constexpr int A4 = 4 * KA, BP = 16 * NB, PS = A4 + 1;
     extern __shared__ double sm[];
-    double* P = sm;                     // complex (A4 a) x (BP b), a fastest
-    double* S1 = P + 2 * A4 * BP;       // complex (32 j3) x (BP b), j3 fastest
-    double* Dt = S1 + 2 * 32 * BP;      // complex D(a, j) at j + 32 a, a < A4
+    double* P = sm;                     // complex (A4 a) x (BP b) at a + PS b
+    double* S1 = P + 2 * PS * BP;       // complex (32 j3) x (BP b) at j3 + 33 b
+    double* Dt = S1 + 2 * 33 * BP;      // complex D(a, j) at j + 32 a, a < A4
     const int m = a.nl + 1, npw = a.npw, half = npw / 2, t = threadIdx.x;
     const int w = t >> 6, lane = t & 63, i16 = lane & 15, k4 = lane >> 4;
     const int mt = w & 1, jt = (w >> 1) & 1, jh = w >> 2;
@@ -979,10 +1233,10 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
             __syncthreads();  // readers of P, S1 of the previous node are done
             const double* slab = pw + 2L * npw * npw * c;
             for (int o = t; o < A4 * BP; o += 512) {
-                const int aa = o % A4, b = o / A4;
+                const int aa = o % A4, b = o / A4, op = aa + PS * b;
                 const bool in = aa < npw && b < npw;
-                P[2 * o] = in ? slab[2 * (aa + npw * b)] : 0.0;
-                P[2 * o + 1] = in ? slab[2 * (aa + npw * b) + 1] : 0.0;
+                P[2 * op] = in ? slab[2 * (aa + npw * b)] : 0.0;
+                P[2 * op + 1] = in ? slab[2 * (aa + npw * b) + 1] : 0.0;
             }
             __syncthreads();
             if (w < 2 * NB) {
@@ -990,7 +1244,7 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
                 d4_t3 sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
 #pragma unroll 2
                 for (int ks = 0; ks < KA; ++ks) {
-                    const int o = (4 * ks + k4) + A4 * (16 * nb + i16);
+                    const int o = (4 * ks + k4) + PS * (16 * nb + i16);
                     const int od = (16 * mt + i16) + 32 * (4 * ks + k4);
                     const double pr = P[2 * o], pi = P[2 * o + 1];
                     const double ar = Dt[2 * od], ai = Dt[2 * od + 1];
@@ -1001,7 +1255,7 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int o = (16 * mt + k4 + 4 * r) + 32 * (16 * nb + i16);
+                    const int o = (16 * mt + k4 + 4 * r) + 33 * (16 * nb + i16);
                     S1[2 * o] = sr[r];
                     S1[2 * o + 1] = si[r];
                 }
@@ -1010,7 +1264,7 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
             d4_t3 s2r = {0, 0, 0, 0}, s2i = {0, 0, 0, 0};
 #pragma unroll 2
             for (int ks = 0; ks < KA; ++ks) {
-                const int o = (16 * mt + i16) + 32 * (4 * ks + k4);
+                const int o = (16 * mt + i16) + 33 * (4 * ks + k4);
                 const int od = (16 * jt + i16) + 32 * (4 * ks + k4);
                 const double xr = S1[2 * o], xi = S1[2 * o + 1];
                 const double br = Dt[2 * od], bi = Dt[2 * od + 1];
@@ -1045,7 +1299,7 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
 template <int KA, int NB>
 static void pw2local3_blk_launch(const Eval3Args& a, hipStream_t s)
 {
-    const size_t lds = ((size_t)2 * 4 * KA * 16 * NB + (size_t)2 * 32 * 16 * NB +
+    const size_t lds = ((size_t)2 * (4 * KA + 1) * 16 * NB + (size_t)2 * 33 * 16 * NB +
                         (size_t)2 * 4 * KA * 32) * sizeof(double);
     allow_lds3(k_pw2local3_blk<KA, NB>, lds);
     k_pw2local3_blk<KA, NB><<<std::min(a.nbox, 8192), 512, lds, s>>>(a);
@@ -1057,7 +1311,7 @@ void launch_pw2local3(const Eval3Args& a, hipStream_t s)
     const int m = a.nl + 1;
     if (m > L3_MAXM) fail(-3, "3D Taylor order above 63 is not supported");
     if (m <= 16 && a.npw <= PW3_N && a.tlayout) {
-        size_t lds = ((size_t)2 * PW3_N * PW3_N + (size_t)2 * (a.npw / 2) * 16) * sizeof(double);
+        size_t lds = ((size_t)2 * PW3_S * PW3_N + (size_t)2 * (a.npw / 2) * 16) * sizeof(double);
         kmark(KID_PW2LOCAL, false, s);
         k_pw2local3_wave<<<std::min(a.nbox, 65536), 64, lds, s>>>(a);
         kmark(KID_PW2LOCAL, true, s);
