@@ -461,9 +461,11 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                 for (int s0 = r.x; s0 < r.y; s0 += 32) {
                     const int cn = min(32, r.y - s0);
                     __syncthreads();
-                    if (t < 32) {
-                        const bool in = t < cn;
-                        const long i = s0 + t;
+                    {
+                        // 16 threads per source, each a strided subset of the orders
+                        const int sl = t & 31, j0 = t >> 5;
+                        const bool in = sl < cn;
+                        const long i = s0 + sl;
                         double x = 0, y = 0, z = 0, q = 0;
                         if (in) {
                             x = (a.xs[3 * i] - cx) * a.dsq;
@@ -471,34 +473,36 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                             z = (a.xs[3 * i + 2] - cz) * a.dsq;
                             q = a.qs[(long)a.nd * i + a.ind];
                         }
-                        double px = 1, py = 1, pz = DIP ? 1.0 : q, qx = 0, qy = 0, qz = 0;
-                        for (int j = 0; j < KP; ++j) {
+                        for (int j = j0; j < KP; j += 16) {
                             const bool jm = in && j < m;
-                            xp[t * KP + j] = jm ? px : 0.0;
-                            yp[t * KP + j] = jm ? py : 0.0;
-                            zq[t * KP + j] = jm ? pz : 0.0;
-                            if (DIP) {
-                                xd[t * KP + j] = jm ? j * qx : 0.0;
-                                yd[t * KP + j] = jm ? j * qy : 0.0;
-                                zd[t * KP + j] = jm ? j * qz : 0.0;
+                            double px = 1, py = 1, pz = DIP ? 1.0 : q, qx = 0, qy = 0, qz = 0;
+                            for (int e = 0; e < j; ++e) {
                                 qx = px;
                                 qy = py;
                                 qz = pz;
+                                px *= x;
+                                py *= y;
+                                pz *= z;
                             }
-                            px *= x;
-                            py *= y;
-                            pz *= z;
+                            xp[sl * KP + j] = jm ? px : 0.0;
+                            yp[sl * KP + j] = jm ? py : 0.0;
+                            zq[sl * KP + j] = jm ? pz : 0.0;
+                            if (DIP) {
+                                xd[sl * KP + j] = jm ? j * qx : 0.0;
+                                yd[sl * KP + j] = jm ? j * qy : 0.0;
+                                zd[sl * KP + j] = jm ? j * qz : 0.0;
+                            }
                         }
-                        if (DIP) {
+                        if (DIP && j0 == 0) {
                             double w1 = 0, w2 = 0, w3 = 0;
                             if (in) {
                                 const double* wv = a.dipw + 3 * ((long)a.nd * i + a.ind);
                                 w1 = wv[0], w2 = wv[1], w3 = wv[2];
                             }
-                            sw[4 * t] = q;
-                            sw[4 * t + 1] = a.dsq * w1;
-                            sw[4 * t + 2] = a.dsq * w2;
-                            sw[4 * t + 3] = a.dsq * w3;
+                            sw[4 * sl] = q;
+                            sw[4 * sl + 1] = a.dsq * w1;
+                            sw[4 * sl + 2] = a.dsq * w2;
+                            sw[4 * sl + 3] = a.dsq * w3;
                         }
                     }
                     __syncthreads();
@@ -566,13 +570,17 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
             }
         }
         __syncthreads();
-        // ---- 3, 4 per z node; P_c double-buffered in LDS at k + PK a ----
+        // ---- 3, 4 per group of G z nodes; P_c double-buffered in LDS at
+        // k + PK a, G buffers per parity ----
         double* out = a.mp + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
-        for (int c = 0; c < half; ++c) {
-            double* Pc = work + (c & 1) * 2 * PK * 16 * NB;
-            const double* Nc = Ng + 2L * KLP * c;
-            for (int tile = w; tile < NB * MT; tile += 8) {
-                const int na = tile % NB, mt = tile / NB;
+        constexpr int G = NB * NB >= 8 ? 1 : 8 / (NB * NB);
+        for (int c0 = 0, par = 0; c0 < half; c0 += G, par ^= 1) {
+            for (int tile = w; tile < G * NB * MT; tile += 8) {
+                const int g = tile / (NB * MT), na = tile % NB, mt = (tile / NB) % MT;
+                const int c = c0 + g;
+                if (c >= half) continue;
+                double* Pc = work + (par * G + g) * 2 * PK * 16 * NB;
+                const double* Nc = Ng + 2L * KLP * c;
                 d4_t3 pr = {0, 0, 0, 0}, pi = {0, 0, 0, 0};
                 for (int ks = 0; ks < KS; ++ks) {
                     const int j = 4 * ks + k4;
@@ -597,8 +605,11 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                 }
             }
             __syncthreads();
-            for (int tile = w; tile < NB * NB; tile += 8) {
-                const int nb = tile % NB, na = tile / NB;
+            for (int tile = w; tile < G * NB * NB; tile += 8) {
+                const int g = tile / (NB * NB), nb = tile % NB, na = (tile / NB) % NB;
+                const int c = c0 + g;
+                if (c >= half) continue;
+                const double* Pc = work + (par * G + g) * 2 * PK * 16 * NB;
                 d4_t3 vr = {0, 0, 0, 0}, vi = {0, 0, 0, 0};
                 for (int ks = 0; ks < KS; ++ks) {
                     const int k = 4 * ks + k4;
@@ -640,7 +651,8 @@ static void form3_mfma_launch(const Form3Args& a, hipStream_t s)
     const int m = a.p + 1, KP = 16 * ((m + 15) / 16), KS = (m + 3) / 4;
     const size_t ct = (size_t)2 * (16 * NB + 1) * 4 * KS;
     const size_t src = (size_t)(DIP ? 6 : 3) * 32 * KP + (DIP ? 4 * 32 : 0);
-    const size_t pc = (size_t)2 * 2 * (KP + 1) * 16 * NB;
+    constexpr int G = NB * NB >= 8 ? 1 : 8 / (NB * NB);
+    const size_t pc = (size_t)2 * G * 2 * (KP + 1) * 16 * NB;
     const size_t lds = (ct + std::max(src, pc)) * sizeof(double);
     allow_lds3(k_form3_mfma<NB, DIP>, lds);
     k_form3_mfma<NB, DIP><<<a.grid, 512, lds, s>>>(a);
@@ -1124,11 +1136,22 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
         for (int c = 0; c < half; ++c) {
             wave_sync3();  // previous readers of P / S1 are done
             const double* slab = pw + 2L * npw * npw * c;
-            for (int o = lane; o < PW3_N * PW3_N; o += 64) {
-                const int aa = o % PW3_N, b = o / PW3_N, op = aa + PW3_S * b;
-                const bool in = aa < npw && b < npw;
-                P[2 * op] = in ? slab[2 * (aa + npw * b)] : 0.0;
-                P[2 * op + 1] = in ? slab[2 * (aa + npw * b) + 1] : 0.0;
+            // the lane's 16 loads in two batches of 8, each issued before
+            // its first store
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                double2 v[8];
+#pragma unroll
+                for (int it = 0; it < 8; ++it) {
+                    const int o = lane + 64 * (8 * h + it), aa = o % PW3_N, b = o / PW3_N;
+                    v[it] = aa < npw && b < npw ? *(const double2*)(slab + 2 * (aa + npw * b))
+                                                : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int it = 0; it < 8; ++it) {
+                    const int o = lane + 64 * (8 * h + it), aa = o % PW3_N, b = o / PW3_N;
+                    *(double2*)(P + 2 * (aa + PW3_S * b)) = v[it];
+                }
             }
             wave_sync3();
             // S1 = D^T PW: two 16-column tiles over b
@@ -1232,11 +1255,21 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
         for (int c = 0; c < half; ++c) {
             __syncthreads();  // readers of P, S1 of the previous node are done
             const double* slab = pw + 2L * npw * npw * c;
-            for (int o = t; o < A4 * BP; o += 512) {
-                const int aa = o % A4, b = o / A4, op = aa + PS * b;
-                const bool in = aa < npw && b < npw;
-                P[2 * op] = in ? slab[2 * (aa + npw * b)] : 0.0;
-                P[2 * op + 1] = in ? slab[2 * (aa + npw * b) + 1] : 0.0;
+            {
+                constexpr int NIT = (A4 * BP + 511) / 512;
+                double2 v[NIT];
+#pragma unroll
+                for (int it = 0; it < NIT; ++it) {
+                    const int o = t + 512 * it, aa = o % A4, b = o / A4;
+                    v[it] = o < A4 * BP && aa < npw && b < npw
+                                ? *(const double2*)(slab + 2 * (aa + npw * b))
+                                : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int it = 0; it < NIT; ++it) {
+                    const int o = t + 512 * it, aa = o % A4, b = o / A4;
+                    if (o < A4 * BP) *(double2*)(P + 2 * (aa + PS * b)) = v[it];
+                }
             }
             __syncthreads();
             if (w < 2 * NB) {
