@@ -1566,6 +1566,9 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     struct DownChunk { int l, e0, e1; };
     std::vector<UpChunk> up;
     std::vector<DownChunk> down;
+    // levels whose leaves all take the Taylor route through a kernel that
+    // fuses the parent -> leaf split: no split launch, no chunk slots
+    std::vector<char> fused(nlevels + 1, 0);
     const size_t per = 2 * (size_t)nexp * nd;  // doubles per slot
     size_t ntay = 1;                             // Taylor buffer, doubles
     int ncap = 0;                                // chunk region, slots
@@ -1608,8 +1611,10 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         // memory: persistent slots, then the chunk region and the Taylor
         // buffer as large as the largest level needs, or as the device allows
         size_t need_c = 0, need_tay = 1, m3max = 1;
-        for (int l = nlevstart + 1; l <= nlevels; ++l)
-            need_c = std::max({need_c, form_l[l].size(), eval_l[l].size()});
+        for (int l = nlevstart + 1; l <= nlevels; ++l) {
+            fused[l] = eval_thresh[l] == 0 && pw2local3_fusable(nlocal[l] + 1, npw);
+            need_c = std::max({need_c, form_l[l].size(), fused[l] ? (size_t)0 : eval_l[l].size()});
+        }
         for (int l = nlevstart; l <= nlevels; ++l) {
             if (eval_thresh[l] == INT_MAX || eval_l[l].empty()) continue;
             const int m = nlocal[l] + 1;
@@ -1671,12 +1676,12 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
             const int m = nlocal[l] + 1;
             const size_t st = taylor3_stride(m, m <= 32);
             size_t cmax = eval_thresh[l] == INT_MAX ? n : std::max<size_t>(1, ntay / st);
-            if (l > nlevstart) cmax = std::min<size_t>(cmax, ncap);
+            if (l > nlevstart && !fused[l]) cmax = std::min<size_t>(cmax, ncap);
             if (cap_env > 0) cmax = std::min(cmax, cap_env);
             for (size_t e0 = 0; e0 < n; e0 += cmax) {
                 const size_t e1 = std::min(n, e0 + cmax);
                 down.push_back({l, (int)e0, (int)e1});
-                if (l > nlevstart)
+                if (l > nlevstart && !fused[l])
                     for (size_t i = e0; i < e1; ++i) slot_loc[eval_l[l][i]] = n_loc + (int)(i - e0);
             }
         }
@@ -1953,7 +1958,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
                 const DownChunk& dc = down[di];
                 const int* boxes = (const int*)D(o_eval[l]) + dc.e0;
                 const int nbox = dc.e1 - dc.e0;
-                if (l > nlevstart) {
+                if (l > nlevstart && !fused[l]) {
                     tm.begin(S_SPLIT);
                     tm.ctx(l - 1, &eval_l[l], 2, 0, dc.e0, dc.e1);
                     launch_split3(boxes, nbox, d_par, d_cen, d_sloc, tab.wms, 8 * (nlevels - l), nd,
@@ -1983,6 +1988,12 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
                     ea.use_trg = use_trg;
                     ea.thresh = eval_thresh[l];
                     ea.tlayout = nlocal[l] + 1 <= 32;
+                    if (fused[l]) {
+                        ea.fused = 1;
+                        ea.par = d_par;
+                        ea.wms = tab.wms;
+                        ea.kbase = 8 * (nlevels - l);
+                    }
                     ea.grid = std::min(2048, ea.nbox);
                     tm.ctx(l, &eval_l[l], 2, 0, dc.e0, dc.e1);
                     if (ea.thresh != INT_MAX) launch_pw2local3(ea, s);

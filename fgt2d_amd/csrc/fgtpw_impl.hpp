@@ -326,7 +326,14 @@ struct Eval3Args {
     // Taylor coefficient layout: 0 L[j3 + m (j2 + m j1)]; 1 (m <= 32) the
     // transposed Lt[j2 + MP (j3 + m j1)], MP = m rounded up to 16, zero columns
     int tlayout;
+    // fused split (PW -> Taylor only): the leaf's expansion is its parent's
+    // (slot of par[box]) times the octant shift wms[kbase + octant]
+    int fused;
+    const int* par;
+    const double* wms;
+    int kbase;
 };
+bool pw2local3_fusable(int m, int npw);
 // doubles per box of the Taylor buffer
 __host__ __device__ inline long taylor3_stride(int m, int tlayout)
 {

@@ -1127,7 +1127,16 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
     for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
         const int box = a.boxes[li];
         if (pw_box(a, box) || !has_eval(a, box)) continue;
-        const double* pw = a.loc + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+        // the leaf's own expansion, or with a fused split its parent's,
+        // multiplied by the octant shift while the slab is staged
+        const int src = a.fused ? a.par[box] : box;
+        const double* pw = a.loc + 2 * ((long)a.slot[src] * a.nd + a.ind) * nexp;
+        const double* wk = nullptr;
+        if (a.fused) {
+            const int k = octant(a.cen[3 * box] - a.cen[3 * src], a.cen[3 * box + 1] - a.cen[3 * src + 1],
+                                 a.cen[3 * box + 2] - a.cen[3 * src + 2]);
+            wk = a.wms + 2 * ((long)(a.kbase + k) * nexp);
+        }
         double L[4][16];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -1146,6 +1155,18 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
                     const int o = lane + 64 * (8 * h + it), aa = o % PW3_N, b = o / PW3_N;
                     v[it] = aa < npw && b < npw ? *(const double2*)(slab + 2 * (aa + npw * b))
                                                 : make_double2(0.0, 0.0);
+                }
+                if (wk) {
+                    const double* ws = wk + 2L * npw * npw * c;
+#pragma unroll
+                    for (int it = 0; it < 8; ++it) {
+                        const int o = lane + 64 * (8 * h + it), aa = o % PW3_N, b = o / PW3_N;
+                        if (aa < npw && b < npw) {
+                            const double2 wv = *(const double2*)(ws + 2 * (aa + npw * b));
+                            v[it] = make_double2(v[it].x * wv.x - v[it].y * wv.y,
+                                                 v[it].x * wv.y + v[it].y * wv.x);
+                        }
+                    }
                 }
 #pragma unroll
                 for (int it = 0; it < 8; ++it) {
@@ -1246,7 +1267,16 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
     for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
         const int box = a.boxes[li];
         if (pw_box(a, box) || !has_eval(a, box)) continue;
-        const double* pw = a.loc + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+        // the leaf's own expansion, or with a fused split its parent's,
+        // multiplied by the octant shift while the slab is staged
+        const int src = a.fused ? a.par[box] : box;
+        const double* pw = a.loc + 2 * ((long)a.slot[src] * a.nd + a.ind) * nexp;
+        const double* wk = nullptr;
+        if (a.fused) {
+            const int k = octant(a.cen[3 * box] - a.cen[3 * src], a.cen[3 * box + 1] - a.cen[3 * src + 1],
+                                 a.cen[3 * box + 2] - a.cen[3 * src + 2]);
+            wk = a.wms + 2 * ((long)(a.kbase + k) * nexp);
+        }
         double L[4][16];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -1264,6 +1294,18 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
                     v[it] = o < A4 * BP && aa < npw && b < npw
                                 ? *(const double2*)(slab + 2 * (aa + npw * b))
                                 : make_double2(0.0, 0.0);
+                }
+                if (wk) {
+                    const double* ws = wk + 2L * npw * npw * c;
+#pragma unroll
+                    for (int it = 0; it < NIT; ++it) {
+                        const int o = t + 512 * it, aa = o % A4, b = o / A4;
+                        if (o < A4 * BP && aa < npw && b < npw) {
+                            const double2 wv = *(const double2*)(ws + 2 * (aa + npw * b));
+                            v[it] = make_double2(v[it].x * wv.x - v[it].y * wv.y,
+                                                 v[it].x * wv.y + v[it].y * wv.x);
+                        }
+                    }
                 }
 #pragma unroll
                 for (int it = 0; it < NIT; ++it) {
@@ -1338,11 +1380,20 @@ static void pw2local3_blk_launch(const Eval3Args& a, hipStream_t s)
     k_pw2local3_blk<KA, NB><<<std::min(a.nbox, 8192), 512, lds, s>>>(a);
 }
 
+// whether launch_pw2local3 runs a kernel that can fuse the parent -> leaf
+// split (Eval3Args::fused) for Taylor order m and npw
+bool pw2local3_fusable(int m, int npw)
+{
+    return (m <= 16 && npw <= PW3_N) || (m > 16 && m <= 32 && npw <= 48);
+}
+
 void launch_pw2local3(const Eval3Args& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
     const int m = a.nl + 1;
     if (m > L3_MAXM) fail(-3, "3D Taylor order above 63 is not supported");
+    if (a.fused && !(pw2local3_fusable(m, a.npw) && a.tlayout))
+        fail(-3, "fused split requested for a PW -> Taylor kernel without it");
     if (m <= 16 && a.npw <= PW3_N && a.tlayout) {
         size_t lds = ((size_t)2 * PW3_S * PW3_N + (size_t)2 * (a.npw / 2) * 16) * sizeof(double);
         kmark(KID_PW2LOCAL, false, s);
