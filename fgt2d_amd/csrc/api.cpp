@@ -1591,15 +1591,18 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
                         if (c >= 0 && T.nch[c] > 0) split_l[l].push_back(c);
                     }
                 }
+                const bool evaluated = T.nch[b] == 0 && !(l == npwlevel && !iflocal[b]);
                 if (l == nlevstart) {
-                    m2l_l.push_back(b);
+                    // local expansions only where they are read: non-leaf
+                    // boxes (split to their children) and evaluated leaves
+                    if (T.nch[b] > 0 || evaluated) m2l_l.push_back(b);
                     if (T.nch[b] == 0 && l >= lform) {
                         form_l[l].push_back(b);
                         slot_mp[b] = n_mp++;
                     }
                 }
-                if (l == nlevstart || T.nch[b] > 0) slot_loc[b] = n_loc++;
-                if (T.nch[b] == 0 && !(l == npwlevel && !iflocal[b])) eval_l[l].push_back(b);
+                if ((l == nlevstart && evaluated) || T.nch[b] > 0) slot_loc[b] = n_loc++;
+                if (evaluated) eval_l[l].push_back(b);
             }
         // leaves below the top level, grouped by parent
         for (int l = nlevstart + 1; l <= nlevels; ++l)

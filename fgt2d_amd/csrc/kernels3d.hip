@@ -880,6 +880,8 @@ void launch_merge3(const int* parents, int np, const int* child8, const double* 
 // evaluation points get a zero expansion.  Multipole and local expansions
 // have separate slot tables; a box without a multipole slot (slot_mp < 0)
 // has a zero multipole expansion.
+constexpr long M2L3_CHUNK = 512;
+
 __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int nb,
                                               const double* __restrict__ cen,
                                               const int* __restrict__ ncol,
@@ -894,8 +896,15 @@ __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int
                                               const double* __restrict__ mp,
                                               double* __restrict__ loc)
 {
-    for (long t = blockIdx.x * 256L + threadIdx.x; t < (long)nb * nexp; t += gridDim.x * 256L) {
-        const long bi = t / nexp, e = t - bi * nexp;
+    // threads run over chunks of M2L3_CHUNK coefficients box after box, so
+    // boxes that share colleagues read the same chunk of a colleague's
+    // expansion close together in time (from cache), in runs long enough
+    // for DRAM
+    const long nchunk = (nexp + M2L3_CHUNK - 1) / M2L3_CHUNK;
+    for (long t = blockIdx.x * 256L + threadIdx.x; t < (long)nb * nchunk * M2L3_CHUNK;
+         t += gridDim.x * 256L) {
+        const long q = t / M2L3_CHUNK, bi = q % nb, e = (q / nb) * M2L3_CHUNK + t % M2L3_CHUNK;
+        if (e >= nexp) continue;
         int b = boxes[bi];
         int npts = 0;
         if (use_trg) npts += trg_se[b].y - trg_se[b].x;
@@ -933,7 +942,7 @@ void launch_m2l3(const int* boxes, int nb, const double* cen, const int* ncol, c
 {
     if (nb <= 0) return;
     kmark(KID_M2L, false, s);
-    k_m2l3<<<gridcap((long)nb * nexp, 256), 256, 0, s>>>(boxes, nb, cen, ncol, col27, nch, src_se,
+    k_m2l3<<<gridcap((long)nb * ((nexp + M2L3_CHUNK - 1) / M2L3_CHUNK) * M2L3_CHUNK, 256), 256, 0, s>>>(boxes, nb, cen, ncol, col27, nch, src_se,
                                                        trg_se, use_src, use_trg, slot_mp, slot_loc,
                                                        wm2l, xmin, do_list, nd, nexp, mp, loc);
     kmark(KID_M2L, true, s);
