@@ -1839,11 +1839,130 @@ __global__ __launch_bounds__(256) void k_pw_eval3(Eval3Args a)
     }
 }
 
+// Plane-wave evaluation of potentials on FP64 matrix cores (npw <= 16 NB):
+//   T_c(p, b) = sum_a e^{i ts_a x_p} PW(a, b, c)       MFMA, 16-point tiles
+//   pot(p) += 2 Re sum_c e^{i ts_c z_p} sum_b e^{i ts_b y_p} T_c(p, b)
+// with the phases from sincos (no recurrence).  One wavefront per segment of
+// at most 256 points of one leaf; lane l supplies the x phases of point l%16
+// and holds T_c(l/16 + 4r, 16 nt + l%16); the b sum runs in registers per
+// lane and over 16 lanes at the end.
+template <int NB>
+__global__ __launch_bounds__(256) void k_pw_eval3_mfma(Eval3Args a)
+{
+    constexpr int KA = 4 * NB;  // k-steps over a (npw <= 4 KA)
+    const int npw = a.npw, half = npw / 2;
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+    const long nexp = (long)npw * npw * half;
+    const int total = a.tstart[a.nbox];
+    for (int g = a.tstart[0] + blockIdx.x * 4 + (threadIdx.x >> 6); g < total;
+         g += gridDim.x * 4) {
+        int lo = 0, hi = a.nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (a.tstart[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        const int b = a.boxes[lo];
+        if (!pw_box(a, b)) continue;
+        const int2 rr = a.se[b];
+        const int p0 = rr.x + (g - a.tstart[lo]) * a.seg, p1 = min(rr.y, p0 + a.seg);
+        const double* pw = a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
+        const double cx = a.cen[3 * b], cy = a.cen[3 * b + 1], cz = a.cen[3 * b + 2], dsq = a.dsq;
+        for (int t0 = p0; t0 < p1; t0 += 16) {
+            // A operands: x phases of point t0 + l%16 at nodes 4 ks + l/16
+            double exr[KA], exi[KA];
+            {
+                const int pa = t0 + i16;
+                const double x = pa < p1 ? (a.pts[3 * (long)pa] - cx) * dsq : 0.0;
+#pragma unroll
+                for (int ks = 0; ks < KA; ++ks) {
+                    const int aa = 4 * ks + k4;
+                    double sn = 0, cs = 0;
+                    if (aa < npw) sincos(a.ts[aa] * x, &sn, &cs);
+                    exr[ks] = cs;
+                    exi[ks] = sn;
+                }
+            }
+            // rows: points t0 + l/16 + 4r; y phases of nodes 16 nt + l%16
+            double eyr[4][NB], eyi[4][NB], zr[4];
+            bool in_r[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int p = t0 + k4 + 4 * r;
+                in_r[r] = p < p1;
+                const double y = in_r[r] ? (a.pts[3 * (long)p + 1] - cy) * dsq : 0.0;
+                zr[r] = in_r[r] ? (a.pts[3 * (long)p + 2] - cz) * dsq : 0.0;
+#pragma unroll
+                for (int nt = 0; nt < NB; ++nt) {
+                    const int bb = 16 * nt + i16;
+                    double sn = 0, cs = 0;
+                    if (bb < npw) sincos(a.ts[bb] * y, &sn, &cs);
+                    eyr[r][nt] = cs;
+                    eyi[r][nt] = sn;
+                }
+            }
+            double ur[4] = {0, 0, 0, 0}, ui[4] = {0, 0, 0, 0};
+            for (int c = 0; c < half; ++c) {
+                const double* pc = pw + 2L * npw * npw * c;
+                d4_t3 tr[NB], ti[NB];
+#pragma unroll
+                for (int nt = 0; nt < NB; ++nt) {
+                    tr[nt] = (d4_t3){0, 0, 0, 0};
+                    ti[nt] = (d4_t3){0, 0, 0, 0};
+                }
+#pragma unroll
+                for (int ks = 0; ks < KA; ++ks) {
+                    const int aa = 4 * ks + k4;
+#pragma unroll
+                    for (int nt = 0; nt < NB; ++nt) {
+                        const int bb = 16 * nt + i16;
+                        double2 v = make_double2(0.0, 0.0);
+                        if (aa < npw && bb < npw) v = *(const double2*)(pc + 2 * (aa + npw * bb));
+                        tr[nt] = mfma3(exr[ks], v.x, tr[nt]);
+                        tr[nt] = mfma3(-exi[ks], v.y, tr[nt]);
+                        ti[nt] = mfma3(exr[ks], v.y, ti[nt]);
+                        ti[nt] = mfma3(exi[ks], v.x, ti[nt]);
+                    }
+                }
+                const double tc = a.ts[c];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    double sr = 0, si = 0;
+#pragma unroll
+                    for (int nt = 0; nt < NB; ++nt) {
+                        sr += tr[nt][r] * eyr[r][nt] - ti[nt][r] * eyi[r][nt];
+                        si += tr[nt][r] * eyi[r][nt] + ti[nt][r] * eyr[r][nt];
+                    }
+                    double szn, szc;
+                    sincos(tc * zr[r], &szn, &szc);
+                    ur[r] += szc * sr - szn * si;
+                    ui[r] += szc * si + szn * sr;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) ur[r] += __shfl_xor(ur[r], o, 64);
+            if (i16 == 0)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (in_r[r]) a.pot[(long)(t0 + k4 + 4 * r) * a.nd + a.ind] += 2 * ur[r];
+        }
+    }
+}
+
 void launch_pw_eval3(const Eval3Args& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
     kmark(KID_PW_EVAL, false, s);
-    if (a.ifpgh >= 3)
+    if (a.ifpgh == 1 && a.npw <= 48) {
+        const int grid = (a.grid + 3) / 4;
+        if (a.npw <= 16)
+            k_pw_eval3_mfma<1><<<grid, 256, 0, s>>>(a);
+        else if (a.npw <= 32)
+            k_pw_eval3_mfma<2><<<grid, 256, 0, s>>>(a);
+        else
+            k_pw_eval3_mfma<3><<<grid, 256, 0, s>>>(a);
+    } else if (a.ifpgh >= 3)
         k_pw_eval3<3><<<a.grid, 256, 0, s>>>(a);
     else if (a.ifpgh == 2)
         k_pw_eval3<2><<<a.grid, 256, 0, s>>>(a);
