@@ -122,6 +122,10 @@ def dlaran_points3d(kind, ns, nt, nd=1, dipoles=False):
               radial offsets 0.002*(u1+u2+u3+u4-2) (directions from three
               centred sums of four uniforms, normalised); targets uniform;
               charges 2u-1.
+    cluster : sources uniform in the cube of side 0.03 centred at
+              (0.5,0.5,0.5), targets uniform in the cube of side 0.06 about
+              the same centre; charges 2u-1 (at delta 1e-5 the tree refines
+              two levels below the cutoff level with few points).
     reftest : the reference 3D test driver's own inputs (test_fgt3d.f:48-98).
 
     dipoles=True also returns normals (ns, 3) and dipole strengths (ns, nd):
@@ -137,6 +141,8 @@ def dlaran_points3d(kind, ns, nt, nd=1, dipoles=False):
     r = O.Hkrand()
     if kind == "uniform":
         src = r.draw(3 * ns).reshape(ns, 3)
+    elif kind == "cluster":
+        src = 0.485 + 0.03 * r.draw(3 * ns).reshape(ns, 3)
     else:
         v = r.draw(12 * ns).reshape(ns, 3, 4).sum(axis=2) - 2.0
         v = v / np.sqrt((v * v).sum(axis=1))[:, None]
@@ -144,6 +150,8 @@ def dlaran_points3d(kind, ns, nt, nd=1, dipoles=False):
         src = 0.5 + rad[:, None] * v
     chg = 2.0 * r.draw(ns * nd).reshape(ns, nd) - 1.0
     tg = r.draw(3 * nt).reshape(nt, 3)
+    if kind == "cluster":
+        tg = 0.47 + 0.06 * tg
     if dipoles:
         rn = 2.0 * r.draw(3 * ns).reshape(ns, 3) - 1.0
         ds = 2.0 * r.draw(ns * nd).reshape(ns, nd) - 1.0

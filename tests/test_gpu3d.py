@@ -27,6 +27,7 @@ CASES = [
     ("uniform", 20000, 2000, 2e-2, 1e-3, 1, 1, 3),     # npw 12: direct plane-wave routes
     ("uniform", 2000, 700, 1e-3, 1e-6, 1, 1, 1),       # npwlevel == nlevels: near field only
     ("uniform", 1000, 300, 30.0, 1e-9, 1, 3, 3),       # large delta: one Taylor expansion
+    ("cluster", 20000, 4000, 1e-5, 1e-9, 1, 1, 2),     # npw 38, Taylor orders 59/30/19 below the cutoff
 ]
 
 

@@ -119,6 +119,7 @@ CASES3 = [
     ("uniform", 20000, 4000, 1e-3, 1e-6, 1, 1, 1, 1),
     ("uniform", 4000, 1000, 1e-4, 1e-6, 1, 3, 3, 1),      # near field only
     ("uniform", 600, 300, 5.0, 1e-6, 2, 3, 3, 1),         # large delta
+    ("cluster", 20000, 4000, 1e-5, 1e-9, 1, 1, 1, 1),     # npw 38, Taylor orders 59/30/19
 ]
 
 
