@@ -1496,6 +1496,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     W.last_valid = false;
     Tree3D& T = W.last3;
     build_tree<3>(T, leaves, nlmax, npwlevel, bs0, cen0);
+    tm.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
     W.last_ns = ns;
     W.last_nt = nt;
     W.last_dim = 3;

@@ -24,12 +24,141 @@
 // Child j of a box has bit (j >> a) & 1 on axis a (x, y, z), as the
 // reference numbers children (tree_routs4.f:28-44, tree_routs3d.f:28-46).
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
+#include <pthread.h>
+#include <thread>
 
 #include "fgtpw_impl.hpp"
 
 namespace fgtpw {
+
+namespace {
+
+// Fork-join over index ranges for the tree builder: a few worker threads
+// that spin briefly between forks (a build forks a few dozen times within a
+// fraction of a millisecond) and sleep otherwise.  One caller at a time; a
+// caller that finds the pool busy (concurrent transforms) runs its loop
+// itself.  Only loops whose result does not depend on the order of their
+// iterations are run here, so the tree is the same as the sequential one.
+class TreePool {
+public:
+    // never destroyed: the workers live until the process ends (no join at
+    // exit, and nothing to join in a forked child)
+    static TreePool& get()
+    {
+        static TreePool* p = new TreePool;
+        return *p;
+    }
+    // a forked child has none of the parent's workers: it runs sequentially
+    bool acquire() { return nw_ > 0 && !forked_.load() && busy_.try_lock(); }
+    void release() { busy_.unlock(); }
+    // f(i0, i1) over [0, n) in chunks of `grain`; the caller holds the pool
+    template <class F>
+    void run(long n, long grain, F& f)
+    {
+        if (n <= grain) {
+            f(0, n);
+            return;
+        }
+        fn_ = [](void* c, long i0, long i1) { (*(F*)c)(i0, i1); };
+        ctx_ = &f;
+        total_ = n;
+        grain_ = grain;
+        next_.store(0, std::memory_order_relaxed);
+        done_.store(0, std::memory_order_relaxed);
+        gen_.fetch_add(1);  // sequentially consistent with the sleepers' check
+        if (sleeping_.load() > 0) {
+            std::lock_guard<std::mutex> lk(sleep_mu_);
+            sleep_cv_.notify_all();
+        }
+        chunks();
+        while (done_.load(std::memory_order_acquire) < nw_) __builtin_ia32_pause();
+    }
+
+private:
+    TreePool()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        nw_ = (int)std::min(7u, hw > 1 ? hw - 1 : 0u);
+        for (int i = 0; i < nw_; ++i) std::thread([this] { worker(); }).detach();
+        pthread_atfork(nullptr, nullptr, [] { forked_.store(true); });
+    }
+    static inline std::atomic<bool> forked_{false};
+    void chunks()
+    {
+        for (;;) {
+            const long i0 = next_.fetch_add(grain_, std::memory_order_relaxed);
+            if (i0 >= total_) break;
+            fn_(ctx_, i0, std::min(total_, i0 + grain_));
+        }
+    }
+    void worker()
+    {
+        unsigned seen = 0;  // the generation at construction, before any fork
+        for (;;) {
+            int spin = 0;
+            while (gen_.load(std::memory_order_acquire) == seen) {
+                if (++spin < 100000) {  // ~1 ms of pause instructions
+                    __builtin_ia32_pause();
+                    continue;
+                }
+                std::unique_lock<std::mutex> lk(sleep_mu_);
+                sleeping_.fetch_add(1);
+                if (gen_.load() == seen)
+                    sleep_cv_.wait_for(lk, std::chrono::milliseconds(5));
+                sleeping_.fetch_sub(1);
+                spin = 0;
+            }
+            seen = gen_.load(std::memory_order_acquire);
+            chunks();
+            done_.fetch_add(1, std::memory_order_acq_rel);
+        }
+    }
+    int nw_ = 0;
+    std::mutex busy_, sleep_mu_;
+    std::condition_variable sleep_cv_;
+    std::atomic<unsigned> gen_{0};
+    std::atomic<long> next_{0};
+    std::atomic<int> done_{0}, sleeping_{0};
+    void (*fn_)(void*, long, long) = nullptr;
+    void* ctx_ = nullptr;
+    long total_ = 0, grain_ = 1;
+};
+
+// parallel loop when the builder holds the pool, sequential otherwise
+struct Par {
+    bool on = false;
+    explicit Par(bool want) : on(want && TreePool::get().acquire()) {}
+    ~Par()
+    {
+        if (on) TreePool::get().release();
+    }
+    template <class F>
+    void range(long n, long grain, F f)
+    {
+        if (on && n > grain)
+            TreePool::get().run(n, grain, f);
+        else
+            f(0, n);
+    }
+};
+thread_local Par* t_par = nullptr;  // the builder's loop runner on this thread
+
+template <class F>
+void par_range(long n, long grain, F f)
+{
+    if (t_par)
+        t_par->range(n, grain, f);
+    else
+        f(0, n);
+}
+
+}  // namespace
 
 // x**n for integer n as Fortran lowers it: binary powering.
 static double ipow(double a, int n)
@@ -264,9 +393,13 @@ struct Builder {
     {
         ncol[0] = 1;
         col[0] = 0;
-        for (int l = 0; l < std::min(T.nlevels, maxlev); ++l)
-            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b)
-                if (nch[b] > 0) family_colleagues(b);
+        for (int l = 0; l < std::min(T.nlevels, maxlev); ++l) {
+            const int b0 = T.laddr[2 * l];
+            par_range(T.laddr[2 * l + 1] - b0 + 1, 512, [&](long i0, long i1) {
+                for (long i = i0; i < i1; ++i)
+                    if (nch[b0 + i] > 0) family_colleagues(b0 + (int)i);
+            });
+        }
     }
 
     // Refine the boxes of [b0,b1] whose flag is positive; children of flag-1
@@ -291,10 +424,10 @@ struct Builder {
     // appended to it (pts_tree_reorg, pts_tree2d4.f:1001-1150,
     // pts_tree3d.f:1172-1329).
     template <class V>
-    void permute(V& v, int width, const std::vector<int>& map)
+    static void permute(V& v, int width, const std::vector<int>& map)
     {
         using E = typename V::value_type;
-        std::vector<char>& scratch = T.scratch;
+        thread_local std::vector<char> scratch;
         if (scratch.size() < v.size() * sizeof(E)) scratch.resize(v.size() * sizeof(E));
         E* old = (E*)scratch.data();
         std::memcpy(old, v.data(), v.size() * sizeof(E));
@@ -318,20 +451,32 @@ struct Builder {
             for (int b = tail[2 * l]; b <= tail[2 * l + 1]; ++b) map[b] = cur++;
             T.laddr[2 * l + 1] = cur - 1;
         }
-        permute(T.lev, 1, map);
-        permute(T.nch, 1, map);
-        permute(T.cen, D, map);
-        permute(T.flag, 1, map);
-        permute(T.pre, 1, map);
-        for (int a = 0; a < D; ++a) permute(T.ic[a], 1, map);
-        permute(T.par, 1, map);
-        permute(T.ch, CH, map);
+        // the arrays are independent: one task each (large trees only)
+        auto each = [&](long i0, long i1) {
+            for (long k = i0; k < i1; ++k) switch (k) {
+                    case 0: permute(T.lev, 1, map); break;
+                    case 1: permute(T.nch, 1, map); break;
+                    case 2: permute(T.cen, D, map); break;
+                    case 3: permute(T.flag, 1, map); break;
+                    case 4: permute(T.pre, 1, map); break;
+                    case 5: permute(T.par, 1, map); break;
+                    case 6: permute(T.ch, CH, map); break;
+                    default: permute(T.ic[k - 7], 1, map); break;
+                }
+        };
+        if (n >= 32768)
+            par_range(7 + D, 1, each);
+        else
+            each(0, 7 + D);
         view();
         const int* mp = map.data();
-        for (int b = 0; b < n; ++b)
-            if (par[b] >= 0) par[b] = mp[par[b]];
-        for (int i = 0; i < CH * n; ++i)
-            if (ch[i] >= 0) ch[i] = mp[ch[i]];
+        par_range(n, 4096, [&](long i0, long i1) {
+            for (long b = i0; b < i1; ++b) {
+                if (par[b] >= 0) par[b] = mp[par[b]];
+                for (int j = 0; j < CH; ++j)
+                    if (ch[CH * b + j] >= 0) ch[CH * b + j] = mp[ch[CH * b + j]];
+            }
+        });
     }
 
     // flag++ boxes with a touching grandchild-level box that has children are
@@ -339,8 +484,14 @@ struct Builder {
     // tree_routs3d.f:226-344).
     void update_flags(int b0, int b1)
     {
-        for (int b = b0; b <= b1; ++b) {
-            if (flag[b] != 3) continue;
+        par_range(b1 - b0 + 1, 1024, [&](long i0, long i1) {
+            for (long i = i0; i < i1; ++i) update_flag(b0 + (int)i);
+        });
+    }
+    void update_flag(int b)
+    {
+        {
+            if (flag[b] != 3) return;
             flag[b] = 0;
             for (int i = 0; i < ncol[b] && flag[b] == 0; ++i) {
                 int jb = col[NC * b + i];
@@ -365,30 +516,47 @@ struct Builder {
         // leaves touching a box two levels finer, seen from its parent
         // (the test depends on the parent only: once per refined box of
         // levels 1..L-1)
-        int nflag = 0;
-        for (int l = L - 1; l >= 1; --l)
-            for (int dad = T.laddr[2 * l]; dad <= T.laddr[2 * l + 1]; ++dad) {
-                if (nch[dad] == 0) continue;
-                int gd = par[dad];
-                for (int i = 0; i < ncol[gd]; ++i) {
-                    int jb = col[NC * gd + i];
-                    if (nch[jb] == 0 && flag[jb] == 0 && near_finer(jb, dad)) {
-                        flag[jb] = 1;
-                        ++nflag;
+        // (flags are set to the same value by whichever iteration gets
+        // there, so the iterations of a level may run concurrently)
+        std::atomic<int> nflag{0};
+        for (int l = L - 1; l >= 1; --l) {
+            const int d0 = T.laddr[2 * l];
+            par_range(T.laddr[2 * l + 1] - d0 + 1, 1024, [&](long i0, long i1) {
+                int nf = 0;
+                for (long i = i0; i < i1; ++i) {
+                    const int dad = d0 + (int)i;
+                    if (nch[dad] == 0) continue;
+                    int gd = par[dad];
+                    for (int k = 0; k < ncol[gd]; ++k) {
+                        int jb = col[NC * gd + k];
+                        if (nch[jb] == 0 && __atomic_load_n(&flag[jb], __ATOMIC_RELAXED) == 0 &&
+                            near_finer(jb, dad)) {
+                            __atomic_store_n(&flag[jb], 1, __ATOMIC_RELAXED);
+                            ++nf;
+                        }
                     }
                 }
-            }
-        if (nflag == 0) return L - 1;  // already level restricted: nothing below changes
+                if (nf) nflag.fetch_add(nf);
+            });
+        }
+        if (nflag.load() == 0) return L - 1;  // already level restricted: nothing below changes
         // coarser leaves touching a flagged box, propagated upwards
-        for (int l = L; l >= 1; --l)
-            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
-                if (flag[b] != 1 && flag[b] != 2) continue;
-                int dad = par[b];
-                for (int i = 0; i < ncol[dad]; ++i) {
-                    int jb = col[NC * dad + i];
-                    if (nch[jb] == 0 && flag[jb] == 0 && near_finer(jb, b)) flag[jb] = 2;
+        for (int l = L; l >= 1; --l) {
+            const int b0 = T.laddr[2 * l];
+            par_range(T.laddr[2 * l + 1] - b0 + 1, 1024, [&](long i0, long i1) {
+                for (long i = i0; i < i1; ++i) {
+                    const int b = b0 + (int)i;
+                    if (flag[b] != 1 && flag[b] != 2) continue;
+                    int dad = par[b];
+                    for (int k = 0; k < ncol[dad]; ++k) {
+                        int jb = col[NC * dad + k];
+                        if (nch[jb] == 0 && __atomic_load_n(&flag[jb], __ATOMIC_RELAXED) == 0 &&
+                            near_finer(jb, b))
+                            __atomic_store_n(&flag[jb], 2, __ATOMIC_RELAXED);
+                    }
                 }
-            }
+            });
+        }
         std::vector<int>& tail = T.tail;
         tail.assign(2 * (L + 1), 0);
         for (int l = 0; l <= L; ++l) tail[2 * l + 1] = -1;
@@ -413,7 +581,10 @@ struct Builder {
             refine_flagged(T.laddr[2 * l], T.laddr[2 * l + 1]);
             refine_flagged(tail[2 * l], tail[2 * l + 1]);
             tail[2 * (l + 1) + 1] = T.nboxes() - 1;
-            for (int b = tail[2 * (l + 1)]; b <= tail[2 * (l + 1) + 1]; ++b) colleagues(b);
+            const int c0 = tail[2 * (l + 1)];
+            par_range(tail[2 * (l + 1) + 1] - c0 + 1, 1024, [&](long i0, long i1) {
+                for (long i = i0; i < i1; ++i) colleagues(c0 + (int)i);
+            });
         }
         reorg(tail);
         return -1;
@@ -460,6 +631,14 @@ void build_tree(TreeD<D>& T, const std::vector<LeafRun>& leaves, int nlmax, int 
         T.pre.reserve(cap);
         for (int a = 0; a < D; ++a) T.ic[a].reserve(cap);
     }
+    // the pool, for trees large enough to repay its forks and when no other
+    // transform holds it
+    Par par(leaves.size() >= 20000);
+    struct Scope {
+        Par* prev;
+        explicit Scope(Par* p) : prev(t_par) { t_par = p; }
+        ~Scope() { t_par = prev; }
+    } scope(&par);
     Builder<D> B(T);
     T.bsz.push_back(bs0);
     B.grow(1);
