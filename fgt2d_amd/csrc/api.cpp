@@ -1692,7 +1692,13 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     }
     Pack pk(W.h_plan);
     size_t o_child = pk.add(T.ch), o_par = pk.add(T.par), o_cen = pk.add(T.cen);
-    size_t o_ncol = pk.add(T.ncol), o_col = pk.add(T.col), o_rank = pk.add(T.rank_of_box);
+    // colleague lists are read on the device by m2l only, for the boxes of
+    // the top level (a contiguous range): only that slice travels
+    const int colb0 = expansions ? T.laddr[2 * nlevstart] : 0;
+    const int colb1 = expansions ? T.laddr[2 * nlevstart + 1] : -1;
+    size_t o_ncol = pk.add(T.ncol);
+    size_t o_col = pk.add(T.col.data() + 27 * (size_t)colb0, 27 * (size_t)(colb1 - colb0 + 1));
+    size_t o_rank = pk.add(T.rank_of_box);
     size_t o_rlo = pk.add(T.rank_lo), o_rhi = pk.add(T.rank_hi);
     size_t o_smp = pk.add(slot_mp), o_sloc = pk.add(slot_loc);
     size_t o_nch = pk.add(T.nch);
@@ -1802,7 +1808,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     const int* d_par = (const int*)D(o_par);
     const double* d_cen = (const double*)D(o_cen);
     const int* d_ncol = (const int*)D(o_ncol);
-    const int* d_col = (const int*)D(o_col);
+    const int* d_col = (const int*)D(o_col) - 27 * (ptrdiff_t)colb0;  // indexed by box
     const int* d_rank = (const int*)D(o_rank);
     const int* d_rlo = (const int*)D(o_rlo);
     const int* d_rhi = (const int*)D(o_rhi);
