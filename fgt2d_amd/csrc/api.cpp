@@ -361,17 +361,22 @@ void copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t s)
 // ---------------------------------------------------------------------------
 struct Pack {
     std::vector<char> own;
-    std::vector<char>& bytes;
+    std::vector<char>& bytes;  // storage, kept from call to call (no refill)
+    size_t used = 0;           // bytes of this plan
     Pack() : bytes(own) {}
-    explicit Pack(std::vector<char>& storage) : bytes(storage) { bytes.clear(); }
+    explicit Pack(std::vector<char>& storage) : bytes(storage) {}
     template <class T>
     size_t add(const T* p, size_t n)
     {
-        size_t off = (bytes.size() + 255) & ~size_t(255);
-        bytes.resize(off + std::max<size_t>(n * sizeof(T), 1));
+        const size_t off = (used + 255) & ~size_t(255);
+        const size_t end = off + std::max<size_t>(n * sizeof(T), 1);
+        if (bytes.size() < end) bytes.resize(std::max(end, 2 * bytes.size()));
         if (n) std::memcpy(bytes.data() + off, p, n * sizeof(T));
+        used = end;
         return off;
     }
+    const char* data() const { return bytes.data(); }
+    size_t size() const { return used; }
     template <class T>
     size_t add(const std::vector<T>& v)
     {
@@ -465,8 +470,8 @@ const Tables2D& tables2d(Workspace& W, int npw, double pmax, int ntermmax, int n
     size_t o_ts = pk.add(ts), o_h2x = pk.add(h2x), o_h2xr = pk.add(h2xr), o_x2l = pk.add(x2l),
            o_rsq = pk.add(rsq);
     size_t o_wms = pk.add(wms), o_wm2l = pk.add(wm2l);
-    char* d = W.get<char>("tables", pk.bytes.size());
-    HIPCHECK(hipMemcpyAsync(d, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice, s));
+    char* d = W.get<char>("tables", pk.size());
+    HIPCHECK(hipMemcpyAsync(d, pk.data(), pk.size(), hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));  // pk is freed on return
     t.ts = (const double*)(d + o_ts);
     t.h2x = (const double*)(d + o_h2x);
@@ -575,8 +580,8 @@ const Tables3D& tables3d(Workspace& W, int npw, double pmax, int ntermmax, int n
     Pack pk;
     size_t o_ws = pk.add(ws), o_ts = pk.add(ts), o_c = pk.add(coef), o_d = pk.add(dcoef);
     size_t o_ms = pk.add(wms), o_ml = pk.add(wm2l);
-    char* d = W.get<char>("tables3", pk.bytes.size());
-    HIPCHECK(hipMemcpyAsync(d, pk.bytes.data(), pk.bytes.size(), hipMemcpyHostToDevice, s));
+    char* d = W.get<char>("tables3", pk.size());
+    HIPCHECK(hipMemcpyAsync(d, pk.data(), pk.size(), hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));  // pk is freed on return
     t.ws = (const double*)(d + o_ws);
     t.ts = (const double*)(d + o_ts);
@@ -781,7 +786,13 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             if (T.lev[b] >= nlevstart) slot[b] = nslot++;
     Pack pk(W.h_plan);
     size_t o_child = pk.add(T.ch), o_par = pk.add(T.par), o_cen = pk.add(T.cen);
-    size_t o_ncol = pk.add(T.ncol), o_col = pk.add(T.col), o_rank = pk.add(T.rank_of_box);
+    // colleague lists are read on the device by m2l only, for the boxes of
+    // the top level (a contiguous range): only that slice travels
+    const int colb0 = expansions ? T.laddr[2 * nlevstart] : 0;
+    const int colb1 = expansions ? T.laddr[2 * nlevstart + 1] : -1;
+    size_t o_ncol = pk.add(T.ncol);
+    size_t o_col = pk.add(T.col.data() + 9 * (size_t)colb0, 9 * (size_t)(colb1 - colb0 + 1));
+    size_t o_rank = pk.add(T.rank_of_box);
     size_t o_rlo = pk.add(T.rank_lo), o_rhi = pk.add(T.rank_hi), o_slot = pk.add(slot);
 
     // work lists of the expansion steps
@@ -897,18 +908,18 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     tm.host_ms = std::chrono::duration<double, std::milli>(th1 - th0).count();
 
     // upload the plan in one copy
-    char* dplan = W.get<char>("plan", pk.bytes.size());
+    char* dplan = W.get<char>("plan", pk.size());
     {
-        void* hp = W.pinned(pk.bytes.size());
-        std::memcpy(hp, pk.bytes.data(), pk.bytes.size());
-        HIPCHECK(hipMemcpyAsync(dplan, hp, pk.bytes.size(), hipMemcpyHostToDevice, s));
+        void* hp = W.pinned(pk.size());
+        std::memcpy(hp, pk.data(), pk.size());
+        HIPCHECK(hipMemcpyAsync(dplan, hp, pk.size(), hipMemcpyHostToDevice, s));
     }
     auto D = [&](size_t off) { return (void*)(dplan + off); };
     const int4* d_child = (const int4*)D(o_child);
     const int* d_par = (const int*)D(o_par);
     const double* d_cen = (const double*)D(o_cen);
     const int* d_ncol = (const int*)D(o_ncol);
-    const int* d_col = (const int*)D(o_col);
+    const int* d_col = (const int*)D(o_col) - 9 * (ptrdiff_t)colb0;  // indexed by box
     const int* d_rank = (const int*)D(o_rank);
     const int* d_rlo = (const int*)D(o_rlo);
     const int* d_rhi = (const int*)D(o_rhi);
@@ -1797,11 +1808,11 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     auto th1 = std::chrono::steady_clock::now();
     tm.host_ms = std::chrono::duration<double, std::milli>(th1 - th0).count();
 
-    char* dplan = W.get<char>("plan", pk.bytes.size());
+    char* dplan = W.get<char>("plan", pk.size());
     {
-        void* hp = W.pinned(pk.bytes.size());
-        std::memcpy(hp, pk.bytes.data(), pk.bytes.size());
-        HIPCHECK(hipMemcpyAsync(dplan, hp, pk.bytes.size(), hipMemcpyHostToDevice, s));
+        void* hp = W.pinned(pk.size());
+        std::memcpy(hp, pk.data(), pk.size());
+        HIPCHECK(hipMemcpyAsync(dplan, hp, pk.size(), hipMemcpyHostToDevice, s));
     }
     auto D = [&](size_t off) { return (void*)(dplan + off); };
     const int* d_child = (const int*)D(o_child);
