@@ -88,6 +88,8 @@ struct Workspace {
     std::map<std::string, Buf> dev;
     void* pin = nullptr;
     size_t pin_cap = 0;
+    void* pin2 = nullptr;
+    size_t pin2_cap = 0;
     hipStream_t own = nullptr;
     // side streams (default priority): coarse levels of formation and
     // evaluation and the near field overlap the finest level's work
@@ -98,7 +100,7 @@ struct Workspace {
     std::vector<hipEvent_t> kev;  // per-kernel timing events, grown on demand
     // host scratch of the plan, reused from call to call
     std::vector<int> h_units, h_uptr, h_ulist, h_nl1, h_l1, h_sp, h_sl, h_fill, h_slot, h_slot2;
-    std::vector<char> h_plan;
+    std::vector<char> h_plan, h_plan_nf;
     Tables2D tab;
     Tables3D tab3;
     // the tree of the last transform, kept for inspection
@@ -150,6 +152,9 @@ struct Workspace {
         if (pin) (void)hipHostFree(pin);
         pin = nullptr;
         pin_cap = 0;
+        if (pin2) (void)hipHostFree(pin2);
+        pin2 = nullptr;
+        pin2_cap = 0;
         for (auto& e : sev)
             if (e) (void)hipEventDestroy(e);
         sev.clear();
@@ -165,16 +170,21 @@ struct Workspace {
         own = nullptr;
         tab.valid = tab3.valid = false;
     }
-    void* pinned(size_t bytes)
+    void* pinned(size_t bytes) { return grow_pinned(pin, pin_cap, bytes); }
+    // a second staging area, for an upload queued while the first one's may
+    // still be in flight
+    void* pinned2(size_t bytes) { return grow_pinned(pin2, pin2_cap, bytes); }
+    static void* grow_pinned(void*& p, size_t& cap, size_t bytes)
     {
-        if (pin_cap < bytes) {
-            if (pin) HIPCHECK(hipHostFree(pin));
-            pin = nullptr;
+        if (cap < bytes) {
+            if (p) HIPCHECK(hipHostFree(p));
+            p = nullptr;
+            cap = 0;
             size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 16);
-            HIPCHECK(hipHostMalloc(&pin, want, hipHostMallocDefault));
-            pin_cap = want;
+            HIPCHECK(hipHostMalloc(&p, want, hipHostMallocDefault));
+            cap = want;
         }
-        return pin;
+        return p;
     }
 };
 
@@ -838,11 +848,14 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
 
     // near field: modified list1 (compute_modified_list1, tree_routs4.f:715-879)
     // inverted to target units; a unit is a leaf at or above the cutoff level
-    // or a box of the cutoff level, and owns every target below it.
+    // or a box of the cutoff level, and owns every target below it.  Built
+    // after steps 1-3 are queued (only the side-stream near field reads
+    // them), so this host work overlaps the GPU's.
     const int nlevend = std::min(npwlevel, nlevels);
     std::vector<int>& unit_ids = W.h_units;
     std::vector<int>& unit_ptr = W.h_uptr;
     std::vector<int>& unit_list = W.h_ulist;
+    auto near_field_lists = [&]() {
     unit_ids.clear();
     unit_ptr.assign(1, 0);
     unit_list.clear();
@@ -903,7 +916,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             unit_ptr.push_back((int)unit_list.size());
         }
     }
-    size_t o_units = pk.add(unit_ids), o_uptr = pk.add(unit_ptr), o_ulist = pk.add(unit_list);
+    };
     auto th1 = std::chrono::steady_clock::now();
     tm.host_ms = std::chrono::duration<double, std::milli>(th1 - th0).count();
 
@@ -966,14 +979,24 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     // on a side stream, concurrently with steps 1-4; it accumulates into the
     // zeroed outputs before any evaluation does (the evaluations wait for it)
     hipStream_t s_nf = W.aux[1], s_lv = W.aux[0];
-    W.order(s, s_nf);
+    W.order(s, s_nf);  // after the ranges and the zeroed outputs
     tm.begin(S_DIRECT, s_nf);
+    auto near_field = [&]() {
+    near_field_lists();
+    Pack pn(W.h_plan_nf);
+    size_t o_units = pn.add(unit_ids), o_uptr = pn.add(unit_ptr), o_ulist = pn.add(unit_list);
+    char* dnf = W.get<char>("plan_nf", pn.size());
+    {
+        void* hp = W.pinned2(pn.size());
+        std::memcpy(hp, pn.data(), pn.size());
+        HIPCHECK(hipMemcpyAsync(dnf, hp, pn.size(), hipMemcpyHostToDevice, s_nf));
+    }
     if (!unit_ids.empty()) {
         DirectArgs da{};
-        da.units = (const int*)D(o_units);
+        da.units = (const int*)(dnf + o_units);
         da.nunits = (int)unit_ids.size();
-        da.list_ptr = (const int*)D(o_uptr);
-        da.list = (const int*)D(o_ulist);
+        da.list_ptr = (const int*)(dnf + o_uptr);
+        da.list = (const int*)(dnf + o_ulist);
         da.src_se = src_se;
         da.xs = xs;
         da.qs = qs;
@@ -1002,6 +1025,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         }
     }
     tm.end(S_DIRECT, s_nf);
+    };
 
     // ---------------- expansions: steps 1-5 ----------------
     if (expansions) {
@@ -1105,6 +1129,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                    use_src, use_trg, d_slot, d_wm2l, npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0],
                    nlevstart == npwlevel, nd, nexp, mp, lc, s);
         tm.end(S_M2L);
+        near_field();
         // the evaluations accumulate after the near field
         W.order(s_nf, s);
         W.order(s_nf, s_lv);
@@ -1165,6 +1190,8 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         eval_level(nlevels, s);
         W.order(s_lv, s);
         tm.end(S_EVAL);
+    } else {
+        near_field();
     }
 
     W.order(s_nf, s);
