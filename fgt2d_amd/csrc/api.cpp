@@ -1760,11 +1760,14 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
 
     // near field: modified list1 of the source leaves at or above the cutoff
     // (compute_modified_list1, tree_routs3d.f:511-620, FP64 distance tests
-    // as there), inverted to target units as in 2D
+    // as there), inverted to target units as in 2D.  Built once steps 1-5
+    // are queued: the near field runs after them, so this host work overlaps
+    // the GPU's.
     const int nlevend = std::min(npwlevel, nlevels);
     std::vector<int>& unit_ids = W.h_units;
     std::vector<int>& unit_ptr = W.h_uptr;
     std::vector<int>& unit_list = W.h_ulist;
+    auto near_field_lists = [&]() {
     unit_ids.clear();
     unit_ptr.assign(1, 0);
     unit_list.clear();
@@ -1831,7 +1834,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
             unit_ptr.push_back((int)unit_list.size());
         }
     }
-    size_t o_units = pk.add(unit_ids), o_uptr = pk.add(unit_ptr), o_ulist = pk.add(unit_list);
+    };
     auto th1 = std::chrono::steady_clock::now();
     tm.host_ms = std::chrono::duration<double, std::milli>(th1 - th0).count();
 
@@ -2069,13 +2072,22 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     }
 
     // ---------------- step 6: near field, dmax = ln(1/eps) delta (fgt3d.f:1466) ----------------
+    near_field_lists();
+    Pack pn(W.h_plan_nf);
+    size_t o_units = pn.add(unit_ids), o_uptr = pn.add(unit_ptr), o_ulist = pn.add(unit_list);
+    char* dnf = W.get<char>("plan_nf", pn.size());
+    {
+        void* hp = W.pinned2(pn.size());
+        std::memcpy(hp, pn.data(), pn.size());
+        HIPCHECK(hipMemcpyAsync(dnf, hp, pn.size(), hipMemcpyHostToDevice, s));
+    }
     tm.begin(S_DIRECT);
     if (!unit_ids.empty()) {
         DirectArgs da{};
-        da.units = (const int*)D(o_units);
+        da.units = (const int*)(dnf + o_units);
         da.nunits = (int)unit_ids.size();
-        da.list_ptr = (const int*)D(o_uptr);
-        da.list = (const int*)D(o_ulist);
+        da.list_ptr = (const int*)(dnf + o_uptr);
+        da.list = (const int*)(dnf + o_ulist);
         da.src_se = src_se;
         da.xs = xs;
         da.qs = qs;
