@@ -101,6 +101,7 @@ struct Workspace {
     // host scratch of the plan, reused from call to call
     std::vector<int> h_units, h_uptr, h_ulist, h_nl1, h_l1, h_sp, h_sl, h_fill, h_slot, h_slot2;
     std::vector<char> h_plan, h_plan_nf;
+    size_t leaf_guess = 0;  // leaf records of the last call (size of the first read-back)
     Tables2D tab;
     Tables3D tab3;
     // the tree of the last transform, kept for inspection
@@ -710,17 +711,24 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     int* nsel = W.get<int>("nsel", 4);
     launch_leafdepth(ks2, ns, nlmax, npwlevel, 10, 80, lvl, flg, s);
     select_flagged(tmp, tmpb, flg, ns, starts, nsel, s);
-    int* hn = (int*)W.pinned(64);
-    copy_to_host(hn, nsel, sizeof(int), s);
-    const int nleaf0 = hn[0];
-    LeafRun* runs = W.get<LeafRun>("leaf_runs", nleaf0);
+    // the leaf records (at most one per source) and their count come back
+    // in one round trip, sized by the last call's count
+    LeafRun* runs = W.get<LeafRun>("leaf_runs", ns);
     launch_emit_leaves(starts, nsel, ns, ks2, lvl, nlmax, runs, s);
     tm.end(S_LEAF);
+    const size_t guess = std::min<size_t>(ns, std::max<size_t>(4096, W.leaf_guess + W.leaf_guess / 4));
+    char* hl = (char*)W.pinned(256 + sizeof(LeafRun) * guess);
+    HIPCHECK(hipMemcpyAsync(hl, nsel, sizeof(int), hipMemcpyDeviceToHost, s));
+    copy_to_host(hl + 256, runs, sizeof(LeafRun) * guess, s);
+    const int nleaf0 = *(const int*)hl;
+    W.leaf_guess = nleaf0;
     std::vector<LeafRun> leaves(nleaf0);
-    {
-        void* hp = W.pinned(sizeof(LeafRun) * (size_t)nleaf0);
-        copy_to_host(hp, runs, sizeof(LeafRun) * (size_t)nleaf0, s);
-        std::memcpy(leaves.data(), hp, sizeof(LeafRun) * (size_t)nleaf0);
+    std::memcpy(leaves.data(), hl + 256, sizeof(LeafRun) * std::min<size_t>(nleaf0, guess));
+    if ((size_t)nleaf0 > guess) {
+        const size_t rest = sizeof(LeafRun) * ((size_t)nleaf0 - guess);
+        void* hp = W.pinned2(rest);
+        copy_to_host(hp, runs + guess, rest, s);
+        std::memcpy(leaves.data() + guess, hp, rest);
     }
 
     // Work that needs no tree, queued now so that the GPU runs it while the
@@ -1487,17 +1495,24 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     int* nsel = W.get<int>("nsel", 4);
     launch_leafdepth(ks2, ns, nlmax, npwlevel, 40, 100, lvl, flg, s, 3);
     select_flagged(tmp, tmpb, flg, ns, starts, nsel, s);
-    int* hn = (int*)W.pinned(64);
-    copy_to_host(hn, nsel, sizeof(int), s);
-    const int nleaf0 = hn[0];
-    LeafRun* runs = W.get<LeafRun>("leaf_runs", nleaf0);
+    // the leaf records (at most one per source) and their count come back
+    // in one round trip, sized by the last call's count
+    LeafRun* runs = W.get<LeafRun>("leaf_runs", ns);
     launch_emit_leaves(starts, nsel, ns, ks2, lvl, nlmax, runs, s, 3);
     tm.end(S_LEAF);
+    const size_t guess = std::min<size_t>(ns, std::max<size_t>(4096, W.leaf_guess + W.leaf_guess / 4));
+    char* hl = (char*)W.pinned(256 + sizeof(LeafRun) * guess);
+    HIPCHECK(hipMemcpyAsync(hl, nsel, sizeof(int), hipMemcpyDeviceToHost, s));
+    copy_to_host(hl + 256, runs, sizeof(LeafRun) * guess, s);
+    const int nleaf0 = *(const int*)hl;
+    W.leaf_guess = nleaf0;
     std::vector<LeafRun> leaves(nleaf0);
-    {
-        void* hp = W.pinned(sizeof(LeafRun) * (size_t)nleaf0);
-        copy_to_host(hp, runs, sizeof(LeafRun) * (size_t)nleaf0, s);
-        std::memcpy(leaves.data(), hp, sizeof(LeafRun) * (size_t)nleaf0);
+    std::memcpy(leaves.data(), hl + 256, sizeof(LeafRun) * std::min<size_t>(nleaf0, guess));
+    if ((size_t)nleaf0 > guess) {
+        const size_t rest = sizeof(LeafRun) * ((size_t)nleaf0 - guess);
+        void* hp = W.pinned2(rest);
+        copy_to_host(hp, runs + guess, rest, s);
+        std::memcpy(leaves.data() + guess, hp, rest);
     }
     tm.begin(S_ASSIGN);
     double* xs = W.get<double>("xs", 3 * (size_t)ns);
