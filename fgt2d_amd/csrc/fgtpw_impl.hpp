@@ -76,7 +76,6 @@ struct TreeD {
     std::vector<int> leaf_of_rank, rank_of_box, rank_lo, rank_hi;
     // builder scratch, kept to reuse its storage
     std::vector<int> map, tail;
-    std::vector<char> scratch;
 };
 using Tree2D = TreeD<2>;
 using Tree3D = TreeD<3>;
