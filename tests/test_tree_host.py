@@ -84,3 +84,22 @@ def test_level_restriction_happens():
                                                              t["nlmax"])), t["nlmax"],
                                          t["npwlevel"])[0])
     assert h["nboxes"] > nonempty_leaves
+
+
+def test_host_tree_large_matches_oracle_tree():
+    # 73k leaves: large enough that the builder forks its order-independent
+    # loops onto worker threads; the tree must still be the reference's
+    rng = np.random.default_rng(7)
+    src = _pts.uniform(rng, 2_000_000)
+    tg = _pts.uniform(rng, 1000)
+    h, t = _host_tree(1e-6, 1e-6, src, tg)
+    assert len(h["lev"]) >= 100_000
+    assert h["nlevels"] == t["nlevels"] and h["nboxes"] == t["nboxes"]
+    for k in ("laddr", "lev", "par", "nch", "ch", "ncol"):
+        assert np.array_equal(h[k], t[k]), k
+    assert np.array_equal(h["cen"], t["cen"])
+    lev = t["lev"]
+    sel = np.flatnonzero(lev <= t["npwlevel"])
+    for b in sel[:: max(1, len(sel) // 2000)]:
+        n_ = t["ncol"][b]
+        assert np.array_equal(h["col"][b, :n_], t["col"][b, :n_])
