@@ -1098,46 +1098,41 @@ __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
 }
 
 
-// PW -> Taylor on matrix cores, one wavefront (block of 64) per leaf, for
-// Taylor orders nl < 16 and npw <= 32:
+// PW -> Taylor on matrix cores for Taylor orders nl < 16 and npw <= 32, with
+// D(a, j) = (i ts_a)^j / j!:
 //   for each z node c < npw/2:
 //     S1(j3, b) = sum_a D(a, j3) PW(a, b, c)      MFMA, a padded to 32
 //     S2(j3, j2) = sum_b S1(j3, b) D(b, j2)       MFMA
 //     L(j3, j2, j1) += 2 Re(D(c, j1) S2(j3, j2))  registers
-// with D(a, j) = (i ts_a)^j / j!.  The slab PW(., ., c) is staged in the
-// wave's LDS (zero padded); S1 then reuses that space.  Lane l holds
-// L(j3 = l/16 + 4r, j2 = l%16, j1) for r < 4, j1 < 16.
 constexpr int PW3_N = 32;  // padded npw
 constexpr int PW3_S = 33;  // LDS stride of a slab column (bank-conflict-free lane stride)
 
-__global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
+// Two wavefronts per leaf: wave w stages half of the slab (times the octant
+// shift when the split is fused) into LDS with padded strides, computes the
+// S1 column tile nt = w into a shared LDS buffer; then both waves form S2
+// (the same tile) and each keeps L for its half of j1 (8 orders) in
+// registers, so that two leaves fit per SIMD.  Lane l holds
+// L(j3 = l/16 + 4r, j2 = l%16, j1 = 8w + j).  Writes the transposed layout
+// Lt[j2 + 16 (j3 + m j1)]; columns j2 >= m hold the zeros computed for them.
+__global__ __launch_bounds__(128, 2) void k_pw2local3_wave2(Eval3Args a)
 {
     extern __shared__ double sm[];
-    const int m = a.nl + 1, npw = a.npw, half = npw / 2, lane = threadIdx.x;
+    const int m = a.nl + 1, npw = a.npw, half = npw / 2, t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
     const long nexp = (long)npw * npw * half;
-    double* P = sm;                  // complex (PW3_N a) x (PW3_N b) at a + PW3_S b
-    double* Dt = sm + 2 * PW3_S * PW3_N;  // complex D(c, j1), (half, 16)
+    double* P = sm;                        // complex (PW3_N a) x (PW3_N b) at a + PW3_S b
+    double* S1 = sm + 2 * PW3_S * PW3_N;   // complex (16 j3) x (PW3_N b) at j3 + 17 b
+    double* Dt = S1 + 2 * 17 * PW3_N;      // complex D(a, j) at j + 16 a, a < PW3_N
     const int i16 = lane & 15, k4 = lane >> 4;
-    // this lane's D(4 ks + l/16, l%16): A operand of S1 and B operand of S2
-    double Dr[PW3_N / 4], Di[PW3_N / 4];
-#pragma unroll
-    for (int ks = 0; ks < PW3_N / 4; ++ks) {
-        const int aa = 4 * ks + k4;
-        const bool in = aa < npw && i16 < m;
-        Dr[ks] = in ? a.dcoef[2 * (aa + a.cstride * i16)] : 0.0;
-        Di[ks] = in ? a.dcoef[2 * (aa + a.cstride * i16) + 1] : 0.0;
-    }
-    for (int o = lane; o < half * 16; o += 64) {
-        const int c = o / 16, j = o % 16;
-        const bool in = j < m;
-        Dt[2 * o] = in ? a.dcoef[2 * (c + a.cstride * j)] : 0.0;
-        Dt[2 * o + 1] = in ? a.dcoef[2 * (c + a.cstride * j) + 1] : 0.0;
+    for (int o = t; o < PW3_N * 16; o += 128) {
+        const int aa = o / 16, j = o % 16;
+        const bool in = aa < npw && j < m;
+        Dt[2 * o] = in ? a.dcoef[2 * (aa + a.cstride * j)] : 0.0;
+        Dt[2 * o + 1] = in ? a.dcoef[2 * (aa + a.cstride * j) + 1] : 0.0;
     }
     for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
         const int box = a.boxes[li];
         if (pw_box(a, box) || !has_eval(a, box)) continue;
-        // the leaf's own expansion, or with a fused split its parent's,
-        // multiplied by the octant shift while the slab is staged
         const int src = a.fused ? a.par[box] : box;
         const double* pw = a.loc + 2 * ((long)a.slot[src] * a.nd + a.ind) * nexp;
         const double* wk = nullptr;
@@ -1146,22 +1141,19 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
                                  a.cen[3 * box + 2] - a.cen[3 * src + 2]);
             wk = a.wms + 2 * ((long)(a.kbase + k) * nexp);
         }
-        double L[4][16];
+        double L[4][8];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
-            for (int j = 0; j < 16; ++j) L[r][j] = 0;
+            for (int j = 0; j < 8; ++j) L[r][j] = 0;
         for (int c = 0; c < half; ++c) {
-            wave_sync3();  // previous readers of P / S1 are done
+            __syncthreads();  // readers of P and S1 of the previous node are done
             const double* slab = pw + 2L * npw * npw * c;
-            // the lane's 16 loads in two batches of 8, each issued before
-            // its first store
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            {
                 double2 v[8];
 #pragma unroll
                 for (int it = 0; it < 8; ++it) {
-                    const int o = lane + 64 * (8 * h + it), aa = o % PW3_N, b = o / PW3_N;
+                    const int o = t + 128 * it, aa = o % PW3_N, b = o / PW3_N;
                     v[it] = aa < npw && b < npw ? *(const double2*)(slab + 2 * (aa + npw * b))
                                                 : make_double2(0.0, 0.0);
                 }
@@ -1169,7 +1161,7 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
                     const double* ws = wk + 2L * npw * npw * c;
 #pragma unroll
                     for (int it = 0; it < 8; ++it) {
-                        const int o = lane + 64 * (8 * h + it), aa = o % PW3_N, b = o / PW3_N;
+                        const int o = t + 128 * it, aa = o % PW3_N, b = o / PW3_N;
                         if (aa < npw && b < npw) {
                             const double2 wv = *(const double2*)(ws + 2 * (aa + npw * b));
                             v[it] = make_double2(v[it].x * wv.x - v[it].y * wv.y,
@@ -1179,67 +1171,60 @@ __global__ __launch_bounds__(64) void k_pw2local3_wave(Eval3Args a)
                 }
 #pragma unroll
                 for (int it = 0; it < 8; ++it) {
-                    const int o = lane + 64 * (8 * h + it), aa = o % PW3_N, b = o / PW3_N;
+                    const int o = t + 128 * it, aa = o % PW3_N, b = o / PW3_N;
                     *(double2*)(P + 2 * (aa + PW3_S * b)) = v[it];
                 }
             }
-            wave_sync3();
-            // S1 = D^T PW: two 16-column tiles over b
-            d4_t3 s1r[2], s1i[2];
-#pragma unroll
-            for (int nt = 0; nt < 2; ++nt) {
-                s1r[nt] = (d4_t3){0, 0, 0, 0};
-                s1i[nt] = (d4_t3){0, 0, 0, 0};
+            __syncthreads();
+            {
+                // S1 column tile nt = w
+                d4_t3 s1r = {0, 0, 0, 0}, s1i = {0, 0, 0, 0};
 #pragma unroll
                 for (int ks = 0; ks < PW3_N / 4; ++ks) {
-                    const int o = (4 * ks + k4) + PW3_S * (16 * nt + i16);
+                    const int o = (4 * ks + k4) + PW3_S * (16 * w + i16);
+                    const int od = i16 + 16 * (4 * ks + k4);
                     const double pr = P[2 * o], pi = P[2 * o + 1];
-                    s1r[nt] = mfma3(Dr[ks], pr, s1r[nt]);
-                    s1r[nt] = mfma3(-Di[ks], pi, s1r[nt]);
-                    s1i[nt] = mfma3(Dr[ks], pi, s1i[nt]);
-                    s1i[nt] = mfma3(Di[ks], pr, s1i[nt]);
+                    const double dr = Dt[2 * od], di = Dt[2 * od + 1];
+                    s1r = mfma3(dr, pr, s1r);
+                    s1r = mfma3(-di, pi, s1r);
+                    s1i = mfma3(dr, pi, s1i);
+                    s1i = mfma3(di, pr, s1i);
                 }
-            }
-            wave_sync3();  // all reads of P done: S1 overwrites it
-            double* S1 = P;  // complex (16 j3) x (PW3_N b) at j3 + 17 b
-#pragma unroll
-            for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int o = (k4 + 4 * r) + 17 * (16 * nt + i16);
-                    S1[2 * o] = s1r[nt][r];
-                    S1[2 * o + 1] = s1i[nt][r];
+                    const int o = (k4 + 4 * r) + 17 * (16 * w + i16);
+                    S1[2 * o] = s1r[r];
+                    S1[2 * o + 1] = s1i[r];
                 }
-            wave_sync3();
-            // S2 = S1 D
+            }
+            __syncthreads();
             d4_t3 s2r = {0, 0, 0, 0}, s2i = {0, 0, 0, 0};
 #pragma unroll
             for (int ks = 0; ks < PW3_N / 4; ++ks) {
                 const int o = i16 + 17 * (4 * ks + k4);
+                const int od = i16 + 16 * (4 * ks + k4);
                 const double xr = S1[2 * o], xi = S1[2 * o + 1];
-                s2r = mfma3(xr, Dr[ks], s2r);
-                s2r = mfma3(-xi, Di[ks], s2r);
-                s2i = mfma3(xr, Di[ks], s2i);
-                s2i = mfma3(xi, Dr[ks], s2i);
+                const double dr = Dt[2 * od], di = Dt[2 * od + 1];
+                s2r = mfma3(xr, dr, s2r);
+                s2r = mfma3(-xi, di, s2r);
+                s2i = mfma3(xr, di, s2i);
+                s2i = mfma3(xi, dr, s2i);
             }
-            // L += 2 Re(D(c, j1) S2)
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const double dr = Dt[2 * (16 * c + j)], di = Dt[2 * (16 * c + j) + 1];
+            for (int j = 0; j < 8; ++j) {
+                const double dr = Dt[2 * (16 * c + 8 * w + j)], di = Dt[2 * (16 * c + 8 * w + j) + 1];  // D(c, j1)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) L[r][j] += 2.0 * (dr * s2r[r] - di * s2i[r]);
             }
         }
-        // transposed layout Lt[j2 + 16 (j3 + m j1)]; columns j2 >= m hold
-        // the zeros computed for them
         double* out = a.taylor + (long)li * taylor3_stride(m, 1);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int j3 = k4 + 4 * r;
             if (j3 < m)
 #pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    if (j < m) out[i16 + 16 * (j3 + (long)m * j)] = L[r][j];
+                for (int j = 0; j < 8; ++j)
+                    if (8 * w + j < m) out[i16 + 16 * (j3 + (long)m * (8 * w + j))] = L[r][j];
         }
     }
 }
@@ -1404,9 +1389,10 @@ void launch_pw2local3(const Eval3Args& a, hipStream_t s)
     if (a.fused && !(pw2local3_fusable(m, a.npw) && a.tlayout))
         fail(-3, "fused split requested for a PW -> Taylor kernel without it");
     if (m <= 16 && a.npw <= PW3_N && a.tlayout) {
-        size_t lds = ((size_t)2 * PW3_S * PW3_N + (size_t)2 * (a.npw / 2) * 16) * sizeof(double);
+        size_t lds = ((size_t)2 * PW3_S * PW3_N + (size_t)2 * 17 * PW3_N + (size_t)2 * PW3_N * 16) *
+                     sizeof(double);
         kmark(KID_PW2LOCAL, false, s);
-        k_pw2local3_wave<<<std::min(a.nbox, 65536), 64, lds, s>>>(a);
+        k_pw2local3_wave2<<<std::min(a.nbox, 65536), 128, lds, s>>>(a);
         kmark(KID_PW2LOCAL, true, s);
         return;
     }
