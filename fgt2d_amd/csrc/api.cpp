@@ -701,20 +701,33 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     tmpb = std::max(tmpb, select_flagged_temp(ns));
     void* tmp = W.get<char>("sort_tmp", tmpb);
     const int keybits = std::max(2, 2 * nlmax);
-    launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, ks, ix, s);
-    sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
+    const bool k32 = keybits <= 31;  // keys fit 32 bits
+    if (k32)
+        launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, (unsigned*)ks, ix, s);
+    else
+        launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, ks, ix, s);
+    if (k32)
+        sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix, isrc, ns, keybits, s);
+    else
+        sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
     tm.end(S_KEYS);
     tm.begin(S_LEAF);
     int* lvl = W.get<int>("leaf_level", ns);
     unsigned char* flg = W.get<unsigned char>("leaf_flag", ns);
     int* starts = W.get<int>("leaf_starts", ns);
     int* nsel = W.get<int>("nsel", 4);
-    launch_leafdepth(ks2, ns, nlmax, npwlevel, 10, 80, lvl, flg, s);
+    if (k32)
+        launch_leafdepth((const unsigned*)ks2, ns, nlmax, npwlevel, 10, 80, lvl, flg, s);
+    else
+        launch_leafdepth(ks2, ns, nlmax, npwlevel, 10, 80, lvl, flg, s);
     select_flagged(tmp, tmpb, flg, ns, starts, nsel, s);
     // the leaf records (at most one per source) and their count come back
     // in one round trip, sized by the last call's count
     LeafRun* runs = W.get<LeafRun>("leaf_runs", ns);
-    launch_emit_leaves(starts, nsel, ns, ks2, lvl, nlmax, runs, s);
+    if (k32)
+        launch_emit_leaves(starts, nsel, ns, (const unsigned*)ks2, lvl, nlmax, runs, s);
+    else
+        launch_emit_leaves(starts, nsel, ns, ks2, lvl, nlmax, runs, s);
     tm.end(S_LEAF);
     const size_t guess = std::min<size_t>(ns, std::max<size_t>(4096, W.leaf_guess + W.leaf_guess / 4));
     char* hl = (char*)W.pinned(256 + sizeof(LeafRun) * guess);
@@ -757,8 +770,14 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         xt = W.get<double>("xt", 2 * (size_t)nt);
         u64* kt = W.get<u64>("keys_t", nt);
         kt2 = W.get<u64>("keys_t2", nt);
-        launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, kt, ix, s);
-        sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
+        if (k32)
+            launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, (unsigned*)kt, ix, s);
+        else
+            launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, kt, ix, s);
+        if (k32)
+            sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s);
+        else
+            sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
         tm.ctx(-1, nullptr, -1, nt);
         launch_gather(q.targ, nullptr, 0, itarg, nt, xt, nullptr, s);
     }
@@ -954,12 +973,18 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     int* ls = W.get<int>("leafstart", nleaves + 1);
     int2* src_se = W.get<int2>("src_se", nb);
     int2* trg_se = W.get<int2>("trg_se", nb);
-    launch_descend_rank(ks2, nullptr, ns, nlmax, d_child, d_rank, rk, s);
+    if (k32)
+        launch_descend_rank((const unsigned*)ks2, nullptr, ns, nlmax, d_child, d_rank, rk, s);
+    else
+        launch_descend_rank(ks2, nullptr, ns, nlmax, d_child, d_rank, rk, s);
     launch_leafstart(rk, ns, nleaves, ls, s);
     launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, s);
     tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
     if (nt > 0) {
-        launch_descend_rank(kt2, nullptr, nt, nlmax, d_child, d_rank, rk, s);
+        if (k32)
+            launch_descend_rank((const unsigned*)kt2, nullptr, nt, nlmax, d_child, d_rank, rk, s);
+        else
+            launch_descend_rank(kt2, nullptr, nt, nlmax, d_child, d_rank, rk, s);
         launch_leafstart(rk, nt, nleaves, ls, s);
         launch_box_ranges(d_rlo, d_rhi, nb, ls, trg_se, s);
     } else {
@@ -1482,23 +1507,36 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     u64* ks2 = W.get<u64>("keys_s2", ns);
     unsigned* ix = W.get<unsigned>("idx_s", nmx);
     unsigned* isrc = W.get<unsigned>("isrc", ns);
-    size_t tmpb = std::max(sort_pairs_u64_temp(nmx), select_flagged_temp(ns));
+    size_t tmpb = std::max({sort_pairs_u64_temp(nmx), sort_pairs_u32_temp(nmx), select_flagged_temp(ns)});
     void* tmp = W.get<char>("sort_tmp", tmpb);
     const int keybits = std::max(3, 3 * nlmax);
-    launch_keys3d(q.src, ns, cen0, bs0, nlmax, ks, ix, s);
-    sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
+    const bool k32 = keybits <= 31;  // keys fit 32 bits
+    if (k32)
+        launch_keys3d(q.src, ns, cen0, bs0, nlmax, (unsigned*)ks, ix, s);
+    else
+        launch_keys3d(q.src, ns, cen0, bs0, nlmax, ks, ix, s);
+    if (k32)
+        sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix, isrc, ns, keybits, s);
+    else
+        sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
     tm.end(S_KEYS);
     tm.begin(S_LEAF);
     int* lvl = W.get<int>("leaf_level", ns);
     unsigned char* flg = W.get<unsigned char>("leaf_flag", ns);
     int* starts = W.get<int>("leaf_starts", ns);
     int* nsel = W.get<int>("nsel", 4);
-    launch_leafdepth(ks2, ns, nlmax, npwlevel, 40, 100, lvl, flg, s, 3);
+    if (k32)
+        launch_leafdepth((const unsigned*)ks2, ns, nlmax, npwlevel, 40, 100, lvl, flg, s, 3);
+    else
+        launch_leafdepth(ks2, ns, nlmax, npwlevel, 40, 100, lvl, flg, s, 3);
     select_flagged(tmp, tmpb, flg, ns, starts, nsel, s);
     // the leaf records (at most one per source) and their count come back
     // in one round trip, sized by the last call's count
     LeafRun* runs = W.get<LeafRun>("leaf_runs", ns);
-    launch_emit_leaves(starts, nsel, ns, ks2, lvl, nlmax, runs, s, 3);
+    if (k32)
+        launch_emit_leaves(starts, nsel, ns, (const unsigned*)ks2, lvl, nlmax, runs, s, 3);
+    else
+        launch_emit_leaves(starts, nsel, ns, ks2, lvl, nlmax, runs, s, 3);
     tm.end(S_LEAF);
     const size_t guess = std::min<size_t>(ns, std::max<size_t>(4096, W.leaf_guess + W.leaf_guess / 4));
     char* hl = (char*)W.pinned(256 + sizeof(LeafRun) * guess);
@@ -1537,8 +1575,14 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         xt = W.get<double>("xt", 3 * (size_t)nt);
         u64* kt = W.get<u64>("keys_t", nt);
         kt2 = W.get<u64>("keys_t2", nt);
-        launch_keys3d(q.targ, nt, cen0, bs0, nlmax, kt, ix, s);
-        sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
+        if (k32)
+            launch_keys3d(q.targ, nt, cen0, bs0, nlmax, (unsigned*)kt, ix, s);
+        else
+            launch_keys3d(q.targ, nt, cen0, bs0, nlmax, kt, ix, s);
+        if (k32)
+            sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s);
+        else
+            sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
         tm.ctx(-1, nullptr, -1, nt);
         launch_gather3(q.targ, nullptr, 0, itarg, nt, xt, nullptr, s);
     }
@@ -1878,12 +1922,18 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     int* ls = W.get<int>("leafstart", nleaves + 1);
     int2* src_se = W.get<int2>("src_se", nb);
     int2* trg_se = W.get<int2>("trg_se", nb);
-    launch_descend_rank3(ks2, ns, nlmax, d_child, d_rank, rk, s);
+    if (k32)
+        launch_descend_rank3((const unsigned*)ks2, ns, nlmax, d_child, d_rank, rk, s);
+    else
+        launch_descend_rank3(ks2, ns, nlmax, d_child, d_rank, rk, s);
     launch_leafstart(rk, ns, nleaves, ls, s);
     launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, s);
     tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
     if (nt > 0) {
-        launch_descend_rank3(kt2, nt, nlmax, d_child, d_rank, rk, s);
+        if (k32)
+            launch_descend_rank3((const unsigned*)kt2, nt, nlmax, d_child, d_rank, rk, s);
+        else
+            launch_descend_rank3(kt2, nt, nlmax, d_child, d_rank, rk, s);
         launch_leafstart(rk, nt, nleaves, ls, s);
         launch_box_ranges(d_rlo, d_rhi, nb, ls, trg_se, s);
     } else {

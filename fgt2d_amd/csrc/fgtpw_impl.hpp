@@ -99,12 +99,18 @@ struct DevArena;  // fwd
 
 void launch_bbox(const double* a, int na, const double* b, int nb, double* part, int nblk,
                  double* out, hipStream_t s);
+// Morton keys of 64 bits, or of 32 bits when the key width (bits per
+// level x nlmax) is at most 31
 void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, int nlmax,
                    unsigned long long* key, unsigned* idx, hipStream_t s);
+void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, int nlmax,
+                   unsigned* key, unsigned* idx, hipStream_t s);
 // bits: key bits per level (2 in 2D, 3 in 3D)
 void launch_leafdepth(const unsigned long long* key, int n, int nlmax, int levcut, int ndiv0,
                       int ndiv, int* level, unsigned char* start_flag, hipStream_t s,
                       int bits = 2);
+void launch_leafdepth(const unsigned* key, int n, int nlmax, int levcut, int ndiv0, int ndiv,
+                      int* level, unsigned char* start_flag, hipStream_t s, int bits = 2);
 size_t sort_pairs_u64_temp(int n);
 void sort_pairs_u64(void* tmp, size_t tmpb, const unsigned long long* kin, unsigned long long* kout,
                     const unsigned* vin, unsigned* vout, int n, int endbit, hipStream_t s);
@@ -116,7 +122,12 @@ void select_flagged(void* tmp, size_t tmpb, const unsigned char* flags, int n, i
                     hipStream_t s);
 void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigned long long* key,
                         const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits = 2);
+void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigned* key,
+                        const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits = 2);
 void launch_descend_rank(const unsigned long long* key, const unsigned* idx, int n, int nlmax,
+                         const int4* child, const int* rank_of_box, unsigned* rank_out,
+                         hipStream_t s);
+void launch_descend_rank(const unsigned* key, const unsigned* idx, int n, int nlmax,
                          const int4* child, const int* rank_of_box, unsigned* rank_out,
                          hipStream_t s);
 void launch_iota(unsigned* v, int n, hipStream_t s);
@@ -250,7 +261,11 @@ void launch_bbox3(const double* a, int na, const double* b, int nb, double* part
                   double* out, hipStream_t s);  // out: xmin,xmax,ymin,ymax,zmin,zmax,bad
 void launch_keys3d(const double* p, int n, const double* c0, double bs0, int nlmax,
                    unsigned long long* key, unsigned* idx, hipStream_t s);
+void launch_keys3d(const double* p, int n, const double* c0, double bs0, int nlmax,
+                   unsigned* key, unsigned* idx, hipStream_t s);
 void launch_descend_rank3(const unsigned long long* key, int n, int nlmax, const int* child8,
+                          const int* rank_of_box, unsigned* rank_out, hipStream_t s);
+void launch_descend_rank3(const unsigned* key, int n, int nlmax, const int* child8,
                           const int* rank_of_box, unsigned* rank_out, hipStream_t s);
 void launch_gather3(const double* p, const double* q, int nd, const unsigned* perm, int n,
                     double* ps, double* qs, hipStream_t s);

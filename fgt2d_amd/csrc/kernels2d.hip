@@ -108,22 +108,23 @@ void launch_bbox(const double* a, int na, const double* b, int nb, double* part,
 // bit-identical centres, so the key prefix of any level names the point's
 // box at that level.  Digit = xbit + 2*ybit, most significant = level 1.
 // ---------------------------------------------------------------------------
+template <class K>  // 32- or 64-bit keys
 __global__ __launch_bounds__(256) void k_keys2d(const double* __restrict__ xy, int n, double cx0,
                                                 double cy0, double bs0, int nlmax,
-                                                u64* __restrict__ key, unsigned* __restrict__ idx)
+                                                K* __restrict__ key, unsigned* __restrict__ idx)
 {
 #pragma clang fp contract(off)
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     double x = xy[2 * (long)i], y = xy[2 * (long)i + 1];
     double cx = cx0, cy = cy0, bs = bs0;
-    u64 k = 0;
+    K k = 0;
     for (int l = 0; l < nlmax; ++l) {
         bs = bs / 2;
         double h = bs / 2;
         int xb = !(x - cx < 0.0);
         int yb = !(y - cy < 0.0);
-        k = (k << 2) | (u64)(xb | (yb << 1));
+        k = (k << 2) | (K)(xb | (yb << 1));
         cx = cx + (xb ? h : -h);
         cy = cy + (yb ? h : -h);
     }
@@ -131,11 +132,22 @@ __global__ __launch_bounds__(256) void k_keys2d(const double* __restrict__ xy, i
     if (idx) idx[i] = (unsigned)i;
 }
 
+template <class K>
+static void keys2d_launch(const double* xy, int n, double cx, double cy, double bs0, int nlmax,
+                          K* key, unsigned* idx, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_keys2d<K><<<grid_for(n, 256), 256, 0, s>>>(xy, n, cx, cy, bs0, nlmax, key, idx);
+}
 void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, int nlmax, u64* key,
                    unsigned* idx, hipStream_t s)
 {
-    if (n <= 0) return;
-    k_keys2d<<<grid_for(n, 256), 256, 0, s>>>(xy, n, cx, cy, bs0, nlmax, key, idx);
+    keys2d_launch(xy, n, cx, cy, bs0, nlmax, key, idx, s);
+}
+void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, int nlmax,
+                   unsigned* key, unsigned* idx, hipStream_t s)
+{
+    keys2d_launch(xy, n, cx, cy, bs0, nlmax, key, idx, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -150,18 +162,19 @@ void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, in
 // contiguous interval); a run that fills the window on either side holds more
 // than lim points.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_leafdepth(const u64* __restrict__ key, int n, int nlmax,
+template <class K>
+__global__ __launch_bounds__(256) void k_leafdepth(const K* __restrict__ key, int n, int nlmax,
                                                    int levcut, int ndiv0, int ndiv, int bits,
                                                    int* __restrict__ level,
                                                    unsigned char* __restrict__ start)
 {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const u64 ki = key[i];
+    const K ki = key[i];
     int L = 0;
     for (int l = nlmax - 1; l >= 0; --l) {
-        const int sh = bits * (nlmax - l);
-        const u64 p = ki >> sh;
+        const int sh = bits * (nlmax - l);  // < the key width (host guarantee)
+        const K p = ki >> sh;
         const int lim = l < levcut ? ndiv0 : ndiv;
         // first index in [max(0, i - lim), i] with prefix p
         int a = max(0, i - lim), b = i;
@@ -185,20 +198,32 @@ __global__ __launch_bounds__(256) void k_leafdepth(const u64* __restrict__ key, 
     level[i] = L;
     const int sh = bits * (nlmax - L);
     unsigned char st = 1;
-    if (i > 0) st = sh >= 64 ? 0 : ((key[i - 1] >> sh) != (ki >> sh));
+    if (i > 0) st = sh >= 8 * (int)sizeof(K) ? 0 : ((key[i - 1] >> sh) != (ki >> sh));
     start[i] = st;
 }
 
+template <class K>
+static void leafdepth_launch(const K* key, int n, int nlmax, int levcut, int ndiv0, int ndiv,
+                             int* level, unsigned char* start_flag, hipStream_t s, int bits)
+{
+    k_leafdepth<K><<<grid_for(n, 256), 256, 0, s>>>(key, n, nlmax, levcut, ndiv0, ndiv, bits,
+                                                    level, start_flag);
+}
 void launch_leafdepth(const u64* key, int n, int nlmax, int levcut, int ndiv0, int ndiv,
                       int* level, unsigned char* start_flag, hipStream_t s, int bits)
 {
-    k_leafdepth<<<grid_for(n, 256), 256, 0, s>>>(key, n, nlmax, levcut, ndiv0, ndiv, bits, level,
-                                                 start_flag);
+    leafdepth_launch(key, n, nlmax, levcut, ndiv0, ndiv, level, start_flag, s, bits);
+}
+void launch_leafdepth(const unsigned* key, int n, int nlmax, int levcut, int ndiv0, int ndiv,
+                      int* level, unsigned char* start_flag, hipStream_t s, int bits)
+{
+    leafdepth_launch(key, n, nlmax, levcut, ndiv0, ndiv, level, start_flag, s, bits);
 }
 
+template <class K>
 __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ starts,
                                                      const int* __restrict__ nsel, int n,
-                                                     const u64* __restrict__ key,
+                                                     const K* __restrict__ key,
                                                      const int* __restrict__ level, int nlmax,
                                                      int bits, LeafRun* __restrict__ out)
 {
@@ -209,7 +234,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ sta
     int e = r + 1 < m ? starts[r + 1] : n;
     int L = level[s];
     LeafRun o;
-    o.key = L == 0 ? 0ull : (key[s] >> (bits * (nlmax - L)));
+    o.key = L == 0 ? 0ull : (u64)(key[s] >> (bits * (nlmax - L)));
     o.level = L;
     o.start = s;
     o.count = e - s;
@@ -219,7 +244,14 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ sta
 void launch_emit_leaves(const int* starts, const int* nsel, int n, const u64* key,
                         const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits)
 {
-    k_emit_leaves<<<grid_for(n, 256), 256, 0, s>>>(starts, nsel, n, key, level, nlmax, bits, out);
+    k_emit_leaves<u64><<<grid_for(n, 256), 256, 0, s>>>(starts, nsel, n, key, level, nlmax, bits,
+                                                        out);
+}
+void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigned* key,
+                        const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits)
+{
+    k_emit_leaves<unsigned><<<grid_for(n, 256), 256, 0, s>>>(starts, nsel, n, key, level, nlmax,
+                                                             bits, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,7 +304,8 @@ void select_flagged(void* tmp, size_t tmpb, const unsigned char* flags, int n, i
 // ---------------------------------------------------------------------------
 // point -> leaf of the final tree, by walking the key digits down the boxes
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_descend_rank(const u64* __restrict__ key,
+template <class K>
+__global__ __launch_bounds__(256) void k_descend_rank(const K* __restrict__ key,
                                                       const unsigned* __restrict__ idx, int n,
                                                       int nlmax, const int4* __restrict__ child,
                                                       const int* __restrict__ rank_of_box,
@@ -280,7 +313,7 @@ __global__ __launch_bounds__(256) void k_descend_rank(const u64* __restrict__ ke
 {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    u64 k = key[i];
+    K k = key[i];
     int b = 0;
     for (int l = 0; l < nlmax; ++l) {
         int4 c = child[b];
@@ -299,8 +332,16 @@ void launch_descend_rank(const u64* key, const unsigned* idx, int n, int nlmax, 
                          const int* rank_of_box, unsigned* rank_out, hipStream_t s)
 {
     if (n <= 0) return;
-    k_descend_rank<<<grid_for(n, 256), 256, 0, s>>>(key, idx, n, nlmax, child, rank_of_box,
-                                                    rank_out);
+    k_descend_rank<u64><<<grid_for(n, 256), 256, 0, s>>>(key, idx, n, nlmax, child, rank_of_box,
+                                                         rank_out);
+}
+void launch_descend_rank(const unsigned* key, const unsigned* idx, int n, int nlmax,
+                         const int4* child, const int* rank_of_box, unsigned* rank_out,
+                         hipStream_t s)
+{
+    if (n <= 0) return;
+    k_descend_rank<unsigned><<<grid_for(n, 256), 256, 0, s>>>(key, idx, n, nlmax, child,
+                                                              rank_of_box, rank_out);
 }
 
 __global__ __launch_bounds__(256) void k_iota(unsigned* __restrict__ v, int n)

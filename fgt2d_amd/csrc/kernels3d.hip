@@ -115,21 +115,22 @@ void launch_bbox3(const double* a, int na, const double* b, int nb, double* part
 // "p - c < 0" tests (sort_pts_to_children, pts_tree3d.f:662-854;
 // tree_refine_boxes, tree_routs3d.f:28-46).  Digit = x + 2y + 4z.
 // ---------------------------------------------------------------------------
+template <class K>  // 32- or 64-bit keys
 __global__ __launch_bounds__(256) void k_keys3d(const double* __restrict__ p, int n, double cx0,
                                                 double cy0, double cz0, double bs0, int nlmax,
-                                                u64* __restrict__ key, unsigned* __restrict__ idx)
+                                                K* __restrict__ key, unsigned* __restrict__ idx)
 {
 #pragma clang fp contract(off)
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     double x = p[3 * (long)i], y = p[3 * (long)i + 1], z = p[3 * (long)i + 2];
     double cx = cx0, cy = cy0, cz = cz0, bs = bs0;
-    u64 k = 0;
+    K k = 0;
     for (int l = 0; l < nlmax; ++l) {
         bs = bs / 2;
         double h = bs / 2;
         int xb = !(x - cx < 0.0), yb = !(y - cy < 0.0), zb = !(z - cz < 0.0);
-        k = (k << 3) | (u64)(xb | (yb << 1) | (zb << 2));
+        k = (k << 3) | (K)(xb | (yb << 1) | (zb << 2));
         cx = cx + (xb ? h : -h);
         cy = cy + (yb ? h : -h);
         cz = cz + (zb ? h : -h);
@@ -142,18 +143,26 @@ void launch_keys3d(const double* p, int n, const double* c0, double bs0, int nlm
                    unsigned* idx, hipStream_t s)
 {
     if (n <= 0) return;
-    k_keys3d<<<gridn(n, 256), 256, 0, s>>>(p, n, c0[0], c0[1], c0[2], bs0, nlmax, key, idx);
+    k_keys3d<u64><<<gridn(n, 256), 256, 0, s>>>(p, n, c0[0], c0[1], c0[2], bs0, nlmax, key, idx);
+}
+void launch_keys3d(const double* p, int n, const double* c0, double bs0, int nlmax,
+                   unsigned* key, unsigned* idx, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_keys3d<unsigned><<<gridn(n, 256), 256, 0, s>>>(p, n, c0[0], c0[1], c0[2], bs0, nlmax, key,
+                                                     idx);
 }
 
 // leaf rank of each point by walking its key digits down the final octree
-__global__ __launch_bounds__(256) void k_descend_rank3(const u64* __restrict__ key, int n,
+template <class K>
+__global__ __launch_bounds__(256) void k_descend_rank3(const K* __restrict__ key, int n,
                                                        int nlmax, const int* __restrict__ child,
                                                        const int* __restrict__ rank_of_box,
                                                        unsigned* __restrict__ rank_out)
 {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    u64 k = key[i];
+    K k = key[i];
     int b = 0;
     for (int l = 0; l < nlmax; ++l) {
         int c0 = child[8 * b];
@@ -168,7 +177,13 @@ void launch_descend_rank3(const u64* key, int n, int nlmax, const int* child8,
                           const int* rank_of_box, unsigned* rank_out, hipStream_t s)
 {
     if (n <= 0) return;
-    k_descend_rank3<<<gridn(n, 256), 256, 0, s>>>(key, n, nlmax, child8, rank_of_box, rank_out);
+    k_descend_rank3<u64><<<gridn(n, 256), 256, 0, s>>>(key, n, nlmax, child8, rank_of_box, rank_out);
+}
+void launch_descend_rank3(const unsigned* key, int n, int nlmax, const int* child8,
+                          const int* rank_of_box, unsigned* rank_out, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_descend_rank3<unsigned><<<gridn(n, 256), 256, 0, s>>>(key, n, nlmax, child8, rank_of_box, rank_out);
 }
 
 __global__ __launch_bounds__(256) void k_gather3(const double* __restrict__ p,
