@@ -112,11 +112,26 @@ def _is_torch_gpu(a):
 
 
 def _outputs(mod, n, nd, ifp, like, dim=2):
+    # not initialised: the transform overwrites every element of a requested
+    # output (no fill on the caller's stream ahead of it)
     shapes = [(n, nd), (n, dim, nd), (n, 3 if dim == 2 else 6, nd)]
     if mod is None:
-        return [(_np.zeros(s) if k < ifp else None) for k, s in enumerate(shapes)]
-    return [(mod.zeros(s, dtype=mod.float64, device=like.device) if k < ifp else None)
+        return [(_np.empty(s) if k < ifp else None) for k, s in enumerate(shapes)]
+    return [(mod.empty(s, dtype=mod.float64, device=like.device) if k < ifp else None)
             for k, s in enumerate(shapes)]
+
+
+_DUMMY = {}
+
+
+def _dummy(mod, like):
+    """A small array for the pointer arguments of quantities not requested
+    (never read or written), one per device, kept."""
+    key = "host" if mod is None else str(like.device)
+    if key not in _DUMMY:
+        _DUMMY[key] = (_np.zeros(8) if mod is None
+                       else mod.zeros(8, dtype=mod.float64, device=like.device))
+    return _DUMMY[key]
 
 
 def fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, ifcharge=1,
@@ -181,7 +196,7 @@ def _transform(dim, delta, eps, sources, charge, targ, ifpgh, ifpghtarg, ifcharg
         raise ValueError("targ must have shape (nt, %d)" % dim)
     outs = (_outputs(mod, ns, nd, ifpgh, src, dim) +
             _outputs(mod, max(nt, 1), nd, ifpghtarg, src, dim))
-    dummy = _np.zeros(8) if mod is None else mod.zeros(8, dtype=mod.float64, device=src.device)
+    dummy = _dummy(mod, src)
     P = [_ptr(o if o is not None else dummy) for o in outs]
     args = [nd, float(delta), float(eps), ns, _ptr(src), ifcharge,
             _ptr(chg if chg is not None else dummy), ifdipole,
