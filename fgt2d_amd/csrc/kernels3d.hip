@@ -37,6 +37,13 @@ __device__ __forceinline__ d4_t3 mfma3(double a, double b, d4_t3 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// i^n (re + i im), exact (swaps and sign flips)
+__device__ __forceinline__ double2 rot_ij(double re, double im, int n)
+{
+    const double x = (n & 1) ? -im : re, y = (n & 1) ? re : im;
+    return (n & 2) ? make_double2(-x, -y) : make_double2(x, y);
+}
+
 __device__ __forceinline__ void wave_sync3()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -430,8 +437,12 @@ __global__ __launch_bounds__(256) void k_form3_hermite(Form3Args a)
 //   3. per z node c: P_c(a, k) = sum_j C(j, a) N(j, k, c), into LDS;
 //   4. PW(a, b, c) = sum_k C(k, b) P_c(a, k), computed transposed (rows b,
 //      columns a) so that the store runs along a.
-// C(j, a) = ws_a (-i ts_a)^j / j! sits in LDS at a + CW j; KP = 16 MT is m
-// rounded up to 16; all padding is zero.
+// C(j, a) = ws_a (-i ts_a)^j / j! = i^j r(a, j) with r real, so every
+// contraction runs on the real r (two real products per complex operand, not
+// four): phase 2 takes the even orders l into Re N and the odd ones into Im N,
+// phase 2 stores N' = i^j N and phase 3 stores P' = i^k P_c, which phases 3
+// and 4 then contract with r.  r sits in LDS at a + CW j for j < JR (JR a
+// multiple of 8 >= m); KP = 16 MT is m rounded up to 16; all padding is zero.
 template <int NB, bool DIP>
 __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
 {
@@ -440,18 +451,24 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
     const int m = a.p + 1, npw = a.npw, half = npw / 2, t = threadIdx.x;
     const int w = t >> 6, lane = t & 63, i16 = lane & 15, k4 = lane >> 4;
     const int MT = (m + 15) / 16, KP = 16 * MT, KS = (m + 3) / 4, PK = KP + 1;
+    const int KSE = ((m + 1) / 2 + 3) / 4, JR = 8 * KSE;  // steps over the even (odd) orders
     const int KLP = KP * m;           // columns (k, l) of M, rows (j, k) of N
     const int NC = (half + 15) / 16;
     const long nexp = (long)npw * npw * half;
     double* Mg = a.scratch + (long)blockIdx.x * a.sstride;  // KP rows j x KLP
     double* Ng = Mg + (long)KP * KLP;                        // complex KLP x half
-    double* Ct = sm;                                         // complex C(j, a), j < 4 KS
-    double* work = Ct + 2 * CW * 4 * KS;                     // sources, then P_c
-    for (int o = t; o < CW * 4 * KS; o += 512) {
+    double* Rt = sm;                                         // real r(a, j), j < JR
+    double* work = Rt + CW * JR;                             // sources, then P_c
+    for (int o = t; o < CW * JR; o += 512) {
         const int aa = o % CW, j = o / CW;
-        const bool in = aa < npw && j < m;
-        Ct[2 * o] = in ? a.coef[2 * (aa + a.cstride * j)] : 0.0;
-        Ct[2 * o + 1] = in ? a.coef[2 * (aa + a.cstride * j) + 1] : 0.0;
+        double v = 0.0;
+        if (aa < npw && j < m) {
+            // C = i^j r: r is the one non-zero part of C, signed
+            const double* c = a.coef + 2 * (aa + a.cstride * j);
+            v = (j & 1) ? c[1] : c[0];
+            if ((j & 3) >= 2) v = -v;
+        }
+        Rt[o] = v;
     }
     for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
         const int box = a.boxes[li];
@@ -558,28 +575,33 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
             }
         }
         __syncthreads();
-        // ---- 2. N^T(c, (j, k)) = sum_l C(l, c) M(j, k, l) ----
+        // ---- 2. N'^T(c, (j, k)) = i^j sum_l C(l, c) M(j, k, l):
+        // Re N = sum over even l = 2e of (-1)^e r(c, l) M, Im N over odd
+        // l = 2e + 1 of (-1)^e r(c, l) M ----
         {
             const int ntile = NC * (KLP / 16);
             for (int tile = w; tile < ntile; tile += 8) {
                 const int nc = tile % NC, nr = tile / NC;
                 const int rho = 16 * nr + i16, j = rho / KP, k = rho % KP;
                 d4_t3 nr_ = {0, 0, 0, 0}, ni_ = {0, 0, 0, 0};
-                for (int ks = 0; ks < KS; ++ks) {
-                    const int l = 4 * ks + k4;
-                    const int oc = (16 * nc + i16) + CW * l;
-                    const double cr = Ct[2 * oc], ci = Ct[2 * oc + 1];
-                    const double mv = l < m ? Mg[(k + KP * l) + (long)KLP * j] : 0.0;
-                    nr_ = mfma3(cr, mv, nr_);
-                    ni_ = mfma3(ci, mv, ni_);
+                for (int ks = 0; ks < KSE; ++ks) {
+                    const int e = 4 * ks + k4, le = 2 * e, lo = 2 * e + 1;
+                    const double sg = (e & 1) ? -1.0 : 1.0;
+                    const double ce = sg * Rt[(16 * nc + i16) + CW * le];
+                    const double co = sg * Rt[(16 * nc + i16) + CW * lo];
+                    const double me = le < m ? Mg[(k + KP * le) + (long)KLP * j] : 0.0;
+                    const double mo = lo < m ? Mg[(k + KP * lo) + (long)KLP * j] : 0.0;
+                    nr_ = mfma3(ce, me, nr_);
+                    ni_ = mfma3(co, mo, ni_);
                 }
 #pragma unroll
                 for (int rr = 0; rr < 4; ++rr) {
                     const int c = 16 * nc + k4 + 4 * rr;
                     if (c < half) {
                         double* d = Ng + 2 * ((16 * nr + i16) + (long)KLP * c);
-                        d[0] = nr_[rr];
-                        d[1] = ni_[rr];
+                        const double2 v = rot_ij(nr_[rr], ni_[rr], j);
+                        d[0] = v.x;
+                        d[1] = v.y;
                     }
                 }
             }
@@ -599,8 +621,7 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                 d4_t3 pr = {0, 0, 0, 0}, pi = {0, 0, 0, 0};
                 for (int ks = 0; ks < KS; ++ks) {
                     const int j = 4 * ks + k4;
-                    const int oc = (16 * na + i16) + CW * j;
-                    const double cr = Ct[2 * oc], ci = Ct[2 * oc + 1];
+                    const double cr = Rt[(16 * na + i16) + CW * j];
                     double nvr = 0, nvi = 0;
                     if (j < m) {
                         const double* nv = Nc + 2 * ((16 * mt + i16) + KP * j);
@@ -608,15 +629,15 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                         nvi = nv[1];
                     }
                     pr = mfma3(cr, nvr, pr);
-                    pr = mfma3(-ci, nvi, pr);
                     pi = mfma3(cr, nvi, pi);
-                    pi = mfma3(ci, nvr, pi);
                 }
+                // P' = i^k P_c, k = 16 mt + i16
 #pragma unroll
                 for (int rr = 0; rr < 4; ++rr) {
                     const int o = (16 * mt + i16) + PK * (16 * na + k4 + 4 * rr);
-                    Pc[2 * o] = pr[rr];
-                    Pc[2 * o + 1] = pi[rr];
+                    const double2 v = rot_ij(pr[rr], pi[rr], i16);
+                    Pc[2 * o] = v.x;
+                    Pc[2 * o + 1] = v.y;
                 }
             }
             __syncthreads();
@@ -628,14 +649,11 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                 d4_t3 vr = {0, 0, 0, 0}, vi = {0, 0, 0, 0};
                 for (int ks = 0; ks < KS; ++ks) {
                     const int k = 4 * ks + k4;
-                    const int oc = (16 * nb + i16) + CW * k;
                     const int op = k + PK * (16 * na + i16);
-                    const double cr = Ct[2 * oc], ci = Ct[2 * oc + 1];
+                    const double cr = Rt[(16 * nb + i16) + CW * k];
                     const double xr = Pc[2 * op], xi = Pc[2 * op + 1];
                     vr = mfma3(cr, xr, vr);
-                    vr = mfma3(-ci, xi, vr);
                     vi = mfma3(cr, xi, vi);
-                    vi = mfma3(ci, xr, vi);
                 }
                 const int aa = 16 * na + i16;
 #pragma unroll
@@ -663,8 +681,8 @@ long form3_mfma_scratch(int m, int npw)
 template <int NB, bool DIP>
 static void form3_mfma_launch(const Form3Args& a, hipStream_t s)
 {
-    const int m = a.p + 1, KP = 16 * ((m + 15) / 16), KS = (m + 3) / 4;
-    const size_t ct = (size_t)2 * (16 * NB + 1) * 4 * KS;
+    const int m = a.p + 1, KP = 16 * ((m + 15) / 16), JR = 8 * (((m + 1) / 2 + 3) / 4);
+    const size_t ct = (size_t)(16 * NB + 1) * JR;
     const size_t src = (size_t)(DIP ? 6 : 3) * 32 * KP + (DIP ? 4 * 32 : 0);
     constexpr int G = NB * NB >= 8 ? 1 : 8 / (NB * NB);
     const size_t pc = (size_t)2 * G * 2 * (KP + 1) * 16 * NB;
@@ -1114,11 +1132,24 @@ __global__ __launch_bounds__(256) void k_pw2local3(Eval3Args a)
 
 
 // PW -> Taylor on matrix cores for Taylor orders nl < 16 and npw <= 32, with
-// D(a, j) = (i ts_a)^j / j!:
+// D(a, j) = (i ts_a)^j / j! = i^j d(a, j), d real:
 //   for each z node c < npw/2:
-//     S1(j3, b) = sum_a D(a, j3) PW(a, b, c)      MFMA, a padded to 32
-//     S2(j3, j2) = sum_b S1(j3, b) D(b, j2)       MFMA
-//     L(j3, j2, j1) += 2 Re(D(c, j1) S2(j3, j2))  registers
+//     S1'(j3, b) = sum_a d(a, j3) PW(a, b, c)     MFMA, a padded to 32
+//     S2'(j3, j2) = sum_b S1'(j3, b) d(b, j2)     MFMA
+//     L(j3, j2, j1) += 2 d(c, j1) Re(i^(j1 + j2 + j3) S2'(j3, j2))  registers
+// (S1 = i^j3 S1', S2 = i^(j2 + j3) S2'), two real products per complex
+// operand instead of four.
+// d(a, j) from the complex table D = i^j d: its one non-zero part, signed
+__device__ __forceinline__ double dreal(const double* __restrict__ dc, int j)
+{
+    const double v = (j & 1) ? dc[1] : dc[0];
+    return (j & 3) >= 2 ? -v : v;
+}
+
+// the factor of Re(i^(j + q) z) = f(j) * (j odd ? Im : Re)(i^q z) for order
+// j: +1 for j = 0, 3 mod 4, -1 for j = 1, 2 mod 4
+__device__ __forceinline__ constexpr double jsign(int j) { return ((j + 1) & 2) ? -1.0 : 1.0; }
+
 constexpr int PW3_N = 32;  // padded npw
 constexpr int PW3_S = 33;  // LDS stride of a slab column (bank-conflict-free lane stride)
 
@@ -1137,13 +1168,12 @@ __global__ __launch_bounds__(128, 2) void k_pw2local3_wave2(Eval3Args a)
     const long nexp = (long)npw * npw * half;
     double* P = sm;                        // complex (PW3_N a) x (PW3_N b) at a + PW3_S b
     double* S1 = sm + 2 * PW3_S * PW3_N;   // complex (16 j3) x (PW3_N b) at j3 + 17 b
-    double* Dt = S1 + 2 * 17 * PW3_N;      // complex D(a, j) at j + 16 a, a < PW3_N
+    double* Dt = S1 + 2 * 17 * PW3_N;      // real d(a, j) at j + 16 a, a < PW3_N
     const int i16 = lane & 15, k4 = lane >> 4;
+    const int q = (i16 + k4) & 3;          // (j2 + j3) mod 4 of this lane's S2' elements
     for (int o = t; o < PW3_N * 16; o += 128) {
         const int aa = o / 16, j = o % 16;
-        const bool in = aa < npw && j < m;
-        Dt[2 * o] = in ? a.dcoef[2 * (aa + a.cstride * j)] : 0.0;
-        Dt[2 * o + 1] = in ? a.dcoef[2 * (aa + a.cstride * j) + 1] : 0.0;
+        Dt[o] = aa < npw && j < m ? dreal(a.dcoef + 2 * (aa + a.cstride * j), j) : 0.0;
     }
     for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
         const int box = a.boxes[li];
@@ -1197,13 +1227,10 @@ __global__ __launch_bounds__(128, 2) void k_pw2local3_wave2(Eval3Args a)
 #pragma unroll
                 for (int ks = 0; ks < PW3_N / 4; ++ks) {
                     const int o = (4 * ks + k4) + PW3_S * (16 * w + i16);
-                    const int od = i16 + 16 * (4 * ks + k4);
                     const double pr = P[2 * o], pi = P[2 * o + 1];
-                    const double dr = Dt[2 * od], di = Dt[2 * od + 1];
-                    s1r = mfma3(dr, pr, s1r);
-                    s1r = mfma3(-di, pi, s1r);
-                    s1i = mfma3(dr, pi, s1i);
-                    s1i = mfma3(di, pr, s1i);
+                    const double d = Dt[i16 + 16 * (4 * ks + k4)];
+                    s1r = mfma3(d, pr, s1r);
+                    s1i = mfma3(d, pi, s1i);
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -1217,19 +1244,23 @@ __global__ __launch_bounds__(128, 2) void k_pw2local3_wave2(Eval3Args a)
 #pragma unroll
             for (int ks = 0; ks < PW3_N / 4; ++ks) {
                 const int o = i16 + 17 * (4 * ks + k4);
-                const int od = i16 + 16 * (4 * ks + k4);
                 const double xr = S1[2 * o], xi = S1[2 * o + 1];
-                const double dr = Dt[2 * od], di = Dt[2 * od + 1];
-                s2r = mfma3(xr, dr, s2r);
-                s2r = mfma3(-xi, di, s2r);
-                s2i = mfma3(xr, di, s2i);
-                s2i = mfma3(xi, dr, s2i);
+                const double d = Dt[i16 + 16 * (4 * ks + k4)];
+                s2r = mfma3(xr, d, s2r);
+                s2i = mfma3(xi, d, s2i);
+            }
+            double zr[4], zi[4];  // i^q S2'
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double2 z = rot_ij(s2r[r], s2i[r], q);
+                zr[r] = z.x;
+                zi[r] = z.y;
             }
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const double dr = Dt[2 * (16 * c + 8 * w + j)], di = Dt[2 * (16 * c + 8 * w + j) + 1];  // D(c, j1)
+            for (int j = 0; j < 8; ++j) {  // j1 = 8 w + j, j1 = j mod 4
+                const double f = 2.0 * jsign(j) * Dt[16 * c + 8 * w + j];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) L[r][j] += 2.0 * (dr * s2r[r] - di * s2i[r]);
+                for (int r = 0; r < 4; ++r) L[r][j] += f * ((j & 1) ? zi[r] : zr[r]);
             }
         }
         double* out = a.taylor + (long)li * taylor3_stride(m, 1);
@@ -1253,8 +1284,9 @@ __global__ __launch_bounds__(128, 2) void k_pw2local3_wave2(Eval3Args a)
 //     j2: (w / 2) % 2) (waves w and w + 4 compute the same tile);
 //   L(j3, j2, j1) += 2 Re(D(c, j1) S2(j3, j2)) for the wave's half of j1
 //     (w / 4), in registers.
-// D(a, j) sits in LDS for every a < 4 KA, j < 32 (zero outside a < npw,
-// j < m).  Writes the transposed layout Lt[j2 + 32 (j3 + m j1)].
+// The products run on the real d(a, j) of D = i^j d, as in
+// k_pw2local3_wave2; d sits in LDS for every a < 4 KA, j < 32 (zero outside
+// a < npw, j < m).  Writes the transposed layout Lt[j2 + 32 (j3 + m j1)].
 template <int KA, int NB>
 __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
 {
@@ -1262,16 +1294,15 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
     extern __shared__ double sm[];
     double* P = sm;                     // complex (A4 a) x (BP b) at a + PS b
     double* S1 = P + 2 * PS * BP;       // complex (32 j3) x (BP b) at j3 + 33 b
-    double* Dt = S1 + 2 * 33 * BP;      // complex D(a, j) at j + 32 a, a < A4
+    double* Dt = S1 + 2 * 33 * BP;      // real d(a, j) at j + 32 a, a < A4
     const int m = a.nl + 1, npw = a.npw, half = npw / 2, t = threadIdx.x;
     const int w = t >> 6, lane = t & 63, i16 = lane & 15, k4 = lane >> 4;
     const int mt = w & 1, jt = (w >> 1) & 1, jh = w >> 2;
+    const int q = (i16 + k4) & 3;       // (j2 + j3) mod 4 of this lane's S2' elements
     const long nexp = (long)npw * npw * half;
     for (int o = t; o < A4 * 32; o += 512) {
         const int aa = o / 32, j = o % 32;
-        const bool in = aa < npw && j < m;
-        Dt[2 * o] = in ? a.dcoef[2 * (aa + a.cstride * j)] : 0.0;
-        Dt[2 * o + 1] = in ? a.dcoef[2 * (aa + a.cstride * j) + 1] : 0.0;
+        Dt[o] = aa < npw && j < m ? dreal(a.dcoef + 2 * (aa + a.cstride * j), j) : 0.0;
     }
     for (int li = blockIdx.x; li < a.nbox; li += gridDim.x) {
         const int box = a.boxes[li];
@@ -1329,13 +1360,10 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
 #pragma unroll 2
                 for (int ks = 0; ks < KA; ++ks) {
                     const int o = (4 * ks + k4) + PS * (16 * nb + i16);
-                    const int od = (16 * mt + i16) + 32 * (4 * ks + k4);
                     const double pr = P[2 * o], pi = P[2 * o + 1];
-                    const double ar = Dt[2 * od], ai = Dt[2 * od + 1];
-                    sr = mfma3(ar, pr, sr);
-                    sr = mfma3(-ai, pi, sr);
-                    si = mfma3(ar, pi, si);
-                    si = mfma3(ai, pr, si);
+                    const double d = Dt[(16 * mt + i16) + 32 * (4 * ks + k4)];
+                    sr = mfma3(d, pr, sr);
+                    si = mfma3(d, pi, si);
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -1349,20 +1377,23 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
 #pragma unroll 2
             for (int ks = 0; ks < KA; ++ks) {
                 const int o = (16 * mt + i16) + 33 * (4 * ks + k4);
-                const int od = (16 * jt + i16) + 32 * (4 * ks + k4);
                 const double xr = S1[2 * o], xi = S1[2 * o + 1];
-                const double br = Dt[2 * od], bi = Dt[2 * od + 1];
-                s2r = mfma3(xr, br, s2r);
-                s2r = mfma3(-xi, bi, s2r);
-                s2i = mfma3(xr, bi, s2i);
-                s2i = mfma3(xi, br, s2i);
+                const double d = Dt[(16 * jt + i16) + 32 * (4 * ks + k4)];
+                s2r = mfma3(xr, d, s2r);
+                s2i = mfma3(xi, d, s2i);
+            }
+            double zr[4], zi[4];  // i^q S2'
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double2 z = rot_ij(s2r[r], s2i[r], q);
+                zr[r] = z.x;
+                zi[r] = z.y;
             }
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int o = 32 * c + 16 * jh + j;
-                const double dr = Dt[2 * o], di = Dt[2 * o + 1];
+            for (int j = 0; j < 16; ++j) {  // j1 = 16 jh + j, j1 = j mod 4
+                const double f = 2.0 * jsign(j) * Dt[32 * c + 16 * jh + j];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) L[r][j] += 2.0 * (dr * s2r[r] - di * s2i[r]);
+                for (int r = 0; r < 4; ++r) L[r][j] += f * ((j & 1) ? zi[r] : zr[r]);
                 if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // bounds the loads in flight
             }
         }
@@ -1384,7 +1415,7 @@ template <int KA, int NB>
 static void pw2local3_blk_launch(const Eval3Args& a, hipStream_t s)
 {
     const size_t lds = ((size_t)2 * (4 * KA + 1) * 16 * NB + (size_t)2 * 33 * 16 * NB +
-                        (size_t)2 * 4 * KA * 32) * sizeof(double);
+                        (size_t)4 * KA * 32) * sizeof(double);
     allow_lds3(k_pw2local3_blk<KA, NB>, lds);
     k_pw2local3_blk<KA, NB><<<std::min(a.nbox, 8192), 512, lds, s>>>(a);
 }
@@ -1404,7 +1435,7 @@ void launch_pw2local3(const Eval3Args& a, hipStream_t s)
     if (a.fused && !(pw2local3_fusable(m, a.npw) && a.tlayout))
         fail(-3, "fused split requested for a PW -> Taylor kernel without it");
     if (m <= 16 && a.npw <= PW3_N && a.tlayout) {
-        size_t lds = ((size_t)2 * PW3_S * PW3_N + (size_t)2 * 17 * PW3_N + (size_t)2 * PW3_N * 16) *
+        size_t lds = ((size_t)2 * PW3_S * PW3_N + (size_t)2 * 17 * PW3_N + (size_t)PW3_N * 16) *
                      sizeof(double);
         kmark(KID_PW2LOCAL, false, s);
         k_pw2local3_wave2<<<std::min(a.nbox, 65536), 128, lds, s>>>(a);
