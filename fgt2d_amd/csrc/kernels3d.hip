@@ -862,6 +862,15 @@ __device__ __forceinline__ int octant(double dx, double dy, double dz)
     return dz > 0 ? k : k + 4;
 }
 
+// The shift kernels run one block per (box, chunk of SH3_E coefficients),
+// chunk-major: the box's indices, octants and slots are block-uniform
+// (scalar loads, once per block), each thread keeps SH3_V independent 16-byte
+// loads per operand in flight, and consecutive blocks take neighbouring boxes
+// at the same chunk, so expansions they share are read from cache.
+constexpr int SH3_V = 4, SH3_E = 256 * SH3_V;
+
+// a parent's expansion is the sum of its children's, shifted: written, not
+// accumulated, so the slot needs no clearing beforehand
 __global__ __launch_bounds__(256) void k_merge3(const int* __restrict__ parents, int np,
                                                 const int* __restrict__ child,
                                                 const double* __restrict__ cen,
@@ -870,28 +879,44 @@ __global__ __launch_bounds__(256) void k_merge3(const int* __restrict__ parents,
                                                 const double* __restrict__ wms, int kbase, int nd,
                                                 long nexp, double* __restrict__ mp)
 {
-    for (long t = blockIdx.x * 256L + threadIdx.x; t < (long)np * nexp; t += gridDim.x * 256L) {
-        const long pi = t / nexp, e = t - pi * nexp;
-        int p = parents[pi];
+    const long nchunk = (nexp + SH3_E - 1) / SH3_E;
+    const double2* w2 = reinterpret_cast<const double2*>(wms);
+    double2* m2 = reinterpret_cast<double2*>(mp);
+    for (long q = blockIdx.x; q < (long)np * nchunk; q += gridDim.x) {
+        const int p = parents[q % np];
+        const long e0 = (q / np) * SH3_E + threadIdx.x;
         for (int ind = 0; ind < nd; ++ind) {
-            // a parent's expansion is the sum of its children's: written, not
-            // accumulated, so the slot needs no clearing beforehand
-            double* dst = mp + 2 * (((long)slot[p] * nd + ind) * nexp + e);
-            double re = 0, im = 0;
+            double re[SH3_V], im[SH3_V];
+#pragma unroll
+            for (int v = 0; v < SH3_V; ++v) re[v] = im[v] = 0;
             for (int j = 0; j < 8; ++j) {
-                int c = child[8 * p + j];
+                const int c = child[8 * p + j];
                 if (c < 0) continue;
-                int2 r = src_se[c];
+                const int2 r = src_se[c];
                 if (r.y - r.x <= 0) continue;
-                int k = octant(cen[3 * p] - cen[3 * c], cen[3 * p + 1] - cen[3 * c + 1],
-                               cen[3 * p + 2] - cen[3 * c + 2]);
-                const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
-                const double* v = mp + 2 * (((long)slot[c] * nd + ind) * nexp + e);
-                re += v[0] * w[0] - v[1] * w[1];
-                im += v[0] * w[1] + v[1] * w[0];
+                const int k = octant(cen[3 * p] - cen[3 * c], cen[3 * p + 1] - cen[3 * c + 1],
+                                     cen[3 * p + 2] - cen[3 * c + 2]);
+                const double2* w = w2 + (long)(kbase + k) * nexp;
+                const double2* x = m2 + ((long)slot[c] * nd + ind) * nexp;
+                double2 xv[SH3_V], wv[SH3_V];
+#pragma unroll
+                for (int v = 0; v < SH3_V; ++v) {
+                    const long e = e0 + 256 * v;
+                    xv[v] = e < nexp ? x[e] : make_double2(0.0, 0.0);
+                    wv[v] = e < nexp ? w[e] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int v = 0; v < SH3_V; ++v) {
+                    re[v] += xv[v].x * wv[v].x - xv[v].y * wv[v].y;
+                    im[v] += xv[v].x * wv[v].y + xv[v].y * wv[v].x;
+                }
             }
-            dst[0] = re;
-            dst[1] = im;
+            double2* dst = m2 + ((long)slot[p] * nd + ind) * nexp;
+#pragma unroll
+            for (int v = 0; v < SH3_V; ++v) {
+                const long e = e0 + 256 * v;
+                if (e < nexp) dst[e] = make_double2(re[v], im[v]);
+            }
         }
     }
 }
@@ -902,8 +927,8 @@ void launch_merge3(const int* parents, int np, const int* child8, const double* 
 {
     if (np <= 0) return;
     kmark(KID_MERGE, false, s);
-    k_merge3<<<gridcap((long)np * nexp, 256), 256, 0, s>>>(parents, np, child8, cen, src_se, slot,
-                                                         wms, kbase, nd, nexp, mp);
+    k_merge3<<<gridcap((long)np * ((nexp + SH3_E - 1) / SH3_E), 1), 256, 0, s>>>(
+        parents, np, child8, cen, src_se, slot, wms, kbase, nd, nexp, mp);
     kmark(KID_MERGE, true, s);
 }
 
@@ -913,8 +938,6 @@ void launch_merge3(const int* parents, int np, const int* child8, const double* 
 // evaluation points get a zero expansion.  Multipole and local expansions
 // have separate slot tables; a box without a multipole slot (slot_mp < 0)
 // has a zero multipole expansion.
-constexpr long M2L3_CHUNK = 512;
-
 __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int nb,
                                               const double* __restrict__ cen,
                                               const int* __restrict__ ncol,
@@ -929,41 +952,60 @@ __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int
                                               const double* __restrict__ mp,
                                               double* __restrict__ loc)
 {
-    // threads run over chunks of M2L3_CHUNK coefficients box after box, so
-    // boxes that share colleagues read the same chunk of a colleague's
-    // expansion close together in time (from cache), in runs long enough
-    // for DRAM
-    const long nchunk = (nexp + M2L3_CHUNK - 1) / M2L3_CHUNK;
-    for (long t = blockIdx.x * 256L + threadIdx.x; t < (long)nb * nchunk * M2L3_CHUNK;
-         t += gridDim.x * 256L) {
-        const long q = t / M2L3_CHUNK, bi = q % nb, e = (q / nb) * M2L3_CHUNK + t % M2L3_CHUNK;
-        if (e >= nexp) continue;
-        int b = boxes[bi];
+    const long nchunk = (nexp + SH3_E - 1) / SH3_E;
+    const double2* w2 = reinterpret_cast<const double2*>(wm2l);
+    const double2* m2 = reinterpret_cast<const double2*>(mp);
+    double2* l2 = reinterpret_cast<double2*>(loc);
+    for (long q = blockIdx.x; q < (long)nb * nchunk; q += gridDim.x) {
+        const int b = boxes[q % nb];
+        const long e0 = (q / nb) * SH3_E + threadIdx.x;
         int npts = 0;
         if (use_trg) npts += trg_se[b].y - trg_se[b].x;
         if (use_src) npts += src_se[b].y - src_se[b].x;
         for (int ind = 0; ind < nd; ++ind) {
-            double* d = loc + 2 * (((long)slot_loc[b] * nd + ind) * nexp + e);
-            double re = 0, im = 0;
+            double re[SH3_V], im[SH3_V];
+#pragma unroll
+            for (int v = 0; v < SH3_V; ++v) re[v] = im[v] = 0;
             if (npts > 0 && do_list)
                 for (int i = 0; i < ncol[b]; ++i) {
-                    int j = col[27 * b + i];
+                    const int j = col[27 * b + i];
                     if (j == b || nch[j] <= 0 || slot_mp[j] < 0) continue;
-                    int jx = (int)rint((cen[3 * b] - cen[3 * j]) / xmin);
-                    int jy = (int)rint((cen[3 * b + 1] - cen[3 * j + 1]) / xmin);
-                    int jz = (int)rint((cen[3 * b + 2] - cen[3 * j + 2]) / xmin);
-                    const double* w = wm2l + 2 * ((long)((jx + 1) + 3 * (jy + 1) + 9 * (jz + 1)) * nexp + e);
-                    const double* v = mp + 2 * (((long)slot_mp[j] * nd + ind) * nexp + e);
-                    re += v[0] * w[0] - v[1] * w[1];
-                    im += v[0] * w[1] + v[1] * w[0];
+                    const int jx = (int)rint((cen[3 * b] - cen[3 * j]) / xmin);
+                    const int jy = (int)rint((cen[3 * b + 1] - cen[3 * j + 1]) / xmin);
+                    const int jz = (int)rint((cen[3 * b + 2] - cen[3 * j + 2]) / xmin);
+                    const double2* w = w2 + (long)((jx + 1) + 3 * (jy + 1) + 9 * (jz + 1)) * nexp;
+                    const double2* x = m2 + ((long)slot_mp[j] * nd + ind) * nexp;
+                    double2 xv[SH3_V], wv[SH3_V];
+#pragma unroll
+                    for (int v = 0; v < SH3_V; ++v) {
+                        const long e = e0 + 256 * v;
+                        xv[v] = e < nexp ? x[e] : make_double2(0.0, 0.0);
+                        wv[v] = e < nexp ? w[e] : make_double2(0.0, 0.0);
+                    }
+#pragma unroll
+                    for (int v = 0; v < SH3_V; ++v) {
+                        re[v] += xv[v].x * wv[v].x - xv[v].y * wv[v].y;
+                        im[v] += xv[v].x * wv[v].y + xv[v].y * wv[v].x;
+                    }
                 }
             if (npts > 0 && slot_mp[b] >= 0) {
-                const double* v = mp + 2 * (((long)slot_mp[b] * nd + ind) * nexp + e);
-                re += v[0];
-                im += v[1];
+                const double2* x = m2 + ((long)slot_mp[b] * nd + ind) * nexp;
+#pragma unroll
+                for (int v = 0; v < SH3_V; ++v) {
+                    const long e = e0 + 256 * v;
+                    if (e < nexp) {
+                        const double2 xv = x[e];
+                        re[v] += xv.x;
+                        im[v] += xv.y;
+                    }
+                }
             }
-            d[0] = re;
-            d[1] = im;
+            double2* dst = l2 + ((long)slot_loc[b] * nd + ind) * nexp;
+#pragma unroll
+            for (int v = 0; v < SH3_V; ++v) {
+                const long e = e0 + 256 * v;
+                if (e < nexp) dst[e] = make_double2(re[v], im[v]);
+            }
         }
     }
 }
@@ -975,9 +1017,9 @@ void launch_m2l3(const int* boxes, int nb, const double* cen, const int* ncol, c
 {
     if (nb <= 0) return;
     kmark(KID_M2L, false, s);
-    k_m2l3<<<gridcap((long)nb * ((nexp + M2L3_CHUNK - 1) / M2L3_CHUNK) * M2L3_CHUNK, 256), 256, 0, s>>>(boxes, nb, cen, ncol, col27, nch, src_se,
-                                                       trg_se, use_src, use_trg, slot_mp, slot_loc,
-                                                       wm2l, xmin, do_list, nd, nexp, mp, loc);
+    k_m2l3<<<gridcap((long)nb * ((nexp + SH3_E - 1) / SH3_E), 1), 256, 0, s>>>(
+        boxes, nb, cen, ncol, col27, nch, src_se, trg_se, use_src, use_trg, slot_mp, slot_loc, wm2l,
+        xmin, do_list, nd, nexp, mp, loc);
     kmark(KID_M2L, true, s);
 }
 
@@ -995,21 +1037,36 @@ __global__ __launch_bounds__(256) void k_split3(const int* __restrict__ children
                                                 const int2* __restrict__ trg_se, int use_src,
                                                 int use_trg)
 {
-    for (long t = blockIdx.x * 256L + threadIdx.x; t < (long)nc * nexp; t += gridDim.x * 256L) {
-        const long ci = t / nexp, e = t - ci * nexp;
-        int c = children[ci], p = par[c];
+    const long nchunk = (nexp + SH3_E - 1) / SH3_E;
+    const double2* w2 = reinterpret_cast<const double2*>(wms);
+    double2* l2 = reinterpret_cast<double2*>(loc);
+    for (long q = blockIdx.x; q < (long)nc * nchunk; q += gridDim.x) {
+        const int c = children[q % nc], p = par[c];
+        const long e0 = (q / nc) * SH3_E + threadIdx.x;
         int npts = 0;
         if (use_trg) npts += trg_se[c].y - trg_se[c].x;
         if (use_src) npts += src_se[c].y - src_se[c].x;
         if (npts <= 0) continue;
-        int k = octant(cen[3 * c] - cen[3 * p], cen[3 * c + 1] - cen[3 * p + 1],
-                       cen[3 * c + 2] - cen[3 * p + 2]);
-        const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
+        const int k = octant(cen[3 * c] - cen[3 * p], cen[3 * c + 1] - cen[3 * p + 1],
+                             cen[3 * c + 2] - cen[3 * p + 2]);
+        const double2* w = w2 + (long)(kbase + k) * nexp;
         for (int ind = 0; ind < nd; ++ind) {
-            const double* v = loc + 2 * (((long)slot[p] * nd + ind) * nexp + e);
-            double* d = loc + 2 * (((long)slot[c] * nd + ind) * nexp + e);
-            d[0] = v[0] * w[0] - v[1] * w[1];
-            d[1] = v[0] * w[1] + v[1] * w[0];
+            const double2* x = l2 + ((long)slot[p] * nd + ind) * nexp;
+            double2* dst = l2 + ((long)slot[c] * nd + ind) * nexp;
+            double2 xv[SH3_V], wv[SH3_V];
+#pragma unroll
+            for (int v = 0; v < SH3_V; ++v) {
+                const long e = e0 + 256 * v;
+                xv[v] = e < nexp ? x[e] : make_double2(0.0, 0.0);
+                wv[v] = e < nexp ? w[e] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int v = 0; v < SH3_V; ++v) {
+                const long e = e0 + 256 * v;
+                if (e < nexp)
+                    dst[e] = make_double2(xv[v].x * wv[v].x - xv[v].y * wv[v].y,
+                                          xv[v].x * wv[v].y + xv[v].y * wv[v].x);
+            }
         }
     }
 }
@@ -1021,9 +1078,8 @@ void launch_split3(const int* children, int nc, const int* par, const double* ce
 {
     if (nc <= 0) return;
     kmark(KID_SPLIT, false, s);
-    k_split3<<<gridcap((long)nc * nexp, 256), 256, 0, s>>>(children, nc, par, cen, slot, wms, kbase,
-                                                         nd, nexp, loc, src_se, trg_se, use_src,
-                                                         use_trg);
+    k_split3<<<gridcap((long)nc * ((nexp + SH3_E - 1) / SH3_E), 1), 256, 0, s>>>(
+        children, nc, par, cen, slot, wms, kbase, nd, nexp, loc, src_se, trg_se, use_src, use_trg);
     kmark(KID_SPLIT, true, s);
 }
 
