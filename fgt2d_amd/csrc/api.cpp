@@ -101,6 +101,8 @@ struct Workspace {
     // host scratch of the plan, reused from call to call
     std::vector<int> h_units, h_uptr, h_ulist, h_nl1, h_l1, h_sp, h_sl, h_fill, h_slot, h_slot2;
     std::vector<char> h_plan, h_plan_nf;
+    std::vector<unsigned long long> h_lkey64;  // smallest key of every leaf, rank order
+    std::vector<unsigned> h_lkey32;
     size_t leaf_guess = 0;  // leaf records of the last call (size of the first read-back)
     Tables2D tab;
     Tables3D tab3;
@@ -394,6 +396,31 @@ struct Pack {
         return add(v.data(), v.size());
     }
 };
+
+// Smallest full-depth Morton key of every leaf of the final tree, in rank
+// (Morton) order: the leaf's prefix shifted past the levels below it.  The
+// GPU finds each leaf's first point by searching the sorted keys for it.
+template <int D>
+size_t pack_leaf_keys(Pack& pk, Workspace& W, const TreeD<D>& T, int nlmax, bool k32)
+{
+    const size_t nl = T.leaf_of_rank.size();
+    if (k32) {
+        std::vector<unsigned>& k = W.h_lkey32;
+        k.resize(nl);
+        for (size_t r = 0; r < nl; ++r) {
+            const int b = T.leaf_of_rank[r];
+            k[r] = (unsigned)(T.pre[b] << (D * (nlmax - T.lev[b])));
+        }
+        return pk.add(k);
+    }
+    std::vector<unsigned long long>& k = W.h_lkey64;
+    k.resize(nl);
+    for (size_t r = 0; r < nl; ++r) {
+        const int b = T.leaf_of_rank[r];
+        k[r] = (unsigned long long)T.pre[b] << (D * (nlmax - T.lev[b]));
+    }
+    return pk.add(k);
+}
 
 // ---------------------------------------------------------------------------
 // plane-wave tables: nodes, Hermite->PW and PW->Taylor 1-D matrices, and the
@@ -829,7 +856,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     const int colb1 = expansions ? T.laddr[2 * nlevstart + 1] : -1;
     size_t o_ncol = pk.add(T.ncol);
     size_t o_col = pk.add(T.col.data() + 9 * (size_t)colb0, 9 * (size_t)(colb1 - colb0 + 1));
-    size_t o_rank = pk.add(T.rank_of_box);
+    size_t o_lkey = pack_leaf_keys(pk, W, T, nlmax, k32);
     size_t o_rlo = pk.add(T.rank_lo), o_rhi = pk.add(T.rank_hi), o_slot = pk.add(slot);
 
     // work lists of the expansion steps
@@ -960,32 +987,29 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     const double* d_cen = (const double*)D(o_cen);
     const int* d_ncol = (const int*)D(o_ncol);
     const int* d_col = (const int*)D(o_col) - 9 * (ptrdiff_t)colb0;  // indexed by box
-    const int* d_rank = (const int*)D(o_rank);
+    const void* d_lkey = D(o_lkey);
     const int* d_rlo = (const int*)D(o_rlo);
     const int* d_rhi = (const int*)D(o_rhi);
     const int* d_slot = (const int*)D(o_slot);
 
     // ---------------- point ranges of the boxes ----------------
-    // leaf rank of every point (non-decreasing in Morton order), first point
-    // of every leaf, then the range of every box
+    // first point of every leaf (a search of the sorted keys), then the
+    // range of every box
     tm.begin(S_RANGES);
-    unsigned* rk = W.get<unsigned>("rank", nmx);
     int* ls = W.get<int>("leafstart", nleaves + 1);
     int2* src_se = W.get<int2>("src_se", nb);
     int2* trg_se = W.get<int2>("trg_se", nb);
     if (k32)
-        launch_descend_rank((const unsigned*)ks2, nullptr, ns, nlmax, d_child, d_rank, rk, s);
+        launch_leafstart_search((const unsigned*)ks2, ns, (const unsigned*)d_lkey, nleaves, ls, s);
     else
-        launch_descend_rank(ks2, nullptr, ns, nlmax, d_child, d_rank, rk, s);
-    launch_leafstart(rk, ns, nleaves, ls, s);
+        launch_leafstart_search(ks2, ns, (const u64*)d_lkey, nleaves, ls, s);
     launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, s);
     tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
     if (nt > 0) {
         if (k32)
-            launch_descend_rank((const unsigned*)kt2, nullptr, nt, nlmax, d_child, d_rank, rk, s);
+            launch_leafstart_search((const unsigned*)kt2, nt, (const unsigned*)d_lkey, nleaves, ls, s);
         else
-            launch_descend_rank(kt2, nullptr, nt, nlmax, d_child, d_rank, rk, s);
-        launch_leafstart(rk, nt, nleaves, ls, s);
+            launch_leafstart_search(kt2, nt, (const u64*)d_lkey, nleaves, ls, s);
         launch_box_ranges(d_rlo, d_rhi, nb, ls, trg_se, s);
     } else {
         HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));
@@ -1795,7 +1819,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     const int colb1 = expansions ? T.laddr[2 * nlevstart + 1] : -1;
     size_t o_ncol = pk.add(T.ncol);
     size_t o_col = pk.add(T.col.data() + 27 * (size_t)colb0, 27 * (size_t)(colb1 - colb0 + 1));
-    size_t o_rank = pk.add(T.rank_of_box);
+    size_t o_lkey = pack_leaf_keys(pk, W, T, nlmax, k32);
     size_t o_rlo = pk.add(T.rank_lo), o_rhi = pk.add(T.rank_hi);
     size_t o_smp = pk.add(slot_mp), o_sloc = pk.add(slot_loc);
     size_t o_nch = pk.add(T.nch);
@@ -1909,7 +1933,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     const double* d_cen = (const double*)D(o_cen);
     const int* d_ncol = (const int*)D(o_ncol);
     const int* d_col = (const int*)D(o_col) - 27 * (ptrdiff_t)colb0;  // indexed by box
-    const int* d_rank = (const int*)D(o_rank);
+    const void* d_lkey = D(o_lkey);
     const int* d_rlo = (const int*)D(o_rlo);
     const int* d_rhi = (const int*)D(o_rhi);
     const int* d_smp = (const int*)D(o_smp);
@@ -1918,23 +1942,20 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
 
     // ---------------- point ranges of the boxes ----------------
     tm.begin(S_RANGES);
-    unsigned* rk = W.get<unsigned>("rank", nmx);
     int* ls = W.get<int>("leafstart", nleaves + 1);
     int2* src_se = W.get<int2>("src_se", nb);
     int2* trg_se = W.get<int2>("trg_se", nb);
     if (k32)
-        launch_descend_rank3((const unsigned*)ks2, ns, nlmax, d_child, d_rank, rk, s);
+        launch_leafstart_search((const unsigned*)ks2, ns, (const unsigned*)d_lkey, nleaves, ls, s);
     else
-        launch_descend_rank3(ks2, ns, nlmax, d_child, d_rank, rk, s);
-    launch_leafstart(rk, ns, nleaves, ls, s);
+        launch_leafstart_search(ks2, ns, (const u64*)d_lkey, nleaves, ls, s);
     launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, s);
     tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
     if (nt > 0) {
         if (k32)
-            launch_descend_rank3((const unsigned*)kt2, nt, nlmax, d_child, d_rank, rk, s);
+            launch_leafstart_search((const unsigned*)kt2, nt, (const unsigned*)d_lkey, nleaves, ls, s);
         else
-            launch_descend_rank3(kt2, nt, nlmax, d_child, d_rank, rk, s);
-        launch_leafstart(rk, nt, nleaves, ls, s);
+            launch_leafstart_search(kt2, nt, (const u64*)d_lkey, nleaves, ls, s);
         launch_box_ranges(d_rlo, d_rhi, nb, ls, trg_se, s);
     } else {
         HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));

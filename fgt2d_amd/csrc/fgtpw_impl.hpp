@@ -124,15 +124,12 @@ void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigne
                         const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits = 2);
 void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigned* key,
                         const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits = 2);
-void launch_descend_rank(const unsigned long long* key, const unsigned* idx, int n, int nlmax,
-                         const int4* child, const int* rank_of_box, unsigned* rank_out,
-                         hipStream_t s);
-void launch_descend_rank(const unsigned* key, const unsigned* idx, int n, int nlmax,
-                         const int4* child, const int* rank_of_box, unsigned* rank_out,
-                         hipStream_t s);
+// ls[r] = first of the n sorted keys >= leafkey[r], r < nleaves; ls[nleaves] = n
+void launch_leafstart_search(const unsigned long long* key, int n, const unsigned long long* leafkey,
+                             int nleaves, int* ls, hipStream_t s);
+void launch_leafstart_search(const unsigned* key, int n, const unsigned* leafkey, int nleaves,
+                             int* ls, hipStream_t s);
 void launch_iota(unsigned* v, int n, hipStream_t s);
-void launch_leafstart(const unsigned* rank_sorted, int n, int nleaves, int* leafstart,
-                      hipStream_t s);
 void launch_box_ranges(const int* rank_lo, const int* rank_hi, int nb, const int* ls, int2* se,
                        hipStream_t s);
 void launch_gather(const double* xy, const double* q, int nd, const unsigned* perm, int n,
@@ -263,10 +260,6 @@ void launch_keys3d(const double* p, int n, const double* c0, double bs0, int nlm
                    unsigned long long* key, unsigned* idx, hipStream_t s);
 void launch_keys3d(const double* p, int n, const double* c0, double bs0, int nlmax,
                    unsigned* key, unsigned* idx, hipStream_t s);
-void launch_descend_rank3(const unsigned long long* key, int n, int nlmax, const int* child8,
-                          const int* rank_of_box, unsigned* rank_out, hipStream_t s);
-void launch_descend_rank3(const unsigned* key, int n, int nlmax, const int* child8,
-                          const int* rank_of_box, unsigned* rank_out, hipStream_t s);
 void launch_gather3(const double* p, const double* q, int nd, const unsigned* perm, int n,
                     double* ps, double* qs, hipStream_t s);
 

@@ -302,46 +302,54 @@ void select_flagged(void* tmp, size_t tmpb, const unsigned char* flags, int n, i
 }
 
 // ---------------------------------------------------------------------------
-// point -> leaf of the final tree, by walking the key digits down the boxes
+// First point of every leaf of the final tree.  The points are sorted by
+// full-depth Morton key and the leaves, in rank (Morton) order, tile the root
+// box, so leaf r starts at the first point whose key is >= the leaf's
+// smallest key, leafkey[r] (its prefix shifted to full depth).  One wavefront
+// per leaf narrows [lo, hi] 64-fold per step: 64 pivots, one ballot.
 // ---------------------------------------------------------------------------
 template <class K>
-__global__ __launch_bounds__(256) void k_descend_rank(const K* __restrict__ key,
-                                                      const unsigned* __restrict__ idx, int n,
-                                                      int nlmax, const int4* __restrict__ child,
-                                                      const int* __restrict__ rank_of_box,
-                                                      unsigned* __restrict__ rank_out)
+__global__ __launch_bounds__(256) void k_leafstart_search(const K* __restrict__ key, int n,
+                                                          const K* __restrict__ leafkey,
+                                                          int nleaves, int* __restrict__ ls)
 {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    K k = key[i];
-    int b = 0;
-    for (int l = 0; l < nlmax; ++l) {
-        int4 c = child[b];
-        if (c.x < 0) break;
-        int d = (int)((k >> (2 * (nlmax - 1 - l))) & 3);
-        b = d == 0 ? c.x : d == 1 ? c.y : d == 2 ? c.z : c.w;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (r > nleaves) return;  // uniform over the wavefront
+    if (r == nleaves) {
+        if (lane == 0) ls[r] = n;
+        return;
     }
-    unsigned r = (unsigned)rank_of_box[b];
-    if (idx)
-        rank_out[idx[i]] = r;
-    else
-        rank_out[i] = r;
+    const K tgt = leafkey[r];
+    // the answer, the first i with i == n or key[i] >= tgt, lies in [lo, hi];
+    // a pivot at or past hi counts as >= tgt and is not read
+    long lo = 0, hi = n;
+    for (;;) {
+        const long stride = hi - lo > 64 ? (hi - lo + 63) / 64 : 1;
+        const long p = lo + lane * stride;
+        const bool ge = p >= hi || key[p] >= tgt;
+        const unsigned long long m = __ballot(ge);
+        const int f = m ? __builtin_ctzll(m) : 64;
+        if (stride == 1 || f == 0) {
+            lo += (long)f * stride;  // stride 1: f <= hi - lo, as pivot hi - lo is >= tgt
+            break;
+        }
+        const long nlo = lo + (long)(f - 1) * stride + 1;
+        if (f < 64 && lo + (long)f * stride < hi) hi = lo + (long)f * stride;
+        lo = nlo;
+    }
+    if (lane == 0) ls[r] = (int)lo;
 }
 
-void launch_descend_rank(const u64* key, const unsigned* idx, int n, int nlmax, const int4* child,
-                         const int* rank_of_box, unsigned* rank_out, hipStream_t s)
+void launch_leafstart_search(const u64* key, int n, const u64* leafkey, int nleaves, int* ls,
+                             hipStream_t s)
 {
-    if (n <= 0) return;
-    k_descend_rank<u64><<<grid_for(n, 256), 256, 0, s>>>(key, idx, n, nlmax, child, rank_of_box,
-                                                         rank_out);
+    k_leafstart_search<u64><<<(nleaves + 1 + 3) / 4, 256, 0, s>>>(key, n, leafkey, nleaves, ls);
 }
-void launch_descend_rank(const unsigned* key, const unsigned* idx, int n, int nlmax,
-                         const int4* child, const int* rank_of_box, unsigned* rank_out,
-                         hipStream_t s)
+void launch_leafstart_search(const unsigned* key, int n, const unsigned* leafkey, int nleaves,
+                             int* ls, hipStream_t s)
 {
-    if (n <= 0) return;
-    k_descend_rank<unsigned><<<grid_for(n, 256), 256, 0, s>>>(key, idx, n, nlmax, child,
-                                                              rank_of_box, rank_out);
+    k_leafstart_search<unsigned><<<(nleaves + 1 + 3) / 4, 256, 0, s>>>(key, n, leafkey, nleaves,
+                                                                       ls);
 }
 
 __global__ __launch_bounds__(256) void k_iota(unsigned* __restrict__ v, int n)
@@ -354,23 +362,6 @@ void launch_iota(unsigned* v, int n, hipStream_t s)
 {
     if (n <= 0) return;
     k_iota<<<grid_for(n, 256), 256, 0, s>>>(v, n);
-}
-
-// leafstart[r] = first sorted position whose leaf rank is >= r, r in [0, nleaves]
-__global__ __launch_bounds__(256) void k_leafstart(const unsigned* __restrict__ rk, int n,
-                                                   int nleaves, int* __restrict__ ls)
-{
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i > n) return;
-    int prev = i == 0 ? -1 : (int)rk[i - 1];
-    int cur = i == n ? nleaves : (int)rk[i];
-    for (int r = prev + 1; r <= cur; ++r) ls[r] = i;
-}
-
-void launch_leafstart(const unsigned* rank_sorted, int n, int nleaves, int* leafstart,
-                      hipStream_t s)
-{
-    k_leafstart<<<grid_for(n + 1, 256), 256, 0, s>>>(rank_sorted, n, nleaves, leafstart);
 }
 
 __global__ __launch_bounds__(256) void k_box_ranges(const int* __restrict__ lo,

@@ -160,39 +160,6 @@ void launch_keys3d(const double* p, int n, const double* c0, double bs0, int nlm
                                                      idx);
 }
 
-// leaf rank of each point by walking its key digits down the final octree
-template <class K>
-__global__ __launch_bounds__(256) void k_descend_rank3(const K* __restrict__ key, int n,
-                                                       int nlmax, const int* __restrict__ child,
-                                                       const int* __restrict__ rank_of_box,
-                                                       unsigned* __restrict__ rank_out)
-{
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    K k = key[i];
-    int b = 0;
-    for (int l = 0; l < nlmax; ++l) {
-        int c0 = child[8 * b];
-        if (c0 < 0) break;
-        int d = (int)((k >> (3 * (nlmax - 1 - l))) & 7);
-        b = child[8 * b + d];
-    }
-    rank_out[i] = (unsigned)rank_of_box[b];
-}
-
-void launch_descend_rank3(const u64* key, int n, int nlmax, const int* child8,
-                          const int* rank_of_box, unsigned* rank_out, hipStream_t s)
-{
-    if (n <= 0) return;
-    k_descend_rank3<u64><<<gridn(n, 256), 256, 0, s>>>(key, n, nlmax, child8, rank_of_box, rank_out);
-}
-void launch_descend_rank3(const unsigned* key, int n, int nlmax, const int* child8,
-                          const int* rank_of_box, unsigned* rank_out, hipStream_t s)
-{
-    if (n <= 0) return;
-    k_descend_rank3<unsigned><<<gridn(n, 256), 256, 0, s>>>(key, n, nlmax, child8, rank_of_box, rank_out);
-}
-
 __global__ __launch_bounds__(256) void k_gather3(const double* __restrict__ p,
                                                  const double* __restrict__ q, int nd,
                                                  const unsigned* __restrict__ perm, int n,
