@@ -64,6 +64,27 @@ def test_octree_matches_oracle(case):
         assert np.array_equal(g["col"][b, :n], t["col"][b, :n])
 
 
+def test_deeper_octree_after_a_shallow_one():
+    """As the 2D test: a shallow octree, one at least two levels deeper
+    (clustered sources), the shallow one again, through one workspace."""
+    cases = [("uniform", 2000, 500, 1e-2, 1e-6), ("cluster", 20000, 4000, 1e-5, 1e-9),
+             ("uniform", 2000, 500, 1e-2, 1e-6)]
+    depth = []
+    for recipe, ns, nt, delta, eps in cases:
+        print("case", recipe, ns, nt, delta, eps, flush=True)
+        src, chg, tg = _pts.dlaran_points3d(recipe, ns, nt, 1)
+        got = fgt2d_amd.fgt3d(delta, eps, src, chg, tg, ifpgh=1, ifpghtarg=1)
+        want = O.oracle_fgt3d(delta, eps, src, chg, tg, ifpgh=1, ifpghtarg=1)
+        assert _rel(got[0], want[0]) < 1e-11 and _rel(got[3], want[3]) < 1e-11
+        g = fgt2d_amd.last_tree3d(ns, nt)
+        t = O.oracle_tree3d(delta, eps, src, tg)
+        assert g["nboxes"] == t["nboxes"] and g["nlevels"] == t["nlevels"]
+        for k in ("laddr", "lev", "par", "nch", "ch", "isrc", "isrcse", "itarg", "itargse"):
+            assert np.array_equal(g[k], t[k]), k
+        depth.append(t["nlevels"])
+    assert depth[1] >= depth[0] + 2, depth
+
+
 GOLD3 = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
                if "3d" in os.path.basename(p))
 

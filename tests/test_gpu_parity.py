@@ -64,6 +64,33 @@ def test_tree_matches_oracle(case):
     assert np.array_equal(g["cen"], t["cen"])
 
 
+def test_deeper_tree_after_a_shallow_one():
+    """Calls of different depth through one workspace (buffers, plan storage
+    and leaf-record sizing kept from call to call): a shallow uniform
+    transform, one several levels deeper (clustered, small delta), then the
+    shallow one again, each against the oracle, trees and permutations
+    exactly."""
+    cases = [("uniform", 3000, 1000, 1e-4, 1e-6), ("mixture", 40000, 10000, 3e-6, 1e-3),
+             ("uniform", 3000, 1000, 1e-4, 1e-6)]
+    depth = []
+    for kind, ns, nt, delta, eps in cases:
+        print("case", kind, ns, nt, delta, eps, flush=True)
+        rng = np.random.default_rng(ns + 7)
+        src = _points(kind, rng, ns)
+        tg = _points(kind, rng, nt)
+        chg = rng.normal(size=(ns, 1))
+        got = fgt2d_amd.fgt2d(delta, eps, src, chg, tg, ifpgh=1, ifpghtarg=1)
+        want = O.oracle_fgt2d(delta, eps, src, chg, tg, ifpgh=1, ifpghtarg=1)
+        assert _rel(got[0], want[0]) < 1e-11 and _rel(got[3], want[3]) < 1e-11
+        g = fgt2d_amd.last_tree2d(ns, nt)
+        t = O.oracle_tree2d(delta, eps, src, tg)
+        assert g["nboxes"] == t["nboxes"] and g["nlevels"] == t["nlevels"]
+        for k in ("laddr", "lev", "par", "nch", "ch", "isrc", "isrcse", "itarg", "itargse"):
+            assert np.array_equal(g[k], t[k]), k
+        depth.append(t["nlevels"])
+    assert depth[1] >= depth[0] + 2, depth
+
+
 def test_accuracy_against_direct_sum():
     """The requested precision against the FP64 direct sum with no cutoff
     (the reference test's oracle, test_fgt2d.f:161-188)."""
