@@ -1303,10 +1303,10 @@ __global__ __launch_bounds__(128, 2) void k_pw2local3_wave2(Eval3Args a)
 //   the slab PW(., ., c) -> LDS P (a padded to 4 KA, b to 16 NB, zeros);
 //   S1(j3, b) = sum_a D(a, j3) P(a, b): waves w < 2 NB, tile (j3: w % 2,
 //     b: w / 2), into LDS;
-//   S2(j3, j2) = sum_b S1(j3, b) D(b, j2): wave w, tile (j3: w % 2,
-//     j2: (w / 2) % 2) (waves w and w + 4 compute the same tile);
-//   L(j3, j2, j1) += 2 Re(D(c, j1) S2(j3, j2)) for the wave's half of j1
-//     (w / 4), in registers.
+//   S2(j3, j2) = sum_b S1(j3, b) D(b, j2): waves 4..7, tile (j3: w % 2,
+//     j2: (w / 2) % 2), into LDS (times i^(j2 + j3), see below);
+//   L(j3, j2, j1) += 2 Re(D(c, j1) S2(j3, j2)) for the wave's tile w % 4
+//     and half of j1 (w / 4), in registers.
 // The products run on the real d(a, j) of D = i^j d, as in
 // k_pw2local3_wave2; d sits in LDS for every a < 4 KA, j < 32 (zero outside
 // a < npw, j < m).  Writes the transposed layout Lt[j2 + 32 (j3 + m j1)].
@@ -1318,6 +1318,7 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
     double* P = sm;                     // complex (A4 a) x (BP b) at a + PS b
     double* S1 = P + 2 * PS * BP;       // complex (32 j3) x (BP b) at j3 + 33 b
     double* Dt = S1 + 2 * 33 * BP;      // real d(a, j) at j + 32 a, a < A4
+    double* S2b = Dt + A4 * 32;         // 4 complex 16 x 16 tiles of i^q S2', lane order
     const int m = a.nl + 1, npw = a.npw, half = npw / 2, t = threadIdx.x;
     const int w = t >> 6, lane = t & 63, i16 = lane & 15, k4 = lane >> 4;
     const int mt = w & 1, jt = (w >> 1) & 1, jh = w >> 2;
@@ -1396,19 +1397,26 @@ __global__ __launch_bounds__(512) void k_pw2local3_blk(Eval3Args a)
                 }
             }
             __syncthreads();
-            d4_t3 s2r = {0, 0, 0, 0}, s2i = {0, 0, 0, 0};
+            double* S2t = S2b + 2 * 256 * (w & 3);  // this wave's tile (mt, jt)
+            if (w >= 4) {
+                d4_t3 s2r = {0, 0, 0, 0}, s2i = {0, 0, 0, 0};
 #pragma unroll 2
-            for (int ks = 0; ks < KA; ++ks) {
-                const int o = (16 * mt + i16) + 33 * (4 * ks + k4);
-                const double xr = S1[2 * o], xi = S1[2 * o + 1];
-                const double d = Dt[(16 * jt + i16) + 32 * (4 * ks + k4)];
-                s2r = mfma3(xr, d, s2r);
-                s2i = mfma3(xi, d, s2i);
+                for (int ks = 0; ks < KA; ++ks) {
+                    const int o = (16 * mt + i16) + 33 * (4 * ks + k4);
+                    const double xr = S1[2 * o], xi = S1[2 * o + 1];
+                    const double d = Dt[(16 * jt + i16) + 32 * (4 * ks + k4)];
+                    s2r = mfma3(xr, d, s2r);
+                    s2i = mfma3(xi, d, s2i);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    *(double2*)(S2t + 2 * (64 * r + lane)) = rot_ij(s2r[r], s2i[r], q);
             }
-            double zr[4], zi[4];  // i^q S2'
+            __syncthreads();
+            double zr[4], zi[4];  // i^q S2' of this lane's elements
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const double2 z = rot_ij(s2r[r], s2i[r], q);
+                const double2 z = *(const double2*)(S2t + 2 * (64 * r + lane));
                 zr[r] = z.x;
                 zi[r] = z.y;
             }
@@ -1438,7 +1446,7 @@ template <int KA, int NB>
 static void pw2local3_blk_launch(const Eval3Args& a, hipStream_t s)
 {
     const size_t lds = ((size_t)2 * (4 * KA + 1) * 16 * NB + (size_t)2 * 33 * 16 * NB +
-                        (size_t)4 * KA * 32) * sizeof(double);
+                        (size_t)4 * KA * 32 + (size_t)2 * 4 * 256) * sizeof(double);
     allow_lds3(k_pw2local3_blk<KA, NB>, lds);
     k_pw2local3_blk<KA, NB><<<std::min(a.nbox, 8192), 512, lds, s>>>(a);
 }
