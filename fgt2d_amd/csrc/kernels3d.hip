@@ -397,8 +397,8 @@ __global__ __launch_bounds__(256) void k_form3_hermite(Form3Args a)
 // Hermite-route formation on matrix cores (m = p + 1 <= 32, npw <= 16 NB),
 // one block of 8 wavefronts per leaf, in four contractions:
 //   1. M(j, k, l) = sum_s x^j W_s(k, l), W = q y^k z^l (+ dipole terms): rows
-//      j, columns (k, l) at k + KP l, sources as K; to per-block scratch
-//      M[(k + KP l) + KP m j];
+//      j, columns (k, l) at k + m l (KLM = m^2 rounded up to 16 columns),
+//      sources as K; to per-block scratch M[(k + m l) + KLM j];
 //   2. N(j, k, c) = sum_l M(j, k, l) C(l, c), computed transposed (rows c,
 //      columns (j, k) at k + KP j) into scratch N[(k + KP j) + KP m c];
 //   3. per z node c: P_c(a, k) = sum_j C(j, a) N(j, k, c), into LDS;
@@ -419,7 +419,8 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
     const int w = t >> 6, lane = t & 63, i16 = lane & 15, k4 = lane >> 4;
     const int MT = (m + 15) / 16, KP = 16 * MT, KS = (m + 3) / 4, PK = KP + 1;
     const int KSE = ((m + 1) / 2 + 3) / 4, JR = 8 * KSE;  // steps over the even (odd) orders
-    const int KLP = KP * m;           // columns (k, l) of M, rows (j, k) of N
+    const int KLP = KP * m;           // rows (j, k) of N, k padded to KP
+    const int KLM = 16 * ((m * m + 15) / 16);  // columns (k, l) of M, packed
     const int NC = (half + 15) / 16;
     const long nexp = (long)npw * npw * half;
     double* Mg = a.scratch + (long)blockIdx.x * a.sstride;  // KP rows j x KLP
@@ -451,7 +452,7 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
             double* yd = xd + 32 * KP;
             double* zd = yd + 32 * KP;
             double* sw = zd + 32 * KP;         // dipoles: q, dsq w1, dsq w2, dsq w3
-            const int ntile = MT * (KLP / 16);
+            const int ntile = MT * (KLM / 16);
             constexpr int TPW = 4;  // tiles per wave and pass
             for (int tp = 0; tp < ntile; tp += 8 * TPW) {
                 d4_t3 acc[TPW];
@@ -511,7 +512,8 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                         const int tile = tp + w + 8 * i;
                         if (tile >= ntile) break;
                         const int mt = tile % MT, nk = tile / MT;
-                        const int j = 16 * mt + i16, kl = 16 * nk + i16, k = kl % KP, l = kl / KP;
+                        // columns past m^2 read l >= m: zero rows of the tables
+                        const int j = 16 * mt + i16, kl = 16 * nk + i16, k = kl % m, l = kl / m;
                         for (int ks = 0; ks < nks; ++ks) {
                             const int sa = 4 * ks + k4;  // this lane's source in the chunk
                             const double av = xp[sa * KP + j];
@@ -537,7 +539,7 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                     const int mt = tile % MT, nk = tile / MT;
 #pragma unroll
                     for (int rr = 0; rr < 4; ++rr)
-                        Mg[(16 * nk + i16) + (long)KLP * (16 * mt + k4 + 4 * rr)] = acc[i][rr];
+                        Mg[(16 * nk + i16) + (long)KLM * (16 * mt + k4 + 4 * rr)] = acc[i][rr];
                 }
             }
         }
@@ -556,8 +558,8 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                     const double sg = (e & 1) ? -1.0 : 1.0;
                     const double ce = sg * Rt[(16 * nc + i16) + CW * le];
                     const double co = sg * Rt[(16 * nc + i16) + CW * lo];
-                    const double me = le < m ? Mg[(k + KP * le) + (long)KLP * j] : 0.0;
-                    const double mo = lo < m ? Mg[(k + KP * lo) + (long)KLP * j] : 0.0;
+                    const double me = le < m && k < m ? Mg[(k + m * le) + (long)KLM * j] : 0.0;
+                    const double mo = lo < m && k < m ? Mg[(k + m * lo) + (long)KLM * j] : 0.0;
                     nr_ = mfma3(ce, me, nr_);
                     ni_ = mfma3(co, mo, ni_);
                 }
