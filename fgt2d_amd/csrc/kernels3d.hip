@@ -400,7 +400,7 @@ __global__ __launch_bounds__(256) void k_form3_hermite(Form3Args a)
 //      j, columns (k, l) at k + m l (KLM = m^2 rounded up to 16 columns),
 //      sources as K; to per-block scratch M[(k + m l) + KLM j];
 //   2. N(j, k, c) = sum_l M(j, k, l) C(l, c), computed transposed (rows c,
-//      columns (j, k) at k + KP j) into scratch N[(k + KP j) + KP m c];
+//      columns (j, k) at k + m j, KLM of them) into scratch N[(k + m j) + KLM c];
 //   3. per z node c: P_c(a, k) = sum_j C(j, a) N(j, k, c), into LDS;
 //   4. PW(a, b, c) = sum_k C(k, b) P_c(a, k), computed transposed (rows b,
 //      columns a) so that the store runs along a.
@@ -419,8 +419,8 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
     const int w = t >> 6, lane = t & 63, i16 = lane & 15, k4 = lane >> 4;
     const int MT = (m + 15) / 16, KP = 16 * MT, KS = (m + 3) / 4, PK = KP + 1;
     const int KSE = ((m + 1) / 2 + 3) / 4, JR = 8 * KSE;  // steps over the even (odd) orders
-    const int KLP = KP * m;           // rows (j, k) of N, k padded to KP
-    const int KLM = 16 * ((m * m + 15) / 16);  // columns (k, l) of M, packed
+    const int KLP = KP * m;           // scratch sizing (KLM <= KLP)
+    const int KLM = 16 * ((m * m + 15) / 16);  // columns (k, l) of M, rows (j, k) of N, packed
     const int NC = (half + 15) / 16;
     const long nexp = (long)npw * npw * half;
     double* Mg = a.scratch + (long)blockIdx.x * a.sstride;  // KP rows j x KLP
@@ -548,18 +548,18 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
         // Re N = sum over even l = 2e of (-1)^e r(c, l) M, Im N over odd
         // l = 2e + 1 of (-1)^e r(c, l) M ----
         {
-            const int ntile = NC * (KLP / 16);
+            const int ntile = NC * (KLM / 16);
             for (int tile = w; tile < ntile; tile += 8) {
                 const int nc = tile % NC, nr = tile / NC;
-                const int rho = 16 * nr + i16, j = rho / KP, k = rho % KP;
+                const int rho = 16 * nr + i16, j = rho / m, k = rho % m;  // rho >= m^2: j >= m
                 d4_t3 nr_ = {0, 0, 0, 0}, ni_ = {0, 0, 0, 0};
                 for (int ks = 0; ks < KSE; ++ks) {
                     const int e = 4 * ks + k4, le = 2 * e, lo = 2 * e + 1;
                     const double sg = (e & 1) ? -1.0 : 1.0;
                     const double ce = sg * Rt[(16 * nc + i16) + CW * le];
                     const double co = sg * Rt[(16 * nc + i16) + CW * lo];
-                    const double me = le < m && k < m ? Mg[(k + m * le) + (long)KLM * j] : 0.0;
-                    const double mo = lo < m && k < m ? Mg[(k + m * lo) + (long)KLM * j] : 0.0;
+                    const double me = le < m && j < m ? Mg[(k + m * le) + (long)KLM * j] : 0.0;
+                    const double mo = lo < m && j < m ? Mg[(k + m * lo) + (long)KLM * j] : 0.0;
                     nr_ = mfma3(ce, me, nr_);
                     ni_ = mfma3(co, mo, ni_);
                 }
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                 for (int rr = 0; rr < 4; ++rr) {
                     const int c = 16 * nc + k4 + 4 * rr;
                     if (c < half) {
-                        double* d = Ng + 2 * ((16 * nr + i16) + (long)KLP * c);
+                        double* d = Ng + 2 * ((16 * nr + i16) + (long)KLM * c);
                         const double2 v = rot_ij(nr_[rr], ni_[rr], j);
                         d[0] = v.x;
                         d[1] = v.y;
@@ -586,14 +586,14 @@ __global__ __launch_bounds__(512) void k_form3_mfma(Form3Args a)
                 const int c = c0 + g;
                 if (c >= half) continue;
                 double* Pc = work + (par * G + g) * 2 * PK * 16 * NB;
-                const double* Nc = Ng + 2L * KLP * c;
+                const double* Nc = Ng + 2L * KLM * c;
                 d4_t3 pr = {0, 0, 0, 0}, pi = {0, 0, 0, 0};
                 for (int ks = 0; ks < KS; ++ks) {
                     const int j = 4 * ks + k4;
                     const double cr = Rt[(16 * na + i16) + CW * j];
                     double nvr = 0, nvi = 0;
-                    if (j < m) {
-                        const double* nv = Nc + 2 * ((16 * mt + i16) + KP * j);
+                    if (j < m && 16 * mt + i16 < m) {
+                        const double* nv = Nc + 2 * ((16 * mt + i16) + m * j);
                         nvr = nv[0];
                         nvi = nv[1];
                     }
