@@ -51,6 +51,15 @@ __device__ __forceinline__ void wave_sync3()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// Block barrier that orders LDS only: global loads in flight (prefetches)
+// are not waited for, as __syncthreads' full fence would.
+__device__ __forceinline__ void lds_barrier3()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // grid of a grid-stride kernel over n items: at most 2^20 blocks
 static inline unsigned gridcap(long n, int bs)
 {
@@ -1216,17 +1225,21 @@ __global__ __launch_bounds__(128, 2) void k_pw2local3_wave2(Eval3Args a)
         for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int j = 0; j < 8; ++j) L[r][j] = 0;
-        for (int c = 0; c < half; ++c) {
-            __syncthreads();  // readers of P and S1 of the previous node are done
+        // the slab of node c + 1 is loaded while node c is computed
+        double2 v[8];
+        auto load_slab = [&](int c) {
             const double* slab = pw + 2L * npw * npw * c;
-            {
-                double2 v[8];
 #pragma unroll
-                for (int it = 0; it < 8; ++it) {
-                    const int o = t + 128 * it, aa = o % PW3_N, b = o / PW3_N;
-                    v[it] = aa < npw && b < npw ? *(const double2*)(slab + 2 * (aa + npw * b))
-                                                : make_double2(0.0, 0.0);
-                }
+            for (int it = 0; it < 8; ++it) {
+                const int o = t + 128 * it, aa = o % PW3_N, b = o / PW3_N;
+                v[it] = aa < npw && b < npw ? *(const double2*)(slab + 2 * (aa + npw * b))
+                                            : make_double2(0.0, 0.0);
+            }
+        };
+        if (half > 0) load_slab(0);
+        for (int c = 0; c < half; ++c) {
+            lds_barrier3();  // readers of P and S1 of the previous node are done
+            {
                 if (wk) {
                     const double* ws = wk + 2L * npw * npw * c;
 #pragma unroll
@@ -1244,12 +1257,13 @@ __global__ __launch_bounds__(128, 2) void k_pw2local3_wave2(Eval3Args a)
                     const int o = t + 128 * it, aa = o % PW3_N, b = o / PW3_N;
                     *(double2*)(P + 2 * (aa + PW3_S * b)) = v[it];
                 }
+                if (c + 1 < half) load_slab(c + 1);
             }
-            __syncthreads();
+            lds_barrier3();
             {
                 // S1 column tile nt = w
                 d4_t3 s1r = {0, 0, 0, 0}, s1i = {0, 0, 0, 0};
-#pragma unroll
+#pragma unroll 2
                 for (int ks = 0; ks < PW3_N / 4; ++ks) {
                     const int o = (4 * ks + k4) + PW3_S * (16 * w + i16);
                     const double pr = P[2 * o], pi = P[2 * o + 1];
@@ -1264,9 +1278,9 @@ __global__ __launch_bounds__(128, 2) void k_pw2local3_wave2(Eval3Args a)
                     S1[2 * o + 1] = s1i[r];
                 }
             }
-            __syncthreads();
+            lds_barrier3();
             d4_t3 s2r = {0, 0, 0, 0}, s2i = {0, 0, 0, 0};
-#pragma unroll
+#pragma unroll 2
             for (int ks = 0; ks < PW3_N / 4; ++ks) {
                 const int o = i16 + 17 * (4 * ks + k4);
                 const double xr = S1[2 * o], xi = S1[2 * o + 1];
