@@ -861,8 +861,10 @@ __global__ __launch_bounds__(256) void k_merge3(const int* __restrict__ parents,
     const double2* w2 = reinterpret_cast<const double2*>(wms);
     double2* m2 = reinterpret_cast<double2*>(mp);
     for (long q = blockIdx.x; q < (long)np * nchunk; q += gridDim.x) {
-        const int p = parents[q % np];
-        const long e0 = (q / np) * SH3_E + threadIdx.x;
+        // parent-major: a parent's children share nothing with other
+        // parents', so their expansions stream chunk after chunk
+        const int p = parents[q / nchunk];
+        const long e0 = (q % nchunk) * SH3_E + threadIdx.x;
         for (int ind = 0; ind < nd; ++ind) {
             double re[SH3_V], im[SH3_V];
 #pragma unroll
