@@ -77,7 +77,7 @@ struct Tables3D {
     int npw = 0, ntermmax = 0, nmax = 0;
     double pmax = 0, xmin2 = 0, xm = 0;
     const double *ws = nullptr, *ts = nullptr, *coef = nullptr, *dcoef = nullptr, *wms = nullptr,
-                 *wm2l = nullptr;
+                 *wm1 = nullptr;  // wm1: complex (3, npw) m2l factors per axis
 };
 
 struct Workspace {
@@ -558,7 +558,7 @@ const Tables3D& tables3d(Workspace& W, int npw, double pmax, int ntermmax, int n
         }
     }
     const int nm = std::max(nmax, 1);
-    std::vector<double> wms(2 * nexp * 8 * nm), wm2l(2 * nexp * 27);
+    std::vector<double> wms(2 * nexp * 8 * nm), wm1(2 * 3 * (size_t)npw);
     {
         std::vector<double> wr((size_t)npw * (nm + 1)), wi((size_t)npw * (nm + 1));
         for (int j = 0; j < npw; ++j) {
@@ -591,33 +591,17 @@ const Tables3D& tables3d(Workspace& W, int npw, double pmax, int ntermmax, int n
                         }
                     }
     }
-    {
-        std::vector<double> pr(npw * 3), pi(npw * 3);
-        for (int j = 0; j < npw; ++j) {
-            const double zr = std::cos(ts[j] * xm), zi = std::sin(ts[j] * xm);
-            pr[3 * j + 0] = zr; pi[3 * j + 0] = -zi;   // offset -1
-            pr[3 * j + 1] = 1;  pi[3 * j + 1] = 0;     // offset 0
-            pr[3 * j + 2] = zr; pi[3 * j + 2] = zi;    // offset +1
-        }
-        for (int kz = 0; kz < 3; ++kz)
-            for (int ky = 0; ky < 3; ++ky)
-                for (int kx = 0; kx < 3; ++kx)
-                    for (int c = 0; c < half; ++c)
-                        for (int b = 0; b < npw; ++b)
-                            for (int a = 0; a < npw; ++a) {
-                                const size_t e = a + (size_t)npw * (b + (size_t)npw * c);
-                                const double ar = pr[3 * a + kx], ai = pi[3 * a + kx];
-                                const double br = pr[3 * b + ky], bi = pi[3 * b + ky];
-                                const double cr = pr[3 * c + kz], ci = pi[3 * c + kz];
-                                const double tr = ar * br - ai * bi, ti = ar * bi + ai * br;
-                                double* o = &wm2l[2 * ((size_t)(kx + 3 * ky + 9 * kz) * nexp + e)];
-                                o[0] = tr * cr - ti * ci;
-                                o[1] = tr * ci + ti * cr;
-                            }
+    // m2l factors e^{i ts_a k xm} for the offsets k = -1, 0, +1 at 3 a + k + 1;
+    // the kernel multiplies the x, y and z factors of a shift
+    for (int j = 0; j < npw; ++j) {
+        const double zr = std::cos(ts[j] * xm), zi = std::sin(ts[j] * xm);
+        wm1[2 * (3 * j + 0)] = zr; wm1[2 * (3 * j + 0) + 1] = -zi;  // offset -1
+        wm1[2 * (3 * j + 1)] = 1;  wm1[2 * (3 * j + 1) + 1] = 0;    // offset 0
+        wm1[2 * (3 * j + 2)] = zr; wm1[2 * (3 * j + 2) + 1] = zi;   // offset +1
     }
     Pack pk;
     size_t o_ws = pk.add(ws), o_ts = pk.add(ts), o_c = pk.add(coef), o_d = pk.add(dcoef);
-    size_t o_ms = pk.add(wms), o_ml = pk.add(wm2l);
+    size_t o_ms = pk.add(wms), o_ml = pk.add(wm1);
     char* d = W.get<char>("tables3", pk.size());
     HIPCHECK(hipMemcpyAsync(d, pk.data(), pk.size(), hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));  // pk is freed on return
@@ -626,7 +610,7 @@ const Tables3D& tables3d(Workspace& W, int npw, double pmax, int ntermmax, int n
     t.coef = (const double*)(d + o_c);
     t.dcoef = (const double*)(d + o_d);
     t.wms = (const double*)(d + o_ms);
-    t.wm2l = (const double*)(d + o_ml);
+    t.wm1 = (const double*)(d + o_ml);
     t.valid = true;
     t.npw = npw;
     t.pmax = pmax;
@@ -2075,7 +2059,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         tm.begin(S_M2L);
         tm.ctx(nlevstart, &m2l_l, 2);
         launch_m2l3((const int*)D(o_m2l), (int)m2l_l.size(), d_cen, d_ncol, d_col, d_nch, src_se,
-                    trg_se, use_src, use_trg, d_smp, d_sloc, tab.wm2l, T.bsz[nlevstart],
+                    trg_se, use_src, use_trg, d_smp, d_sloc, tab.wm1, npw, T.bsz[nlevstart],
                     nlevstart == npwlevel, nd, nexp, mp, lc, s);
         tm.end(S_M2L);
         // steps 4-5 by level: non-leaf children get their persistent local

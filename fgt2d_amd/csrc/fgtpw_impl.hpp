@@ -295,7 +295,7 @@ void launch_merge3(const int* parents, int np, const int* child8, const double* 
                    long nexp, double* mp, hipStream_t s);
 void launch_m2l3(const int* boxes, int nb, const double* cen, const int* ncol, const int* col27,
                  const int* nch, const int2* src_se, const int2* trg_se, int use_src, int use_trg,
-                 const int* slot_mp, const int* slot_loc, const double* wm2l, double xmin,
+                 const int* slot_mp, const int* slot_loc, const double* wm1, int npw, double xmin,
                  int do_list, int nd, long nexp, const double* mp, double* loc, hipStream_t s);
 void launch_split3(const int* children, int nc, const int* par, const double* cen,
                    const int* slot, const double* wms, int kbase, int nd, long nexp, double* loc,

@@ -915,7 +915,10 @@ void launch_merge3(const int* parents, int np, const int* child8, const double* 
 // Step 3 (fgt3d.f:1162-1210): at the cutoff level, loc = sum over colleagues
 // j != self that have children (gt3d_computelistpw, tree_routs3d.f:695) of
 // mp_j shifted by (jx, jy, jz) boxes, then + mp_self; boxes without
-// evaluation points get a zero expansion.  Multipole and local expansions
+// evaluation points get a zero expansion.  The shift factor of coefficient
+// (a, b, c) is the product of the per-axis factors wm1(a, jx) wm1(b, jy)
+// wm1(c, jz), formed from an LDS copy of wm1: only the colleagues'
+// expansions are read from memory.  Multipole and local expansions
 // have separate slot tables; a box without a multipole slot (slot_mp < 0)
 // has a zero multipole expansion.
 __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int nb,
@@ -927,13 +930,15 @@ __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int
                                               const int2* __restrict__ trg_se, int use_src,
                                               int use_trg, const int* __restrict__ slot_mp,
                                               const int* __restrict__ slot_loc,
-                                              const double* __restrict__ wm2l, double xmin,
-                                              int do_list, int nd, long nexp,
+                                              const double* __restrict__ wm1, int npw,
+                                              double xmin, int do_list, int nd, long nexp,
                                               const double* __restrict__ mp,
                                               double* __restrict__ loc)
 {
+    extern __shared__ double2 f1[];  // complex (3, npw): offset -1, 0, +1 at 3 a + k + 1
+    for (int o = threadIdx.x; o < 3 * npw; o += 256) f1[o] = reinterpret_cast<const double2*>(wm1)[o];
+    __syncthreads();
     const long nchunk = (nexp + SH3_E - 1) / SH3_E;
-    const double2* w2 = reinterpret_cast<const double2*>(wm2l);
     const double2* m2 = reinterpret_cast<const double2*>(mp);
     double2* l2 = reinterpret_cast<double2*>(loc);
     for (long q = blockIdx.x; q < (long)nb * nchunk; q += gridDim.x) {
@@ -942,6 +947,16 @@ __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int
         int npts = 0;
         if (use_trg) npts += trg_se[b].y - trg_se[b].x;
         if (use_src) npts += src_se[b].y - src_se[b].x;
+        // (a, b, c) of this thread's coefficients, as LDS row offsets
+        int ra[SH3_V], rb[SH3_V], rc[SH3_V];
+#pragma unroll
+        for (int v = 0; v < SH3_V; ++v) {
+            const long e = e0 + 256 * v;
+            const int ee = e < nexp ? (int)e : 0, bc = ee / npw;
+            ra[v] = 3 * (ee - bc * npw) + 1;
+            rb[v] = 3 * (bc % npw) + 1;
+            rc[v] = 3 * (bc / npw) + 1;
+        }
         for (int ind = 0; ind < nd; ++ind) {
             double re[SH3_V], im[SH3_V];
 #pragma unroll
@@ -953,19 +968,20 @@ __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int
                     const int jx = (int)rint((cen[3 * b] - cen[3 * j]) / xmin);
                     const int jy = (int)rint((cen[3 * b + 1] - cen[3 * j + 1]) / xmin);
                     const int jz = (int)rint((cen[3 * b + 2] - cen[3 * j + 2]) / xmin);
-                    const double2* w = w2 + (long)((jx + 1) + 3 * (jy + 1) + 9 * (jz + 1)) * nexp;
                     const double2* x = m2 + ((long)slot_mp[j] * nd + ind) * nexp;
-                    double2 xv[SH3_V], wv[SH3_V];
+                    double2 xv[SH3_V];
 #pragma unroll
                     for (int v = 0; v < SH3_V; ++v) {
                         const long e = e0 + 256 * v;
                         xv[v] = e < nexp ? x[e] : make_double2(0.0, 0.0);
-                        wv[v] = e < nexp ? w[e] : make_double2(0.0, 0.0);
                     }
 #pragma unroll
                     for (int v = 0; v < SH3_V; ++v) {
-                        re[v] += xv[v].x * wv[v].x - xv[v].y * wv[v].y;
-                        im[v] += xv[v].x * wv[v].y + xv[v].y * wv[v].x;
+                        const double2 fa = f1[ra[v] + jx], fb = f1[rb[v] + jy], fc = f1[rc[v] + jz];
+                        const double tr = fa.x * fb.x - fa.y * fb.y, ti = fa.x * fb.y + fa.y * fb.x;
+                        const double wr = tr * fc.x - ti * fc.y, wi = tr * fc.y + ti * fc.x;
+                        re[v] += xv[v].x * wr - xv[v].y * wi;
+                        im[v] += xv[v].x * wi + xv[v].y * wr;
                     }
                 }
             if (npts > 0 && slot_mp[b] >= 0) {
@@ -992,14 +1008,16 @@ __global__ __launch_bounds__(256) void k_m2l3(const int* __restrict__ boxes, int
 
 void launch_m2l3(const int* boxes, int nb, const double* cen, const int* ncol, const int* col27,
                  const int* nch, const int2* src_se, const int2* trg_se, int use_src, int use_trg,
-                 const int* slot_mp, const int* slot_loc, const double* wm2l, double xmin,
+                 const int* slot_mp, const int* slot_loc, const double* wm1, int npw, double xmin,
                  int do_list, int nd, long nexp, const double* mp, double* loc, hipStream_t s)
 {
     if (nb <= 0) return;
+    const size_t lds = (size_t)3 * npw * sizeof(double2);
+    allow_lds3(k_m2l3, lds);
     kmark(KID_M2L, false, s);
-    k_m2l3<<<gridcap((long)nb * ((nexp + SH3_E - 1) / SH3_E), 1), 256, 0, s>>>(
-        boxes, nb, cen, ncol, col27, nch, src_se, trg_se, use_src, use_trg, slot_mp, slot_loc, wm2l,
-        xmin, do_list, nd, nexp, mp, loc);
+    k_m2l3<<<gridcap((long)nb * ((nexp + SH3_E - 1) / SH3_E), 1), 256, lds, s>>>(
+        boxes, nb, cen, ncol, col27, nch, src_se, trg_se, use_src, use_trg, slot_mp, slot_loc, wm1,
+        npw, xmin, do_list, nd, nexp, mp, loc);
     kmark(KID_M2L, true, s);
 }
 
