@@ -44,13 +44,6 @@ __device__ __forceinline__ double2 rot_ij(double re, double im, int n)
     return (n & 2) ? make_double2(-x, -y) : make_double2(x, y);
 }
 
-__device__ __forceinline__ void wave_sync3()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
 // Block barrier that orders LDS only: global loads in flight (prefetches)
 // are not waited for, as __syncthreads' full fence would.
 __device__ __forceinline__ void lds_barrier3()
@@ -1211,7 +1204,9 @@ constexpr int PW3_S = 33;  // LDS stride of a slab column (bank-conflict-free la
 // shift when the split is fused) into LDS with padded strides, computes the
 // S1 column tile nt = w into a shared LDS buffer; then both waves form S2
 // (the same tile) and each keeps L for its half of j1 (8 orders) in
-// registers, so that two leaves fit per SIMD.  Lane l holds
+// registers, so that two leaves fit per SIMD.  The next z node's slab is
+// loaded while the current one is computed: the block barriers order LDS
+// only, so those loads stay in flight across them.  Lane l holds
 // L(j3 = l/16 + 4r, j2 = l%16, j1 = 8w + j).  Writes the transposed layout
 // Lt[j2 + 16 (j3 + m j1)]; columns j2 >= m hold the zeros computed for them.
 __global__ __launch_bounds__(128, 2) void k_pw2local3_wave2(Eval3Args a)
