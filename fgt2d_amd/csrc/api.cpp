@@ -980,24 +980,21 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     // first point of every leaf (a search of the sorted keys), then the
     // range of every box
     tm.begin(S_RANGES);
-    int* ls = W.get<int>("leafstart", nleaves + 1);
+    // sources and targets in one launch each
+    int* ls = W.get<int>("leafstart", 2 * ((size_t)nleaves + 1));
+    int* ls2 = ls + nleaves + 1;
     int2* src_se = W.get<int2>("src_se", nb);
     int2* trg_se = W.get<int2>("trg_se", nb);
+    const bool tg = nt > 0;
     if (k32)
-        launch_leafstart_search((const unsigned*)ks2, ns, (const unsigned*)d_lkey, nleaves, ls, s);
+        launch_leafstart_search((const unsigned*)ks2, ns, tg ? (const unsigned*)kt2 : nullptr, nt,
+                                (const unsigned*)d_lkey, nleaves, ls, ls2, s);
     else
-        launch_leafstart_search(ks2, ns, (const u64*)d_lkey, nleaves, ls, s);
-    launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, s);
+        launch_leafstart_search(ks2, ns, tg ? kt2 : nullptr, nt, (const u64*)d_lkey, nleaves, ls,
+                                ls2, s);
+    launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, ls2, tg ? trg_se : nullptr, s);
     tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
-    if (nt > 0) {
-        if (k32)
-            launch_leafstart_search((const unsigned*)kt2, nt, (const unsigned*)d_lkey, nleaves, ls, s);
-        else
-            launch_leafstart_search(kt2, nt, (const u64*)d_lkey, nleaves, ls, s);
-        launch_box_ranges(d_rlo, d_rhi, nb, ls, trg_se, s);
-    } else {
-        HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));
-    }
+    if (!tg) HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));
     tm.end(S_RANGES);
 
     // sorted outputs
@@ -1926,24 +1923,21 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
 
     // ---------------- point ranges of the boxes ----------------
     tm.begin(S_RANGES);
-    int* ls = W.get<int>("leafstart", nleaves + 1);
+    // sources and targets in one launch each
+    int* ls = W.get<int>("leafstart", 2 * ((size_t)nleaves + 1));
+    int* ls2 = ls + nleaves + 1;
     int2* src_se = W.get<int2>("src_se", nb);
     int2* trg_se = W.get<int2>("trg_se", nb);
+    const bool tg = nt > 0;
     if (k32)
-        launch_leafstart_search((const unsigned*)ks2, ns, (const unsigned*)d_lkey, nleaves, ls, s);
+        launch_leafstart_search((const unsigned*)ks2, ns, tg ? (const unsigned*)kt2 : nullptr, nt,
+                                (const unsigned*)d_lkey, nleaves, ls, ls2, s);
     else
-        launch_leafstart_search(ks2, ns, (const u64*)d_lkey, nleaves, ls, s);
-    launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, s);
+        launch_leafstart_search(ks2, ns, tg ? kt2 : nullptr, nt, (const u64*)d_lkey, nleaves, ls,
+                                ls2, s);
+    launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, ls2, tg ? trg_se : nullptr, s);
     tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
-    if (nt > 0) {
-        if (k32)
-            launch_leafstart_search((const unsigned*)kt2, nt, (const unsigned*)d_lkey, nleaves, ls, s);
-        else
-            launch_leafstart_search(kt2, nt, (const u64*)d_lkey, nleaves, ls, s);
-        launch_box_ranges(d_rlo, d_rhi, nb, ls, trg_se, s);
-    } else {
-        HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));
-    }
+    if (!tg) HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));
     tm.end(S_RANGES);
 
     const int ws_ = use_src ? ns : 0, wt_ = use_trg ? nt : 0;

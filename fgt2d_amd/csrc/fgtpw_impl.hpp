@@ -124,14 +124,18 @@ void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigne
                         const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits = 2);
 void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigned* key,
                         const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits = 2);
-// ls[r] = first of the n sorted keys >= leafkey[r], r < nleaves; ls[nleaves] = n
-void launch_leafstart_search(const unsigned long long* key, int n, const unsigned long long* leafkey,
-                             int nleaves, int* ls, hipStream_t s);
-void launch_leafstart_search(const unsigned* key, int n, const unsigned* leafkey, int nleaves,
-                             int* ls, hipStream_t s);
+// ls[r] = first of the n sorted keys >= leafkey[r], r < nleaves; ls[nleaves] = n;
+// the same into ls2 for key2 (n2 keys) when key2 is not null
+void launch_leafstart_search(const unsigned long long* key, int n, const unsigned long long* key2,
+                             int n2, const unsigned long long* leafkey, int nleaves, int* ls,
+                             int* ls2, hipStream_t s);
+void launch_leafstart_search(const unsigned* key, int n, const unsigned* key2, int n2,
+                             const unsigned* leafkey, int nleaves, int* ls, int* ls2,
+                             hipStream_t s);
 void launch_iota(unsigned* v, int n, hipStream_t s);
+// se[b] = (ls[rank_lo[b]], ls[rank_hi[b]]), and se2 from ls2 when se2 is not null
 void launch_box_ranges(const int* rank_lo, const int* rank_hi, int nb, const int* ls, int2* se,
-                       hipStream_t s);
+                       const int* ls2, int2* se2, hipStream_t s);
 void launch_gather(const double* xy, const double* q, int nd, const unsigned* perm, int n,
                    double* xy_s, double* q_s, hipStream_t s);
 void launch_zero(double* p, size_t n, hipStream_t s);

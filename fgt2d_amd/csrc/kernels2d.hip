@@ -306,13 +306,21 @@ void select_flagged(void* tmp, size_t tmpb, const unsigned char* flags, int n, i
 // full-depth Morton key and the leaves, in rank (Morton) order, tile the root
 // box, so leaf r starts at the first point whose key is >= the leaf's
 // smallest key, leafkey[r] (its prefix shifted to full depth).  One wavefront
-// per leaf narrows [lo, hi] 64-fold per step: 64 pivots, one ballot.
+// per leaf narrows [lo, hi] 64-fold per step: 64 pivots, one ballot.  Grid
+// row 1, when present, does the same for a second point set (the targets).
 // ---------------------------------------------------------------------------
 template <class K>
 __global__ __launch_bounds__(256) void k_leafstart_search(const K* __restrict__ key, int n,
+                                                          const K* __restrict__ key2, int n2,
                                                           const K* __restrict__ leafkey,
-                                                          int nleaves, int* __restrict__ ls)
+                                                          int nleaves, int* __restrict__ ls,
+                                                          int* __restrict__ ls2)
 {
+    if (blockIdx.y) {
+        key = key2;
+        n = n2;
+        ls = ls2;
+    }
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (r > nleaves) return;  // uniform over the wavefront
     if (r == nleaves) {
@@ -340,16 +348,23 @@ __global__ __launch_bounds__(256) void k_leafstart_search(const K* __restrict__ 
     if (lane == 0) ls[r] = (int)lo;
 }
 
-void launch_leafstart_search(const u64* key, int n, const u64* leafkey, int nleaves, int* ls,
+template <class K>
+static void leafstart_search_launch(const K* key, int n, const K* key2, int n2, const K* leafkey,
+                                    int nleaves, int* ls, int* ls2, hipStream_t s)
+{
+    const dim3 grid((nleaves + 1 + 3) / 4, key2 ? 2 : 1);
+    k_leafstart_search<K><<<grid, 256, 0, s>>>(key, n, key2, n2, leafkey, nleaves, ls, ls2);
+}
+void launch_leafstart_search(const u64* key, int n, const u64* key2, int n2, const u64* leafkey,
+                             int nleaves, int* ls, int* ls2, hipStream_t s)
+{
+    leafstart_search_launch(key, n, key2, n2, leafkey, nleaves, ls, ls2, s);
+}
+void launch_leafstart_search(const unsigned* key, int n, const unsigned* key2, int n2,
+                             const unsigned* leafkey, int nleaves, int* ls, int* ls2,
                              hipStream_t s)
 {
-    k_leafstart_search<u64><<<(nleaves + 1 + 3) / 4, 256, 0, s>>>(key, n, leafkey, nleaves, ls);
-}
-void launch_leafstart_search(const unsigned* key, int n, const unsigned* leafkey, int nleaves,
-                             int* ls, hipStream_t s)
-{
-    k_leafstart_search<unsigned><<<(nleaves + 1 + 3) / 4, 256, 0, s>>>(key, n, leafkey, nleaves,
-                                                                       ls);
+    leafstart_search_launch(key, n, key2, n2, leafkey, nleaves, ls, ls2, s);
 }
 
 __global__ __launch_bounds__(256) void k_iota(unsigned* __restrict__ v, int n)
@@ -367,17 +382,21 @@ void launch_iota(unsigned* v, int n, hipStream_t s)
 __global__ __launch_bounds__(256) void k_box_ranges(const int* __restrict__ lo,
                                                     const int* __restrict__ hi, int nb,
                                                     const int* __restrict__ ls,
-                                                    int2* __restrict__ se)
+                                                    int2* __restrict__ se,
+                                                    const int* __restrict__ ls2,
+                                                    int2* __restrict__ se2)
 {
     int b = blockIdx.x * 256 + threadIdx.x;
     if (b >= nb) return;
-    se[b] = make_int2(ls[lo[b]], ls[hi[b]]);
+    const int l = lo[b], h = hi[b];
+    se[b] = make_int2(ls[l], ls[h]);
+    if (se2) se2[b] = make_int2(ls2[l], ls2[h]);
 }
 
 void launch_box_ranges(const int* rank_lo, const int* rank_hi, int nb, const int* ls, int2* se,
-                       hipStream_t s)
+                       const int* ls2, int2* se2, hipStream_t s)
 {
-    k_box_ranges<<<grid_for(nb, 256), 256, 0, s>>>(rank_lo, rank_hi, nb, ls, se);
+    k_box_ranges<<<grid_for(nb, 256), 256, 0, s>>>(rank_lo, rank_hi, nb, ls, se, ls2, se2);
 }
 
 __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ xy,
