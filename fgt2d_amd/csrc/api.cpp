@@ -1006,9 +1006,14 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         if (q.ifpgh >= 2) { gs = W.get<double>("grad_s", 2 * (size_t)nd * ws_); launch_zero(gs, 2 * (size_t)nd * ws_, s); }
         if (q.ifpgh >= 3) { hs = W.get<double>("hess_s", 3 * (size_t)nd * ws_); launch_zero(hs, 3 * (size_t)nd * ws_, s); }
     }
+    // the expansion stores are accumulated into: cleared together with the
+    // target potentials, in one launch
+    const size_t per = 2 * (size_t)nexp * nd;
+    double* mp = expansions ? W.get<double>("mp", per * nslot) : nullptr;
+    double* lc = expansions ? W.get<double>("loc", per * nslot) : nullptr;
+    if (use_trg) pt = W.get<double>("pot_t", (size_t)nd * wt_);
+    launch_zero3(pt, use_trg ? (size_t)nd * wt_ : 0, mp, per * nslot, lc, per * nslot, s);
     if (use_trg) {
-        pt = W.get<double>("pot_t", (size_t)nd * wt_);
-        launch_zero(pt, (size_t)nd * wt_, s);
         if (q.ifpghtarg >= 2) { gt = W.get<double>("grad_t", 2 * (size_t)nd * wt_); launch_zero(gt, 2 * (size_t)nd * wt_, s); }
         if (q.ifpghtarg >= 3) { ht = W.get<double>("hess_t", 3 * (size_t)nd * wt_); launch_zero(ht, 3 * (size_t)nd * wt_, s); }
     }
@@ -1067,11 +1072,6 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
 
     // ---------------- expansions: steps 1-5 ----------------
     if (expansions) {
-        size_t per = 2 * (size_t)nexp * nd;
-        double* mp = W.get<double>("mp", per * nslot);
-        double* lc = W.get<double>("loc", per * nslot);
-        launch_zero(mp, per * nslot, s);
-        launch_zero(lc, per * nslot, s);
         const double* d_h2x = tab.h2x;
         const double* d_h2xr = tab.h2xr;
         const double* d_x2l = tab.x2l;

@@ -452,6 +452,30 @@ __global__ __launch_bounds__(256) void k_zero(double* __restrict__ p, size_t n)
     for (size_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0;
 }
 
+// up to three buffers cleared by one launch (null or empty ones skipped)
+__global__ __launch_bounds__(256) void k_zero3(double* __restrict__ a, size_t na,
+                                               double* __restrict__ b, size_t nb,
+                                               double* __restrict__ c, size_t nc)
+{
+    const size_t n = na > nb ? (na > nc ? na : nc) : (nb > nc ? nb : nc);
+    for (size_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        if (i < na) a[i] = 0;
+        if (i < nb) b[i] = 0;
+        if (i < nc) c[i] = 0;
+    }
+}
+
+void launch_zero3(double* a, size_t na, double* b, size_t nb, double* c, size_t nc, hipStream_t s)
+{
+    if (!a) na = 0;
+    if (!b) nb = 0;
+    if (!c) nc = 0;
+    const size_t n = std::max(na, std::max(nb, nc));
+    if (n == 0) return;
+    const size_t g = std::min<size_t>((n + 255) / 256, 1 << 20);
+    k_zero3<<<(unsigned)g, 256, 0, s>>>(a, na, b, nb, c, nc);
+}
+
 void launch_zero(double* p, size_t n, hipStream_t s)
 {
     if (n == 0) return;
