@@ -52,6 +52,7 @@ def load():
     lib.fgtpw_init.argtypes = [i]
     lib.fgtpw_fgt2d.argtypes = [i, d, d, i, dp, i, dp, i, dp, dp, i, i, dp, dp, dp, i, dp, i, dp, dp, dp]
     lib.fgtpw_fgt2d_device.argtypes = lib.fgtpw_fgt2d.argtypes + [vp]
+    lib.fgtpw_fgt2d_device_noalloc.argtypes = lib.fgtpw_fgt2d_device.argtypes
     lib.fgtpw_fgt2dpart_direct.argtypes = [i, d, d, i, i, i, i, dp, i, dp, i, dp, dp, dp, i, dp, dp, dp]
     lib.fgtpw_last_stats.argtypes = [ip, i]
     lib.fgtpw_set_timing.argtypes = [i]
@@ -74,7 +75,8 @@ def load():
     lib.fgtpw_wait.argtypes = [_C.c_longlong, _C.POINTER(_C.c_double)]
     lib.fgtpw_serve_stats.argtypes = [_C.POINTER(_C.c_longlong), i]
     for f in ("fgtpw_serve_start", "fgtpw_wait", "fgtpw_serve_stats", "fgtpw_serve_stop",
-              "fgtpw_init", "fgtpw_fgt2d", "fgtpw_fgt2d_device", "fgtpw_fgt2dpart_direct",
+              "fgtpw_init", "fgtpw_fgt2d", "fgtpw_fgt2d_device", "fgtpw_fgt2d_device_noalloc",
+              "fgtpw_fgt2dpart_direct",
               "fgtpw_last_stats", "fgtpw_set_timing", "fgtpw_stage_times", "fgtpw_last_tree2d",
               "fgtpw_tree2d_from_leaves", "fgtpw_fgt3d", "fgtpw_fgt3d_device",
               "fgtpw_fgt3dpart_direct", "fgtpw_last_tree3d"):

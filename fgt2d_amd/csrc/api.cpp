@@ -113,10 +113,37 @@ struct Workspace {
     int last_ns = 0, last_nt = 0;
     bool last_valid = false;
 
+    // Arena mode (the no-allocation entry point): get() hands out pieces of
+    // a fixed device region, reset at the start of every call, and the host
+    // staging areas are pageable.  Nothing is allocated on the device.
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    std::map<std::string, std::pair<size_t, size_t>> arena_map;  // name -> offset, bytes
+    std::vector<char> stage, stage2;                              // pageable host staging
+    void arena_reset()
+    {
+        arena_used = 0;
+        arena_map.clear();
+        tab.valid = tab3.valid = false;  // the tables live in the arena too
+    }
+
     template <class T>
     T* get(const std::string& name, size_t count)
     {
         size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+        if (arena) {
+            auto it = arena_map.find(name);
+            if (it != arena_map.end() && it->second.second >= bytes)
+                return (T*)(arena + it->second.first);
+            const size_t off = (arena_used + 255) & ~size_t(255);
+            if (off + bytes > arena_cap)
+                fail(FGTPW_ENOMEM, "the tree and expansions of this transform need more than the " +
+                                       std::to_string(arena_cap >> 20) +
+                                       " MiB of the no-allocation scratch (buffer '" + name + "')");
+            arena_used = off + bytes;
+            arena_map[name] = {off, bytes};
+            return (T*)(arena + off);
+        }
         Buf& b = dev[name];
         if (b.cap < bytes) {
             if (b.p) HIPCHECK(hipFree(b.p));
@@ -137,6 +164,7 @@ struct Workspace {
     // `from` before it
     void order(hipStream_t from, hipStream_t to)
     {
+        if (from == to) return;  // one stream: program order
         if (n_order == order_ev.size()) {
             hipEvent_t e;
             HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -173,10 +201,26 @@ struct Workspace {
         own = nullptr;
         tab.valid = tab3.valid = false;
     }
-    void* pinned(size_t bytes) { return grow_pinned(pin, pin_cap, bytes); }
+    void* pinned(size_t bytes)
+    {
+        if (arena) return grow_stage(stage, bytes);
+        return grow_pinned(pin, pin_cap, bytes);
+    }
     // a second staging area, for an upload queued while the first one's may
     // still be in flight
-    void* pinned2(size_t bytes) { return grow_pinned(pin2, pin2_cap, bytes); }
+    void* pinned2(size_t bytes)
+    {
+        if (arena) return grow_stage(stage2, bytes);
+        return grow_pinned(pin2, pin2_cap, bytes);
+    }
+    // pageable staging: a copy from pageable memory has read its source
+    // when hipMemcpyAsync returns, and reads into it are followed by a
+    // stream synchronisation
+    static void* grow_stage(std::vector<char>& v, size_t bytes)
+    {
+        if (v.size() < bytes) v.resize(bytes);
+        return v.data();
+    }
     static void* grow_pinned(void*& p, size_t& cap, size_t bytes)
     {
         if (cap < bytes) {
@@ -994,7 +1038,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                                 ls2, s);
     launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, ls2, tg ? trg_se : nullptr, s);
     tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
-    if (!tg) HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));
+    if (!tg) launch_zero((double*)trg_se, (size_t)nb, s);  // int2 pairs, 8 bytes each
     tm.end(S_RANGES);
 
     // sorted outputs
@@ -1937,7 +1981,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
                                 ls2, s);
     launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, ls2, tg ? trg_se : nullptr, s);
     tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
-    if (!tg) HIPCHECK(hipMemsetAsync(trg_se, 0, sizeof(int2) * nb, s));
+    if (!tg) launch_zero((double*)trg_se, (size_t)nb, s);  // int2 pairs, 8 bytes each
     tm.end(S_RANGES);
 
     const int ws_ = use_src ? ns : 0, wt_ = use_trg ? nt : 0;
@@ -2288,6 +2332,106 @@ int fgtpw_fgt2d_device(int nd, double delta, double eps, int ns, const double* s
                nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg};
         if (ifdipole) { q.rn = rnormal; q.ds = dipstr; }
         hipStream_t s = (hipStream_t)stream;
+        run2d(W, q, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The no-allocation variant of fgtpw_fgt2d_device.  Every device buffer of
+// the transform (keys, sort scratch, sorted points and outputs, the plan,
+// expansions, partial moments, Taylor coefficients, tables) is carved out of
+// `scratch`, one static array per device that the library holds until the
+// process ends; host staging is pageable memory; all work runs on the
+// caller's stream.  The array keeps the last call's contents: every buffer
+// the transform reads before writing is cleared by a kernel of the call.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace fgtpw {
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};   // device address of `scratch`, per device
+Workspace* g_noalloc_ws[kMaxDevices] = {};
+
+// Device address of the current device's copy of `scratch`, looked up once
+// per device (with that device current).  Host code uses this address,
+// never the array's name, which on the host is only a stand-in.
+char* scratch_ptr(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) fail(FGTPW_EUNSUPPORTED, "device index above 63");
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fail(FGTPW_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return (char*)g_scratch_ptr[dev];
+}
+
+// Bytes of the buffers that scale with the points (keys, permutations, leaf
+// records, sorted points, densities and outputs, sort scratch), each
+// rounded to the arena's 256-byte alignment; the tree-sized rest is checked
+// as it is carved out.
+size_t noalloc_point_bytes(int nd, int ns, int nt, int ifdipole, int ifpgh, int ifpghtarg)
+{
+    auto r = [](size_t b) { return (std::max<size_t>(b, 256) + 255) & ~size_t(255); };
+    const size_t NS = ns, NT = nt, ND = nd, nmx = std::max(ns, nt);
+    auto outs = [&](int ifp, size_t n) {
+        return (ifp >= 1 ? r(8 * ND * n) : 0) + (ifp >= 2 ? r(16 * ND * n) : 0) +
+               (ifp >= 3 ? r(24 * ND * n) : 0);
+    };
+    size_t tmpb = std::max(sort_pairs_u64_temp((int)nmx), sort_pairs_u32_temp((int)nmx));
+    tmpb = std::max(tmpb, select_flagged_temp(ns));
+    size_t b = 2 * r(8 * NS) + r(4 * nmx) + r(4 * NS) + r(tmpb) + r(4 * NS) + r(NS) + r(4 * NS) +
+               r(sizeof(LeafRun) * NS) + r(16 * NS) + r(8 * ND * NS) + (ifdipole ? r(16 * ND * NS) : 0) +
+               outs(ifpgh, NS);
+    if (nt > 0) b += r(4 * NT) + r(16 * NT) + 2 * r(8 * NT) + outs(ifpghtarg, NT);
+    return b;
+}
+}  // namespace
+}  // namespace fgtpw
+
+extern "C" {
+
+int fgtpw_fgt2d_device_noalloc(int nd, double delta, double eps, int ns, const double* sources,
+                               int ifcharge, const double* charge, int ifdipole,
+                               const double* rnormal, const double* dipstr, int iper, int ifpgh,
+                               double* pot, double* grad, double* hess, int nt, const double* targ,
+                               int ifpghtarg, double* pottarg, double* gradtarg, double* hesstarg,
+                               void* stream)
+{
+    (void)iper;
+    return run_guarded([&] {
+        validate(nd, delta, eps, ns, ifcharge, ifdipole, ifpgh, nt, ifpghtarg);
+        check_strengths(ifcharge, charge, ifdipole, rnormal, dipstr);
+        const size_t need = noalloc_point_bytes(nd, ns, nt, ifdipole, ifpgh, ifpghtarg);
+        if (need > kScratchBytes)
+            fail(FGTPW_EINVAL, "ns = " + std::to_string(ns) + ", nt = " + std::to_string(nt) +
+                                   ", nd = " + std::to_string(nd) + " need " +
+                                   std::to_string(need >> 20) + " MiB of point buffers; the "
+                                   "no-allocation entry point has " +
+                                   std::to_string(kScratchBytes >> 20) + " MiB of scratch");
+        std::lock_guard<std::mutex> lk(g_mutex);
+        select_device();
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        char* base = scratch_ptr(dev);
+        Workspace*& wp = g_noalloc_ws[dev];
+        if (!wp) wp = new Workspace();  // host object only: no streams, no device memory
+        Workspace& W = *wp;
+        W.arena = base;
+        W.arena_cap = kScratchBytes;
+        W.arena_reset();
+        hipStream_t s = (hipStream_t)stream;
+        W.own = W.aux[0] = W.aux[1] = s;  // one stream, the caller's
+        Call q{nd, delta, eps, ns, sources, ifcharge ? charge : nullptr, ifpgh, pot, grad, hess,
+               nt, targ, ifpghtarg, pottarg, gradtarg, hesstarg};
+        if (ifdipole) { q.rn = rnormal; q.ds = dipstr; }
         run2d(W, q, s);
         HIPCHECK(hipStreamSynchronize(s));
     });

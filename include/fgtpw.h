@@ -58,6 +58,31 @@ int fgtpw_fgt2d_device(int nd, double delta, double eps, int ns, const double *s
                        double *gradtarg, double *hesstarg, void *stream);
 
 /*
+ * fgtpw_fgt2d_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * inside a HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every device buffer of the transform comes out of one static
+ * scratch array of 256 MiB per device (held by the library until the
+ * process ends); host staging is pageable; all work runs on `stream`.  It
+ * calls no hipMalloc, hipFree, hipHostMalloc or allocating library routine
+ * (the host tree build uses the host heap).  Returns FGTPW_EINVAL when the
+ * point-proportional buffers of (ns, nt, nd) exceed the scratch (about
+ * 1.5M sources + 1.5M targets for nd = 1, potentials only), and
+ * FGTPW_ENOMEM when the tree and expansions of the input do not fit beside
+ * them.  Every call on a device shares that device's array: calls are
+ * serialised by the library lock and return after `stream` has drained;
+ * two processes' calls do not share it (each holds its own copy); work on
+ * one device that overlapped a call would race on the array.  Results are
+ * bit-identical to fgtpw_fgt2d_device's.
+ */
+int fgtpw_fgt2d_device_noalloc(int nd, double delta, double eps, int ns, const double *sources,
+                               int ifcharge, const double *charge, int ifdipole,
+                               const double *rnormal, const double *dipstr, int iper, int ifpgh,
+                               double *pot, double *grad, double *hess, int nt, const double *targ,
+                               int ifpghtarg, double *pottarg, double *gradtarg, double *hesstarg,
+                               void *stream);
+
+/*
  * Replaces fgt2dpart_direct_vec (2d/planewave/src/fgt2d.f:1354-1512): adds to
  * pot/grad/hess at targets jstart..jend (1-based, inclusive) the direct
  * Gaussian sum over sources istart..iend, pairs with r^2 >= ln(1/eps) delta
