@@ -39,11 +39,11 @@ def workload(ns=NS, nt=NT, seed=SEED):
     return src, chg, tg
 
 
-# MI355X published peaks
-PEAK = {"mfma": (78.6, "TFLOP/s"), "hbm": (8000.0, "GB/s")}
+# MI355X published peaks (dense; FP32 matrix and vector 157.3 TF, FP64 78.6 TF)
+PEAK = {"mfma": (78.6, "TFLOP/s"), "mfma32": (157.3, "TFLOP/s"), "hbm": (8000.0, "GB/s")}
 
 
-def per_unit(kernel, npw, nd, p):
+def per_unit(kernel, npw, nd, p, level=-1, mixed=False):
     """Algorithmic work of one unit of an instrumented kernel on the route it
     runs, (bound, amount, what), for expansion order p at its level.
     Hermite formation accumulates one FMA per coefficient (j, k <= p) and
@@ -51,18 +51,25 @@ def per_unit(kernel, npw, nd, p):
     evaluation point (potential); plane-wave evaluation contracts the
     npw/2 x npw half-spectrum per point, 8 (npw/2) npw + 8 npw flop per
     density; translations move 48 bytes per complex coefficient and box
-    (16 B read, 32 B read-modify-write); gathers and scatters their bytes
-    per point."""
+    (16 B read, 32 B read-modify-write), 24 with FP32 expansions; the
+    source gather reads and writes a point (16 B each way), its densities
+    (8 nd each way) and reads its index (4 B); the target gather (level -2)
+    the point and index only; the scatter reads, writes and indexes the
+    outputs."""
     half = npw // 2
+    mf = "mfma32" if mixed else "mfma"
     if kernel in ("form_moments", "form_valu"):
-        return "mfma", 2.0 * (p + 1) ** 2 * nd, "flop per source"
+        return mf if kernel == "form_moments" else "mfma", 2.0 * (p + 1) ** 2 * nd, \
+            "flop per source"
     if kernel == "taylor_eval":
-        return "mfma", 2.0 * (p + 1) ** 2 * nd, "flop per evaluation point"
+        return (mf if p + 1 <= 32 else "mfma"), 2.0 * (p + 1) ** 2 * nd, "flop per evaluation point"
     if kernel == "pw_eval":
         return "mfma", (8.0 * half * npw + 8.0 * npw) * nd, "flop per evaluation point"
     if kernel in ("merge", "m2l", "split"):
-        return "hbm", 48.0 * half * npw * nd, "bytes per box"
+        return "hbm", (24.0 if mixed else 48.0) * half * npw * nd, "bytes per box"
     if kernel == "gather":
+        if level == -2:
+            return "hbm", 2 * 16.0 + 4, "bytes per point"
         return "hbm", 2 * 8.0 * (2 + nd) + 4, "bytes per point"
     if kernel == "scatter":
         return "hbm", 2 * 8.0 * nd + 4, "bytes per point"
@@ -80,19 +87,34 @@ def pw_equivalent(kernel, npw, nd):
     return None
 
 
-def rocprof_name(kernel, nl, ifpgh):
+def rocprof_name(kernel, nl, ifpgh, mixed=False):
     """Symbol of the HIP kernel an instrumented launch runs (as rocprofv3
-    lists it), for the Taylor order nl of its level."""
+    lists it, template arguments included), for the Taylor order nl of its
+    level."""
     tm = min(4, (nl + 1 + 15) // 16)
-    return {"form_moments": "k_form_moments<%d>" % tm,
-            "taylor_eval": "k_taylor_mfma<%d, %d>" % (tm, ifpgh)}.get(kernel, "k_" + kernel)
+    f32 = "_f32" if mixed else ""
+    if kernel == "form_moments":
+        return "k_form_wave%s<%d, false>" % (f32, tm)
+    if kernel == "taylor_eval":
+        if nl + 1 <= 32:
+            return "k_taylor_wave%s<%d, %d>" % (f32 if mixed else "", tm, ifpgh)
+        return "k_taylor_mfma<%d, %d>" % (tm, ifpgh)
+    if kernel == "form_h2pw":
+        return "k_form_h2pw<%s>" % ("float" if mixed else "double")
+    if kernel in ("merge", "m2l", "split"):
+        return "k_%s<%s>" % (kernel, "float" if mixed else "double")
+    if kernel == "pw2local":
+        return "k_pw2local<%s>" % ("float, float" if mixed else "double, double")
+    if kernel == "gather":
+        return "k_gather<true>"
+    return "k_" + kernel
 
 
-def read_traffic(name):
+def read_traffic(name, workload="c2"):
     """HBM bytes per launch of kernel `name` from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel factor),
     or None."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_c2.txt")
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_%s.txt" % workload)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -103,10 +125,15 @@ def read_traffic(name):
 
 
 def kernel_profile(step, torch, reps=5):
-    """Runs `reps` more transforms with timing on; per (kernel, level): total
-    ms, launches and units, plus the stage times of the last transform."""
+    """Runs `reps` more transforms with serial timing: the side streams' work
+    queued on the call's stream (no two kernels overlap), every instrumented
+    kernel dispatched with a start and an end event that the dispatch itself
+    stamps, so each record is the kernel's own duration (as rocprofv3's
+    kernel trace of a run with FGTPW_SERIAL=1 measures it); per (kernel,
+    level): total ms, launches and units, plus the stage times of the last
+    transform."""
     import fgt2d_amd
-    fgt2d_amd.set_timing(True)
+    fgt2d_amd.set_timing("serial")
     agg = {}
     try:
         for _ in range(reps):
@@ -123,7 +150,7 @@ def kernel_profile(step, torch, reps=5):
     return agg, st
 
 
-def rooflines(agg, stats, nd, count=4):
+def rooflines(agg, stats, nd, count=5, mixed=False):
     """Roofline records of the instrumented kernels with an algorithmic
     per-unit figure, largest total time first."""
     npw, nlocal = stats["npw"], stats["nlocal"]
@@ -132,21 +159,25 @@ def rooflines(agg, stats, nd, count=4):
     out = []
     for (kernel, level), (ms, n, units) in ranked:
         nl = nlocal[level] if 0 <= level < len(nlocal) else 0
-        pu = per_unit(kernel, npw, nd, nl)
+        pu = per_unit(kernel, npw, nd, nl, level, mixed)
         if pu is None:
             continue
         bound, amount, what = pu
         launch_s = ms / n / 1e3
         work = amount * units / n
         peak, unit = PEAK[bound]
-        scale = 1e12 if bound == "mfma" else 1e9
+        scale = 1e12 if bound.startswith("mfma") else 1e9
         achieved = work / launch_s / scale
-        name = rocprof_name(kernel, nl, 1)
-        rec = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+        f32 = mixed and (bound == "mfma32" or kernel in ("merge", "m2l", "split"))
+        name = rocprof_name(kernel, nl, 1, f32)
+        rec = {"bound": "mfma" if bound.startswith("mfma") else "hbm",
+               "achieved": round(achieved, 3), "peak": peak, "unit": unit,
                "frac": round(achieved / peak, 4), "traffic": read_traffic(name),
                "kernel": name, "level": level, "launch_us": round(launch_s * 1e6, 2),
                "units_per_launch": units // n, "per_unit": "%g %s" % (amount, what),
                "share_of_kernel_time": round(ms / total, 3)}
+        if bound == "mfma32":
+            rec["precision"] = "fp32"
         eq = pw_equivalent(kernel, npw, nd)
         if eq is not None:
             rec["pw_equivalent_rate"] = round(eq * units / n / launch_s / scale, 3)
@@ -172,20 +203,14 @@ def cpu_baseline():
         os.unlink(path)
 
 
-def cpu_leg(src, chg, tg, budget_s=30.0):
-    """The reference itself (oracle/_ref, single core: its OpenMP build does
-    not compile) on the same workload, or the C restatement when the
-    reference build is absent."""
-    from oracle import oracle as O
-    kind = "reference" if O.have_ref() else "port"
-    f = O.ref_fgt2d if kind == "reference" else O.oracle_fgt2d
-    # bounded sample: whole transforms of the first n points of the
-    # workload, n halved until one call takes under budget_s / 3; at most
-    # 3 calls and budget_s / 3 seconds of repetitions
+def _timed_transforms(f, src, chg, tg, budget_s, **kw):
+    """Whole transforms of the first n points of the workload, n halved
+    until one call takes under budget_s / 3; then up to 3 calls within
+    budget_s / 3 seconds.  Returns (n, reps, elapsed seconds)."""
     n = len(src)
     while True:
         t0 = time.perf_counter()
-        f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1)
+        f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1, **kw)
         dt = time.perf_counter() - t0
         if dt < budget_s / 3 or n <= 100_000:
             break
@@ -193,13 +218,35 @@ def cpu_leg(src, chg, tg, budget_s=30.0):
     reps, elapsed = 1, dt
     while elapsed < budget_s / 3 and reps < 3:
         t0 = time.perf_counter()
-        f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1)
+        f(DELTA, EPS, src[:n], chg[:n], tg[:n], ifpgh=0, ifpghtarg=1, **kw)
         elapsed += time.perf_counter() - t0
         reps += 1
-    rate = 2 * n * reps / elapsed / 1e6
-    return {"value": round(rate, 4), "unit": "Mpts/s", "cores": 1, "kind": kind,
+    return n, reps, elapsed
+
+
+def cpu_leg(src, chg, tg, budget_s=30.0):
+    """The CPU baseline on the same workload.  Headline: the oracle's C
+    restatement built with OpenMP (oracle/liborc_omp.so: its per-box loops
+    threaded, results bit-identical to the sequential build) on all the
+    host's cores given to this process (OMP_NUM_THREADS, else the CPU
+    count).  Beside it: the reference itself (oracle/_ref, one core: its own
+    OpenMP build does not compile), when built."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1
+    used = O.threads_in_use(threads)
+    n, reps, el = _timed_transforms(O.oracle_fgt2d, src, chg, tg, budget_s, threads=used)
+    rec = {"value": round(2 * n * reps / el / 1e6, 4), "unit": "Mpts/s", "cores": used,
+           "kind": "port", "nproc": os.cpu_count(),
+           "sample": "%d full transforms of the first %d sources and %d targets of the workload "
+                     "on %d OpenMP threads, %.1f s" % (reps, n, n, used, el)}
+    if O.have_ref():
+        n, reps, el = _timed_transforms(O.ref_fgt2d, src, chg, tg, budget_s)
+        rec["reference_1core"] = {
+            "value": round(2 * n * reps / el / 1e6, 4), "unit": "Mpts/s", "cores": 1,
+            "kind": "reference",
             "sample": "%d full transforms of the first %d sources and %d targets of the workload, "
-                      "%.1f s" % (reps, n, n, elapsed)}
+                      "%.1f s" % (reps, n, n, el)}
+    return rec
 
 
 N3 = 1_000_000
@@ -237,7 +284,7 @@ def bench3d(args, torch, fgt2d_amd):
         chg = rng.uniform(-1, 1, (N3, 1))
         tg = None
         delta, eps, nsel = DELTA3, EPS3, 5000
-        npts = N3
+        npts = 2 * N3  # Ns + N_eval, potentials evaluated at the 1M sources
     dev = torch.device("cuda:0")
     d_src = torch.from_numpy(src).to(dev)
     d_chg = torch.from_numpy(chg).to(dev)
@@ -388,6 +435,11 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=1000,
                     help="requests of the serving stream reported beside c2 (0: none)")
     ap.add_argument("--slots", type=int, default=4, help="server slots of the serving stream")
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
+                    help="c2 legs: both (default; the mixed-precision route is the headline), "
+                         "or one of them")
+    ap.add_argument("--host-steps", type=int, default=5,
+                    help="c2: timed transforms through the host-array entry point (0: none)")
     ap.add_argument("--cpu-leg", metavar="OUT", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calib", action="store_true",
                     help="end with one 256 MiB device copy (the reference kernel for PMC byte counts)")
@@ -422,43 +474,55 @@ def main():
     d_chg = torch.from_numpy(chg).to(dev)
     d_tg = torch.from_numpy(tg).to(dev)
     torch.cuda.synchronize()
+    from oracle import oracle as O
 
     def step():
         return fgt2d_amd.fgt2d(DELTA, EPS, d_src, d_chg, d_tg, ifpgh=0, ifpghtarg=1)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = step()
-    torch.cuda.synchronize()
-    t = (time.perf_counter() - t0) / args.steps
-    stats = fgt2d_amd.last_stats()
-
-    # per-kernel and per-stage times of 5 more calls, HIP events on the
-    # call's stream around each instrumented launch
-    agg, st = kernel_profile(step, torch)
-    if args.stages:
-        ks = {"%s@%d" % k: [round(v[0] / v[1] * 1e3, 2), v[1], v[2] // v[1]]
-              for k, v in sorted(agg.items(), key=lambda kv: -kv[1][0])}
-        print(json.dumps({"stages_ms": st, "stats": stats, "kernels_us_n_units": ks}),
-              file=sys.stderr)
-
-    # accuracy of this very output against the FP64 direct sum without
-    # cutoff on 10k targets (the GPU all-pairs kernel, itself parity-tested
-    # against the oracle), cross-checked by the oracle on 50 of them
-    from oracle import oracle as O
+    # the FP64 direct sum without cutoff on 10k targets (the GPU all-pairs
+    # kernel, itself parity-tested against the oracle), cross-checked by the
+    # oracle on 50 of them
     sel = np.random.default_rng(1).permutation(NT)[:10000]
-    pt = out[3].cpu().numpy()[sel]
     ex = fgt2d_amd.fgt2dpart_direct(DELTA, 1e-100, src, chg, tg[sel])[0]
-    err = O.derr(ex, pt)
     ex50 = O.direct2d(DELTA, 1e-100, src, chg, tg[sel[:50]])[0]
     direct_check = float(np.abs(ex50 - ex[:50]).max() / np.abs(ex50).max())
 
+    def leg(precision):
+        """W untimed and K timed transforms in one precision mode; the rel.
+        l2 error of the last output against the direct sum; per-kernel times
+        of 5 more transforms (serial timing)."""
+        fgt2d_amd.set_precision(precision)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = step()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        ran = fgt2d_amd.last_precision()
+        stats = fgt2d_amd.last_stats()
+        err = O.derr(ex, out[3].cpu().numpy()[sel])
+        del out
+        agg, st = kernel_profile(step, torch)
+        if args.stages:
+            ks = {"%s@%d" % k: [round(v[0] / v[1] * 1e3, 2), v[1], v[2] // v[1]]
+                  for k, v in sorted(agg.items(), key=lambda kv: -kv[1][0])}
+            print(json.dumps({"leg": precision, "stages_ms": st, "stats": stats,
+                              "kernels_us_n_units": ks}), file=sys.stderr)
+        fgt2d_amd.set_precision("fp64")
+        return dict(t=t, ran=ran, stats=stats, err=err, agg=agg, st=st)
+
     npts = NS + NT
+    legs = {}
+    for p in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+        legs[p] = leg(p)
+    head = legs["mixed"] if "mixed" in legs else legs["fp64"]
+    mixed = head["ran"] == "mixed"
+    stats, t = head["stats"], head["t"]
     rec = {
-        "metric": "Mpts/s, 2D N=2M eps=1e-6 plane-wave FGT (one MI355X, FP64)",
+        "metric": "Mpts/s, 2D N=2M eps=1e-6 plane-wave FGT, mixed-precision MFMA route vs FP64 "
+                  "(one MI355X)",
         "value": round(npts / t / 1e6, 3),
         "unit": "Mpts/s",
         "n_gpus": 1,
@@ -468,18 +532,39 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32-mfma+f64" if mixed else "f64",
         "data": "synthetic",
-        "config": {"workload": "fgt2d mixture 2M sources + 2M targets, delta=1e-5, eps=1e-6, pot at targets",
+        "config": {"workload": "fgt2d mixture 2M sources + 2M targets, delta=1e-5, eps=1e-6, "
+                               "pot at targets",
                    "ns": NS, "nt": NT, "delta": DELTA, "eps": EPS, "nlevels": stats.get("nlevels"),
-                   "nboxes": stats.get("nboxes"), "npw": stats.get("npw")},
-        "rel_l2_err_vs_direct": err,
+                   "nboxes": stats.get("nboxes"), "npw": stats.get("npw"),
+                   "route": head["ran"]},
+        "rel_l2_err_vs_direct": head["err"],
         "direct_oracle_check": direct_check,
-        "stages_ms": {k: round(v, 4) for k, v in st.items()},
+        "stages_ms": {k: round(v, 4) for k, v in head["st"].items()},
     }
-    rl = rooflines(agg, stats, 1)
+    for p, L in legs.items():
+        rec["leg_" + p] = {"Mpts_s": round(npts / L["t"] / 1e6, 3),
+                           "ms_per_step": round(L["t"] * 1e3, 4), "route": L["ran"],
+                           "rel_l2_err_vs_direct": L["err"]}
+    if "fp64" in legs and "mixed" in legs:
+        rec["mixed_vs_fp64"] = round(legs["fp64"]["t"] / legs["mixed"]["t"], 4)
+    rl = rooflines(head["agg"], stats, 1, mixed=mixed)
     rec["roofline"] = rl[0] if rl else None
     rec["roofline_next"] = rl[1:]
+    if args.host_steps > 0:
+        # the host-array entry (what the Fortran drop-in calls): transfers
+        # in and out included, FP64
+        for _ in range(2):
+            fgt2d_amd.fgt2d(DELTA, EPS, src, chg, tg, ifpgh=0, ifpghtarg=1)
+        t0 = time.perf_counter()
+        for _ in range(args.host_steps):
+            fgt2d_amd.fgt2d(DELTA, EPS, src, chg, tg, ifpgh=0, ifpghtarg=1)
+        th = (time.perf_counter() - t0) / args.host_steps
+        rec["host_arrays"] = {"Mpts_s": round(npts / th / 1e6, 3), "ms_per_step": round(th * 1e3, 4),
+                              "steps": args.host_steps, "route": "fp64",
+                              "what": "fgtpw_fgt2d on host arrays: H2D of sources, charges and "
+                                      "targets, the transform, D2H of the target potentials"}
     if args.copy_calib:
         a = torch.empty(32 << 20, dtype=torch.float64, device=dev).fill_(1.0)
         b = torch.empty_like(a)

@@ -18,7 +18,8 @@ import numpy as _np
 
 __all__ = ["fgt2d", "fgt3d", "fgt2dpart_direct", "fgt3dpart_direct", "FGTError", "Server",
            "last_stats", "last_tree2d", "last_tree3d", "tree2d_from_leaves", "set_timing",
-           "stage_times", "kernel_launches", "library_path", "load"]
+           "stage_times", "kernel_launches", "library_path", "load", "set_precision",
+           "last_precision"]
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
 _LIB = None
@@ -56,6 +57,8 @@ def load():
     lib.fgtpw_fgt2dpart_direct.argtypes = [i, d, d, i, i, i, i, dp, i, dp, i, dp, dp, dp, i, dp, dp, dp]
     lib.fgtpw_last_stats.argtypes = [ip, i]
     lib.fgtpw_set_timing.argtypes = [i]
+    lib.fgtpw_set_precision.argtypes = [i]
+    lib.fgtpw_last_precision.argtypes = []
     lib.fgtpw_stage_times.argtypes = [_C.POINTER(_C.c_double), i]
     lib.fgtpw_stage_name.restype = _C.c_char_p
     lib.fgtpw_stage_name.argtypes = [i]
@@ -78,6 +81,7 @@ def load():
               "fgtpw_init", "fgtpw_fgt2d", "fgtpw_fgt2d_device", "fgtpw_fgt2d_device_noalloc",
               "fgtpw_fgt2dpart_direct",
               "fgtpw_last_stats", "fgtpw_set_timing", "fgtpw_stage_times", "fgtpw_last_tree2d",
+              "fgtpw_set_precision", "fgtpw_last_precision",
               "fgtpw_tree2d_from_leaves", "fgtpw_fgt3d", "fgtpw_fgt3d_device",
               "fgtpw_fgt3dpart_direct", "fgtpw_last_tree3d"):
         getattr(lib, f).restype = _C.c_int
@@ -279,7 +283,29 @@ def last_stats():
 
 
 def set_timing(on=True):
-    load().fgtpw_set_timing(1 if on else 0)
+    """Per-stage and per-kernel timing of the next transforms: True (events
+    on the streams as they run), "serial" (the side streams' work queued on
+    the call's stream, so that each kernel's time is its own) or False."""
+    load().fgtpw_set_timing(2 if on == "serial" else (1 if on else 0))
+
+
+def set_precision(mode):
+    """Arithmetic of the next transforms: "fp64" (default) or "mixed" (FP32
+    matrix-core moments and Taylor evaluation and FP32 plane-wave expansions
+    for 2D transforms with eps >= 1e-6; see include/fgtpw.h).  Returns the
+    previous mode."""
+    modes = {"fp64": 0, "mixed": 1}
+    if mode not in modes:
+        raise ValueError("precision mode must be 'fp64' or 'mixed'")
+    prev = load().fgtpw_set_precision(modes[mode])
+    if prev < 0:
+        _check(prev)
+    return "mixed" if prev == 1 else "fp64"
+
+
+def last_precision():
+    """'mixed' when the last transform of this thread took the mixed route."""
+    return "mixed" if load().fgtpw_last_precision() == 1 else "fp64"
 
 
 def stage_times():

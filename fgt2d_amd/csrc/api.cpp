@@ -36,7 +36,26 @@ namespace {
 thread_local std::string t_err;
 thread_local std::vector<int> t_stats;
 thread_local std::vector<double> t_stage_ms;
+thread_local bool t_mixed = false;  // the last 2D transform took the mixed-precision route
 bool g_timing = false;
+// timing mode 2, or FGTPW_SERIAL=1 for every call: the side streams' work
+// runs on the call's stream (no two kernels of a call overlap)
+const bool g_serial_env = [] {
+    const char* e = std::getenv("FGTPW_SERIAL");
+    return e && e[0] == '1';
+}();
+bool g_serial = g_serial_env;
+// 0: FP64 everywhere; 1: the mixed-precision route for 2D transforms with
+// eps >= kMixedEpsMin (FP32 matrix-core moments and Taylor evaluation, FP32
+// plane-wave expansions and translation factors)
+int g_precision = 0;
+constexpr double kMixedEpsMin = 1e-6;
+// 2D evaluation of Taylor orders below 32: PW -> Taylor inside the
+// evaluation waves (1, default) or as its own kernel (FGTPW_FUSE_PW2LOCAL=0)
+const bool g_fuse_pw2local = [] {
+    const char* e = std::getenv("FGTPW_FUSE_PW2LOCAL");
+    return !(e && e[0] == '0');
+}();
 std::mutex g_mutex;  // one transform at a time per process (shared workspace)
 int g_device = -1;
 
@@ -70,6 +89,7 @@ struct Tables2D {
     double pmax = 0, xmin2 = 0, xm = 0;
     const double *ts = nullptr, *h2x = nullptr, *h2xr = nullptr, *x2l = nullptr, *rsq = nullptr,
                  *wms = nullptr, *wm2l = nullptr;
+    const float *wms32 = nullptr, *wm2l32 = nullptr;  // the same factors rounded to FP32
 };
 
 struct Tables3D {
@@ -94,6 +114,10 @@ struct Workspace {
     // side streams (default priority): coarse levels of formation and
     // evaluation and the near field overlap the finest level's work
     hipStream_t aux[2] = {};
+    // pinned staging ring of the host-array entry points: two halves, each
+    // with the event of the copy that last used it
+    char* ring = nullptr;
+    hipEvent_t ring_ev[2] = {};
     std::vector<hipEvent_t> order_ev;  // stream-ordering events, reused from call to call
     size_t n_order = 0;
     std::vector<hipEvent_t> sev;  // stage timing events, grown on demand
@@ -186,6 +210,11 @@ struct Workspace {
         if (pin2) (void)hipHostFree(pin2);
         pin2 = nullptr;
         pin2_cap = 0;
+        if (ring) (void)hipHostFree(ring);
+        ring = nullptr;
+        for (auto& e : ring_ev)
+            if (e) (void)hipEventDestroy(e);
+        ring_ev[0] = ring_ev[1] = nullptr;
         for (auto& e : sev)
             if (e) (void)hipEventDestroy(e);
         sev.clear();
@@ -304,6 +333,13 @@ struct LaunchClock : KernelClock {
         } else {
             HIPCHECK(hipEventRecord(recs.back().e1, st));
         }
+    }
+    void pair(int id, hipEvent_t* e0, hipEvent_t* e1) override
+    {
+        Rec r{id, level, side, boxes, r0, r1, units, ev(2 * recs.size()), ev(2 * recs.size() + 1)};
+        recs.push_back(r);
+        *e0 = r.e0;
+        *e1 = r.e1;
     }
 };
 
@@ -548,10 +584,12 @@ const Tables2D& tables2d(Workspace& W, int npw, double pmax, int ntermmax, int n
                     wm2l[o] = xr * yr - xi * yi;
                     wm2l[o + 1] = xr * yi + xi * yr;
                 }
+    std::vector<float> wms32(wms.begin(), wms.end()), wm2l32(wm2l.begin(), wm2l.end());
     Pack pk;
     size_t o_ts = pk.add(ts), o_h2x = pk.add(h2x), o_h2xr = pk.add(h2xr), o_x2l = pk.add(x2l),
            o_rsq = pk.add(rsq);
     size_t o_wms = pk.add(wms), o_wm2l = pk.add(wm2l);
+    size_t o_wms32 = pk.add(wms32), o_wm2l32 = pk.add(wm2l32);
     char* d = W.get<char>("tables", pk.size());
     HIPCHECK(hipMemcpyAsync(d, pk.data(), pk.size(), hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));  // pk is freed on return
@@ -562,6 +600,8 @@ const Tables2D& tables2d(Workspace& W, int npw, double pmax, int ntermmax, int n
     t.rsq = (const double*)(d + o_rsq);
     t.wms = (const double*)(d + o_wms);
     t.wm2l = (const double*)(d + o_wm2l);
+    t.wms32 = (const float*)(d + o_wms32);
+    t.wm2l32 = (const float*)(d + o_wm2l32);
     t.npw = npw; t.pmax = pmax; t.ntermmax = ntermmax; t.nmax = nmax; t.xmin2 = xmin2; t.xm = xm;
     t.valid = true;
     return t;
@@ -687,6 +727,7 @@ void zero_outputs(const Call& q, int dim, hipStream_t s)
 
 void run2d(Workspace& W, const Call& q, hipStream_t s)
 {
+    t_mixed = false;
     W.n_order = 0;
     Timer tm(W, s);
     const int ns = q.ns, nt = q.nt, nd = q.nd;
@@ -833,7 +874,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s);
         else
             sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
-        tm.ctx(-1, nullptr, -1, nt);
+        tm.ctx(-2, nullptr, -1, nt);  // level -2 tags the gather without densities
         launch_gather(q.targ, nullptr, 0, itarg, nt, xt, nullptr, s);
     }
     tm.end(S_ASSIGN);
@@ -868,6 +909,10 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     const bool expansions = npwlevel < nlevels;
     const int half = npw / 2, nexp = half * npw;
     const double dsq = 1.0 / std::sqrt(q.delta);
+    // the mixed-precision route: FP32 expansions, matrix-core moments and
+    // Taylor evaluation where the order allows (formation orders below 64)
+    const bool mixed = g_precision == 1 && expansions && q.eps >= kMixedEpsMin && ntermmax < 63;
+    const size_t esz = mixed ? sizeof(float) : sizeof(double);
 
     // box arrays
     std::vector<int>& slot = W.h_slot;
@@ -1052,11 +1097,12 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     }
     // the expansion stores are accumulated into: cleared together with the
     // target potentials, in one launch
-    const size_t per = 2 * (size_t)nexp * nd;
-    double* mp = expansions ? W.get<double>("mp", per * nslot) : nullptr;
-    double* lc = expansions ? W.get<double>("loc", per * nslot) : nullptr;
+    const size_t per = 2 * (size_t)nexp * nd;  // elements (double or float) per slot
+    double* mp = expansions ? (double*)W.get<char>("mp", per * nslot * esz) : nullptr;
+    double* lc = expansions ? (double*)W.get<char>("loc", per * nslot * esz) : nullptr;
     if (use_trg) pt = W.get<double>("pot_t", (size_t)nd * wt_);
-    launch_zero3(pt, use_trg ? (size_t)nd * wt_ : 0, mp, per * nslot, lc, per * nslot, s);
+    const size_t nexp_d = per * nslot * esz / sizeof(double);  // per is even
+    launch_zero3(pt, use_trg ? (size_t)nd * wt_ : 0, mp, nexp_d, lc, nexp_d, s);
     if (use_trg) {
         if (q.ifpghtarg >= 2) { gt = W.get<double>("grad_t", 2 * (size_t)nd * wt_); launch_zero(gt, 2 * (size_t)nd * wt_, s); }
         if (q.ifpghtarg >= 3) { ht = W.get<double>("hess_t", 3 * (size_t)nd * wt_); launch_zero(ht, 3 * (size_t)nd * wt_, s); }
@@ -1065,7 +1111,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     // ---------------- step 6: near field, cutoff eps/90 (fgt2d.f:1316) ----------------
     // on a side stream, concurrently with steps 1-4; it accumulates into the
     // zeroed outputs before any evaluation does (the evaluations wait for it)
-    hipStream_t s_nf = W.aux[1], s_lv = W.aux[0];
+    hipStream_t s_nf = g_serial ? s : W.aux[1], s_lv = g_serial ? s : W.aux[0];
     W.order(s, s_nf);  // after the ranges and the zeroed outputs
     tm.begin(S_DIRECT, s_nf);
     auto near_field = [&]() {
@@ -1121,13 +1167,15 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         const double* d_x2l = tab.x2l;
         const double* d_rsq = tab.rsq;
         const double* d_ts = tab.ts;
-        const double* d_wms = tab.wms;
-        const double* d_wm2l = tab.wm2l;
+        const double* d_wms = mixed ? (const double*)tab.wms32 : tab.wms;
+        const double* d_wm2l = mixed ? (const double*)tab.wm2l32 : tab.wm2l;
         // Taylor coefficients of every evaluated box, one region per level
         // (levels are evaluated concurrently)
         std::vector<size_t> o_taylor(nlevels + 2, 0);
-        for (int l = 0; l <= nlevels; ++l)
-            o_taylor[l + 1] = o_taylor[l] + (nlocal[l] <= 50 ? eval_l[l].size() * (size_t)(nlocal[l] + 1) * (nlocal[l] + 1) : 0);
+        for (int l = 0; l <= nlevels; ++l) {
+            const bool fused = g_fuse_pw2local && nlocal[l] + 1 <= 32 && npw <= 64;
+            o_taylor[l + 1] = o_taylor[l] + (nlocal[l] <= 50 && !fused ? eval_l[l].size() * (size_t)(nlocal[l] + 1) * (nlocal[l] + 1) : 0);
+        }
         double* taylor = W.get<double>("taylor", o_taylor[nlevels + 1]);
         // segment numbering of every formation and evaluation list (one
         // launch): formation over sources in segments of SEGF, evaluation
@@ -1194,6 +1242,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                             dsq, nlocal[l], d_h2x, d_h2xr, d_rsq, npw, mp, ts_form[l], segf(l),
                             (int)std::min<long>(GRIDMAX, nb_l + ns / segf(l) + 1),
                             partial + o_part[l], m * m, sb_form[l], wsd};
+                fa.prec = mixed ? 1 : 0;
                 tm.ctx(l, &form_l[l], 0);
                 launch_form(fa, l == nlevels ? s : s_lv);
             }
@@ -1203,13 +1252,13 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         tm.begin(S_MERGE);
         for (int l = nlevels - 1; l >= nlevstart; --l)
             tm.ctx(l, &merge_l[l], 2), launch_merge((const int*)D(o_merge[l]), (int)merge_l[l].size(), d_child, d_cen, src_se,
-                         d_slot, d_wms, 4 * (nlevels - l - 1), nd, nexp, mp, s);
+                         d_slot, d_wms, 4 * (nlevels - l - 1), nd, nexp, mp, s, mixed);
         tm.end(S_MERGE);
         tm.begin(S_M2L);
         tm.ctx(nlevstart, &m2l_l, 2);
         launch_m2l((const int*)D(o_m2l), (int)m2l_l.size(), d_cen, d_ncol, d_col, src_se, trg_se,
                    use_src, use_trg, d_slot, d_wm2l, npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0],
-                   nlevstart == npwlevel, nd, nexp, mp, lc, s);
+                   nlevstart == npwlevel, nd, nexp, mp, lc, s, mixed);
         tm.end(S_M2L);
         near_field();
         // the evaluations accumulate after the near field
@@ -1235,8 +1284,12 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                 ea.loc = lc;
                 ea.taylor = taylor + o_taylor[l];
                 bool pwroute = nlocal[l] > 50;  // nlocalswitch, fgt2d.f:855
+                ea.exp32 = mixed ? 1 : 0;
+                ea.tay32 = mixed && !pwroute && nlocal[l] + 1 <= 32 ? 1 : 0;
+                // orders below 32: PW -> Taylor inside the evaluation waves
+                ea.fused = !pwroute && g_fuse_pw2local && nlocal[l] + 1 <= 32 && npw <= 64 ? 1 : 0;
                 tm.ctx(l, &eval_l[l], 2);
-                if (!pwroute) launch_pw2local(ea, st);
+                if (!pwroute && !ea.fused) launch_pw2local(ea, st);
                 for (int side = 0; side < 2; ++side) {
                     if (side == 0 && !use_trg) continue;
                     if (side == 1 && !use_src) continue;
@@ -1265,7 +1318,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             eval_level(l, s_lv);
             tm.ctx(l, &split_l[l], 2);
             launch_split((const int*)D(o_split[l]), (int)split_l[l].size(), d_par, d_cen, d_slot,
-                         d_wms, 4 * (nlevels - l - 1), nd, nexp, lc, s);
+                         d_wms, 4 * (nlevels - l - 1), nd, nexp, lc, s, mixed);
         }
         tm.end(S_SPLIT);
         tm.begin(S_EVAL);
@@ -1298,6 +1351,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     tm.finish();
     W.last_valid = true;
 
+    t_mixed = mixed;
     t_stats.assign(6 + nlevels + 1, 0);
     t_stats[0] = npwlevel;
     t_stats[1] = nlmax;
@@ -1495,6 +1549,7 @@ void run_large_delta3(Workspace& W, const Call& q, hipStream_t s, int npwlevel, 
 // dmax = ln(1/eps) delta and the 3D modified list1 (tree_routs3d.f:511-620).
 void run3d(Workspace& W, const Call& q, hipStream_t s)
 {
+    t_mixed = false;
     if (!q.chg && !q.rn) {
         zero_outputs(q, 3, s);
         return;
@@ -2441,6 +2496,71 @@ int fgtpw_fgt2d_device_noalloc(int nd, double delta, double eps, int ns, const d
 
 namespace fgtpw {
 namespace {
+// Host-array transfers through the workspace's pinned ring: chunks of
+// kRingHalf bytes alternate between the two halves, the host's copy of one
+// chunk (split over the worker threads) running while the DMA of the other
+// is in flight.
+constexpr size_t kRingHalf = (size_t)8 << 20;
+// FGTPW_STAGED_COPIES=0: plain pageable copies instead (for comparison)
+const bool g_staged = [] {
+    const char* e = std::getenv("FGTPW_STAGED_COPIES");
+    return !(e && e[0] == '0');
+}();
+
+void ring_init(Workspace& W, hipStream_t s)
+{
+    if (W.ring) return;
+    HIPCHECK(hipHostMalloc((void**)&W.ring, 2 * kRingHalf, hipHostMallocDefault));
+    for (auto& e : W.ring_ev) {
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPCHECK(hipEventRecord(e, s));  // both halves start free
+    }
+}
+
+void staged_h2d(Workspace& W, void* dst, const void* src, size_t n, hipStream_t s)
+{
+    if (!g_staged) {
+        HIPCHECK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s));
+        return;
+    }
+    for (size_t off = 0, i = 0; off < n; off += kRingHalf, ++i) {
+        const int h = (int)(i & 1);
+        const size_t len = std::min(kRingHalf, n - off);
+        char* buf = W.ring + h * kRingHalf;
+        HIPCHECK(hipEventSynchronize(W.ring_ev[h]));  // the half's last DMA is done
+        parallel_copy(buf, (const char*)src + off, len);
+        HIPCHECK(hipMemcpyAsync((char*)dst + off, buf, len, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipEventRecord(W.ring_ev[h], s));
+    }
+}
+
+void staged_d2h(Workspace& W, void* dst, const void* src, size_t n, hipStream_t s)
+{
+    if (!g_staged) {
+        HIPCHECK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s));
+        return;
+    }
+    const size_t nch = (n + kRingHalf - 1) / kRingHalf;
+    auto issue = [&](size_t i) {
+        const int h = (int)(i & 1);
+        const size_t off = i * kRingHalf, len = std::min(kRingHalf, n - off);
+        HIPCHECK(hipEventSynchronize(W.ring_ev[h]));  // an upload may still read it
+        HIPCHECK(hipMemcpyAsync(W.ring + h * kRingHalf, (const char*)src + off, len,
+                                hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipEventRecord(W.ring_ev[h], s));
+    };
+    if (nch) issue(0);
+    for (size_t i = 0; i < nch; ++i) {
+        // the next chunk lands in the other half (emptied by the host in
+        // the previous iteration) while this one is copied out
+        if (i + 1 < nch) issue(i + 1);
+        const int h = (int)(i & 1);
+        const size_t off = i * kRingHalf, len = std::min(kRingHalf, n - off);
+        HIPCHECK(hipEventSynchronize(W.ring_ev[h]));
+        parallel_copy((char*)dst + off, W.ring + h * kRingHalf, len);
+    }
+}
+
 // host-array entry (fgt2d / fgt3d): copies in, runs on the library's own
 // stream, copies the requested outputs back
 // charge == nullptr: no charges; rnormal == nullptr: no dipoles
@@ -2458,14 +2578,14 @@ void host_transform(Workspace& W, int dim, int nd, double delta, double eps, int
     double* d_rn = rnormal ? W.get<double>("in_rn", nps) : nullptr;
     double* d_ds = rnormal ? W.get<double>("in_ds", (size_t)nd * ns) : nullptr;
     double* d_trg = W.get<double>("in_trg", std::max<size_t>(npt, 3));
-    HIPCHECK(hipMemcpyAsync(d_src, sources, nps * 8, hipMemcpyHostToDevice, s));
-    if (charge)
-        HIPCHECK(hipMemcpyAsync(d_chg, charge, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+    if (g_staged) ring_init(W, s);
+    staged_h2d(W, d_src, sources, nps * 8, s);
+    if (nt > 0) staged_h2d(W, d_trg, targ, npt * 8, s);
+    if (charge) staged_h2d(W, d_chg, charge, (size_t)nd * ns * 8, s);
     if (rnormal) {
-        HIPCHECK(hipMemcpyAsync(d_rn, rnormal, nps * 8, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d_ds, dipstr, (size_t)nd * ns * 8, hipMemcpyHostToDevice, s));
+        staged_h2d(W, d_rn, rnormal, nps * 8, s);
+        staged_h2d(W, d_ds, dipstr, (size_t)nd * ns * 8, s);
     }
-    if (nt > 0) HIPCHECK(hipMemcpyAsync(d_trg, targ, npt * 8, hipMemcpyHostToDevice, s));
     const size_t wps = ifpgh > 0 ? (size_t)ns : 0, wpt = ifpghtarg > 0 ? (size_t)nt : 0;
     double* o_p = W.get<double>("out_pot", std::max<size_t>(nd * wps, 1));
     double* o_g = W.get<double>("out_grad", std::max<size_t>(ng * nd * wps, 1));
@@ -2481,15 +2601,12 @@ void host_transform(Workspace& W, int dim, int nd, double delta, double eps, int
         run2d(W, q, s);
     else
         run3d(W, q, s);
-    if (ifpgh >= 1) HIPCHECK(hipMemcpyAsync(pot, o_p, nd * wps * 8, hipMemcpyDeviceToHost, s));
-    if (ifpgh >= 2) HIPCHECK(hipMemcpyAsync(grad, o_g, ng * nd * wps * 8, hipMemcpyDeviceToHost, s));
-    if (ifpgh >= 3) HIPCHECK(hipMemcpyAsync(hess, o_h, nh * nd * wps * 8, hipMemcpyDeviceToHost, s));
-    if (ifpghtarg >= 1 && nt > 0)
-        HIPCHECK(hipMemcpyAsync(pottarg, o_pt, nd * wpt * 8, hipMemcpyDeviceToHost, s));
-    if (ifpghtarg >= 2 && nt > 0)
-        HIPCHECK(hipMemcpyAsync(gradtarg, o_gt, ng * nd * wpt * 8, hipMemcpyDeviceToHost, s));
-    if (ifpghtarg >= 3 && nt > 0)
-        HIPCHECK(hipMemcpyAsync(hesstarg, o_ht, nh * nd * wpt * 8, hipMemcpyDeviceToHost, s));
+    if (ifpgh >= 1) staged_d2h(W, pot, o_p, nd * wps * 8, s);
+    if (ifpgh >= 2) staged_d2h(W, grad, o_g, ng * nd * wps * 8, s);
+    if (ifpgh >= 3) staged_d2h(W, hess, o_h, nh * nd * wps * 8, s);
+    if (ifpghtarg >= 1 && nt > 0) staged_d2h(W, pottarg, o_pt, nd * wpt * 8, s);
+    if (ifpghtarg >= 2 && nt > 0) staged_d2h(W, gradtarg, o_gt, ng * nd * wpt * 8, s);
+    if (ifpghtarg >= 3 && nt > 0) staged_d2h(W, hesstarg, o_ht, nh * nd * wpt * 8, s);
     HIPCHECK(hipStreamSynchronize(s));
 }
 
@@ -2792,9 +2909,23 @@ int fgtpw_last_stats(int* out, int n)
     return k;
 }
 
+int fgtpw_set_precision(int mode)
+{
+    if (mode != 0 && mode != 1) {
+        t_err = "precision mode must be 0 (FP64) or 1 (mixed)";
+        return FGTPW_EINVAL;
+    }
+    const int prev = g_precision;
+    g_precision = mode;
+    return prev;
+}
+
+int fgtpw_last_precision(void) { return t_mixed ? 1 : 0; }
+
 int fgtpw_set_timing(int on)
 {
     g_timing = on != 0;
+    g_serial = on == 2 || g_serial_env;
     return FGTPW_OK;
 }
 

@@ -3,10 +3,12 @@
 // include/fgtpw.h.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace fgtpw {
@@ -31,13 +33,43 @@ enum KernelId {
     KID_TAYLOR_EVAL, KID_PW_EVAL, KID_DIRECT, KID_GATHER, KID_SCATTER, KID_N
 };
 struct KernelClock {
+    // events recorded on the stream before (end = false) and after the
+    // launches of kernel `id` that follow
     virtual void mark(int id, bool end, hipStream_t s) = 0;
+    // a pair of events for one launch of kernel `id`, stamped by the
+    // dispatch itself at the kernel's start and end (hipExtLaunchKernel)
+    virtual void pair(int id, hipEvent_t* e0, hipEvent_t* e1) = 0;
     virtual ~KernelClock() = default;
 };
 extern thread_local KernelClock* t_kclock;
 inline void kmark(int id, bool end, hipStream_t s)
 {
     if (t_kclock) t_kclock->mark(id, end, s);
+}
+
+// Launch of an instrumented kernel: with a clock installed, the dispatch
+// stamps a start and an end event of its own (the kernel's own duration,
+// without the gap before it when the queue ran dry); otherwise a plain
+// launch.
+template <class... P, class... A>
+void klaunch(int id, void (*f)(P...), dim3 grid, dim3 block, unsigned shm, hipStream_t s,
+             A... a)
+{
+    if (!t_kclock) {
+        f<<<grid, block, shm, s>>>(a...);
+        return;
+    }
+    hipEvent_t e0, e1;
+    t_kclock->pair(id, &e0, &e1);
+    // the kernel's own parameter types, so that the argument buffer holds
+    // exactly what the kernel reads
+    auto args = std::tuple<P...>{a...};
+    void* ptrs[sizeof...(P)];
+    std::apply([&](auto&... v) {
+        int k = 0;
+        ((ptrs[k++] = (void*)&v), ...);
+    }, args);
+    HIPCHECK(hipExtLaunchKernel((const void*)f, grid, block, ptrs, shm, s, e0, e1, 0));
 }
 
 // ---------------------------------------------------------------------------
@@ -91,6 +123,8 @@ inline void build_tree2d(Tree2D& T, const std::vector<LeafRun>& leaves, int nlma
     build_tree<2>(T, leaves, nlmax, npwlevel, bs0, cen0);
 }
 void planewave_nodes3(double pmax, int npw, double* ws, double* ts);
+// host memcpy split over the worker threads of the tree builder
+void parallel_copy(void* dst, const void* src, size_t n);
 
 // ---------------------------------------------------------------------------
 // device kernels (kernels2d.hip)
@@ -167,6 +201,9 @@ struct FormArgs {
     int pstride;
     const int* segbox;      // list index of the box of every segment (or nullptr)
     const double* ws;       // dipole weights d n, (n, nd, 2) sorted, or nullptr
+    // 1: FP32 moments on matrix cores (FP32 operands from FP64 offsets, FP64
+    // sums across segments) and FP32 plane-wave output (mp holds floats)
+    int prec = 0;
 };
 void launch_form(const FormArgs& a, hipStream_t s);
 
@@ -187,16 +224,18 @@ struct SegLists {
 };
 void launch_seg_scan(const SegLists& L, hipStream_t s);
 
+// f32: the expansions (mp, loc) and factor tables (wms, wm2l) hold floats
+// (complex interleaved) instead of doubles; the pointers are reinterpreted
 void launch_merge(const int* parents, int np, const int4* child, const double* cen,
                   const int2* src_se, const int* slot, const double* wms, int kidx_base,
-                  int nd, int nexp, double* mp, hipStream_t s);
+                  int nd, int nexp, double* mp, hipStream_t s, bool f32 = false);
 void launch_m2l(const int* boxes, int nb, const double* cen, const int* ncol, const int* col,
                 const int2* src_se, const int2* trg_se, int use_src, int use_trg,
                 const int* slot, const double* wm2l, double xmin, int do_list, int nd, int nexp,
-                const double* mp, double* loc, hipStream_t s);
+                const double* mp, double* loc, hipStream_t s, bool f32 = false);
 void launch_split(const int* children, int nc, const int* par, const double* cen,
                   const int* slot, const double* wms, int kidx_base, int nd, int nexp,
-                  double* loc, hipStream_t s);
+                  double* loc, hipStream_t s, bool f32 = false);
 
 struct EvalArgs {
     const int* boxes;
@@ -219,6 +258,10 @@ struct EvalArgs {
     const int* tstart;
     int seg, grid;
     const int* segbox;
+    int exp32 = 0;          // 1: loc holds FP32 expansions
+    int tay32 = 0;          // 1: FP32 Taylor coefficients, evaluated on FP32 matrix cores
+    int fused = 0;          // 1: the evaluation forms the Taylor coefficients itself
+                            //    (no k_pw2local launch, a.taylor unused)
 };
 void launch_pw2local(const EvalArgs& a, hipStream_t s);
 void launch_taylor_eval(const EvalArgs& a, hipStream_t s);

@@ -399,6 +399,8 @@ void launch_box_ranges(const int* rank_lo, const int* rank_hi, int nb, const int
     k_box_ranges<<<grid_for(nb, 256), 256, 0, s>>>(rank_lo, rank_hi, nb, ls, se, ls2, se2);
 }
 
+// V2: the points are 16-byte aligned and move as one 16-byte access each
+template <bool V2>
 __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ xy,
                                                 const double* __restrict__ q, int nd,
                                                 const unsigned* __restrict__ perm, int n,
@@ -407,10 +409,18 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ xy,
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     long j = perm[i];
-    xys[2 * (long)i] = xy[2 * j];
-    xys[2 * (long)i + 1] = xy[2 * j + 1];
-    if (q)
-        for (int d = 0; d < nd; ++d) qs[(long)nd * i + d] = q[(long)nd * j + d];
+    if (V2) {
+        ((double2*)xys)[i] = ((const double2*)xy)[j];
+    } else {
+        xys[2 * (long)i] = xy[2 * j];
+        xys[2 * (long)i + 1] = xy[2 * j + 1];
+    }
+    if (q) {
+        if (nd == 1)
+            qs[i] = q[j];
+        else
+            for (int d = 0; d < nd; ++d) qs[(long)nd * i + d] = q[(long)nd * j + d];
+    }
 }
 
 // dipole weights in sorted order: w(c, ind, i) = rnormal(c, perm i) dipstr(ind, perm i)
@@ -442,9 +452,10 @@ void launch_gather(const double* xy, const double* q, int nd, const unsigned* pe
                    double* xy_s, double* q_s, hipStream_t s)
 {
     if (n <= 0) return;
-    kmark(KID_GATHER, false, s);
-    k_gather<<<grid_for(n, 256), 256, 0, s>>>(xy, q, nd, perm, n, xy_s, q_s);
-    kmark(KID_GATHER, true, s);
+    if (((uintptr_t)xy & 15) == 0 && ((uintptr_t)xy_s & 15) == 0)
+        klaunch(KID_GATHER, k_gather<true>, grid_for(n, 256), 256, 0, s, xy, q, nd, perm, n, xy_s, q_s);
+    else
+        klaunch(KID_GATHER, k_gather<false>, grid_for(n, 256), 256, 0, s, xy, q, nd, perm, n, xy_s, q_s);
 }
 
 __global__ __launch_bounds__(256) void k_zero(double* __restrict__ p, size_t n)
@@ -500,9 +511,7 @@ void launch_scatter(const double* v_s, const unsigned* perm, int n, int w, doubl
                     hipStream_t s)
 {
     if (n <= 0) return;
-    kmark(KID_SCATTER, false, s);
-    k_scatter<<<grid_for((long)n * w, 256), 256, 0, s>>>(v_s, perm, n, w, v);
-    kmark(KID_SCATTER, true, s);
+    klaunch(KID_SCATTER, k_scatter, grid_for((long)n * w, 256), 256, 0, s, v_s, perm, n, w, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -724,7 +733,7 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 template <class F>
 static void allow_lds(F* kernel, size_t bytes)
 {
-    if (bytes > 65536)
+    if (bytes >= 65536)
         HIPCHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)bytes));
 }
@@ -958,6 +967,137 @@ __global__ __launch_bounds__(256) void k_form_wave(FormArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 matrix cores.  v_mfma_f32_16x16x4f32: the operands take the same lane
+// layout as the FP64 instruction (lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]); lane l holds D[4 (l/16) + r][l%16] in element r of the
+// result (rows of a lane contiguous, unlike the FP64 form).
+// ---------------------------------------------------------------------------
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4_t mfma16x16x4f32(float a, float b, f4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Step 1a with FP32 operands (the mixed-precision route): k_form_wave with
+// the offsets xh, yh formed in FP64 from the FP64 box centre, their powers
+// (and charge and dipole factors) computed in FP64 and rounded once to FP32
+// in the LDS tables, FP32 accumulation of one segment (at most WSEG
+// sources) on the matrix cores, and the segment's partial moments stored in
+// FP64 (summed across segments in FP64 by k_form_h2pw).
+template <int TM, bool DIP>
+__global__ __launch_bounds__(256) void k_form_wave_f32(FormArgs a)
+{
+    extern __shared__ float smf[];
+    constexpr int MP = 16 * TM, RS = MP + 1, CH = 32;
+    constexpr int NT = DIP ? 4 : 2;
+    const int m = a.p + 1, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const int s_l = lane >> 4, e_l = lane & 15, u = lane & 31, side = lane >> 5;
+    float* Px = smf + (size_t)w * NT * CH * RS;
+    float* Py = Px + CH * RS;
+    float* Pxd = Py + CH * RS;
+    float* Pyw = Pxd + CH * RS;
+    float* mine = (side ? Py : Px) + u * RS;
+    float* mine2 = (side ? Pyw : Pxd) + u * RS;
+    const int total = seg_count(a.tstart, a.nbox);
+    for (int g = blockIdx.x * nw + w; g < total; g += gridDim.x * nw) {
+        const SegRange r = seg_at(a.tstart, a.nbox, a.boxes, a.src_se, a.seg, g, a.segbox);
+        const int box = a.boxes[r.li], n = r.p1 - r.p0;
+        const double c = a.cen[2 * box + side], dsq = a.dsq;
+        f4_t acc[TM][TM];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TM; ++j) acc[i][j] = (f4_t){0, 0, 0, 0};
+        double v = 0, f = 0, g1 = 0, f2 = 0, g2 = 0;
+        auto load = [&](int k, double& v_, double& f_, double& g_, double& f2_, double& g2_) {
+            v_ = f_ = g_ = f2_ = g2_ = 0;
+            if (k < n) {
+                const long i = r.p0 + k;
+                v_ = (a.xs[2 * i + side] - c) * dsq;
+                f_ = side ? a.qs[(long)a.nd * i + a.ind] : 1.0;
+                if (DIP) {
+                    const double* wv = a.ws + 2 * ((long)a.nd * i + a.ind);
+                    g_ = side ? dsq * wv[1] : 0.0;
+                    f2_ = side ? wv[0] : 0.0;
+                    g2_ = side ? 0.0 : dsq;
+                }
+            }
+        };
+        load(u, v, f, g1, f2, g2);
+        for (int c0 = 0; c0 < n; c0 += CH) {
+            double vn, fn, gn, f2n, g2n;
+            load(c0 + CH + u, vn, fn, gn, f2n, g2n);
+            wave_lds_sync();
+            if (!DIP) {
+                double pw = f;
+#pragma unroll
+                for (int e = 0; e < MP; ++e) {
+                    mine[e] = (float)pw;
+                    pw *= v;
+                }
+            } else {
+                double pw = 1.0, prev = 0.0;
+#pragma unroll
+                for (int e = 0; e < MP; ++e) {
+                    mine[e] = (float)(f * pw + g1 * e * prev);
+                    mine2[e] = (float)(f2 * pw + g2 * e * prev);
+                    prev = pw;
+                    pw *= v;
+                }
+            }
+            wave_lds_sync();
+            const int nc = min(CH, n - c0);
+            for (int gg = 0; 4 * gg < nc; ++gg) {
+                const int src = 4 * gg + s_l;
+                float av[TM], bv[TM];
+#pragma unroll
+                for (int t = 0; t < TM; ++t) {
+                    av[t] = Px[src * RS + e_l + 16 * t];
+                    bv[t] = Py[src * RS + e_l + 16 * t];
+                }
+#pragma unroll
+                for (int ii = 0; ii < TM; ++ii)
+#pragma unroll
+                    for (int jj = 0; jj < TM; ++jj)
+                        acc[ii][jj] = mfma16x16x4f32(av[ii], bv[jj], acc[ii][jj]);
+                if (DIP) {
+#pragma unroll
+                    for (int t = 0; t < TM; ++t) {
+                        av[t] = Pxd[src * RS + e_l + 16 * t];
+                        bv[t] = Pyw[src * RS + e_l + 16 * t];
+                    }
+#pragma unroll
+                    for (int ii = 0; ii < TM; ++ii)
+#pragma unroll
+                        for (int jj = 0; jj < TM; ++jj)
+                            acc[ii][jj] = mfma16x16x4f32(av[ii], bv[jj], acc[ii][jj]);
+                }
+            }
+            v = vn;
+            f = fn;
+            g1 = gn;
+            f2 = f2n;
+            g2 = g2n;
+        }
+        double* out = a.partial + (long)g * a.pstride;
+#pragma unroll
+        for (int ii = 0; ii < TM; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < TM; ++jj)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int xo = 4 * s_l + q + 16 * ii, yo = e_l + 16 * jj;
+                    if (xo < m && yo < m) out[yo + m * xo] = (double)acc[ii][jj][q];
+                }
+    }
+}
+
+// E: element type of the plane-wave expansions written (double, or float
+// on the mixed-precision route); the conversion itself runs in FP64.
+template <class E>
 __global__ __launch_bounds__(256) void k_form_h2pw(FormArgs a)
 {
     extern __shared__ double sm[];
@@ -988,7 +1128,7 @@ __global__ __launch_bounds__(256) void k_form_h2pw(FormArgs a)
         T[2 * o + 1] = im;
     }
     __syncthreads();
-    double* out = a.mp + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
+    E* out = (E*)a.mp + 2 * ((long)a.slot[box] * a.nd + a.ind) * nexp;
     for (int o = t; o < nexp; o += 256) {
         int aa = o % half, b = o / half;
         double re = 0, im = 0;
@@ -998,9 +1138,158 @@ __global__ __launch_bounds__(256) void k_form_h2pw(FormArgs a)
             re += wr * tr - wi * ti;
             im += wr * ti + wi * tr;
         }
-        out[2 * o] = re;
-        out[2 * o + 1] = im;
+        out[2 * o] = (E)re;
+        out[2 * o + 1] = (E)im;
     }
+}
+
+// Step 1b on matrix cores, one wavefront per leaf (four leaves per block):
+// the segment partials of the leaf summed in segment order into
+// H(j,k) = sum_s xh_s^j q_s yh_s^k, then PW = h2xr^T H h2xr as
+// g2dh2pw_real_vec (g2drouts.f:1242-1329) computes it.  With
+// h2xr(j,a) = i^j r(j,a), r real (ws_a (-ts_a)^j / j!), the complex
+// product splits by (j + k) mod 4:
+//   PW(a,b) = sum_{j,k} i^(j+k) r(j,a) H(j,k) r(k,b)
+//           = [R^T G_re R](a,b) + i [R^T G_im R](a,b),
+//   G_re = H on j+k = 0 mod 4, -H on 2;  G_im = H on 1, -H on 3.
+// Y = G R (rows j, columns b) runs first; its accumulators are the B
+// operands of R^T Y as they stand (lane l holds Y(l/16 + 4 r + 16 t, .),
+// which K step r + 4 t of the second product reads from lane l).
+// TM: 16-row tiles of the order (m <= 16 TM); NB: 16-column tiles of b
+// (npw <= 16 NB); HT: 16-row tiles of a (npw/2 <= 16 HT).
+template <int TM, int NB, int HT, class E>
+__global__ __launch_bounds__(256) void k_form_h2pw_wave(FormArgs a)
+{
+    constexpr int MP = 16 * TM;
+    extern __shared__ double smh[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = blockIdx.x * 4 + w;
+    if (li >= a.nbox) return;
+    const int g0 = a.tstart[li], g1 = a.tstart[li + 1];
+    if (g1 <= g0) return;  // no sources: the expansion stays zero
+    const int box = a.boxes[li];
+    const int m = a.p + 1, npw = a.npw, half = npw / 2;
+    const int c16 = lane & 15, q4 = lane >> 4;
+    double* Gre = smh + (size_t)w * 2 * MP * MP;  // G(j,k) at j + MP k
+    double* Gim = Gre + MP * MP;
+    // H(j,k): partials are y order fastest (k + m j); segments in order
+    for (int o = lane; o < MP * MP; o += 64) {
+        const int j = o % MP, k = o / MP;
+        double h = 0;
+        if (j < m && k < m)
+            for (int g = g0; g < g1; ++g) h += a.partial[(long)g * a.pstride + k + m * j];
+        const int q = (j + k) & 3;
+        Gre[o] = q == 0 ? h : (q == 2 ? -h : 0.0);
+        Gim[o] = q == 1 ? h : (q == 3 ? -h : 0.0);
+    }
+    wave_lds_sync();
+    // r(j,a) from the complex table h2xr(j,a) = i^j r(j,a)
+    auto rtab = [&](int j, int aa) -> double {
+        const double* h = a.h2xr + 2 * ((long)j * npw + aa);
+        switch (j & 3) {
+        case 0: return h[0];
+        case 1: return h[1];
+        case 2: return -h[0];
+        default: return -h[1];
+        }
+    };
+    // Y(j,b) = sum_k G(j,k) r(k,b), re and im
+    d4_t Y[TM][NB][2];
+#pragma unroll
+    for (int t = 0; t < TM; ++t)
+#pragma unroll
+        for (int bt = 0; bt < NB; ++bt) Y[t][bt][0] = Y[t][bt][1] = (d4_t){0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4 * TM; ++s) {
+        const int k = 4 * s + q4;
+        double bv[NB];
+#pragma unroll
+        for (int bt = 0; bt < NB; ++bt) {
+            const int b = c16 + 16 * bt;
+            bv[bt] = (k < m && b < npw) ? rtab(k, b) : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < TM; ++t) {
+            const int j = c16 + 16 * t;
+            const double ar = Gre[j + MP * k], ai = Gim[j + MP * k];
+#pragma unroll
+            for (int bt = 0; bt < NB; ++bt) {
+                Y[t][bt][0] = mfma16x16x4(ar, bv[bt], Y[t][bt][0]);
+                Y[t][bt][1] = mfma16x16x4(ai, bv[bt], Y[t][bt][1]);
+            }
+        }
+    }
+    // PW(a,b) = sum_j r(j,a) Y(j,b), a < half
+    d4_t P[HT][NB][2];
+#pragma unroll
+    for (int ht = 0; ht < HT; ++ht)
+#pragma unroll
+        for (int bt = 0; bt < NB; ++bt) P[ht][bt][0] = P[ht][bt][1] = (d4_t){0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < TM; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = 16 * t + 4 * r + q4;  // K index of step r + 4 t in this lane
+            double av[HT];
+#pragma unroll
+            for (int ht = 0; ht < HT; ++ht) {
+                const int aa = c16 + 16 * ht;
+                av[ht] = (j < m && aa < half) ? rtab(j, aa) : 0.0;
+            }
+#pragma unroll
+            for (int ht = 0; ht < HT; ++ht)
+#pragma unroll
+                for (int bt = 0; bt < NB; ++bt) {
+                    P[ht][bt][0] = mfma16x16x4(av[ht], Y[t][bt][0][r], P[ht][bt][0]);
+                    P[ht][bt][1] = mfma16x16x4(av[ht], Y[t][bt][1][r], P[ht][bt][1]);
+                }
+        }
+    E* out = (E*)a.mp + 2 * ((long)a.slot[box] * a.nd + a.ind) * (long)(half * npw);
+#pragma unroll
+    for (int ht = 0; ht < HT; ++ht)
+#pragma unroll
+        for (int bt = 0; bt < NB; ++bt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int aa = q4 + 4 * r + 16 * ht, b = c16 + 16 * bt;
+                if (aa < half && b < npw) {
+                    out[2 * (aa + half * b)] = (E)P[ht][bt][0][r];
+                    out[2 * (aa + half * b) + 1] = (E)P[ht][bt][1][r];
+                }
+            }
+}
+
+template <int TM, int NB, int HT>
+static void h2pw_wave_launch(const FormArgs& a, hipStream_t s)
+{
+    const unsigned lds = 4 * 2 * (16 * TM) * (16 * TM) * sizeof(double);
+    const unsigned grid = (unsigned)((a.nbox + 3) / 4);
+    if (a.prec) {
+        allow_lds(k_form_h2pw_wave<TM, NB, HT, float>, lds);
+        klaunch(KID_FORM_H2PW, k_form_h2pw_wave<TM, NB, HT, float>, grid, 256, lds, s, a);
+    } else {
+        allow_lds(k_form_h2pw_wave<TM, NB, HT, double>, lds);
+        klaunch(KID_FORM_H2PW, k_form_h2pw_wave<TM, NB, HT, double>, grid, 256, lds, s, a);
+    }
+}
+
+// the matrix-core Hermite -> PW for orders below 32 and npw <= 64; false
+// when it does not apply
+static bool launch_h2pw_wave(const FormArgs& a, hipStream_t s)
+{
+    const int m = a.p + 1, nb = (a.npw + 15) / 16, ht = (a.npw / 2 + 15) / 16;
+    if (m > 32 || a.npw > 64 || !a.tstart) return false;
+    const int tm = (m + 15) / 16;
+#define FGT_H2PW(T_, B_, H_) \
+    if (tm == T_ && nb <= B_ && ht <= H_) { h2pw_wave_launch<T_, B_, H_>(a, s); return true; }
+    FGT_H2PW(1, 1, 1)
+    FGT_H2PW(1, 2, 1)
+    FGT_H2PW(1, 4, 2)
+    FGT_H2PW(2, 1, 1)
+    FGT_H2PW(2, 2, 1)
+    FGT_H2PW(2, 4, 2)
+#undef FGT_H2PW
+    return false;
 }
 
 void launch_form(const FormArgs& a, hipStream_t s)
@@ -1009,16 +1298,21 @@ void launch_form(const FormArgs& a, hipStream_t s)
     int m = a.p + 1;
     if (m <= 64 && a.tstart) {
         const int TM = (m + 15) / 16, MP = 16 * TM;
-        kmark(KID_FORM_MOMENTS, false, s);
         if (a.segbox && a.seg <= WSEG) {
             // per wave: two (four with dipoles) 32 x (MP + 1) power tables
             const bool dip = a.ws != nullptr;
             const int nwave = dip && TM >= 3 ? 2 : 4;
-            size_t lds = (size_t)nwave * (dip ? 4 : 2) * 32 * (MP + 1) * sizeof(double);
+            size_t lds = (size_t)nwave * (dip ? 4 : 2) * 32 * (MP + 1) *
+                         (a.prec ? sizeof(float) : sizeof(double));
             const int grid = (int)std::min<long>(2L * a.grid, (long)a.grid * 4 / nwave);
 #define FGT_FORM_WAVE(T_, D_)                                                                  \
-    allow_lds(k_form_wave<T_, D_>, lds);                                                       \
-    k_form_wave<T_, D_><<<grid, 64 * nwave, lds, s>>>(a)
+    if (a.prec) {                                                                              \
+        allow_lds(k_form_wave_f32<T_, D_>, lds);                                               \
+        klaunch(KID_FORM_MOMENTS, k_form_wave_f32<T_, D_>, grid, 64 * nwave, lds, s, a);                              \
+    } else {                                                                                   \
+        allow_lds(k_form_wave<T_, D_>, lds);                                                   \
+        klaunch(KID_FORM_MOMENTS, k_form_wave<T_, D_>, grid, 64 * nwave, lds, s, a);                                  \
+    }
             if (!dip) {
                 switch (TM) {
                 case 1: FGT_FORM_WAVE(1, false); break;
@@ -1038,20 +1332,25 @@ void launch_form(const FormArgs& a, hipStream_t s)
         } else {
             size_t lds = std::max((size_t)2 * a.seg * (MP + 1), (size_t)MP * MP) * sizeof(double);
             switch (TM) {
-            case 1: allow_lds(k_form_moments<1>, lds); k_form_moments<1><<<a.grid, 256, lds, s>>>(a); break;
-            case 2: allow_lds(k_form_moments<2>, lds); k_form_moments<2><<<a.grid, 256, lds, s>>>(a); break;
-            case 3: allow_lds(k_form_moments<3>, lds); k_form_moments<3><<<a.grid, 256, lds, s>>>(a); break;
-            default: allow_lds(k_form_moments<4>, lds); k_form_moments<4><<<a.grid, 256, lds, s>>>(a); break;
+            case 1: allow_lds(k_form_moments<1>, lds); klaunch(KID_FORM_MOMENTS, k_form_moments<1>, a.grid, 256, lds, s, a); break;
+            case 2: allow_lds(k_form_moments<2>, lds); klaunch(KID_FORM_MOMENTS, k_form_moments<2>, a.grid, 256, lds, s, a); break;
+            case 3: allow_lds(k_form_moments<3>, lds); klaunch(KID_FORM_MOMENTS, k_form_moments<3>, a.grid, 256, lds, s, a); break;
+            default: allow_lds(k_form_moments<4>, lds); klaunch(KID_FORM_MOMENTS, k_form_moments<4>, a.grid, 256, lds, s, a); break;
             }
         }
-        kmark(KID_FORM_MOMENTS, true, s);
         size_t lds2 = ((size_t)m * m + 2 * (size_t)m * a.npw + (size_t)a.npw * m) * sizeof(double);
-        allow_lds(k_form_h2pw, lds2);
-        kmark(KID_FORM_H2PW, false, s);
-    k_form_h2pw<<<a.nbox, 256, lds2, s>>>(a);
-    kmark(KID_FORM_H2PW, true, s);
+        if (launch_h2pw_wave(a, s)) {
+            // matrix-core conversion, one wavefront per leaf
+        } else if (a.prec) {
+            allow_lds(k_form_h2pw<float>, lds2);
+            klaunch(KID_FORM_H2PW, k_form_h2pw<float>, a.nbox, 256, lds2, s, a);
+        } else {
+            allow_lds(k_form_h2pw<double>, lds2);
+            klaunch(KID_FORM_H2PW, k_form_h2pw<double>, a.nbox, 256, lds2, s, a);
+        }
         return;
     }
+    if (a.prec) fail(-3, "FP32 expansions need the segmented formation with Hermite order < 64");
     const bool dip = a.ws != nullptr;
     int chunk = (dip ? 2048 : 4096) / m;
     if (chunk > 256) chunk = 256;
@@ -1060,14 +1359,13 @@ void launch_form(const FormArgs& a, hipStream_t s)
     size_t lds2 = ((size_t)m * m + (size_t)a.npw * m) * sizeof(double);
     if (lds2 > lds) lds = lds2;
     int ne = (m * m + 255) / 256;
-    kmark(KID_FORM_VALU, false, s);
 #define FGT_FORM_VALU(E_)                                                                      \
     if (dip) {                                                                                 \
         allow_lds(k_form<E_, true>, lds);                                                      \
-        k_form<E_, true><<<a.nbox, 256, lds, s>>>(a, chunk);                                   \
+        klaunch(KID_FORM_VALU, k_form<E_, true>, a.nbox, 256, lds, s, a, chunk);                                   \
     } else {                                                                                   \
         allow_lds(k_form<E_, false>, lds);                                                     \
-        k_form<E_, false><<<a.nbox, 256, lds, s>>>(a, chunk);                                  \
+        klaunch(KID_FORM_VALU, k_form<E_, false>, a.nbox, 256, lds, s, a, chunk);                                  \
     }
     if (ne <= 1) {
         FGT_FORM_VALU(1)
@@ -1083,7 +1381,6 @@ void launch_form(const FormArgs& a, hipStream_t s)
         fail(-3, "Hermite order too large for the formation kernel");
     }
 #undef FGT_FORM_VALU
-    kmark(KID_FORM_VALU, true, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1097,13 +1394,16 @@ __device__ __forceinline__ int quadrant(double dx, double dy)
     return dx > 0 ? (dy > 0 ? 0 : 1) : (dy > 0 ? 2 : 3);
 }
 
+// E: element type of the expansions and factor tables (double; float on
+// the mixed-precision route), complex interleaved.
+template <class E>
 __global__ __launch_bounds__(256) void k_merge(const int* __restrict__ parents, int np,
                                                const int4* __restrict__ child,
                                                const double* __restrict__ cen,
                                                const int2* __restrict__ src_se,
                                                const int* __restrict__ slot,
-                                               const double* __restrict__ wms, int kbase, int nd,
-                                               int nexp, double* __restrict__ mp)
+                                               const E* __restrict__ wms, int kbase, int nd,
+                                               int nexp, E* __restrict__ mp)
 {
     long t = blockIdx.x * 256L + threadIdx.x;
     int pi = (int)(t / nexp), e = (int)(t - (long)pi * nexp);
@@ -1112,16 +1412,16 @@ __global__ __launch_bounds__(256) void k_merge(const int* __restrict__ parents, 
     int4 c4 = child[p];
     int cs[4] = {c4.x, c4.y, c4.z, c4.w};
     for (int ind = 0; ind < nd; ++ind) {
-        double* dst = mp + 2 * (((long)slot[p] * nd + ind) * nexp + e);
-        double re = dst[0], im = dst[1];
+        E* dst = mp + 2 * (((long)slot[p] * nd + ind) * nexp + e);
+        E re = dst[0], im = dst[1];
         for (int j = 0; j < 4; ++j) {
             int c = cs[j];
             if (c < 0) continue;
             int2 r = src_se[c];
             if (r.y - r.x <= 0) continue;
             int k = quadrant(cen[2 * p] - cen[2 * c], cen[2 * p + 1] - cen[2 * c + 1]);
-            const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
-            const double* v = mp + 2 * (((long)slot[c] * nd + ind) * nexp + e);
+            const E* w = wms + 2 * ((long)(kbase + k) * nexp + e);
+            const E* v = mp + 2 * (((long)slot[c] * nd + ind) * nexp + e);
             re += v[0] * w[0] - v[1] * w[1];
             im += v[0] * w[1] + v[1] * w[0];
         }
@@ -1132,17 +1432,21 @@ __global__ __launch_bounds__(256) void k_merge(const int* __restrict__ parents, 
 
 void launch_merge(const int* parents, int np, const int4* child, const double* cen,
                   const int2* src_se, const int* slot, const double* wms, int kbase, int nd,
-                  int nexp, double* mp, hipStream_t s)
+                  int nexp, double* mp, hipStream_t s, bool f32)
 {
     if (np <= 0) return;
-    kmark(KID_MERGE, false, s);
-    k_merge<<<grid_for((long)np * nexp, 256), 256, 0, s>>>(parents, np, child, cen, src_se, slot,
-                                                           wms, kbase, nd, nexp, mp);
-    kmark(KID_MERGE, true, s);
+    const unsigned grid = grid_for((long)np * nexp, 256);
+    if (f32)
+        klaunch(KID_MERGE, k_merge<float>, grid, 256, 0, s, parents, np, child, cen, src_se, slot,
+                                            (const float*)wms, kbase, nd, nexp, (float*)mp);
+    else
+        klaunch(KID_MERGE, k_merge<double>, grid, 256, 0, s, parents, np, child, cen, src_se, slot, wms, kbase,
+                                             nd, nexp, mp);
 }
 
 // Step 3 (fgt2d.f:1026-1064): loc = sum over same-level neighbours j != self
 // of mp_j shifted by (jx, jy) boxes, then + mp_self.
+template <class E>
 __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ boxes, int nb,
                                              const double* __restrict__ cen,
                                              const int* __restrict__ ncol,
@@ -1150,10 +1454,10 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ boxes, int 
                                              const int2* __restrict__ src_se,
                                              const int2* __restrict__ trg_se, int use_src,
                                              int use_trg, const int* __restrict__ slot,
-                                             const double* __restrict__ wm2l, double xmin,
+                                             const E* __restrict__ wm2l, double xmin,
                                              int do_list, int nd, int nexp,
-                                             const double* __restrict__ mp,
-                                             double* __restrict__ loc)
+                                             const E* __restrict__ mp,
+                                             E* __restrict__ loc)
 {
     long t = blockIdx.x * 256L + threadIdx.x;
     int bi = (int)(t / nexp), e = (int)(t - (long)bi * nexp);
@@ -1164,7 +1468,7 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ boxes, int 
     if (use_src) np += src_se[b].y - src_se[b].x;
     if (np <= 0) return;
     for (int ind = 0; ind < nd; ++ind) {
-        double re = 0, im = 0;
+        E re = 0, im = 0;
         if (do_list)
             for (int i = 0; i < ncol[b]; ++i) {
                 int j = col[9 * b + i];
@@ -1173,15 +1477,15 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ boxes, int 
                 int jy = (int)lround((cen[2 * b + 1] - cen[2 * j + 1]) / xmin);
                 jx = jx < -1 ? -1 : (jx > 1 ? 1 : jx);  // colleagues are +-1 box away
                 jy = jy < -1 ? -1 : (jy > 1 ? 1 : jy);
-                const double* w = wm2l + 2 * ((long)((jx + 1) + 3 * (jy + 1)) * nexp + e);
-                const double* v = mp + 2 * (((long)slot[j] * nd + ind) * nexp + e);
+                const E* w = wm2l + 2 * ((long)((jx + 1) + 3 * (jy + 1)) * nexp + e);
+                const E* v = mp + 2 * (((long)slot[j] * nd + ind) * nexp + e);
                 re += v[0] * w[0] - v[1] * w[1];
                 im += v[0] * w[1] + v[1] * w[0];
             }
-        const double* v = mp + 2 * (((long)slot[b] * nd + ind) * nexp + e);
+        const E* v = mp + 2 * (((long)slot[b] * nd + ind) * nexp + e);
         re += v[0];
         im += v[1];
-        double* d = loc + 2 * (((long)slot[b] * nd + ind) * nexp + e);
+        E* d = loc + 2 * (((long)slot[b] * nd + ind) * nexp + e);
         d[0] = re;
         d[1] = im;
     }
@@ -1190,33 +1494,38 @@ __global__ __launch_bounds__(256) void k_m2l(const int* __restrict__ boxes, int 
 void launch_m2l(const int* boxes, int nb, const double* cen, const int* ncol, const int* col,
                 const int2* src_se, const int2* trg_se, int use_src, int use_trg,
                 const int* slot, const double* wm2l, double xmin, int do_list, int nd, int nexp,
-                const double* mp, double* loc, hipStream_t s)
+                const double* mp, double* loc, hipStream_t s, bool f32)
 {
     if (nb <= 0) return;
-    kmark(KID_M2L, false, s);
-    k_m2l<<<grid_for((long)nb * nexp, 256), 256, 0, s>>>(boxes, nb, cen, ncol, col, src_se,
-                                                         trg_se, use_src, use_trg, slot, wm2l,
-                                                         xmin, do_list, nd, nexp, mp, loc);
-    kmark(KID_M2L, true, s);
+    const unsigned grid = grid_for((long)nb * nexp, 256);
+    if (f32)
+        klaunch(KID_M2L, k_m2l<float>, grid, 256, 0, s, boxes, nb, cen, ncol, col, src_se, trg_se, use_src,
+                                          use_trg, slot, (const float*)wm2l, xmin, do_list, nd,
+                                          nexp, (const float*)mp, (float*)loc);
+    else
+        klaunch(KID_M2L, k_m2l<double>, grid, 256, 0, s, boxes, nb, cen, ncol, col, src_se, trg_se, use_src,
+                                           use_trg, slot, wm2l, xmin, do_list, nd, nexp, mp,
+                                           loc);
 }
 
 // Step 4 (fgt2d.f:1082-1125): loc(child) = loc(parent) shifted to the child.
+template <class E>
 __global__ __launch_bounds__(256) void k_split(const int* __restrict__ children, int nc,
                                                const int* __restrict__ par,
                                                const double* __restrict__ cen,
                                                const int* __restrict__ slot,
-                                               const double* __restrict__ wms, int kbase, int nd,
-                                               int nexp, double* __restrict__ loc)
+                                               const E* __restrict__ wms, int kbase, int nd,
+                                               int nexp, E* __restrict__ loc)
 {
     long t = blockIdx.x * 256L + threadIdx.x;
     int ci = (int)(t / nexp), e = (int)(t - (long)ci * nexp);
     if (ci >= nc) return;
     int c = children[ci], p = par[c];
     int k = quadrant(cen[2 * c] - cen[2 * p], cen[2 * c + 1] - cen[2 * p + 1]);
-    const double* w = wms + 2 * ((long)(kbase + k) * nexp + e);
+    const E* w = wms + 2 * ((long)(kbase + k) * nexp + e);
     for (int ind = 0; ind < nd; ++ind) {
-        const double* v = loc + 2 * (((long)slot[p] * nd + ind) * nexp + e);
-        double* d = loc + 2 * (((long)slot[c] * nd + ind) * nexp + e);
+        const E* v = loc + 2 * (((long)slot[p] * nd + ind) * nexp + e);
+        E* d = loc + 2 * (((long)slot[c] * nd + ind) * nexp + e);
         d[0] += v[0] * w[0] - v[1] * w[1];
         d[1] += v[0] * w[1] + v[1] * w[0];
     }
@@ -1224,28 +1533,34 @@ __global__ __launch_bounds__(256) void k_split(const int* __restrict__ children,
 
 void launch_split(const int* children, int nc, const int* par, const double* cen,
                   const int* slot, const double* wms, int kbase, int nd, int nexp, double* loc,
-                  hipStream_t s)
+                  hipStream_t s, bool f32)
 {
     if (nc <= 0) return;
-    kmark(KID_SPLIT, false, s);
-    k_split<<<grid_for((long)nc * nexp, 256), 256, 0, s>>>(children, nc, par, cen, slot, wms,
-                                                           kbase, nd, nexp, loc);
-    kmark(KID_SPLIT, true, s);
+    const unsigned grid = grid_for((long)nc * nexp, 256);
+    if (f32)
+        klaunch(KID_SPLIT, k_split<float>, grid, 256, 0, s, children, nc, par, cen, slot, (const float*)wms,
+                                            kbase, nd, nexp, (float*)loc);
+    else
+        klaunch(KID_SPLIT, k_split<double>, grid, 256, 0, s, children, nc, par, cen, slot, wms, kbase, nd, nexp,
+                                             loc);
 }
 
 // ---------------------------------------------------------------------------
 // Step 5a: plane wave -> Taylor (g2dpw2local_real_vec, g2drouts.f:1417-1540)
 //   lx(k,b)  = i^k sum_a x2l(a,k) PW(a,b)          a positive x nodes
 //   L(k,l)   = 2 Re[ i^l sum_{b<half} x2l(b,l) (lx(k,b) + (-1)^l lx(k,b+half)) ]
-// with x2l(a,k) = ts_a^k / k!.  One block per leaf.
+// with x2l(a,k) = ts_a^k / k!.  One block per leaf.  E: element type of
+// the expansions read, TT of the Taylor coefficients written (float on the
+// mixed-precision route); the arithmetic is FP64.
 // ---------------------------------------------------------------------------
+template <class E, class TT>
 __global__ __launch_bounds__(256) void k_pw2local(EvalArgs a)
 {
     extern __shared__ double sm[];
     int bi = blockIdx.x;
     int b = a.boxes[bi];
     const int npw = a.npw, half = npw / 2, nexp = half * npw, m = a.nl + 1, t = threadIdx.x;
-    const double* pw = a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
+    const E* pw = (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
     double* P = sm;              // complex nexp
     double* LX = sm + 2 * nexp;  // complex (m, npw): k fastest
     for (int o = t; o < 2 * nexp; o += 256) P[o] = pw[o];
@@ -1269,7 +1584,7 @@ __global__ __launch_bounds__(256) void k_pw2local(EvalArgs a)
         LX[2 * o + 1] = im;
     }
     __syncthreads();
-    double* L = a.taylor + (long)bi * m * m;
+    TT* L = (TT*)a.taylor + (long)bi * m * m;
     for (int o = t; o < m * m; o += 256) {
         int k = o % m, l = o / m;
         double sg = (l & 1) ? -1.0 : 1.0;
@@ -1286,7 +1601,7 @@ __global__ __launch_bounds__(256) void k_pw2local(EvalArgs a)
         case 2: v = -2 * cr; break;
         default: v = 2 * ci; break;
         }
-        L[o] = v;
+        L[o] = (TT)v;
     }
 }
 
@@ -1295,9 +1610,12 @@ void launch_pw2local(const EvalArgs& a, hipStream_t s)
     if (a.nbox <= 0) return;
     int half = a.npw / 2, m = a.nl + 1;
     size_t lds = (size_t)(2 * half * a.npw + 2 * m * a.npw) * sizeof(double);
-    kmark(KID_PW2LOCAL, false, s);
-    k_pw2local<<<a.nbox, 256, lds, s>>>(a);
-    kmark(KID_PW2LOCAL, true, s);
+    if (a.exp32 && a.tay32)
+        klaunch(KID_PW2LOCAL, k_pw2local<float, float>, a.nbox, 256, lds, s, a);
+    else if (a.exp32)
+        klaunch(KID_PW2LOCAL, k_pw2local<float, double>, a.nbox, 256, lds, s, a);
+    else
+        klaunch(KID_PW2LOCAL, k_pw2local<double, double>, a.nbox, 256, lds, s, a);
 }
 
 // Step 5b: Taylor evaluation (g2dlevalp/g/h_vec, g2drouts.f:847-1080) by
@@ -1489,16 +1807,145 @@ __global__ __launch_bounds__(256) void k_taylor_mfma(EvalArgs a)
     }
 }
 
+// Step 5a inside the evaluation wave: the Taylor coefficients of one leaf
+// from its plane-wave expansion on the wave's FP64 matrix cores (the
+// mathematics of g2dpw2local_real_vec, g2drouts.f:1417-1540, as k_pw2local):
+//   c(k,b)  = sum_{a<half} x2l(a,k) PW(a,b),   lx(k,b) = i^k c(k,b)
+//   L(k,l)  = 2 Re[ i^l sum_{b<half} x2l(b,l) (lx(k,b) + (-1)^l lx(k,b+half)) ]
+// With Er(k,b) = Re lx(k,b) + Re lx(k,b+half) and Oi(k,b) = Im lx(k,b) -
+// Im lx(k,b+half), L(k,l) = s sum_b x2l(b,l) Er(k,b) for even l (s = 2 at
+// l = 0 mod 4, -2 at 2) and s sum_b x2l(b,l) Oi(k,b) for odd l (s = -2 at 1,
+// 2 at 3).  The first product takes the columns b and b + half as two tiles
+// of the same lanes, so Er and Oi form lane by lane.  Writes L(k,l) (x order
+// k, y order l) to buf[k + MP l], zero beyond m; buf holds
+// PW2L_BUF(TM, NBH) doubles.  Every lane of the wave calls it.
+#define PW2L_BUF(TM, NBH) (2 * 16 * (TM) * 16 * (NBH) > 256 * (TM) * (TM) ? 2 * 16 * (TM) * 16 * (NBH) : 256 * (TM) * (TM))
+
+template <int TM, int NBH, class E>
+__device__ __forceinline__ void pw2local_wave(const EvalArgs& a, const E* __restrict__ pw,
+                                              double* buf, int lane)
+{
+    constexpr int MP = 16 * TM, BW = 16 * NBH;
+    const int m = a.nl + 1, npw = a.npw, half = npw / 2;
+    const int c16 = lane & 15, q4 = lane >> 4;
+    const double* __restrict__ x2l = a.x2l;  // x2l(a,k) at a + npw k
+    d4_t C[TM][NBH][4];  // [it][jt][lo re, lo im, hi re, hi im]
+#pragma unroll
+    for (int it = 0; it < TM; ++it)
+#pragma unroll
+        for (int jt = 0; jt < NBH; ++jt)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) C[it][jt][u] = (d4_t){0, 0, 0, 0};
+    for (int k0 = 0; k0 < half; k0 += 4) {
+        const int an = k0 + q4;  // the node of this lane's K index
+        double av[TM];
+#pragma unroll
+        for (int it = 0; it < TM; ++it) {
+            const int k = c16 + 16 * it;
+            av[it] = (an < half && k < m) ? x2l[an + npw * k] : 0.0;
+        }
+#pragma unroll
+        for (int jt = 0; jt < NBH; ++jt) {
+            const int bc = c16 + 16 * jt;
+            const bool ok = an < half && bc < half;
+            const E* p0 = pw + 2 * (an + half * bc);           // PW(a, b)
+            const E* p1 = pw + 2 * (an + half * (bc + half));  // PW(a, b + half)
+            const double b0r = ok ? (double)p0[0] : 0.0, b0i = ok ? (double)p0[1] : 0.0;
+            const double b1r = ok ? (double)p1[0] : 0.0, b1i = ok ? (double)p1[1] : 0.0;
+#pragma unroll
+            for (int it = 0; it < TM; ++it) {
+                C[it][jt][0] = mfma16x16x4(av[it], b0r, C[it][jt][0]);
+                C[it][jt][1] = mfma16x16x4(av[it], b0i, C[it][jt][1]);
+                C[it][jt][2] = mfma16x16x4(av[it], b1r, C[it][jt][2]);
+                C[it][jt][3] = mfma16x16x4(av[it], b1i, C[it][jt][3]);
+            }
+        }
+    }
+    double* Er = buf;            // Er(k,b) at k + MP b, b < BW
+    double* Oi = buf + MP * BW;  // Oi(k,b) likewise
+    wave_lds_sync();             // the previous segment's reads of buf are done
+#pragma unroll
+    for (int it = 0; it < TM; ++it)
+#pragma unroll
+        for (int jt = 0; jt < NBH; ++jt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = q4 + 4 * r + 16 * it, b = c16 + 16 * jt;
+                const double lr = C[it][jt][0][r], li = C[it][jt][1][r];
+                const double hr = C[it][jt][2][r], hi = C[it][jt][3][r];
+                double er, oi;  // i^k times lo and hi
+                switch (k & 3) {
+                case 0: er = lr + hr; oi = li - hi; break;
+                case 1: er = -li - hi; oi = lr - hr; break;
+                case 2: er = -lr - hr; oi = -li + hi; break;
+                default: er = li + hi; oi = -lr + hr; break;
+                }
+                Er[k + MP * b] = er;
+                Oi[k + MP * b] = oi;
+            }
+    wave_lds_sync();
+    d4_t P[2][TM][TM];
+#pragma unroll
+    for (int it = 0; it < TM; ++it)
+#pragma unroll
+        for (int lt = 0; lt < TM; ++lt) P[0][it][lt] = P[1][it][lt] = (d4_t){0, 0, 0, 0};
+    for (int b0 = 0; b0 < half; b0 += 4) {
+        const int bn = b0 + q4;
+        double ae[TM], ao[TM], bv[TM];
+#pragma unroll
+        for (int it = 0; it < TM; ++it) {
+            const int k = c16 + 16 * it;
+            ae[it] = bn < half ? Er[k + MP * bn] : 0.0;
+            ao[it] = bn < half ? Oi[k + MP * bn] : 0.0;
+        }
+#pragma unroll
+        for (int lt = 0; lt < TM; ++lt) {
+            const int l = c16 + 16 * lt;
+            bv[lt] = (bn < half && l < m) ? x2l[bn + npw * l] : 0.0;
+        }
+#pragma unroll
+        for (int it = 0; it < TM; ++it)
+#pragma unroll
+            for (int lt = 0; lt < TM; ++lt) {
+                P[0][it][lt] = mfma16x16x4(ae[it], bv[lt], P[0][it][lt]);
+                P[1][it][lt] = mfma16x16x4(ao[it], bv[lt], P[1][it][lt]);
+            }
+    }
+    wave_lds_sync();  // Er, Oi read: buf takes L
+#pragma unroll
+    for (int it = 0; it < TM; ++it)
+#pragma unroll
+        for (int lt = 0; lt < TM; ++lt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = q4 + 4 * r + 16 * it, l = c16 + 16 * lt;
+                double v;
+                switch (l & 3) {
+                case 0: v = 2 * P[0][it][lt][r]; break;
+                case 1: v = -2 * P[1][it][lt][r]; break;
+                case 2: v = -2 * P[0][it][lt][r]; break;
+                default: v = 2 * P[1][it][lt][r]; break;
+                }
+                buf[k + MP * l] = (k < m && l < m) ? v : 0.0;
+            }
+    wave_lds_sync();
+}
+
 // Step 5b on matrix cores, one wavefront per segment (at most WSEG points):
 // the lane's A operands L(4 ks + l/16, l%16 + 16 j) (and the x-derivative
 // variants) are loaded once per segment into registers, the segment's
 // scaled points are staged in the wave's slice of LDS in one coalesced
 // pass, then every tile of 16 points runs as in k_taylor_mfma.
-template <int TM, int PGH>
+//
+// FUSE = NBH > 0: the Taylor coefficients are not read from a.taylor but
+// formed by the wave itself from the leaf's plane-wave expansion (element
+// type E), pw2local_wave<TM, NBH, E>, into its slice of dynamic LDS.
+template <int TM, int PGH, int FUSE = 0, class E = double>
 __global__ __launch_bounds__(256) void k_taylor_wave(EvalArgs a)
 {
     constexpr int MP = 16 * TM, KS = MP / 4, NV = PGH >= 3 ? 3 : (PGH >= 2 ? 2 : 1);
     __shared__ double stage[4][2][WSEG];
+    extern __shared__ double dsm[];
     const int m = a.nl + 1, lane = threadIdx.x & 63;
     double* sx = stage[threadIdx.x >> 6][0];
     double* sy = stage[threadIdx.x >> 6][1];
@@ -1508,6 +1955,14 @@ __global__ __launch_bounds__(256) void k_taylor_wave(EvalArgs a)
         const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g, a.segbox);
         const int b = a.boxes[sr.li];
         const double* Lg = a.taylor + (long)sr.li * m * m;  // L(j,k) at j + m k
+        double* Lb = nullptr;                                // fused: L(j,k) at j + MP k
+        if (FUSE) {
+            Lb = dsm + (threadIdx.x >> 6) * PW2L_BUF(TM, FUSE);
+            const long nexp = (long)(a.npw / 2) * a.npw;
+            pw2local_wave<TM, (FUSE > 0 ? FUSE : 1), E>(
+                a, (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp, Lb, lane);
+        }
+        auto Lv = [&](int j, int k) -> double { return FUSE ? Lb[j + MP * k] : Lg[j + m * k]; };
         double A[NV][KS][TM];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
@@ -1515,11 +1970,11 @@ __global__ __launch_bounds__(256) void k_taylor_wave(EvalArgs a)
             for (int jt = 0; jt < TM; ++jt) {
                 const int j = 4 * ks + kq, k = col + 16 * jt;
                 const bool kin = k < m;
-                A[0][ks][jt] = (kin && j < m) ? Lg[j + m * k] : 0.0;
-                if (NV >= 2) A[NV >= 2 ? 1 : 0][ks][jt] = (kin && j + 1 < m) ? (j + 1) * Lg[(j + 1) + m * k] : 0.0;
+                A[0][ks][jt] = (kin && j < m) ? Lv(j, k) : 0.0;
+                if (NV >= 2) A[NV >= 2 ? 1 : 0][ks][jt] = (kin && j + 1 < m) ? (j + 1) * Lv(j + 1, k) : 0.0;
                 if (NV >= 3)
                     A[NV >= 3 ? 2 : 0][ks][jt] =
-                        (kin && j + 2 < m) ? (double)((j + 1) * (j + 2)) * Lg[(j + 2) + m * k] : 0.0;
+                        (kin && j + 2 < m) ? (double)((j + 1) * (j + 2)) * Lv(j + 2, k) : 0.0;
             }
         const double cx = a.cen[2 * b], cy = a.cen[2 * b + 1], dsq = a.dsq;
         const int npts = sr.p1 - sr.p0;
@@ -1628,15 +2083,218 @@ __global__ __launch_bounds__(256) void k_taylor_wave(EvalArgs a)
     }
 }
 
-template <int TM>
+// Step 5b with FP32 operands (the mixed-precision route): k_taylor_wave
+// with FP32 Taylor coefficients (written by k_pw2local<., float>) and FP32
+// powers of the offsets, which are formed in FP64 from the FP64 box centre;
+// FP32 matrix-core products over the x orders, then the y weighting, the
+// lane sums and the accumulation into the outputs in FP64.  A lane holds y
+// orders 4 (l/16) + r + 16 j (the FP32 result layout).
+template <int TM, int PGH, int FUSE = 0, class E = float>
+__global__ __launch_bounds__(256) void k_taylor_wave_f32(EvalArgs a)
+{
+    constexpr int MP = 16 * TM, KS = MP / 4, NV = PGH >= 3 ? 3 : (PGH >= 2 ? 2 : 1);
+    __shared__ float stage[4][2][WSEG];
+    extern __shared__ double dsm[];
+    const int m = a.nl + 1, lane = threadIdx.x & 63;
+    float* sx = stage[threadIdx.x >> 6][0];
+    float* sy = stage[threadIdx.x >> 6][1];
+    const int col = lane & 15, kq = lane >> 4;
+    const int total = seg_count(a.tstart, a.nbox);
+    for (int g = blockIdx.x * 4 + (threadIdx.x >> 6); g < total; g += gridDim.x * 4) {
+        const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g, a.segbox);
+        const int b = a.boxes[sr.li];
+        const float* Lg = (const float*)a.taylor + (long)sr.li * m * m;  // L(j,k) at j + m k
+        double* Lb = nullptr;  // fused: FP64 L(j,k) at j + MP k, rounded to FP32 below
+        if (FUSE) {
+            Lb = dsm + (threadIdx.x >> 6) * PW2L_BUF(TM, FUSE);
+            const long nexp = (long)(a.npw / 2) * a.npw;
+            pw2local_wave<TM, (FUSE > 0 ? FUSE : 1), E>(
+                a, (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp, Lb, lane);
+        }
+        // (j+1) L and (j+1)(j+2) L as in the unfused form: FP32 products of
+        // the FP32 coefficient
+        auto Lv = [&](int j, int k) -> float { return FUSE ? (float)Lb[j + MP * k] : Lg[j + m * k]; };
+        float A[NV][KS][TM];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int jt = 0; jt < TM; ++jt) {
+                const int j = 4 * ks + kq, k = col + 16 * jt;
+                const bool kin = k < m;
+                A[0][ks][jt] = (kin && j < m) ? Lv(j, k) : 0.f;
+                if (NV >= 2)
+                    A[NV >= 2 ? 1 : 0][ks][jt] = (kin && j + 1 < m) ? (float)(j + 1) * Lv(j + 1, k) : 0.f;
+                if (NV >= 3)
+                    A[NV >= 3 ? 2 : 0][ks][jt] =
+                        (kin && j + 2 < m) ? (float)((j + 1) * (j + 2)) * Lv(j + 2, k) : 0.f;
+            }
+        const double cx = a.cen[2 * b], cy = a.cen[2 * b + 1], dsq = a.dsq;
+        const int npts = sr.p1 - sr.p0;
+        wave_lds_sync();  // the previous segment's reads are done
+#pragma unroll
+        for (int k = 0; k < WSEG / 64; ++k) {
+            const int u = lane + 64 * k;
+            if (u < npts) {
+                const long i = sr.p0 + u;
+                sx[u] = (float)((a.pts[2 * i] - cx) * dsq);
+                sy[u] = (float)((a.pts[2 * i + 1] - cy) * dsq);
+            }
+        }
+        wave_lds_sync();
+        for (int base = 0; base < npts; base += 16) {
+            const int ip = base + col;
+            const bool act = ip < npts;
+            const long i = sr.p0 + (act ? ip : 0);
+            const float x = act ? sx[ip] : 0.f;
+            const double y = act ? (double)sy[ip] : 0.0;
+            float p = 1.f;
+            for (int r = 0; r < kq; ++r) p *= x;  // x^(l/16)
+            float x4 = x * x;
+            x4 *= x4;
+            f4_t acc[NV][TM];
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int j = 0; j < TM; ++j) acc[v][j] = (f4_t){0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+#pragma unroll
+                    for (int j = 0; j < TM; ++j) acc[v][j] = mfma16x16x4f32(A[v][ks][j], p, acc[v][j]);
+                p *= x4;
+            }
+            // y powers of this lane's rows: y^k, y^(k-1), y^(k-2) for
+            // k = 4 kq + r + 16 j, advanced by y per r; at the first row of
+            // tile j: t0 = y^(4 kq + 16 j) and y^(k-1), y^(k-2) from t1, t2
+            // (tile 0; zero where the exponent is negative, whose factor k
+            // or k (k-1) is zero) or u1, u2 (later tiles)
+            const double y2 = y * y, y4 = y2 * y2, y8 = y4 * y4, y16 = y8 * y8;
+            double tq = 1.0;  // y^(4 (kq - 1)) for kq >= 1
+            for (int r = 1; r < kq; ++r) tq *= y4;
+            double t0 = kq >= 1 ? tq * y4 : 1.0;
+            const double t1 = kq >= 1 ? tq * y2 * y : 0.0, t2 = kq >= 1 ? tq * y2 : 0.0;
+            double u1 = t0 * (y8 * y4 * y2 * y), u2 = t0 * (y8 * y4 * y2);
+            double V = 0, Vy = 0, Vyy = 0, U = 0, Uy = 0, Z = 0;
+#pragma unroll
+            for (int j = 0; j < TM; ++j) {
+                double p0 = t0, p1 = j == 0 ? t1 : u1, p2 = j == 0 ? t2 : u2;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double k = 4 * kq + r + 16 * j;
+                    const double c0 = acc[0][j][r];
+                    V += c0 * p0;
+                    if (PGH >= 2) {
+                        U += (double)acc[NV >= 2 ? 1 : 0][j][r] * p0;
+                        Vy += k * c0 * p1;
+                    }
+                    if (PGH >= 3) {
+                        Z += (double)acc[NV >= 3 ? 2 : 0][j][r] * p0;
+                        Uy += k * (double)acc[NV >= 2 ? 1 : 0][j][r] * p1;
+                        Vyy += k * (k - 1) * c0 * p2;
+                    }
+                    p2 = p1;
+                    p1 = p0;
+                    p0 *= y;
+                }
+                t0 *= y16;
+                if (j > 0) {
+                    u1 *= y16;
+                    u2 *= y16;
+                }
+            }
+            V += __shfl_xor(V, 16);
+            V += __shfl_xor(V, 32);
+            if (PGH >= 2) {
+                U += __shfl_xor(U, 16);
+                U += __shfl_xor(U, 32);
+                Vy += __shfl_xor(Vy, 16);
+                Vy += __shfl_xor(Vy, 32);
+            }
+            if (PGH >= 3) {
+                Z += __shfl_xor(Z, 16);
+                Z += __shfl_xor(Z, 32);
+                Uy += __shfl_xor(Uy, 16);
+                Uy += __shfl_xor(Uy, 32);
+                Vyy += __shfl_xor(Vyy, 16);
+                Vyy += __shfl_xor(Vyy, 32);
+            }
+            if (kq == 0 && act) {
+                a.pot[i * a.nd + a.ind] += V;
+                if (PGH >= 2) {
+                    const long gi = i * 2 * a.nd + a.ind;
+                    a.grad[gi] += U * dsq;
+                    a.grad[gi + a.nd] += Vy * dsq;
+                }
+                if (PGH >= 3) {
+                    const long h = i * 3 * a.nd + a.ind;
+                    a.hess[h] += Z * dsq * dsq;
+                    a.hess[h + a.nd] += Uy * dsq * dsq;
+                    a.hess[h + 2 * a.nd] += Vyy * dsq * dsq;
+                }
+            }
+        }
+    }
+}
+
+// FUSE = NBH: PW -> Taylor inside the wave (a.fused), from FP64 or (a.exp32)
+// FP32 expansions
+template <int TM, int FUSE, class E>
+static void taylor_wave_f32_launch(const EvalArgs& a, hipStream_t s)
+{
+    const unsigned lds = FUSE ? 4 * PW2L_BUF(TM, FUSE) * sizeof(double) : 0;
+    if (a.ifpgh >= 3) {
+        allow_lds(k_taylor_wave_f32<TM, 3, FUSE, E>, lds + sizeof(float) * 8 * WSEG);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave_f32<TM, 3, FUSE, E>, a.grid, 256, lds, s, a);
+    } else if (a.ifpgh == 2) {
+        allow_lds(k_taylor_wave_f32<TM, 2, FUSE, E>, lds + sizeof(float) * 8 * WSEG);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave_f32<TM, 2, FUSE, E>, a.grid, 256, lds, s, a);
+    } else {
+        allow_lds(k_taylor_wave_f32<TM, 1, FUSE, E>, lds + sizeof(float) * 8 * WSEG);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave_f32<TM, 1, FUSE, E>, a.grid, 256, lds, s, a);
+    }
+}
+
+template <int TM, int FUSE, class E>
 static void taylor_wave_launch(const EvalArgs& a, hipStream_t s)
 {
-    if (a.ifpgh >= 3)
-        k_taylor_wave<TM, 3><<<a.grid, 256, 0, s>>>(a);
-    else if (a.ifpgh == 2)
-        k_taylor_wave<TM, 2><<<a.grid, 256, 0, s>>>(a);
-    else
-        k_taylor_wave<TM, 1><<<a.grid, 256, 0, s>>>(a);
+    const unsigned lds = FUSE ? 4 * PW2L_BUF(TM, FUSE) * sizeof(double) : 0;
+    if (a.ifpgh >= 3) {
+        allow_lds(k_taylor_wave<TM, 3, FUSE, E>, lds + sizeof(double) * 8 * WSEG);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave<TM, 3, FUSE, E>, a.grid, 256, lds, s, a);
+    } else if (a.ifpgh == 2) {
+        allow_lds(k_taylor_wave<TM, 2, FUSE, E>, lds + sizeof(double) * 8 * WSEG);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave<TM, 2, FUSE, E>, a.grid, 256, lds, s, a);
+    } else {
+        allow_lds(k_taylor_wave<TM, 1, FUSE, E>, lds + sizeof(double) * 8 * WSEG);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave<TM, 1, FUSE, E>, a.grid, 256, lds, s, a);
+    }
+}
+
+// the wave kernels of one Taylor order tile count: unfused (Taylor
+// coefficients from a.taylor) or fused with NBH = 1 (npw <= 32) or 2
+// (npw <= 64)
+template <int TM>
+static void taylor_wave_any(const EvalArgs& a, hipStream_t s)
+{
+    const int nbh = (a.npw / 2 + 15) / 16;
+    if (!a.fused) {
+        if (a.tay32)
+            taylor_wave_f32_launch<TM, 0, float>(a, s);
+        else
+            taylor_wave_launch<TM, 0, double>(a, s);
+    } else if (a.tay32) {
+        if (nbh == 1)
+            taylor_wave_f32_launch<TM, 1, float>(a, s);
+        else
+            taylor_wave_f32_launch<TM, 2, float>(a, s);
+    } else if (a.exp32) {
+        fail(-3, "FP32 expansions with FP64 Taylor evaluation are not fused");
+    } else if (nbh == 1) {
+        taylor_wave_launch<TM, 1, double>(a, s);
+    } else {
+        taylor_wave_launch<TM, 2, double>(a, s);
+    }
 }
 
 template <int TM>
@@ -1648,13 +2306,13 @@ static void taylor_mfma_launch(const EvalArgs& a, hipStream_t s)
     const int grid = a.tstart ? a.grid : a.nbox;
     if (a.ifpgh >= 3) {
         allow_lds(k_taylor_mfma<TM, 3>, lds);
-        k_taylor_mfma<TM, 3><<<grid, 256, lds, s>>>(a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_mfma<TM, 3>, grid, 256, lds, s, a);
     } else if (a.ifpgh == 2) {
         allow_lds(k_taylor_mfma<TM, 2>, lds);
-        k_taylor_mfma<TM, 2><<<grid, 256, lds, s>>>(a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_mfma<TM, 2>, grid, 256, lds, s, a);
     } else {
         allow_lds(k_taylor_mfma<TM, 1>, lds);
-        k_taylor_mfma<TM, 1><<<grid, 256, lds, s>>>(a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_mfma<TM, 1>, grid, 256, lds, s, a);
     }
 }
 
@@ -1662,15 +2320,16 @@ void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
     int m = a.nl + 1;
-    kmark(KID_TAYLOR_EVAL, false, s);
     if (m <= 32 && a.tstart && a.segbox && a.seg <= WSEG) {
+        if (a.fused && a.npw > 64) fail(-3, "fused PW -> Taylor needs npw <= 64");
         if (m <= 16)
-            taylor_wave_launch<1>(a, s);
+            taylor_wave_any<1>(a, s);
         else
-            taylor_wave_launch<2>(a, s);
-        kmark(KID_TAYLOR_EVAL, true, s);
+            taylor_wave_any<2>(a, s);
         return;
     }
+    if (a.tay32 || a.fused)
+        fail(-3, "FP32 or fused Taylor coefficients need the segmented evaluation with order < 32");
     if (m <= 64) {
         switch ((m + 15) / 16) {
         case 1: taylor_mfma_launch<1>(a, s); break;
@@ -1678,18 +2337,16 @@ void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
         case 3: taylor_mfma_launch<3>(a, s); break;
         default: taylor_mfma_launch<4>(a, s); break;
         }
-        kmark(KID_TAYLOR_EVAL, true, s);
         return;
     }
     size_t lds = (size_t)m * m * sizeof(double);
     const int grid = a.tstart ? a.grid : a.nbox;
     if (a.ifpgh >= 3)
-        k_taylor_eval<3><<<grid, 256, lds, s>>>(a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_eval<3>, grid, 256, lds, s, a);
     else if (a.ifpgh == 2)
-        k_taylor_eval<2><<<grid, 256, lds, s>>>(a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_eval<2>, grid, 256, lds, s, a);
     else
-        k_taylor_eval<1><<<grid, 256, lds, s>>>(a);
-    kmark(KID_TAYLOR_EVAL, true, s);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_eval<1>, grid, 256, lds, s, a);
 }
 
 // Step 5, plane-wave route (g2dpwevalp/g/h_vec, g2drouts.f:1887-2200):
@@ -1697,7 +2354,7 @@ void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
 // from one sincos per axis and its odd powers.  One block per leaf.
 constexpr int PW_MAXH = 32;
 
-template <int PGH>
+template <int PGH, class E>
 __global__ __launch_bounds__(256) void k_pw_eval(EvalArgs a)
 {
     extern __shared__ double sm[];
@@ -1706,7 +2363,7 @@ __global__ __launch_bounds__(256) void k_pw_eval(EvalArgs a)
     for (int g = blockIdx.x; g < total; g += gridDim.x) {
     const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g);
     const int b = a.boxes[sr.li];
-    const double* pw = a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
+    const E* pw = (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
     __syncthreads();
     for (int o = t; o < 2 * nexp; o += 256) sm[o] = pw[o];
     __syncthreads();
@@ -1798,14 +2455,19 @@ void launch_pw_eval(const EvalArgs& a, hipStream_t s)
     if (a.npw / 2 > PW_MAXH) fail(-3, "plane-wave length too large for the evaluation kernel");
     size_t lds = (size_t)a.npw * a.npw * sizeof(double);
     const int grid = a.tstart ? a.grid : a.nbox;
-    kmark(KID_PW_EVAL, false, s);
-    if (a.ifpgh >= 3)
-        k_pw_eval<3><<<grid, 256, lds, s>>>(a);
+    if (a.exp32) {
+        if (a.ifpgh >= 3)
+            klaunch(KID_PW_EVAL, k_pw_eval<3, float>, grid, 256, lds, s, a);
+        else if (a.ifpgh == 2)
+            klaunch(KID_PW_EVAL, k_pw_eval<2, float>, grid, 256, lds, s, a);
+        else
+            klaunch(KID_PW_EVAL, k_pw_eval<1, float>, grid, 256, lds, s, a);
+    } else if (a.ifpgh >= 3)
+        klaunch(KID_PW_EVAL, k_pw_eval<3, double>, grid, 256, lds, s, a);
     else if (a.ifpgh == 2)
-        k_pw_eval<2><<<grid, 256, lds, s>>>(a);
+        klaunch(KID_PW_EVAL, k_pw_eval<2, double>, grid, 256, lds, s, a);
     else
-        k_pw_eval<1><<<grid, 256, lds, s>>>(a);
-    kmark(KID_PW_EVAL, true, s);
+        klaunch(KID_PW_EVAL, k_pw_eval<1, double>, grid, 256, lds, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -1901,15 +2563,13 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
 void launch_direct(const DirectArgs& a, hipStream_t s)
 {
     if (a.nunits <= 0) return;
-    kmark(KID_DIRECT, false, s);
     const bool dip = a.ws != nullptr;
     if (a.ifpgh >= 3)
-        dip ? k_direct<3, true><<<a.nunits, 256, 0, s>>>(a) : k_direct<3, false><<<a.nunits, 256, 0, s>>>(a);
+        dip ? klaunch(KID_DIRECT, k_direct<3, true>, a.nunits, 256, 0, s, a) : klaunch(KID_DIRECT, k_direct<3, false>, a.nunits, 256, 0, s, a);
     else if (a.ifpgh == 2)
-        dip ? k_direct<2, true><<<a.nunits, 256, 0, s>>>(a) : k_direct<2, false><<<a.nunits, 256, 0, s>>>(a);
+        dip ? klaunch(KID_DIRECT, k_direct<2, true>, a.nunits, 256, 0, s, a) : klaunch(KID_DIRECT, k_direct<2, false>, a.nunits, 256, 0, s, a);
     else
-        dip ? k_direct<1, true><<<a.nunits, 256, 0, s>>>(a) : k_direct<1, false><<<a.nunits, 256, 0, s>>>(a);
-    kmark(KID_DIRECT, true, s);
+        dip ? klaunch(KID_DIRECT, k_direct<1, true>, a.nunits, 256, 0, s, a) : klaunch(KID_DIRECT, k_direct<1, false>, a.nunits, 256, 0, s, a);
 }
 
 // ---------------------------------------------------------------------------

@@ -742,6 +742,24 @@ void build_tree(TreeD<D>& T, const std::vector<LeafRun>& leaves, int nlmax, int 
         }
 }
 
+// memcpy of n bytes split over the tree builder's worker pool (in slices of
+// at least 1 MiB); sequential when the pool is busy or absent
+void parallel_copy(void* dst, const void* src, size_t n)
+{
+    constexpr size_t kSlice = (size_t)1 << 20;
+    if (n < 2 * kSlice) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    Par par(true);
+    const long slices = (long)((n + kSlice - 1) / kSlice);
+    auto f = [&](long i0, long i1) {
+        const size_t a = (size_t)i0 * kSlice, b = std::min(n, (size_t)i1 * kSlice);
+        std::memcpy((char*)dst + a, (const char*)src + a, b - a);
+    };
+    par.range(slices, 1, f);
+}
+
 template void build_tree<2>(TreeD<2>&, const std::vector<LeafRun>&, int, int, double,
                             const double*);
 template void build_tree<3>(TreeD<3>&, const std::vector<LeafRun>&, int, int, double,

@@ -137,10 +137,33 @@ int fgtpw_fgt3dpart_direct(int nd, double delta, double dmax, int istart, int ie
 int fgtpw_last_stats(int *out, int n);
 
 /*
+ * Arithmetic of the next transforms (process-wide; returns the previous
+ * mode, or FGTPW_EINVAL).  No reference counterpart: the reference computes
+ * in FP64 only.
+ *   0  FP64 everywhere (default): the reference's arithmetic.
+ *   1  mixed precision for 2D transforms with eps >= 1e-6 that form
+ *      expansions: the leaf moments and the Taylor evaluation run on FP32
+ *      matrix cores (operands rounded to FP32 from offsets formed in FP64
+ *      about FP64 box centres; FP32 accumulation within a segment of at most
+ *      256 points, FP64 across segments and into the outputs), and the
+ *      plane-wave expansions and their translation factors are FP32 through
+ *      merge, m2l and split.  The Hermite->plane-wave and plane-wave->Taylor
+ *      conversions, plane-wave evaluation phases, the tree and the near
+ *      field stay FP64.  Other transforms (3D, eps < 1e-6) run in FP64.
+ * fgtpw_last_precision() returns 1 when the last transform of the calling
+ * thread took the mixed route, else 0.
+ */
+int fgtpw_set_precision(int mode);
+int fgtpw_last_precision(void);
+
+/*
  * Stage timing of the next transforms: when on (1), each call records HIP
  * events around its stages on its stream and fgtpw_stage_times() returns the
  * milliseconds of the last call per stage, in the order of
- * fgtpw_stage_name(i).  Returns the number of stages written.
+ * fgtpw_stage_name(i).  Returns the number of stages written.  on = 2 does
+ * the same with the work of the side streams (near field, coarse levels)
+ * queued on the call's stream instead, so that no two kernels overlap and
+ * the per-kernel times of fgtpw_kernel_launches are each kernel's own.
  */
 int fgtpw_set_timing(int on);
 int fgtpw_stage_times(double *ms, int n);

@@ -21,9 +21,16 @@
  *   - against the FP64 direct sum within the requested eps;
  *   - closed forms: quadrature identity of the plane-wave nodes, term counts.
  *
- * Scope: charges (ifcharge=1, ifdipole=0), potentials and gradients and
- * Hessians at sources and/or targets (ifpgh, ifpghtarg in 0..3), any nd >= 1.
- * Dipoles return -3 (unsupported).
+ * Scope: charges and dipoles, alone or together, potentials and gradients
+ * and Hessians at sources and/or targets (ifpgh, ifpghtarg in 0..3), any
+ * nd >= 1.
+ *
+ * Threads.  Built with -fopenmp (liborc_omp.so, the all-core CPU baseline)
+ * the per-box loops of the tree sort, formation, translations, evaluation,
+ * near field and reorders run in parallel.  Every output keeps the order of
+ * its additions (the near field runs per target box over its source boxes
+ * in the sequential loop's order), so the threaded results are bit-identical
+ * to the sequential ones.  Without -fopenmp (liborc.so) the pragmas vanish.
  */
 #include <math.h>
 #include <stdint.h>
@@ -32,6 +39,26 @@
 #include <time.h>
 
 #define ORC_MAXLEV 200
+
+#ifdef _OPENMP
+#include <omp.h>
+#define OMP(x) _Pragma(#x)
+#else
+#define OMP(x)
+#endif
+
+/* threads of the OpenMP build (0: its default); returns the count in use,
+ * 1 for the sequential build */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 /* ------------------------------------------------------------------ */
 /* small helpers                                                       */
@@ -219,9 +246,9 @@ static void ot_sort_children(const otree *T, int ibox, const double *xy,
         int p = perm[i];
         int xb = !(xy[2 * p] - T->cen[2 * ibox] < 0);
         int yb = !(xy[2 * p + 1] - T->cen[2 * ibox + 1] < 0);
-        tmp[pos[xb + 2 * yb]++] = p;
+        tmp[s + pos[xb + 2 * yb]++] = p;  /* the box's own slice of tmp */
     }
-    memcpy(perm + s, tmp, sizeof(int) * (e - s));
+    memcpy(perm + s, tmp + s, sizeof(int) * (e - s));
     for (int j = 0; j < 4; j++) {
         int jb = T->ch[4 * ibox + j];
         se[2 * jb] = s + off[j];
@@ -234,6 +261,7 @@ static void ot_sort_children(const otree *T, int ibox, const double *xy,
  * found among the children of the parent's colleagues. */
 static void ot_colleagues(otree *T, int b0, int b1)
 {
+    OMP(omp parallel for schedule(dynamic, 64))
     for (int ib = b0; ib <= b1; ib++) {
         int lv = T->lev[ib];
         T->ncol[ib] = 0;
@@ -434,6 +462,8 @@ static void ot_build(otree *T, const double *src, int ns, int nlmax, int levcut,
         if (!any) break;
         T->bsz[ilev + 1] = T->bsz[ilev] / 2;
         T->laddr[2 * (ilev + 1)] = T->nboxes;
+        /* children in parent order, then each parent's points partitioned
+         * (disjoint index ranges) */
         for (int ib = f0; ib <= f1; ib++) {
             if (se[2 * ib + 1] - se[2 * ib] <= lim) continue;
             ot_grow(T, T->nboxes + 4);
@@ -444,8 +474,10 @@ static void ot_build(otree *T, const double *src, int ns, int nlmax, int levcut,
             int first = T->nboxes;
             T->nboxes += 4;
             ot_make_children(T, ib, T->bsz[ilev + 1], ilev + 1, first);
-            ot_sort_children(T, ib, src, perm, se, tmp);
         }
+        OMP(omp parallel for schedule(dynamic, 16))
+        for (int ib = f0; ib <= f1; ib++)
+            if (T->nch[ib] > 0) ot_sort_children(T, ib, src, perm, se, tmp);
         T->laddr[2 * (ilev + 1) + 1] = T->nboxes - 1;
     }
     T->nlevels = ilev;
@@ -461,9 +493,11 @@ static void ot_sort_points(const otree *T, const double *xy, int n, int *perm, i
     int *tmp = xcalloc(n, sizeof(int));
     for (int i = 0; i < n; i++) perm[i] = i;
     se[0] = 0; se[1] = n;
-    for (int l = 0; l < T->nlevels; l++)
+    for (int l = 0; l < T->nlevels; l++) {
+        OMP(omp parallel for schedule(dynamic, 16))
         for (int ib = T->laddr[2 * l]; ib <= T->laddr[2 * l + 1]; ib++)
             if (T->nch[ib] > 0) ot_sort_children(T, ib, xy, perm, se, tmp);
+    }
     free(tmp);
 }
 
@@ -1205,6 +1239,7 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
     double *ss = xcalloc(2 * (size_t)ns, sizeof(double)), *cs = xcalloc((size_t)nd * ns, sizeof(double));
     double *tsrt = xcalloc(2 * (size_t)(nt ? nt : 1), sizeof(double));
     double *rs = xcalloc(2 * (size_t)ns, sizeof(double)), *dss = xcalloc((size_t)nd * ns, sizeof(double));
+    OMP(omp parallel for)
     for (int i = 0; i < ns; i++) {
         ss[2 * i] = sources[2 * (size_t)isrc[i]];
         ss[2 * i + 1] = sources[2 * (size_t)isrc[i] + 1];
@@ -1217,6 +1252,7 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
         }
     }
     const double *cq = charge ? cs : NULL, *crn = rnormal ? rs : NULL;
+    OMP(omp parallel for)
     for (int i = 0; i < nt; i++) {
         tsrt[2 * i] = targ[2 * (size_t)itarg[i]];
         tsrt[2 * i + 1] = targ[2 * (size_t)itarg[i] + 1];
@@ -1291,13 +1327,16 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
             }
 
         int mt = ntermmax + 1;
-        double *hexp = xcalloc((size_t)mt * mt * nd, sizeof(double));
-        double *pxhy = xcalloc(2 * (size_t)mt * npw, sizeof(double));
-        double *xp = xcalloc(mt + 1, sizeof(double)), *yp = xcalloc(mt + 1, sizeof(double));
-        double *dxp = xcalloc(mt + 1, sizeof(double)), *dyp = xcalloc(mt + 1, sizeof(double));
         /* step 1: leaf Hermite expansions -> PW, fgt2d.f:871-959 */
         int lstop = npwlevel + 1 > 0 ? npwlevel + 1 : 0;
         for (int l = nlevels; l >= lstop; l--)
+            OMP(omp parallel)
+            {
+            double *hexp = xcalloc((size_t)mt * mt * nd, sizeof(double));
+            double *pxhy = xcalloc(2 * (size_t)mt * npw, sizeof(double));
+            double *xp = xcalloc(mt + 1, sizeof(double)), *yp = xcalloc(mt + 1, sizeof(double));
+            double *dxp = xcalloc(mt + 1, sizeof(double)), *dyp = xcalloc(mt + 1, sizeof(double));
+            OMP(omp for schedule(dynamic, 4))
             for (int ib = T.laddr[2 * l]; ib <= T.laddr[2 * l + 1]; ib++) {
                 int s = isrcse[2 * ib], n = isrcse[2 * ib + 1] - s;
                 if (T.nch[ib] != 0 || n <= 0) continue;
@@ -1307,8 +1346,11 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
                           nlocal[l], hexp, xp, yp, dxp, dyp);
                 orc_h2pw(nd, nlocal[l], npw, rh2x, hexp, mp + per * ib, pxhy);
             }
+            free(hexp); free(pxhy); free(xp); free(yp); free(dxp); free(dyp);
+            }
         /* step 2: merge children into parents, fgt2d.f:976-1009 */
         for (int l = nlevels - 1; l >= nlevstart; l--)
+            OMP(omp parallel for schedule(dynamic, 16))
             for (int ib = T.laddr[2 * l]; ib <= T.laddr[2 * l + 1]; ib++)
                 for (int i = 0; i < T.nch[ib]; i++) {
                     int jb = T.ch[4 * ib + i];
@@ -1322,6 +1364,7 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
                                      mp + per * jb + 2 * ((size_t)d * nexp + e), w + 2 * e);
                 }
         /* step 3: mp -> loc at the cutoff level, fgt2d.f:1026-1064 */
+        OMP(omp parallel for schedule(dynamic, 16))
         for (int ib = T.laddr[2 * nlevstart]; ib <= T.laddr[2 * nlevstart + 1]; ib++) {
             int np = 0;
             if (ifpghtarg > 0) np += itargse[2 * ib + 1] - itargse[2 * ib];
@@ -1343,6 +1386,7 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
         }
         /* step 4: split loc to children, fgt2d.f:1082-1125 */
         for (int l = nlevstart; l <= nlevels - 1; l++)
+            OMP(omp parallel for schedule(dynamic, 16))
             for (int ib = T.laddr[2 * l]; ib <= T.laddr[2 * l + 1]; ib++) {
                 int np = 0;
                 if (ifpghtarg > 0) np += itargse[2 * ib + 1] - itargse[2 * ib];
@@ -1360,9 +1404,12 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
                 }
             }
         /* step 5: evaluate at leaves, fgt2d.f:1141-1271 */
-        double *loc = xcalloc((size_t)mt * mt * nd, sizeof(double));
-        double *lxpy = xcalloc(2 * (size_t)mt * npw, sizeof(double));
         for (int l = nlevstart; l <= nlevels; l++)
+            OMP(omp parallel)
+            {
+            double *loc = xcalloc((size_t)mt * mt * nd, sizeof(double));
+            double *lxpy = xcalloc(2 * (size_t)mt * npw, sizeof(double));
+            OMP(omp for schedule(dynamic, 4))
             for (int ib = T.laddr[2 * l]; ib <= T.laddr[2 * l + 1]; ib++) {
                 if (l == npwlevel && !iflocal[ib]) continue;
                 if (T.nch[ib] != 0) continue;
@@ -1390,18 +1437,35 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
                                   hs + (size_t)3 * nd * s0);
                 }
             }
-        free(loc); free(lxpy); free(hexp); free(pxhy); free(xp); free(yp); free(dxp); free(dyp);
+            free(loc); free(lxpy);
+            }
         free(iflocal); free(wms); free(wm2l); free(rh2x); free(rx2l);
     }
 
     /* step 6: direct interactions of leaves at or above the cutoff,
-     * fgt2d.f:1287-1333, cutoff eps/90 */
+     * fgt2d.f:1287-1333, cutoff eps/90.  The loop runs over target boxes
+     * ib, each over its source boxes jb in the order of the reference's
+     * loop over jb (so every target adds its source boxes in the same
+     * order): a CSR of (ib <- jb) pairs, filled in that order. */
+    int *nf_ptr = xcalloc((size_t)nboxes + 1, sizeof(int));
     for (int l = 0; l <= nlevend; l++)
-        for (int jb = T.laddr[2 * l]; jb <= T.laddr[2 * l + 1]; jb++) {
-            if (!ifhung[jb]) continue;
+        for (int jb = T.laddr[2 * l]; jb <= T.laddr[2 * l + 1]; jb++)
+            if (ifhung[jb])
+                for (int i = 0; i < nlist1[jb]; i++) nf_ptr[list1[13 * jb + i] + 1]++;
+    for (int b = 0; b < nboxes; b++) nf_ptr[b + 1] += nf_ptr[b];
+    int *nf_src = xcalloc((size_t)nf_ptr[nboxes] + 1, sizeof(int));
+    int *nf_fill = xcalloc((size_t)nboxes + 1, sizeof(int));
+    memcpy(nf_fill, nf_ptr, sizeof(int) * nboxes);
+    for (int l = 0; l <= nlevend; l++)
+        for (int jb = T.laddr[2 * l]; jb <= T.laddr[2 * l + 1]; jb++)
+            if (ifhung[jb])
+                for (int i = 0; i < nlist1[jb]; i++) nf_src[nf_fill[list1[13 * jb + i]]++] = jb;
+    OMP(omp parallel for schedule(dynamic, 8))
+    for (int ib = 0; ib < nboxes; ib++)
+        for (int k = nf_ptr[ib]; k < nf_ptr[ib + 1]; k++) {
+            int jb = nf_src[k];
             int js = isrcse[2 * jb], jn = isrcse[2 * jb + 1] - js;
-            for (int i = 0; i < nlist1[jb]; i++) {
-                int ib = list1[13 * jb + i];
+            {
                 int st = itargse[2 * ib], ntg = itargse[2 * ib + 1] - st;
                 int s0 = isrcse[2 * ib], nsr = isrcse[2 * ib + 1] - s0;
                 const double *q = cq ? cs + (size_t)nd * js : NULL;
@@ -1417,9 +1481,11 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
                                     gs + (size_t)2 * nd * s0, hs + (size_t)3 * nd * s0);
             }
         }
+    free(nf_ptr); free(nf_src); free(nf_fill);
 
     /* back to input order, dreorderi / dreorderiv fmmcommon2d.f:29-90 */
     if (ifpgh > 0)
+        OMP(omp parallel for)
         for (int i = 0; i < ns; i++) {
             for (int d = 0; d < nd; d++) pot[(size_t)nd * isrc[i] + d] = ps[(size_t)nd * i + d];
             if (ifpgh >= 2)
@@ -1428,6 +1494,7 @@ int orc_fgt2d(int nd, double delta, double eps, int ns, const double *sources,
                 for (int d = 0; d < 3 * nd; d++) hess[(size_t)3 * nd * isrc[i] + d] = hs[(size_t)3 * nd * i + d];
         }
     if (ifpghtarg > 0)
+        OMP(omp parallel for)
         for (int i = 0; i < nt; i++) {
             for (int d = 0; d < nd; d++) pottarg[(size_t)nd * itarg[i] + d] = pt[(size_t)nd * i + d];
             if (ifpghtarg >= 2)

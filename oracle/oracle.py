@@ -7,7 +7,9 @@ cpu_baseline leg of bench.py, always as the checker.  The product package
 Two libraries are wrapped:
 
 * ``liborc.so`` -- the C restatement in ``fgt2d_oracle.c`` (built by
-  ``make -C oracle``);
+  ``make -C oracle``), and ``liborc_omp.so``, the same source built with
+  OpenMP (its per-box loops threaded, results bit-identical): the CPU
+  baseline on all host cores (``oracle_fgt2d(..., threads=n)``);
 * ``_ref/libref2d.so`` -- the reference Fortran itself, compiled from
   ``2d/planewave/src`` by ``make -C oracle ref`` where the reference sources
   exist.  On hosts without it, ``have_ref()`` is False and the callers skip.
@@ -26,13 +28,19 @@ _ORC = None
 _REF = None
 
 
-def _orc():
-    global _ORC
-    if _ORC is None:
-        path = os.path.join(_HERE, "liborc.so")
+_ORC_OMP = None
+
+
+def _orc(threaded=False):
+    global _ORC, _ORC_OMP
+    if (_ORC_OMP if threaded else _ORC) is None:
+        name = "liborc_omp.so" if threaded else "liborc.so"
+        path = os.path.join(_HERE, name)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liborc.so is not built (run make -C oracle)")
+            raise RuntimeError("oracle/%s is not built (run make -C oracle)" % name)
         lib = C.CDLL(path)
+        lib.orc_set_threads.restype = C.c_int
+        lib.orc_set_threads.argtypes = [C.c_int]
         dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
         lib.orc_fgt2d.restype = C.c_int
         lib.orc_fgt2d.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, dp, C.c_int, dp, C.c_int,
@@ -64,8 +72,16 @@ def _orc():
         lib.orc_derr.argtypes = [dp, dp, C.c_int]
         lib.orc_dlaran.restype = None
         lib.orc_dlaran.argtypes = [C.POINTER(C.c_uint64), C.c_int, dp]
-        _ORC = lib
-    return _ORC
+        if threaded:
+            _ORC_OMP = lib
+        else:
+            _ORC = lib
+    return _ORC_OMP if threaded else _ORC
+
+
+def threads_in_use(n=0):
+    """Sets (n > 0) and returns the thread count of the OpenMP oracle."""
+    return _orc(True).orc_set_threads(n)
 
 
 def _d(a):
@@ -104,12 +120,15 @@ def _ptr_or(a, n):
 
 
 def oracle_fgt2d(delta, eps, sources, charge, targ=None, ifpgh=1, ifpghtarg=0, rnormal=None,
-                 dipstr=None):
+                 dipstr=None, threads=None):
     """The C restatement of fgt2d (fgt2d.f:19).  charge=None: no charges;
     rnormal/dipstr (ns, 2) and (ns, nd): dipoles.  Returns (pot, grad, hess,
     pottarg, gradtarg, hesstarg) shaped (n, nd), (n, 2, nd), (n, 3, nd) when
-    requested."""
-    lib = _orc()
+    requested.  threads: run the OpenMP build on that many threads (0: its
+    default) instead of the sequential one; the results are the same."""
+    lib = _orc(threads is not None)
+    if threads is not None:
+        lib.orc_set_threads(threads)
     src = _pts(sources)
     ns = src.shape[0]
     nd, chg, rn, ds = _strengths(ns, charge, rnormal, dipstr)
