@@ -432,9 +432,12 @@ def main():
                          "c3: the 3D 1M uniform configuration; "
                          "c4: the serving stream alone (10k requests at 200/s); "
                          "c5: the 3D 8M shell + 8M targets at eps 1e-9")
-    ap.add_argument("--serve-requests", type=int, default=1000,
+    ap.add_argument("--serve-requests", type=int, default=10000,
                     help="requests of the serving stream reported beside c2 (0: none)")
     ap.add_argument("--slots", type=int, default=4, help="server slots of the serving stream")
+    ap.add_argument("--serve-capacity", type=int, default=500,
+                    help="requests of the same mix submitted at once, for the server's capacity "
+                         "(0: none)")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
                     help="c2 legs: both (default; the mixed-precision route is the headline), "
                          "or one of them")
@@ -573,6 +576,12 @@ def main():
         torch.cuda.synchronize()
     if args.serve_requests > 0:
         rec["serving"] = serve_stream(fgt2d_amd, args.serve_requests, slots=args.slots)
+    if args.serve_capacity > 0:
+        # the same request mix submitted all at once: delivered Mpts/s over
+        # the makespan is the server's capacity (latencies are queueing)
+        cap = serve_stream(fgt2d_amd, args.serve_capacity, rate=1e9, slots=args.slots, check=0)
+        rec["serving_capacity"] = {k: cap[k] for k in ("requests", "slots", "Mpts_s", "span_s",
+                                                       "peak_running")}
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline()
     print(json.dumps(rec))

@@ -19,7 +19,7 @@ import numpy as _np
 __all__ = ["fgt2d", "fgt3d", "fgt2dpart_direct", "fgt3dpart_direct", "FGTError", "Server",
            "last_stats", "last_tree2d", "last_tree3d", "tree2d_from_leaves", "set_timing",
            "stage_times", "kernel_launches", "library_path", "load", "set_precision",
-           "last_precision"]
+           "last_precision", "tuning_table"]
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
 _LIB = None
@@ -59,6 +59,8 @@ def load():
     lib.fgtpw_set_timing.argtypes = [i]
     lib.fgtpw_set_precision.argtypes = [i]
     lib.fgtpw_last_precision.argtypes = []
+    lib.fgtpw_tuning_table.restype = _C.c_char_p
+    lib.fgtpw_tuning_table.argtypes = []
     lib.fgtpw_stage_times.argtypes = [_C.POINTER(_C.c_double), i]
     lib.fgtpw_stage_name.restype = _C.c_char_p
     lib.fgtpw_stage_name.argtypes = [i]
@@ -301,6 +303,18 @@ def set_precision(mode):
     if prev < 0:
         _check(prev)
     return "mixed" if prev == 1 else "fp64"
+
+
+def tuning_table():
+    """The tile shapes in use (see include/fgtpw.h, fgtpw_tuning_table), as a
+    dict with the key "source" naming where they came from."""
+    text = load().fgtpw_tuning_table().decode()
+    rest, src = text.split(";", 1)
+    out = {"source": src.split("=", 1)[1]}
+    for kv in rest.split():
+        k, v = kv.split("=")
+        out[k] = int(v)
+    return out
 
 
 def last_precision():

@@ -27,6 +27,8 @@
 #include <string>
 #include <vector>
 
+#include <dlfcn.h>
+
 #include "fgtpw.h"
 #include "fgtpw_impl.hpp"
 
@@ -56,6 +58,85 @@ const bool g_fuse_pw2local = [] {
     const char* e = std::getenv("FGTPW_FUSE_PW2LOCAL");
     return !(e && e[0] == '0');
 }();
+// Tile shapes of the 2D segmented kernels, per Taylor / Hermite order tile
+// count TM (orders up to 16 TM): points per formation segment (segf),
+// points per evaluation segment (sege, at most 256: the wave kernels' LDS
+// stage) and consecutive segments per wavefront of the fused evaluation
+// (spw).  Defaults below; a table written by the autotuner
+// (python -m fgt2d_amd.autotune) in tuning.txt next to the library replaces
+// them, and FGTPW_TUNE="key=value,..." overrides both (keys segf1..segf4,
+// sege1..sege4, spw1..spw4, sort).
+struct Tuning {
+    int segf[4] = {256, 256, 128, 128};
+    int sege[4] = {256, 256, 256, 256};
+    int spw[4] = {2, 2, 2, 2};
+    int sort = 0;  // block shape of the 32-bit key sort (sort_pairs_u32)
+    std::string source = "defaults";
+    bool set(const std::string& k, int v)
+    {
+        if (k == "sort") {
+            if (v < 0 || v > 3) return false;
+            sort = v;
+            return true;
+        }
+        int* arr = k.rfind("segf", 0) == 0 ? segf : k.rfind("sege", 0) == 0 ? sege
+                   : k.rfind("spw", 0) == 0 ? spw : nullptr;
+        if (!arr || k.size() < 4) return false;
+        const int t = k.back() - '1';
+        if (t < 0 || t > 3) return false;
+        const bool ok = arr == spw ? (v >= 1 && v <= 16)
+                                   : ((v == 64 || v == 128 || v == 256) || (arr == segf && v == 512));
+        if (ok) arr[t] = v;
+        return ok;
+    }
+    void parse(const std::string& text, char sep)
+    {
+        size_t i = 0;
+        while (i < text.size()) {
+            size_t j = text.find(sep, i);
+            if (j == std::string::npos) j = text.size();
+            std::string line = text.substr(i, j - i);
+            i = j + 1;
+            const size_t h = line.find('#');
+            if (h != std::string::npos) line.resize(h);
+            const size_t eq = line.find('=');
+            if (eq == std::string::npos) continue;
+            auto trim = [](std::string x) {
+                const size_t a = x.find_first_not_of(" \t\r"), b = x.find_last_not_of(" \t\r");
+                return a == std::string::npos ? std::string() : x.substr(a, b - a + 1);
+            };
+            set(trim(line.substr(0, eq)), std::atoi(trim(line.substr(eq + 1)).c_str()));
+        }
+    }
+};
+
+const Tuning& tuning()
+{
+    static const Tuning t = [] {
+        Tuning u;
+        Dl_info info;
+        if (dladdr((const void*)&tuning, &info) && info.dli_fname) {
+            std::string path(info.dli_fname);
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning.txt";
+            if (FILE* f = std::fopen(path.c_str(), "r")) {
+                std::string text;
+                char buf[4096];
+                size_t n;
+                while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+                std::fclose(f);
+                u.parse(text, '\n');
+                u.source = path;
+            }
+        }
+        if (const char* e = std::getenv("FGTPW_TUNE")) {
+            u.parse(e, ',');
+            u.source += "+FGTPW_TUNE";
+        }
+        return u;
+    }();
+    return t;
+}
 std::mutex g_mutex;  // one transform at a time per process (shared workspace)
 int g_device = -1;
 
@@ -803,7 +884,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     else
         launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, ks, ix, s);
     if (k32)
-        sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix, isrc, ns, keybits, s);
+        sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix, isrc, ns, keybits, s, tuning().sort);
     else
         sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
     tm.end(S_KEYS);
@@ -871,7 +952,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         else
             launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, kt, ix, s);
         if (k32)
-            sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s);
+            sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s, tuning().sort);
         else
             sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
         tm.ctx(-2, nullptr, -1, nt);  // level -2 tags the gather without densities
@@ -1179,9 +1260,12 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         double* taylor = W.get<double>("taylor", o_taylor[nlevels + 1]);
         // segment numbering of every formation and evaluation list (one
         // launch): formation over sources in segments of SEGF, evaluation
-        // over targets / sources in segments of SEGE
-        auto segf = [&](int l) { int tm_ = (nlocal[l] + 16) / 16; return tm_ <= 2 ? 256 : 128; };
-        const int SEGE = 256, GRIDMAX = 4096;
+        // over targets / sources in segments of sege(l)
+        const Tuning& tune = tuning();
+        auto tmof = [&](int l) { return std::min(4, (nlocal[l] + 16) / 16); };
+        auto segf = [&](int l) { return tune.segf[tmof(l) - 1]; };
+        auto sege = [&](int l) { return tune.sege[tmof(l) - 1]; };
+        const int GRIDMAX = 4096;
         std::vector<int*> ts_form(nlevels + 1, nullptr), ts_trg(nlevels + 1, nullptr),
             ts_src(nlevels + 1, nullptr), sb_form(nlevels + 1, nullptr),
             sb_trg(nlevels + 1, nullptr), sb_src(nlevels + 1, nullptr);
@@ -1194,7 +1278,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                 if (l >= std::max(npwlevel + 1, 0) && !form_l[l].empty())
                     nseg += form_l[l].size() + ns / segf(l) + 1;
                 if (l >= nlevstart && !eval_l[l].empty())
-                    nseg += 2 * eval_l[l].size() + (size_t)(nt + ns) / SEGE + 2;
+                    nseg += 2 * eval_l[l].size() + (size_t)(nt + ns) / sege(l) + 2;
             }
             int* pool = W.get<int>("tstart", ntot);
             int* spool = W.get<int>("segbox", nseg);
@@ -1217,8 +1301,8 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                     ts_form[l] = add(form_l[l], o_form[l], src_se, segf(l), ns, &sb_form[l]);
             for (int l = nlevstart; l <= nlevels; ++l) {
                 if (eval_l[l].empty()) continue;
-                if (use_trg) ts_trg[l] = add(eval_l[l], o_eval[l], trg_se, SEGE, nt, &sb_trg[l]);
-                if (use_src) ts_src[l] = add(eval_l[l], o_eval[l], src_se, SEGE, ns, &sb_src[l]);
+                if (use_trg) ts_trg[l] = add(eval_l[l], o_eval[l], trg_se, sege(l), nt, &sb_trg[l]);
+                if (use_src) ts_src[l] = add(eval_l[l], o_eval[l], src_se, sege(l), ns, &sb_src[l]);
             }
             launch_seg_scan(L, s);
         }
@@ -1296,8 +1380,10 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                     EvalArgs eb = ea;
                     eb.tstart = side == 0 ? ts_trg[l] : ts_src[l];
                     eb.segbox = side == 0 ? sb_trg[l] : sb_src[l];
-                    eb.seg = SEGE;
-                    eb.grid = (int)std::min<long>(GRIDMAX, ea.nbox + (side == 0 ? nt : ns) / SEGE + 1);
+                    eb.seg = sege(l);
+                    eb.segmax = ea.nbox + (side == 0 ? nt : ns) / eb.seg + 1;
+                    eb.grid = (int)std::min<long>(GRIDMAX, eb.segmax);
+                    eb.spw = tune.spw[tmof(l) - 1];
                     eb.se = side == 0 ? trg_se : src_se;
                     eb.pts = side == 0 ? xt : xs;
                     eb.ifpgh = side == 0 ? q.ifpghtarg : q.ifpgh;
@@ -1620,7 +1706,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     else
         launch_keys3d(q.src, ns, cen0, bs0, nlmax, ks, ix, s);
     if (k32)
-        sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix, isrc, ns, keybits, s);
+        sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix, isrc, ns, keybits, s, tuning().sort);
     else
         sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
     tm.end(S_KEYS);
@@ -1684,7 +1770,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         else
             launch_keys3d(q.targ, nt, cen0, bs0, nlmax, kt, ix, s);
         if (k32)
-            sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s);
+            sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s, tuning().sort);
         else
             sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
         tm.ctx(-1, nullptr, -1, nt);
@@ -2921,6 +3007,21 @@ int fgtpw_set_precision(int mode)
 }
 
 int fgtpw_last_precision(void) { return t_mixed ? 1 : 0; }
+
+const char* fgtpw_tuning_table(void)
+{
+    static std::string text;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    const Tuning& t = tuning();
+    text.clear();
+    for (int i = 0; i < 4; ++i)
+        text += "segf" + std::to_string(i + 1) + "=" + std::to_string(t.segf[i]) + " sege" +
+                std::to_string(i + 1) + "=" + std::to_string(t.sege[i]) + " spw" +
+                std::to_string(i + 1) + "=" + std::to_string(t.spw[i]) + " ";
+    text += "sort=" + std::to_string(t.sort) + ";source=" + t.source;
+    return text.c_str();
+}
 
 int fgtpw_set_timing(int on)
 {

@@ -149,8 +149,9 @@ size_t sort_pairs_u64_temp(int n);
 void sort_pairs_u64(void* tmp, size_t tmpb, const unsigned long long* kin, unsigned long long* kout,
                     const unsigned* vin, unsigned* vout, int n, int endbit, hipStream_t s);
 size_t sort_pairs_u32_temp(int n);
+// shape: block shape of the onesweep kernel (tuning table key `sort`, 0..3)
 void sort_pairs_u32(void* tmp, size_t tmpb, const unsigned* kin, unsigned* kout, const unsigned* vin,
-                    unsigned* vout, int n, int endbit, hipStream_t s);
+                    unsigned* vout, int n, int endbit, hipStream_t s, int shape = 0);
 size_t select_flagged_temp(int n);
 void select_flagged(void* tmp, size_t tmpb, const unsigned char* flags, int n, int* out, int* nsel,
                     hipStream_t s);
@@ -262,6 +263,8 @@ struct EvalArgs {
     int tay32 = 0;          // 1: FP32 Taylor coefficients, evaluated on FP32 matrix cores
     int fused = 0;          // 1: the evaluation forms the Taylor coefficients itself
                             //    (no k_pw2local launch, a.taylor unused)
+    int spw = 2;            // fused: consecutive segments per wavefront
+    int segmax = 0;         // an upper bound of the segment count (boxes + points / seg + 1)
 };
 void launch_pw2local(const EvalArgs& a, hipStream_t s);
 void launch_taylor_eval(const EvalArgs& a, hipStream_t s);

@@ -271,18 +271,75 @@ void sort_pairs_u64(void* tmp, size_t tmpb, const u64* kin, u64* kout, const uns
     HIPCHECK(rocprim::radix_sort_pairs(tmp, tmpb, kin, kout, vin, vout, n, 0, endbit, s));
 }
 
-size_t sort_pairs_u32_temp(int n)
+// 32-bit keys: onesweep passes of 9, 11 or 10 bits so that the key widths
+// of the trees (2 or 3 bits per level) take two passes up to 22 bits and
+// three up to 30, instead of the 8-bit default's three and four (block
+// shapes of the gfx950 default for 4-byte keys and values).
+// The sort kernel's block shape is a tile shape of the tuning table: shape
+// 0 is the gfx950 default for 4-byte keys and values (1024 threads x 16
+// keys, about 120 blocks at 2M keys), 1..3 smaller blocks (512 x 8,
+// 256 x 16, 256 x 8) for more blocks in flight.
+template <unsigned BITS, unsigned BS, unsigned IPT>
+using Onesweep32 = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>,
+                                        rocprim::kernel_config<BS, IPT>, BITS,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+
+template <class Cfg>
+static size_t sort32_temp(int n, int endbit)
 {
     size_t b = 0;
-    HIPCHECK(rocprim::radix_sort_pairs(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                       (const unsigned*)nullptr, (unsigned*)nullptr, n, 0, 32));
+    HIPCHECK(rocprim::radix_sort_pairs<Cfg>(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                            (const unsigned*)nullptr, (unsigned*)nullptr, n, 0,
+                                            endbit));
     return b;
 }
 
-void sort_pairs_u32(void* tmp, size_t tmpb, const unsigned* kin, unsigned* kout,
-                    const unsigned* vin, unsigned* vout, int n, int endbit, hipStream_t s)
+template <unsigned BS, unsigned IPT>
+static size_t sort32_temp_all(int n)
 {
-    HIPCHECK(rocprim::radix_sort_pairs(tmp, tmpb, kin, kout, vin, vout, n, 0, endbit, s));
+    return std::max({sort32_temp<Onesweep32<9, BS, IPT>>(n, 18),
+                     sort32_temp<Onesweep32<11, BS, IPT>>(n, 22),
+                     sort32_temp<Onesweep32<10, BS, IPT>>(n, 30)});
+}
+
+size_t sort_pairs_u32_temp(int n)
+{
+    return std::max({sort32_temp_all<1024, 16>(n), sort32_temp_all<512, 8>(n),
+                     sort32_temp_all<256, 16>(n), sort32_temp_all<256, 8>(n),
+                     sort32_temp<rocprim::default_config>(n, 32)});
+}
+
+template <unsigned BS, unsigned IPT>
+static void sort32(void* tmp, size_t tmpb, const unsigned* kin, unsigned* kout,
+                   const unsigned* vin, unsigned* vout, int n, int endbit, hipStream_t s)
+{
+    using C9 = Onesweep32<9, BS, IPT>;
+    using C10 = Onesweep32<10, BS, IPT>;
+    using C11 = Onesweep32<11, BS, IPT>;
+    if (endbit <= 18)
+        HIPCHECK(rocprim::radix_sort_pairs<C9>(tmp, tmpb, kin, kout, vin, vout, n, 0, endbit, s));
+    else if (endbit <= 22)
+        HIPCHECK(rocprim::radix_sort_pairs<C11>(tmp, tmpb, kin, kout, vin, vout, n, 0, endbit, s));
+    else
+        HIPCHECK(rocprim::radix_sort_pairs<C10>(tmp, tmpb, kin, kout, vin, vout, n, 0, endbit, s));
+}
+
+void sort_pairs_u32(void* tmp, size_t tmpb, const unsigned* kin, unsigned* kout,
+                    const unsigned* vin, unsigned* vout, int n, int endbit, hipStream_t s,
+                    int shape)
+{
+    if (endbit > 30) {
+        HIPCHECK(rocprim::radix_sort_pairs(tmp, tmpb, kin, kout, vin, vout, n, 0, endbit, s));
+        return;
+    }
+    switch (shape) {
+    case 1: sort32<512, 8>(tmp, tmpb, kin, kout, vin, vout, n, endbit, s); break;
+    case 2: sort32<256, 16>(tmp, tmpb, kin, kout, vin, vout, n, endbit, s); break;
+    case 3: sort32<256, 8>(tmp, tmpb, kin, kout, vin, vout, n, endbit, s); break;
+    default: sort32<1024, 16>(tmp, tmpb, kin, kout, vin, vout, n, endbit, s); break;
+    }
 }
 
 size_t select_flagged_temp(int n)
@@ -1951,16 +2008,26 @@ __global__ __launch_bounds__(256) void k_taylor_wave(EvalArgs a)
     double* sy = stage[threadIdx.x >> 6][1];
     const int col = lane & 15, kq = lane >> 4;
     const int total = seg_count(a.tstart, a.nbox);
-    for (int g = blockIdx.x * 4 + (threadIdx.x >> 6); g < total; g += gridDim.x * 4) {
+    // fused: each wave takes a run of a.spw consecutive segments (mostly of
+    // one leaf) and forms a leaf's coefficients once per run; unfused: the
+    // segments stride over the waves
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int g_lo = FUSE ? wid * a.spw : wid, g_hi = FUSE ? min(total, g_lo + a.spw) : total;
+    const int g_step = FUSE ? 1 : gridDim.x * 4;
+    int last_li = -1;
+    for (int g = g_lo; g < g_hi; g += g_step) {
         const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g, a.segbox);
         const int b = a.boxes[sr.li];
         const double* Lg = a.taylor + (long)sr.li * m * m;  // L(j,k) at j + m k
         double* Lb = nullptr;                                // fused: L(j,k) at j + MP k
         if (FUSE) {
             Lb = dsm + (threadIdx.x >> 6) * PW2L_BUF(TM, FUSE);
-            const long nexp = (long)(a.npw / 2) * a.npw;
-            pw2local_wave<TM, (FUSE > 0 ? FUSE : 1), E>(
-                a, (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp, Lb, lane);
+            if (sr.li != last_li) {  // wave-uniform
+                const long nexp = (long)(a.npw / 2) * a.npw;
+                pw2local_wave<TM, (FUSE > 0 ? FUSE : 1), E>(
+                    a, (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp, Lb, lane);
+                last_li = sr.li;
+            }
         }
         auto Lv = [&](int j, int k) -> double { return FUSE ? Lb[j + MP * k] : Lg[j + m * k]; };
         double A[NV][KS][TM];
@@ -2100,16 +2167,26 @@ __global__ __launch_bounds__(256) void k_taylor_wave_f32(EvalArgs a)
     float* sy = stage[threadIdx.x >> 6][1];
     const int col = lane & 15, kq = lane >> 4;
     const int total = seg_count(a.tstart, a.nbox);
-    for (int g = blockIdx.x * 4 + (threadIdx.x >> 6); g < total; g += gridDim.x * 4) {
+    // fused: each wave takes a run of a.spw consecutive segments (mostly of
+    // one leaf) and forms a leaf's coefficients once per run; unfused: the
+    // segments stride over the waves
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int g_lo = FUSE ? wid * a.spw : wid, g_hi = FUSE ? min(total, g_lo + a.spw) : total;
+    const int g_step = FUSE ? 1 : gridDim.x * 4;
+    int last_li = -1;
+    for (int g = g_lo; g < g_hi; g += g_step) {
         const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g, a.segbox);
         const int b = a.boxes[sr.li];
         const float* Lg = (const float*)a.taylor + (long)sr.li * m * m;  // L(j,k) at j + m k
         double* Lb = nullptr;  // fused: FP64 L(j,k) at j + MP k, rounded to FP32 below
         if (FUSE) {
             Lb = dsm + (threadIdx.x >> 6) * PW2L_BUF(TM, FUSE);
-            const long nexp = (long)(a.npw / 2) * a.npw;
-            pw2local_wave<TM, (FUSE > 0 ? FUSE : 1), E>(
-                a, (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp, Lb, lane);
+            if (sr.li != last_li) {  // wave-uniform
+                const long nexp = (long)(a.npw / 2) * a.npw;
+                pw2local_wave<TM, (FUSE > 0 ? FUSE : 1), E>(
+                    a, (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp, Lb, lane);
+                last_li = sr.li;
+            }
         }
         // (j+1) L and (j+1)(j+2) L as in the unfused form: FP32 products of
         // the FP32 coefficient
@@ -2239,19 +2316,27 @@ __global__ __launch_bounds__(256) void k_taylor_wave_f32(EvalArgs a)
 
 // FUSE = NBH: PW -> Taylor inside the wave (a.fused), from FP64 or (a.exp32)
 // FP32 expansions
+// fused launches: a.spw segments per wave over at most a.segmax segments
+static unsigned fused_grid(const EvalArgs& a)
+{
+    if (a.spw < 1 || a.segmax < 1) fail(-1, "fused evaluation without a segment bound");
+    return (unsigned)std::max(1, (a.segmax + 4 * a.spw - 1) / (4 * a.spw));
+}
+
 template <int TM, int FUSE, class E>
 static void taylor_wave_f32_launch(const EvalArgs& a, hipStream_t s)
 {
     const unsigned lds = FUSE ? 4 * PW2L_BUF(TM, FUSE) * sizeof(double) : 0;
+    const unsigned grid = FUSE ? fused_grid(a) : (unsigned)a.grid;
     if (a.ifpgh >= 3) {
         allow_lds(k_taylor_wave_f32<TM, 3, FUSE, E>, lds + sizeof(float) * 8 * WSEG);
-        klaunch(KID_TAYLOR_EVAL, k_taylor_wave_f32<TM, 3, FUSE, E>, a.grid, 256, lds, s, a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave_f32<TM, 3, FUSE, E>, grid, 256, lds, s, a);
     } else if (a.ifpgh == 2) {
         allow_lds(k_taylor_wave_f32<TM, 2, FUSE, E>, lds + sizeof(float) * 8 * WSEG);
-        klaunch(KID_TAYLOR_EVAL, k_taylor_wave_f32<TM, 2, FUSE, E>, a.grid, 256, lds, s, a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave_f32<TM, 2, FUSE, E>, grid, 256, lds, s, a);
     } else {
         allow_lds(k_taylor_wave_f32<TM, 1, FUSE, E>, lds + sizeof(float) * 8 * WSEG);
-        klaunch(KID_TAYLOR_EVAL, k_taylor_wave_f32<TM, 1, FUSE, E>, a.grid, 256, lds, s, a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave_f32<TM, 1, FUSE, E>, grid, 256, lds, s, a);
     }
 }
 
@@ -2259,15 +2344,16 @@ template <int TM, int FUSE, class E>
 static void taylor_wave_launch(const EvalArgs& a, hipStream_t s)
 {
     const unsigned lds = FUSE ? 4 * PW2L_BUF(TM, FUSE) * sizeof(double) : 0;
+    const unsigned grid = FUSE ? fused_grid(a) : (unsigned)a.grid;
     if (a.ifpgh >= 3) {
         allow_lds(k_taylor_wave<TM, 3, FUSE, E>, lds + sizeof(double) * 8 * WSEG);
-        klaunch(KID_TAYLOR_EVAL, k_taylor_wave<TM, 3, FUSE, E>, a.grid, 256, lds, s, a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave<TM, 3, FUSE, E>, grid, 256, lds, s, a);
     } else if (a.ifpgh == 2) {
         allow_lds(k_taylor_wave<TM, 2, FUSE, E>, lds + sizeof(double) * 8 * WSEG);
-        klaunch(KID_TAYLOR_EVAL, k_taylor_wave<TM, 2, FUSE, E>, a.grid, 256, lds, s, a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave<TM, 2, FUSE, E>, grid, 256, lds, s, a);
     } else {
         allow_lds(k_taylor_wave<TM, 1, FUSE, E>, lds + sizeof(double) * 8 * WSEG);
-        klaunch(KID_TAYLOR_EVAL, k_taylor_wave<TM, 1, FUSE, E>, a.grid, 256, lds, s, a);
+        klaunch(KID_TAYLOR_EVAL, k_taylor_wave<TM, 1, FUSE, E>, grid, 256, lds, s, a);
     }
 }
 

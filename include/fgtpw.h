@@ -157,6 +157,17 @@ int fgtpw_set_precision(int mode);
 int fgtpw_last_precision(void);
 
 /*
+ * The tile shapes the 2D kernels use, as "segf1=.. sege1=.. spw1=.. ...
+ * sort=..;source=<where from>": points per formation and evaluation segment and
+ * segments per wavefront of the fused evaluation for each order tile count
+ * (orders up to 16, 32, 48, 64), and the radix sort's block shape.  They
+ * come from defaults, from tuning.txt beside the library (written by
+ * python -m fgt2d_amd.autotune) and from FGTPW_TUNE="key=value,...", read
+ * once per process.  No reference counterpart.
+ */
+const char *fgtpw_tuning_table(void);
+
+/*
  * Stage timing of the next transforms: when on (1), each call records HIP
  * events around its stages on its stream and fgtpw_stage_times() returns the
  * milliseconds of the last call per stage, in the order of
