@@ -52,11 +52,13 @@ bool g_serial = g_serial_env;
 // plane-wave expansions and translation factors)
 int g_precision = 0;
 constexpr double kMixedEpsMin = 1e-6;
-// 2D evaluation of Taylor orders below 32: PW -> Taylor inside the
-// evaluation waves (1, default) or as its own kernel (FGTPW_FUSE_PW2LOCAL=0)
+// 2D evaluation of Taylor orders below 32: PW -> Taylor as its own
+// matrix-core kernel, one wavefront per leaf (default), or inside the
+// evaluation waves (FGTPW_FUSE_PW2LOCAL=1; measured slower: the conversion's
+// latency is not hidden at the evaluation kernel's occupancy)
 const bool g_fuse_pw2local = [] {
     const char* e = std::getenv("FGTPW_FUSE_PW2LOCAL");
-    return !(e && e[0] == '0');
+    return e && e[0] == '1';
 }();
 // Tile shapes of the 2D segmented kernels, per Taylor / Hermite order tile
 // count TM (orders up to 16 TM): points per formation segment (segf),

@@ -1662,9 +1662,12 @@ __global__ __launch_bounds__(256) void k_pw2local(EvalArgs a)
     }
 }
 
+static bool launch_pw2local_wave(const EvalArgs& a, hipStream_t s);
+
 void launch_pw2local(const EvalArgs& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
+    if (launch_pw2local_wave(a, s)) return;  // matrix cores, one wavefront per leaf
     int half = a.npw / 2, m = a.nl + 1;
     size_t lds = (size_t)(2 * half * a.npw + 2 * m * a.npw) * sizeof(double);
     if (a.exp32 && a.tay32)
@@ -1986,6 +1989,61 @@ __device__ __forceinline__ void pw2local_wave(const EvalArgs& a, const E* __rest
                 buf[k + MP * l] = (k < m && l < m) ? v : 0.0;
             }
     wave_lds_sync();
+}
+
+// Step 5a on matrix cores as its own kernel: one wavefront per leaf (four
+// per block), pw2local_wave into the wave's LDS slice, then L(k,l) stored
+// at k + m l as TT (FP32 on the mixed-precision route) for the Taylor
+// evaluation kernels.
+template <int TM, int NBH, class E, class TT>
+__global__ __launch_bounds__(256) void k_pw2local_wave(EvalArgs a)
+{
+    constexpr int MP = 16 * TM;
+    extern __shared__ double dsm[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int bi = blockIdx.x * 4 + w;
+    if (bi >= a.nbox) return;  // wave-uniform: no block barrier below
+    const int b = a.boxes[bi], m = a.nl + 1;
+    double* buf = dsm + w * PW2L_BUF(TM, NBH);
+    const long nexp = (long)(a.npw / 2) * a.npw;
+    pw2local_wave<TM, NBH, E>(a, (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp, buf,
+                              lane);
+    TT* L = (TT*)a.taylor + (long)bi * m * m;
+    for (int o = lane; o < m * m; o += 64) L[o] = (TT)buf[o % m + MP * (o / m)];
+}
+
+template <int TM, int NBH>
+static void pw2local_wave_launch(const EvalArgs& a, hipStream_t s)
+{
+    const unsigned lds = 4 * PW2L_BUF(TM, NBH) * sizeof(double);
+    const unsigned grid = (unsigned)((a.nbox + 3) / 4);
+#define FGT_P2L(E_, T_)                                                                        \
+    {                                                                                          \
+        allow_lds(k_pw2local_wave<TM, NBH, E_, T_>, lds);                                      \
+        klaunch(KID_PW2LOCAL, k_pw2local_wave<TM, NBH, E_, T_>, grid, 256, lds, s, a);         \
+    }
+    if (a.exp32 && a.tay32)
+        FGT_P2L(float, float)
+    else if (a.exp32)
+        FGT_P2L(float, double)
+    else
+        FGT_P2L(double, double)
+#undef FGT_P2L
+}
+
+// orders up to 48 and npw <= 64; false when the VALU kernel has to run
+static bool launch_pw2local_wave(const EvalArgs& a, hipStream_t s)
+{
+    const int m = a.nl + 1, nbh = (a.npw / 2 + 15) / 16;
+    if (m > 48 || a.npw > 64) return false;
+    const int tm = (m + 15) / 16;
+    if (tm == 1 && nbh == 1) pw2local_wave_launch<1, 1>(a, s);
+    else if (tm == 1) pw2local_wave_launch<1, 2>(a, s);
+    else if (tm == 2 && nbh == 1) pw2local_wave_launch<2, 1>(a, s);
+    else if (tm == 2) pw2local_wave_launch<2, 2>(a, s);
+    else if (nbh == 1) pw2local_wave_launch<3, 1>(a, s);
+    else pw2local_wave_launch<3, 2>(a, s);
+    return true;
 }
 
 // Step 5b on matrix cores, one wavefront per segment (at most WSEG points):
