@@ -107,7 +107,7 @@ struct TreeD {
     // leaves in Morton order and, per box, its interval of leaf ranks
     std::vector<int> leaf_of_rank, rank_of_box, rank_lo, rank_hi;
     // builder scratch, kept to reuse its storage
-    std::vector<int> map, tail;
+    std::vector<int> map, tail, tail2;
 };
 using Tree2D = TreeD<2>;
 using Tree3D = TreeD<3>;

@@ -389,11 +389,16 @@ struct Builder {
         ncol[b] = n;
     }
 
-    void all_colleagues(int maxlev)
+    // lists of the boxes of levels lo+1..hi from those of level lo (lo >= 0;
+    // the root's list is set at lo = 0)
+    void colleagues_between(int lo, int hi)
     {
-        ncol[0] = 1;
-        col[0] = 0;
-        for (int l = 0; l < std::min(T.nlevels, maxlev); ++l) {
+        if (lo <= 0) {
+            lo = 0;
+            ncol[0] = 1;
+            col[0] = 0;
+        }
+        for (int l = lo; l < std::min(T.nlevels, hi); ++l) {
             const int b0 = T.laddr[2 * l];
             par_range(T.laddr[2 * l + 1] - b0 + 1, 512, [&](long i0, long i1) {
                 for (long i = i0; i < i1; ++i)
@@ -401,6 +406,7 @@ struct Builder {
             });
         }
     }
+    void all_colleagues(int maxlev) { colleagues_between(0, maxlev); }
 
     // Refine the boxes of [b0,b1] whose flag is positive; children of flag-1
     // boxes are marked 3 (still to be checked), of flag-2 boxes 0
@@ -420,63 +426,120 @@ struct Builder {
         }
     }
 
-    // Renumber so that each level lists its original boxes, then the boxes
-    // appended to it (pts_tree_reorg, pts_tree2d4.f:1001-1150,
-    // pts_tree3d.f:1172-1329).
+    // The reference's final numbering (pts_tree_reorg, pts_tree2d4.f:1001-1150,
+    // pts_tree3d.f:1172-1329, which runs after each restriction pass): every
+    // level lists the boxes of the adaptive tree, then those the first pass
+    // appended to it, then those of the second pass, each group in creation
+    // order.  The passes run on creation indices; this one permutation then
+    // renumbers every per-box array and every stored box index (parents,
+    // children, colleague lists).
+    // The permutation moves whole runs of boxes (a level's group of one
+    // pass, in creation order) to contiguous places.
+    struct Run {
+        int from, to, n;  // first old index, first new index, count
+    };
     template <class V>
-    static void permute(V& v, int width, const std::vector<int>& map)
+    static void permute(V& v, int width, const std::vector<Run>& runs)
     {
         using E = typename V::value_type;
         thread_local std::vector<char> scratch;
         if (scratch.size() < v.size() * sizeof(E)) scratch.resize(v.size() * sizeof(E));
         E* old = (E*)scratch.data();
         std::memcpy(old, v.data(), v.size() * sizeof(E));
-        E* dst = v.data();
-        const int* mp = map.data();
-        for (size_t b = 0; b < map.size(); ++b)
-            for (int k = 0; k < width; ++k) dst[(size_t)width * mp[b] + k] = old[(size_t)width * b + k];
+        for (const Run& r : runs)
+            std::memcpy(v.data() + (size_t)width * r.to, old + (size_t)width * r.from,
+                        sizeof(E) * width * (size_t)r.n);
     }
-    void reorg(const std::vector<int>& tail)
+    void renumber(const std::vector<int>& t1, const std::vector<int>& t2)
     {
         const int n = T.nboxes();
-        std::vector<int>& map = T.map;
-        map.resize(n);
-        for (int l = 0; l <= 1 && l <= T.nlevels; ++l)
-            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) map[b] = b;
-        int cur = T.laddr[4];
-        for (int l = 2; l <= T.nlevels; ++l) {
-            int f0 = T.laddr[2 * l], f1 = T.laddr[2 * l + 1];
+        thread_local std::vector<Run> runs;
+        runs.clear();
+        int cur = 0;
+        bool moved = false;
+        for (int l = 0; l <= T.nlevels; ++l) {
+            const int f0 = T.laddr[2 * l], f1 = T.laddr[2 * l + 1];
             T.laddr[2 * l] = cur;
-            for (int b = f0; b <= f1; ++b) map[b] = cur++;
-            for (int b = tail[2 * l]; b <= tail[2 * l + 1]; ++b) map[b] = cur++;
+            const int grp[3][2] = {{f0, f1}, {t1[2 * l], t1[2 * l + 1]}, {t2[2 * l], t2[2 * l + 1]}};
+            for (const auto& g : grp) {
+                const int b0 = g[0], b1 = g[1];
+                if (b1 < b0) continue;
+                moved = moved || b0 != cur;
+                runs.push_back(Run{b0, cur, b1 - b0 + 1});
+                cur += b1 - b0 + 1;
+            }
             T.laddr[2 * l + 1] = cur - 1;
         }
-        // the arrays are independent: one task each (large trees only)
-        auto each = [&](long i0, long i1) {
+        if (!moved) return;  // the identity
+        // new index of every old one, shifted by one so that -1 maps to -1
+        std::vector<int>& map = T.map;
+        map.resize(n + 1);
+        map[0] = -1;
+        for (const Run& r : runs)
+            for (int i = 0; i < r.n; ++i) map[r.from + i + 1] = r.to + i;
+        // the arrays are independent: one task each (large trees only; the
+        // tasks may run on other threads, hence the run table by pointer)
+        const std::vector<Run>* rp = &runs;
+        auto each = [&, rp](long i0, long i1) {
+            const std::vector<Run>& R = *rp;
             for (long k = i0; k < i1; ++k) switch (k) {
-                    case 0: permute(T.lev, 1, map); break;
-                    case 1: permute(T.nch, 1, map); break;
-                    case 2: permute(T.cen, D, map); break;
-                    case 3: permute(T.flag, 1, map); break;
-                    case 4: permute(T.pre, 1, map); break;
-                    case 5: permute(T.par, 1, map); break;
-                    case 6: permute(T.ch, CH, map); break;
-                    default: permute(T.ic[k - 7], 1, map); break;
+                    case 0: permute(T.lev, 1, R); break;
+                    case 1: permute(T.nch, 1, R); break;
+                    case 2: permute(T.cen, D, R); break;
+                    case 3: permute(T.flag, 1, R); break;
+                    case 4: permute(T.pre, 1, R); break;
+                    case 5: permute(T.par, 1, R); break;
+                    case 6: permute(T.ch, CH, R); break;
+                    case 7: permute(T.ncol, 1, R); break;
+                    case 8: permute(T.col, NC, R); break;
+                    default: permute(T.ic[k - 9], 1, R); break;
                 }
         };
         if (n >= 32768)
-            par_range(7 + D, 1, each);
+            par_range(9 + D, 1, each);
         else
-            each(0, 7 + D);
+            each(0, 9 + D);
         view();
-        const int* mp = map.data();
+        // stored indices (-1 for none: children of leaves, unused list slots)
+        const int* mp = map.data() + 1;
         par_range(n, 4096, [&](long i0, long i1) {
-            for (long b = i0; b < i1; ++b) {
-                if (par[b] >= 0) par[b] = mp[par[b]];
-                for (int j = 0; j < CH; ++j)
-                    if (ch[CH * b + j] >= 0) ch[CH * b + j] = mp[ch[CH * b + j]];
-            }
+            for (long b = i0; b < i1; ++b) par[b] = mp[par[b]];
+            for (long k = CH * i0; k < CH * i1; ++k) ch[k] = mp[ch[k]];
+            for (long k = NC * i0; k < NC * i1; ++k) col[k] = mp[col[k]];
         });
+    }
+
+    // Colleague lists after the first restriction pass, as recomputing them
+    // from the root would give them, for levels up to hi.  The pass only adds
+    // children to leaves; a box's list depends on its parent's list and on
+    // the children of the parent's colleagues, so the lists that change are
+    // those of the children of boxes touching a refined box (colleague
+    // relation is symmetric).  The boxes the pass adds to a list of level l
+    // have no children, so the lists of level l+1 do not depend on that
+    // change.  Level by level: a list is read only after its own level is
+    // done.
+    void refresh_after_first_pass(int hi)
+    {
+        thread_local std::vector<char> mark;
+        thread_local std::vector<int> todo;
+        mark.assign(T.nboxes(), 0);
+        for (int l = 1; l < std::min(hi, T.nlevels - 1); ++l) {
+            todo.clear();
+            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
+                if (flag[b] <= 0) continue;  // refined by the pass
+                for (int i = 0; i < ncol[b]; ++i) {
+                    const int c = col[NC * b + i];
+                    if (nch[c] > 0 && !mark[c]) {
+                        mark[c] = 1;
+                        todo.push_back(c);
+                    }
+                }
+            }
+            const int* tp = todo.data();  // this thread's list, read by the workers
+            par_range((long)todo.size(), 256, [&, tp](long i0, long i1) {
+                for (long i = i0; i < i1; ++i) family_colleagues(tp[i]);
+            });
+        }
     }
 
     // flag++ boxes with a touching grandchild-level box that has children are
@@ -507,8 +570,9 @@ struct Builder {
     }
 
     // Level restriction, pts_tree_fix_lr (pts_tree2d4.f:693-996,
-    // pts_tree3d.f:855-1171).  Returns the deepest level whose colleague
-    // lists are still valid afterwards.
+    // pts_tree3d.f:855-1171), on colleague lists valid through level L-2 (the
+    // deepest the procedure reads).  Returns the deepest level whose
+    // colleague lists are still valid afterwards.
     int fix_level_restriction()
     {
         const int L = T.nlevels;
@@ -539,7 +603,7 @@ struct Builder {
                 if (nf) nflag.fetch_add(nf);
             });
         }
-        if (nflag.load() == 0) return L - 1;  // already level restricted: nothing below changes
+        if (nflag.load() == 0) return L - 2;  // already level restricted: nothing below changes
         // coarser leaves touching a flagged box, propagated upwards
         for (int l = L; l >= 1; --l) {
             const int b0 = T.laddr[2 * l];
@@ -557,37 +621,47 @@ struct Builder {
                 }
             });
         }
-        std::vector<int>& tail = T.tail;
-        tail.assign(2 * (L + 1), 0);
-        for (int l = 0; l <= L; ++l) tail[2 * l + 1] = -1;
+        // boxes appended to each level by the first (t1) and second (t2) pass;
+        // a level's boxes are its range in laddr, then t1, then t2
+        std::vector<int>& t1 = T.tail;
+        std::vector<int>& t2 = T.tail2;
+        t1.assign(2 * (L + 1), 0);
+        t2.assign(2 * (L + 1), 0);
+        for (int l = 0; l <= L; ++l) t1[2 * l + 1] = t2[2 * l + 1] = -1;
         for (int l = 1; l <= L - 2; ++l) {
-            tail[2 * (l + 1)] = T.nboxes();
+            t1[2 * (l + 1)] = T.nboxes();
             refine_flagged(T.laddr[2 * l], T.laddr[2 * l + 1]);
-            tail[2 * (l + 1) + 1] = T.nboxes() - 1;
+            t1[2 * (l + 1) + 1] = T.nboxes() - 1;
         }
-        reorg(tail);
-        all_colleagues(L - 1);
+        refresh_after_first_pass(L - 2);
         int n3 = 0;
         for (int b = 0, n = T.nboxes(); b < n; ++b) {
             if (flag[b] != 3) flag[b] = 0;
             else ++n3;
         }
-        if (n3 == 0) return L - 1;  // no flag++ box: the second pass adds nothing
-        for (int l = 0; l <= L; ++l) { tail[2 * l] = 0; tail[2 * l + 1] = -1; }
-        for (int l = 2; l <= L - 2; ++l) {
-            update_flags(T.laddr[2 * l], T.laddr[2 * l + 1]);
-            update_flags(tail[2 * l], tail[2 * l + 1]);
-            tail[2 * (l + 1)] = T.nboxes();
-            refine_flagged(T.laddr[2 * l], T.laddr[2 * l + 1]);
-            refine_flagged(tail[2 * l], tail[2 * l + 1]);
-            tail[2 * (l + 1) + 1] = T.nboxes() - 1;
-            const int c0 = tail[2 * (l + 1)];
-            par_range(tail[2 * (l + 1) + 1] - c0 + 1, 1024, [&](long i0, long i1) {
-                for (long i = i0; i < i1; ++i) colleagues(c0 + (int)i);
-            });
+        int valid = L - 2;
+        if (n3 > 0) {  // flag++ boxes: the second pass
+            for (int l = 2; l <= L - 2; ++l) {
+                update_flags(T.laddr[2 * l], T.laddr[2 * l + 1]);
+                update_flags(t1[2 * l], t1[2 * l + 1]);
+                update_flags(t2[2 * l], t2[2 * l + 1]);
+                t2[2 * (l + 1)] = T.nboxes();
+                refine_flagged(T.laddr[2 * l], T.laddr[2 * l + 1]);
+                refine_flagged(t1[2 * l], t1[2 * l + 1]);
+                refine_flagged(t2[2 * l], t2[2 * l + 1]);
+                t2[2 * (l + 1) + 1] = T.nboxes() - 1;
+                const int c0 = t2[2 * (l + 1)];
+                // the new boxes get lists; the lists of the boxes of level l+1
+                // that they touch are not updated (the reference does not
+                // either, and the pass reads them in that state)
+                if (T.nboxes() > c0) valid = std::min(valid, l);
+                par_range(T.nboxes() - c0, 1024, [&](long i0, long i1) {
+                    for (long i = i0; i < i1; ++i) colleagues(c0 + (int)i);
+                });
+            }
         }
-        reorg(tail);
-        return -1;
+        renumber(t1, t2);
+        return valid;
     }
 };
 
@@ -692,15 +766,16 @@ void build_tree(TreeD<D>& T, const std::vector<LeafRun>& leaves, int nlmax, int 
     }
     T.nlevels = l;
     // Colleague lists: the level restriction reads them down to level
-    // nlevels-1; the transform itself only at and above the cutoff level
+    // nlevels-2; the transform itself only at and above the cutoff level
     // (m2l, iflocal, near-field lists).  Deeper lists are filled on demand
     // by complete_colleagues().
-    B.all_colleagues(T.nlevels - 1);
-    int valid = T.nlevels - 1;
-    if (T.nlevels >= 2) valid = B.fix_level_restriction();
-    const int keep = std::min(std::max(npwlevel, 0), T.nlevels);
+    const int L = T.nlevels;
+    B.all_colleagues(L - 2);
+    int valid = std::max(L - 2, 0);
+    if (L >= 2) valid = B.fix_level_restriction();
+    const int keep = std::min(std::max(npwlevel, 0), L);
     if (valid < keep) {
-        B.all_colleagues(keep);
+        B.colleagues_between(valid, keep);
         valid = keep;
     }
     T.coll_levels = valid;
