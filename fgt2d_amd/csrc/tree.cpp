@@ -450,7 +450,9 @@ struct Builder {
             std::memcpy(v.data() + (size_t)width * r.to, old + (size_t)width * r.from,
                         sizeof(E) * width * (size_t)r.n);
     }
-    void renumber(const std::vector<int>& t1, const std::vector<int>& t2)
+    // colleague lists are remapped for levels up to `valid` (the deeper ones
+    // are stale and recomputed before anything reads them)
+    void renumber(const std::vector<int>& t1, const std::vector<int>& t2, int valid)
     {
         const int n = T.nboxes();
         thread_local std::vector<Run> runs;
@@ -502,10 +504,11 @@ struct Builder {
         view();
         // stored indices (-1 for none: children of leaves, unused list slots)
         const int* mp = map.data() + 1;
+        const long nc = valid >= 0 ? T.laddr[2 * std::min(valid, T.nlevels) + 1] + 1 : 0;
         par_range(n, 4096, [&](long i0, long i1) {
             for (long b = i0; b < i1; ++b) par[b] = mp[par[b]];
             for (long k = CH * i0; k < CH * i1; ++k) ch[k] = mp[ch[k]];
-            for (long k = NC * i0; k < NC * i1; ++k) col[k] = mp[col[k]];
+            for (long k = NC * i0; k < NC * std::min(i1, nc); ++k) col[k] = mp[col[k]];
         });
     }
 
@@ -660,7 +663,7 @@ struct Builder {
                 });
             }
         }
-        renumber(t1, t2);
+        renumber(t1, t2, valid);
         return valid;
     }
 };
@@ -714,8 +717,40 @@ void build_tree(TreeD<D>& T, const std::vector<LeafRun>& leaves, int nlmax, int 
         ~Scope() { t_par = prev; }
     } scope(&par);
     Builder<D> B(T);
+    // The refined boxes are the proper prefixes of the leaves.  In Morton
+    // order a leaf shares its ancestors down to the level where its prefix
+    // first differs from the previous leaf's and brings new refined boxes
+    // below that; the most recent refined box of a level is the parent of
+    // the next level's new ones.  Level l+1 holds the children of the
+    // refined boxes of level l in their order, so one pass counts the
+    // refined boxes per level (the level sizes) and a second creates every
+    // box at its final index: the numbering of the level-by-level
+    // construction.
+    const int nleaf = (int)leaves.size();
+    auto first_new = [&](int i) {  // the shallowest level with a box new for leaf i
+        if (i == 0) return 0;
+        const LeafRun& p = leaves[i - 1];
+        const LeafRun& q = leaves[i];
+        const int m = std::min(p.level, q.level);
+        const uint64_t x = (p.key >> (D * (p.level - m))) ^ (q.key >> (D * (q.level - m)));
+        if (x == 0) return m + 1;  // not two disjoint leaves: nothing shared below m
+        return m - (63 - __builtin_clzll(x)) / D;
+    };
+    int maxlev = 0;
+    for (const LeafRun& r : leaves) maxlev = std::max(maxlev, r.level);
+    std::vector<int> cnt(maxlev + 1, 0), first(maxlev + 2, 0), cur(maxlev + 1, 0);
+    for (int i = 0; i < nleaf; ++i)
+        for (int l = first_new(i); l < leaves[i].level; ++l) ++cnt[l];
+    int total = 1;
+    for (int l = 0; l <= maxlev; ++l) {
+        first[l + 1] = total;
+        total += CH * cnt[l];
+    }
+    int nlev = 0;
+    while (nlev < maxlev && cnt[nlev] > 0) ++nlev;
     T.bsz.push_back(bs0);
-    B.grow(1);
+    for (int l = 0; l < nlev; ++l) T.bsz.push_back(T.bsz[l] / 2);
+    B.grow(total);
     B.lev[0] = 0;
     B.par[0] = -1;
     for (int a = 0; a < D; ++a) {
@@ -723,48 +758,22 @@ void build_tree(TreeD<D>& T, const std::vector<LeafRun>& leaves, int nlmax, int 
         B.ic[a][0] = 0;
     }
     B.pre[0] = 0;
-    T.laddr = {0, 0};
-    // frontier: boxes of the current level with their range of leaf records
-    std::vector<int> fbox{0}, fa{0}, fe{(int)leaves.size()}, nb, na, ne;
-    int l = 0;
-    for (;;) {
-        int nref = 0;
-        for (size_t i = 0; i < fbox.size(); ++i)
-            if (fa[i] < fe[i] && leaves[fa[i]].level > l) ++nref;
-        if (nref == 0) break;
-        T.bsz.push_back(T.bsz[l] / 2);
-        const int first = B.grow(CH * nref);
-        int c0 = first;
-        nb.clear();
-        na.clear();
-        ne.clear();
-        for (size_t i = 0; i < fbox.size(); ++i) {
-            int a = fa[i], e = fe[i];
-            if (!(a < e && leaves[a].level > l)) continue;
-            B.set_children(fbox[i], c0);
-            int pos = a;
-            for (int d = 0; d < CH; ++d) {
-                int s = pos;
-                while (pos < e) {
-                    const LeafRun& r = leaves[pos];
-                    int digit = (int)((r.key >> (D * (r.level - (l + 1)))) & (CH - 1));
-                    if (digit != d) break;
-                    ++pos;
-                }
-                nb.push_back(c0 + d);
-                na.push_back(s);
-                ne.push_back(pos);
-            }
-            c0 += CH;
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int i = 0; i < nleaf; ++i) {
+        const LeafRun& r = leaves[i];
+        for (int l = first_new(i); l < r.level; ++l) {
+            const int digit = (int)((r.key >> (D * (r.level - l))) & (CH - 1));
+            const int b = l == 0 ? 0 : first[l] + CH * cur[l - 1] + digit;
+            cur[l] = cnt[l]++;
+            B.set_children(b, first[l + 1] + CH * cur[l]);
         }
-        T.laddr.push_back(first);
-        T.laddr.push_back(T.nboxes() - 1);
-        fbox.swap(nb);
-        fa.swap(na);
-        fe.swap(ne);
-        ++l;
     }
-    T.nlevels = l;
+    T.laddr.assign(2 * (nlev + 1), 0);
+    for (int l = 0; l <= nlev; ++l) {
+        T.laddr[2 * l] = first[l];
+        T.laddr[2 * l + 1] = first[l + 1] - 1;
+    }
+    T.nlevels = nlev;
     // Colleague lists: the level restriction reads them down to level
     // nlevels-2; the transform itself only at and above the cutoff level
     // (m2l, iflocal, near-field lists).  Deeper lists are filled on demand
@@ -780,40 +789,43 @@ void build_tree(TreeD<D>& T, const std::vector<LeafRun>& leaves, int nlmax, int 
     }
     T.coll_levels = valid;
 
-    // leaves in Morton order: depth-first, children in digit order
-    int nbox = T.nboxes();
+    // leaves in Morton order (depth-first, children in digit order) and the
+    // interval of leaf ranks of every box: the leaf counts bottom-up, then
+    // each child's first rank from its parent's and its elder siblings'
+    const int nbox = T.nboxes();
     T.rank_of_box.assign(nbox, -1);
     T.rank_lo.assign(nbox, 0);
     T.rank_hi.assign(nbox, 0);
-    {
-        std::vector<int>& stack = T.map;
-        stack.assign(1, 0);
-        while (!stack.empty()) {
-            int b = stack.back();
-            stack.pop_back();
-            if (T.nch[b] == 0) {
-                T.rank_of_box[b] = (int)T.leaf_of_rank.size();
-                T.leaf_of_rank.push_back(b);
-            } else {
-                for (int j = CH - 1; j >= 0; --j) stack.push_back(T.ch[CH * b + j]);
-            }
-        }
-    }
+    int* lo = T.rank_lo.data();
+    int* hi = T.rank_hi.data();  // the leaf count first, the interval's end after
+    const int* nc = T.nch.data();
+    const int* chd = T.ch.data();
     for (int lv = T.nlevels; lv >= 0; --lv)
         for (int b = T.laddr[2 * lv]; b <= T.laddr[2 * lv + 1]; ++b) {
-            if (T.nch[b] == 0) {
-                T.rank_lo[b] = T.rank_of_box[b];
-                T.rank_hi[b] = T.rank_of_box[b] + 1;
-            } else {
-                int lo = 1 << 30, hi = -1;
-                for (int j = 0; j < CH; ++j) {
-                    int c = T.ch[CH * b + j];
-                    lo = std::min(lo, T.rank_lo[c]);
-                    hi = std::max(hi, T.rank_hi[c]);
-                }
-                T.rank_lo[b] = lo;
-                T.rank_hi[b] = hi;
+            int k = 1;
+            if (nc[b] > 0) {
+                k = 0;
+                for (int j = 0; j < CH; ++j) k += hi[chd[CH * b + j]];
             }
+            hi[b] = k;
+        }
+    T.leaf_of_rank.resize(hi[0]);
+    lo[0] = 0;
+    for (int lv = 0; lv <= T.nlevels; ++lv)
+        for (int b = T.laddr[2 * lv]; b <= T.laddr[2 * lv + 1]; ++b) {
+            const int r = lo[b];
+            if (nc[b] > 0) {
+                int next = r;
+                for (int j = 0; j < CH; ++j) {
+                    const int c = chd[CH * b + j];
+                    lo[c] = next;
+                    next += hi[c];
+                }
+            } else {
+                T.rank_of_box[b] = r;
+                T.leaf_of_rank[r] = b;
+            }
+            hi[b] += r;
         }
 }
 
