@@ -367,7 +367,8 @@ Workspace& workspace()
 
 const char* const kKernels[KID_N] = {"form_moments", "form_h2pw", "form_valu", "merge",
                                      "m2l",          "split",     "pw2local",  "taylor_eval",
-                                     "pw_eval",      "direct",    "gather",    "scatter"};
+                                     "pw_eval",      "direct",    "gather",    "scatter",
+                                     "sort"};
 struct KLaunch {
     int id, level;
     double ms;
@@ -438,7 +439,7 @@ struct Timer {
     int open[S_N];
     LaunchClock clock;
     const int2 *src_se = nullptr, *trg_se = nullptr;
-    int nb = 0;
+    int nb = 0, nd = 1;
     Timer(Workspace& w, hipStream_t st) : W(w), s(st), on(g_timing), clock(w.kev)
     {
         t_klaunch.clear();
@@ -458,6 +459,14 @@ struct Timer {
     void ctx(int l, const std::vector<int>* b, int sd, long long u = 0, int b0 = 0, int b1 = -1)
     {
         if (on) clock.ctx(l, b, sd, u, b0, b1);
+    }
+    // events around a library call's launches (the radix sorts), in context
+    // (level, units)
+    void bracket(int id, bool end, int l, long long u)
+    {
+        if (!on) return;
+        if (!end) clock.ctx(l, nullptr, -1, u);
+        clock.mark(id, end, s);
     }
     void begin(int k, hipStream_t st = nullptr)
     {
@@ -502,6 +511,19 @@ struct Timer {
                 const auto& se = r.side == 0 ? hs : ht;
                 u = 0;
                 for (int i = r.r0; i < b1; ++i) u += se[(*r.boxes)[i]].y - se[(*r.boxes)[i]].x;
+            }
+            if ((r.side == 3 || r.side == 4) && nb > 0) {
+                // near field: source-target pairs of the unit lists, times nd
+                // (side 3: the targets, 4: the sources as targets)
+                const auto& te = r.side == 3 ? ht : hs;
+                u = 0;
+                for (size_t k = 0; k < W.h_units.size(); ++k) {
+                    long long ns_ = 0;
+                    for (int li = W.h_uptr[k]; li < W.h_uptr[k + 1]; ++li)
+                        ns_ += hs[W.h_ulist[li]].y - hs[W.h_ulist[li]].x;
+                    u += ns_ * (te[W.h_units[k]].y - te[W.h_units[k]].x);
+                }
+                u *= nd;
             }
             t_klaunch.push_back(KLaunch{r.id, r.level, ms, u});
         }
@@ -885,10 +907,12 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, (unsigned*)ks, ix, s);
     else
         launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, ks, ix, s);
+    tm.bracket(KID_SORT, false, -1, ns);
     if (k32)
         sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix, isrc, ns, keybits, s, tuning().sort);
     else
         sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
+    tm.bracket(KID_SORT, true, -1, ns);
     tm.end(S_KEYS);
     tm.begin(S_LEAF);
     int* lvl = W.get<int>("leaf_level", ns);
@@ -953,10 +977,12 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, (unsigned*)kt, ix, s);
         else
             launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, kt, ix, s);
+        tm.bracket(KID_SORT, false, -2, nt);
         if (k32)
             sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s, tuning().sort);
         else
             sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
+        tm.bracket(KID_SORT, true, -2, nt);
         tm.ctx(-2, nullptr, -1, nt);  // level -2 tags the gather without densities
         launch_gather(q.targ, nullptr, 0, itarg, nt, xt, nullptr, s);
     }
@@ -1165,7 +1191,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         launch_leafstart_search(ks2, ns, tg ? kt2 : nullptr, nt, (const u64*)d_lkey, nleaves, ls,
                                 ls2, s);
     launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, ls2, tg ? trg_se : nullptr, s);
-    tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
+    tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb, tm.nd = nd;
     if (!tg) launch_zero((double*)trg_se, (size_t)nb, s);  // int2 pairs, 8 bytes each
     tm.end(S_RANGES);
 
@@ -1227,7 +1253,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             d2.xt = xt;
             d2.ifpgh = q.ifpghtarg;
             d2.pot = pt; d2.grad = gt; d2.hess = ht;
-            tm.ctx(npwlevel, nullptr, -1, nt);
+            tm.ctx(npwlevel, nullptr, 3);  // units: the pairs, counted after the call
             launch_direct(d2, s_nf);
         }
         if (use_src) {
@@ -1236,7 +1262,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             d2.xt = xs;
             d2.ifpgh = q.ifpgh;
             d2.pot = ps; d2.grad = gs; d2.hess = hs;
-            tm.ctx(npwlevel, nullptr, -1, ns);
+            tm.ctx(npwlevel, nullptr, 4);
             launch_direct(d2, s_nf);
         }
     }
@@ -1707,10 +1733,12 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         launch_keys3d(q.src, ns, cen0, bs0, nlmax, (unsigned*)ks, ix, s);
     else
         launch_keys3d(q.src, ns, cen0, bs0, nlmax, ks, ix, s);
+    tm.bracket(KID_SORT, false, -1, ns);
     if (k32)
         sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix, isrc, ns, keybits, s, tuning().sort);
     else
         sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
+    tm.bracket(KID_SORT, true, -1, ns);
     tm.end(S_KEYS);
     tm.begin(S_LEAF);
     int* lvl = W.get<int>("leaf_level", ns);
@@ -1771,10 +1799,12 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
             launch_keys3d(q.targ, nt, cen0, bs0, nlmax, (unsigned*)kt, ix, s);
         else
             launch_keys3d(q.targ, nt, cen0, bs0, nlmax, kt, ix, s);
+        tm.bracket(KID_SORT, false, -2, nt);
         if (k32)
             sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s, tuning().sort);
         else
             sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
+        tm.bracket(KID_SORT, true, -2, nt);
         tm.ctx(-1, nullptr, -1, nt);
         launch_gather3(q.targ, nullptr, 0, itarg, nt, xt, nullptr, s);
     }
@@ -2123,7 +2153,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         launch_leafstart_search(ks2, ns, tg ? kt2 : nullptr, nt, (const u64*)d_lkey, nleaves, ls,
                                 ls2, s);
     launch_box_ranges(d_rlo, d_rhi, nb, ls, src_se, ls2, tg ? trg_se : nullptr, s);
-    tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb;
+    tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb, tm.nd = nd;
     if (!tg) launch_zero((double*)trg_se, (size_t)nb, s);  // int2 pairs, 8 bytes each
     tm.end(S_RANGES);
 
@@ -2352,7 +2382,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
             d2.xt = xt;
             d2.ifpgh = q.ifpghtarg;
             d2.pot = pt; d2.grad = gt; d2.hess = ht;
-            tm.ctx(npwlevel, nullptr, -1, nt);
+            tm.ctx(npwlevel, nullptr, 3);  // units: the pairs, counted after the call
             launch_direct3(d2, s);
         }
         if (use_src) {
@@ -2361,7 +2391,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
             d2.xt = xs;
             d2.ifpgh = q.ifpgh;
             d2.pot = ps; d2.grad = gs; d2.hess = hs;
-            tm.ctx(npwlevel, nullptr, -1, ns);
+            tm.ctx(npwlevel, nullptr, 4);
             launch_direct3(d2, s);
         }
     }

@@ -30,7 +30,7 @@ void check_hip(hipError_t e, const char* what);
 // ---------------------------------------------------------------------------
 enum KernelId {
     KID_FORM_MOMENTS, KID_FORM_H2PW, KID_FORM_VALU, KID_MERGE, KID_M2L, KID_SPLIT, KID_PW2LOCAL,
-    KID_TAYLOR_EVAL, KID_PW_EVAL, KID_DIRECT, KID_GATHER, KID_SCATTER, KID_N
+    KID_TAYLOR_EVAL, KID_PW_EVAL, KID_DIRECT, KID_GATHER, KID_SCATTER, KID_SORT, KID_N
 };
 struct KernelClock {
     // events recorded on the stream before (end = false) and after the

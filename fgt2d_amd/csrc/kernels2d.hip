@@ -2493,125 +2493,200 @@ void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
         klaunch(KID_TAYLOR_EVAL, k_taylor_eval<1>, grid, 256, lds, s, a);
 }
 
-// Step 5, plane-wave route (g2dpwevalp/g/h_vec, g2drouts.f:1887-2200):
-// pot = 2 Re sum_b w_y(b) sum_{a<half} PW(a,b) w_x(a), w(a) = e^{i ts_a u}
-// from one sincos per axis and its odd powers.  One block per leaf.
-constexpr int PW_MAXH = 32;
+// Step 5, plane-wave route (g2dpwevalp/g/h_vec, g2drouts.f:1887-2200), on
+// FP64 matrix cores: pot = 2 Re sum_b e^{i ts_b y} sum_{a<half} PW(a, b)
+// e^{i ts_a x}.  A leaf's expansion PW(a, b) (x nodes
+// ts_a, a < half; y nodes ts_b, b < npw, ts_{b+half} = -ts_b) is staged in
+// LDS as real and imaginary planes, zero padded to KP x NP.  For 16 points
+// of a wavefront, U(p, b) = sum_a e^{i ts_a x_p} PW(a, b) takes four real
+// 16x16x4 products per tile (lane l supplies the phase of point l%16 and
+// node l/16 + 4 ks); gradients add the products with ts_a e^{...}, Hessians
+// with ts_a^2 e^{...}.  Each 16-column tile of U goes through LDS to four
+// lanes per point, which add Re[U(p, b) E_b] with E_b = e^{i ts_b y_p} and
+// the y derivatives (factors i ts_b).
+constexpr int PWM_KS = 13;  // half <= 52
 
 template <int PGH, class E>
-__global__ __launch_bounds__(256) void k_pw_eval(EvalArgs a)
+__global__ __launch_bounds__(256) void k_pw_eval_mfma(EvalArgs a)
 {
     extern __shared__ double sm[];
-    const int npw = a.npw, half = npw / 2, nexp = half * npw, t = threadIdx.x;
+    const int npw = a.npw, half = npw / 2, nexp = half * npw;
+    const int KS = (half + 3) / 4, NT = (npw + 15) / 16, KP = 4 * KS, NP = 16 * NT;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    double* Pr = sm;
+    double* Pi = sm + KP * NP;
+    double* Us = sm + 2 * KP * NP + w * (PGH * 512);  // [product][re, im][16 points][16 columns]
+    const double* ts = a.ts;
+    const double dsq = a.dsq;
+    // the x nodes of this lane's A-operand rows, a0 + 4 ks (0 past half:
+    // those rows of PW are zero)
+    const int a0 = lane >> 4;
+    double fa[PWM_KS];
+#pragma unroll
+    for (int ks = 0; ks < PWM_KS; ++ks) fa[ks] = a0 + 4 * ks < half ? ts[a0 + 4 * ks] : 0.0;
+    const int pq = lane >> 2, cq = lane & 3;  // y sums: point pq, columns 4 cq .. 4 cq + 3
     const int total = seg_count(a.tstart, a.nbox);
     for (int g = blockIdx.x; g < total; g += gridDim.x) {
-    const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g);
-    const int b = a.boxes[sr.li];
-    const E* pw = (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
-    __syncthreads();
-    for (int o = t; o < 2 * nexp; o += 256) sm[o] = pw[o];
-    __syncthreads();
-    const double cx = a.cen[2 * b], cy = a.cen[2 * b + 1], dsq = a.dsq;
-    const double* ts = a.ts;
-    for (int i = sr.p0 + t; i < sr.p1; i += 256) {
-        double x = (a.pts[2 * (long)i] - cx) * dsq, y = (a.pts[2 * (long)i + 1] - cy) * dsq;
-        double wr[PW_MAXH], wi[PW_MAXH];
-        {
-            double s, c;
-            sincos(ts[0] * x, &s, &c);
-            double sr = c * c - s * s, si = 2 * c * s, qr = c, qi = s;
-#pragma unroll
-            for (int aa = 0; aa < PW_MAXH; ++aa) {
-                wr[aa] = qr;
-                wi[aa] = qi;
-                double tmp = qr * sr - qi * si;
-                qi = qr * si + qi * sr;
-                qr = tmp;
-            }
+        const SegRange sr = seg_at(a.tstart, a.nbox, a.boxes, a.se, a.seg, g, a.segbox);
+        const int b = a.boxes[sr.li];
+        const E* pw = (const E*)a.loc + 2 * ((long)a.slot[b] * a.nd + a.ind) * nexp;
+        __syncthreads();
+        for (int o = t; o < 2 * KP * NP; o += 256) sm[o] = 0.0;
+        __syncthreads();
+        for (int e = t; e < nexp; e += 256) {
+            const int aa = e % half, bb = e / half;
+            Pr[aa * NP + bb] = (double)pw[2 * e];
+            Pi[aa * NP + bb] = (double)pw[2 * e + 1];
         }
-        double s, c;
-        sincos(ts[0] * y, &s, &c);
-        double sr = c * c - s * s, si = 2 * c * s, qr = c, qi = s;
-        double zr = 0, gx = 0, gy = 0, hxx = 0, hxy = 0, hyy = 0;
-        for (int bb = 0; bb < half; ++bb) {
-            // c_b and c_{b+half}: sums over the x nodes
-            double c0r = 0, c0i = 0, c1r = 0, c1i = 0;
-            double d0r = 0, d0i = 0, d1r = 0, d1i = 0;   // with factor ts_a
-            double e0r = 0, e0i = 0, e1r = 0, e1i = 0;   // with factor ts_a^2
+        __syncthreads();
+        const double cx = a.cen[2 * b], cy = a.cen[2 * b + 1];
+        for (int base = sr.p0 + 16 * w; base < sr.p1; base += 64) {
+            // x phases of point base + lane%16 (clamped into the segment;
+            // rows past its end are computed and not written)
+            double wr[PWM_KS], wi[PWM_KS];
+            {
+                const int pa = min(base + (lane & 15), sr.p1 - 1);
+                const double x = (a.pts[2 * (long)pa] - cx) * dsq;
+                double s, c, s4, c4;
+                sincos(fa[0] * x, &s, &c);
+                sincos(8.0 * ts[0] * x, &s4, &c4);  // e^{i 4h x}, 4h = 8 ts_0
 #pragma unroll
-            for (int aa = 0; aa < PW_MAXH; ++aa) {
-                if (aa < half) {
-                    const double* p0 = sm + 2 * (aa + half * bb);
-                    const double* p1 = sm + 2 * (aa + half * (bb + half));
-                    double v0r = p0[0] * wr[aa] - p0[1] * wi[aa], v0i = p0[0] * wi[aa] + p0[1] * wr[aa];
-                    double v1r = p1[0] * wr[aa] - p1[1] * wi[aa], v1i = p1[0] * wi[aa] + p1[1] * wr[aa];
-                    c0r += v0r; c0i += v0i; c1r += v1r; c1i += v1i;
-                    if (PGH >= 2) {
-                        double f = ts[aa];
-                        d0r += f * v0r; d0i += f * v0i; d1r += f * v1r; d1i += f * v1i;
-                        if (PGH >= 3) {
-                            double f2 = f * f;
-                            e0r += f2 * v0r; e0i += f2 * v0i; e1r += f2 * v1r; e1i += f2 * v1i;
+                for (int ks = 0; ks < PWM_KS; ++ks) {
+                    wr[ks] = c;
+                    wi[ks] = s;
+                    const double tmp = c * c4 - s * s4;
+                    s = c * s4 + s * c4;
+                    c = tmp;
+                }
+            }
+            // y phase data of point pq
+            const int py = min(base + pq, sr.p1 - 1);
+            const double y = (a.pts[2 * (long)py + 1] - cy) * dsq;
+            double sh, ch;
+            sincos(2.0 * ts[0] * y, &sh, &ch);  // e^{i h y}
+            double zr = 0, gx = 0, gy = 0, hxx = 0, hxy = 0, hyy = 0;
+            for (int nt = 0; nt < NT; ++nt) {
+                d4_t u[PGH][2];
+#pragma unroll
+                for (int q = 0; q < PGH; ++q) u[q][0] = u[q][1] = d4_t{0, 0, 0, 0};
+                const double* brow = Pr + a0 * NP + (lane & 15) + 16 * nt;
+                const double* bimg = Pi + a0 * NP + (lane & 15) + 16 * nt;
+#pragma unroll
+                for (int ks = 0; ks < PWM_KS; ++ks) {
+                    if (ks < KS) {
+                        const double br = brow[4 * ks * NP], bi = bimg[4 * ks * NP];
+                        double xr = wr[ks], xi = wi[ks];
+#pragma unroll
+                        for (int q = 0; q < PGH; ++q) {
+                            u[q][0] = mfma16x16x4(xr, br, u[q][0]);
+                            u[q][0] = mfma16x16x4(-xi, bi, u[q][0]);
+                            u[q][1] = mfma16x16x4(xr, bi, u[q][1]);
+                            u[q][1] = mfma16x16x4(xi, br, u[q][1]);
+                            xr *= fa[ks];
+                            xi *= fa[ks];
                         }
                     }
                 }
+                // the tile to LDS: lane holds rows lane/16 + 4 r of column lane%16
+#pragma unroll
+                for (int q = 0; q < PGH; ++q)
+#pragma unroll
+                    for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            Us[((q * 2 + ri) * 16 + (lane >> 4) + 4 * r) * 16 + (lane & 15)] = u[q][ri][r];
+                wave_lds_sync();
+                double er = 0, ei = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int col = 4 * cq + j, bc = 16 * nt + col;
+                    if (bc >= npw) break;
+                    const double tb = ts[bc];
+                    if (j == 0 || bc == half) {
+                        sincos(tb * y, &ei, &er);
+                    } else {
+                        // next node: +h below half, -h above
+                        const double si = bc < half ? sh : -sh;
+                        const double tmp = er * ch - ei * si;
+                        ei = er * si + ei * ch;
+                        er = tmp;
+                    }
+                    const int o = pq * 16 + col;
+                    const double ur = Us[o], ui = Us[256 + o];
+                    const double re = ur * er - ui * ei, im = ur * ei + ui * er;
+                    zr += re;
+                    if (PGH >= 2) {
+                        const double vr = Us[512 + o], vi = Us[768 + o];
+                        const double rex = vr * er - vi * ei, imx = vr * ei + vi * er;
+                        gx -= imx;
+                        gy -= tb * im;
+                        if (PGH >= 3) {
+                            const double xr2 = Us[1024 + o], xi2 = Us[1280 + o];
+                            hxx -= xr2 * er - xi2 * ei;
+                            hxy -= tb * rex;
+                            hyy -= tb * tb * re;
+                        }
+                    }
+                }
+                wave_lds_sync();
             }
-            // y phases: w(bb) = q, w(bb+half) = conj(q); node ts_bb and -ts_bb
-            double fy = ts[bb];
-            // Re[c0 q] + Re[c1 conj(q)]
-            zr += (c0r * qr - c0i * qi) + (c1r * qr + c1i * qi);
-            if (PGH >= 2) {
-                // d/dx: i ts_a -> Re[i d q] = -Im[d q]
-                gx += -(d0r * qi + d0i * qr) - (-d1r * qi + d1i * qr);
-                // d/dy: i ts_b: Re[i fy c0 q] + Re[-i fy c1 conj(q)]
-                gy += -fy * (c0r * qi + c0i * qr) + fy * (-c1r * qi + c1i * qr);
+            // the point's four lanes
+#pragma unroll
+            for (int m = 1; m <= 2; m <<= 1) {
+                zr += __shfl_xor(zr, m);
+                if (PGH >= 2) {
+                    gx += __shfl_xor(gx, m);
+                    gy += __shfl_xor(gy, m);
+                }
                 if (PGH >= 3) {
-                    hxx += -((e0r * qr - e0i * qi) + (e1r * qr + e1i * qi));
-                    // (i ts_a)(i ts_b) = -ts_a ts_b
-                    hxy += -fy * (d0r * qr - d0i * qi) + fy * (d1r * qr + d1i * qi);
-                    hyy += -fy * fy * ((c0r * qr - c0i * qi) + (c1r * qr + c1i * qi));
+                    hxx += __shfl_xor(hxx, m);
+                    hxy += __shfl_xor(hxy, m);
+                    hyy += __shfl_xor(hyy, m);
                 }
             }
-            double tmp = qr * sr - qi * si;
-            qi = qr * si + qi * sr;
-            qr = tmp;
-        }
-        long o = (long)i * a.nd + a.ind;
-        a.pot[o] += 2 * zr;
-        if (PGH >= 2) {
-            long g = ((long)i * 2) * a.nd + a.ind;
-            a.grad[g] += 2 * gx * dsq;
-            a.grad[g + a.nd] += 2 * gy * dsq;
-        }
-        if (PGH >= 3) {
-            long h = ((long)i * 3) * a.nd + a.ind;
-            a.hess[h] += 2 * hxx * dsq * dsq;
-            a.hess[h + a.nd] += 2 * hxy * dsq * dsq;
-            a.hess[h + 2 * a.nd] += 2 * hyy * dsq * dsq;
+            const int i = base + pq;
+            if (cq == 0 && i < sr.p1) {
+                const long o = (long)i * a.nd + a.ind;
+                a.pot[o] += 2 * zr;
+                if (PGH >= 2) {
+                    const long gi = ((long)i * 2) * a.nd + a.ind;
+                    a.grad[gi] += 2 * gx * dsq;
+                    a.grad[gi + a.nd] += 2 * gy * dsq;
+                }
+                if (PGH >= 3) {
+                    const long h = ((long)i * 3) * a.nd + a.ind;
+                    a.hess[h] += 2 * hxx * dsq * dsq;
+                    a.hess[h + a.nd] += 2 * hxy * dsq * dsq;
+                    a.hess[h + 2 * a.nd] += 2 * hyy * dsq * dsq;
+                }
+            }
         }
     }
-    }
+}
+
+template <int PGH, class E>
+static void pw_eval_mfma_launch(const EvalArgs& a, hipStream_t s)
+{
+    const int half = a.npw / 2, KP = 4 * ((half + 3) / 4), NP = 16 * ((a.npw + 15) / 16);
+    const size_t lds = (2 * (size_t)KP * NP + 4 * PGH * 512) * sizeof(double);
+    allow_lds(k_pw_eval_mfma<PGH, E>, lds);
+    const int grid = a.tstart ? a.grid : a.nbox;
+    klaunch(KID_PW_EVAL, k_pw_eval_mfma<PGH, E>, grid, 256, lds, s, a);
 }
 
 void launch_pw_eval(const EvalArgs& a, hipStream_t s)
 {
     if (a.nbox <= 0) return;
-    if (a.npw / 2 > PW_MAXH) fail(-3, "plane-wave length too large for the evaluation kernel");
-    size_t lds = (size_t)a.npw * a.npw * sizeof(double);
-    const int grid = a.tstart ? a.grid : a.nbox;
+    // the reference's evaluation holds at most 100 nodes (g2dpwevalp_vec,
+    // g2drouts.f:1921-1922)
+    if (a.npw > 100) fail(-3, "plane-wave length above 100");
     if (a.exp32) {
-        if (a.ifpgh >= 3)
-            klaunch(KID_PW_EVAL, k_pw_eval<3, float>, grid, 256, lds, s, a);
-        else if (a.ifpgh == 2)
-            klaunch(KID_PW_EVAL, k_pw_eval<2, float>, grid, 256, lds, s, a);
-        else
-            klaunch(KID_PW_EVAL, k_pw_eval<1, float>, grid, 256, lds, s, a);
-    } else if (a.ifpgh >= 3)
-        klaunch(KID_PW_EVAL, k_pw_eval<3, double>, grid, 256, lds, s, a);
-    else if (a.ifpgh == 2)
-        klaunch(KID_PW_EVAL, k_pw_eval<2, double>, grid, 256, lds, s, a);
-    else
-        klaunch(KID_PW_EVAL, k_pw_eval<1, double>, grid, 256, lds, s, a);
+        if (a.ifpgh >= 3) pw_eval_mfma_launch<3, float>(a, s);
+        else if (a.ifpgh == 2) pw_eval_mfma_launch<2, float>(a, s);
+        else pw_eval_mfma_launch<1, float>(a, s);
+    } else if (a.ifpgh >= 3) pw_eval_mfma_launch<3, double>(a, s);
+    else if (a.ifpgh == 2) pw_eval_mfma_launch<2, double>(a, s);
+    else pw_eval_mfma_launch<1, double>(a, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -20,6 +20,9 @@ CASES = [
     ("uniform", 5000, 2000, 1e-2, 1e-3, 1, 2, 3),
     ("uniform", 1000, 300, 5.0, 1e-6, 1, 1, 1),       # large delta: one Taylor expansion
     ("mixture", 50000, 20000, 3e-6, 1e-3, 3, 1, 1),   # deep tree, nd = 3
+    # eps 1e-14 with Hessians: plane waves to eps/100, npw = 66 (above 64),
+    # PW evaluation at the cutoff level (Taylor order 91)
+    ("uniform", 20000, 5000, 1e-4, 1e-14, 1, 3, 3),
 ]
 
 
