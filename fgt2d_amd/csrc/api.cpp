@@ -307,7 +307,7 @@ struct Workspace {
         for (auto& e : order_ev) (void)hipEventDestroy(e);
         order_ev.clear();
         for (auto& a : aux)
-            if (a) (void)hipStreamDestroy(a);
+            if (a && a != own) (void)hipStreamDestroy(a);
         aux[0] = aux[1] = nullptr;
         if (own) (void)hipStreamDestroy(own);
         own = nullptr;
@@ -347,11 +347,20 @@ struct Workspace {
     }
 };
 
-Workspace* new_workspace()
+// side_streams = false: the workspace runs everything on its own stream
+// (the server's slots: the device has few hardware queues, and a slot whose
+// streams spread over several of them would put its long kernels ahead of
+// the other slots' work)
+Workspace* new_workspace(bool side_streams = true)
 {
     Workspace* w = new Workspace();
     HIPCHECK(hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking));
-    for (auto& a : w->aux) HIPCHECK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+    for (auto& a : w->aux) {
+        if (side_streams)
+            HIPCHECK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+        else
+            a = w->own;
+    }
     return w;
 }
 
@@ -3238,7 +3247,7 @@ int fgtpw_serve_start(int slots, double hbm_budget_gib)
         S->budget = hbm_budget_gib > 0 ? (size_t)(hbm_budget_gib * 1073741824.0)
                                        : (size_t)(0.6 * (double)free_b);
         try {
-            for (int i = 0; i < slots; ++i) S->slots.push_back(new_workspace());
+            for (int i = 0; i < slots; ++i) S->slots.push_back(new_workspace(false));
         } catch (...) {
             for (auto* w : S->slots) { w->release(); delete w; }
             delete S;
