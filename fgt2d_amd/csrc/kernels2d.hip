@@ -956,6 +956,50 @@ __global__ __launch_bounds__(256) void k_form_wave(FormArgs a)
                 }
             }
         };
+        if (!DIP && a.seg <= 8 * CH) {
+            // the whole segment's coordinates and charges requested at once
+            // (one memory latency per segment instead of one per chunk)
+            double V[8], F[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = u + CH * j;
+                V[j] = F[j] = 0;
+                if (k < n) {
+                    const long i = r.p0 + k;
+                    V[j] = a.xs[2 * i + side];
+                    F[j] = side ? a.qs[(long)a.nd * i + a.ind] : 1.0;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int c0 = CH * j;
+                if (c0 >= n) break;
+                wave_lds_sync();
+                const double vv = (V[j] - c) * dsq;
+                double pw = F[j];
+#pragma unroll
+                for (int e = 0; e < MP; ++e) {
+                    mine[e] = pw;
+                    pw *= vv;
+                }
+                wave_lds_sync();
+                const int nc = min(CH, n - c0);
+                for (int gg = 0; 4 * gg < nc; ++gg) {
+                    const int src = 4 * gg + s_l;
+                    double av[TM], bv[TM];
+#pragma unroll
+                    for (int t = 0; t < TM; ++t) {
+                        av[t] = Px[src * RS + e_l + 16 * t];
+                        bv[t] = Py[src * RS + e_l + 16 * t];
+                    }
+#pragma unroll
+                    for (int ii = 0; ii < TM; ++ii)
+#pragma unroll
+                        for (int jj = 0; jj < TM; ++jj)
+                            acc[ii][jj] = mfma16x16x4(av[ii], bv[jj], acc[ii][jj]);
+                }
+            }
+        } else {
         load(u, v, f, g1, f2, g2);
         for (int c0 = 0; c0 < n; c0 += CH) {
             double vn, fn, gn, f2n, g2n;  // next chunk, loaded ahead
@@ -1010,6 +1054,7 @@ __global__ __launch_bounds__(256) void k_form_wave(FormArgs a)
             g1 = gn;
             f2 = f2n;
             g2 = g2n;
+        }
         }
         double* out = a.partial + (long)g * a.pstride;
 #pragma unroll
@@ -1083,6 +1128,49 @@ __global__ __launch_bounds__(256) void k_form_wave_f32(FormArgs a)
                 }
             }
         };
+        if (!DIP && a.seg <= 8 * CH) {
+            // the whole segment's coordinates and charges requested at once
+            double V[8], F[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = u + CH * j;
+                V[j] = F[j] = 0;
+                if (k < n) {
+                    const long i = r.p0 + k;
+                    V[j] = a.xs[2 * i + side];
+                    F[j] = side ? a.qs[(long)a.nd * i + a.ind] : 1.0;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int c0 = CH * j;
+                if (c0 >= n) break;
+                wave_lds_sync();
+                const double vv = (V[j] - c) * dsq;
+                double pw = F[j];
+#pragma unroll
+                for (int e = 0; e < MP; ++e) {
+                    mine[e] = (float)pw;
+                    pw *= vv;
+                }
+                wave_lds_sync();
+                const int nc = min(CH, n - c0);
+                for (int gg = 0; 4 * gg < nc; ++gg) {
+                    const int src = 4 * gg + s_l;
+                    float av[TM], bv[TM];
+#pragma unroll
+                    for (int t = 0; t < TM; ++t) {
+                        av[t] = Px[src * RS + e_l + 16 * t];
+                        bv[t] = Py[src * RS + e_l + 16 * t];
+                    }
+#pragma unroll
+                    for (int ii = 0; ii < TM; ++ii)
+#pragma unroll
+                        for (int jj = 0; jj < TM; ++jj)
+                            acc[ii][jj] = mfma16x16x4f32(av[ii], bv[jj], acc[ii][jj]);
+                }
+            }
+        } else {
         load(u, v, f, g1, f2, g2);
         for (int c0 = 0; c0 < n; c0 += CH) {
             double vn, fn, gn, f2n, g2n;
@@ -1138,6 +1226,7 @@ __global__ __launch_bounds__(256) void k_form_wave_f32(FormArgs a)
             g1 = gn;
             f2 = f2n;
             g2 = g2n;
+        }
         }
         double* out = a.partial + (long)g * a.pstride;
 #pragma unroll
