@@ -110,10 +110,10 @@ def pw_equivalent(kernel, npw, nd):
     return None
 
 
-def rocprof_name(kernel, nl, ifpgh, mixed=False, npw=0):
+def rocprof_name(kernel, nl, ifpgh, mixed=False, npw=0, sort=4):
     """Symbol of the HIP kernel an instrumented launch runs (as rocprofv3
     lists it, template arguments included), for the Taylor order nl of its
-    level."""
+    level; `sort` is the 32-bit key sort in use (tuning key sort)."""
     tm = min(4, (nl + 1 + 15) // 16)
     f32 = "_f32" if mixed else ""
     if kernel == "form_moments":
@@ -139,6 +139,8 @@ def rocprof_name(kernel, nl, ifpgh, mixed=False, npw=0):
     if kernel == "direct":
         return "k_direct<%d, false>" % ifpgh
     if kernel == "sort":
+        if sort == 4:
+            return "k_rs_count + k_rs_offsets + k_rs_scatter (LSD radix passes)"
         return "rocprim radix sort (onesweep passes, histogram, fills)"
     return "k_" + kernel
 
@@ -186,7 +188,9 @@ def kernel_profile(step, torch, reps=5):
 def rooflines(agg, stats, nd, count=5, mixed=False):
     """Roofline records of the instrumented kernels with an algorithmic
     per-unit figure, largest total time first."""
+    import fgt2d_amd
     npw, nlocal = stats["npw"], stats["nlocal"]
+    sort = fgt2d_amd.tuning_table().get("sort", 4)
     ranked = sorted(agg.items(), key=lambda kv: -kv[1][0])
     total = sum(v[0] for v in agg.values())
     out = []
@@ -202,7 +206,7 @@ def rooflines(agg, stats, nd, count=5, mixed=False):
         scale = 1e12 if bound.startswith("mfma") else 1e9
         achieved = work / launch_s / scale
         f32 = mixed and (bound == "mfma32" or kernel in ("merge", "m2l", "split"))
-        name = rocprof_name(kernel, nl, 1, f32, npw)
+        name = rocprof_name(kernel, nl, 1, f32, npw, sort)
         rec = {"bound": "mfma" if bound.startswith("mfma") else bound,
                "achieved": round(achieved, 3), "peak": peak, "unit": unit,
                "frac": round(achieved / peak, 4),
@@ -470,6 +474,7 @@ def serve_stream(fgt2d_amd, nreq, rate=200.0, slots=4, seed=SEED, check=24):
     th.start()
     lat = np.zeros(nreq)
     done = np.zeros(nreq)
+    qr = np.zeros((nreq, 2))
     errs = {}
     try:
         for _ in range(nreq):
@@ -477,6 +482,7 @@ def serve_stream(fgt2d_amd, nreq, rate=200.0, slots=4, seed=SEED, check=24):
             outs, ms = srv.wait(t)
             lat[i] = submitted[i] - arrive[i] + ms[2] / 1e3
             done[i] = submitted[i] + ms[2] / 1e3
+            qr[i] = ms[0], ms[1]
             if i in checked:
                 src, chg = req(i)
                 sel = np.random.default_rng(i).permutation(len(src))[:1000]
@@ -502,7 +508,13 @@ def serve_stream(fgt2d_amd, nreq, rate=200.0, slots=4, seed=SEED, check=24):
             "offered_Mpts_s": round(pts / arrive[-1] / 1e6, 3), "span_s": round(float(span), 3),
             "peak_running": stats.get("peak_running"),
             "max_rel_l2_err_by_eps": {k: float("%.3g" % v) for k, v in sorted(errs.items())},
-            "checked_requests": len(checked)}
+            "checked_requests": len(checked),
+            # the slowest requests: where their latency went
+            "worst": [{"arrive_s": round(float(arrive[i]), 3), "dim": int(dims[i]), "n": int(npts[i]),
+                       "eps": float(epss[i]),
+                       "submit_delay_ms": round(float(submitted[i] - arrive[i]) * 1e3, 2),
+                       "queue_ms": round(float(qr[i, 0]), 2), "run_ms": round(float(qr[i, 1]), 2)}
+                      for i in np.argsort(-lat_ms)[:5]]}
 
 
 def main():
@@ -642,7 +654,8 @@ def main():
     if "fp64" in legs and "mixed" in legs:
         rec["mixed_vs_fp64"] = round(legs["fp64"]["t"] / legs["mixed"]["t"], 4)
     # the headline record: the hand-written kernel with the largest total
-    # time; the library sort and the rest follow in roofline_next
+    # time (rocPRIM's sort, when chosen, is not one); the rest follow in
+    # roofline_next
     rl = rooflines(head["agg"], stats, 1, count=9, mixed=mixed)
     own = [r for r in rl if not r["kernel"].startswith("rocprim")]
     rec["roofline"] = own[0] if own else None

@@ -11,7 +11,9 @@ tile count TM of a tree level (orders up to 16 TM):
 * ``segeTM``  points per evaluation segment;
 * ``spwTM``   consecutive segments per wavefront of the fused PW -> Taylor
   evaluation;
-* ``sort``    block shape of the 32-bit key radix sort (0..3).
+* ``sort``    32-bit key radix sort: 4 and 5 the library's stable LSD sort
+  (tiles of 2048 or 4096 keys), 0..3 the block shape of rocPRIM's onesweep
+  sort.
 
 Search: coordinate descent from the current table, one key at a time; a
 candidate replaces the incumbent when its median step time is more than
@@ -34,7 +36,7 @@ TABLE = os.path.join(HERE, "tuning.txt")
 
 # candidate values per key, the first one of each the built-in default
 SPACE = [
-    ("sort", [0, 1, 2, 3]),
+    ("sort", [4, 5, 0, 1, 2, 3]),
     ("segf1", [256, 128, 512]),
     ("sege1", [256, 128]),
     ("spw1", [2, 1, 4]),

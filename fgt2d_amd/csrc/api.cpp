@@ -72,12 +72,13 @@ struct Tuning {
     int segf[4] = {256, 256, 128, 128};
     int sege[4] = {256, 256, 256, 256};
     int spw[4] = {2, 2, 2, 2};
-    int sort = 0;  // block shape of the 32-bit key sort (sort_pairs_u32)
+    int sort = 4;  // 32-bit key sort: 4, 5 the library's LSD sort (tiles of 2048, 4096 keys),
+                   // 0..3 a rocPRIM onesweep block shape
     std::string source = "defaults";
     bool set(const std::string& k, int v)
     {
         if (k == "sort") {
-            if (v < 0 || v > 3) return false;
+            if (v < 0 || v > 5) return false;
             sort = v;
             return true;
         }
@@ -144,10 +145,11 @@ int g_device = -1;
 
 const char* const kStages[] = {"bbox",  "keys_sort", "leafdepth", "host_tree", "assign",
                                "ranges", "form",     "merge",     "m2l",       "split",
-                               "eval",  "direct",    "scatter",   "host_build"};
-// host_build: the box tree itself, part of host_tree (host clock)
+                               "eval",  "direct",    "scatter",   "host_build", "gather"};
+// host_build: the box tree itself, part of host_tree (host clock); gather:
+// the sources' points and charges into Morton order (side stream)
 enum Stage { S_BBOX, S_KEYS, S_LEAF, S_HOST, S_ASSIGN, S_RANGES, S_FORM, S_MERGE, S_M2L, S_SPLIT,
-             S_EVAL, S_DIRECT, S_SCATTER, S_HOST_BUILD, S_N };
+             S_EVAL, S_DIRECT, S_SCATTER, S_HOST_BUILD, S_GATHER, S_N };
 }  // namespace
 
 thread_local KernelClock* t_kclock = nullptr;
@@ -908,7 +910,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     unsigned* ix = W.get<unsigned>("idx_s", nmx);
     unsigned* isrc = W.get<unsigned>("isrc", ns);
     size_t tmpb = std::max(sort_pairs_u64_temp(nmx), sort_pairs_u32_temp(nmx));
-    tmpb = std::max(tmpb, select_flagged_temp(ns));
+    tmpb = std::max(tmpb, leaf_records_temp(ns));
     void* tmp = W.get<char>("sort_tmp", tmpb);
     const int keybits = std::max(2, 2 * nlmax);
     const bool k32 = keybits <= 31;  // keys fit 32 bits
@@ -926,25 +928,24 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     tm.begin(S_LEAF);
     int* lvl = W.get<int>("leaf_level", ns);
     unsigned char* flg = W.get<unsigned char>("leaf_flag", ns);
-    int* starts = W.get<int>("leaf_starts", ns);
-    int* nsel = W.get<int>("nsel", 4);
+    // the leaf count (first word) and the leaf records (from byte 256) in
+    // one buffer: one read-back, sized by the last call's count
+    char* rec = W.get<char>("leaf_runs", 256 + sizeof(LeafRun) * (size_t)ns);
+    int* nsel = (int*)rec;
+    LeafRun* runs = (LeafRun*)(rec + 256);
+    unsigned* bcnt = (unsigned*)tmp;  // free after the sort
     if (k32)
-        launch_leafdepth((const unsigned*)ks2, ns, nlmax, npwlevel, 10, 80, lvl, flg, s);
+        launch_leafdepth((const unsigned*)ks2, ns, nlmax, npwlevel, 10, 80, lvl, flg, bcnt, s);
     else
-        launch_leafdepth(ks2, ns, nlmax, npwlevel, 10, 80, lvl, flg, s);
-    select_flagged(tmp, tmpb, flg, ns, starts, nsel, s);
-    // the leaf records (at most one per source) and their count come back
-    // in one round trip, sized by the last call's count
-    LeafRun* runs = W.get<LeafRun>("leaf_runs", ns);
+        launch_leafdepth(ks2, ns, nlmax, npwlevel, 10, 80, lvl, flg, bcnt, s);
     if (k32)
-        launch_emit_leaves(starts, nsel, ns, (const unsigned*)ks2, lvl, nlmax, runs, s);
+        launch_leaf_records(flg, bcnt, ns, (const unsigned*)ks2, lvl, nlmax, nsel, runs, s);
     else
-        launch_emit_leaves(starts, nsel, ns, ks2, lvl, nlmax, runs, s);
+        launch_leaf_records(flg, bcnt, ns, ks2, lvl, nlmax, nsel, runs, s);
     tm.end(S_LEAF);
     const size_t guess = std::min<size_t>(ns, std::max<size_t>(4096, W.leaf_guess + W.leaf_guess / 4));
     char* hl = (char*)W.pinned(256 + sizeof(LeafRun) * guess);
-    HIPCHECK(hipMemcpyAsync(hl, nsel, sizeof(int), hipMemcpyDeviceToHost, s));
-    copy_to_host(hl + 256, runs, sizeof(LeafRun) * guess, s);
+    copy_to_host(hl, rec, 256 + sizeof(LeafRun) * guess, s);
     const int nleaf0 = *(const int*)hl;
     W.leaf_guess = nleaf0;
     std::vector<LeafRun> leaves(nleaf0);
@@ -956,24 +957,34 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         std::memcpy(leaves.data() + guess, hp, rest);
     }
 
-    // Work that needs no tree, queued now so that the GPU runs it while the
-    // host builds the tree: targets to Morton order, points and charges
-    // gathered into that order.
-    tm.begin(S_ASSIGN);
+    // points and charges into Morton order on a side stream: only the
+    // expansion steps read them, so the gathers overlap the host's tree.
+    // Queued once the leaves are back: beside a gather, the leaf searches
+    // take twice as long, the one-block scan waits for a free CU and the
+    // read-back's copy kernel takes 15 times as long.
+    hipStream_t s_g = g_serial ? s : W.aux[0];
+    W.order(s, s_g);
+    tm.begin(S_GATHER, s_g);
     double* xs = W.get<double>("xs", 2 * (size_t)ns);
     double* qs = W.get<double>("qs", (size_t)nd * ns);
     tm.ctx(-1, nullptr, -1, ns);
     if (q.chg)
-        launch_gather(q.src, q.chg, nd, isrc, ns, xs, qs, s);
+        launch_gather(q.src, q.chg, nd, isrc, ns, xs, qs, s_g);
     else {
-        launch_gather(q.src, nullptr, nd, isrc, ns, xs, nullptr, s);
-        launch_zero(qs, (size_t)nd * ns, s);
+        launch_gather(q.src, nullptr, nd, isrc, ns, xs, nullptr, s_g);
+        launch_zero(qs, (size_t)nd * ns, s_g);
     }
     double* wsd = nullptr;  // dipole weights d n in sorted order
     if (q.rn) {
         wsd = W.get<double>("dip_w", 2 * (size_t)nd * ns);
-        launch_gather_dip(q.rn, q.ds, nd, isrc, ns, wsd, s);
+        launch_gather_dip(q.rn, q.ds, nd, isrc, ns, wsd, s_g);
     }
+    tm.end(S_GATHER, s_g);
+
+    // Work that needs no tree, queued now so that the GPU runs it while the
+    // host builds the tree: targets to Morton order, their points gathered
+    // into that order.
+    tm.begin(S_ASSIGN);
     unsigned* itarg = nullptr;
     u64* kt2 = nullptr;
     double* xt = nullptr;
@@ -1172,6 +1183,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         std::memcpy(hp, pk.data(), pk.size());
         HIPCHECK(hipMemcpyAsync(dplan, hp, pk.size(), hipMemcpyHostToDevice, s));
     }
+    W.order(s_g, s);  // the sorted points, charges and dipole weights
     auto D = [&](size_t off) { return (void*)(dplan + off); };
     const int4* d_child = (const int4*)D(o_child);
     const int* d_par = (const int*)D(o_par);
@@ -1734,7 +1746,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     u64* ks2 = W.get<u64>("keys_s2", ns);
     unsigned* ix = W.get<unsigned>("idx_s", nmx);
     unsigned* isrc = W.get<unsigned>("isrc", ns);
-    size_t tmpb = std::max({sort_pairs_u64_temp(nmx), sort_pairs_u32_temp(nmx), select_flagged_temp(ns)});
+    size_t tmpb = std::max({sort_pairs_u64_temp(nmx), sort_pairs_u32_temp(nmx), leaf_records_temp(ns)});
     void* tmp = W.get<char>("sort_tmp", tmpb);
     const int keybits = std::max(3, 3 * nlmax);
     const bool k32 = keybits <= 31;  // keys fit 32 bits
@@ -1752,25 +1764,24 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     tm.begin(S_LEAF);
     int* lvl = W.get<int>("leaf_level", ns);
     unsigned char* flg = W.get<unsigned char>("leaf_flag", ns);
-    int* starts = W.get<int>("leaf_starts", ns);
-    int* nsel = W.get<int>("nsel", 4);
+    // the leaf count (first word) and the leaf records (from byte 256) in
+    // one buffer: one read-back, sized by the last call's count
+    char* rec = W.get<char>("leaf_runs", 256 + sizeof(LeafRun) * (size_t)ns);
+    int* nsel = (int*)rec;
+    LeafRun* runs = (LeafRun*)(rec + 256);
+    unsigned* bcnt = (unsigned*)tmp;  // free after the sort
     if (k32)
-        launch_leafdepth((const unsigned*)ks2, ns, nlmax, npwlevel, 40, 100, lvl, flg, s, 3);
+        launch_leafdepth((const unsigned*)ks2, ns, nlmax, npwlevel, 40, 100, lvl, flg, bcnt, s, 3);
     else
-        launch_leafdepth(ks2, ns, nlmax, npwlevel, 40, 100, lvl, flg, s, 3);
-    select_flagged(tmp, tmpb, flg, ns, starts, nsel, s);
-    // the leaf records (at most one per source) and their count come back
-    // in one round trip, sized by the last call's count
-    LeafRun* runs = W.get<LeafRun>("leaf_runs", ns);
+        launch_leafdepth(ks2, ns, nlmax, npwlevel, 40, 100, lvl, flg, bcnt, s, 3);
     if (k32)
-        launch_emit_leaves(starts, nsel, ns, (const unsigned*)ks2, lvl, nlmax, runs, s, 3);
+        launch_leaf_records(flg, bcnt, ns, (const unsigned*)ks2, lvl, nlmax, nsel, runs, s, 3);
     else
-        launch_emit_leaves(starts, nsel, ns, ks2, lvl, nlmax, runs, s, 3);
+        launch_leaf_records(flg, bcnt, ns, ks2, lvl, nlmax, nsel, runs, s, 3);
     tm.end(S_LEAF);
     const size_t guess = std::min<size_t>(ns, std::max<size_t>(4096, W.leaf_guess + W.leaf_guess / 4));
     char* hl = (char*)W.pinned(256 + sizeof(LeafRun) * guess);
-    HIPCHECK(hipMemcpyAsync(hl, nsel, sizeof(int), hipMemcpyDeviceToHost, s));
-    copy_to_host(hl + 256, runs, sizeof(LeafRun) * guess, s);
+    copy_to_host(hl, rec, 256 + sizeof(LeafRun) * guess, s);
     const int nleaf0 = *(const int*)hl;
     W.leaf_guess = nleaf0;
     std::vector<LeafRun> leaves(nleaf0);
@@ -1781,21 +1792,28 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         copy_to_host(hp, runs + guess, rest, s);
         std::memcpy(leaves.data() + guess, hp, rest);
     }
-    tm.begin(S_ASSIGN);
+
+    // points and charges into Morton order on a side stream, queued once the
+    // leaves are back, overlapping the host's tree
+    hipStream_t s_g = g_serial ? s : W.aux[0];
+    W.order(s, s_g);
+    tm.begin(S_GATHER, s_g);
     double* xs = W.get<double>("xs", 3 * (size_t)ns);
     double* qs = W.get<double>("qs", (size_t)nd * ns);
     tm.ctx(-1, nullptr, -1, ns);
     if (q.chg)
-        launch_gather3(q.src, q.chg, nd, isrc, ns, xs, qs, s);
+        launch_gather3(q.src, q.chg, nd, isrc, ns, xs, qs, s_g);
     else {
-        launch_gather3(q.src, nullptr, nd, isrc, ns, xs, nullptr, s);
-        launch_zero(qs, (size_t)nd * ns, s);
+        launch_gather3(q.src, nullptr, nd, isrc, ns, xs, nullptr, s_g);
+        launch_zero(qs, (size_t)nd * ns, s_g);
     }
     double* wsd = nullptr;  // dipole weights d n in sorted order
     if (q.rn) {
         wsd = W.get<double>("dip_w", 3 * (size_t)nd * ns);
-        launch_gather_dip3(q.rn, q.ds, nd, isrc, ns, wsd, s);
+        launch_gather_dip3(q.rn, q.ds, nd, isrc, ns, wsd, s_g);
     }
+    tm.end(S_GATHER, s_g);
+    tm.begin(S_ASSIGN);
     unsigned* itarg = nullptr;
     u64* kt2 = nullptr;
     double* xt = nullptr;
@@ -2134,6 +2152,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         std::memcpy(hp, pk.data(), pk.size());
         HIPCHECK(hipMemcpyAsync(dplan, hp, pk.size(), hipMemcpyHostToDevice, s));
     }
+    W.order(s_g, s);  // the sorted points, charges and dipole weights
     auto D = [&](size_t off) { return (void*)(dplan + off); };
     const int* d_child = (const int*)D(o_child);
     const int* d_par = (const int*)D(o_par);
@@ -2568,10 +2587,10 @@ size_t noalloc_point_bytes(int nd, int ns, int nt, int ifdipole, int ifpgh, int 
                (ifp >= 3 ? r(24 * ND * n) : 0);
     };
     size_t tmpb = std::max(sort_pairs_u64_temp((int)nmx), sort_pairs_u32_temp((int)nmx));
-    tmpb = std::max(tmpb, select_flagged_temp(ns));
-    size_t b = 2 * r(8 * NS) + r(4 * nmx) + r(4 * NS) + r(tmpb) + r(4 * NS) + r(NS) + r(4 * NS) +
-               r(sizeof(LeafRun) * NS) + r(16 * NS) + r(8 * ND * NS) + (ifdipole ? r(16 * ND * NS) : 0) +
-               outs(ifpgh, NS);
+    tmpb = std::max(tmpb, leaf_records_temp(ns));
+    size_t b = 2 * r(8 * NS) + r(4 * nmx) + r(4 * NS) + r(tmpb) + r(4 * NS) + r(NS) +
+               r(256 + sizeof(LeafRun) * NS) + r(16 * NS) + r(8 * ND * NS) +
+               (ifdipole ? r(16 * ND * NS) : 0) + outs(ifpgh, NS);
     if (nt > 0) b += r(4 * NT) + r(16 * NT) + 2 * r(8 * NT) + outs(ifpghtarg, NT);
     return b;
 }
@@ -3008,7 +3027,7 @@ int fgtpw_tree2d_from_leaves(int nlmax, int npwlevel, double bs0, const double* 
 {
     return run_guarded([&] {
         std::vector<LeafRun> leaves(nleaves);
-        for (int i = 0; i < nleaves; ++i) leaves[i] = LeafRun{keys[i], levels[i], 0, 0};
+        for (int i = 0; i < nleaves; ++i) leaves[i] = LeafRun{keys[i], levels[i], 0};
         Tree2D T;
         build_tree2d(T, leaves, nlmax, npwlevel, bs0, cen0);
         complete_colleagues(T);

@@ -84,9 +84,8 @@ void planewave_nodes(double pmax, int npw, double* ws, double* ts);
 // ---------------------------------------------------------------------------
 struct LeafRun {            // one non-empty leaf of the adaptive tree before
     uint64_t key;           // level restriction: its Morton prefix (level
-    int level;              // digits, most significant first), its level,
-    int start;              // and its first position in the key-sorted
-    int count;              // source list, and its number of sources
+    int level;              // digits, most significant first), its level
+    int start;              // and its first position in the key-sorted source list
 };
 
 // Box tree in the reference's itree layout (D = 2: quadtree, D = 3: octree)
@@ -139,26 +138,32 @@ void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, in
                    unsigned long long* key, unsigned* idx, hipStream_t s);
 void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, int nlmax,
                    unsigned* key, unsigned* idx, hipStream_t s);
-// bits: key bits per level (2 in 2D, 3 in 3D)
+// bits: key bits per level (2 in 2D, 3 in 3D); bcnt: the number of leaf
+// starts of every block of 256 points (leaf_records_temp(n) bytes)
 void launch_leafdepth(const unsigned long long* key, int n, int nlmax, int levcut, int ndiv0,
-                      int ndiv, int* level, unsigned char* start_flag, hipStream_t s,
-                      int bits = 2);
+                      int ndiv, int* level, unsigned char* start_flag, unsigned* bcnt,
+                      hipStream_t s, int bits = 2);
 void launch_leafdepth(const unsigned* key, int n, int nlmax, int levcut, int ndiv0, int ndiv,
-                      int* level, unsigned char* start_flag, hipStream_t s, int bits = 2);
+                      int* level, unsigned char* start_flag, unsigned* bcnt, hipStream_t s,
+                      int bits = 2);
+size_t leaf_records_temp(int n);
+// one record per leaf start, in Morton order, and their number (*nleaves);
+// bcnt as launch_leafdepth left it (scanned in place)
+void launch_leaf_records(const unsigned char* start_flag, unsigned* bcnt, int n,
+                         const unsigned long long* key, const int* level, int nlmax, int* nleaves,
+                         LeafRun* out, hipStream_t s, int bits = 2);
+void launch_leaf_records(const unsigned char* start_flag, unsigned* bcnt, int n,
+                         const unsigned* key, const int* level, int nlmax, int* nleaves,
+                         LeafRun* out, hipStream_t s, int bits = 2);
 size_t sort_pairs_u64_temp(int n);
 void sort_pairs_u64(void* tmp, size_t tmpb, const unsigned long long* kin, unsigned long long* kout,
                     const unsigned* vin, unsigned* vout, int n, int endbit, hipStream_t s);
 size_t sort_pairs_u32_temp(int n);
-// shape: block shape of the onesweep kernel (tuning table key `sort`, 0..3)
+// shape (tuning table key `sort`): 0..3 the block shape of rocPRIM's onesweep
+// kernel, 4 and 5 the library's own stable LSD sort (tiles of 2048 keys, or
+// 4096 for digits of up to 10 bits)
 void sort_pairs_u32(void* tmp, size_t tmpb, const unsigned* kin, unsigned* kout, const unsigned* vin,
                     unsigned* vout, int n, int endbit, hipStream_t s, int shape = 0);
-size_t select_flagged_temp(int n);
-void select_flagged(void* tmp, size_t tmpb, const unsigned char* flags, int n, int* out, int* nsel,
-                    hipStream_t s);
-void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigned long long* key,
-                        const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits = 2);
-void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigned* key,
-                        const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits = 2);
 // ls[r] = first of the n sorted keys >= leafkey[r], r < nleaves; ls[nleaves] = n;
 // the same into ls2 for key2 (n2 keys) when key2 is not null
 void launch_leafstart_search(const unsigned long long* key, int n, const unsigned long long* key2,

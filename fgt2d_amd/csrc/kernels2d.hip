@@ -160,108 +160,208 @@ void launch_keys2d(const double* xy, int n, double cx, double cy, double bs0, in
 // level-l prefixes in the sorted keys is located by two binary searches in
 // the window [i - lim, i + lim] (prefixes are sorted, so "same prefix" is a
 // contiguous interval); a run that fills the window on either side holds more
-// than lim points.
+// than lim points.  A point whose leaf prefix differs from its predecessor's
+// starts a leaf; each block of 256 points also counts its starts.
 // ---------------------------------------------------------------------------
 template <class K>
 __global__ __launch_bounds__(256) void k_leafdepth(const K* __restrict__ key, int n, int nlmax,
                                                    int levcut, int ndiv0, int ndiv, int bits,
                                                    int* __restrict__ level,
-                                                   unsigned char* __restrict__ start)
+                                                   unsigned char* __restrict__ start,
+                                                   unsigned* __restrict__ bcnt)
 {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const K ki = key[i];
-    int L = 0;
-    for (int l = nlmax - 1; l >= 0; --l) {
-        const int sh = bits * (nlmax - l);  // < the key width (host guarantee)
-        const K p = ki >> sh;
-        const int lim = l < levcut ? ndiv0 : ndiv;
-        // first index in [max(0, i - lim), i] with prefix p
-        int a = max(0, i - lim), b = i;
-        while (a < b) {
-            int m = (a + b) >> 1;
-            if ((key[m] >> sh) == p) b = m; else a = m + 1;
-        }
-        const int lo = a;
-        // last index in [i, min(n - 1, i + lim)] with prefix p
-        a = i;
-        b = min(n - 1, i + lim);
-        while (a < b) {
-            int m = (a + b + 1) >> 1;
-            if ((key[m] >> sh) == p) a = m; else b = m - 1;
-        }
-        if (a - lo + 1 > lim) {
-            L = l + 1;
-            break;
-        }
+    // the block's keys and a halo of max(ndiv0, ndiv) on each side: every
+    // window the searches visit, in LDS
+    extern __shared__ __align__(8) unsigned char ld_s[];
+    K* sk = (K*)ld_s;
+    const int halo = max(ndiv0, ndiv);
+    const int b0 = blockIdx.x * 256, lo0 = b0 - halo;
+    for (int j = threadIdx.x; j < 256 + 2 * halo; j += 256) {
+        const int g = lo0 + j;
+        sk[j] = g >= 0 && g < n ? key[g] : K(0);
     }
-    level[i] = L;
-    const int sh = bits * (nlmax - L);
-    unsigned char st = 1;
-    if (i > 0) st = sh >= 8 * (int)sizeof(K) ? 0 : ((key[i - 1] >> sh) != (ki >> sh));
-    start[i] = st;
+    __syncthreads();
+    const int i = b0 + threadIdx.x;
+    int st = 0;
+    if (i < n) {
+        const K ki = sk[i - lo0];
+        int L = 0;
+        for (int l = nlmax - 1; l >= 0; --l) {
+            const int sh = bits * (nlmax - l);  // < the key width (host guarantee)
+            const K p = ki >> sh;
+            const int lim = l < levcut ? ndiv0 : ndiv;
+            // first index in [max(0, i - lim), i] with prefix p
+            int a = max(0, i - lim), b = i;
+            while (a < b) {
+                int m = (a + b) >> 1;
+                if ((sk[m - lo0] >> sh) == p) b = m; else a = m + 1;
+            }
+            const int lo = a;
+            // last index in [i, min(n - 1, i + lim)] with prefix p
+            a = i;
+            b = min(n - 1, i + lim);
+            while (a < b) {
+                int m = (a + b + 1) >> 1;
+                if ((sk[m - lo0] >> sh) == p) a = m; else b = m - 1;
+            }
+            if (a - lo + 1 > lim) {
+                L = l + 1;
+                break;
+            }
+        }
+        level[i] = L;
+        const int sh = bits * (nlmax - L);
+        st = 1;
+        if (i > 0) st = sh >= 8 * (int)sizeof(K) ? 0 : ((sk[i - 1 - lo0] >> sh) != (ki >> sh));
+        start[i] = (unsigned char)st;
+    }
+    const int c = __syncthreads_count(st);  // every thread of the block reaches it
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = (unsigned)c;
 }
 
 template <class K>
 static void leafdepth_launch(const K* key, int n, int nlmax, int levcut, int ndiv0, int ndiv,
-                             int* level, unsigned char* start_flag, hipStream_t s, int bits)
+                             int* level, unsigned char* start_flag, unsigned* bcnt, hipStream_t s,
+                             int bits)
 {
-    k_leafdepth<K><<<grid_for(n, 256), 256, 0, s>>>(key, n, nlmax, levcut, ndiv0, ndiv, bits,
-                                                    level, start_flag);
+    const size_t lds = sizeof(K) * (256 + 2 * (size_t)std::max(ndiv0, ndiv));
+    k_leafdepth<K><<<grid_for(n, 256), 256, lds, s>>>(key, n, nlmax, levcut, ndiv0, ndiv, bits,
+                                                      level, start_flag, bcnt);
 }
 void launch_leafdepth(const u64* key, int n, int nlmax, int levcut, int ndiv0, int ndiv,
-                      int* level, unsigned char* start_flag, hipStream_t s, int bits)
+                      int* level, unsigned char* start_flag, unsigned* bcnt, hipStream_t s, int bits)
 {
-    leafdepth_launch(key, n, nlmax, levcut, ndiv0, ndiv, level, start_flag, s, bits);
+    leafdepth_launch(key, n, nlmax, levcut, ndiv0, ndiv, level, start_flag, bcnt, s, bits);
 }
 void launch_leafdepth(const unsigned* key, int n, int nlmax, int levcut, int ndiv0, int ndiv,
-                      int* level, unsigned char* start_flag, hipStream_t s, int bits)
+                      int* level, unsigned char* start_flag, unsigned* bcnt, hipStream_t s, int bits)
 {
-    leafdepth_launch(key, n, nlmax, levcut, ndiv0, ndiv, level, start_flag, s, bits);
+    leafdepth_launch(key, n, nlmax, levcut, ndiv0, ndiv, level, start_flag, bcnt, s, bits);
 }
 
-template <class K>
-__global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ starts,
-                                                     const int* __restrict__ nsel, int n,
-                                                     const K* __restrict__ key,
-                                                     const int* __restrict__ level, int nlmax,
-                                                     int bits, LeafRun* __restrict__ out)
+// exclusive scan in place of the nb block counts (one block of 1024
+// threads); *total gets their sum.  Chunks of 8192 counts pass through LDS
+// (coalesced loads and stores, all in flight at once): each thread scans 8
+// consecutive counts of the chunk, the waves' shuffles and totals do the
+// rest.
+__global__ __launch_bounds__(1024) void k_scan_counts(unsigned* __restrict__ c, int nb,
+                                                      int* __restrict__ total)
 {
-    int r = blockIdx.x * 256 + threadIdx.x;
-    int m = *nsel;
-    if (r >= m) return;
-    int s = starts[r];
-    int e = r + 1 < m ? starts[r + 1] : n;
-    int L = level[s];
+    __shared__ unsigned buf[8192];
+    __shared__ unsigned wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    unsigned carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += 8192) {
+        const int m = min(8192, nb - c0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int b = k * 1024 + t;
+            buf[b] = b < m ? c[c0 + b] : 0u;
+        }
+        __syncthreads();
+        unsigned v[8], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            v[k] = buf[8 * t + k];
+            sum += v[k];
+        }
+        unsigned x = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        unsigned run = carry + x - sum, all = 0;
+        for (int ww = 0; ww < 16; ++ww) {
+            if (ww < w) run += wsum[ww];
+            all += wsum[ww];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            buf[8 * t + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int b = k * 1024 + t;
+            if (b < m) c[c0 + b] = buf[b];
+        }
+        carry += all;
+        __syncthreads();  // buf and wsum are rewritten by the next chunk
+    }
+    if (t == 0) *total = (int)carry;
+}
+
+// the leaf records in Morton order: a leaf's place is the number of starts
+// before it (its block's offset plus the starts of the earlier lanes of the
+// block)
+template <class K>
+__global__ __launch_bounds__(256) void k_leaf_records(const unsigned char* __restrict__ start,
+                                                      const unsigned* __restrict__ boff, int n,
+                                                      const K* __restrict__ key,
+                                                      const int* __restrict__ level, int nlmax,
+                                                      int bits, LeafRun* __restrict__ out)
+{
+    __shared__ unsigned wc[4];
+    const int i = blockIdx.x * 256 + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool st = i < n && start[i];
+    const unsigned long long m = __ballot(st);
+    if (lane == 0) wc[w] = (unsigned)__popcll(m);
+    __syncthreads();
+    if (!st) return;
+    unsigned r = boff[blockIdx.x] + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    for (int ww = 0; ww < w; ++ww) r += wc[ww];
+    const int L = level[i];
     LeafRun o;
-    o.key = L == 0 ? 0ull : (u64)(key[s] >> (bits * (nlmax - L)));
+    o.key = L == 0 ? 0ull : (u64)(key[i] >> (bits * (nlmax - L)));
     o.level = L;
-    o.start = s;
-    o.count = e - s;
+    o.start = i;
     out[r] = o;
 }
 
-void launch_emit_leaves(const int* starts, const int* nsel, int n, const u64* key,
-                        const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits)
+size_t leaf_records_temp(int n) { return 4 * ((size_t)n / 256 + 2); }
+
+template <class K>
+static void leaf_records_launch(const unsigned char* start_flag, unsigned* bcnt, int n,
+                                const K* key, const int* level, int nlmax, int* nleaves,
+                                LeafRun* out, hipStream_t s, int bits)
 {
-    k_emit_leaves<u64><<<grid_for(n, 256), 256, 0, s>>>(starts, nsel, n, key, level, nlmax, bits,
-                                                        out);
+    const int nb = grid_for(n, 256);
+    k_scan_counts<<<1, 1024, 0, s>>>(bcnt, nb, nleaves);
+    k_leaf_records<K><<<nb, 256, 0, s>>>(start_flag, bcnt, n, key, level, nlmax, bits, out);
 }
-void launch_emit_leaves(const int* starts, const int* nsel, int n, const unsigned* key,
-                        const int* level, int nlmax, LeafRun* out, hipStream_t s, int bits)
+void launch_leaf_records(const unsigned char* start_flag, unsigned* bcnt, int n, const u64* key,
+                         const int* level, int nlmax, int* nleaves, LeafRun* out, hipStream_t s,
+                         int bits)
 {
-    k_emit_leaves<unsigned><<<grid_for(n, 256), 256, 0, s>>>(starts, nsel, n, key, level, nlmax,
-                                                             bits, out);
+    leaf_records_launch(start_flag, bcnt, n, key, level, nlmax, nleaves, out, s, bits);
+}
+void launch_leaf_records(const unsigned char* start_flag, unsigned* bcnt, int n,
+                         const unsigned* key, const int* level, int nlmax, int* nleaves,
+                         LeafRun* out, hipStream_t s, int bits)
+{
+    leaf_records_launch(start_flag, bcnt, n, key, level, nlmax, nleaves, out, s, bits);
 }
 
 // ---------------------------------------------------------------------------
-// sorting and selection (rocPRIM)
+// sorting (rocPRIM)
 // ---------------------------------------------------------------------------
+// The scratch queries below ask rocPRIM for every configuration, and each
+// query reads device attributes; the sizes depend on n only, so the last
+// answer per thread is kept (a transform asks before its first launch).
 size_t sort_pairs_u64_temp(int n)
 {
+    thread_local int last_n = -1;
+    thread_local size_t last_b = 0;
+    if (n == last_n) return last_b;
     size_t b = 0;
     HIPCHECK(rocprim::radix_sort_pairs(nullptr, b, (const u64*)nullptr, (u64*)nullptr,
                                        (const unsigned*)nullptr, (unsigned*)nullptr, n, 0, 64));
+    last_n = n;
+    last_b = b;
     return b;
 }
 
@@ -304,11 +404,276 @@ static size_t sort32_temp_all(int n)
                      sort32_temp<Onesweep32<10, BS, IPT>>(n, 30)});
 }
 
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (32-bit key, 32-bit index) pairs for keys of at
+// most 30 bits (tuning table key `sort` = 4 or 5): passes of at most 11 bits,
+// each in three kernels over tiles of 256 R items (4 wavefronts x R rounds x
+// 64 consecutive items, so that (wave, round, lane) is input order; R = 8,
+// or 16 for digits of up to 10 bits under shape 5):
+//   count:   the tile's digit histogram (LDS atomics), cnt[digit][tile];
+//   offsets: per digit, the exclusive scan over tiles and the digit total;
+//   scatter: an item's rank among its tile's items of its digit from
+//            ballots (the lanes of its round with the same digit, then a
+//            running count per wave and digit in LDS), plus the earlier
+//            waves', tiles' and digits' counts.
+// Equal keys keep their input order: the same permutation as any stable
+// sort.  Without input values the first pass takes the indices 0..n-1.
+// ---------------------------------------------------------------------------
+
+// exclusive prefix of v over the 256 threads of a block (wavefront shuffles,
+// then the four wave totals through LDS); *total gets the block's sum.
+// wtot: 4 words of LDS, free again after the next barrier.
+__device__ __forceinline__ unsigned rs_block_scan(unsigned v, unsigned* wtot, unsigned* total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wtot[w] = x;
+    __syncthreads();
+    unsigned pre = 0;
+    for (int ww = 0; ww < w; ++ww) pre += wtot[ww];
+    *total = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+    return pre + x - v;
+}
+
+// exclusive scan, in place, of a[0..n) (n <= 256 * per) held in LDS; a
+// barrier follows
+__device__ __forceinline__ unsigned rs_lds_scan(unsigned* a, int n, unsigned* wtot)
+{
+    const int per = (n + 255) / 256, t = threadIdx.x;
+    const int d0 = min(n, t * per), d1 = min(n, d0 + per);
+    unsigned sum = 0;
+    for (int d = d0; d < d1; ++d) sum += a[d];
+    unsigned total;
+    unsigned run = rs_block_scan(sum, wtot, &total);
+    for (int d = d0; d < d1; ++d) {
+        const unsigned c = a[d];
+        a[d] = run;
+        run += c;
+    }
+    __syncthreads();
+    return total;
+}
+
+// the tile's digit histogram, cnt[digit][tile]
+template <int R>
+__global__ __launch_bounds__(256) void k_rs_count(const unsigned* __restrict__ key, int n, int shift,
+                                                  int bits, unsigned* __restrict__ cnt, int ntile)
+{
+    constexpr int RS_TILE = 256 * R;
+    extern __shared__ unsigned rs_h[];
+    const int nd = 1 << bits;
+    for (int d = threadIdx.x; d < nd; d += 256) rs_h[d] = 0;
+    __syncthreads();
+    // one LDS atomic per distinct digit of a wavefront's 64 items (the
+    // lanes with the same digit found by ballots): tiles of clustered
+    // points hold few distinct high digits
+    const long base = (long)blockIdx.x * RS_TILE;
+    const int lane = threadIdx.x & 63;
+    unsigned kk[RS_TILE / 256];  // all loads in flight at once
+#pragma unroll
+    for (int r = 0; r < RS_TILE / 256; ++r) {
+        const long i = base + threadIdx.x + 256 * r;
+        kk[r] = i < n ? key[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_TILE / 256; ++r) {
+        const long i = base + threadIdx.x + 256 * r;
+        const bool valid = i < n;
+        const int d = valid ? (int)((kk[r] >> shift) & (unsigned)(nd - 1)) : 0;
+        unsigned long long peers = __ballot(valid);
+        for (int bit = 0; bit < bits; ++bit) {
+            const unsigned long long m = __ballot((d >> bit) & 1);
+            peers &= ((d >> bit) & 1) ? m : ~m;
+        }
+        if (valid && lane == __ffsll((long long)peers) - 1)
+            atomicAdd(&rs_h[d], (unsigned)__popcll(peers));
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < nd; d += 256) cnt[(size_t)d * ntile + blockIdx.x] = rs_h[d];
+}
+
+// one block per digit: cnt[d][.] becomes its exclusive scan over the
+// tiles, tot[d] its sum
+__global__ __launch_bounds__(256) void k_rs_offsets(unsigned* __restrict__ cnt, int ntile,
+                                                    unsigned* __restrict__ tot)
+{
+    __shared__ unsigned wtot[4];
+    unsigned* c = cnt + (size_t)blockIdx.x * ntile;
+    const int per = (ntile + 255) / 256, t = threadIdx.x;
+    const int b0 = min(ntile, t * per), b1 = min(ntile, b0 + per);
+    unsigned sum = 0;
+    for (int b = b0; b < b1; ++b) sum += c[b];
+    unsigned total;
+    unsigned run = rs_block_scan(sum, wtot, &total);
+    for (int b = b0; b < b1; ++b) {
+        const unsigned v = c[b];
+        c[b] = run;
+        run += v;
+    }
+    if (t == 0) tot[blockIdx.x] = total;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_rs_scatter(const unsigned* __restrict__ kin,
+                                                    const unsigned* __restrict__ vin,
+                                                    unsigned* __restrict__ kout,
+                                                    unsigned* __restrict__ vout, int n, int shift,
+                                                    int bits, const unsigned* __restrict__ pre,
+                                                    const unsigned* __restrict__ tot, int ntile)
+{
+    constexpr int RS_ROUNDS = R, RS_TILE = 256 * R;
+    extern __shared__ unsigned rs_s[];
+    const int nd = 1 << bits, t = threadIdx.x, lane = t & 63, w = t >> 6;
+    unsigned* wh = rs_s;               // 4 x nd: per-wave running counts, then wave offsets
+    unsigned* dbase = rs_s + 4 * nd;   // nd: digits' first output position
+    unsigned* lstart = rs_s + 5 * nd;  // nd: digits' first position in the tile
+    unsigned* tk = rs_s + 6 * nd;      // RS_TILE keys in digit order
+    unsigned* tv = tk + RS_TILE;       // RS_TILE values in digit order
+    __shared__ unsigned wtot[4];
+    const long i0 = (long)blockIdx.x * RS_TILE + (long)w * (RS_ROUNDS * 64);
+    unsigned kk[RS_ROUNDS], vv[RS_ROUNDS], rk[RS_ROUNDS];
+    int dg[RS_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {  // all loads in flight at once
+        const long i = i0 + r * 64 + lane;
+        kk[r] = i < n ? kin[i] : 0u;
+        vv[r] = i < n ? (vin ? vin[i] : (unsigned)i) : 0u;
+    }
+    for (int d = t; d < 4 * nd; d += 256) wh[d] = 0;
+    for (int d = t; d < nd; d += 256) dbase[d] = tot[d];
+    __syncthreads();
+    rs_lds_scan(dbase, nd, wtot);  // the digit totals' exclusive scan
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned* my = wh + w * nd;
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const long i = i0 + r * 64 + lane;
+        const bool valid = i < n;
+        const int d = (int)((kk[r] >> shift) & (unsigned)(nd - 1));
+        unsigned long long peers = __ballot(valid);
+        for (int bit = 0; bit < bits; ++bit) {
+            const unsigned long long m = __ballot((d >> bit) & 1);
+            peers &= ((d >> bit) & 1) ? m : ~m;
+        }
+        const unsigned old = my[d];
+        if (valid && lane == __ffsll((long long)peers) - 1) my[d] = old + (unsigned)__popcll(peers);
+        rk[r] = old + (unsigned)__popcll(peers & below);
+        dg[r] = valid ? d : -1;
+    }
+    __syncthreads();
+    // the waves' counts per digit to exclusive offsets across the waves;
+    // the tile's count per digit into lstart
+    for (int d = t; d < nd; d += 256) {
+        unsigned run = 0;
+#pragma unroll
+        for (int ww = 0; ww < 4; ++ww) {
+            const unsigned c = wh[ww * nd + d];
+            wh[ww * nd + d] = run;
+            run += c;
+        }
+        lstart[d] = run;
+    }
+    __syncthreads();
+    rs_lds_scan(lstart, nd, wtot);  // each digit's first place in the tile
+    // the tile in digit order through LDS, then written out in that order:
+    // consecutive threads write consecutive places of a digit's run
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int d = dg[r];
+        if (d < 0) continue;
+        const unsigned lp = lstart[d] + my[d] + rk[r];
+        tk[lp] = kk[r];
+        tv[lp] = vv[r];
+    }
+    __syncthreads();
+    const int cnt = (int)min((long)RS_TILE, (long)n - (long)blockIdx.x * RS_TILE);
+    for (int q = t; q < cnt; q += 256) {
+        const unsigned k = tk[q];
+        const int d = (int)((k >> shift) & (unsigned)(nd - 1));
+        const unsigned pos = dbase[d] + pre[(size_t)d * ntile + blockIdx.x] + (q - lstart[d]);
+        kout[pos] = k;
+        vout[pos] = tv[q];
+    }
+}
+
+static int rs_bits(int endbit, int* passes)
+{
+    const int np = (endbit + 10) / 11;
+    *passes = np;
+    return (endbit + np - 1) / np;
+}
+
+// scratch of the sort at the smaller tile (the larger one needs less)
+static size_t rs_temp(int n)
+{
+    const size_t ntile = ((size_t)n + 2047) / 2048;
+    return 256 + 2 * (((size_t)n * 4 + 255) & ~(size_t)255) + 4 * 2048 * (ntile + 1) + 4 * 2048;
+}
+
+template <int R>
+static void rs_pass(const unsigned* ki, const unsigned* vi, unsigned* ko, unsigned* vo, int n,
+                    int shift, int b, unsigned* cnt, unsigned* tot, hipStream_t s)
+{
+    const int ndb = 1 << b, ntile = (n + 256 * R - 1) / (256 * R);
+    k_rs_count<R><<<ntile, 256, (size_t)ndb * 4, s>>>(ki, n, shift, b, cnt, ntile);
+    k_rs_offsets<<<ndb, 256, 0, s>>>(cnt, ntile, tot);
+    const size_t lds = ((size_t)6 * ndb + 2 * 256 * R) * 4;  // at most 64 KiB
+    if (lds >= 65536) {
+        static const bool set = [] {
+            HIPCHECK(hipFuncSetAttribute((const void*)k_rs_scatter<R>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 65536));
+            return true;
+        }();
+        (void)set;
+    }
+    k_rs_scatter<R><<<ntile, 256, lds, s>>>(ki, vi, ko, vo, n, shift, b, cnt, tot, ntile);
+}
+
+static void rs_sort(void* tmp, size_t tmpb, const unsigned* kin, unsigned* kout, const unsigned* vin,
+                    unsigned* vout, int n, int endbit, hipStream_t s, bool wide)
+{
+    if (n <= 0) return;
+    if (tmpb < rs_temp(n)) fail(-2, "radix sort scratch too small");
+    int np = 0;
+    const int bits = rs_bits(std::max(endbit, 1), &np);
+    const int ntile = (n + 2047) / 2048;
+    const size_t pad = ((size_t)n * 4 + 255) & ~(size_t)255;
+    char* p = (char*)tmp;
+    unsigned* kb[2] = {(unsigned*)p, kout};
+    unsigned* vb[2] = {(unsigned*)(p + pad), vout};
+    unsigned* cnt = (unsigned*)(p + 2 * pad);
+    unsigned* tot = cnt + (size_t)2048 * (ntile + 1);
+    // ping-pong so that the last pass writes kout / vout
+    const unsigned* ki = kin;
+    const unsigned* vi = vin;  // null: the indices 0..n-1
+    for (int pass = 0; pass < np; ++pass) {
+        const int dst = (np - 1 - pass) % 2 == 0 ? 1 : 0;
+        const int shift = pass * bits;
+        const int b = std::min(bits, endbit - shift);
+        if (wide && b <= 10)
+            rs_pass<16>(ki, vi, kb[dst], vb[dst], n, shift, b, cnt, tot, s);
+        else
+            rs_pass<8>(ki, vi, kb[dst], vb[dst], n, shift, b, cnt, tot, s);
+        ki = kb[dst];
+        vi = vb[dst];
+    }
+}
+
 size_t sort_pairs_u32_temp(int n)
 {
-    return std::max({sort32_temp_all<1024, 16>(n), sort32_temp_all<512, 8>(n),
-                     sort32_temp_all<256, 16>(n), sort32_temp_all<256, 8>(n),
-                     sort32_temp<rocprim::default_config>(n, 32)});
+    thread_local int last_n = -1;
+    thread_local size_t last_b = 0;
+    if (n == last_n) return last_b;
+    last_b = std::max({sort32_temp_all<1024, 16>(n), sort32_temp_all<512, 8>(n),
+                       sort32_temp_all<256, 16>(n), sort32_temp_all<256, 8>(n),
+                       sort32_temp<rocprim::default_config>(n, 32), rs_temp(n)});
+    last_n = n;
+    return last_b;
 }
 
 template <unsigned BS, unsigned IPT>
@@ -335,27 +700,13 @@ void sort_pairs_u32(void* tmp, size_t tmpb, const unsigned* kin, unsigned* kout,
         return;
     }
     switch (shape) {
+    case 4: rs_sort(tmp, tmpb, kin, kout, vin, vout, n, endbit, s, false); break;
+    case 5: rs_sort(tmp, tmpb, kin, kout, vin, vout, n, endbit, s, true); break;
     case 1: sort32<512, 8>(tmp, tmpb, kin, kout, vin, vout, n, endbit, s); break;
     case 2: sort32<256, 16>(tmp, tmpb, kin, kout, vin, vout, n, endbit, s); break;
     case 3: sort32<256, 8>(tmp, tmpb, kin, kout, vin, vout, n, endbit, s); break;
     default: sort32<1024, 16>(tmp, tmpb, kin, kout, vin, vout, n, endbit, s); break;
     }
-}
-
-size_t select_flagged_temp(int n)
-{
-    size_t b = 0;
-    HIPCHECK(rocprim::select(nullptr, b, rocprim::counting_iterator<int>(0),
-                             (const unsigned char*)nullptr, (int*)nullptr, (int*)nullptr,
-                             (size_t)n));
-    return b;
-}
-
-void select_flagged(void* tmp, size_t tmpb, const unsigned char* flags, int n, int* out,
-                    int* nsel, hipStream_t s)
-{
-    HIPCHECK(rocprim::select(tmp, tmpb, rocprim::counting_iterator<int>(0), flags, out, nsel,
-                             (size_t)n, s));
 }
 
 // ---------------------------------------------------------------------------
