@@ -545,6 +545,46 @@ struct Builder {
         }
     }
 
+    // Colleague lists after the second restriction pass, for levels up to
+    // hi, from the first level l0 that pass refined (lists of levels <= l0
+    // are valid).  The pass ran on lists it did not update, so any list of
+    // level l+1 may lack children of a box the pass refined at level l, or
+    // of one the pass created; a box refined at level l by the pass has flag
+    // 1 afterwards, and a created box with children was itself refined by
+    // the pass.  A list of level l+1 is recomputed with its family, so the
+    // families to recompute are those of the boxes (with children) in the
+    // list of a box the pass refined at level l, colleagues being
+    // symmetric.  Level by level, so that the lists of level l are final
+    // when their families are read.
+    void refresh_after_second_pass(int l0, int hi, const std::vector<int>& t1,
+                                   const std::vector<int>& t2)
+    {
+        thread_local std::vector<char> mark;
+        thread_local std::vector<int> todo;
+        mark.assign(T.nboxes(), 0);
+        for (int l = l0; l < hi; ++l) {
+            todo.clear();
+            const int grp[3][2] = {{T.laddr[2 * l], T.laddr[2 * l + 1]},
+                                   {t1[2 * l], t1[2 * l + 1]},
+                                   {t2[2 * l], t2[2 * l + 1]}};
+            for (const auto& g : grp)
+                for (int b = g[0]; b <= g[1]; ++b) {
+                    if (flag[b] != 1) continue;
+                    for (int i = 0; i < ncol[b]; ++i) {
+                        const int c = col[NC * b + i];
+                        if (nch[c] > 0 && !mark[c]) {
+                            mark[c] = 1;
+                            todo.push_back(c);
+                        }
+                    }
+                }
+            const int* tp = todo.data();  // this thread's list, read by the workers
+            par_range((long)todo.size(), 256, [&, tp](long i0, long i1) {
+                for (long i = i0; i < i1; ++i) family_colleagues(tp[i]);
+            });
+        }
+    }
+
     // flag++ boxes with a touching grandchild-level box that has children are
     // marked for refinement (updateflags, tree_routs4.f:217-329,
     // tree_routs3d.f:226-344).
@@ -575,8 +615,8 @@ struct Builder {
     // Level restriction, pts_tree_fix_lr (pts_tree2d4.f:693-996,
     // pts_tree3d.f:855-1171), on colleague lists valid through level L-2 (the
     // deepest the procedure reads).  Returns the deepest level whose
-    // colleague lists are still valid afterwards.
-    int fix_level_restriction()
+    // colleague lists are valid afterwards: at least min(want, L-2).
+    int fix_level_restriction(int want)
     {
         const int L = T.nlevels;
         std::fill(T.flag.begin(), T.flag.end(), 0);
@@ -642,7 +682,7 @@ struct Builder {
             if (flag[b] != 3) flag[b] = 0;
             else ++n3;
         }
-        int valid = L - 2;
+        int valid = L - 2, first2 = -1;
         if (n3 > 0) {  // flag++ boxes: the second pass
             for (int l = 2; l <= L - 2; ++l) {
                 update_flags(T.laddr[2 * l], T.laddr[2 * l + 1]);
@@ -657,11 +697,16 @@ struct Builder {
                 // the new boxes get lists; the lists of the boxes of level l+1
                 // that they touch are not updated (the reference does not
                 // either, and the pass reads them in that state)
-                if (T.nboxes() > c0) valid = std::min(valid, l);
+                if (T.nboxes() > c0 && first2 < 0) first2 = l;
                 par_range(T.nboxes() - c0, 1024, [&](long i0, long i1) {
                     for (long i = i0; i < i1; ++i) colleagues(c0 + (int)i);
                 });
             }
+        }
+        if (first2 >= 0) {
+            const int hi = std::min(want, L - 2);
+            if (hi > first2) refresh_after_second_pass(first2, hi, t1, t2);
+            valid = std::max(hi, first2);
         }
         renumber(t1, t2, valid);
         return valid;
@@ -781,8 +826,8 @@ void build_tree(TreeD<D>& T, const std::vector<LeafRun>& leaves, int nlmax, int 
     const int L = T.nlevels;
     B.all_colleagues(L - 2);
     int valid = std::max(L - 2, 0);
-    if (L >= 2) valid = B.fix_level_restriction();
     const int keep = std::min(std::max(npwlevel, 0), L);
+    if (L >= 2) valid = B.fix_level_restriction(keep);
     if (valid < keep) {
         B.colleagues_between(valid, keep);
         valid = keep;
