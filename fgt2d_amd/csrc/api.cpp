@@ -979,6 +979,27 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         wsd = W.get<double>("dip_w", 2 * (size_t)nd * ns);
         launch_gather_dip(q.rn, q.ds, nd, isrc, ns, wsd, s_g);
     }
+    // the sorted outputs, accumulated into by the near field and the
+    // evaluations, cleared here too
+    const bool use_src = q.ifpgh > 0, use_trg = q.ifpghtarg > 0 && nt > 0;
+    const int ws_ = use_src ? ns : 0, wt_ = use_trg ? nt : 0;
+    double *ps = nullptr, *gs = nullptr, *hs = nullptr, *pt = nullptr, *gt = nullptr, *ht = nullptr;
+    auto cleared = [&](const char* name, int k, int ifp, int n) {
+        if (ifp < k) return (double*)nullptr;
+        double* p = W.get<double>(name, (size_t)k * nd * n);
+        launch_zero(p, (size_t)k * nd * n, s_g);
+        return p;
+    };
+    if (use_src) {
+        ps = cleared("pot_s", 1, q.ifpgh, ws_);
+        gs = cleared("grad_s", 2, q.ifpgh, ws_);
+        hs = cleared("hess_s", 3, q.ifpgh, ws_);
+    }
+    if (use_trg) {
+        pt = cleared("pot_t", 1, q.ifpghtarg, wt_);
+        gt = cleared("grad_t", 2, q.ifpghtarg, wt_);
+        ht = cleared("hess_t", 3, q.ifpghtarg, wt_);
+    }
     tm.end(S_GATHER, s_g);
 
     // Work that needs no tree, queued now so that the GPU runs it while the
@@ -1018,7 +1039,6 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     W.last_nt = nt;
     W.last_dim = 2;
     const int nb = T.nboxes(), nlevels = T.nlevels, nleaves = (int)T.leaf_of_rank.size();
-    const bool use_src = q.ifpgh > 0, use_trg = q.ifpghtarg > 0 && nt > 0;
     const int npg = std::max(q.ifpgh, q.ifpghtarg);
     const int nadd = ifdipole + npg;  // fgt2d.f:340
     std::vector<int> nlocal(nlevels + 1, 0);
@@ -1183,7 +1203,13 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         std::memcpy(hp, pk.data(), pk.size());
         HIPCHECK(hipMemcpyAsync(dplan, hp, pk.size(), hipMemcpyHostToDevice, s));
     }
-    W.order(s_g, s);  // the sorted points, charges and dipole weights
+    // the expansion stores are accumulated into: cleared on the side stream
+    // while the point ranges are found
+    const size_t per = 2 * (size_t)nexp * nd;  // elements (double or float) per slot
+    double* mp = expansions ? (double*)W.get<char>("mp", per * nslot * esz) : nullptr;
+    double* lc = expansions ? (double*)W.get<char>("loc", per * nslot * esz) : nullptr;
+    const size_t nexp_d = per * nslot * esz / sizeof(double);  // per is even
+    if (expansions) launch_zero3(nullptr, 0, mp, nexp_d, lc, nexp_d, s_g);
     auto D = [&](size_t off) { return (void*)(dplan + off); };
     const int4* d_child = (const int4*)D(o_child);
     const int* d_par = (const int*)D(o_par);
@@ -1215,28 +1241,9 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     tm.src_se = src_se, tm.trg_se = trg_se, tm.nb = nb, tm.nd = nd;
     if (!tg) launch_zero((double*)trg_se, (size_t)nb, s);  // int2 pairs, 8 bytes each
     tm.end(S_RANGES);
-
-    // sorted outputs
-    const int ws_ = use_src ? ns : 0, wt_ = use_trg ? nt : 0;
-    double *ps = nullptr, *gs = nullptr, *hs = nullptr, *pt = nullptr, *gt = nullptr, *ht = nullptr;
-    if (use_src) {
-        ps = W.get<double>("pot_s", (size_t)nd * ws_);
-        launch_zero(ps, (size_t)nd * ws_, s);
-        if (q.ifpgh >= 2) { gs = W.get<double>("grad_s", 2 * (size_t)nd * ws_); launch_zero(gs, 2 * (size_t)nd * ws_, s); }
-        if (q.ifpgh >= 3) { hs = W.get<double>("hess_s", 3 * (size_t)nd * ws_); launch_zero(hs, 3 * (size_t)nd * ws_, s); }
-    }
-    // the expansion stores are accumulated into: cleared together with the
-    // target potentials, in one launch
-    const size_t per = 2 * (size_t)nexp * nd;  // elements (double or float) per slot
-    double* mp = expansions ? (double*)W.get<char>("mp", per * nslot * esz) : nullptr;
-    double* lc = expansions ? (double*)W.get<char>("loc", per * nslot * esz) : nullptr;
-    if (use_trg) pt = W.get<double>("pot_t", (size_t)nd * wt_);
-    const size_t nexp_d = per * nslot * esz / sizeof(double);  // per is even
-    launch_zero3(pt, use_trg ? (size_t)nd * wt_ : 0, mp, nexp_d, lc, nexp_d, s);
-    if (use_trg) {
-        if (q.ifpghtarg >= 2) { gt = W.get<double>("grad_t", 2 * (size_t)nd * wt_); launch_zero(gt, 2 * (size_t)nd * wt_, s); }
-        if (q.ifpghtarg >= 3) { ht = W.get<double>("hess_t", 3 * (size_t)nd * wt_); launch_zero(ht, 3 * (size_t)nd * wt_, s); }
-    }
+    // everything after reads the side stream's work: the sorted points,
+    // charges and dipole weights, and the cleared outputs and expansions
+    W.order(s_g, s);
 
     // ---------------- step 6: near field, cutoff eps/90 (fgt2d.f:1316) ----------------
     // on a side stream, concurrently with steps 1-4; it accumulates into the

@@ -1051,12 +1051,11 @@ __global__ __launch_bounds__(256) void k_form(FormArgs a, int chunk)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_seg_scan(SegLists L)
 {
-    __shared__ int buf[1024];
-    __shared__ int carry;
+    // rounds of 1024 boxes: wavefront shuffles, then the 16 wave totals
+    __shared__ int wsum[16];
     const SegList d = L.l[blockIdx.x];
-    const int t = threadIdx.x;
-    if (t == 0) carry = 0;
-    __syncthreads();
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int carry = 0;
     for (int base = 0; base <= d.n; base += 1024) {
         const int i = base + t;
         int c = 0;
@@ -1065,21 +1064,27 @@ __global__ __launch_bounds__(1024) void k_seg_scan(SegLists L)
             int len = r.y - r.x;
             c = len > 0 ? (len + d.seg - 1) / d.seg : 0;
         }
-        buf[t] = c;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int v = t >= off ? buf[t - off] : 0;
-            __syncthreads();
-            buf[t] += v;
-            __syncthreads();
+        int x = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
         }
-        const int incl = buf[t];
-        if (i <= d.n) d.tstart[i] = carry + incl - c;
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int ww = 0; ww < 16; ++ww) {
+            const int v = wsum[ww];
+            pre += ww < w ? v : 0;
+            tot += v;
+        }
+        const int excl = carry + pre + x - c;
+        if (i <= d.n) d.tstart[i] = excl;
         if (d.segbox)
-            for (int j = 0; j < c; ++j) d.segbox[carry + incl - c + j] = i;
-        __syncthreads();
-        if (t == 1023) carry += incl;
-        __syncthreads();
+            for (int j = 0; j < c; ++j) d.segbox[excl + j] = i;
+        carry += tot;
+        __syncthreads();  // wsum is rewritten by the next round
     }
 }
 
