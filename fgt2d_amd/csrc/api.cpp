@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <map>
 #include <memory>
@@ -28,6 +29,7 @@
 #include <vector>
 
 #include <dlfcn.h>
+#include <unistd.h>
 
 #include "fgtpw.h"
 #include "fgtpw_impl.hpp"
@@ -185,6 +187,80 @@ struct Tables3D {
                  *wm1 = nullptr;  // wm1: complex (3, npw) m2l factors per axis
 };
 
+// One thread that runs a workspace's host-side job beside the launches of
+// the same call (the near-field lists), started on first use.  wait()
+// returns once the posted job has run and rethrows what it threw.
+class HostHelper {
+public:
+    ~HostHelper()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_ && pid_ == getpid()) {
+            th_->join();
+            delete th_;
+        }
+    }
+    void post(std::function<void()> f)
+    {
+        // a forked child has none of its parent's threads: it starts its own
+        // (the parent's std::thread object is left alone)
+        if (!th_ || pid_ != getpid()) {
+            th_ = new std::thread([this] { loop(); });
+            pid_ = getpid();
+        }
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            job_ = std::move(f);
+            pending_ = true;
+            err_ = nullptr;
+        }
+        cv_.notify_all();
+    }
+    void wait()
+    {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return !pending_; });
+        if (err_) {
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+
+private:
+    void loop()
+    {
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            cv_.wait(lk, [&] { return pending_ || stop_; });
+            if (!pending_) return;  // stopping with nothing posted
+            std::function<void()> f = std::move(job_);
+            lk.unlock();
+            std::exception_ptr e;
+            try {
+                f();
+            } catch (...) {
+                e = std::current_exception();
+            }
+            lk.lock();
+            err_ = e;
+            pending_ = false;
+            cv_.notify_all();
+        }
+    }
+    std::thread* th_ = nullptr;
+    pid_t pid_ = 0;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::function<void()> job_;
+    bool pending_ = false, stop_ = false;
+    std::exception_ptr err_;
+};
+
 struct Workspace {
     struct Buf {
         void* p = nullptr;
@@ -213,6 +289,7 @@ struct Workspace {
     std::vector<unsigned long long> h_lkey64;  // smallest key of every leaf, rank order
     std::vector<unsigned> h_lkey32;
     size_t leaf_guess = 0;  // leaf records of the last call (size of the first read-back)
+    std::unique_ptr<HostHelper> helper;  // host work beside a call's launches
     Tables2D tab;
     Tables3D tab3;
     // the tree of the last transform, kept for inspection
@@ -990,15 +1067,25 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         launch_zero(p, (size_t)k * nd * n, s_g);
         return p;
     };
+    // the near field has outputs of its own, added in the final scatter:
+    // nothing orders it against the evaluations
+    double *nps = nullptr, *ngs = nullptr, *nhs = nullptr, *npt = nullptr, *ngt = nullptr,
+           *nht = nullptr;
     if (use_src) {
         ps = cleared("pot_s", 1, q.ifpgh, ws_);
         gs = cleared("grad_s", 2, q.ifpgh, ws_);
         hs = cleared("hess_s", 3, q.ifpgh, ws_);
+        nps = cleared("nf_pot_s", 1, q.ifpgh, ws_);
+        ngs = cleared("nf_grad_s", 2, q.ifpgh, ws_);
+        nhs = cleared("nf_hess_s", 3, q.ifpgh, ws_);
     }
     if (use_trg) {
         pt = cleared("pot_t", 1, q.ifpghtarg, wt_);
         gt = cleared("grad_t", 2, q.ifpghtarg, wt_);
         ht = cleared("hess_t", 3, q.ifpghtarg, wt_);
+        npt = cleared("nf_pot_t", 1, q.ifpghtarg, wt_);
+        ngt = cleared("nf_grad_t", 2, q.ifpghtarg, wt_);
+        nht = cleared("nf_hess_t", 3, q.ifpghtarg, wt_);
     }
     tm.end(S_GATHER, s_g);
 
@@ -1203,6 +1290,22 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         std::memcpy(hp, pk.data(), pk.size());
         HIPCHECK(hipMemcpyAsync(dplan, hp, pk.size(), hipMemcpyHostToDevice, s));
     }
+    // the near field's lists on the helper thread, while this one queues
+    // the expansion steps; the guard waits for them on every way out (the
+    // job reads this call's locals)
+    if (!W.helper) W.helper.reset(new HostHelper);
+    W.helper->post(near_field_lists);
+    struct HelperJoin {
+        HostHelper* h;
+        ~HelperJoin()
+        {
+            if (h) try {
+                    h->wait();
+                } catch (...) {
+                }
+        }
+    } nf_join{W.helper.get()};
+
     // the expansion stores are accumulated into: cleared on the side stream
     // while the point ranges are found
     const size_t per = 2 * (size_t)nexp * nd;  // elements (double or float) per slot
@@ -1246,13 +1349,14 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     W.order(s_g, s);
 
     // ---------------- step 6: near field, cutoff eps/90 (fgt2d.f:1316) ----------------
-    // on a side stream, concurrently with steps 1-4; it accumulates into the
-    // zeroed outputs before any evaluation does (the evaluations wait for it)
+    // on a side stream, into outputs of its own (cleared with the others);
+    // its lists are built on the host once every expansion step is queued
     hipStream_t s_nf = g_serial ? s : W.aux[1], s_lv = g_serial ? s : W.aux[0];
-    W.order(s, s_nf);  // after the ranges and the zeroed outputs
-    tm.begin(S_DIRECT, s_nf);
+    W.order(s, s_nf);  // after the ranges, the sorted points and the cleared outputs
     auto near_field = [&]() {
-    near_field_lists();
+    nf_join.h = nullptr;
+    W.helper->wait();  // the lists
+    tm.begin(S_DIRECT, s_nf);
     Pack pn(W.h_plan_nf);
     size_t o_units = pn.add(unit_ids), o_uptr = pn.add(unit_ptr), o_ulist = pn.add(unit_list);
     char* dnf = W.get<char>("plan_nf", pn.size());
@@ -1280,7 +1384,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             d2.tgt_se = trg_se;
             d2.xt = xt;
             d2.ifpgh = q.ifpghtarg;
-            d2.pot = pt; d2.grad = gt; d2.hess = ht;
+            d2.pot = npt; d2.grad = ngt; d2.hess = nht;
             tm.ctx(npwlevel, nullptr, 3);  // units: the pairs, counted after the call
             launch_direct(d2, s_nf);
         }
@@ -1289,7 +1393,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             d2.tgt_se = src_se;
             d2.xt = xs;
             d2.ifpgh = q.ifpgh;
-            d2.pot = ps; d2.grad = gs; d2.hess = hs;
+            d2.pot = nps; d2.grad = ngs; d2.hess = nhs;
             tm.ctx(npwlevel, nullptr, 4);
             launch_direct(d2, s_nf);
         }
@@ -1400,10 +1504,6 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                    use_src, use_trg, d_slot, d_wm2l, npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0],
                    nlevstart == npwlevel, nd, nexp, mp, lc, s, mixed);
         tm.end(S_M2L);
-        near_field();
-        // the evaluations accumulate after the near field
-        W.order(s_nf, s);
-        W.order(s_nf, s_lv);
         // level l is evaluated (on s_lv, the finest level on s) once the
         // split that completes its local expansions is queued
         auto eval_level = [&](int l, hipStream_t st) {
@@ -1467,26 +1567,26 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         eval_level(nlevels, s);
         W.order(s_lv, s);
         tm.end(S_EVAL);
-    } else {
-        near_field();
     }
+    near_field();
 
+    // s_lv's work (the gathers, the coarse levels) is joined already; each
+    // join costs a host-side wait of about 10 us
     W.order(s_nf, s);
-    W.order(s_lv, s);
 
     // ---------------- back to input order ----------------
     tm.begin(S_SCATTER);
     if (use_src) {
         tm.ctx(-1, nullptr, -1, ns);
-        launch_scatter(ps, isrc, ns, nd, q.pot, s);
-        if (q.ifpgh >= 2) launch_scatter(gs, isrc, ns, 2 * nd, q.grad, s);
-        if (q.ifpgh >= 3) launch_scatter(hs, isrc, ns, 3 * nd, q.hess, s);
+        launch_scatter(ps, isrc, ns, nd, q.pot, s, nps);
+        if (q.ifpgh >= 2) launch_scatter(gs, isrc, ns, 2 * nd, q.grad, s, ngs);
+        if (q.ifpgh >= 3) launch_scatter(hs, isrc, ns, 3 * nd, q.hess, s, nhs);
     }
     if (use_trg) {
         tm.ctx(-1, nullptr, -1, nt);
-        launch_scatter(pt, itarg, nt, nd, q.pottarg, s);
-        if (q.ifpghtarg >= 2) launch_scatter(gt, itarg, nt, 2 * nd, q.gradtarg, s);
-        if (q.ifpghtarg >= 3) launch_scatter(ht, itarg, nt, 3 * nd, q.hesstarg, s);
+        launch_scatter(pt, itarg, nt, nd, q.pottarg, s, npt);
+        if (q.ifpghtarg >= 2) launch_scatter(gt, itarg, nt, 2 * nd, q.gradtarg, s, ngt);
+        if (q.ifpghtarg >= 3) launch_scatter(ht, itarg, nt, 3 * nd, q.hesstarg, s, nht);
     }
     tm.end(S_SCATTER);
     HIPCHECK(hipGetLastError());
@@ -2595,10 +2695,11 @@ size_t noalloc_point_bytes(int nd, int ns, int nt, int ifdipole, int ifpgh, int 
     };
     size_t tmpb = std::max(sort_pairs_u64_temp((int)nmx), sort_pairs_u32_temp((int)nmx));
     tmpb = std::max(tmpb, leaf_records_temp(ns));
+    // outputs twice: the evaluations' and the near field's
     size_t b = 2 * r(8 * NS) + r(4 * nmx) + r(4 * NS) + r(tmpb) + r(4 * NS) + r(NS) +
                r(256 + sizeof(LeafRun) * NS) + r(16 * NS) + r(8 * ND * NS) +
-               (ifdipole ? r(16 * ND * NS) : 0) + outs(ifpgh, NS);
-    if (nt > 0) b += r(4 * NT) + r(16 * NT) + 2 * r(8 * NT) + outs(ifpghtarg, NT);
+               (ifdipole ? r(16 * ND * NS) : 0) + 2 * outs(ifpgh, NS);
+    if (nt > 0) b += r(4 * NT) + r(16 * NT) + 2 * r(8 * NT) + 2 * outs(ifpghtarg, NT);
     return b;
 }
 }  // namespace
@@ -3187,9 +3288,12 @@ int quantities(int dim, int ifpgh)
 size_t footprint(const ServeRequest& r, double extent)
 {
     const double d = r.dim, n = (double)r.ns + r.nt, nch = std::pow(2.0, d);
+    // per output quantity: the output, its sorted accumulator, staging and,
+    // in 2D, the near field's own accumulator
+    const double per_out = r.dim == 2 ? 32.0 : 24.0;
     double bytes = 16.0 * ((d + r.nd) * r.ns + d * r.nt) +
-                   24.0 * r.nd * ((double)quantities(r.dim, r.ifpgh) * r.ns +
-                                  (double)quantities(r.dim, r.ifpghtarg) * r.nt) +
+                   per_out * r.nd * ((double)quantities(r.dim, r.ifpgh) * r.ns +
+                                     (double)quantities(r.dim, r.ifpghtarg) * r.nt) +
                    48.0 * n;
     const double lne = std::log(1.0 / r.eps);
     const double dc = std::sqrt(r.delta * lne);

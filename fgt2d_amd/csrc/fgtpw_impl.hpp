@@ -181,7 +181,7 @@ void launch_gather(const double* xy, const double* q, int nd, const unsigned* pe
 void launch_zero(double* p, size_t n, hipStream_t s);
 void launch_zero3(double* a, size_t na, double* b, size_t nb, double* c, size_t nc, hipStream_t s);
 void launch_scatter(const double* v_s, const unsigned* perm, int n, int w, double* v,
-                    hipStream_t s);
+                    hipStream_t s, const double* v_s2 = nullptr);
 
 struct FormArgs {
     const int* boxes;

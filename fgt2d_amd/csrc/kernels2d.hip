@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_select.hpp>
@@ -904,7 +905,10 @@ void launch_zero(double* p, size_t n, hipStream_t s)
 }
 
 // v[perm[i]*w + c] = v_s[i*w + c]  (dreorderi / dreorderiv, fmmcommon2d.f:29-90)
+// v[perm[i]] = vs[i] (+ vs2[i]): the sorted outputs back to input order,
+// with the near field's own outputs added when vs2 is given
 __global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ vs,
+                                                 const double* __restrict__ vs2,
                                                  const unsigned* __restrict__ perm, int n, int w,
                                                  double* __restrict__ v)
 {
@@ -912,14 +916,17 @@ __global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ vs,
     if (t >= (long)n * w) return;
     long i = t / w;
     int c = (int)(t - i * w);
-    v[(long)perm[i] * w + c] = vs[t];
+    double x = vs[t];
+    if (vs2) x = vs2[t] + x;
+    v[(long)perm[i] * w + c] = x;
 }
 
 void launch_scatter(const double* v_s, const unsigned* perm, int n, int w, double* v,
-                    hipStream_t s)
+                    hipStream_t s, const double* v_s2)
 {
     if (n <= 0) return;
-    klaunch(KID_SCATTER, k_scatter, grid_for((long)n * w, 256), 256, 0, s, v_s, perm, n, w, v);
+    klaunch(KID_SCATTER, k_scatter, grid_for((long)n * w, 256), 256, 0, s, v_s, v_s2, perm, n, w,
+            v);
 }
 
 // ---------------------------------------------------------------------------
@@ -1142,13 +1149,24 @@ __device__ __forceinline__ int seg_count(const int* tstart, int nbox)
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
 // Kernels that take more than 64 KB of dynamic LDS (gfx950 has 160 KB per
-// workgroup) declare it once per instantiation.
+// workgroup) declare it.  The attribute is a limit: it is raised only when a
+// launch needs more than the kernel's largest earlier size, since the call
+// costs host time on every launch path.
 template <class F>
 static void allow_lds(F* kernel, size_t bytes)
 {
-    if (bytes >= 65536)
-        HIPCHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)bytes));
+    if (bytes < 65536) return;
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, size_t>> set;  // per kernel, the limit set
+    const void* k = (const void*)kernel;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = std::find_if(set.begin(), set.end(), [k](const auto& e) { return e.first == k; });
+    if (it != set.end() && it->second >= bytes) return;
+    HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    if (it != set.end())
+        it->second = bytes;
+    else
+        set.emplace_back(k, bytes);
 }
 
 __device__ __forceinline__ d4_t mfma16x16x4(double a, double b, d4_t c)

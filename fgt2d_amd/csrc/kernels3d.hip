@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "fgtpw_impl.hpp"
 
@@ -227,12 +228,23 @@ void launch_gather3(const double* p, const double* q, int nd, const unsigned* pe
 // ---------------------------------------------------------------------------
 constexpr int F3_CHUNK = 32;  // sources staged per pass
 
+// the dynamic LDS limit of a kernel, raised only when a launch needs more
+// than the kernel's largest earlier size (the call costs host time)
 template <class F>
 static void allow_lds3(F* kernel, size_t bytes)
 {
-    if (bytes > 65536)
-        HIPCHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)bytes));
+    if (bytes <= 65536) return;
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, size_t>> set;  // per kernel, the limit set
+    const void* k = (const void*)kernel;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = std::find_if(set.begin(), set.end(), [k](const auto& e) { return e.first == k; });
+    if (it != set.end() && it->second >= bytes) return;
+    HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    if (it != set.end())
+        it->second = bytes;
+    else
+        set.emplace_back(k, bytes);
 }
 
 // F3_MAXM: register tile of the moment accumulators (Hermite orders up to
