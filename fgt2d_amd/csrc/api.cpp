@@ -261,6 +261,41 @@ private:
     std::exception_ptr err_;
 };
 
+// Growable host storage of a call's plan: pinned, so that its upload is one
+// DMA from where it was packed, or pageable for the no-allocation entry
+// point.  Grows only while a plan is packed, when the last call's upload
+// from it has completed (a read-back on the same stream came since).
+struct PlanStore {
+    std::vector<char> pageable;
+    char* pin = nullptr;
+    size_t cap = 0;
+    bool pinned = true;
+    char* reserve(size_t n, size_t keep)
+    {
+        if (!pinned) {
+            if (pageable.size() < n) pageable.resize(std::max(n, 2 * pageable.size()));
+            return pageable.data();
+        }
+        if (cap < n) {
+            const size_t want = std::max<size_t>(n + n / 2, 1 << 16);
+            char* q = nullptr;
+            HIPCHECK(hipHostMalloc((void**)&q, want, hipHostMallocDefault));
+            if (keep) std::memcpy(q, pin, keep);
+            if (pin) HIPCHECK(hipHostFree(pin));
+            pin = q;
+            cap = want;
+        }
+        return pin;
+    }
+    char* data() { return pinned ? pin : pageable.data(); }
+    void release()
+    {
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        cap = 0;
+    }
+};
+
 struct Workspace {
     struct Buf {
         void* p = nullptr;
@@ -285,11 +320,14 @@ struct Workspace {
     std::vector<hipEvent_t> kev;  // per-kernel timing events, grown on demand
     // host scratch of the plan, reused from call to call
     std::vector<int> h_units, h_uptr, h_ulist, h_nl1, h_l1, h_sp, h_sl, h_fill, h_slot, h_slot2;
-    std::vector<char> h_plan, h_plan_nf;
+    PlanStore h_plan, h_plan_nf;
     std::vector<unsigned long long> h_lkey64;  // smallest key of every leaf, rank order
     std::vector<unsigned> h_lkey32;
     size_t leaf_guess = 0;  // leaf records of the last call (size of the first read-back)
     std::unique_ptr<HostHelper> helper;  // host work beside a call's launches
+    std::vector<std::vector<int>> h_form, h_merge, h_split, h_eval;  // per-level work lists
+    std::vector<int> h_m2l;
+    std::vector<char> h_iflocal;
     Tables2D tab;
     Tables3D tab3;
     // the tree of the last transform, kept for inspection
@@ -369,6 +407,8 @@ struct Workspace {
         if (pin) (void)hipHostFree(pin);
         pin = nullptr;
         pin_cap = 0;
+        h_plan.release();
+        h_plan_nf.release();
         if (pin2) (void)hipHostFree(pin2);
         pin2 = nullptr;
         pin2_cap = 0;
@@ -646,22 +686,22 @@ void copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t s)
 // packed host->device upload of the per-call plan
 // ---------------------------------------------------------------------------
 struct Pack {
-    std::vector<char> own;
-    std::vector<char>& bytes;  // storage, kept from call to call (no refill)
-    size_t used = 0;           // bytes of this plan
-    Pack() : bytes(own) {}
-    explicit Pack(std::vector<char>& storage) : bytes(storage) {}
+    PlanStore own;    // a pageable store of its own (the tables)
+    PlanStore& st;    // storage, kept from call to call (no refill)
+    size_t used = 0;  // bytes of this plan
+    Pack() : st(own) { own.pinned = false; }
+    Pack(PlanStore& storage, bool pinned) : st(storage) { st.pinned = pinned; }
     template <class T>
     size_t add(const T* p, size_t n)
     {
         const size_t off = (used + 255) & ~size_t(255);
         const size_t end = off + std::max<size_t>(n * sizeof(T), 1);
-        if (bytes.size() < end) bytes.resize(std::max(end, 2 * bytes.size()));
-        if (n) std::memcpy(bytes.data() + off, p, n * sizeof(T));
+        char* b = st.reserve(end, used);
+        if (n) std::memcpy(b + off, p, n * sizeof(T));
         used = end;
         return off;
     }
-    const char* data() const { return bytes.data(); }
+    const char* data() { return st.data(); }
     size_t size() const { return used; }
     template <class T>
     size_t add(const std::vector<T>& v)
@@ -1157,7 +1197,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     if (expansions)
         for (int b = 0; b < nb; ++b)
             if (T.lev[b] >= nlevstart) slot[b] = nslot++;
-    Pack pk(W.h_plan);
+    Pack pk(W.h_plan, !W.arena);
     size_t o_child = pk.add(T.ch), o_par = pk.add(T.par), o_cen = pk.add(T.cen);
     // colleague lists are read on the device by m2l only, for the boxes of
     // the top level (a contiguous range): only that slice travels
@@ -1168,32 +1208,53 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     size_t o_lkey = pack_leaf_keys(pk, W, T, nlmax, k32);
     size_t o_rlo = pk.add(T.rank_lo), o_rhi = pk.add(T.rank_hi), o_slot = pk.add(slot);
 
-    // work lists of the expansion steps
-    std::vector<std::vector<int>> form_l(nlevels + 1), merge_l(nlevels + 1), split_l(nlevels + 1),
-        eval_l(nlevels + 1);
-    std::vector<int> m2l_l;
+    // work lists of the expansion steps (storage kept from call to call)
+    auto lists = [&](std::vector<std::vector<int>>& v) -> std::vector<std::vector<int>>& {
+        if (v.size() < (size_t)nlevels + 1) v.resize(nlevels + 1);
+        for (auto& x : v) x.clear();
+        return v;
+    };
+    std::vector<std::vector<int>>& form_l = lists(W.h_form);
+    std::vector<std::vector<int>>& merge_l = lists(W.h_merge);
+    std::vector<std::vector<int>>& split_l = lists(W.h_split);
+    std::vector<std::vector<int>>& eval_l = lists(W.h_eval);
+    std::vector<int>& m2l_l = W.h_m2l;
+    m2l_l.clear();
     std::vector<size_t> o_form(nlevels + 1), o_merge(nlevels + 1), o_split(nlevels + 1),
         o_eval(nlevels + 1);
     size_t o_m2l = 0;
     if (expansions) {
-        std::vector<char> iflocal(nb, 0);
+        // leaves of the cutoff level with a refined colleague (ifhung)
+        std::vector<char>& iflocal = W.h_iflocal;
+        iflocal.assign(nb, 0);
         if (npwlevel >= 0 && npwlevel <= nlevels)
             for (int b = T.laddr[2 * npwlevel]; b <= T.laddr[2 * npwlevel + 1]; ++b) {
                 if (T.nch[b] != 0) continue;
                 for (int i = 0; i < T.ncol[b]; ++i)
                     if (T.nch[T.col[9 * b + i]] > 0) { iflocal[b] = 1; break; }
             }
-        for (int l = 0; l <= nlevels; ++l)
-            for (int b = T.laddr[2 * l]; b <= T.laddr[2 * l + 1]; ++b) {
-                if (T.nch[b] == 0 && l >= std::max(npwlevel + 1, 0)) form_l[l].push_back(b);
-                if (T.nch[b] > 0 && l >= nlevstart && l <= nlevels - 1) {
-                    merge_l[l].push_back(b);
-                    for (int j = 0; j < 4; ++j) split_l[l].push_back(T.ch[4 * b + j]);
+        const int* nch = T.nch.data();
+        const int* ch = T.ch.data();
+        for (int l = 0; l <= nlevels; ++l) {
+            const int b0 = T.laddr[2 * l], b1 = T.laddr[2 * l + 1];
+            const bool fl = l >= std::max(npwlevel + 1, 0), el = l >= nlevstart,
+                       ml = l >= nlevstart && l <= nlevels - 1;
+            std::vector<int>& f = form_l[l];
+            std::vector<int>& m = merge_l[l];
+            std::vector<int>& sp = split_l[l];
+            std::vector<int>& e = eval_l[l];
+            for (int b = b0; b <= b1; ++b) {
+                if (nch[b] == 0) {
+                    if (fl) f.push_back(b);
+                    if (el && !(l == npwlevel && !iflocal[b])) e.push_back(b);
+                } else if (ml) {
+                    m.push_back(b);
+                    sp.insert(sp.end(), ch + 4 * b, ch + 4 * b + 4);
                 }
-                if (l == nlevstart) m2l_l.push_back(b);
-                if (T.nch[b] == 0 && l >= nlevstart && !(l == npwlevel && !iflocal[b]))
-                    eval_l[l].push_back(b);
             }
+            if (l == nlevstart)
+                for (int b = b0; b <= b1; ++b) m2l_l.push_back(b);
+        }
         for (int l = 0; l <= nlevels; ++l) {
             o_form[l] = pk.add(form_l[l]);
             o_merge[l] = pk.add(merge_l[l]);
@@ -1286,9 +1347,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     // upload the plan in one copy
     char* dplan = W.get<char>("plan", pk.size());
     {
-        void* hp = W.pinned(pk.size());
-        std::memcpy(hp, pk.data(), pk.size());
-        HIPCHECK(hipMemcpyAsync(dplan, hp, pk.size(), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(dplan, pk.data(), pk.size(), hipMemcpyHostToDevice, s));
     }
     // the near field's lists on the helper thread, while this one queues
     // the expansion steps; the guard waits for them on every way out (the
@@ -1357,13 +1416,11 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     nf_join.h = nullptr;
     W.helper->wait();  // the lists
     tm.begin(S_DIRECT, s_nf);
-    Pack pn(W.h_plan_nf);
+    Pack pn(W.h_plan_nf, !W.arena);
     size_t o_units = pn.add(unit_ids), o_uptr = pn.add(unit_ptr), o_ulist = pn.add(unit_list);
     char* dnf = W.get<char>("plan_nf", pn.size());
     {
-        void* hp = W.pinned2(pn.size());
-        std::memcpy(hp, pn.data(), pn.size());
-        HIPCHECK(hipMemcpyAsync(dnf, hp, pn.size(), hipMemcpyHostToDevice, s_nf));
+        HIPCHECK(hipMemcpyAsync(dnf, pn.data(), pn.size(), hipMemcpyHostToDevice, s_nf));
     }
     if (!unit_ids.empty()) {
         DirectArgs da{};
@@ -2143,7 +2200,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
             }
         }
     }
-    Pack pk(W.h_plan);
+    Pack pk(W.h_plan, !W.arena);
     size_t o_child = pk.add(T.ch), o_par = pk.add(T.par), o_cen = pk.add(T.cen);
     // colleague lists are read on the device by m2l only, for the boxes of
     // the top level (a contiguous range): only that slice travels
@@ -2255,9 +2312,7 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
 
     char* dplan = W.get<char>("plan", pk.size());
     {
-        void* hp = W.pinned(pk.size());
-        std::memcpy(hp, pk.data(), pk.size());
-        HIPCHECK(hipMemcpyAsync(dplan, hp, pk.size(), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(dplan, pk.data(), pk.size(), hipMemcpyHostToDevice, s));
     }
     W.order(s_g, s);  // the sorted points, charges and dipole weights
     auto D = [&](size_t off) { return (void*)(dplan + off); };
@@ -2489,13 +2544,11 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
 
     // ---------------- step 6: near field, dmax = ln(1/eps) delta (fgt3d.f:1466) ----------------
     near_field_lists();
-    Pack pn(W.h_plan_nf);
+    Pack pn(W.h_plan_nf, !W.arena);
     size_t o_units = pn.add(unit_ids), o_uptr = pn.add(unit_ptr), o_ulist = pn.add(unit_list);
     char* dnf = W.get<char>("plan_nf", pn.size());
     {
-        void* hp = W.pinned2(pn.size());
-        std::memcpy(hp, pn.data(), pn.size());
-        HIPCHECK(hipMemcpyAsync(dnf, hp, pn.size(), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(dnf, pn.data(), pn.size(), hipMemcpyHostToDevice, s));
     }
     tm.begin(S_DIRECT);
     if (!unit_ids.empty()) {
