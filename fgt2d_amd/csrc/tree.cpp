@@ -436,7 +436,7 @@ struct Builder {
     // The permutation moves whole runs of boxes (a level's group of one
     // pass, in creation order) to contiguous places.
     struct Run {
-        int from, to, n;  // first old index, first new index, count
+        int from, to, n, l;  // first old index, first new index, count, level
     };
     template <class V>
     static void permute(V& v, int width, const std::vector<Run>& runs)
@@ -450,8 +450,29 @@ struct Builder {
             std::memcpy(v.data() + (size_t)width * r.to, old + (size_t)width * r.from,
                         sizeof(E) * width * (size_t)r.n);
     }
+    // colleague lists of the levels up to `valid`, in the new order; the
+    // deeper ones are stale (recomputed before anything reads them): cleared
+    void permute_col(const std::vector<Run>& runs, int valid)
+    {
+        thread_local std::vector<int> scratch;
+        if (scratch.size() < T.col.size()) scratch.resize(T.col.size());
+        for (const Run& r : runs)
+            if (r.l <= valid)
+                std::memcpy(scratch.data() + (size_t)NC * r.from, T.col.data() + (size_t)NC * r.from,
+                            sizeof(int) * NC * (size_t)r.n);
+        for (const Run& r : runs) {
+            int* dst = T.col.data() + (size_t)NC * r.to;
+            if (r.l <= valid) {
+                std::memcpy(dst, scratch.data() + (size_t)NC * r.from, sizeof(int) * NC * (size_t)r.n);
+            } else {
+                std::fill(dst, dst + (size_t)NC * r.n, -1);
+                std::fill(T.ncol.data() + r.to, T.ncol.data() + r.to + r.n, 0);
+            }
+        }
+    }
     // colleague lists are remapped for levels up to `valid` (the deeper ones
-    // are stale and recomputed before anything reads them)
+    // are stale and recomputed before anything reads them); the restriction
+    // flags are not kept
     void renumber(const std::vector<int>& t1, const std::vector<int>& t2, int valid)
     {
         const int n = T.nboxes();
@@ -467,7 +488,7 @@ struct Builder {
                 const int b0 = g[0], b1 = g[1];
                 if (b1 < b0) continue;
                 moved = moved || b0 != cur;
-                runs.push_back(Run{b0, cur, b1 - b0 + 1});
+                runs.push_back(Run{b0, cur, b1 - b0 + 1, l});
                 cur += b1 - b0 + 1;
             }
             T.laddr[2 * l + 1] = cur - 1;
@@ -488,19 +509,21 @@ struct Builder {
                     case 0: permute(T.lev, 1, R); break;
                     case 1: permute(T.nch, 1, R); break;
                     case 2: permute(T.cen, D, R); break;
-                    case 3: permute(T.flag, 1, R); break;
-                    case 4: permute(T.pre, 1, R); break;
-                    case 5: permute(T.par, 1, R); break;
-                    case 6: permute(T.ch, CH, R); break;
-                    case 7: permute(T.ncol, 1, R); break;
-                    case 8: permute(T.col, NC, R); break;
-                    default: permute(T.ic[k - 9], 1, R); break;
+                    case 3: permute(T.pre, 1, R); break;
+                    case 4: permute(T.par, 1, R); break;
+                    case 5: permute(T.ch, CH, R); break;
+                    case 6:  // the lists and their counts together
+                        permute(T.ncol, 1, R);
+                        permute_col(R, valid);
+                        break;
+                    default: permute(T.ic[k - 7], 1, R); break;
                 }
         };
         if (n >= 32768)
-            par_range(9 + D, 1, each);
+            par_range(7 + D, 1, each);
         else
-            each(0, 9 + D);
+            each(0, 7 + D);
+        std::fill(T.flag.begin(), T.flag.end(), 0);
         view();
         // stored indices (-1 for none: children of leaves, unused list slots)
         const int* mp = map.data() + 1;
@@ -601,6 +624,7 @@ struct Builder {
             flag[b] = 0;
             for (int i = 0; i < ncol[b] && flag[b] == 0; ++i) {
                 int jb = col[NC * b + i];
+                if (nch[jb] == 0) continue;  // no children to look at
                 for (int j = 0; j < CH; ++j) {
                     int kb = ch[CH * jb + j];
                     if (kb >= 0 && nch[kb] > 0 && near_finer(b, kb)) {
