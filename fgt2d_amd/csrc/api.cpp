@@ -296,6 +296,19 @@ struct PlanStore {
     }
 };
 
+// Waits for a helper's job on every way out of a scope whose locals the job
+// reads (h = nullptr once it has been waited for)
+struct HelperJoin {
+    HostHelper* h;
+    ~HelperJoin()
+    {
+        if (h) try {
+                h->wait();
+            } catch (...) {
+            }
+    }
+};
+
 struct Workspace {
     struct Buf {
         void* p = nullptr;
@@ -328,6 +341,13 @@ struct Workspace {
     std::vector<std::vector<int>> h_form, h_merge, h_split, h_eval;  // per-level work lists
     std::vector<int> h_m2l;
     std::vector<char> h_iflocal;
+    // the finest level formed before the tree is built (2D): its boxes' leaf
+    // records, centres and ranges, and the plan they travel in
+    std::vector<int> h_erec, h_eiota;
+    std::vector<double> h_ecen;
+    std::vector<int2> h_ese;
+    PlanStore h_plan_early;
+    int last_nslot = 0;
     Tables2D tab;
     Tables3D tab3;
     // the tree of the last transform, kept for inspection
@@ -409,6 +429,7 @@ struct Workspace {
         pin_cap = 0;
         h_plan.release();
         h_plan_nf.release();
+        h_plan_early.release();
         if (pin2) (void)hipHostFree(pin2);
         pin2 = nullptr;
         pin2_cap = 0;
@@ -1156,12 +1177,141 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     }
     tm.end(S_ASSIGN);
 
+    // ---------------- the finest level, formed beside the host tree ----------------
+    // The level restriction never refines the two deepest levels, so the
+    // finest level's boxes are those of the adaptive tree: the children of
+    // the refined boxes one level up, in the order the leaf records first
+    // reach them, with the records' point ranges.  Their centres follow the
+    // builder's recursion (tree.cpp set_children), and the slot numbering puts
+    // them first (slots 0..nL-1), so the helper thread forms them on the side
+    // stream while this thread builds the tree.  After the tree is built its
+    // boxes, centres and slot count are checked against what was used; on
+    // any difference the level is formed again with the rest.  Not in the
+    // timing modes (the per-kernel records follow the main thread) nor for
+    // the no-allocation entry point (its scratch is sized without it).
+    int L0 = 0;
+    for (const LeafRun& r : leaves) L0 = std::max(L0, r.level);
+    std::vector<double> bsz0(L0 + 1);
+    bsz0[0] = bs0;
+    for (int l = 0; l < L0; ++l) bsz0[l + 1] = bsz0[l] / 2;  // as the builder halves
+    const bool early_ok = !g_serial && !tm.on && !W.arena && npwlevel < L0 && L0 >= 1 &&
+                          L0 >= std::max(npwlevel + 1, 0);
+    struct Early {
+        bool on = false;
+        int nL = 0, nl = 0, npw = 0;
+        bool mixed = false;
+        size_t cap = 0;  // slots the expansion store holds
+    } early;
+    if (early_ok) {
+        early.on = true;
+        if (!W.helper) W.helper.reset(new HostHelper);
+        auto job = [&]() {
+            const int L = L0;
+            // leaf record of every finest-level box (4 per refined parent), or -1
+            std::vector<int>& rec = W.h_erec;
+            rec.clear();
+            unsigned long long lastp = ~0ull;
+            for (int r = 0; r < nleaf0; ++r) {
+                if (leaves[r].level != L) continue;
+                const unsigned long long par = leaves[r].key >> 2;
+                if (par != lastp) {
+                    rec.insert(rec.end(), 4, -1);
+                    lastp = par;
+                }
+                rec[rec.size() - 4 + (size_t)(leaves[r].key & 3)] = r;
+            }
+            const int nL = (int)rec.size() / 4 * 4;
+            early.nL = nL;
+            std::vector<double>& ec = W.h_ecen;
+            std::vector<int2>& se = W.h_ese;
+            std::vector<int>& iota = W.h_eiota;
+            ec.resize(2 * (size_t)nL);
+            se.resize(nL);
+            iota.resize(nL);
+            for (int k = 0; k < nL; k += 4) {
+                int r0 = -1;  // any record of the family gives the parent's prefix
+                for (int j = 0; j < 4 && r0 < 0; ++j) r0 = rec[k + j];
+                const unsigned long long par = leaves[r0].key >> 2;
+                for (int j = 0; j < 4; ++j) {
+                    const unsigned long long key = (par << 2) | (unsigned long long)j;
+                    double c[2] = {cen0[0], cen0[1]};
+                    for (int l = 0; l < L; ++l) {
+                        const int dg = (int)((key >> (2 * (L - 1 - l))) & 3);
+                        const double bs = bsz0[l + 1];
+                        for (int a = 0; a < 2; ++a) c[a] = c[a] + (((dg >> a) & 1) ? bs / 2 : -bs / 2);
+                    }
+                    ec[2 * (size_t)(k + j)] = c[0];
+                    ec[2 * (size_t)(k + j) + 1] = c[1];
+                    const int r = rec[k + j];
+                    se[k + j] = r < 0 ? int2{0, 0}
+                                      : int2{leaves[r].start, r + 1 < nleaf0 ? leaves[r + 1].start : ns};
+                    iota[k + j] = k + j;
+                }
+            }
+            // terms, nodes and tables as the transform computes them after the tree
+            const int npg0 = std::max(q.ifpgh, q.ifpghtarg), nadd0 = ifdipole + npg0;
+            int ntm = 0, nlL = 0;
+            for (int i = std::max(npwlevel, 0); i <= L; ++i) {
+                int nl = hermite_terms(2, bsz0[i], q.delta, q.eps);
+                if (ifdipole || q.ifpgh >= 2 || q.ifpghtarg >= 2) nl += nadd0;
+                ntm = std::max(ntm, nl);
+                if (i == L) nlL = nl;
+            }
+            double pmax0 = 0;
+            int npw0 = 0;
+            planewave_terms(2 * bs0 / std::pow(2.0, std::max(npwlevel, 0)), q.delta,
+                            nadd0 > 2 ? q.eps / 100 : q.eps, &pmax0, &npw0);
+            const bool mixed0 = g_precision == 1 && q.eps >= kMixedEpsMin && ntm < 63;
+            const size_t esz0 = mixed0 ? sizeof(float) : sizeof(double);
+            early.nl = nlL;
+            early.npw = npw0;
+            early.mixed = mixed0;
+            const int nlevstart0 = std::max(npwlevel, 0);
+            const Tables2D& tab0 =
+                tables2d(W, npw0, pmax0, ntm, L - nlevstart0, bsz0[L] / std::sqrt(q.delta) / 2,
+                         (npwlevel >= 0 ? bsz0[npwlevel] : bsz0[0]) / std::sqrt(q.delta), s_g);
+            // the expansion store, sized for the slots of this call as the
+            // last call's tree suggests (checked after the tree)
+            const size_t per0 = 2 * (size_t)(npw0 / 2) * npw0 * nd;
+            early.cap = std::max<size_t>(4 * (size_t)nL, (size_t)W.last_nslot + W.last_nslot / 4) + 1024;
+            W.get<char>("mp", per0 * early.cap * esz0);
+            Pack pe(W.h_plan_early, true);
+            const size_t o_b = pe.add(iota), o_c = pe.add(ec), o_s = pe.add(se);
+            char* d = W.get<char>("plan_early", pe.size());
+            HIPCHECK(hipMemcpyAsync(d, pe.data(), pe.size(), hipMemcpyHostToDevice, s_g));
+            const Tuning& tune = tuning();
+            const int m = nlL + 1, seg = tune.segf[std::min(4, (nlL + 16) / 16) - 1];
+            int* ts = W.get<int>("tstart_early", (size_t)nL + 1);
+            int* sb = W.get<int>("segbox_early", (size_t)nL + ns / seg + 1);
+            double* part = W.get<double>("form_partial_early", ((size_t)nL + ns / seg + 1) * m * m);
+            SegLists sl{};
+            sl.l[0] = SegList{(const int*)(d + o_b), nL, (const int2*)(d + o_s), seg, ts, sb};
+            sl.count = 1;
+            launch_seg_scan(sl, s_g);
+            for (int ind = 0; ind < nd; ++ind) {
+                FormArgs fa{(const int*)(d + o_b), nL, (const double*)(d + o_c), (const int2*)(d + o_s),
+                            (const int*)(d + o_b), xs, qs, nd, ind, 1.0 / std::sqrt(q.delta), nlL,
+                            tab0.h2x, tab0.h2xr, tab0.rsq, npw0,
+                            (double*)W.get<char>("mp", per0 * early.cap * esz0), ts, seg,
+                            (int)std::min<long>(4096, nL + ns / seg + 1), part, m * m, sb, wsd};
+                fa.prec = mixed0 ? 1 : 0;
+                launch_form(fa, s_g);
+            }
+        };
+        W.helper->post(job);
+    }
+    HelperJoin early_join{early.on ? W.helper.get() : nullptr};
+
     // ---------------- host: tree, lists, tables ----------------
     auto th0 = std::chrono::steady_clock::now();
     W.last_valid = false;
     Tree2D& T = W.last;
     build_tree2d(T, leaves, nlmax, npwlevel, bs0, cen0);
     tm.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
+    if (early.on) {
+        early_join.h = nullptr;
+        W.helper->wait();  // the early formation is queued (rethrows its errors)
+    }
     W.last_ns = ns;
     W.last_nt = nt;
     W.last_dim = 2;
@@ -1190,13 +1340,26 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     const bool mixed = g_precision == 1 && expansions && q.eps >= kMixedEpsMin && ntermmax < 63;
     const size_t esz = mixed ? sizeof(float) : sizeof(double);
 
-    // box arrays
+    // box arrays; expansion slots: the finest level first (its leaves are
+    // formed first, possibly before the tree), then the other levels from
+    // the cutoff level down
     std::vector<int>& slot = W.h_slot;
     slot.assign(nb, -1);
     int nslot = 0;
-    if (expansions)
-        for (int b = 0; b < nb; ++b)
+    const int bL0 = T.laddr[2 * nlevels], bL1 = T.laddr[2 * nlevels + 1], nL = bL1 - bL0 + 1;
+    if (expansions) {
+        for (int b = bL0; b <= bL1; ++b) slot[b] = nslot++;
+        for (int b = 0; b < bL0; ++b)
             if (T.lev[b] >= nlevstart) slot[b] = nslot++;
+    }
+    // the early formation holds if it formed these boxes with these centres
+    // and parameters and its store has room for every slot
+    if (early.on)
+        early.on = expansions && nlevels == L0 && nL == early.nL && (size_t)nslot <= early.cap &&
+                   nlocal[nlevels] == early.nl && npw == early.npw && mixed == early.mixed &&
+                   std::memcmp(T.cen.data() + 2 * (size_t)bL0, W.h_ecen.data(),
+                               sizeof(double) * 2 * (size_t)nL) == 0;
+    W.last_nslot = nslot;
     Pack pk(W.h_plan, !W.arena);
     size_t o_child = pk.add(T.ch), o_par = pk.add(T.par), o_cen = pk.add(T.cen);
     // colleague lists are read on the device by m2l only, for the boxes of
@@ -1354,16 +1517,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     // job reads this call's locals)
     if (!W.helper) W.helper.reset(new HostHelper);
     W.helper->post(near_field_lists);
-    struct HelperJoin {
-        HostHelper* h;
-        ~HelperJoin()
-        {
-            if (h) try {
-                    h->wait();
-                } catch (...) {
-                }
-        }
-    } nf_join{W.helper.get()};
+    HelperJoin nf_join{W.helper.get()};
 
     // the expansion stores are accumulated into: cleared on the side stream
     // while the point ranges are found
@@ -1371,7 +1525,10 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     double* mp = expansions ? (double*)W.get<char>("mp", per * nslot * esz) : nullptr;
     double* lc = expansions ? (double*)W.get<char>("loc", per * nslot * esz) : nullptr;
     const size_t nexp_d = per * nslot * esz / sizeof(double);  // per is even
-    if (expansions) launch_zero3(nullptr, 0, mp, nexp_d, lc, nexp_d, s_g);
+    // the finest level's leaves store their expansions (no clearing); the
+    // other boxes accumulate (merges)
+    const size_t nfin_d = expansions ? per * nL * esz / sizeof(double) : 0;
+    if (expansions) launch_zero3(nullptr, 0, mp + nfin_d, nexp_d - nfin_d, lc, nexp_d, s_g);
     auto D = [&](size_t off) { return (void*)(dplan + off); };
     const int4* d_child = (const int4*)D(o_child);
     const int* d_par = (const int*)D(o_par);
@@ -1513,7 +1670,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                 L.l[L.count++] = SegList{(const int*)D(off), (int)v.size(), se, seg, t, *sb};
                 return t;
             };
-            for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l)
+            for (int l = nlevels - (early.on ? 1 : 0); l >= std::max(npwlevel + 1, 0); --l)
                 if (!form_l[l].empty())
                     ts_form[l] = add(form_l[l], o_form[l], src_se, segf(l), ns, &sb_form[l]);
             for (int l = nlevstart; l <= nlevels; ++l) {
@@ -1535,8 +1692,10 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         double* partial = W.get<double>("form_partial", o_part[nlevels + 1]);
         tm.begin(S_FORM);
         W.order(s, s_lv);
+        // the finest level on s (unless formed already), the others on s_lv
+        const int lfirst = nlevels - (early.on ? 1 : 0);
         for (int ind = 0; ind < nd; ++ind) {
-            for (int l = nlevels; l >= std::max(npwlevel + 1, 0); --l) {
+            for (int l = lfirst; l >= std::max(npwlevel + 1, 0); --l) {
                 if (form_l[l].empty()) continue;
                 const int nb_l = (int)form_l[l].size(), m = nlocal[l] + 1;
                 FormArgs fa{(const int*)D(o_form[l]), nb_l, d_cen, src_se, d_slot, xs, qs, nd, ind,
@@ -1545,7 +1704,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                             partial + o_part[l], m * m, sb_form[l], wsd};
                 fa.prec = mixed ? 1 : 0;
                 tm.ctx(l, &form_l[l], 0);
-                launch_form(fa, l == nlevels ? s : s_lv);
+                launch_form(fa, l == lfirst ? s : s_lv);
             }
         }
         W.order(s_lv, s);
