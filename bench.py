@@ -86,7 +86,9 @@ def per_unit(kernel, npw, nd, p, level=-1, mixed=False, keybits=18):
             return "hbm", 2 * 16.0 + 4, "bytes per point"
         return "hbm", 2 * 8.0 * (2 + nd) + 4, "bytes per point"
     if kernel == "scatter":
-        return "hbm", 2 * 8.0 * nd + 4, "bytes per point"
+        # the evaluations' and the near field's sorted outputs read, their
+        # sum written in input order, the permutation index
+        return "hbm", 3 * 8.0 * nd + 4, "bytes per point"
     if kernel == "sort":
         # per pass: key and index read and written (16 B); the digit
         # histograms of all passes: one read of the keys (4 B)
@@ -139,8 +141,8 @@ def rocprof_name(kernel, nl, ifpgh, mixed=False, npw=0, sort=4):
     if kernel == "direct":
         return "k_direct<%d, false>" % ifpgh
     if kernel == "sort":
-        if sort == 4:
-            return "k_rs_count + k_rs_offsets + k_rs_scatter (LSD radix passes)"
+        if sort in (4, 5):
+            return "k_rs_count+k_rs_offsets+k_rs_scatter"
         return "rocprim radix sort (onesweep passes, histogram, fills)"
     return "k_" + kernel
 
