@@ -1046,19 +1046,22 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     u64* ks = W.get<u64>("keys_s", ns);
     u64* ks2 = W.get<u64>("keys_s2", ns);
     unsigned* ix = W.get<unsigned>("idx_s", nmx);
+    // the library's LSD sort numbers the points itself in its first pass:
+    // no index array is written and read
     unsigned* isrc = W.get<unsigned>("isrc", ns);
     size_t tmpb = std::max(sort_pairs_u64_temp(nmx), sort_pairs_u32_temp(nmx));
     tmpb = std::max(tmpb, leaf_records_temp(ns));
     void* tmp = W.get<char>("sort_tmp", tmpb);
     const int keybits = std::max(2, 2 * nlmax);
     const bool k32 = keybits <= 31;  // keys fit 32 bits
+    unsigned* ix32 = keybits <= 30 && (tuning().sort == 4 || tuning().sort == 5) ? nullptr : ix;
     if (k32)
-        launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, (unsigned*)ks, ix, s);
+        launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, (unsigned*)ks, ix32, s);
     else
         launch_keys2d(q.src, ns, cen0[0], cen0[1], bs0, nlmax, ks, ix, s);
     tm.bracket(KID_SORT, false, -1, ns);
     if (k32)
-        sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix, isrc, ns, keybits, s, tuning().sort);
+        sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix32, isrc, ns, keybits, s, tuning().sort);
     else
         sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
     tm.bracket(KID_SORT, true, -1, ns);
@@ -1163,12 +1166,12 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         u64* kt = W.get<u64>("keys_t", nt);
         kt2 = W.get<u64>("keys_t2", nt);
         if (k32)
-            launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, (unsigned*)kt, ix, s);
+            launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, (unsigned*)kt, ix32, s);
         else
             launch_keys2d(q.targ, nt, cen0[0], cen0[1], bs0, nlmax, kt, ix, s);
         tm.bracket(KID_SORT, false, -2, nt);
         if (k32)
-            sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s, tuning().sort);
+            sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix32, itarg, nt, keybits, s, tuning().sort);
         else
             sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
         tm.bracket(KID_SORT, true, -2, nt);
@@ -2068,18 +2071,21 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
     u64* ks = W.get<u64>("keys_s", ns);
     u64* ks2 = W.get<u64>("keys_s2", ns);
     unsigned* ix = W.get<unsigned>("idx_s", nmx);
+    // the library's LSD sort numbers the points itself in its first pass:
+    // no index array is written and read
     unsigned* isrc = W.get<unsigned>("isrc", ns);
     size_t tmpb = std::max({sort_pairs_u64_temp(nmx), sort_pairs_u32_temp(nmx), leaf_records_temp(ns)});
     void* tmp = W.get<char>("sort_tmp", tmpb);
     const int keybits = std::max(3, 3 * nlmax);
     const bool k32 = keybits <= 31;  // keys fit 32 bits
+    unsigned* ix32 = keybits <= 30 && (tuning().sort == 4 || tuning().sort == 5) ? nullptr : ix;
     if (k32)
-        launch_keys3d(q.src, ns, cen0, bs0, nlmax, (unsigned*)ks, ix, s);
+        launch_keys3d(q.src, ns, cen0, bs0, nlmax, (unsigned*)ks, ix32, s);
     else
         launch_keys3d(q.src, ns, cen0, bs0, nlmax, ks, ix, s);
     tm.bracket(KID_SORT, false, -1, ns);
     if (k32)
-        sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix, isrc, ns, keybits, s, tuning().sort);
+        sort_pairs_u32(tmp, tmpb, (unsigned*)ks, (unsigned*)ks2, ix32, isrc, ns, keybits, s, tuning().sort);
     else
         sort_pairs_u64(tmp, tmpb, ks, ks2, ix, isrc, ns, keybits, s);
     tm.bracket(KID_SORT, true, -1, ns);
@@ -2146,12 +2152,12 @@ void run3d(Workspace& W, const Call& q, hipStream_t s)
         u64* kt = W.get<u64>("keys_t", nt);
         kt2 = W.get<u64>("keys_t2", nt);
         if (k32)
-            launch_keys3d(q.targ, nt, cen0, bs0, nlmax, (unsigned*)kt, ix, s);
+            launch_keys3d(q.targ, nt, cen0, bs0, nlmax, (unsigned*)kt, ix32, s);
         else
             launch_keys3d(q.targ, nt, cen0, bs0, nlmax, kt, ix, s);
         tm.bracket(KID_SORT, false, -2, nt);
         if (k32)
-            sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix, itarg, nt, keybits, s, tuning().sort);
+            sort_pairs_u32(tmp, tmpb, (unsigned*)kt, (unsigned*)kt2, ix32, itarg, nt, keybits, s, tuning().sort);
         else
             sort_pairs_u64(tmp, tmpb, kt, kt2, ix, itarg, nt, keybits, s);
         tm.bracket(KID_SORT, true, -2, nt);
