@@ -157,3 +157,31 @@ def test_invalid_arguments_are_rejected():
     assert rc == fgt2d_amd.FGTPW_EINVAL
     # the library still works after errors
     fgt2d_amd.fgt2d(1e-3, 1e-6, src, chg, None, ifpgh=1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_finest_level_formed_beside_the_tree_is_bit_identical(precision):
+    """The finest level's leaves are formed while the host builds the tree
+    (boxes, ranges and centres from the leaf records); with timing on they
+    are formed after the tree, with the other levels.  Both must give the
+    same bits, in FP64 and on the mixed route, also through a call whose
+    tree is deeper than the last one's (the expansion store sized from the
+    last call is then too small and the level is formed again)."""
+    prev = fgt2d_amd.set_precision(precision)
+    try:
+        rng = np.random.default_rng(21)
+        cases = [(_pts.mixture(rng, 60000), _pts.mixture(rng, 20000), 1e-5, 1e-6),
+                 (_pts.uniform(rng, 30000), _pts.uniform(rng, 10000), 1e-4, 1e-6),
+                 (_pts.mixture(rng, 80000), _pts.mixture(rng, 20000), 3e-6, 1e-6)]
+        for src, tg, delta, eps in cases:
+            print("case", len(src), len(tg), delta, eps, precision, flush=True)
+            chg = rng.normal(size=(len(src), 1))
+            a = fgt2d_amd.fgt2d(delta, eps, src, chg, tg, ifpgh=1, ifpghtarg=1)
+            fgt2d_amd.set_timing(True)
+            try:
+                b = fgt2d_amd.fgt2d(delta, eps, src, chg, tg, ifpgh=1, ifpghtarg=1)
+            finally:
+                fgt2d_amd.set_timing(False)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[3], b[3])
+    finally:
+        fgt2d_amd.set_precision(prev)
