@@ -1694,9 +1694,12 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         }
         double* partial = W.get<double>("form_partial", o_part[nlevels + 1]);
         tm.begin(S_FORM);
-        W.order(s, s_lv);
-        // the finest level on s (unless formed already), the others on s_lv
+        // the finest level on s (unless formed already), the others on s_lv;
+        // the orders only when s_lv has a level to form (each costs host time)
         const int lfirst = nlevels - (early.on ? 1 : 0);
+        bool coarse = false;
+        for (int l = lfirst - 1; l >= std::max(npwlevel + 1, 0); --l) coarse = coarse || !form_l[l].empty();
+        if (coarse) W.order(s, s_lv);
         for (int ind = 0; ind < nd; ++ind) {
             for (int l = lfirst; l >= std::max(npwlevel + 1, 0); --l) {
                 if (form_l[l].empty()) continue;
@@ -1710,7 +1713,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                 launch_form(fa, l == lfirst ? s : s_lv);
             }
         }
-        W.order(s_lv, s);
+        if (coarse) W.order(s_lv, s);
         tm.end(S_FORM);
         tm.begin(S_MERGE);
         for (int l = nlevels - 1; l >= nlevstart; --l)

@@ -460,8 +460,11 @@ __device__ __forceinline__ unsigned rs_lds_scan(unsigned* a, int n, unsigned* wt
     return total;
 }
 
-// the tile's digit histogram, cnt[digit][tile]
-template <int R>
+// the tile's digit histogram, cnt[digit][tile]; AGG: one LDS atomic per
+// distinct digit of a wavefront (peers found by ballots), for passes whose
+// digits repeat within a tile (high digits of clustered points); else one
+// per key (low digits, nearly all distinct within a wavefront)
+template <int R, bool AGG>
 __global__ __launch_bounds__(256) void k_rs_count(const unsigned* __restrict__ key, int n, int shift,
                                                   int bits, unsigned* __restrict__ cnt, int ntile)
 {
@@ -470,9 +473,6 @@ __global__ __launch_bounds__(256) void k_rs_count(const unsigned* __restrict__ k
     const int nd = 1 << bits;
     for (int d = threadIdx.x; d < nd; d += 256) rs_h[d] = 0;
     __syncthreads();
-    // one LDS atomic per distinct digit of a wavefront's 64 items (the
-    // lanes with the same digit found by ballots): tiles of clustered
-    // points hold few distinct high digits
     const long base = (long)blockIdx.x * RS_TILE;
     const int lane = threadIdx.x & 63;
     unsigned kk[RS_TILE / 256];  // all loads in flight at once
@@ -486,6 +486,10 @@ __global__ __launch_bounds__(256) void k_rs_count(const unsigned* __restrict__ k
         const long i = base + threadIdx.x + 256 * r;
         const bool valid = i < n;
         const int d = valid ? (int)((kk[r] >> shift) & (unsigned)(nd - 1)) : 0;
+        if (!AGG) {
+            if (valid) atomicAdd(&rs_h[d], 1u);
+            continue;
+        }
         unsigned long long peers = __ballot(valid);
         for (int bit = 0; bit < bits; ++bit) {
             const unsigned long long m = __ballot((d >> bit) & 1);
@@ -621,7 +625,10 @@ static void rs_pass(const unsigned* ki, const unsigned* vi, unsigned* ko, unsign
                     int shift, int b, unsigned* cnt, unsigned* tot, hipStream_t s)
 {
     const int ndb = 1 << b, ntile = (n + 256 * R - 1) / (256 * R);
-    k_rs_count<R><<<ntile, 256, (size_t)ndb * 4, s>>>(ki, n, shift, b, cnt, ntile);
+    if (shift == 0)  // the lowest digit
+        k_rs_count<R, false><<<ntile, 256, (size_t)ndb * 4, s>>>(ki, n, shift, b, cnt, ntile);
+    else
+        k_rs_count<R, true><<<ntile, 256, (size_t)ndb * 4, s>>>(ki, n, shift, b, cnt, ntile);
     k_rs_offsets<<<ndb, 256, 0, s>>>(cnt, ntile, tot);
     const size_t lds = ((size_t)6 * ndb + 2 * 256 * R) * 4;  // at most 64 KiB
     if (lds >= 65536) {
