@@ -106,3 +106,23 @@ def test_submit_errors_are_reported_and_the_server_goes_on():
         (pot, *_), _ = srv.wait(t)
     same((pot,), (alone(dict(dim=2, delta=1e-3, eps=1e-6, sources=src, charge=chg, targ=None,
                                   ifpgh=1, ifpghtarg=0))[0],))
+
+
+def test_served_large_requests_match_each_call_alone():
+    """Requests up to 10^6 points (the serving stream's range) among small
+    ones, 2D and 3D, each served result bit-identical to the same call made
+    alone; several run at once."""
+    rng = np.random.default_rng(31)
+    spec = [(2, 1_000_000, 1e-5, 1e-6), (3, 400_000, 1e-4, 1e-6), (2, 3_000, 1e-3, 1e-9),
+            (2, 250_000, 3e-6, 1e-3), (3, 5_000, 1e-3, 1e-3), (2, 600_000, 1e-4, 1e-9)]
+    reqs = []
+    for dim, n, delta, eps in spec:
+        reqs.append(dict(dim=dim, delta=delta, eps=eps, sources=rng.random((n, dim)),
+                         charge=rng.normal(size=(n, 1)), targ=rng.random((n // 4 + 1, dim)),
+                         ifpgh=1, ifpghtarg=1))
+    ref = [alone(r) for r in reqs]
+    got, stats = served(reqs, slots=4, budget_gib=0.0)
+    for k, (a, b) in enumerate(zip(ref, got)):
+        print("request", k, spec[k], flush=True)
+        same(a, b)
+    assert stats["peak_running"] >= 2
