@@ -68,8 +68,8 @@ def per_unit(kernel, npw, nd, p, level=-1, mixed=False, keybits=18):
     (16 B read, 32 B read-modify-write), 24 with FP32 expansions; the
     source gather reads and writes a point (16 B each way), its densities
     (8 nd each way) and reads its index (4 B); the target gather (level -2)
-    the point and index only; the scatter reads, writes and indexes the
-    outputs."""
+    the point and index only; the scatter (the points that no evaluation
+    covers) reads, writes and indexes their outputs."""
     half = npw // 2
     mf = "mfma32" if mixed else "mfma"
     if kernel in ("form_moments", "form_valu"):
@@ -86,9 +86,9 @@ def per_unit(kernel, npw, nd, p, level=-1, mixed=False, keybits=18):
             return "hbm", 2 * 16.0 + 4, "bytes per point"
         return "hbm", 2 * 8.0 * (2 + nd) + 4, "bytes per point"
     if kernel == "scatter":
-        # the evaluations' and the near field's sorted outputs read, their
-        # sum written in input order, the permutation index
-        return "hbm", 3 * 8.0 * nd + 4, "bytes per point"
+        # points no evaluation covers: the near field's sorted output read,
+        # written in input order, the permutation index
+        return "hbm", 2 * 8.0 * nd + 4, "bytes per point"
     if kernel == "sort":
         # per pass: key and index read and written (16 B); the digit
         # histograms of all passes: one read of the keys (4 B)
@@ -140,6 +140,10 @@ def rocprof_name(kernel, nl, ifpgh, mixed=False, npw=0, sort=4):
         return "k_gather<true>"
     if kernel == "direct":
         return "k_direct<%d, false>" % ifpgh
+    if kernel == "scatter":
+        # the uncovered leaves' points (k_scatter when a transform has no
+        # expansions at all)
+        return "k_scatter_boxes"
     if kernel == "sort":
         if sort in (4, 5):
             return "k_rs_count+k_rs_offsets+k_rs_scatter"

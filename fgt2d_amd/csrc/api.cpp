@@ -220,6 +220,12 @@ public:
         }
         cv_.notify_all();
     }
+    // the posted job has ended (wait() then returns at once)
+    bool done()
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        return !pending_;
+    }
     void wait()
     {
         std::unique_lock<std::mutex> lk(m_);
@@ -340,6 +346,7 @@ struct Workspace {
     std::unique_ptr<HostHelper> helper;  // host work beside a call's launches
     std::vector<std::vector<int>> h_form, h_merge, h_split, h_eval;  // per-level work lists
     std::vector<int> h_m2l;
+    std::vector<int> h_resid;  // leaves that no evaluation covers
     std::vector<char> h_iflocal;
     // the finest level formed before the tree is built (2D): its boxes' leaf
     // records, centres and ranges, and the plan they travel in
@@ -403,6 +410,28 @@ struct Workspace {
     {
         auto it = dev.find(name);
         return it == dev.end() ? 0 : it->second.cap;
+    }
+    // an event after the work queued on `from` so far (a record costs the
+    // host about 1 us; a wait on it, later, about 10 us)
+    struct Mark {
+        hipEvent_t e = nullptr;
+        hipStream_t from = nullptr;
+    };
+    Mark mark(hipStream_t from)
+    {
+        if (n_order == order_ev.size()) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            order_ev.push_back(e);
+        }
+        Mark m{order_ev[n_order++], from};
+        HIPCHECK(hipEventRecord(m.e, from));
+        return m;
+    }
+    // work queued on `to` after this call waits for the marked work
+    void wait(hipStream_t to, const Mark& m)
+    {
+        if (m.e && m.from != to) HIPCHECK(hipStreamWaitEvent(to, m.e, 0));
     }
     // work queued on `to` after this call waits for the work queued on
     // `from` before it
@@ -1120,33 +1149,27 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         wsd = W.get<double>("dip_w", 2 * (size_t)nd * ns);
         launch_gather_dip(q.rn, q.ds, nd, isrc, ns, wsd, s_g);
     }
-    // the sorted outputs, accumulated into by the near field and the
-    // evaluations, cleared here too
+    // the near field's sorted outputs, accumulated into by the near field
+    // and cleared here; the evaluations add them to their own values and
+    // write the sums to the caller's arrays in input order, and the points
+    // that no evaluation covers get them alone (the sorted order's full
+    // outputs are never formed)
     const bool use_src = q.ifpgh > 0, use_trg = q.ifpghtarg > 0 && nt > 0;
     const int ws_ = use_src ? ns : 0, wt_ = use_trg ? nt : 0;
-    double *ps = nullptr, *gs = nullptr, *hs = nullptr, *pt = nullptr, *gt = nullptr, *ht = nullptr;
     auto cleared = [&](const char* name, int k, int ifp, int n) {
         if (ifp < k) return (double*)nullptr;
         double* p = W.get<double>(name, (size_t)k * nd * n);
         launch_zero(p, (size_t)k * nd * n, s_g);
         return p;
     };
-    // the near field has outputs of its own, added in the final scatter:
-    // nothing orders it against the evaluations
     double *nps = nullptr, *ngs = nullptr, *nhs = nullptr, *npt = nullptr, *ngt = nullptr,
            *nht = nullptr;
     if (use_src) {
-        ps = cleared("pot_s", 1, q.ifpgh, ws_);
-        gs = cleared("grad_s", 2, q.ifpgh, ws_);
-        hs = cleared("hess_s", 3, q.ifpgh, ws_);
         nps = cleared("nf_pot_s", 1, q.ifpgh, ws_);
         ngs = cleared("nf_grad_s", 2, q.ifpgh, ws_);
         nhs = cleared("nf_hess_s", 3, q.ifpgh, ws_);
     }
     if (use_trg) {
-        pt = cleared("pot_t", 1, q.ifpghtarg, wt_);
-        gt = cleared("grad_t", 2, q.ifpghtarg, wt_);
-        ht = cleared("hess_t", 3, q.ifpghtarg, wt_);
         npt = cleared("nf_pot_t", 1, q.ifpghtarg, wt_);
         ngt = cleared("nf_grad_t", 2, q.ifpghtarg, wt_);
         nht = cleared("nf_hess_t", 3, q.ifpghtarg, wt_);
@@ -1386,6 +1409,8 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     std::vector<std::vector<int>>& eval_l = lists(W.h_eval);
     std::vector<int>& m2l_l = W.h_m2l;
     m2l_l.clear();
+    std::vector<int>& resid = W.h_resid;
+    resid.clear();
     std::vector<size_t> o_form(nlevels + 1), o_merge(nlevels + 1), o_split(nlevels + 1),
         o_eval(nlevels + 1);
     size_t o_m2l = 0;
@@ -1412,7 +1437,10 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
             for (int b = b0; b <= b1; ++b) {
                 if (nch[b] == 0) {
                     if (fl) f.push_back(b);
-                    if (el && !(l == npwlevel && !iflocal[b])) e.push_back(b);
+                    if (el && !(l == npwlevel && !iflocal[b]))
+                        e.push_back(b);
+                    else
+                        resid.push_back(b);
                 } else if (ml) {
                     m.push_back(b);
                     sp.insert(sp.end(), ch + 4 * b, ch + 4 * b + 4);
@@ -1429,6 +1457,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         }
         o_m2l = pk.add(m2l_l);
     }
+    const size_t o_resid = pk.add(resid);
     const Tables2D& tab = expansions ? tables2d(W, npw, pmax, ntermmax, nlevels - nlevstart,
                                                 T.bsz[nlevels] / std::sqrt(q.delta) / 2,
                                                 (npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0]) /
@@ -1572,7 +1601,10 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
     // its lists are built on the host once every expansion step is queued
     hipStream_t s_nf = g_serial ? s : W.aux[1], s_lv = g_serial ? s : W.aux[0];
     W.order(s, s_nf);  // after the ranges, the sorted points and the cleared outputs
+    Workspace::Mark nf_ready;
+    bool nf_queued = false;
     auto near_field = [&]() {
+    nf_queued = true;
     nf_join.h = nullptr;
     W.helper->wait();  // the lists
     tm.begin(S_DIRECT, s_nf);
@@ -1616,6 +1648,42 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         }
     }
     tm.end(S_DIRECT, s_nf);
+    // the points that no evaluation writes: the near field alone, in input
+    // order (with no expansions, every point)
+    tm.begin(S_SCATTER, s_nf);
+    auto residual = [&](const unsigned* perm, int n, const int2* se, double* v, const double* nf,
+                        int w) {
+        if (!v) return;
+        if (!expansions)
+            launch_scatter(nullptr, perm, n, w, v, s_nf, nf);
+        else
+            launch_scatter_boxes((const int*)D(o_resid), (int)resid.size(), se, nf, perm, w, v,
+                                 s_nf);
+    };
+    if (use_src) {
+        if (expansions)
+            tm.ctx(-1, &resid, 0);  // units: the boxes' points, counted after the call
+        else
+            tm.ctx(-1, nullptr, -1, ns);
+        residual(isrc, ns, src_se, q.pot, nps, nd);
+        if (q.ifpgh >= 2) residual(isrc, ns, src_se, q.grad, ngs, 2 * nd);
+        if (q.ifpgh >= 3) residual(isrc, ns, src_se, q.hess, nhs, 3 * nd);
+    }
+    if (use_trg) {
+        if (expansions)
+            tm.ctx(-1, &resid, 1);
+        else
+            tm.ctx(-1, nullptr, -1, nt);
+        residual(itarg, nt, trg_se, q.pottarg, npt, nd);
+        if (q.ifpghtarg >= 2) residual(itarg, nt, trg_se, q.gradtarg, ngt, 2 * nd);
+        if (q.ifpghtarg >= 3) residual(itarg, nt, trg_se, q.hesstarg, nht, 3 * nd);
+    }
+    tm.end(S_SCATTER, s_nf);
+    nf_ready = W.mark(s_nf);
+    };
+    // queued as soon as the helper has its lists (checked between steps)
+    auto near_field_if_ready = [&]() {
+        if (!nf_queued && W.helper->done()) near_field();
     };
 
     // ---------------- expansions: steps 1-5 ----------------
@@ -1715,6 +1783,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
         }
         if (coarse) W.order(s_lv, s);
         tm.end(S_FORM);
+        near_field_if_ready();
         tm.begin(S_MERGE);
         for (int l = nlevels - 1; l >= nlevstart; --l)
             tm.ctx(l, &merge_l[l], 2), launch_merge((const int*)D(o_merge[l]), (int)merge_l[l].size(), d_child, d_cen, src_se,
@@ -1726,8 +1795,7 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                    use_src, use_trg, d_slot, d_wm2l, npwlevel >= 0 ? T.bsz[npwlevel] : T.bsz[0],
                    nlevstart == npwlevel, nd, nexp, mp, lc, s, mixed);
         tm.end(S_M2L);
-        // level l is evaluated (on s_lv, the finest level on s) once the
-        // split that completes its local expansions is queued
+        near_field_if_ready();
         auto eval_level = [&](int l, hipStream_t st) {
             for (int ind = 0; ind < nd; ++ind) {
                 if (eval_l[l].empty()) continue;
@@ -1765,9 +1833,13 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                     eb.se = side == 0 ? trg_se : src_se;
                     eb.pts = side == 0 ? xt : xs;
                     eb.ifpgh = side == 0 ? q.ifpghtarg : q.ifpgh;
-                    eb.pot = side == 0 ? pt : ps;
-                    eb.grad = side == 0 ? gt : gs;
-                    eb.hess = side == 0 ? ht : hs;
+                    eb.pot = side == 0 ? q.pottarg : q.pot;
+                    eb.grad = side == 0 ? q.gradtarg : q.grad;
+                    eb.hess = side == 0 ? q.hesstarg : q.hess;
+                    eb.perm = side == 0 ? itarg : isrc;
+                    eb.npot = side == 0 ? npt : nps;
+                    eb.ngrad = side == 0 ? ngt : ngs;
+                    eb.nhess = side == 0 ? nht : nhs;
                     tm.ctx(l, &eval_l[l], side == 0 ? 1 : 0);
                     if (pwroute)
                         launch_pw_eval(eb, st);
@@ -1776,41 +1848,36 @@ void run2d(Workspace& W, const Call& q, hipStream_t s)
                 }
             }
         };
+        // Every evaluation adds the near field's outputs, so it waits for
+        // them.  The splits and the finest level's evaluation on s; each
+        // coarse level's evaluation on s_nf after the near field, as soon as
+        // the split before it has completed its local expansions (an event
+        // on s), so that it overlaps the splits and the finest level.
+        // Levels write disjoint points.
+        if (!nf_queued) near_field();
         tm.begin(S_SPLIT);
         for (int l = nlevstart; l <= nlevels - 1; ++l) {
-            W.order(s, s_lv);
-            eval_level(l, s_lv);
+            if (!eval_l[l].empty()) {
+                W.wait(s_nf, W.mark(s));
+                eval_level(l, s_nf);
+            }
             tm.ctx(l, &split_l[l], 2);
             launch_split((const int*)D(o_split[l]), (int)split_l[l].size(), d_par, d_cen, d_slot,
                          d_wms, 4 * (nlevels - l - 1), nd, nexp, lc, s, mixed);
         }
         tm.end(S_SPLIT);
         tm.begin(S_EVAL);
+        W.wait(s, nf_ready);
         eval_level(nlevels, s);
-        W.order(s_lv, s);
         tm.end(S_EVAL);
+    } else {
+        near_field();
     }
-    near_field();
 
-    // s_lv's work (the gathers, the coarse levels) is joined already; each
-    // join costs a host-side wait of about 10 us
+    // s_lv's work (the gathers, the coarse levels' formation) is joined
+    // already; each join costs a host-side wait of about 10 us
     W.order(s_nf, s);
 
-    // ---------------- back to input order ----------------
-    tm.begin(S_SCATTER);
-    if (use_src) {
-        tm.ctx(-1, nullptr, -1, ns);
-        launch_scatter(ps, isrc, ns, nd, q.pot, s, nps);
-        if (q.ifpgh >= 2) launch_scatter(gs, isrc, ns, 2 * nd, q.grad, s, ngs);
-        if (q.ifpgh >= 3) launch_scatter(hs, isrc, ns, 3 * nd, q.hess, s, nhs);
-    }
-    if (use_trg) {
-        tm.ctx(-1, nullptr, -1, nt);
-        launch_scatter(pt, itarg, nt, nd, q.pottarg, s, npt);
-        if (q.ifpghtarg >= 2) launch_scatter(gt, itarg, nt, 2 * nd, q.gradtarg, s, ngt);
-        if (q.ifpghtarg >= 3) launch_scatter(ht, itarg, nt, 3 * nd, q.hesstarg, s, nht);
-    }
-    tm.end(S_SCATTER);
     HIPCHECK(hipGetLastError());
     tm.finish();
     W.last_valid = true;

@@ -182,6 +182,10 @@ void launch_zero(double* p, size_t n, hipStream_t s);
 void launch_zero3(double* a, size_t na, double* b, size_t nb, double* c, size_t nc, hipStream_t s);
 void launch_scatter(const double* v_s, const unsigned* perm, int n, int w, double* v,
                     hipStream_t s, const double* v_s2 = nullptr);
+// v[perm[i]*w + c] = nf[i*w + c] + 0 for the points of the listed boxes
+// (ranges se): the points that no evaluation writes
+void launch_scatter_boxes(const int* boxes, int nbox, const int2* se, const double* nf,
+                          const unsigned* perm, int w, double* v, hipStream_t s);
 
 struct FormArgs {
     const int* boxes;
@@ -259,7 +263,11 @@ struct EvalArgs {
     int npw;
     const double* loc;      // PW local expansions per slot
     double* taylor;         // scratch: (nbox, (nl+1)^2)
-    double *pot, *grad, *hess;  // sorted outputs
+    // outputs in input order, written at perm[i] for sorted point i as the
+    // near field's sorted value plus the evaluated one
+    double *pot, *grad, *hess;
+    const unsigned* perm;
+    const double *npot, *ngrad, *nhess;
     // point segments (as FormArgs); tstart == nullptr: one block per box
     const int* tstart;
     int seg, grid;

@@ -912,8 +912,8 @@ void launch_zero(double* p, size_t n, hipStream_t s)
 }
 
 // v[perm[i]*w + c] = v_s[i*w + c]  (dreorderi / dreorderiv, fmmcommon2d.f:29-90)
-// v[perm[i]] = vs[i] (+ vs2[i]): the sorted outputs back to input order,
-// with the near field's own outputs added when vs2 is given
+// v[perm[i]] = vs[i] (+ vs2[i]): sorted values back to input order, with
+// a second sorted array added when vs2 is given; vs == nullptr reads zeros
 __global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ vs,
                                                  const double* __restrict__ vs2,
                                                  const unsigned* __restrict__ perm, int n, int w,
@@ -923,9 +923,33 @@ __global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ vs,
     if (t >= (long)n * w) return;
     long i = t / w;
     int c = (int)(t - i * w);
-    double x = vs[t];
+    double x = vs ? vs[t] : 0.0;
     if (vs2) x = vs2[t] + x;
     v[(long)perm[i] * w + c] = x;
+}
+
+// the same for the points of the listed boxes, with vs == nullptr: one
+// block per box (the leaves that no evaluation covers)
+__global__ __launch_bounds__(256) void k_scatter_boxes(const int* __restrict__ boxes,
+                                                       const int2* __restrict__ se,
+                                                       const double* __restrict__ nf,
+                                                       const unsigned* __restrict__ perm, int w,
+                                                       double* __restrict__ v)
+{
+    const int2 r = se[boxes[blockIdx.x]];
+    const long t0 = (long)r.x * w, t1 = (long)r.y * w;
+    for (long t = t0 + threadIdx.x; t < t1; t += 256) {
+        const long i = t / w;
+        const int c = (int)(t - i * w);
+        v[(long)perm[i] * w + c] = nf[t] + 0.0;
+    }
+}
+
+void launch_scatter_boxes(const int* boxes, int nbox, const int2* se, const double* nf,
+                          const unsigned* perm, int w, double* v, hipStream_t s)
+{
+    if (nbox <= 0) return;
+    klaunch(KID_SCATTER, k_scatter_boxes, nbox, 256, 0, s, boxes, se, nf, perm, w, v);
 }
 
 void launch_scatter(const double* v_s, const unsigned* perm, int n, int w, double* v,
@@ -1111,6 +1135,55 @@ void launch_seg_scan(const SegLists& L, hipStream_t s)
 struct SegRange {
     int li, p0, p1;
 };
+
+// One point's evaluated values, for sorted point i at input position j =
+// a.perm[i]: pot, the gradient's two and the Hessian's three entries, each
+// added to the near field's sorted value as nf + (0 + v) and written to the
+// caller's arrays at j (the sorted order's outputs are never formed).
+// nfp is the near field's potential of the point (staged by the caller).
+template <int PGH>
+__device__ __forceinline__ void eval_out(const EvalArgs& a, long i, long j, double nfp, double v,
+                                         double g0, double g1, double h0, double h1, double h2)
+{
+    const int nd = a.nd, c = a.ind;
+    a.pot[j * nd + c] = nfp + (0.0 + v);
+    if (PGH >= 2) {
+        const long gi = i * 2 * nd + c, go = j * 2 * nd + c;
+        a.grad[go] = a.ngrad[gi] + (0.0 + g0);
+        a.grad[go + nd] = a.ngrad[gi + nd] + (0.0 + g1);
+    }
+    if (PGH >= 3) {
+        const long hi = i * 3 * nd + c, ho = j * 3 * nd + c;
+        a.hess[ho] = a.nhess[hi] + (0.0 + h0);
+        a.hess[ho + nd] = a.nhess[hi + nd] + (0.0 + h1);
+        a.hess[ho + 2 * nd] = a.nhess[hi + 2 * nd] + (0.0 + h2);
+    }
+}
+// the same with the permutation and the near field read here; with no
+// permutation (a.perm == nullptr: one box of points in input order, the
+// large-delta branch) the values are added into the outputs at i
+template <int PGH>
+__device__ __forceinline__ void eval_out(const EvalArgs& a, long i, double v, double g0, double g1,
+                                         double h0, double h1, double h2)
+{
+    if (a.perm) {
+        eval_out<PGH>(a, i, (long)a.perm[i], a.npot[i * a.nd + a.ind], v, g0, g1, h0, h1, h2);
+        return;
+    }
+    const int nd = a.nd, c = a.ind;
+    a.pot[i * nd + c] += v;
+    if (PGH >= 2) {
+        const long g = i * 2 * nd + c;
+        a.grad[g] += g0;
+        a.grad[g + nd] += g1;
+    }
+    if (PGH >= 3) {
+        const long h = i * 3 * nd + c;
+        a.hess[h] += h0;
+        a.hess[h + nd] += h1;
+        a.hess[h + 2 * nd] += h2;
+    }
+}
 
 // segment g of a list: its box index li and point range [p0, p1)
 __device__ __forceinline__ SegRange seg_at(const int* tstart, int nbox, const int* boxes,
@@ -2187,19 +2260,8 @@ __global__ __launch_bounds__(256) void k_taylor_eval(EvalArgs a)
             }
             V = V * y + q;
         }
-        long o = (long)i * a.nd + a.ind;
-        a.pot[o] += V;
-        if (PGH >= 2) {
-            long g = ((long)i * 2) * a.nd + a.ind;
-            a.grad[g] += U * dsq;
-            a.grad[g + a.nd] += Vy * dsq;
-        }
-        if (PGH >= 3) {
-            long h = ((long)i * 3) * a.nd + a.ind;
-            a.hess[h] += 2 * Z * dsq * dsq;
-            a.hess[h + a.nd] += Uy * dsq * dsq;
-            a.hess[h + 2 * a.nd] += 2 * Vyy * dsq * dsq;
-        }
+        eval_out<PGH>(a, i, V, U * dsq, Vy * dsq, 2 * Z * dsq * dsq, Uy * dsq * dsq,
+                      2 * Vyy * dsq * dsq);
     }
     }
 }
@@ -2319,20 +2381,9 @@ __global__ __launch_bounds__(256) void k_taylor_mfma(EvalArgs a)
             Vyy += __shfl_xor(Vyy, 16);
             Vyy += __shfl_xor(Vyy, 32);
         }
-        if (kq == 0 && act) {
-            a.pot[i * a.nd + a.ind] += V;
-            if (PGH >= 2) {
-                const long g = i * 2 * a.nd + a.ind;
-                a.grad[g] += U * dsq;
-                a.grad[g + a.nd] += Vy * dsq;
-            }
-            if (PGH >= 3) {
-                const long h = i * 3 * a.nd + a.ind;
-                a.hess[h] += Z * dsq * dsq;
-                a.hess[h + a.nd] += Uy * dsq * dsq;
-                a.hess[h + 2 * a.nd] += Vyy * dsq * dsq;
-            }
-        }
+        if (kq == 0 && act)
+            eval_out<PGH>(a, i, V, U * dsq, Vy * dsq, Z * dsq * dsq, Uy * dsq * dsq,
+                          Vyy * dsq * dsq);
     }
     }
 }
@@ -2530,10 +2581,14 @@ __global__ __launch_bounds__(256) void k_taylor_wave(EvalArgs a)
 {
     constexpr int MP = 16 * TM, KS = MP / 4, NV = PGH >= 3 ? 3 : (PGH >= 2 ? 2 : 1);
     __shared__ double stage[4][2][WSEG];
+    __shared__ double stage_nf[4][WSEG];   // the near field's potentials
+    __shared__ unsigned stage_j[4][WSEG];  // input positions
     extern __shared__ double dsm[];
     const int m = a.nl + 1, lane = threadIdx.x & 63;
     double* sx = stage[threadIdx.x >> 6][0];
     double* sy = stage[threadIdx.x >> 6][1];
+    double* snf = stage_nf[threadIdx.x >> 6];
+    unsigned* sj = stage_j[threadIdx.x >> 6];
     const int col = lane & 15, kq = lane >> 4;
     const int total = seg_count(a.tstart, a.nbox);
     // fused: each wave takes a run of a.spw consecutive segments (mostly of
@@ -2581,6 +2636,8 @@ __global__ __launch_bounds__(256) void k_taylor_wave(EvalArgs a)
                 const long i = sr.p0 + u;
                 sx[u] = (a.pts[2 * i] - cx) * dsq;
                 sy[u] = (a.pts[2 * i + 1] - cy) * dsq;
+                snf[u] = a.npot[i * a.nd + a.ind];
+                sj[u] = a.perm[i];
             }
         }
         wave_lds_sync();
@@ -2660,20 +2717,9 @@ __global__ __launch_bounds__(256) void k_taylor_wave(EvalArgs a)
                 Vyy += __shfl_xor(Vyy, 16);
                 Vyy += __shfl_xor(Vyy, 32);
             }
-            if (kq == 0 && act) {
-                a.pot[i * a.nd + a.ind] += V;
-                if (PGH >= 2) {
-                    const long gi = i * 2 * a.nd + a.ind;
-                    a.grad[gi] += U * dsq;
-                    a.grad[gi + a.nd] += Vy * dsq;
-                }
-                if (PGH >= 3) {
-                    const long h = i * 3 * a.nd + a.ind;
-                    a.hess[h] += Z * dsq * dsq;
-                    a.hess[h + a.nd] += Uy * dsq * dsq;
-                    a.hess[h + 2 * a.nd] += Vyy * dsq * dsq;
-                }
-            }
+            if (kq == 0 && act)
+                eval_out<PGH>(a, i, sj[ip], snf[ip], V, U * dsq, Vy * dsq, Z * dsq * dsq,
+                              Uy * dsq * dsq, Vyy * dsq * dsq);
         }
     }
 }
@@ -2689,10 +2735,14 @@ __global__ __launch_bounds__(256) void k_taylor_wave_f32(EvalArgs a)
 {
     constexpr int MP = 16 * TM, KS = MP / 4, NV = PGH >= 3 ? 3 : (PGH >= 2 ? 2 : 1);
     __shared__ float stage[4][2][WSEG];
+    __shared__ double stage_nf[4][WSEG];   // the near field's potentials
+    __shared__ unsigned stage_j[4][WSEG];  // input positions
     extern __shared__ double dsm[];
     const int m = a.nl + 1, lane = threadIdx.x & 63;
     float* sx = stage[threadIdx.x >> 6][0];
     float* sy = stage[threadIdx.x >> 6][1];
+    double* snf = stage_nf[threadIdx.x >> 6];
+    unsigned* sj = stage_j[threadIdx.x >> 6];
     const int col = lane & 15, kq = lane >> 4;
     const int total = seg_count(a.tstart, a.nbox);
     // fused: each wave takes a run of a.spw consecutive segments (mostly of
@@ -2743,6 +2793,8 @@ __global__ __launch_bounds__(256) void k_taylor_wave_f32(EvalArgs a)
                 const long i = sr.p0 + u;
                 sx[u] = (float)((a.pts[2 * i] - cx) * dsq);
                 sy[u] = (float)((a.pts[2 * i + 1] - cy) * dsq);
+                snf[u] = a.npot[i * a.nd + a.ind];
+                sj[u] = a.perm[i];
             }
         }
         wave_lds_sync();
@@ -2824,20 +2876,9 @@ __global__ __launch_bounds__(256) void k_taylor_wave_f32(EvalArgs a)
                 Vyy += __shfl_xor(Vyy, 16);
                 Vyy += __shfl_xor(Vyy, 32);
             }
-            if (kq == 0 && act) {
-                a.pot[i * a.nd + a.ind] += V;
-                if (PGH >= 2) {
-                    const long gi = i * 2 * a.nd + a.ind;
-                    a.grad[gi] += U * dsq;
-                    a.grad[gi + a.nd] += Vy * dsq;
-                }
-                if (PGH >= 3) {
-                    const long h = i * 3 * a.nd + a.ind;
-                    a.hess[h] += Z * dsq * dsq;
-                    a.hess[h + a.nd] += Uy * dsq * dsq;
-                    a.hess[h + 2 * a.nd] += Vyy * dsq * dsq;
-                }
-            }
+            if (kq == 0 && act)
+                eval_out<PGH>(a, i, sj[ip], snf[ip], V, U * dsq, Vy * dsq, Z * dsq * dsq,
+                              Uy * dsq * dsq, Vyy * dsq * dsq);
         }
     }
 }
@@ -2936,6 +2977,7 @@ void launch_taylor_eval(const EvalArgs& a, hipStream_t s)
     int m = a.nl + 1;
     if (m <= 32 && a.tstart && a.segbox && a.seg <= WSEG) {
         if (a.fused && a.npw > 64) fail(-3, "fused PW -> Taylor needs npw <= 64");
+        if (!a.perm || !a.npot) fail(-1, "the segmented evaluation writes permuted outputs");
         if (m <= 16)
             taylor_wave_any<1>(a, s);
         else
@@ -3115,21 +3157,9 @@ __global__ __launch_bounds__(256) void k_pw_eval_mfma(EvalArgs a)
                 }
             }
             const int i = base + pq;
-            if (cq == 0 && i < sr.p1) {
-                const long o = (long)i * a.nd + a.ind;
-                a.pot[o] += 2 * zr;
-                if (PGH >= 2) {
-                    const long gi = ((long)i * 2) * a.nd + a.ind;
-                    a.grad[gi] += 2 * gx * dsq;
-                    a.grad[gi + a.nd] += 2 * gy * dsq;
-                }
-                if (PGH >= 3) {
-                    const long h = ((long)i * 3) * a.nd + a.ind;
-                    a.hess[h] += 2 * hxx * dsq * dsq;
-                    a.hess[h + a.nd] += 2 * hxy * dsq * dsq;
-                    a.hess[h + 2 * a.nd] += 2 * hyy * dsq * dsq;
-                }
-            }
+            if (cq == 0 && i < sr.p1)
+                eval_out<PGH>(a, i, 2 * zr, 2 * gx * dsq, 2 * gy * dsq, 2 * hxx * dsq * dsq,
+                              2 * hxy * dsq * dsq, 2 * hyy * dsq * dsq);
         }
     }
 }
