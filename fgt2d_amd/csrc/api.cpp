@@ -2983,11 +2983,12 @@ size_t noalloc_point_bytes(int nd, int ns, int nt, int ifdipole, int ifpgh, int 
     };
     size_t tmpb = std::max(sort_pairs_u64_temp((int)nmx), sort_pairs_u32_temp((int)nmx));
     tmpb = std::max(tmpb, leaf_records_temp(ns));
-    // outputs twice: the evaluations' and the near field's
+    // outputs once: the near field's sorted accumulators (the evaluations
+    // write the caller's arrays)
     size_t b = 2 * r(8 * NS) + r(4 * nmx) + r(4 * NS) + r(tmpb) + r(4 * NS) + r(NS) +
                r(256 + sizeof(LeafRun) * NS) + r(16 * NS) + r(8 * ND * NS) +
-               (ifdipole ? r(16 * ND * NS) : 0) + 2 * outs(ifpgh, NS);
-    if (nt > 0) b += r(4 * NT) + r(16 * NT) + 2 * r(8 * NT) + 2 * outs(ifpghtarg, NT);
+               (ifdipole ? r(16 * ND * NS) : 0) + outs(ifpgh, NS);
+    if (nt > 0) b += r(4 * NT) + r(16 * NT) + 2 * r(8 * NT) + outs(ifpghtarg, NT);
     return b;
 }
 }  // namespace
@@ -3576,9 +3577,9 @@ int quantities(int dim, int ifpgh)
 size_t footprint(const ServeRequest& r, double extent)
 {
     const double d = r.dim, n = (double)r.ns + r.nt, nch = std::pow(2.0, d);
-    // per output quantity: the output, its sorted accumulator, staging and,
-    // in 2D, the near field's own accumulator
-    const double per_out = r.dim == 2 ? 32.0 : 24.0;
+    // per output quantity: the output, staging and a sorted accumulator
+    // (2D: the near field's; 3D: the evaluations')
+    const double per_out = 24.0;
     double bytes = 16.0 * ((d + r.nd) * r.ns + d * r.nt) +
                    per_out * r.nd * ((double)quantities(r.dim, r.ifpgh) * r.ns +
                                      (double)quantities(r.dim, r.ifpghtarg) * r.nt) +

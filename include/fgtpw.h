@@ -67,7 +67,7 @@ int fgtpw_fgt2d_device(int nd, double delta, double eps, int ns, const double *s
  * calls no hipMalloc, hipFree, hipHostMalloc or allocating library routine
  * (the host tree build uses the host heap).  Returns FGTPW_EINVAL when the
  * point-proportional buffers of (ns, nt, nd) exceed the scratch (about
- * 1.5M sources + 1.5M targets for nd = 1, potentials only), and
+ * 2.0M sources + 2.0M targets for nd = 1, potentials only), and
  * FGTPW_ENOMEM when the tree and expansions of the input do not fit beside
  * them.  Every call on a device shares that device's array: calls are
  * serialised by the library lock and return after `stream` has drained;
